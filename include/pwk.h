@@ -1,0 +1,176 @@
+/*
+ * pwk — MI355X (gfx950) Kohn-Sham Hamiltonian kernels behind a plain C ABI.
+ *
+ * Drop-in boundary for the GPU-backend and Hamiltonian-apply calls of
+ * pwdft-rs (paths relative to pwdft/pwdft-core/). Every entry point returns
+ * an int status (PWK_OK = 0); the message of the last failure on the calling
+ * thread is available from pwk_last_error(). No C++ exceptions cross the ABI.
+ *
+ * Conventions
+ *  - Energies eV, lengths Å, densities e/Å³ (the reference's internal units).
+ *  - Complex arrays are interleaved (re, im) pairs.
+ *  - Grids are row-major (x slowest): flat index i1*ny*nz + i2*nz + i3
+ *    (src/scf/grid.rs:120-147).
+ *  - Wavefunctions are stored like the reference's faer Mat<c64> of shape
+ *    npw × nbands in column-major order: band-major, each band contiguous.
+ *  - Host-pointer entry points (pwk_*) copy in, run on the context's stream
+ *    and synchronise before returning. Device-pointer entry points
+ *    (pwk_dev_*) take HIP device pointers, enqueue on the context's stream and
+ *    return without synchronising.
+ *  - Precision: PWK_FP64 runs everything in FP64. PWK_FP32 runs FFTs,
+ *    pointwise kernels and the nonlocal GEMMs in FP32 (device arrays are
+ *    complex64 / float32). PWK_SPLIT_BF16 is PWK_FP32 with the nonlocal GEMMs
+ *    as three-term split-bf16 MFMA products accumulated in FP32.
+ *  - A context owns one device, one HIP stream and the device state of one
+ *    calculation (grid, basis, k-point data, V_eff). Contexts are independent;
+ *    calls on one context must not overlap.
+ */
+#ifndef PWK_H
+#define PWK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pwk_ctx pwk_ctx;
+
+enum pwk_status {
+  PWK_OK = 0,
+  PWK_E_INVALID = 1,     /* bad argument or shape */
+  PWK_E_NO_DEVICE = 2,   /* no HIP device / bad ordinal */
+  PWK_E_DEVICE = 3,      /* HIP runtime error */
+  PWK_E_OOM = 4,         /* device allocation failed */
+  PWK_E_UNSUPPORTED = 5, /* grid/basis shape the kernels do not handle */
+  PWK_E_STATE = 6        /* call order violated (e.g. H·ψ before pwk_set_veff) */
+};
+
+enum pwk_precision { PWK_FP64 = 0, PWK_FP32 = 1, PWK_SPLIT_BF16 = 2 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pwk_last_error(void);
+/* ABI version (major*100 + minor). */
+int pwk_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Context. Replaces GpuAccelerator::try_new (src/gpu/mod.rs:148-183): returns
+ * PWK_E_NO_DEVICE where the reference returns None. There is no CPU fallback.
+ * ------------------------------------------------------------------------- */
+int pwk_ctx_create(int device, pwk_ctx** out);
+int pwk_ctx_destroy(pwk_ctx* ctx);
+int pwk_ctx_sync(pwk_ctx* ctx);
+/* The context's hipStream_t, for callers that enqueue their own work. */
+void* pwk_ctx_stream(pwk_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Grid kernels. Mirrors GpuAccelerator (src/gpu/mod.rs).
+ * ------------------------------------------------------------------------- */
+/* prepare_buffers (src/gpu/mod.rs:197-364): FFT dims and |G|² per grid point
+ * (length N = nx*ny*nz, FP64, grid order) uploaded once. */
+int pwk_prepare(pwk_ctx* ctx, const int dims[3], const double* g_squared);
+
+/* hartree_potential (src/gpu/mod.rs:398-494; semantic src/scf/energy.rs:343-358):
+ * v_h[i] = fourpi_e2 * rho_g[i] / |G_i|², 0 where |G|² <= 1e-12. Complex N. */
+int pwk_hartree(pwk_ctx* ctx, const double* rho_g, double fourpi_e2, double* v_h, int precision);
+
+/* v_eff_assembly (src/gpu/mod.rs:508-610; semantic src/scf/energy.rs:293-301):
+ * v_eff = v_local + v_h + v_xc, complex N. */
+int pwk_veff(pwk_ctx* ctx, const double* v_local, const double* v_h, const double* v_xc, double* v_eff,
+             int precision);
+
+/* Fused Hartree + V_eff (K1+K3): v_eff = v_local + 4πe²ρ(G)/|G|² + v_xc in one pass. */
+int pwk_hartree_veff(pwk_ctx* ctx, const double* rho_g, double fourpi_e2, const double* v_local,
+                     const double* v_xc, double* v_eff, int precision);
+
+/* lda_xc (src/gpu/mod.rs:627-725; semantic src/potential/xc.rs:129-180):
+ * PZ-LDA ε_xc and v_xc (eV) from ρ (e/Å³), real N. */
+int pwk_lda_xc(pwk_ctx* ctx, const double* rho, double* exc, double* vxc, int precision);
+
+/* lda_xc_spin_grid (src/potential/xc.rs:427-440): LSDA (ε_xc, v↑, v↓). */
+int pwk_lsda_xc(pwk_ctx* ctx, const double* rho_up, const double* rho_dn, double* exc, double* v_up,
+                double* v_dn, int precision);
+
+/* Device-pointer forms (complex arrays are double2/float2 per precision). */
+int pwk_dev_hartree_veff(pwk_ctx* ctx, const void* rho_g, double fourpi_e2, const void* v_local,
+                         const void* v_xc, void* v_eff, int precision);
+int pwk_dev_lda_xc(pwk_ctx* ctx, const void* rho, void* exc, void* vxc, int precision);
+int pwk_dev_lsda_xc(pwk_ctx* ctx, const void* rho_up, const void* rho_dn, void* exc, void* v_up, void* v_dn,
+                    int precision);
+
+/* ---------------------------------------------------------------------------
+ * 3D FFT with the reference's convention (src/fft.rs:19-114): unnormalised
+ * both ways, forward e^{-iG·r}, inverse e^{+iG·r}. sign = -1 forward, +1
+ * inverse. In place on N complex values (requires pwk_prepare).
+ * ------------------------------------------------------------------------- */
+int pwk_fft3d(pwk_ctx* ctx, double* data, int sign, int precision);
+int pwk_dev_fft3d(pwk_ctx* ctx, void* data, int batch, int sign, int precision);
+
+/* ---------------------------------------------------------------------------
+ * Hamiltonian apply
+ * ------------------------------------------------------------------------- */
+/* Plane-wave basis (src/basis.rs:36-63): Miller indices in the reference's
+ * enumeration order (n1 -> n2 -> n3 ascending), npw*3 int32. Builds the
+ * G -> FFT map (src/scf/grid.rs:100-106, 142-147) and the z-stick tables on
+ * the device. Requires pwk_prepare. */
+int pwk_set_basis(pwk_ctx* ctx, int npw, const int32_t* miller);
+
+/* Number of k-points whose kinetic diagonal / projectors are held. */
+int pwk_set_nk(pwk_ctx* ctx, int nk);
+
+/* Kinetic diagonal of k-point ik: t[i] = ħ²/2m |k+G_i|² (src/scf/potentials.rs:178). */
+int pwk_set_kinetic(pwk_ctx* ctx, int ik, const double* t);
+
+/* V_eff(G) (complex N, the reference's v_eff_fft) for the following H·ψ and
+ * dense-H calls. The device keeps V_eff(G) and V(r) = IFFT_unnorm(V_eff(G)). */
+int pwk_set_veff(pwk_ctx* ctx, const double* v_eff_g);
+int pwk_dev_set_veff(pwk_ctx* ctx, const void* v_eff_g /* double2, device */);
+
+/* Kleinman-Bylander projectors of k-point ik (src/potential/nonlocal.rs:110-126):
+ * B (npw × nch, column-major complex) and D·B^H (nch × npw, column-major
+ * complex), so that H_NL = B·(D·B^H) (nonlocal.rs:432-450). nch = 0 clears. */
+int pwk_set_nonlocal(pwk_ctx* ctx, int ik, int nch, const double* B, const double* DBH);
+
+/* Builds B and D·B^H for k-point ik on the device from radial projector
+ * tables, as NonlocalPotential::new (src/potential/nonlocal.rs:190-382).
+ *   kvec[3]: k (1/Å); gvec: Cartesian G per basis vector (npw*3, 1/Å);
+ *   natoms, tau (natoms*3 Cartesian Å), atom_species (natoms);
+ *   per species s: nproj[s], proj_l (concatenated), dij (concatenated nproj²
+ *   row-major, eV), mesh[s], r, rab, chi (concatenated mesh-length arrays,
+ *   χ = r·β in Å^{-1/2}, projector-major per species);
+ *   omega (Å³). */
+int pwk_build_nonlocal(pwk_ctx* ctx, int ik, const double* kvec, const double* gvec, int natoms,
+                       const double* tau, const int32_t* atom_species, int nspecies, const int32_t* nproj,
+                       const int32_t* proj_l, const double* dij, const int32_t* mesh, const double* r,
+                       const double* rab, const double* chi, double omega);
+/* Number of nonlocal channels held for k-point ik. */
+int pwk_nonlocal_channels(pwk_ctx* ctx, int ik);
+
+/* H·ψ for nb bands of k-point ik (semantic: dense H of
+ * fill_hamiltonian_with_v_eff + add_to_hamiltonian applied to ψ):
+ *   Hψ = T⊙ψ + (1/N)·gather FFT_f[V(r) ⊙ IFFT(scatter ψ)] + B·(D·B^H·ψ).
+ * Host form: psi/hpsi complex128 npw×nb. */
+int pwk_hpsi(pwk_ctx* ctx, int ik, int nb, const double* psi, double* hpsi, int precision);
+/* Device form: psi/hpsi are double2 (PWK_FP64) or float2 (PWK_FP32,
+ * PWK_SPLIT_BF16) device arrays of npw×nb. */
+int pwk_dev_hpsi(pwk_ctx* ctx, int ik, int nb, const void* psi, void* hpsi, int precision);
+
+/* Dense Hamiltonian of k-point ik (src/scf/potentials.rs:163-190, then
+ * nonlocal.rs:432-450 when with_nl): h is npw×npw column-major complex128. */
+int pwk_dense_h(pwk_ctx* ctx, int ik, int with_nl, double* h);
+int pwk_dev_dense_h(pwk_ctx* ctx, int ik, int with_nl, void* h /* double2 npw×npw */);
+
+/* Density accumulation (src/scf/density.rs:59-97): for each band b with
+ * fw[b] >= 1e-15, rho[r] += fw[b]·|IFFT_unnorm(scatter ψ_b)(r)|². rho is real
+ * N (FP64 accumulator, host or device), psi as in pwk_hpsi. */
+int pwk_density_accumulate(pwk_ctx* ctx, int nb, const double* psi, const double* fw, double* rho,
+                           int precision);
+int pwk_dev_density_accumulate(pwk_ctx* ctx, int nb, const void* psi, const double* fw_host, void* rho,
+                               int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PWK_H */

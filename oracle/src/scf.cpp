@@ -82,20 +82,19 @@ ScfResult run_scf(const System& s, const std::vector<KPoint>& kpts, const ScfPar
       for (size_t b = 0; b < nb; ++b) occ[k][b] = fermi_dirac_occ(evals[k][b], ef, prm.sigma) * spin_factor;
 
     // density (src/scf/density.rs:41-111)
-    // per-thread accumulation then reduction, as the rayon fold/reduce (density.rs:59-97)
+    // per-k accumulation in parallel, then a fixed-order sum over k so the result is
+    // independent of the thread count (the rayon fold/reduce of density.rs:59-97)
     std::vector<double> rho_new(N, 0.0);
-#pragma omp parallel
-    {
-      std::vector<double> acc(N, 0.0);
-#pragma omp for schedule(dynamic)
-      for (size_t k = 0; k < nk; ++k) {
-        std::vector<double> fw(nb);
-        for (size_t b = 0; b < nb; ++b) fw[b] = occ[k][b] * wk[k];
-        accumulate_density(s, wfn[k].data(), nb, fw.data(), acc);
-      }
-#pragma omp critical
-      for (size_t i = 0; i < N; ++i) rho_new[i] += acc[i];
+    std::vector<std::vector<double>> per_k(nk);
+#pragma omp parallel for schedule(dynamic)
+    for (size_t k = 0; k < nk; ++k) {
+      std::vector<double> fw(nb);
+      for (size_t b = 0; b < nb; ++b) fw[b] = occ[k][b] * wk[k];
+      per_k[k].assign(N, 0.0);
+      accumulate_density(s, wfn[k].data(), nb, fw.data(), per_k[k]);
     }
+    for (size_t k = 0; k < nk; ++k)
+      for (size_t i = 0; i < N; ++i) rho_new[i] += per_k[k][i];
     double dvol = s.omega / (double)N;
     double integral = 0.0;
     for (double v : rho_new) integral += v;

@@ -1,0 +1,438 @@
+// C-ABI entry points (include/pwk.h). Each converts exceptions into a status
+// code plus a thread-local message; nothing here falls back to the CPU.
+#include <cstring>
+#include <memory>
+
+#include "common.hpp"
+#include "fft_sizes.hpp"
+
+namespace pwk {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& m) { g_last_error = m; }
+}  // namespace pwk
+
+struct pwk_ctx {
+  pwk::Context c;
+};
+
+using pwk::Context;
+using pwk::DevBuf;
+using pwk::Error;
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return PWK_OK;
+  } catch (const Error& e) {
+    pwk::set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    pwk::set_last_error("host allocation failed");
+    return PWK_E_OOM;
+  } catch (const std::exception& e) {
+    pwk::set_last_error(e.what());
+    return PWK_E_DEVICE;
+  }
+}
+
+Context& ctx_of(pwk_ctx* p) {
+  PWK_REQUIRE(p != nullptr, PWK_E_INVALID, "null context");
+  PWK_HIP(hipSetDevice(p->c.device));
+  return p->c;
+}
+
+void check_precision(int precision) {
+  PWK_REQUIRE(precision == PWK_FP64 || precision == PWK_FP32 || precision == PWK_SPLIT_BF16, PWK_E_INVALID,
+              "unknown precision");
+}
+
+size_t csize(int precision) { return precision == PWK_FP64 ? 16 : 8; }
+size_t rsize(int precision) { return precision == PWK_FP64 ? 8 : 4; }
+
+// Host double array (n doubles) -> device buffer in the requested precision.
+void to_device(Context& c, DevBuf& d, const double* h, size_t ndoubles, int precision) {
+  if (precision == PWK_FP64) {
+    d.ensure(ndoubles * 8);
+    PWK_HIP(hipMemcpyAsync(d.p, h, ndoubles * 8, hipMemcpyHostToDevice, c.stream));
+  } else {
+    c.host_in.ensure(ndoubles * 8);
+    PWK_HIP(hipMemcpyAsync(c.host_in.p, h, ndoubles * 8, hipMemcpyHostToDevice, c.stream));
+    d.ensure(ndoubles * 4);
+    pwk::launch_convert(c, c.host_in.p, d.p, ndoubles, true);
+  }
+}
+
+// Device buffer (n values in precision) -> host double array.
+void to_host(Context& c, double* h, const DevBuf& d, size_t ndoubles, int precision) {
+  if (precision == PWK_FP64) {
+    PWK_HIP(hipMemcpyAsync(h, d.p, ndoubles * 8, hipMemcpyDeviceToHost, c.stream));
+  } else {
+    c.host_out.ensure(ndoubles * 8);
+    pwk::launch_convert(c, d.p, c.host_out.p, ndoubles, false);
+    PWK_HIP(hipMemcpyAsync(h, c.host_out.p, ndoubles * 8, hipMemcpyDeviceToHost, c.stream));
+  }
+  PWK_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pwk_last_error(void) { return pwk::g_last_error.c_str(); }
+
+int pwk_version(void) { return 100; }
+
+int pwk_ctx_create(int device, pwk_ctx** out) {
+  return guarded([&] {
+    PWK_REQUIRE(out != nullptr, PWK_E_INVALID, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    PWK_REQUIRE(e == hipSuccess && n > 0, PWK_E_NO_DEVICE, "no HIP device available");
+    PWK_REQUIRE(device >= 0 && device < n, PWK_E_NO_DEVICE, "device ordinal out of range");
+    PWK_HIP(hipSetDevice(device));
+    auto p = std::make_unique<pwk_ctx>();
+    p->c.device = device;
+    PWK_HIP(hipStreamCreateWithFlags(&p->c.stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    PWK_HIP(hipGetDeviceProperties(&prop, device));
+    p->c.cu_count = prop.multiProcessorCount;
+    *out = p.release();
+  });
+}
+
+int pwk_ctx_destroy(pwk_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    hipStream_t s = ctx->c.stream;
+    delete ctx;
+    (void)hipStreamDestroy(s);
+  });
+}
+
+int pwk_ctx_sync(pwk_ctx* ctx) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+void* pwk_ctx_stream(pwk_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+int pwk_prepare(pwk_ctx* ctx, const int dims[3], const double* g2) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_REQUIRE(dims && g2, PWK_E_INVALID, "null dims or |G|^2");
+    for (int d = 0; d < 3; ++d) PWK_REQUIRE(dims[d] > 0 && dims[d] <= 1024, PWK_E_INVALID, "bad FFT dims");
+    c.dims[0] = dims[0];
+    c.dims[1] = dims[1];
+    c.dims[2] = dims[2];
+    c.N = (size_t)dims[0] * dims[1] * dims[2];
+    c.g2_64.ensure(c.N * 8);
+    c.g2_32.ensure(c.N * 4);
+    PWK_HIP(hipMemcpyAsync(c.g2_64.p, g2, c.N * 8, hipMemcpyHostToDevice, c.stream));
+    pwk::launch_convert(c, c.g2_64.p, c.g2_32.p, c.N, true);
+    c.npw = 0;
+    c.have_v = false;
+    PWK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+// ---- grid kernels ----------------------------------------------------------
+int pwk_hartree(pwk_ctx* ctx, const double* rho_g, double fourpi_e2, double* v_h, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_grid();
+    DevBuf in, out;
+    to_device(c, in, rho_g, 2 * c.N, precision);
+    out.ensure(2 * c.N * rsize(precision));
+    pwk::launch_hartree_veff(c, in.p, fourpi_e2, nullptr, nullptr, out.p, precision, false, false, true);
+    to_host(c, v_h, out, 2 * c.N, precision);
+  });
+}
+
+int pwk_veff(pwk_ctx* ctx, const double* v_local, const double* v_h, const double* v_xc, double* v_eff,
+             int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_grid();
+    DevBuf a, b, d, out;
+    to_device(c, a, v_local, 2 * c.N, precision);
+    to_device(c, b, v_h, 2 * c.N, precision);
+    to_device(c, d, v_xc, 2 * c.N, precision);
+    out.ensure(2 * c.N * rsize(precision));
+    pwk::launch_veff_sum(c, a.p, b.p, d.p, out.p, precision);
+    to_host(c, v_eff, out, 2 * c.N, precision);
+  });
+}
+
+int pwk_hartree_veff(pwk_ctx* ctx, const double* rho_g, double fourpi_e2, const double* v_local, const double* v_xc,
+                     double* v_eff, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_grid();
+    DevBuf r, a, d, out;
+    to_device(c, r, rho_g, 2 * c.N, precision);
+    to_device(c, a, v_local, 2 * c.N, precision);
+    to_device(c, d, v_xc, 2 * c.N, precision);
+    out.ensure(2 * c.N * rsize(precision));
+    pwk::launch_hartree_veff(c, r.p, fourpi_e2, a.p, d.p, out.p, precision, true, true, true);
+    to_host(c, v_eff, out, 2 * c.N, precision);
+  });
+}
+
+int pwk_lda_xc(pwk_ctx* ctx, const double* rho, double* exc, double* vxc, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_grid();
+    DevBuf in, e, v;
+    to_device(c, in, rho, c.N, precision);
+    e.ensure(c.N * rsize(precision));
+    v.ensure(c.N * rsize(precision));
+    pwk::launch_lda(c, in.p, e.p, v.p, precision);
+    to_host(c, exc, e, c.N, precision);
+    to_host(c, vxc, v, c.N, precision);
+  });
+}
+
+int pwk_lsda_xc(pwk_ctx* ctx, const double* rho_up, const double* rho_dn, double* exc, double* v_up, double* v_dn,
+                int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_grid();
+    DevBuf u, d, e, vu, vd;
+    to_device(c, u, rho_up, c.N, precision);
+    to_device(c, d, rho_dn, c.N, precision);
+    e.ensure(c.N * rsize(precision));
+    vu.ensure(c.N * rsize(precision));
+    vd.ensure(c.N * rsize(precision));
+    pwk::launch_lsda(c, u.p, d.p, e.p, vu.p, vd.p, precision);
+    to_host(c, exc, e, c.N, precision);
+    to_host(c, v_up, vu, c.N, precision);
+    to_host(c, v_dn, vd, c.N, precision);
+  });
+}
+
+int pwk_dev_hartree_veff(pwk_ctx* ctx, const void* rho_g, double fourpi_e2, const void* v_local, const void* v_xc,
+                         void* v_eff, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    pwk::launch_hartree_veff(c, rho_g, fourpi_e2, v_local, v_xc, v_eff, precision, v_local != nullptr,
+                             v_xc != nullptr, rho_g != nullptr);
+  });
+}
+
+int pwk_dev_lda_xc(pwk_ctx* ctx, const void* rho, void* exc, void* vxc, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    pwk::launch_lda(c, rho, exc, vxc, precision);
+  });
+}
+
+int pwk_dev_lsda_xc(pwk_ctx* ctx, const void* rho_up, const void* rho_dn, void* exc, void* v_up, void* v_dn,
+                    int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    pwk::launch_lsda(c, rho_up, rho_dn, exc, v_up, v_dn, precision);
+  });
+}
+
+// ---- FFT ------------------------------------------------------------------------
+int pwk_fft3d(pwk_ctx* ctx, double* data, int sign, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_grid();
+    PWK_REQUIRE(sign == 1 || sign == -1, PWK_E_INVALID, "sign must be +1 or -1");
+    DevBuf d;
+    to_device(c, d, data, 2 * c.N, precision);
+    pwk::launch_fft3d(c, d.p, 1, sign, precision);
+    to_host(c, data, d, 2 * c.N, precision);
+  });
+}
+
+int pwk_dev_fft3d(pwk_ctx* ctx, void* data, int batch, int sign, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    PWK_REQUIRE(sign == 1 || sign == -1, PWK_E_INVALID, "sign must be +1 or -1");
+    PWK_REQUIRE(batch > 0, PWK_E_INVALID, "batch must be positive");
+    pwk::launch_fft3d(c, data, batch, sign, precision);
+  });
+}
+
+// ---- Hamiltonian apply ---------------------------------------------------------
+int pwk_set_basis(pwk_ctx* ctx, int npw, const int32_t* miller) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    pwk::setup_basis(c, npw, miller);
+    c.kd.clear();
+  });
+}
+
+int pwk_set_nk(pwk_ctx* ctx, int nk) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_REQUIRE(nk >= 0, PWK_E_INVALID, "nk must be non-negative");
+    c.kd.resize((size_t)nk);
+  });
+}
+
+int pwk_set_kinetic(pwk_ctx* ctx, int ik, const double* t) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    c.check_basis();
+    pwk::KData& kd = c.k(ik);
+    kd.kin64.ensure((size_t)c.npw * 8);
+    kd.kin32.ensure((size_t)c.npw * 4);
+    PWK_HIP(hipMemcpyAsync(kd.kin64.p, t, (size_t)c.npw * 8, hipMemcpyHostToDevice, c.stream));
+    pwk::launch_convert(c, kd.kin64.p, kd.kin32.p, (size_t)c.npw, true);
+    PWK_HIP(hipStreamSynchronize(c.stream));
+    kd.have_kin = true;
+  });
+}
+
+int pwk_set_veff(pwk_ctx* ctx, const double* v_eff_g) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    c.check_grid();
+    DevBuf d;
+    d.ensure(c.N * 16);
+    PWK_HIP(hipMemcpyAsync(d.p, v_eff_g, c.N * 16, hipMemcpyHostToDevice, c.stream));
+    pwk::set_veff_device(c, d.p);
+    PWK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int pwk_dev_set_veff(pwk_ctx* ctx, const void* v_eff_g) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    pwk::set_veff_device(c, v_eff_g);
+  });
+}
+
+int pwk_set_nonlocal(pwk_ctx* ctx, int ik, int nch, const double* B, const double* DBH) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    c.check_basis();
+    pwk::KData& kd = c.k(ik);
+    PWK_REQUIRE(nch >= 0, PWK_E_INVALID, "nch must be non-negative");
+    kd.nch = nch;
+    if (nch == 0) return;
+    size_t n = (size_t)c.npw * nch;
+    kd.B64.ensure(n * 16);
+    kd.DBH64.ensure(n * 16);
+    kd.B32.ensure(n * 8);
+    kd.DBH32.ensure(n * 8);
+    PWK_HIP(hipMemcpyAsync(kd.B64.p, B, n * 16, hipMemcpyHostToDevice, c.stream));
+    PWK_HIP(hipMemcpyAsync(kd.DBH64.p, DBH, n * 16, hipMemcpyHostToDevice, c.stream));
+    pwk::launch_convert(c, kd.B64.p, kd.B32.p, 2 * n, true);
+    pwk::launch_convert(c, kd.DBH64.p, kd.DBH32.p, 2 * n, true);
+    PWK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int pwk_build_nonlocal(pwk_ctx* ctx, int ik, const double* kvec, const double* gvec, int natoms, const double* tau,
+                       const int32_t* atom_species, int nspecies, const int32_t* nproj, const int32_t* proj_l,
+                       const double* dij, const int32_t* mesh, const double* r, const double* rab, const double* chi,
+                       double omega) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_REQUIRE(kvec && gvec && nproj && mesh, PWK_E_INVALID, "null projector input");
+    pwk::build_nonlocal_device(c, ik, kvec, gvec, natoms, tau, atom_species, nspecies, nproj, proj_l, dij, mesh, r,
+                               rab, chi, omega);
+  });
+}
+
+int pwk_nonlocal_channels(pwk_ctx* ctx, int ik) {
+  int n = -1;
+  int st = guarded([&] {
+    Context& c = ctx_of(ctx);
+    n = c.k(ik).nch;
+  });
+  return st == PWK_OK ? n : -st;
+}
+
+int pwk_hpsi(pwk_ctx* ctx, int ik, int nb, const double* psi, double* hpsi, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_basis();
+    PWK_REQUIRE(nb > 0 && psi && hpsi, PWK_E_INVALID, "bad psi/hpsi");
+    size_t nd = (size_t)2 * c.npw * nb;
+    DevBuf in, out;
+    to_device(c, in, psi, nd, precision);
+    out.ensure(nd * rsize(precision));
+    pwk::launch_hpsi(c, ik, nb, in.p, out.p, precision);
+    to_host(c, hpsi, out, nd, precision);
+  });
+}
+
+int pwk_dev_hpsi(pwk_ctx* ctx, int ik, int nb, const void* psi, void* hpsi, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    pwk::launch_hpsi(c, ik, nb, psi, hpsi, precision);
+  });
+}
+
+int pwk_dense_h(pwk_ctx* ctx, int ik, int with_nl, double* h) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    c.check_basis();
+    DevBuf d;
+    size_t nd = (size_t)2 * c.npw * c.npw;
+    d.ensure(nd * 8);
+    pwk::launch_dense_h(c, ik, with_nl, d.p);
+    to_host(c, h, d, nd, PWK_FP64);
+  });
+}
+
+int pwk_dev_dense_h(pwk_ctx* ctx, int ik, int with_nl, void* h) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    pwk::launch_dense_h(c, ik, with_nl, h);
+  });
+}
+
+int pwk_density_accumulate(pwk_ctx* ctx, int nb, const double* psi, const double* fw, double* rho, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_basis();
+    PWK_REQUIRE(nb > 0 && psi && fw && rho, PWK_E_INVALID, "bad density arguments");
+    size_t nd = (size_t)2 * c.npw * nb;
+    DevBuf in, r;
+    to_device(c, in, psi, nd, precision);
+    r.ensure(c.N * 8);
+    PWK_HIP(hipMemcpyAsync(r.p, rho, c.N * 8, hipMemcpyHostToDevice, c.stream));
+    pwk::launch_density(c, nb, in.p, fw, r.p, precision);
+    to_host(c, rho, r, c.N, PWK_FP64);
+  });
+}
+
+int pwk_dev_density_accumulate(pwk_ctx* ctx, int nb, const void* psi, const double* fw_host, void* rho,
+                               int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    pwk::launch_density(c, nb, psi, fw_host, rho, precision);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,159 @@
+// Internal declarations shared by the pwk translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pwk.h"
+
+namespace pwk {
+
+// Error carried to the ABI boundary and turned into a status code there.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PWK_HIP(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      throw ::pwk::Error(e_ == hipErrorOutOfMemory ? PWK_E_OOM : PWK_E_DEVICE,                    \
+                         std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+  } while (0)
+
+#define PWK_REQUIRE(cond, code, msg)                   \
+  do {                                                 \
+    if (!(cond)) throw ::pwk::Error((code), (msg));    \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    if (p) PWK_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    PWK_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Per-k-point device data.
+struct KData {
+  bool have_kin = false;
+  DevBuf kin64, kin32;  // ħ²/2m |k+G|², npw
+  int nch = 0;
+  DevBuf B64, DBH64;    // double2, npw×nch and nch×npw (column-major)
+  DevBuf B32, DBH32;    // float2 copies
+};
+
+// Device state of one calculation.
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+
+  // FFT grid (pwk_prepare)
+  int dims[3] = {0, 0, 0};
+  size_t N = 0;
+  DevBuf g2_64, g2_32;
+
+  // plane-wave basis (pwk_set_basis)
+  int npw = 0;
+  DevBuf miller;   // int32 npw*3
+  DevBuf g2fft;    // int32 npw: flat grid index of each G
+  // z-sticks: one per distinct (n1, n2); basis vectors of a stick are
+  // contiguous in the reference's n1 -> n2 -> n3 enumeration.
+  int nsticks = 0;
+  DevBuf st_pw0;   // int32: first basis index of the stick
+  DevBuf st_cnt;   // int32: number of basis vectors in the stick
+  DevBuf st_n3lo;  // int32: n3 of the first basis vector
+  DevBuf st_xy;    // int32: i1*ny + i2 (wrapped grid column)
+  int nx1 = 0;     // distinct wrapped i1 values carrying sticks
+  DevBuf x1list;   // int32 nx1
+  DevBuf st_x1;    // int32: index into x1list of the stick's i1
+
+  // effective potential (pwk_set_veff)
+  bool have_v = false;
+  DevBuf veffg64;      // double2 N, V_eff(G)
+  DevBuf vr64, vr32;   // real N, V(r) in plane-major order [i3][i1][i2]
+
+  std::vector<KData> kd;
+
+  // workspaces
+  DevBuf work_a, work_b;  // stick buffers / FFT scratch
+  DevBuf proj;            // nonlocal projections
+  DevBuf gemm_ws;         // split-K partial sums
+  DevBuf host_in, host_out;  // staging for host-pointer entry points
+
+  void check_grid() const { PWK_REQUIRE(N > 0, PWK_E_STATE, "pwk_prepare has not been called"); }
+  void check_basis() const {
+    check_grid();
+    PWK_REQUIRE(npw > 0, PWK_E_STATE, "pwk_set_basis has not been called");
+  }
+  KData& k(int ik) {
+    PWK_REQUIRE(ik >= 0 && ik < (int)kd.size(), PWK_E_INVALID, "k-point index out of range (pwk_set_nk)");
+    return kd[(size_t)ik];
+  }
+};
+
+// Thread-local last error (ABI)
+void set_last_error(const std::string& m);
+
+// ---- kernel launch helpers (defined in the .hip units) ------------------------
+// grid.hip
+void launch_hartree_veff(Context& c, const void* rho_g, double fourpi_e2, const void* v_local, const void* v_xc,
+                         void* out, int precision, bool with_local, bool with_xc, bool with_hartree);
+void launch_veff_sum(Context& c, const void* a, const void* b, const void* d, void* out, int precision);
+void launch_lda(Context& c, const void* rho, void* exc, void* vxc, int precision);
+void launch_lsda(Context& c, const void* ru, const void* rd, void* exc, void* vu, void* vd, int precision);
+void launch_convert(Context& c, const void* src, void* dst, size_t n, bool to_fp32);  // double<->float arrays
+// fft3d.hip
+bool fft_size_supported(int n);
+void launch_fft3d(Context& c, void* data, int batch, int sign, int precision);
+// hpsi.hip
+void setup_basis(Context& c, int npw, const int32_t* miller_host);
+void set_veff_device(Context& c, const void* veff_g_dev);
+void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int precision);
+void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision);
+// dense.hip
+void launch_dense_h(Context& c, int ik, int with_nl, void* h);
+// gemm.hip: complex C = alpha·op(A)·B + beta·C, column-major, op = N only
+void launch_zgemm(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
+                  bool accumulate, int precision);
+// nonlocal.hip
+void build_nonlocal_device(Context& c, int ik, const double* kvec, const double* gvec, int natoms, const double* tau,
+                           const int32_t* atom_species, int nspecies, const int32_t* nproj, const int32_t* proj_l,
+                           const double* dij, const int32_t* mesh, const double* r, const double* rab,
+                           const double* chi, double omega);
+
+inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace pwk

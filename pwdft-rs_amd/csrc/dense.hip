@@ -1,0 +1,50 @@
+// Dense Kohn-Sham Hamiltonian of one k-point, FP64, column-major npw×npw:
+//   H[i,j] = V_eff[wrap(m_i - m_j)] + δ_ij ħ²/2m |k+G_i|²   (src/scf/potentials.rs:163-190)
+// then H += B·(D·B^H)                                        (src/potential/nonlocal.rs:432-450)
+// The gather is an HBM-write-bound pass (npw²·16 B); V_eff(G) stays in L2.
+#include "common.hpp"
+
+namespace pwk {
+
+namespace {
+
+__device__ __forceinline__ int wrapd(int n, int d) {
+  int r = n % d;
+  return r < 0 ? r + d : r;
+}
+
+__global__ void __launch_bounds__(256) k_dense_h(int npw, const int* __restrict__ miller,
+                                                 const double2* __restrict__ veff, const double* __restrict__ kin,
+                                                 int nx, int ny, int nz, double2* __restrict__ H) {
+  int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= npw || j >= npw) return;
+  int d1 = miller[3 * i] - miller[3 * j];
+  int d2 = miller[3 * i + 1] - miller[3 * j + 1];
+  int d3 = miller[3 * i + 2] - miller[3 * j + 2];
+  size_t idx = ((size_t)wrapd(d1, nx) * ny + wrapd(d2, ny)) * nz + wrapd(d3, nz);
+  double2 v = veff[idx];
+  if (i == j) {
+    v.x = kin[i] + v.x;
+    v.y = 0.0 + v.y;
+  }
+  H[(size_t)j * npw + i] = v;
+}
+
+}  // namespace
+
+void launch_dense_h(Context& c, int ik, int with_nl, void* h) {
+  c.check_basis();
+  KData& kd = c.k(ik);
+  PWK_REQUIRE(kd.have_kin, PWK_E_STATE, "kinetic diagonal of this k-point not set (pwk_set_kinetic)");
+  PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
+  int n = c.npw;
+  dim3 grid(blocks_for((size_t)n, 64), blocks_for((size_t)n, 4));
+  k_dense_h<<<grid, 256, 0, c.stream>>>(n, c.miller.as<int>(), c.veffg64.as<double2>(), kd.kin64.as<double>(),
+                                        c.dims[0], c.dims[1], c.dims[2], (double2*)h);
+  PWK_HIP(hipGetLastError());
+  if (with_nl && kd.nch > 0)
+    launch_zgemm(c, n, n, kd.nch, kd.B64.p, n, kd.DBH64.p, kd.nch, h, n, true, PWK_FP64);
+}
+
+}  // namespace pwk

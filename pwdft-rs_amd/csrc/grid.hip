@@ -1,0 +1,283 @@
+// Pointwise grid kernels: Hartree, V_eff assembly, PZ-LDA and LSDA XC.
+// Each is a single HBM pass (bandwidth-bound); FP64 and FP32 variants.
+#include "common.hpp"
+
+namespace pwk {
+
+namespace {
+
+// Physical constants as in src/consts.rs:15-49 (RHO_FLOOR src/functionals/mod.rs:5)
+constexpr double kPiD = 3.141592653589793;
+constexpr double kHaToEv = 27.211386245988;
+constexpr double kBohr = 0.529177210903;
+constexpr double kBohr3 = kBohr * kBohr * kBohr;
+constexpr double kRhoFloor = 1e-30;
+constexpr double kG2Zero = 1e-12;
+
+template <typename T>
+struct Cx;
+template <>
+struct Cx<double> {
+  using t = double2;
+};
+template <>
+struct Cx<float> {
+  using t = float2;
+};
+
+// v_eff = [v_local] + [4πe² ρ(G)/|G|²] + [v_xc]  (src/scf/energy.rs:293-301, 343-358)
+template <typename T>
+__global__ void __launch_bounds__(256) k_hartree_veff(size_t n, const typename Cx<T>::t* __restrict__ rho,
+                                                      const T* __restrict__ g2, T fourpi_e2,
+                                                      const typename Cx<T>::t* __restrict__ vl,
+                                                      const typename Cx<T>::t* __restrict__ vxc,
+                                                      typename Cx<T>::t* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  typename Cx<T>::t acc;
+  acc.x = 0;
+  acc.y = 0;
+  if (rho) {
+    T gg = g2[i];
+    if (gg > (T)kG2Zero) {
+      typename Cx<T>::t r = rho[i];
+      acc.x = r.x * fourpi_e2 / gg;
+      acc.y = r.y * fourpi_e2 / gg;
+    }
+  }
+  if (vl) {
+    typename Cx<T>::t a = vl[i];
+    acc.x = a.x + acc.x;
+    acc.y = a.y + acc.y;
+  }
+  if (vxc) {
+    typename Cx<T>::t a = vxc[i];
+    acc.x += a.x;
+    acc.y += a.y;
+  }
+  out[i] = acc;
+}
+
+// (a + b) + d, the reference's left-to-right sum vl + vh + vxc
+template <typename T>
+__global__ void __launch_bounds__(256) k_sum3(size_t n, const typename Cx<T>::t* __restrict__ a,
+                                              const typename Cx<T>::t* __restrict__ b,
+                                              const typename Cx<T>::t* __restrict__ d,
+                                              typename Cx<T>::t* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  typename Cx<T>::t x = a[i], y = b[i], z = d[i], r;
+  r.x = (x.x + y.x) + z.x;
+  r.y = (x.y + y.y) + z.y;
+  out[i] = r;
+}
+
+// ---- PZ-LDA (src/potential/xc.rs:129-141, 223-288) ----------------------------
+__device__ __forceinline__ void lda_point(double rho, double& exc, double& vxc) {
+  if (rho < kRhoFloor) {
+    exc = 0.0;
+    vxc = 0.0;
+    return;
+  }
+  double rb = rho * kBohr3;
+  double ex_ha = -0.75 * cbrt(3.0 * rb / kPiD);
+  double vx_ha = (4.0 / 3.0) * ex_ha;
+  double rs = cbrt(3.0 / (4.0 * kPiD * rb));
+  double ec, vc;
+  if (rs >= 1.0) {
+    const double gamma = -0.1423, b1 = 1.0529, b2 = 0.3334;
+    double srs = sqrt(rs);
+    double den = 1.0 + b1 * srs + b2 * rs;
+    ec = gamma / den;
+    double dec = -gamma * (b1 / (2.0 * srs) + b2) / (den * den);
+    vc = fma(rs / 3.0, -dec, ec);
+  } else {
+    const double a = 0.0311, b = -0.048, c = 0.0020, d = -0.0116;
+    double lr = log(rs);
+    ec = fma(d, rs, fma(c * rs, lr, fma(a, lr, b)));
+    double dec = fma(c, lr + 1.0, a / rs) + d;
+    vc = fma(rs / 3.0, -dec, ec);
+  }
+  exc = ex_ha * kHaToEv + ec * kHaToEv;
+  vxc = vx_ha * kHaToEv + vc * kHaToEv;
+}
+
+__device__ __forceinline__ void lda_point(float rho, float& exc, float& vxc) {
+  if (rho < 1e-30f) {
+    exc = 0.0f;
+    vxc = 0.0f;
+    return;
+  }
+  const float pi = 3.14159265358979f;
+  float rb = rho * (float)kBohr3;
+  float ex_ha = -0.75f * cbrtf(3.0f * rb / pi);
+  float vx_ha = (4.0f / 3.0f) * ex_ha;
+  float rs = cbrtf(3.0f / (4.0f * pi * rb));
+  float ec, vc;
+  if (rs >= 1.0f) {
+    const float gamma = -0.1423f, b1 = 1.0529f, b2 = 0.3334f;
+    float srs = sqrtf(rs);
+    float den = 1.0f + b1 * srs + b2 * rs;
+    ec = gamma / den;
+    float dec = -gamma * (b1 / (2.0f * srs) + b2) / (den * den);
+    vc = ec - rs / 3.0f * dec;
+  } else {
+    const float a = 0.0311f, b = -0.048f, c = 0.0020f, d = -0.0116f;
+    float lr = logf(rs);
+    ec = a * lr + b + c * rs * lr + d * rs;
+    float dec = a / rs + c * (lr + 1.0f) + d;
+    vc = ec - rs / 3.0f * dec;
+  }
+  exc = (ex_ha + ec) * (float)kHaToEv;
+  vxc = (vx_ha + vc) * (float)kHaToEv;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lda(size_t n, const T* __restrict__ rho, T* __restrict__ exc,
+                                             T* __restrict__ vxc) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T e, v;
+  lda_point(rho[i], e, v);
+  exc[i] = e;
+  vxc[i] = v;
+}
+
+// ---- LSDA (src/potential/xc.rs:374-392, 465-596) ------------------------------
+template <typename T>
+__device__ __forceinline__ void pz_rs(T rs, bool pol, T& ec, T& vc) {
+  if (rs >= (T)1) {
+    T gamma = pol ? (T)-0.0843 : (T)-0.1423, b1 = pol ? (T)1.3981 : (T)1.0529, b2 = pol ? (T)0.2611 : (T)0.3334;
+    T srs = sqrt(rs);
+    T den = fma(b2, rs, fma(b1, srs, (T)1));
+    ec = gamma / den;
+    T dec = -gamma * (b1 / ((T)2 * srs) + b2) / (den * den);
+    vc = fma(rs / (T)3, -dec, ec);
+  } else {
+    T a = pol ? (T)0.01555 : (T)0.0311, b = pol ? (T)-0.0269 : (T)-0.048, c = pol ? (T)0.0007 : (T)0.0020,
+      d = pol ? (T)-0.0048 : (T)-0.0116;
+    T lr = log(rs);
+    ec = fma(d, rs, fma(c * rs, lr, fma(a, lr, b)));
+    T dec = fma(c, lr + (T)1, a / rs) + d;
+    vc = fma(rs / (T)3, -dec, ec);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void lsda_point(T ru, T rd, T& exc, T& vu, T& vd) {
+  const T floor_ = (T)kRhoFloor;
+  T rho = ru + rd;
+  if (rho < floor_) {
+    exc = vu = vd = (T)0;
+    return;
+  }
+  const T pi = (T)kPiD, b3 = (T)kBohr3, ha = (T)kHaToEv;
+  T rub = ru * b3, rdb = rd * b3;
+  T exu = rub > floor_ ? (T)-0.75 * cbrt((T)6 * rub / pi) : (T)0;
+  T exd = rdb > floor_ ? (T)-0.75 * cbrt((T)6 * rdb / pi) : (T)0;
+  T rb = rho * b3;
+  T ex = fma(rub, exu, rdb * exd) / rb;
+  T vxu = ((T)4 / (T)3) * exu, vxd = ((T)4 / (T)3) * exd;
+  T rs = cbrt((T)3 / ((T)4 * pi * rb));
+  T zeta = fmin(fmax((ru - rd) / rho, (T)-1), (T)1);
+  T ecu, vcu, ecp, vcp;
+  pz_rs<T>(rs, false, ecu, vcu);
+  pz_rs<T>(rs, true, ecp, vcp);
+  T fden = fma(cbrt((T)2), (T)2, (T)-2);
+  T op = fmax((T)1 + zeta, (T)0), om = fmax((T)1 - zeta, (T)0);
+  T cop = cbrt(op), com = cbrt(om);
+  T fz = (cop * op + com * om - (T)2) / fden;
+  T dfz = ((T)4 / (T)3) * (cop - com) / fden;
+  T ec = ecu + fz * (ecp - ecu);
+  T vrs = vcu + fz * (vcp - vcu);
+  T dedz = dfz * (ecp - ecu);
+  T vcu_ = fma((T)1 - zeta, dedz, vrs);
+  T vcd_ = fma((T)1 + zeta, -dedz, vrs);
+  exc = ex * ha + ec * ha;
+  vu = vxu * ha + vcu_ * ha;
+  vd = vxd * ha + vcd_ * ha;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lsda(size_t n, const T* __restrict__ ru, const T* __restrict__ rd,
+                                              T* __restrict__ exc, T* __restrict__ vu, T* __restrict__ vd) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T e, u, d;
+  lsda_point<T>(ru[i], rd[i], e, u, d);
+  exc[i] = e;
+  vu[i] = u;
+  vd[i] = d;
+}
+
+__global__ void __launch_bounds__(256) k_d2f(size_t n, const double* __restrict__ s, float* __restrict__ d) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (float)s[i];
+}
+__global__ void __launch_bounds__(256) k_f2d(size_t n, const float* __restrict__ s, double* __restrict__ d) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (double)s[i];
+}
+
+}  // namespace
+
+void launch_hartree_veff(Context& c, const void* rho_g, double fourpi_e2, const void* v_local, const void* v_xc,
+                         void* out, int precision, bool with_local, bool with_xc, bool with_hartree) {
+  c.check_grid();
+  size_t n = c.N;
+  if (precision == PWK_FP64) {
+    k_hartree_veff<double><<<blocks_for(n, 256), 256, 0, c.stream>>>(
+        n, with_hartree ? (const double2*)rho_g : nullptr, c.g2_64.as<double>(), fourpi_e2,
+        with_local ? (const double2*)v_local : nullptr, with_xc ? (const double2*)v_xc : nullptr, (double2*)out);
+  } else {
+    k_hartree_veff<float><<<blocks_for(n, 256), 256, 0, c.stream>>>(
+        n, with_hartree ? (const float2*)rho_g : nullptr, c.g2_32.as<float>(), (float)fourpi_e2,
+        with_local ? (const float2*)v_local : nullptr, with_xc ? (const float2*)v_xc : nullptr, (float2*)out);
+  }
+  PWK_HIP(hipGetLastError());
+}
+
+void launch_veff_sum(Context& c, const void* a, const void* b, const void* d, void* out, int precision) {
+  c.check_grid();
+  size_t n = c.N;
+  if (precision == PWK_FP64)
+    k_sum3<double><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const double2*)a, (const double2*)b,
+                                                             (const double2*)d, (double2*)out);
+  else
+    k_sum3<float><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const float2*)a, (const float2*)b,
+                                                            (const float2*)d, (float2*)out);
+  PWK_HIP(hipGetLastError());
+}
+
+void launch_lda(Context& c, const void* rho, void* exc, void* vxc, int precision) {
+  c.check_grid();
+  size_t n = c.N;
+  if (precision == PWK_FP64)
+    k_lda<double><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const double*)rho, (double*)exc, (double*)vxc);
+  else
+    k_lda<float><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const float*)rho, (float*)exc, (float*)vxc);
+  PWK_HIP(hipGetLastError());
+}
+
+void launch_lsda(Context& c, const void* ru, const void* rd, void* exc, void* vu, void* vd, int precision) {
+  c.check_grid();
+  size_t n = c.N;
+  if (precision == PWK_FP64)
+    k_lsda<double><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const double*)ru, (const double*)rd,
+                                                             (double*)exc, (double*)vu, (double*)vd);
+  else
+    k_lsda<float><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const float*)ru, (const float*)rd, (float*)exc,
+                                                            (float*)vu, (float*)vd);
+  PWK_HIP(hipGetLastError());
+}
+
+void launch_convert(Context& c, const void* src, void* dst, size_t n, bool to_fp32) {
+  if (n == 0) return;
+  if (to_fp32)
+    k_d2f<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const double*)src, (float*)dst);
+  else
+    k_f2d<<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const float*)src, (double*)dst);
+  PWK_HIP(hipGetLastError());
+}
+
+}  // namespace pwk

@@ -1,0 +1,466 @@
+// Local part of H·ψ and the band density on the sphere-pruned FFT.
+//
+// The plane-wave sphere occupies S z-sticks (distinct (n1, n2)); the
+// reference enumerates the basis n1 -> n2 -> n3 (src/basis.rs:47-59), so the
+// coefficients of one stick are contiguous. One H·ψ is three kernels:
+//   k_zinv   : per (band, stick) scatter ψ along z, inverse FFT along z
+//              -> stick buffer T[band][i3][stick]            (S·nz values)
+//   k_plane  : per (band, z-plane) in LDS: y-inverse on the rows carrying
+//              sticks, x-inverse, × V(r), x-forward, y-forward -> T
+//   k_zfwd   : per (band, stick) forward FFT along z, gather the sphere,
+//              Hψ = T_kin ψ + (1/N)·(...)                  (src/scf/potentials.rs:163-190)
+// which equals the dense fill_hamiltonian_with_v_eff product because the
+// wrapped Miller difference there is a circular convolution.
+// The density reuses k_zinv and a plane kernel that accumulates f·|ψ(r)|²
+// (src/scf/density.rs:59-97).
+#include <algorithm>
+#include <map>
+
+#include "common.hpp"
+#include "fft_reg.hpp"
+#include "fft_sizes.hpp"
+
+namespace pwk {
+
+namespace {
+
+template <typename T>
+struct V2t;
+template <>
+struct V2t<double> {
+  using t = double2;
+};
+template <>
+struct V2t<float> {
+  using t = float2;
+};
+
+// index of n3 ≡ i3 (mod nz) inside a stick starting at n3 = lo, or >= cnt if absent
+__device__ __forceinline__ int stick_slot(int i3, int lo, int nz) {
+  int j = i3 - lo;
+  if (j < 0) j += nz;
+  if (j >= nz) j -= nz;
+  return j;
+}
+
+template <typename T, int NZ>
+__global__ void __launch_bounds__(64) k_zinv(const typename V2t<T>::t* __restrict__ psi,
+                                             typename V2t<T>::t* __restrict__ tbuf, int npw, int S,
+                                             const int* __restrict__ st_pw0, const int* __restrict__ st_cnt,
+                                             const int* __restrict__ st_lo) {
+  int s = blockIdx.x * 64 + threadIdx.x;
+  int b = blockIdx.y;
+  if (s >= S) return;
+  int pw0 = st_pw0[s], cnt = st_cnt[s], lo = st_lo[s];
+  const typename V2t<T>::t* src = psi + (size_t)b * npw + pw0;
+  T re[NZ], im[NZ];
+#pragma unroll
+  for (int i3 = 0; i3 < NZ; ++i3) {
+    int j = stick_slot(i3, lo, NZ);
+    typename V2t<T>::t v;
+    v.x = 0;
+    v.y = 0;
+    if (j < cnt) v = src[j];
+    re[i3] = v.x;
+    im[i3] = v.y;
+  }
+  fft_dif<T, NZ, 1>(re, im);
+  typename V2t<T>::t* dst = tbuf + (size_t)b * NZ * S + s;
+#pragma unroll
+  for (int f = 0; f < NZ; ++f) {
+    typename V2t<T>::t v;
+    v.x = re[FftOrder<NZ>::pos(f)];
+    v.y = im[FftOrder<NZ>::pos(f)];
+    dst[(size_t)f * S] = v;
+  }
+}
+
+template <typename T, int NZ>
+__global__ void __launch_bounds__(64) k_zfwd(const typename V2t<T>::t* __restrict__ tbuf,
+                                             const typename V2t<T>::t* __restrict__ psi,
+                                             typename V2t<T>::t* __restrict__ hpsi, const T* __restrict__ kin,
+                                             int npw, int S, const int* __restrict__ st_pw0,
+                                             const int* __restrict__ st_cnt, const int* __restrict__ st_lo,
+                                             T inv_n) {
+  int s = blockIdx.x * 64 + threadIdx.x;
+  int b = blockIdx.y;
+  if (s >= S) return;
+  int pw0 = st_pw0[s], cnt = st_cnt[s], lo = st_lo[s];
+  const typename V2t<T>::t* src = tbuf + (size_t)b * NZ * S + s;
+  T re[NZ], im[NZ];
+#pragma unroll
+  for (int i3 = 0; i3 < NZ; ++i3) {
+    typename V2t<T>::t v = src[(size_t)i3 * S];
+    re[i3] = v.x;
+    im[i3] = v.y;
+  }
+  fft_dif<T, NZ, -1>(re, im);
+  const typename V2t<T>::t* p = psi + (size_t)b * npw + pw0;
+  typename V2t<T>::t* h = hpsi + (size_t)b * npw + pw0;
+  const T* t = kin + pw0;
+#pragma unroll
+  for (int i3 = 0; i3 < NZ; ++i3) {
+    int j = stick_slot(i3, lo, NZ);
+    if (j < cnt) {
+      typename V2t<T>::t ps = p[j];
+      T tk = t[j];
+      typename V2t<T>::t o;
+      o.x = tk * ps.x + re[FftOrder<NZ>::pos(i3)] * inv_n;
+      o.y = tk * ps.y + im[FftOrder<NZ>::pos(i3)] * inv_n;
+      h[j] = o;
+    }
+  }
+}
+
+// LDS plane P[NX][NY+1] (one padding column against bank conflicts on row access)
+template <typename T, int NX, int NY>
+struct Plane {
+  static constexpr int LD = NY + 1;
+  static constexpr size_t bytes = (size_t)NX * LD * sizeof(typename V2t<T>::t);
+};
+
+template <typename T, int NY, int SIGN>
+__device__ __forceinline__ void plane_rows(typename V2t<T>::t* P, int nx1, const int* __restrict__ x1list) {
+  constexpr int LD = NY + 1;
+  for (int r = threadIdx.x; r < nx1; r += blockDim.x) {
+    typename V2t<T>::t* row = P + x1list[r] * LD;
+    T re[NY], im[NY];
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+      re[j] = row[j].x;
+      im[j] = row[j].y;
+    }
+    fft_dif<T, NY, SIGN>(re, im);
+#pragma unroll
+    for (int f = 0; f < NY; ++f) {
+      row[f].x = re[FftOrder<NY>::pos(f)];
+      row[f].y = im[FftOrder<NY>::pos(f)];
+    }
+  }
+}
+
+// H·ψ plane pass for one (band, z-plane).
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__(64) k_plane_hpsi(typename V2t<T>::t* __restrict__ tbuf, const T* __restrict__ vr,
+                                                   int S, int nz, const int* __restrict__ st_xy, int nx1,
+                                                   const int* __restrict__ x1list) {
+  extern __shared__ unsigned char smem_raw[];
+  using V2 = typename V2t<T>::t;
+  constexpr int LD = NY + 1;
+  V2* P = reinterpret_cast<V2*>(smem_raw);
+  const int i3 = blockIdx.x, b = blockIdx.y;
+  V2* tp = tbuf + ((size_t)b * nz + i3) * S;
+  for (int i = threadIdx.x; i < NX * LD; i += blockDim.x) {
+    P[i].x = 0;
+    P[i].y = 0;
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    int xy = st_xy[s];
+    P[(xy / NY) * LD + (xy % NY)] = tp[s];
+  }
+  __syncthreads();
+  plane_rows<T, NY, 1>(P, nx1, x1list);
+  __syncthreads();
+  const T* v = vr + (size_t)i3 * NX * NY;
+  for (int i2 = threadIdx.x; i2 < NY; i2 += blockDim.x) {
+    T re[NX], im[NX];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+      re[j] = P[j * LD + i2].x;
+      im[j] = P[j * LD + i2].y;
+    }
+    fft_dif<T, NX, 1>(re, im);
+    T r2[NX], m2[NX];
+#pragma unroll
+    for (int f = 0; f < NX; ++f) {
+      T vv = v[f * NY + i2];
+      r2[f] = re[FftOrder<NX>::pos(f)] * vv;
+      m2[f] = im[FftOrder<NX>::pos(f)] * vv;
+    }
+    fft_dif<T, NX, -1>(r2, m2);
+#pragma unroll
+    for (int f = 0; f < NX; ++f) {
+      P[f * LD + i2].x = r2[FftOrder<NX>::pos(f)];
+      P[f * LD + i2].y = m2[FftOrder<NX>::pos(f)];
+    }
+  }
+  __syncthreads();
+  plane_rows<T, NY, -1>(P, nx1, x1list);
+  __syncthreads();
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    int xy = st_xy[s];
+    tp[s] = P[(xy / NY) * LD + (xy % NY)];
+  }
+}
+
+// Density plane pass: one block per z-plane, loops over bands, accumulates
+// Σ_b fw_b |ψ_b(r)|² in an FP64 LDS plane, then adds it to rho (row-major grid).
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__(64) k_plane_density(const typename V2t<T>::t* __restrict__ tbuf, int nb,
+                                                      const double* __restrict__ fw, double* __restrict__ rho,
+                                                      int S, int nz, const int* __restrict__ st_xy, int nx1,
+                                                      const int* __restrict__ x1list) {
+  extern __shared__ unsigned char smem_raw[];
+  using V2 = typename V2t<T>::t;
+  constexpr int LD = NY + 1;
+  V2* P = reinterpret_cast<V2*>(smem_raw);
+  double* acc = reinterpret_cast<double*>(smem_raw + Plane<T, NX, NY>::bytes);
+  const int i3 = blockIdx.x;
+  for (int i = threadIdx.x; i < NX * NY; i += blockDim.x) acc[i] = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    double f = fw[b];
+    if (f < 1e-15) continue;  // src/scf/density.rs:14,72
+    const V2* tp = tbuf + ((size_t)b * nz + i3) * S;
+    __syncthreads();
+    for (int i = threadIdx.x; i < NX * LD; i += blockDim.x) {
+      P[i].x = 0;
+      P[i].y = 0;
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < S; s += blockDim.x) {
+      int xy = st_xy[s];
+      P[(xy / NY) * LD + (xy % NY)] = tp[s];
+    }
+    __syncthreads();
+    plane_rows<T, NY, 1>(P, nx1, x1list);
+    __syncthreads();
+    for (int i2 = threadIdx.x; i2 < NY; i2 += blockDim.x) {
+      T re[NX], im[NX];
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        re[j] = P[j * LD + i2].x;
+        im[j] = P[j * LD + i2].y;
+      }
+      fft_dif<T, NX, 1>(re, im);
+#pragma unroll
+      for (int q = 0; q < NX; ++q) {
+        double x = (double)re[FftOrder<NX>::pos(q)], y = (double)im[FftOrder<NX>::pos(q)];
+        acc[q * NY + i2] += f * (x * x + y * y);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NX * NY; i += blockDim.x) {
+    int i1 = i / NY, i2 = i % NY;
+    rho[((size_t)i1 * NY + i2) * nz + i3] += acc[i];
+  }
+}
+
+// V(r) = Re IFFT_unnorm(V_eff(G)) re-ordered to plane-major [i3][i1][i2]
+__global__ void __launch_bounds__(256) k_vr_reorder(const double2* __restrict__ vg_r, double* __restrict__ v64,
+                                                    float* __restrict__ v32, int nx, int ny, int nz) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)nx * ny * nz;
+  if (idx >= n) return;
+  // idx in plane-major order
+  int i2 = (int)(idx % ny);
+  int i1 = (int)((idx / ny) % nx);
+  int i3 = (int)(idx / ((size_t)nx * ny));
+  double v = vg_r[((size_t)i1 * ny + i2) * nz + i3].x;
+  v64[idx] = v;
+  v32[idx] = (float)v;
+}
+
+// ---- dispatch ------------------------------------------------------------------
+template <typename T>
+void hpsi_local(Context& c, int ik, int nb, const void* psi, void* hpsi) {
+  using V2 = typename V2t<T>::t;
+  int nx = c.dims[0], ny = c.dims[1], nz = c.dims[2];
+  PWK_REQUIRE(nx == ny, PWK_E_UNSUPPORTED, "H·psi plane kernels need nx == ny");
+  KData& kd = c.k(ik);
+  PWK_REQUIRE(kd.have_kin, PWK_E_STATE, "kinetic diagonal of this k-point not set (pwk_set_kinetic)");
+  PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
+  size_t tb = (size_t)nb * nz * c.nsticks * sizeof(V2);
+  c.work_a.ensure(tb);
+  V2* T_ = c.work_a.as<V2>();
+  const T* vr = std::is_same<T, double>::value ? (const T*)c.vr64.p : (const T*)c.vr32.p;
+  const T* kin = std::is_same<T, double>::value ? (const T*)kd.kin64.p : (const T*)kd.kin32.p;
+  dim3 gz(blocks_for((size_t)c.nsticks, 64), (unsigned)nb);
+  switch (nz) {
+#define PWK_CASE(L)                                                                                     \
+  case L:                                                                                               \
+    k_zinv<T, L><<<gz, 64, 0, c.stream>>>((const V2*)psi, T_, c.npw, c.nsticks, c.st_pw0.as<int>(),     \
+                                          c.st_cnt.as<int>(), c.st_n3lo.as<int>());                    \
+    break;
+    PWK_FOR_EACH_FFT_SIZE(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "nz has no compiled kernel");
+  }
+  PWK_HIP(hipGetLastError());
+  dim3 gp((unsigned)nz, (unsigned)nb);
+  switch (nx) {
+#define PWK_CASE(L)                                                                                      \
+  case L: {                                                                                              \
+    size_t sh = Plane<T, L, L>::bytes;                                                                   \
+    PWK_HIP(hipFuncSetAttribute((const void*)k_plane_hpsi<T, L, L>,                                      \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));                  \
+    k_plane_hpsi<T, L, L><<<gp, 64, sh, c.stream>>>(T_, vr, c.nsticks, nz, c.st_xy.as<int>(), c.nx1,     \
+                                                    c.x1list.as<int>());                                \
+    break;                                                                                               \
+  }
+    PWK_FOR_EACH_FFT_SIZE(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "nx has no compiled kernel");
+  }
+  PWK_HIP(hipGetLastError());
+  T inv_n = (T)(1.0 / (double)c.N);
+  switch (nz) {
+#define PWK_CASE(L)                                                                                     \
+  case L:                                                                                               \
+    k_zfwd<T, L><<<gz, 64, 0, c.stream>>>(T_, (const V2*)psi, (V2*)hpsi, kin, c.npw, c.nsticks,         \
+                                          c.st_pw0.as<int>(), c.st_cnt.as<int>(), c.st_n3lo.as<int>(),  \
+                                          inv_n);                                                       \
+    break;
+    PWK_FOR_EACH_FFT_SIZE(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "nz has no compiled kernel");
+  }
+  PWK_HIP(hipGetLastError());
+}
+
+template <typename T>
+void density_t(Context& c, int nb, const void* psi, const double* fw_host, void* rho) {
+  using V2 = typename V2t<T>::t;
+  int nx = c.dims[0], ny = c.dims[1], nz = c.dims[2];
+  PWK_REQUIRE(nx == ny, PWK_E_UNSUPPORTED, "density plane kernels need nx == ny");
+  size_t tb = (size_t)nb * nz * c.nsticks * sizeof(V2);
+  c.work_a.ensure(tb);
+  c.work_b.ensure((size_t)nb * sizeof(double));
+  PWK_HIP(hipMemcpyAsync(c.work_b.p, fw_host, (size_t)nb * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  V2* T_ = c.work_a.as<V2>();
+  dim3 gz(blocks_for((size_t)c.nsticks, 64), (unsigned)nb);
+  switch (nz) {
+#define PWK_CASE(L)                                                                                     \
+  case L:                                                                                               \
+    k_zinv<T, L><<<gz, 64, 0, c.stream>>>((const V2*)psi, T_, c.npw, c.nsticks, c.st_pw0.as<int>(),     \
+                                          c.st_cnt.as<int>(), c.st_n3lo.as<int>());                    \
+    break;
+    PWK_FOR_EACH_FFT_SIZE(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "nz has no compiled kernel");
+  }
+  PWK_HIP(hipGetLastError());
+  switch (nx) {
+#define PWK_CASE(L)                                                                                      \
+  case L: {                                                                                              \
+    size_t sh = Plane<T, L, L>::bytes + (size_t)L * L * sizeof(double);                                  \
+    PWK_HIP(hipFuncSetAttribute((const void*)k_plane_density<T, L, L>,                                   \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));                  \
+    k_plane_density<T, L, L><<<(unsigned)nz, 64, sh, c.stream>>>(T_, nb, c.work_b.as<double>(),          \
+                                                                 (double*)rho, c.nsticks, nz,            \
+                                                                 c.st_xy.as<int>(), c.nx1,              \
+                                                                 c.x1list.as<int>());                   \
+    break;                                                                                               \
+  }
+    PWK_FOR_EACH_FFT_SIZE(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "nx has no compiled kernel");
+  }
+  PWK_HIP(hipGetLastError());
+}
+
+inline int wrap(int n, int d) { return ((n % d) + d) % d; }
+
+}  // namespace
+
+void setup_basis(Context& c, int npw, const int32_t* m) {
+  c.check_grid();
+  PWK_REQUIRE(npw > 0 && m != nullptr, PWK_E_INVALID, "empty basis");
+  const int nx = c.dims[0], ny = c.dims[1], nz = c.dims[2];
+  std::vector<int32_t> g2fft((size_t)npw), pw0, cnt, lo, xy, sx1;
+  std::map<int, int> x1index;
+  std::map<int, int> seen_xy;
+  for (int i = 0; i < npw; ++i) {
+    const int n1 = m[3 * i], n2 = m[3 * i + 1], n3 = m[3 * i + 2];
+    g2fft[(size_t)i] = (int32_t)(((size_t)wrap(n1, nx) * ny + wrap(n2, ny)) * nz + wrap(n3, nz));
+    bool cont = i > 0 && m[3 * i - 3] == n1 && m[3 * i - 2] == n2 && m[3 * i - 1] + 1 == n3;
+    if (!cont) {
+      int col = wrap(n1, nx) * ny + wrap(n2, ny);
+      PWK_REQUIRE(seen_xy.find(col) == seen_xy.end(), PWK_E_UNSUPPORTED,
+                  "basis is not ordered n1 -> n2 -> n3 or two sticks alias on the grid");
+      seen_xy[col] = (int)pw0.size();
+      pw0.push_back(i);
+      cnt.push_back(0);
+      lo.push_back(n3);
+      xy.push_back(col);
+      int i1 = wrap(n1, nx);
+      if (!x1index.count(i1)) x1index[i1] = 0;
+    }
+    cnt.back() += 1;
+    PWK_REQUIRE(cnt.back() <= nz, PWK_E_UNSUPPORTED, "a z-stick is longer than nz (basis aliases on the grid)");
+  }
+  std::vector<int32_t> x1list;
+  for (auto& kv : x1index) {
+    kv.second = (int)x1list.size();
+    x1list.push_back(kv.first);
+  }
+  for (size_t s = 0; s < xy.size(); ++s) sx1.push_back(x1index[xy[s] / ny]);
+  c.npw = npw;
+  c.nsticks = (int)pw0.size();
+  c.nx1 = (int)x1list.size();
+  auto up = [&](DevBuf& d, const void* h, size_t bytes) {
+    d.ensure(bytes);
+    PWK_HIP(hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice));
+  };
+  up(c.miller, m, (size_t)npw * 3 * sizeof(int32_t));
+  up(c.g2fft, g2fft.data(), g2fft.size() * sizeof(int32_t));
+  up(c.st_pw0, pw0.data(), pw0.size() * sizeof(int32_t));
+  up(c.st_cnt, cnt.data(), cnt.size() * sizeof(int32_t));
+  up(c.st_n3lo, lo.data(), lo.size() * sizeof(int32_t));
+  up(c.st_xy, xy.data(), xy.size() * sizeof(int32_t));
+  up(c.x1list, x1list.data(), x1list.size() * sizeof(int32_t));
+  up(c.st_x1, sx1.data(), sx1.size() * sizeof(int32_t));
+}
+
+void set_veff_device(Context& c, const void* veff_g_dev) {
+  c.check_grid();
+  size_t n = c.N;
+  c.veffg64.ensure(n * sizeof(double2));
+  c.vr64.ensure(n * sizeof(double));
+  c.vr32.ensure(n * sizeof(float));
+  c.work_b.ensure(n * sizeof(double2));
+  PWK_HIP(hipMemcpyAsync(c.veffg64.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(c.work_b.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
+  launch_fft3d(c, c.work_b.p, 1, +1, PWK_FP64);
+  k_vr_reorder<<<blocks_for(n, 256), 256, 0, c.stream>>>(c.work_b.as<double2>(), c.vr64.as<double>(),
+                                                         c.vr32.as<float>(), c.dims[0], c.dims[1], c.dims[2]);
+  PWK_HIP(hipGetLastError());
+  c.have_v = true;
+}
+
+void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int precision) {
+  c.check_basis();
+  PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
+  KData& kd = c.k(ik);
+  if (precision == PWK_FP64) {
+    hpsi_local<double>(c, ik, nb, psi, hpsi);
+  } else {
+    hpsi_local<float>(c, ik, nb, psi, hpsi);
+  }
+  if (kd.nch > 0) {
+    // P = (D·B^H)·ψ  (nch × nb), then Hψ += B·P   (src/potential/nonlocal.rs:432-450)
+    size_t es = precision == PWK_FP64 ? sizeof(double2) : sizeof(float2);
+    c.proj.ensure((size_t)kd.nch * nb * es);
+    const void* dbh = precision == PWK_FP64 ? kd.DBH64.p : kd.DBH32.p;
+    const void* bm = precision == PWK_FP64 ? kd.B64.p : kd.B32.p;
+    launch_zgemm(c, kd.nch, nb, c.npw, dbh, kd.nch, psi, c.npw, c.proj.p, kd.nch, false, precision);
+    launch_zgemm(c, c.npw, nb, kd.nch, bm, c.npw, c.proj.p, kd.nch, hpsi, c.npw, true, precision);
+  }
+}
+
+void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision) {
+  c.check_basis();
+  PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
+  if (precision == PWK_FP64)
+    density_t<double>(c, nb, psi, fw_host, rho);
+  else
+    density_t<float>(c, nb, psi, fw_host, rho);
+}
+
+}  // namespace pwk

@@ -1,0 +1,291 @@
+"""Python binding of libpwk (include/pwk.h) over ctypes.
+
+Mirrors the reference's GPU backend (pwdft/pwdft-core/src/gpu/mod.rs,
+``GpuAccelerator``) and its Hamiltonian-apply calls
+(src/scf/potentials.rs:163-190, src/potential/nonlocal.rs:432-450,
+src/scf/density.rs:41-111). Host-array methods take and return numpy arrays;
+``*_dev`` methods take device pointers (e.g. ``torch.Tensor.data_ptr()``).
+
+There is no CPU fallback: if libpwk.so is missing or no GPU is present,
+construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(HERE, "libpwk.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "pwk.h")
+
+PWK_FP64, PWK_FP32, PWK_SPLIT_BF16 = 0, 1, 2
+PRECISIONS = {"fp64": PWK_FP64, "fp32": PWK_FP32, "split_bf16": PWK_SPLIT_BF16}
+STATUS = {0: "PWK_OK", 1: "PWK_E_INVALID", 2: "PWK_E_NO_DEVICE", 3: "PWK_E_DEVICE", 4: "PWK_E_OOM",
+          5: "PWK_E_UNSUPPORTED", 6: "PWK_E_STATE"}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+_lib = None
+
+
+class PwkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def header_symbols():
+    """Entry points declared in include/pwk.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(pwk_[a-z0-9_]+)\s*\(", text)))
+
+
+_SIGS = {
+    "pwk_last_error": (C.c_char_p, []),
+    "pwk_version": (C.c_int, []),
+    "pwk_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "pwk_ctx_destroy": (C.c_int, [_vp]),
+    "pwk_ctx_sync": (C.c_int, [_vp]),
+    "pwk_ctx_stream": (_vp, [_vp]),
+    "pwk_prepare": (C.c_int, [_vp, _ip, _dp]),
+    "pwk_hartree": (C.c_int, [_vp, _dp, C.c_double, _dp, C.c_int]),
+    "pwk_veff": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_int]),
+    "pwk_hartree_veff": (C.c_int, [_vp, _dp, C.c_double, _dp, _dp, _dp, C.c_int]),
+    "pwk_lda_xc": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int]),
+    "pwk_lsda_xc": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_int]),
+    "pwk_dev_hartree_veff": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp, _vp, C.c_int]),
+    "pwk_dev_lda_xc": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
+    "pwk_dev_lsda_xc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
+    "pwk_fft3d": (C.c_int, [_vp, _dp, C.c_int, C.c_int]),
+    "pwk_dev_fft3d": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int]),
+    "pwk_set_basis": (C.c_int, [_vp, C.c_int, _ip]),
+    "pwk_set_nk": (C.c_int, [_vp, C.c_int]),
+    "pwk_set_kinetic": (C.c_int, [_vp, C.c_int, _dp]),
+    "pwk_set_veff": (C.c_int, [_vp, _dp]),
+    "pwk_dev_set_veff": (C.c_int, [_vp, _vp]),
+    "pwk_set_nonlocal": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    "pwk_build_nonlocal": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, _dp, _ip, C.c_int, _ip, _ip, _dp, _ip, _dp,
+                                     _dp, _dp, C.c_double]),
+    "pwk_nonlocal_channels": (C.c_int, [_vp, C.c_int]),
+    "pwk_hpsi": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, C.c_int]),
+    "pwk_dev_hpsi": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int]),
+    "pwk_dense_h": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "pwk_dev_dense_h": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+    "pwk_density_accumulate": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_int]),
+    "pwk_dev_density_accumulate": (C.c_int, [_vp, C.c_int, _vp, _dp, _vp, C.c_int]),
+}
+
+
+def lib():
+    """Loads libpwk.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing; build it with pwdft-rs_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != 0:
+        raise PwkError(code, lib().pwk_last_error().decode())
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _prec(p):
+    return PRECISIONS[p] if isinstance(p, str) else int(p)
+
+
+class GpuAccelerator:
+    """Counterpart of the reference's ``GpuAccelerator`` (src/gpu/mod.rs:36-44).
+
+    ``try_new`` returns None when no device is present, like the reference;
+    the constructor raises instead.
+    """
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().pwk_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.dims = None
+        self.n_grid = 0
+        self.npw = 0
+
+    @classmethod
+    def try_new(cls, device: int = 0):
+        try:
+            return cls(device)
+        except PwkError as e:
+            if e.code == 2:
+                return None
+            raise
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pwk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return lib().pwk_ctx_stream(self.h)
+
+    def sync(self):
+        _check(lib().pwk_ctx_sync(self.h))
+
+    # ---- src/gpu/mod.rs:197-364 -------------------------------------------------
+    def prepare_buffers(self, dims, g_squared):
+        d = np.ascontiguousarray(dims, dtype=np.int32)
+        g2 = _f64(g_squared)
+        assert g2.size == int(np.prod(d))
+        _check(lib().pwk_prepare(self.h, d.ctypes.data_as(_ip), _d(g2)))
+        self.dims = tuple(int(x) for x in d)
+        self.n_grid = int(np.prod(d))
+
+    # ---- src/gpu/mod.rs:398-494 -------------------------------------------------
+    def hartree_potential(self, rho_g, fourpi_e2, precision="fp64"):
+        r = _c128(rho_g)
+        out = np.empty_like(r)
+        _check(lib().pwk_hartree(self.h, _d(r.view(np.float64)), fourpi_e2, _d(out.view(np.float64)),
+                                 _prec(precision)))
+        return out
+
+    # ---- src/gpu/mod.rs:508-610 -------------------------------------------------
+    def v_eff_assembly(self, v_local, v_h, v_xc, precision="fp64"):
+        a, b, c = _c128(v_local), _c128(v_h), _c128(v_xc)
+        out = np.empty_like(a)
+        _check(lib().pwk_veff(self.h, _d(a.view(np.float64)), _d(b.view(np.float64)), _d(c.view(np.float64)),
+                              _d(out.view(np.float64)), _prec(precision)))
+        return out
+
+    def hartree_veff(self, rho_g, fourpi_e2, v_local, v_xc, precision="fp64"):
+        r, a, c = _c128(rho_g), _c128(v_local), _c128(v_xc)
+        out = np.empty_like(a)
+        _check(lib().pwk_hartree_veff(self.h, _d(r.view(np.float64)), fourpi_e2, _d(a.view(np.float64)),
+                                      _d(c.view(np.float64)), _d(out.view(np.float64)), _prec(precision)))
+        return out
+
+    # ---- src/gpu/mod.rs:627-725 -------------------------------------------------
+    def lda_xc(self, rho_r, precision="fp64"):
+        r = _f64(rho_r)
+        e, v = np.empty_like(r), np.empty_like(r)
+        _check(lib().pwk_lda_xc(self.h, _d(r), _d(e), _d(v), _prec(precision)))
+        return e, v
+
+    def lsda_xc(self, rho_up, rho_dn, precision="fp64"):
+        u, d = _f64(rho_up), _f64(rho_dn)
+        e, vu, vd = np.empty_like(u), np.empty_like(u), np.empty_like(u)
+        _check(lib().pwk_lsda_xc(self.h, _d(u), _d(d), _d(e), _d(vu), _d(vd), _prec(precision)))
+        return e, vu, vd
+
+    def fft3d(self, data, sign, precision="fp64"):
+        a = _c128(data).copy()
+        _check(lib().pwk_fft3d(self.h, _d(a.view(np.float64)), int(sign), _prec(precision)))
+        return a
+
+    # ---- Hamiltonian apply ----------------------------------------------------------
+    def set_basis(self, miller):
+        m = np.ascontiguousarray(miller, dtype=np.int32)
+        _check(lib().pwk_set_basis(self.h, m.shape[0], m.ctypes.data_as(_ip)))
+        self.npw = int(m.shape[0])
+
+    def set_nk(self, nk):
+        _check(lib().pwk_set_nk(self.h, int(nk)))
+
+    def set_kinetic(self, ik, t):
+        _check(lib().pwk_set_kinetic(self.h, ik, _d(_f64(t))))
+
+    def set_veff(self, v_eff_g):
+        v = _c128(v_eff_g)
+        _check(lib().pwk_set_veff(self.h, _d(v.view(np.float64))))
+
+    def set_veff_dev(self, ptr):
+        _check(lib().pwk_dev_set_veff(self.h, ptr))
+
+    def set_nonlocal(self, ik, B, DBH):
+        """B: (npw, nch) complex; DBH: (nch, npw) complex (the reference's faer layouts)."""
+        if B is None or B.shape[1] == 0:
+            _check(lib().pwk_set_nonlocal(self.h, ik, 0, None, None))
+            return
+        Bc = _c128(np.asarray(B).T)   # column-major npw×nch
+        Dc = _c128(np.asarray(DBH).T)  # column-major nch×npw
+        _check(lib().pwk_set_nonlocal(self.h, ik, B.shape[1], _d(Bc.view(np.float64)), _d(Dc.view(np.float64))))
+
+    def build_nonlocal(self, ik, k, gvec, tau, atom_species, species, omega):
+        """Builds B and D·B^H of k-point ik on the device (src/potential/nonlocal.rs:190-382).
+
+        species: list of dicts with keys l (projector l's), dij (nproj×nproj, eV),
+        r, rab (radial mesh, Å) and chi (nproj×mesh, χ = r·β in Å^-1/2).
+        Returns the number of channels.
+        """
+        nproj = [len(s["l"]) for s in species]
+        proj_l = np.concatenate([np.asarray(s["l"], dtype=np.int32) for s in species])
+        dij = np.concatenate([_f64(s["dij"]).reshape(-1) for s in species])
+        mesh = [len(s["r"]) for s in species]
+        r = np.concatenate([_f64(s["r"]) for s in species])
+        rab = np.concatenate([_f64(s["rab"]) for s in species])
+        chi = np.concatenate([_f64(s["chi"]).reshape(-1) for s in species])
+        return self.build_nonlocal_raw(ik, k, gvec, tau, atom_species, nproj, proj_l, dij, mesh, r, rab, chi, omega)
+
+    def build_nonlocal_raw(self, ik, k, gvec, tau, atom_species, nproj, proj_l, dij, mesh, r, rab, chi, omega):
+        asp = np.ascontiguousarray(atom_species, dtype=np.int32)
+        npj = np.ascontiguousarray(nproj, dtype=np.int32)
+        pl = np.ascontiguousarray(proj_l, dtype=np.int32)
+        ms = np.ascontiguousarray(mesh, dtype=np.int32)
+        _check(lib().pwk_build_nonlocal(self.h, ik, _d(_f64(k)), _d(_f64(gvec).reshape(-1)), len(asp),
+                                        _d(_f64(tau).reshape(-1)), asp.ctypes.data_as(_ip), len(npj),
+                                        npj.ctypes.data_as(_ip), pl.ctypes.data_as(_ip), _d(_f64(dij)),
+                                        ms.ctypes.data_as(_ip), _d(_f64(r)), _d(_f64(rab)), _d(_f64(chi)),
+                                        float(omega)))
+        return lib().pwk_nonlocal_channels(self.h, ik)
+
+    def hpsi(self, ik, psi, precision="fp64"):
+        """psi: (nb, npw) complex128, band-major. Returns (nb, npw)."""
+        p = _c128(psi)
+        out = np.empty_like(p)
+        _check(lib().pwk_hpsi(self.h, ik, p.shape[0], _d(p.view(np.float64)), _d(out.view(np.float64)),
+                              _prec(precision)))
+        return out
+
+    def hpsi_dev(self, ik, nb, psi_ptr, hpsi_ptr, precision="fp32"):
+        _check(lib().pwk_dev_hpsi(self.h, ik, nb, psi_ptr, hpsi_ptr, _prec(precision)))
+
+    def dense_h(self, ik, with_nl=True):
+        h = np.empty((self.npw, self.npw), dtype=np.complex128)  # filled column-major
+        _check(lib().pwk_dense_h(self.h, ik, int(with_nl), _d(h.view(np.float64))))
+        return h.T.copy()
+
+    def density(self, psi, fw, rho=None, precision="fp64"):
+        p = _c128(psi)
+        acc = np.zeros(self.n_grid) if rho is None else _f64(rho).copy()
+        _check(lib().pwk_density_accumulate(self.h, p.shape[0], _d(p.view(np.float64)), _d(_f64(fw)), _d(acc),
+                                            _prec(precision)))
+        return acc
+
+    def density_dev(self, nb, psi_ptr, fw, rho_ptr, precision="fp32"):
+        _check(lib().pwk_dev_density_accumulate(self.h, nb, psi_ptr, _d(_f64(fw)), rho_ptr, _prec(precision)))

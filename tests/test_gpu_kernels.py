@@ -1,0 +1,215 @@
+"""GPU parity of libpwk against the FP64 oracle, through the C ABI.
+
+Tolerances: FP64 paths agree with the oracle to ~1e-12 relative (different
+summation order only); FP32 paths to the single-precision bound stated per
+test (the reference's own GPU-vs-CPU tolerances are 0.01/0.015 eV on XC,
+tests/gpu_consistency.rs:144-193).
+"""
+import numpy as np
+import pytest
+
+from conftest import SI_UPF
+
+pytestmark = pytest.mark.gpu
+
+A_SI = 5.431
+
+
+def si_lat():
+    h = A_SI / 2.0
+    return np.array([[0.0, h, h], [h, 0.0, h], [h, h, 0.0]])
+
+
+SI_FRAC = [[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]]
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def si_sys(oracle):
+    # Si FCC at 150 eV: reference grid rule gives an 18³ grid
+    s = oracle.System(si_lat(), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 150.0)
+    return s
+
+
+@pytest.fixture()
+def acc(gpu, si_sys):
+    gpu.prepare_buffers(si_sys.dims, si_sys.g2())
+    return gpu
+
+
+def test_grid_dims_of_deck(si_sys):
+    print("deck dims", si_sys.dims, "npw", si_sys.npw, flush=True)
+    assert si_sys.dims[0] == si_sys.dims[1]
+
+
+# ---- grid kernels ---------------------------------------------------------------
+def test_hartree_veff(acc, si_sys, oracle):
+    rng = np.random.default_rng(11)
+    n = si_sys.n_grid
+    rho_g = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    vl, vx = si_sys.vloc(), rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    g2 = si_sys.g2()
+    fe = 4 * np.pi * 14.399645351950548
+    vh_ref = oracle.hartree(rho_g, g2)
+    veff_ref = oracle.veff(vl, vh_ref, vx)
+    vh = acc.hartree_potential(rho_g, fe, "fp64")
+    assert np.array_equal(vh, vh_ref)  # same operations, same order
+    assert np.array_equal(acc.v_eff_assembly(vl, vh_ref, vx, "fp64"), veff_ref)
+    assert np.array_equal(acc.hartree_veff(rho_g, fe, vl, vx, "fp64"), veff_ref)
+    assert vh[0] == 0  # G = 0 removed
+    assert rel(acc.hartree_potential(rho_g, fe, "fp32"), vh_ref) < 2e-6
+    assert rel(acc.hartree_veff(rho_g, fe, vl, vx, "fp32"), veff_ref) < 2e-6
+
+
+def test_lda_xc(acc, si_sys, oracle):
+    n = si_sys.n_grid
+    rho = np.logspace(-4, 1, n)  # the reference's GPU sweep range (gpu_consistency.rs:144-193)
+    rho[:5] = [0.0, 1e-31, 1e-20, 0.25, 3.0]
+    e_ref, v_ref = oracle.lda_xc(rho)
+    e, v = acc.lda_xc(rho, "fp64")
+    assert rel(e, e_ref) < 1e-14 and rel(v, v_ref) < 1e-14
+    assert e[0] == 0 and v[0] == 0 and e[1] == 0
+    e32, v32 = acc.lda_xc(rho, "fp32")
+    assert np.abs(e32 - e_ref).max() < 0.01 and np.abs(v32 - v_ref).max() < 0.015
+    assert rel(e32, e_ref) < 1e-5
+
+
+def test_lsda_xc(acc, si_sys, oracle):
+    n = si_sys.n_grid
+    rng = np.random.default_rng(2)
+    rt = np.logspace(-4, 1, n)
+    zeta = rng.uniform(-1, 1, n)
+    zeta[:3] = [-1.0, 0.0, 1.0]
+    ru, rd = rt * (1 + zeta) / 2, rt * (1 - zeta) / 2
+    ref = oracle.lsda_xc(ru, rd)
+    out = acc.lsda_xc(ru, rd, "fp64")
+    for a, b in zip(out, ref):
+        assert rel(a, b) < 1e-13
+    out32 = acc.lsda_xc(ru, rd, "fp32")
+    for a, b in zip(out32, ref):
+        assert np.abs(a - b).max() < 0.015
+
+
+# ---- FFT ------------------------------------------------------------------------
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 3e-6)])
+def test_fft3d_matches_dft(gpu, prec, tol):
+    rng = np.random.default_rng(4)
+    for dims in [(15, 15, 15), (16, 18, 20), (24, 25, 27), (60, 60, 60)]:
+        gpu.prepare_buffers(dims, np.zeros(int(np.prod(dims))))
+        x = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
+        f = gpu.fft3d(x.reshape(-1), -1, prec).reshape(dims)
+        assert rel(f, np.fft.fftn(x)) < tol, dims
+        b = gpu.fft3d(x.reshape(-1), +1, prec).reshape(dims)
+        assert rel(b, np.fft.ifftn(x) * x.size) < tol, dims
+
+
+def test_unsupported_grid_is_an_error(gpu, pwk_mod):
+    gpu.prepare_buffers((7, 7, 7), np.zeros(343))
+    with pytest.raises(pwk_mod.PwkError) as e:
+        gpu.fft3d(np.zeros(343, complex), -1)
+    assert e.value.code == 5
+
+
+# ---- Hamiltonian -----------------------------------------------------------------
+@pytest.fixture()
+def ham(acc, si_sys, oracle):
+    rng = np.random.default_rng(7)
+    vr = rng.uniform(-1, 1, si_sys.dims) * 13.6
+    veff = np.fft.fftn(vr).reshape(-1) / si_sys.n_grid
+    k = np.array([0.11, -0.23, 0.05])
+    acc.set_basis(si_sys.miller())
+    acc.set_nk(2)
+    acc.set_kinetic(0, si_sys.kinetic(k))
+    acc.set_kinetic(1, si_sys.kinetic([0, 0, 0]))
+    acc.set_veff(veff)
+    B, DBH, _ = si_sys.nonlocal_(k)
+    return dict(veff=veff, k=k, B=B, DBH=DBH)
+
+
+def test_state_errors(gpu, si_sys, pwk_mod):
+    gpu.prepare_buffers(si_sys.dims, si_sys.g2())
+    gpu.set_basis(si_sys.miller())
+    gpu.set_nk(1)
+    gpu.set_kinetic(0, si_sys.kinetic([0, 0, 0]))
+    psi = np.ones((1, si_sys.npw), complex)
+    with pytest.raises(pwk_mod.PwkError) as e:
+        gpu.hpsi(0, psi)  # V_eff not set
+    assert e.value.code == 6
+    with pytest.raises(pwk_mod.PwkError) as e:
+        gpu.hpsi(3, psi)  # k index out of range
+    assert e.value.code == 1
+
+
+def test_dense_h_local_bitwise(acc, ham, si_sys):
+    h = acc.dense_h(0, with_nl=False)
+    ref = si_sys.dense_h(ham["k"], ham["veff"], with_nl=False)
+    assert np.array_equal(h, ref)
+
+
+def test_dense_h_with_nonlocal(acc, ham, si_sys):
+    acc.set_nonlocal(0, ham["B"], ham["DBH"])
+    h = acc.dense_h(0, with_nl=True)
+    ref = si_sys.dense_h(ham["k"], ham["veff"], with_nl=True)
+    assert rel(h, ref) < 1e-13
+    assert rel(h, h.conj().T) < 1e-13
+
+
+def test_device_nonlocal_build(acc, ham, si_sys, oracle):
+    from pwk import decks
+
+    u = decks.read_upf(SI_UPF)
+    nch = acc.build_nonlocal(0, ham["k"], si_sys.gvec(), np.array(SI_FRAC) @ si_lat(), [0, 0],
+                             [dict(l=u["l"], dij=u["dij"], r=u["r"], rab=u["rab"], chi=u["chi"])], si_sys.omega)
+    assert nch == ham["B"].shape[1] == 36
+    h = acc.dense_h(0, with_nl=True)
+    ref = si_sys.dense_h(ham["k"], ham["veff"], with_nl=True)
+    assert rel(h, ref) < 1e-12
+
+
+@pytest.mark.parametrize("with_nl", [False, True])
+def test_hpsi_fp64_matches_dense(acc, ham, si_sys, with_nl):
+    rng = np.random.default_rng(9)
+    nb = 7
+    psi = rng.standard_normal((nb, si_sys.npw)) + 1j * rng.standard_normal((nb, si_sys.npw))
+    if with_nl:
+        acc.set_nonlocal(0, ham["B"], ham["DBH"])
+    else:
+        acc.set_nonlocal(0, None, None)
+    H = si_sys.dense_h(ham["k"], ham["veff"], with_nl=with_nl)
+    ref = (H @ psi.T).T
+    out = acc.hpsi(0, psi, "fp64")
+    assert rel(out, ref) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("split_bf16", 2e-5)])
+def test_hpsi_single_precision(acc, ham, si_sys, prec, tol):
+    rng = np.random.default_rng(10)
+    nb = 5
+    psi = rng.standard_normal((nb, si_sys.npw)) + 1j * rng.standard_normal((nb, si_sys.npw))
+    acc.set_nonlocal(0, ham["B"], ham["DBH"])
+    ref = si_sys.hpsi_with_nl(ham["k"], ham["veff"], ham["B"], ham["DBH"], psi)
+    out = acc.hpsi(0, psi, prec)
+    assert rel(out, ref) < tol
+
+
+def test_hpsi_second_kpoint_and_single_band(acc, ham, si_sys):
+    rng = np.random.default_rng(12)
+    psi = rng.standard_normal((1, si_sys.npw)) + 1j * rng.standard_normal((1, si_sys.npw))
+    acc.set_nonlocal(1, None, None)
+    H = si_sys.dense_h([0, 0, 0], ham["veff"], with_nl=False)
+    assert rel(acc.hpsi(1, psi, "fp64"), (H @ psi.T).T) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_density_accumulate(acc, ham, si_sys, prec, tol):
+    rng = np.random.default_rng(13)
+    nb = 6
+    psi = rng.standard_normal((nb, si_sys.npw)) + 1j * rng.standard_normal((nb, si_sys.npw))
+    fw = np.array([2.0, 1.5, 1e-16, 0.25, 0.0, 0.75])  # bands below 1e-15 are skipped
+    rho0 = rng.uniform(0, 1, si_sys.n_grid)
+    ref = si_sys.density(psi, fw, rho0)
+    out = acc.density(psi, fw, rho0, prec)
+    assert rel(out, ref) < tol

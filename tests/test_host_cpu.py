@@ -1,0 +1,104 @@
+"""CPU checks of the boundary and of the host-side setup (no GPU calls)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import SI_UPF, C_UPF
+
+
+def test_library_exports_every_header_symbol(pwk_mod):
+    syms = pwk_mod.header_symbols()
+    assert len(syms) >= 30
+    L = ctypes.CDLL(pwk_mod.LIB_PATH)
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert missing == []
+    assert pwk_mod.lib().pwk_version() == 100
+
+
+def test_status_codes_match_header(pwk_mod):
+    text = open(pwk_mod.HEADER).read()
+    for code, name in pwk_mod.STATUS.items():
+        assert f"{name} = {code}" in text
+
+
+def test_null_context_is_rejected(pwk_mod):
+    L = pwk_mod.lib()
+    rc = L.pwk_ctx_sync(None)
+    assert rc == 1  # PWK_E_INVALID
+    assert b"null context" in L.pwk_last_error()
+
+
+def test_no_device_means_none(pwk_mod):
+    # GpuAccelerator::try_new returns None without a device (src/gpu/mod.rs:148-183)
+    h = ctypes.c_void_p()
+    rc = pwk_mod.lib().pwk_ctx_create(0, ctypes.byref(h))
+    if rc == 0:
+        pwk_mod.lib().pwk_ctx_destroy(h)
+        pytest.skip("a GPU is present")
+    assert rc == 2
+    assert pwk_mod.GpuAccelerator.try_new() is None
+
+
+@pytest.mark.parametrize("deck", ["si_fcc_200", "si_fcc_12ry_1026", "si64_20ry", "c64_30ry", "si8_16ry"])
+def test_host_basis_and_grid_bit_identical_to_oracle(oracle, deck):
+    from pwk import decks
+
+    if deck == "si_fcc_200":
+        lat, frac = decks.fcc_primitive(5.431)
+        ecut = 200.0
+    elif deck == "si_fcc_12ry_1026":
+        lat, frac = decks.fcc_primitive(10.26 * decks.BOHR_TO_ANG)
+        ecut = 12 * decks.RY_TO_EV
+    elif deck == "si64_20ry":
+        lat, frac = decks.diamond_supercell(10.26 * decks.BOHR_TO_ANG, (2, 2, 2))
+        ecut = 20 * decks.RY_TO_EV
+    elif deck == "c64_30ry":
+        lat, frac = decks.diamond_supercell(6.74 * decks.BOHR_TO_ANG, (2, 2, 2))
+        ecut = 30 * decks.RY_TO_EV
+    else:
+        lat, frac = decks.diamond_supercell(10.26 * decks.BOHR_TO_ANG)
+        ecut = 16 * decks.RY_TO_EV
+    m, g = decks.basis(lat, ecut)
+    mo, go = oracle.basis(lat, ecut)
+    assert np.array_equal(m, mo)
+    assert np.array_equal(g, go)  # bitwise: same operation order
+    dims = decks.fft_dims(m)
+    if deck == "si_fcc_200":
+        s = oracle.System(lat, [0] * len(frac), [14] * len(frac), frac, [SI_UPF], ecut)
+        assert tuple(s.dims) == dims
+        assert np.array_equal(decks.grid_g2(lat, dims), s.g2())
+    # the workload sizes quoted for the configs
+    expected = {"si_fcc_12ry_1026": (169, (15, 15, 15)), "si64_20ry": (13133, (60, 60, 60)),
+                "c64_30ry": (6859, (45, 45, 45)), "si8_16ry": (1189, (25, 25, 25))}
+    if deck in expected:
+        assert (len(m), dims) == expected[deck]
+
+
+def test_host_upf_reader_matches_oracle(oracle):
+    from pwk import decks
+
+    for path in (SI_UPF, C_UPF):
+        u = decks.read_upf(path)
+        o = oracle.Upf(path)
+        assert u["z_valence"] == o.z_valence
+        assert u["l"] == [o.proj_l(i) for i in range(o.n_proj)]
+        assert np.array_equal(u["r"], o.array(0)) and np.array_equal(u["rab"], o.array(1))
+        assert np.array_equal(u["dij"], o.dij())
+        for i in range(o.n_proj):
+            assert np.array_equal(u["chi"][i], o.array(10 + i))
+
+
+def test_seeded_inputs_are_reproducible():
+    from pwk import decks
+
+    g2 = np.linspace(0, 10, 50)
+    a = decks.seeded_psi(g2, 3, 20261202)
+    b = decks.seeded_psi(g2, 3, 20261202)
+    assert np.array_equal(a, b)
+    assert np.allclose(np.linalg.norm(a, axis=1), 1.0)
+    v = decks.seeded_veff_g((6, 6, 6), 7)
+    vr = np.fft.ifftn(v.reshape(6, 6, 6)) * 216
+    assert np.abs(vr.imag).max() < 1e-12
+    assert np.abs(vr.real).max() <= decks.RY_TO_EV
