@@ -65,6 +65,18 @@ int pwk_ctx_sync(pwk_ctx* ctx);
 void* pwk_ctx_stream(pwk_ctx* ctx);
 
 /* ---------------------------------------------------------------------------
+ * Kernel timing (no counterpart in the reference, which has no GPU timers):
+ * when enabled, every kernel the context launches is bracketed with HIP
+ * events on its stream; pwk_profile_read synchronises and returns the total
+ * device time and launch count per kernel name.
+ * ------------------------------------------------------------------------- */
+int pwk_profile_enable(pwk_ctx* ctx, int on);
+int pwk_profile_reset(pwk_ctx* ctx);
+/* Fills up to cap entries: names (cap × 64 chars, NUL-terminated), total_ms,
+ * launches. Returns the number of kernel names recorded (may exceed cap). */
+int pwk_profile_read(pwk_ctx* ctx, int cap, char* names, double* total_ms, int64_t* launches);
+
+/* ---------------------------------------------------------------------------
  * Grid kernels. Mirrors GpuAccelerator (src/gpu/mod.rs).
  * ------------------------------------------------------------------------- */
 /* prepare_buffers (src/gpu/mod.rs:197-364): FFT dims and |G|² per grid point

@@ -75,6 +75,7 @@ def lib():
             "orc_symmetry": (C.c_int, [_dp, C.c_int, _ip, _dp, C.c_double, C.c_int, _ip, _dp]),
             "orc_kpoints": (C.c_int, [_dp, C.c_int, _ip, _dp, _ip, _ip, C.c_int, C.c_double, _dp, _dp]),
             "orc_scf": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_set_threads": (C.c_int, [C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -98,6 +99,11 @@ def f64(a):
 
 def c128(a):
     return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def set_threads(n: int) -> int:
+    """Sets the OpenMP thread count of the oracle; returns the count in effect."""
+    return lib().orc_set_threads(n)
 
 
 def const(name: str) -> float:

@@ -339,3 +339,9 @@ int orc_scf(void* sp, int nk, const double* k, const double* w, const double* pa
 }
 
 }  // extern "C"
+
+#include <omp.h>
+extern "C" int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}

@@ -352,12 +352,13 @@ void hpsi_fft(const System& s, V3 k, const NonlocalK* nl, const cplx* v_eff_g, c
   std::vector<cplx> vr(v_eff_g, v_eff_g + N);
   fft3d(vr, s.dims, +1);  // V(r) = IFFT_unnorm(V_eff(G))
   std::vector<double> t = kinetic_diag(s, k);
-  std::vector<cplx> work(N);
   double invN = 1.0 / (double)N;
+  // bands are independent: one band per thread
+#pragma omp parallel for schedule(dynamic)
   for (size_t b = 0; b < nb; ++b) {
+    std::vector<cplx> work(N);
     const cplx* p = psi + b * npw;
     cplx* hp = hpsi + b * npw;
-    std::fill(work.begin(), work.end(), cplx(0.0, 0.0));
     for (size_t ig = 0; ig < npw; ++ig) work[s.g_to_fft[ig]] = p[ig];
     fft3d(work, s.dims, +1);
     for (size_t i = 0; i < N; ++i) work[i] *= vr[i];

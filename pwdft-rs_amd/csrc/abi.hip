@@ -126,6 +126,41 @@ int pwk_ctx_sync(pwk_ctx* ctx) {
 
 void* pwk_ctx_stream(pwk_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
 
+int pwk_profile_enable(pwk_ctx* ctx, int on) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    c.prof.on = on != 0;
+  });
+}
+
+int pwk_profile_reset(pwk_ctx* ctx) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    c.prof.collect();
+    c.prof.totals.clear();
+  });
+}
+
+int pwk_profile_read(pwk_ctx* ctx, int cap, char* names, double* total_ms, int64_t* launches) {
+  int n = 0;
+  int st = guarded([&] {
+    Context& c = ctx_of(ctx);
+    c.prof.collect();
+    for (auto& kv : c.prof.totals) {
+      if (n < cap) {
+        if (names) {
+          std::strncpy(names + 64 * n, kv.first.c_str(), 63);
+          names[64 * n + 63] = 0;
+        }
+        if (total_ms) total_ms[n] = kv.second.first;
+        if (launches) launches[n] = kv.second.second;
+      }
+      ++n;
+    }
+  });
+  return st == PWK_OK ? n : -st;
+}
+
 int pwk_prepare(pwk_ctx* ctx, const int dims[3], const double* g2) {
   return guarded([&] {
     Context& c = ctx_of(ctx);

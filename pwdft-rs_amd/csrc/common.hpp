@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -65,6 +66,48 @@ struct DevBuf {
   }
 };
 
+// Per-kernel device timing with HIP events on the context stream (opt-in).
+struct Profiler {
+  bool on = false;
+  struct Pending {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long>> totals;  // name -> (ms, launches)
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    PWK_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void collect() {
+    for (auto& p : pending) {
+      PWK_HIP(hipEventSynchronize(p.b));
+      float ms = 0;
+      PWK_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+      auto& t = totals[p.name];
+      t.first += ms;
+      t.second += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~Profiler() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 // Per-k-point device data.
 struct KData {
   bool have_kin = false;
@@ -113,6 +156,8 @@ struct Context {
   DevBuf gemm_ws;         // split-K partial sums
   DevBuf host_in, host_out;  // staging for host-pointer entry points
 
+  Profiler prof;
+
   void check_grid() const { PWK_REQUIRE(N > 0, PWK_E_STATE, "pwk_prepare has not been called"); }
   void check_basis() const {
     check_grid();
@@ -155,5 +200,25 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
                            const double* chi, double omega);
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Brackets a launch with events when the context profiles.
+struct TimedScope {
+  Context& c;
+  const char* name;
+  hipEvent_t a = nullptr;
+  TimedScope(Context& cc, const char* n) : c(cc), name(n) {
+    if (c.prof.on) {
+      a = c.prof.get();
+      PWK_HIP(hipEventRecord(a, c.stream));
+    }
+  }
+  ~TimedScope() noexcept(false) {
+    if (a) {
+      hipEvent_t b = c.prof.get();
+      PWK_HIP(hipEventRecord(b, c.stream));
+      c.prof.pending.push_back({name, a, b});
+    }
+  }
+};
 
 }  // namespace pwk

@@ -277,19 +277,23 @@ void hpsi_local(Context& c, int ik, int nb, const void* psi, void* hpsi) {
   const T* vr = std::is_same<T, double>::value ? (const T*)c.vr64.p : (const T*)c.vr32.p;
   const T* kin = std::is_same<T, double>::value ? (const T*)kd.kin64.p : (const T*)kd.kin32.p;
   dim3 gz(blocks_for((size_t)c.nsticks, 64), (unsigned)nb);
-  switch (nz) {
+  {
+    TimedScope ts(c, "hpsi_zinv");
+    switch (nz) {
 #define PWK_CASE(L)                                                                                     \
   case L:                                                                                               \
     k_zinv<T, L><<<gz, 64, 0, c.stream>>>((const V2*)psi, T_, c.npw, c.nsticks, c.st_pw0.as<int>(),     \
                                           c.st_cnt.as<int>(), c.st_n3lo.as<int>());                    \
     break;
-    PWK_FOR_EACH_FFT_SIZE(PWK_CASE)
+      PWK_FOR_EACH_FFT_SIZE(PWK_CASE)
 #undef PWK_CASE
-    default:
-      throw Error(PWK_E_UNSUPPORTED, "nz has no compiled kernel");
+      default:
+        throw Error(PWK_E_UNSUPPORTED, "nz has no compiled kernel");
+    }
+    PWK_HIP(hipGetLastError());
   }
-  PWK_HIP(hipGetLastError());
   dim3 gp((unsigned)nz, (unsigned)nb);
+  TimedScope ts_plane(c, "hpsi_plane");
   switch (nx) {
 #define PWK_CASE(L)                                                                                      \
   case L: {                                                                                              \
@@ -307,6 +311,7 @@ void hpsi_local(Context& c, int ik, int nb, const void* psi, void* hpsi) {
   }
   PWK_HIP(hipGetLastError());
   T inv_n = (T)(1.0 / (double)c.N);
+  TimedScope ts_zfwd(c, "hpsi_zfwd");
   switch (nz) {
 #define PWK_CASE(L)                                                                                     \
   case L:                                                                                               \
@@ -449,7 +454,11 @@ void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int pr
     c.proj.ensure((size_t)kd.nch * nb * es);
     const void* dbh = precision == PWK_FP64 ? kd.DBH64.p : kd.DBH32.p;
     const void* bm = precision == PWK_FP64 ? kd.B64.p : kd.B32.p;
-    launch_zgemm(c, kd.nch, nb, c.npw, dbh, kd.nch, psi, c.npw, c.proj.p, kd.nch, false, precision);
+    {
+      TimedScope ts(c, "hpsi_nl_project");
+      launch_zgemm(c, kd.nch, nb, c.npw, dbh, kd.nch, psi, c.npw, c.proj.p, kd.nch, false, precision);
+    }
+    TimedScope ts(c, "hpsi_nl_expand");
     launch_zgemm(c, c.npw, nb, kd.nch, bm, c.npw, c.proj.p, kd.nch, hpsi, c.npw, true, precision);
   }
 }

@@ -51,6 +51,9 @@ _SIGS = {
     "pwk_ctx_destroy": (C.c_int, [_vp]),
     "pwk_ctx_sync": (C.c_int, [_vp]),
     "pwk_ctx_stream": (_vp, [_vp]),
+    "pwk_profile_enable": (C.c_int, [_vp, C.c_int]),
+    "pwk_profile_reset": (C.c_int, [_vp]),
+    "pwk_profile_read": (C.c_int, [_vp, C.c_int, C.c_char_p, _dp, C.POINTER(C.c_int64)]),
     "pwk_prepare": (C.c_int, [_vp, _ip, _dp]),
     "pwk_hartree": (C.c_int, [_vp, _dp, C.c_double, _dp, C.c_int]),
     "pwk_veff": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_int]),
@@ -157,6 +160,29 @@ class GpuAccelerator:
 
     def sync(self):
         _check(lib().pwk_ctx_sync(self.h))
+
+    # ---- kernel timing -----------------------------------------------------------
+    def profile(self, on=True):
+        _check(lib().pwk_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        _check(lib().pwk_profile_reset(self.h))
+
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} since the last reset."""
+        n = lib().pwk_profile_read(self.h, 0, None, None, None)
+        if n < 0:
+            _check(-n)
+        names = C.create_string_buffer(64 * max(n, 1))
+        ms = np.zeros(max(n, 1))
+        cnt = np.zeros(max(n, 1), dtype=np.int64)
+        lib().pwk_profile_read(self.h, n, names, _d(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64)))
+        raw = names.raw
+        out = {}
+        for i in range(n):
+            nm = raw[64 * i:64 * i + 64].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[i]), int(cnt[i]))
+        return out
 
     # ---- src/gpu/mod.rs:197-364 -------------------------------------------------
     def prepare_buffers(self, dims, g_squared):
