@@ -193,6 +193,9 @@ void launch_dense_h(Context& c, int ik, int with_nl, void* h);
 // gemm.hip: complex C = alpha·op(A)·B + beta·C, column-major, op = N only
 void launch_zgemm(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
                   bool accumulate, int precision);
+// gemm_mfma.hip: the same product on the matrix cores (FP32, FP64 or 3-term split-bf16)
+void launch_zgemm_mfma(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C,
+                       int ldc, bool accumulate, int precision);
 // nonlocal.hip
 void build_nonlocal_device(Context& c, int ik, const double* kvec, const double* gvec, int natoms, const double* tau,
                            const int32_t* atom_species, int nspecies, const int32_t* nproj, const int32_t* proj_l,

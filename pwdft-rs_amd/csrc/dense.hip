@@ -44,7 +44,7 @@ void launch_dense_h(Context& c, int ik, int with_nl, void* h) {
                                         c.dims[0], c.dims[1], c.dims[2], (double2*)h);
   PWK_HIP(hipGetLastError());
   if (with_nl && kd.nch > 0)
-    launch_zgemm(c, n, n, kd.nch, kd.B64.p, n, kd.DBH64.p, kd.nch, h, n, true, PWK_FP64);
+    launch_zgemm_mfma(c, n, n, kd.nch, kd.B64.p, n, kd.DBH64.p, kd.nch, h, n, true, PWK_FP64);
 }
 
 }  // namespace pwk

@@ -456,10 +456,10 @@ void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int pr
     const void* bm = precision == PWK_FP64 ? kd.B64.p : kd.B32.p;
     {
       TimedScope ts(c, "hpsi_nl_project");
-      launch_zgemm(c, kd.nch, nb, c.npw, dbh, kd.nch, psi, c.npw, c.proj.p, kd.nch, false, precision);
+      launch_zgemm_mfma(c, kd.nch, nb, c.npw, dbh, kd.nch, psi, c.npw, c.proj.p, kd.nch, false, precision);
     }
     TimedScope ts(c, "hpsi_nl_expand");
-    launch_zgemm(c, c.npw, nb, kd.nch, bm, c.npw, c.proj.p, kd.nch, hpsi, c.npw, true, precision);
+    launch_zgemm_mfma(c, c.npw, nb, kd.nch, bm, c.npw, c.proj.p, kd.nch, hpsi, c.npw, true, precision);
   }
 }
 
