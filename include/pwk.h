@@ -173,6 +173,17 @@ int pwk_dev_hpsi(pwk_ctx* ctx, int ik, int nb, const void* psi, void* hpsi, int 
 int pwk_dense_h(pwk_ctx* ctx, int ik, int with_nl, double* h);
 int pwk_dev_dense_h(pwk_ctx* ctx, int ik, int with_nl, void* h /* double2 npw×npw */);
 
+/* Lowest nb eigenpairs of nk Hermitian n×n matrices, FP64, eigenvalues
+ * ascending: diagonalize_lowest (src/eigensolver/dense.rs:87-98) applied to
+ * each matrix of a batch (the reference calls it once per k-point from
+ * diagonalize_dispatch, src/scf/driver.rs:125-184). h holds the nk matrices
+ * back to back, column-major complex128, and is overwritten; w receives nk×nb
+ * eigenvalues, v nk column-major n×nb eigenvector blocks (unit norm, phase
+ * arbitrary as in the reference). Returns PWK_E_UNSUPPORTED above the size the
+ * one-workgroup-per-matrix solver holds in LDS (n about 2500). */
+int pwk_dense_eigh(pwk_ctx* ctx, int n, int nb, int nk, double* h, double* w, double* v);
+int pwk_dev_dense_eigh(pwk_ctx* ctx, int n, int nb, int nk, void* h, void* w, void* v);
+
 /* Density accumulation (src/scf/density.rs:59-97): for each band b with
  * fw[b] >= 1e-15, rho[r] += fw[b]·|IFFT_unnorm(scatter ψ_b)(r)|². rho is real
  * N (FP64 accumulator, host or device), psi as in pwk_hpsi. */
@@ -180,6 +191,109 @@ int pwk_density_accumulate(pwk_ctx* ctx, int nb, const double* psi, const double
                            int precision);
 int pwk_dev_density_accumulate(pwk_ctx* ctx, int nb, const void* psi, const double* fw_host, void* rho,
                                int precision);
+
+/* ---- Integer tables (host; bit-exact with the reference) -------------------------- */
+
+/* G-sphere basis |G|² ≤ ecut/(ħ²/2m), Miller indices in n1 → n2 → n3 loop
+ * order (BasisSet::new, src/basis.rs:36-63). lattice: rows a, b, c in Å.
+ * Writes min(npw, cap) triplets; *npw receives the full count. */
+int pwk_basis(const double lattice[9], double ecut_ev, int cap, int32_t* miller, int* npw);
+
+/* FFT grid of a basis: 2,3,5-smooth ≥ 2·ceil(√ratio)·n_max + 1 per axis
+ * (FftGrid::new, src/scf/grid.rs:44-86; fft_grid_size, src/fft.rs:257-279). */
+int pwk_fft_dims(const int32_t* miller, int npw, int ecutrho_ratio, int dims[3]);
+
+/* Space-group operations (find_symmetry_operations, src/symmetry/detect.rs:22-178):
+ * rot row-major 3×3 integer matrices on fractional coordinates, tau in [0,1).
+ * z: atomic number per atom. Writes min(nops, cap) operations. */
+int pwk_symmetry(const double lattice[9], int natoms, const double* frac, const int32_t* z, double symprec,
+                 int cap, int32_t* rot, double* tau, int* nops);
+
+/* Monkhorst-Pack mesh reduced to the irreducible wedge (monkhorst_pack,
+ * src/kpoints.rs:245-288; reduce_kpoints, src/symmetry/kpoints.rs:31-143).
+ * use_symmetry = 0 keeps the identity only (no time reversal); 1 detects the
+ * operations and adds time reversal. kcart: Cartesian 1/Å, weights sum to 1. */
+int pwk_kpoints(const double lattice[9], int natoms, const double* frac, const int32_t* z, const int kgrid[3],
+                const int kshift[3], int use_symmetry, double symprec, int cap, double* kcart, double* weight,
+                int* nk);
+
+/* ---- Self-consistent field solve on the device ------------------------------------ */
+
+/* One pseudopotential species on its radial mesh, in the units the reference's
+ * UPF reader produces (src/pseudopotential/upf/parse.rs:20-140): r, rab in Å,
+ * v_local in eV, beta as χ = r·β in Å^-1/2 (nproj × mesh, row-major), dij in
+ * eV (nproj × nproj, row-major), rho_atom = 4πr²ρ in e/Å (NULL if absent),
+ * rho_core in e/Å³ (NULL without a core correction). */
+typedef struct pwk_species {
+  int atomic_number;
+  double z_valence;
+  int mesh;
+  const double* r;
+  const double* rab;
+  const double* v_local;
+  int nproj;
+  const int32_t* proj_l;
+  const double* beta;
+  const double* dij;
+  const double* rho_atom;
+  const double* rho_core;
+} pwk_species;
+
+/* Crystal (src/crystal.rs:83-121): lattice rows a, b, c in Å, fractional
+ * positions, a species index per atom into species_data (at most 8 species). */
+typedef struct pwk_crystal {
+  double lattice[9];
+  int natoms;
+  const double* frac;
+  const int32_t* species;
+  int nspecies;
+  const pwk_species* species_data;
+} pwk_crystal;
+
+/* Parameters of ScfParams (src/scf/mod.rs:72-402) used by the nspin = 1 driver. */
+typedef struct pwk_scf_params {
+  double ecut_ev;
+  int ecutrho_ratio;      /* 4 in the reference */
+  int fft_dims[3];        /* all > 0: explicit grid; else derived */
+  int kgrid[3];
+  int kshift[3];
+  int use_symmetry;       /* 0: identity only; 1: detected operations + time reversal */
+  double symprec;         /* 1e-5 */
+  int n_bands;
+  int max_iter;
+  double conv_threshold;  /* RMS Δρ in e/Å³ (density_diff, src/scf/energy.rs:227-235) */
+  double energy_threshold;/* |ΔE_total| in eV */
+  double mixing_beta;
+  int mixing_ndim;
+  double smearing_sigma;  /* Fermi-Dirac width, eV */
+  double sad_sigma;       /* width of the Gaussian start density without PP_RHOATOM, Å */
+  int precision;          /* PWK_FP64, or PWK_FP32 for the density FFTs */
+  int verbose;
+} pwk_scf_params;
+
+/* ScfResult + EnergyComponents (src/scf/energy.rs:597-670), eV. */
+typedef struct pwk_scf_result {
+  int converged, n_iter, nk, n_bands, npw, n_ops;
+  int dims[3];
+  double e_total, e_harris, fermi, ts, final_delta, integrated_charge;
+  double e_band, e_kinetic, e_local, e_nonlocal, e_hartree, e_xc, e_vxc, e_ewald, e_smearing;
+  double seconds;
+} pwk_scf_result;
+
+/* run_scf for nspin = 1 (src/scf/driver.rs:353-735) with the dense
+ * eigensolver: setup, V_eff, dense H per k, batched eigensolve, Fermi-Dirac
+ * occupations, density, symmetrisation, energies and Anderson mixing, all on
+ * the context's device. eigenvalues (nk × n_bands) and rho (the converged
+ * density on the FFT grid) may be NULL; their sizes come from
+ * pwk_scf_tables. Returns PWK_OK also when the solve did not converge
+ * (result->converged = 0). */
+int pwk_scf_run(pwk_ctx* ctx, const pwk_crystal* crystal, const pwk_scf_params* params, pwk_scf_result* result,
+                double* eigenvalues, double* rho);
+
+/* Sizes pwk_scf_run will use: plane waves, FFT grid, irreducible k-points and
+ * symmetry operations. Host only. */
+int pwk_scf_tables(const pwk_crystal* crystal, const pwk_scf_params* params, int* npw, int dims[3], int* nk,
+                   int* nops);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@
 
 #include "common.hpp"
 #include "fft_sizes.hpp"
+#include "tables.hpp"
 
 namespace pwk {
 namespace {
@@ -445,6 +446,36 @@ int pwk_dev_dense_h(pwk_ctx* ctx, int ik, int with_nl, void* h) {
   });
 }
 
+int pwk_dense_eigh(pwk_ctx* ctx, int n, int nb, int nk, double* h, double* w, double* v) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_REQUIRE(h && w && v, PWK_E_INVALID, "null eigensolver buffer");
+    PWK_REQUIRE(n >= 1 && nb >= 1 && nb <= n && nk >= 1, PWK_E_INVALID, "bad eigensolver shape");
+    PWK_REQUIRE(pwk::heev_supported(n, nb), PWK_E_UNSUPPORTED, "matrix too large for the batched eigensolver");
+    DevBuf dh, dw, dv;
+    size_t nh = (size_t)2 * n * n * nk, nv = (size_t)2 * n * nb * nk, nw = (size_t)nb * nk;
+    dh.ensure(nh * 8);
+    dw.ensure(nw * 8);
+    dv.ensure(nv * 8);
+    PWK_HIP(hipMemcpyAsync(dh.p, h, nh * 8, hipMemcpyHostToDevice, c.stream));
+    c.eig_work.ensure(pwk::heev_work_doubles(n, nb, nk) * 8);
+    pwk::launch_heev_batched(c, n, nb, nk, dh.p, c.eig_work.p, dw.as<double>(), dv.p, nullptr);
+    PWK_HIP(hipMemcpyAsync(h, dh.p, nh * 8, hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipMemcpyAsync(w, dw.p, nw * 8, hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipMemcpyAsync(v, dv.p, nv * 8, hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int pwk_dev_dense_eigh(pwk_ctx* ctx, int n, int nb, int nk, void* h, void* w, void* v) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_REQUIRE(h && w && v, PWK_E_INVALID, "null eigensolver buffer");
+    c.eig_work.ensure(pwk::heev_work_doubles(n, nb, nk) * 8);
+    pwk::launch_heev_batched(c, n, nb, nk, h, c.eig_work.p, (double*)w, v, nullptr);
+  });
+}
+
 int pwk_density_accumulate(pwk_ctx* ctx, int nb, const double* psi, const double* fw, double* rho, int precision) {
   return guarded([&] {
     Context& c = ctx_of(ctx);
@@ -467,6 +498,109 @@ int pwk_dev_density_accumulate(pwk_ctx* ctx, int nb, const void* psi, const doub
     Context& c = ctx_of(ctx);
     check_precision(precision);
     pwk::launch_density(c, nb, psi, fw_host, rho, precision);
+  });
+}
+
+// ---- integer tables --------------------------------------------------------------
+namespace {
+pwk::tab::Lat lat_of(const double* l) {
+  pwk::tab::Lat a;
+  for (int i = 0; i < 3; ++i) a.v[i] = {l[3 * i], l[3 * i + 1], l[3 * i + 2]};
+  return a;
+}
+std::vector<pwk::tab::AtomSite> sites_of(int natoms, const double* frac, const int32_t* z) {
+  PWK_REQUIRE(natoms >= 0 && (natoms == 0 || (frac && z)), PWK_E_INVALID, "bad atom arrays");
+  std::vector<pwk::tab::AtomSite> s((size_t)natoms);
+  for (int a = 0; a < natoms; ++a) {
+    s[(size_t)a].z = z[a];
+    for (int d = 0; d < 3; ++d) s[(size_t)a].frac[d] = frac[3 * a + d];
+  }
+  return s;
+}
+}  // namespace
+
+int pwk_basis(const double lattice[9], double ecut_ev, int cap, int32_t* miller, int* npw) {
+  return guarded([&] {
+    PWK_REQUIRE(lattice && npw && ecut_ev > 0.0 && cap >= 0 && (cap == 0 || miller), PWK_E_INVALID,
+                "bad basis arguments");
+    std::vector<int32_t> m;
+    std::vector<pwk::tab::Vec3> g;
+    pwk::tab::sphere_basis(lat_of(lattice), ecut_ev, m, g);
+    *npw = (int)g.size();
+    const size_t n = std::min<size_t>((size_t)cap, g.size());
+    std::memcpy(miller, m.data(), n * 3 * sizeof(int32_t));
+  });
+}
+
+int pwk_fft_dims(const int32_t* miller, int npw, int ecutrho_ratio, int dims[3]) {
+  return guarded([&] {
+    PWK_REQUIRE(miller && npw > 0 && dims && ecutrho_ratio > 0, PWK_E_INVALID, "bad grid arguments");
+    std::vector<int32_t> m(miller, miller + (size_t)npw * 3);
+    pwk::tab::grid_dims(m, ecutrho_ratio, nullptr, dims);
+  });
+}
+
+int pwk_symmetry(const double lattice[9], int natoms, const double* frac, const int32_t* z, double symprec,
+                 int cap, int32_t* rot, double* tau, int* nops) {
+  return guarded([&] {
+    PWK_REQUIRE(lattice && nops && cap >= 0 && (cap == 0 || (rot && tau)), PWK_E_INVALID,
+                "bad symmetry arguments");
+    auto ops = pwk::tab::detect_symmetry(lat_of(lattice), sites_of(natoms, frac, z), symprec);
+    *nops = (int)ops.size();
+    for (size_t i = 0; i < std::min<size_t>((size_t)cap, ops.size()); ++i) {
+      std::memcpy(rot + 9 * i, ops[i].rot, 9 * sizeof(int32_t));
+      std::memcpy(tau + 3 * i, ops[i].tau, 3 * sizeof(double));
+    }
+  });
+}
+
+int pwk_kpoints(const double lattice[9], int natoms, const double* frac, const int32_t* z, const int kgrid[3],
+                const int kshift[3], int use_symmetry, double symprec, int cap, double* kcart, double* weight,
+                int* nk) {
+  return guarded([&] {
+    PWK_REQUIRE(lattice && kgrid && kshift && nk && cap >= 0 && (cap == 0 || (kcart && weight)), PWK_E_INVALID,
+                "bad k-point arguments");
+    for (int d = 0; d < 3; ++d) PWK_REQUIRE(kgrid[d] >= 1, PWK_E_INVALID, "k-grid must be positive");
+    pwk::tab::Lat lat = lat_of(lattice);
+    std::vector<pwk::tab::SymOp> ops;
+    if (use_symmetry) ops = pwk::tab::detect_symmetry(lat, sites_of(natoms, frac, z), symprec);
+    if (ops.empty()) {
+      pwk::tab::SymOp id{};
+      id.rot[0] = id.rot[4] = id.rot[8] = 1;
+      ops.push_back(id);
+    }
+    auto ks = pwk::tab::kpoints(lat, kgrid, kshift, ops, use_symmetry != 0);
+    *nk = (int)ks.size();
+    for (size_t i = 0; i < std::min<size_t>((size_t)cap, ks.size()); ++i) {
+      kcart[3 * i] = ks[i].cart.x;
+      kcart[3 * i + 1] = ks[i].cart.y;
+      kcart[3 * i + 2] = ks[i].cart.z;
+      weight[i] = ks[i].weight;
+    }
+  });
+}
+
+// ---- SCF -------------------------------------------------------------------------
+int pwk_scf_run(pwk_ctx* ctx, const pwk_crystal* crystal, const pwk_scf_params* params, pwk_scf_result* result,
+                double* eigenvalues, double* rho) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_REQUIRE(crystal && params && result, PWK_E_INVALID, "null SCF argument");
+    check_precision(params->precision);
+    pwk::scf_run(c, *crystal, *params, *result, eigenvalues, rho);
+  });
+}
+
+int pwk_scf_tables(const pwk_crystal* crystal, const pwk_scf_params* params, int* npw, int dims[3], int* nk,
+                   int* nops) {
+  return guarded([&] {
+    PWK_REQUIRE(crystal && params && npw && dims && nk && nops, PWK_E_INVALID, "null argument");
+    PWK_REQUIRE(crystal->natoms >= 1 && crystal->species && crystal->species_data && crystal->frac, PWK_E_INVALID,
+                "bad crystal description");
+    for (int a = 0; a < crystal->natoms; ++a)
+      PWK_REQUIRE(crystal->species[a] >= 0 && crystal->species[a] < crystal->nspecies, PWK_E_INVALID,
+                  "atom species out of range");
+    pwk::scf_tables(*crystal, *params, npw, dims, nk, nops);
   });
 }
 

@@ -154,6 +154,7 @@ struct Context {
   DevBuf work_a, work_b;  // stick buffers / FFT scratch
   DevBuf proj;            // nonlocal projections
   DevBuf gemm_ws;         // split-K partial sums
+  DevBuf eig_work;        // dense eigensolver scratch
   DevBuf host_in, host_out;  // staging for host-pointer entry points
 
   Profiler prof;
@@ -196,11 +197,20 @@ void launch_zgemm(Context& c, int M, int N, int K, const void* A, int lda, const
 // gemm_mfma.hip: the same product on the matrix cores (FP32, FP64 or 3-term split-bf16)
 void launch_zgemm_mfma(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C,
                        int ldc, bool accumulate, int precision);
+// eig.hip: lowest nb eigenpairs of nk Hermitian n×n matrices (H overwritten)
+size_t heev_work_doubles(int n, int nb, int nk);
+bool heev_supported(int n, int nb);
+void launch_heev_batched(Context& c, int n, int nb, int nk, void* H, void* work, double* w, void* V, int* status);
 // nonlocal.hip
 void build_nonlocal_device(Context& c, int ik, const double* kvec, const double* gvec, int natoms, const double* tau,
                            const int32_t* atom_species, int nspecies, const int32_t* nproj, const int32_t* proj_l,
                            const double* dij, const int32_t* mesh, const double* r, const double* rab,
                            const double* chi, double omega);
+
+// scf.hip
+void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
+             double* rho);
+void scf_tables(const pwk_crystal& cr, const pwk_scf_params& p, int* npw, int dims[3], int* nk, int* nops);
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 

@@ -1,0 +1,1074 @@
+// Self-consistent field solve on the device: run_scf_unpolarized
+// (src/scf/driver.rs:353-735) with the dense eigensolver path, Fermi-Dirac
+// smearing, G-space density symmetrisation and plain Anderson mixing.
+//
+// Setup (src/scf/context.rs:102-231) runs here too: the basis and the integer
+// tables on the host (tables.hip), the radial transforms V_loc(G), ρ_core and
+// the superposition-of-atoms start density as one device kernel over the FFT
+// grid, Ewald as device FP64 reductions, the KB projectors per k-point on the
+// device (nonlocal.hip).
+//
+// Per iteration every grid-sized array stays in HBM: V_eff assembly (Hartree,
+// LDA, r→G FFTs), dense H per k, the batched eigensolver over all k at once,
+// density accumulation, symmetrisation, energies (deterministic two-stage
+// reductions) and the Anderson update. The host sees only nk·nb eigenvalues
+// (Fermi level) and a handful of reduction results per iteration.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+#include "tables.hpp"
+
+namespace pwk {
+
+using namespace tab;
+
+namespace {
+
+constexpr int MAXSP = 8;
+constexpr int RED_NT = 256;
+constexpr int RED_BLOCKS = 128;
+constexpr int MAX_TERMS = 48;
+
+// ---------------------------------------------------------------------------------------
+// radial transforms on the grid: V_loc(G) (src/scf/potentials.rs:18-51 with
+// v_local_of_g, src/pseudopotential/upf/mod.rs:166-213), ρ_core(G)
+// (src/scf/potentials.rs:72-130) and the atomic start density
+// (src/scf/initial_density.rs:64-166). The Simpson coefficients and every
+// G-independent factor are folded into per-species weight tables on the host.
+// ---------------------------------------------------------------------------------------
+struct RadialArgs {
+  size_t n;
+  int dims[3];
+  double b[9];  // reciprocal vectors, rows
+  double omega, sad_sigma;
+  int nspecies, natoms;
+  int off[MAXSP], mesh[MAXSP];
+  double z[MAXSP], v0[MAXSP];
+  int has_core[MAXSP], has_rho[MAXSP];
+  const double* r;
+  const double* wv;  // Simpson weight · rab · r² · (v_loc + Z e² erf(r)/r)
+  const double* wc;  // Simpson weight · rab · r² · ρ_core
+  const double* wr;  // Simpson weight · rab · 4πr²ρ_atom
+  const double* tau;  // natoms × 3, Cartesian Å
+  const int* sp;
+  double2* vloc;
+  double2* core;
+  double2* sad;
+};
+
+__device__ inline double sinc_d(double x) { return x < 1e-10 ? 1.0 - x * x / 6.0 : sin(x) / x; }
+
+__global__ void __launch_bounds__(256) k_radial(RadialArgs a) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= a.n) return;
+  const int nx = a.dims[0], ny = a.dims[1], nz = a.dims[2];
+  const int i1 = (int)(idx / ((size_t)ny * nz)), i2 = (int)((idx / nz) % ny), i3 = (int)(idx % nz);
+  const double n1 = i1 > nx / 2 ? i1 - nx : i1, n2 = i2 > ny / 2 ? i2 - ny : i2, n3 = i3 > nz / 2 ? i3 - nz : i3;
+  const double gx = (n1 * a.b[0] + n2 * a.b[3]) + n3 * a.b[6];
+  const double gy = (n1 * a.b[1] + n2 * a.b[4]) + n3 * a.b[7];
+  const double gz = (n1 * a.b[2] + n2 * a.b[5]) + n3 * a.b[8];
+  const double gn = sqrt(gx * gx + gy * gy + gz * gz);
+  const double fourpi = 4.0 * 3.141592653589793;
+  double2 vl = make_double2(0.0, 0.0), co = vl, sa = vl;
+  for (int s = 0; s < a.nspecies; ++s) {
+    const double* r = a.r + a.off[s];
+    const double* wv = a.wv + a.off[s];
+    const double* wc = a.wc + a.off[s];
+    const double* wr = a.wr + a.off[s];
+    double iv = 0.0, ic = 0.0, ir = 0.0;
+    for (int i = 0; i < a.mesh[s]; ++i) {
+      const double j0 = sinc_d(gn * r[i]);
+      iv += wv[i] * j0;
+      ic += wc[i] * j0;
+      ir += wr[i] * j0;
+    }
+    double vg;
+    if (gn < 1e-12) {
+      vg = a.v0[s];
+    } else {
+      const double g2 = gn * gn;
+      vg = fourpi / a.omega * (iv / 3.0) - fourpi * a.z[s] * 14.399645351950548 * exp(-g2 / 4.0) / (a.omega * g2);
+    }
+    const double cg = a.has_core[s] ? fourpi * (ic / 3.0) / a.omega : 0.0;
+    const double sg = a.has_rho[s] ? (ir / 3.0) / a.omega
+                                   : a.z[s] * exp(-0.5 * gn * gn * a.sad_sigma * a.sad_sigma) / a.omega;
+    for (int at = 0; at < a.natoms; ++at) {
+      if (a.sp[at] != s) continue;
+      const double ph = -(gx * a.tau[3 * at] + gy * a.tau[3 * at + 1] + gz * a.tau[3 * at + 2]);
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      vl.x += cs * vg;
+      vl.y += sn * vg;
+      co.x += cs * cg;
+      co.y += sn * cg;
+      sa.x += cs * sg;
+      sa.y += sn * sg;
+    }
+  }
+  a.vloc[idx] = vl;
+  a.core[idx] = co;
+  a.sad[idx] = sa;
+}
+
+// ---------------------------------------------------------------------------------------
+// Ewald sums (src/ewald.rs:43-143): per-block partial sums, summed on the host in
+// block order.
+// ---------------------------------------------------------------------------------------
+struct EwaldArgs {
+  int na;
+  int nr[3], nl[3];
+  double eta, eta2;
+  double b[9], l[9];
+  const double* pos;  // na × 3
+  const double* q;
+};
+
+__device__ double block_reduce(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) sh[t >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (t == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+  return s;  // valid on thread 0
+}
+
+__global__ void __launch_bounds__(256) k_ewald_recip(EwaldArgs a, double* partial) {
+  __shared__ double sh[4];
+  const long w1 = 2 * a.nr[0] + 1, w2 = 2 * a.nr[1] + 1, w3 = 2 * a.nr[2] + 1;
+  const long total = w1 * w2 * w3;
+  double acc = 0.0;
+  for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long)gridDim.x * 256) {
+    const int i1 = (int)(idx / (w2 * w3)) - a.nr[0], i2 = (int)((idx / w3) % w2) - a.nr[1],
+              i3 = (int)(idx % w3) - a.nr[2];
+    if (i1 == 0 && i2 == 0 && i3 == 0) continue;
+    const double gx = (i1 * a.b[0] + i2 * a.b[3]) + i3 * a.b[6];
+    const double gy = (i1 * a.b[1] + i2 * a.b[4]) + i3 * a.b[7];
+    const double gz = (i1 * a.b[2] + i2 * a.b[5]) + i3 * a.b[8];
+    const double g2 = gx * gx + gy * gy + gz * gz;
+    if (g2 < 1e-12) continue;
+    double sr = 0.0, si = 0.0;
+    for (int i = 0; i < a.na; ++i) {
+      double sn, cs;
+      sincos(gx * a.pos[3 * i] + gy * a.pos[3 * i + 1] + gz * a.pos[3 * i + 2], &sn, &cs);
+      sr += a.q[i] * cs;
+      si += a.q[i] * sn;
+    }
+    acc += (sr * sr + si * si) * exp(-g2 / (4.0 * a.eta2)) / g2;
+  }
+  const double s = block_reduce(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) k_ewald_real(EwaldArgs a, double* partial) {
+  __shared__ double sh[4];
+  const long w1 = 2 * a.nl[0] + 1, w2 = 2 * a.nl[1] + 1, w3 = 2 * a.nl[2] + 1;
+  const long na2 = (long)a.na * a.na;
+  const long total = w1 * w2 * w3 * na2;
+  double acc = 0.0;
+  for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long)gridDim.x * 256) {
+    const long cell = idx / na2, pair = idx % na2;
+    const int i = (int)(pair / a.na), j = (int)(pair % a.na);
+    const int t1 = (int)(cell / (w2 * w3)) - a.nl[0], t2 = (int)((cell / w3) % w2) - a.nl[1],
+              t3 = (int)(cell % w3) - a.nl[2];
+    const double tx = (t1 * a.l[0] + t2 * a.l[3]) + t3 * a.l[6];
+    const double ty = (t1 * a.l[1] + t2 * a.l[4]) + t3 * a.l[7];
+    const double tz = (t1 * a.l[2] + t2 * a.l[5]) + t3 * a.l[8];
+    const double rx = a.pos[3 * i] - a.pos[3 * j] + tx, ry = a.pos[3 * i + 1] - a.pos[3 * j + 1] + ty,
+                 rz = a.pos[3 * i + 2] - a.pos[3 * j + 2] + tz;
+    const double rn = sqrt(rx * rx + ry * ry + rz * rz);
+    if (rn < 1e-10) continue;
+    acc += a.q[i] * a.q[j] * erfc(a.eta * rn) / rn;
+  }
+  const double s = block_reduce(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// grid helpers
+// ---------------------------------------------------------------------------------------
+__global__ void k_r2c_scaled(size_t n, const double* __restrict__ r, double s, double2* __restrict__ c) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = make_double2(r[i] * s, 0.0);
+}
+
+__global__ void k_c2r_scaled(size_t n, const double2* __restrict__ c, double s, double* __restrict__ r,
+                             int clamp_nonneg) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = c[i].x * s;
+  if (clamp_nonneg && v < 0.0) v = 0.0;
+  r[i] = v;
+}
+
+// src/scf/energy.rs:314-324: ρ_xc = max(ρ + ρ_core, 0) (no clamp without a core)
+__global__ void k_add_core(size_t n, const double* __restrict__ rho, const double* __restrict__ core,
+                           double* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = core ? fmax(rho[i] + core[i], 0.0) : rho[i];
+}
+
+__global__ void k_scale(size_t n, double* __restrict__ x, double s) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= s;
+}
+
+__global__ void k_sub(size_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ o) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = a[i] - b[i];
+}
+
+// ρ_sym(G) = (1/N_ops) Σ_S e^{-2πi n·τ_S} ρ(R_S^T n)  (src/symmetry/density/g_space.rs:147-252)
+__global__ void __launch_bounds__(256) k_sym_gather(size_t n, int nx, int ny, int nz, int nops,
+                                                    const int* __restrict__ rot, const double* __restrict__ tau,
+                                                    const double2* __restrict__ in, double2* __restrict__ out,
+                                                    double inv) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n) return;
+  const int i1 = (int)(idx / ((size_t)ny * nz)), i2 = (int)((idx / nz) % ny), i3 = (int)(idx % nz);
+  const int nd[3] = {i1 > nx / 2 ? i1 - nx : i1, i2 > ny / 2 ? i2 - ny : i2, i3 > nz / 2 ? i3 - nz : i3};
+  const int dd[3] = {nx, ny, nz};
+  double ar = 0.0, ai = 0.0;
+  for (int o = 0; o < nops; ++o) {
+    const int* R = rot + 9 * o;
+    int src[3];
+    for (int c = 0; c < 3; ++c) {
+      const int ns = R[c] * nd[0] + R[3 + c] * nd[1] + R[6 + c] * nd[2];
+      src[c] = ((ns % dd[c]) + dd[c]) % dd[c];
+    }
+    const double2 v = in[((size_t)src[0] * ny + src[1]) * nz + src[2]];
+    const double arg = 6.283185307179586 * ((double)nd[0] * tau[3 * o] + (double)nd[1] * tau[3 * o + 1] +
+                                            (double)nd[2] * tau[3 * o + 2]);
+    double sn, cs;
+    sincos(arg, &sn, &cs);
+    // (cos, -sin) · v
+    ar += cs * v.x + sn * v.y;
+    ai += cs * v.y - sn * v.x;
+  }
+  out[idx] = make_double2(ar * inv, ai * inv);
+}
+
+// ---------------------------------------------------------------------------------------
+// deterministic dot products: out[t] = Σ_i (a_t - as_t)_i · (b_t - bs_t)_i, a missing b
+// counts as 1, kind 1 is the Hartree sum Σ_{|G|²>1e-12} |ρ(G)|²/|G|² (a = ρ(G)
+// interleaved, b = |G|²).
+// ---------------------------------------------------------------------------------------
+struct DotTerm {
+  const double* a;
+  const double* as;
+  const double* b;
+  const double* bs;
+  int kind;
+};
+struct DotArgs {
+  int nterms;
+  DotTerm t[MAX_TERMS];
+};
+
+__global__ void __launch_bounds__(RED_NT) k_dots(size_t n, DotArgs args, double* __restrict__ partial) {
+  __shared__ double sh[RED_NT / 64];
+  const DotTerm T = args.t[blockIdx.y];
+  double acc = 0.0;
+  for (size_t i = (size_t)blockIdx.x * RED_NT + threadIdx.x; i < n; i += (size_t)gridDim.x * RED_NT) {
+    if (T.kind == 1) {
+      const double g2 = T.b[i];
+      if (g2 > 1e-12) {
+        const double re = T.a[2 * i], im = T.a[2 * i + 1];
+        acc += (re * re + im * im) / g2;
+      }
+    } else {
+      const double x = T.a[i] - (T.as ? T.as[i] : 0.0);
+      const double y = T.b ? T.b[i] - (T.bs ? T.bs[i] : 0.0) : 1.0;
+      acc += x * y;
+    }
+  }
+  const double s = block_reduce(acc, sh);
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void k_dots_finish(int nterms, int nblk, const double* __restrict__ partial, double* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t >= nterms) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(size_t)t * nblk + b];
+  out[t] = s;
+}
+
+// per-band expectation values: e[2b] = Σ_G |ψ|² t_G, e[2b+1] = Re Σ_G conj(ψ)·x
+__global__ void __launch_bounds__(256) k_band_expect(int npw, const double2* __restrict__ psi,
+                                                    const double* __restrict__ t, const double2* __restrict__ x,
+                                                    double* __restrict__ e) {
+  __shared__ double sh[4];
+  const int b = blockIdx.x;
+  const double2* p = psi + (size_t)b * npw;
+  const double2* q = x + (size_t)b * npw;
+  double kin = 0.0, nl = 0.0;
+  for (int i = threadIdx.x; i < npw; i += 256) {
+    const double2 v = p[i], w = q[i];
+    kin += (v.x * v.x + v.y * v.y) * t[i];
+    nl += v.x * w.x + v.y * w.y;
+  }
+  const double s1 = block_reduce(kin, sh);
+  __syncthreads();
+  const double s2 = block_reduce(nl, sh);
+  if (threadIdx.x == 0) {
+    e[2 * b] = s1;
+    e[2 * b + 1] = s2;
+  }
+}
+
+// Anderson update: out = c_last·(in_l + β r_l) + Σ_j α_j (in_j + β r_j)
+struct MixArgs {
+  int m;
+  double beta;
+  double coef[16];
+  const double* in[16];
+  const double* res[16];
+};
+
+__global__ void k_anderson_combine(size_t n, MixArgs a, double* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int last = a.m - 1;
+  double v = a.coef[last] * (a.in[last][i] + a.beta * a.res[last][i]);
+  for (int j = 0; j < last; ++j) v += a.coef[j] * (a.in[j][i] + a.beta * a.res[j][i]);
+  out[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------------------
+
+// Simpson coefficients of src/numerics.rs:29-58 (odd mesh: 1,4,2,...,4,1; even mesh:
+// the QE/DFTK end correction), scaled by 3 so that Σ c_i f_i rab_i / 3 is the rule.
+std::vector<double> simpson_coeffs(int n) {
+  std::vector<double> c((size_t)n, 0.0);
+  if (n < 3) {
+    for (auto& x : c) x = 3.0;
+    return c;
+  }
+  for (int i = 1; i < n - 1; ++i) c[(size_t)i] = (i % 2 == 1) ? 4.0 : 2.0;
+  c[0] = 1.0;
+  if (n % 2 == 1) {
+    c[(size_t)n - 1] = 1.0;
+  } else {
+    c[(size_t)n - 3] += -0.25;
+    c[(size_t)n - 2] += 1.0;
+    c[(size_t)n - 1] = 1.25;
+  }
+  return c;
+}
+
+// src/scf/smearing.rs:112-130
+double fd_occ(double e, double ef, double sigma) {
+  if (sigma < 1e-15) return e < ef ? 1.0 : (std::fabs(e - ef) < 1e-12 ? 0.5 : 0.0);
+  const double x = (e - ef) / sigma;
+  if (x > 40.0) return 0.0;
+  if (x < -40.0) return 1.0;
+  return 1.0 / (1.0 + std::exp(x));
+}
+
+// src/scf/smearing.rs:49-90: bisection on the electron count
+double fermi_level(const std::vector<double>& ev, int nk, int nb, const std::vector<double>& wk, double n_el,
+                   double sigma, double spin) {
+  double lo = INFINITY, hi = -INFINITY;
+  for (double e : ev) {
+    lo = std::min(lo, e);
+    hi = std::max(hi, e);
+  }
+  const double pad = 10.0 * std::max(sigma, 0.1);
+  lo -= pad;
+  hi += pad;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    double count = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      double sk = 0.0;
+      for (int b = 0; b < nb; ++b) sk += wk[(size_t)k] * (fd_occ(ev[(size_t)k * nb + b], mid, sigma) * spin);
+      count += sk;
+    }
+    if (count < n_el)
+      lo = mid;
+    else
+      hi = mid;
+    if (std::fabs(hi - lo) < 1e-14) break;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// src/scf/smearing.rs:202-223: -TS of Fermi-Dirac smearing, returned as TS
+double smearing_ts(const std::vector<double>& ev, int nk, int nb, const std::vector<double>& wk, double ef,
+                   double sigma, double spin) {
+  if (sigma < 1e-15) return 0.0;
+  double s = 0.0;
+  for (int k = 0; k < nk; ++k)
+    for (int b = 0; b < nb; ++b) {
+      const double x = (ev[(size_t)k * nb + b] - ef) / sigma;
+      double w = 0.0;
+      if (std::fabs(x) <= 30.0) {
+        double f = 1.0 / (1.0 + std::exp(x));
+        f = std::min(std::max(f, 1e-30), 1.0 - 1e-30);
+        w = -(f * std::log(f) + (1.0 - f) * std::log(1.0 - f));
+      }
+      s += wk[(size_t)k] * w;
+    }
+  return s * sigma * spin;
+}
+
+// src/scf/mixing/linalg.rs:24-52: LU with partial pivoting; a singular pivot
+// returns the uniform vector 1/(n+1).
+std::vector<double> small_solve(std::vector<double> A, std::vector<double> b, int n) {
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)p * n + c])) p = r;
+    if (p != c) {
+      for (int j = 0; j < n; ++j) std::swap(A[(size_t)c * n + j], A[(size_t)p * n + j]);
+      std::swap(b[(size_t)c], b[(size_t)p]);
+    }
+    if (std::fabs(A[(size_t)c * n + c]) < 1e-15) return std::vector<double>((size_t)n, 1.0 / (double)(n + 1));
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[(size_t)r * n + c] / A[(size_t)c * n + c];
+      for (int j = c; j < n; ++j) A[(size_t)r * n + j] -= f * A[(size_t)c * n + j];
+      b[(size_t)r] -= f * b[(size_t)c];
+    }
+  }
+  std::vector<double> x((size_t)n);
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[(size_t)i];
+    for (int j = i + 1; j < n; ++j) s -= A[(size_t)i * n + j] * x[(size_t)j];
+    x[(size_t)i] = s / A[(size_t)i * n + i];
+  }
+  return x;
+}
+
+// ---------------------------------------------------------------------------------------
+// the solve
+// ---------------------------------------------------------------------------------------
+struct Solve {
+  Context& c;
+  const pwk_crystal& cr;
+  const pwk_scf_params& p;
+  pwk_scf_result& res;
+
+  Lat lat, rec;
+  double omega = 0, n_el = 0;
+  std::vector<int32_t> miller;
+  std::vector<Vec3> gvec;
+  int npw = 0, nb = 0, nk = 0;
+  size_t N = 0;
+  std::vector<KPt> kpts;
+  std::vector<double> wk;
+  std::vector<SymOp> ops;
+  bool have_core = false;
+
+  // device arrays
+  DevBuf vloc_g, vloc_r, core_r, rho_in, rho_g_in, rho_new, rho_g_new, rho_xc, exc, vxc, vxc_g, veff_g, cwork;
+  DevBuf Hb, W, V, V32, sym_rot, sym_tau, partial, dots_out, hist, nl_p, nl_x, expect;
+  double* host_small = nullptr;  // pinned
+
+  Solve(Context& cc, const pwk_crystal& a, const pwk_scf_params& b, pwk_scf_result& r) : c(cc), cr(a), p(b), res(r) {}
+  ~Solve() {
+    if (host_small) (void)hipHostFree(host_small);
+  }
+
+  unsigned gb(size_t n) const { return blocks_for(n, 256); }
+
+  // Σ terms on the device, results copied to host_small[0..nterms)
+  void dots(const std::vector<DotTerm>& terms, size_t n) {
+    PWK_REQUIRE(terms.size() <= (size_t)MAX_TERMS, PWK_E_INVALID, "too many reduction terms");
+    DotArgs a{};
+    a.nterms = (int)terms.size();
+    for (size_t i = 0; i < terms.size(); ++i) a.t[i] = terms[i];
+    const int nblk = (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (n + RED_NT - 1) / RED_NT));
+    k_dots<<<dim3((unsigned)nblk, (unsigned)terms.size()), RED_NT, 0, c.stream>>>(n, a, partial.as<double>());
+    PWK_HIP(hipGetLastError());
+    k_dots_finish<<<1, 64, 0, c.stream>>>((int)terms.size(), nblk, partial.as<double>(), dots_out.as<double>());
+    PWK_HIP(hipGetLastError());
+    PWK_HIP(hipMemcpyAsync(host_small, dots_out.p, terms.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipStreamSynchronize(c.stream));
+  }
+
+  // ρ(r) → ρ(G) = FFT_fwd(ρ)/N  (src/scf/energy.rs:250-259)
+  void r_to_g(const double* r, double2* g) {
+    k_r2c_scaled<<<gb(N), 256, 0, c.stream>>>(N, r, 1.0 / (double)N, g);
+    PWK_HIP(hipGetLastError());
+    launch_fft3d(c, g, 1, -1, PWK_FP64);
+  }
+
+  void setup();
+  double ewald();
+  void veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out, double* vxc_out);
+  void run(double* eig_out, double* rho_out);
+};
+
+void Solve::setup() {
+  PWK_REQUIRE(cr.natoms >= 1 && cr.frac && cr.species && cr.nspecies >= 1 && cr.nspecies <= MAXSP &&
+                  cr.species_data,
+              PWK_E_INVALID, "bad crystal description (1..8 species)");
+  for (int i = 0; i < 3; ++i) lat.v[i] = {cr.lattice[3 * i], cr.lattice[3 * i + 1], cr.lattice[3 * i + 2]};
+  rec = reciprocal(lat);
+  omega = volume(lat);
+  PWK_REQUIRE(omega > 0.0, PWK_E_INVALID, "degenerate lattice");
+  n_el = 0.0;
+  for (int a = 0; a < cr.natoms; ++a) {
+    PWK_REQUIRE(cr.species[a] >= 0 && cr.species[a] < cr.nspecies, PWK_E_INVALID, "atom species out of range");
+    n_el += cr.species_data[cr.species[a]].z_valence;
+  }
+  // basis, grid (src/basis.rs:36-63, src/scf/grid.rs:44-86)
+  PWK_REQUIRE(p.ecut_ev > 0.0, PWK_E_INVALID, "ecut must be positive");
+  sphere_basis(lat, p.ecut_ev, miller, gvec);
+  npw = (int)gvec.size();
+  int dims[3];
+  grid_dims(miller, p.ecutrho_ratio > 0 ? p.ecutrho_ratio : 4, p.fft_dims, dims);
+  for (int d = 0; d < 3; ++d) PWK_REQUIRE(dims[d] > 0 && dims[d] <= 1024, PWK_E_INVALID, "FFT grid exceeds 1024");
+  nb = std::min(p.n_bands, npw);
+  PWK_REQUIRE(nb >= 1, PWK_E_INVALID, "n_bands must be positive");
+  PWK_REQUIRE(heev_supported(npw, nb), PWK_E_UNSUPPORTED, "basis too large for the dense eigensolver");
+  // |G|² per grid point (src/scf/context.rs:146-156)
+  c.dims[0] = dims[0];
+  c.dims[1] = dims[1];
+  c.dims[2] = dims[2];
+  N = (size_t)dims[0] * dims[1] * dims[2];
+  c.N = N;
+  std::vector<double> g2(N);
+  for (size_t idx = 0; idx < N; ++idx) {
+    const int i1 = (int)(idx / ((size_t)dims[1] * dims[2])), i2 = (int)((idx / dims[2]) % dims[1]),
+              i3 = (int)(idx % dims[2]);
+    const int n1 = i1 > dims[0] / 2 ? i1 - dims[0] : i1, n2 = i2 > dims[1] / 2 ? i2 - dims[1] : i2,
+              n3 = i3 > dims[2] / 2 ? i3 - dims[2] : i3;
+    g2[idx] = vnorm2(miller_vec(rec, n1, n2, n3));
+  }
+  c.g2_64.ensure(N * 8);
+  c.g2_32.ensure(N * 4);
+  PWK_HIP(hipMemcpyAsync(c.g2_64.p, g2.data(), N * 8, hipMemcpyHostToDevice, c.stream));
+  launch_convert(c, c.g2_64.p, c.g2_32.p, N, true);
+  c.npw = 0;
+  c.have_v = false;
+  setup_basis(c, npw, miller.data());
+  // k-points and symmetry (src/kpoints.rs:245-288, src/symmetry/*)
+  std::vector<AtomSite> sites((size_t)cr.natoms);
+  for (int a = 0; a < cr.natoms; ++a) {
+    sites[(size_t)a].z = cr.species_data[cr.species[a]].atomic_number;
+    for (int d = 0; d < 3; ++d) sites[(size_t)a].frac[d] = cr.frac[3 * a + d];
+  }
+  const double tol = p.symprec > 0.0 ? p.symprec : 1e-5;
+  bool trev = false;
+  if (p.use_symmetry) {
+    ops = detect_symmetry(lat, sites, tol);
+    trev = true;
+  }
+  if (ops.empty()) {
+    SymOp id{};
+    id.rot[0] = id.rot[4] = id.rot[8] = 1;
+    ops.push_back(id);
+  }
+  int grid[3], shift[3];
+  for (int d = 0; d < 3; ++d) {
+    grid[d] = p.kgrid[d] > 0 ? p.kgrid[d] : 1;
+    shift[d] = p.kshift[d];
+  }
+  kpts = kpoints(lat, grid, shift, ops, trev);
+  nk = (int)kpts.size();
+  wk.resize((size_t)nk);
+  for (int k = 0; k < nk; ++k) wk[(size_t)k] = kpts[(size_t)k].weight;
+  c.kd.clear();
+  c.kd.resize((size_t)nk);
+
+  // radial tables → V_loc(G), ρ_core(r), start density
+  RadialArgs ra{};
+  ra.n = N;
+  for (int d = 0; d < 3; ++d) ra.dims[d] = dims[d];
+  for (int i = 0; i < 3; ++i) {
+    ra.b[3 * i] = rec.v[i].x;
+    ra.b[3 * i + 1] = rec.v[i].y;
+    ra.b[3 * i + 2] = rec.v[i].z;
+  }
+  ra.omega = omega;
+  ra.sad_sigma = p.sad_sigma > 0.0 ? p.sad_sigma : 1.0;
+  ra.nspecies = cr.nspecies;
+  ra.natoms = cr.natoms;
+  std::vector<double> hr, hwv, hwc, hwr;
+  int off = 0;
+  const double fourpi = 4.0 * kPiC;
+  for (int s = 0; s < cr.nspecies; ++s) {
+    const pwk_species& sp = cr.species_data[s];
+    PWK_REQUIRE(sp.mesh >= 1 && sp.r && sp.rab && sp.v_local, PWK_E_INVALID, "species without radial mesh");
+    const int n = sp.mesh;
+    std::vector<double> cf = simpson_coeffs(n);
+    ra.off[s] = off;
+    ra.mesh[s] = n;
+    ra.z[s] = sp.z_valence;
+    ra.has_core[s] = sp.rho_core != nullptr;
+    bool rho_nonzero = false;
+    if (sp.rho_atom)
+      for (int i = 0; i < n; ++i) rho_nonzero |= std::fabs(sp.rho_atom[i]) > 1e-20;
+    ra.has_rho[s] = rho_nonzero;
+    have_core |= sp.rho_core != nullptr;
+    // G = 0 term: bare Coulomb tail removed (upf/mod.rs:171-180)
+    double s0 = 0.0;
+    const double tsp = 2.0 / std::sqrt(kPiC);
+    for (int i = 0; i < n; ++i) {
+      const double r = sp.r[i];
+      const double vs0 = sp.v_local[i] + sp.z_valence * kE2 / std::max(r, 1e-20);
+      s0 += cf[(size_t)i] * sp.rab[i] * (r * r * vs0);
+      const double eor = r < 1e-20 ? tsp : std::erf(r) / r;
+      const double vs = sp.v_local[i] + sp.z_valence * kE2 * eor;
+      hr.push_back(r);
+      hwv.push_back(cf[(size_t)i] * sp.rab[i] * (r * r * vs));
+      hwc.push_back(sp.rho_core ? cf[(size_t)i] * sp.rab[i] * (sp.rho_core[i] * r * r) : 0.0);
+      hwr.push_back(rho_nonzero ? cf[(size_t)i] * sp.rab[i] * sp.rho_atom[i] : 0.0);
+    }
+    ra.v0[s] = fourpi / omega * (s0 / 3.0);
+    off += n;
+  }
+  std::vector<double> tau((size_t)cr.natoms * 3);
+  std::vector<int> spi((size_t)cr.natoms);
+  for (int a = 0; a < cr.natoms; ++a) {
+    const Vec3 t = frac_to_cart(lat, &cr.frac[3 * a]);
+    tau[(size_t)3 * a] = t.x;
+    tau[(size_t)3 * a + 1] = t.y;
+    tau[(size_t)3 * a + 2] = t.z;
+    spi[(size_t)a] = cr.species[a];
+  }
+  DevBuf dr, dwv, dwc, dwr, dtau, dsp, core_g, sad_g;
+  dr.ensure(hr.size() * 8);
+  dwv.ensure(hr.size() * 8);
+  dwc.ensure(hr.size() * 8);
+  dwr.ensure(hr.size() * 8);
+  dtau.ensure(tau.size() * 8);
+  dsp.ensure(spi.size() * 4);
+  PWK_HIP(hipMemcpyAsync(dr.p, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(dwv.p, hwv.data(), hr.size() * 8, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(dwc.p, hwc.data(), hr.size() * 8, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(dwr.p, hwr.data(), hr.size() * 8, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(dtau.p, tau.data(), tau.size() * 8, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(dsp.p, spi.data(), spi.size() * 4, hipMemcpyHostToDevice, c.stream));
+  vloc_g.ensure(N * 16);
+  core_g.ensure(N * 16);
+  sad_g.ensure(N * 16);
+  ra.r = dr.as<double>();
+  ra.wv = dwv.as<double>();
+  ra.wc = dwc.as<double>();
+  ra.wr = dwr.as<double>();
+  ra.tau = dtau.as<double>();
+  ra.sp = dsp.as<int>();
+  ra.vloc = vloc_g.as<double2>();
+  ra.core = core_g.as<double2>();
+  ra.sad = sad_g.as<double2>();
+  {
+    TimedScope ts(c, "scf_radial");
+    k_radial<<<gb(N), 256, 0, c.stream>>>(ra);
+    PWK_HIP(hipGetLastError());
+  }
+  // working arrays
+  for (DevBuf* b : {&vloc_r, &core_r, &rho_in, &rho_new, &rho_xc, &exc, &vxc}) b->ensure(N * 8);
+  for (DevBuf* b : {&rho_g_in, &rho_g_new, &vxc_g, &veff_g, &cwork}) b->ensure(N * 16);
+  partial.ensure((size_t)MAX_TERMS * RED_BLOCKS * 8);
+  dots_out.ensure(MAX_TERMS * 8);
+  PWK_HIP(hipHostMalloc((void**)&host_small, 64 * sizeof(double), hipHostMallocDefault));
+  // V_loc(r) for the local energy, ρ_core(r), the start density (IFFT, real part)
+  PWK_HIP(hipMemcpyAsync(cwork.p, vloc_g.p, N * 16, hipMemcpyDeviceToDevice, c.stream));
+  launch_fft3d(c, cwork.p, 1, +1, PWK_FP64);
+  k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, cwork.as<double2>(), 1.0, vloc_r.as<double>(), 0);
+  if (have_core) {
+    launch_fft3d(c, core_g.p, 1, +1, PWK_FP64);
+    k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, core_g.as<double2>(), 1.0, core_r.as<double>(), 0);
+  }
+  launch_fft3d(c, sad_g.p, 1, +1, PWK_FP64);
+  k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, sad_g.as<double2>(), 1.0, rho_in.as<double>(), 1);
+  PWK_HIP(hipGetLastError());
+  // renormalise to N_el (initial_density.rs:150-166)
+  dots({DotTerm{rho_in.as<double>(), nullptr, nullptr, nullptr, 0}}, N);
+  const double post = host_small[0] * (omega / (double)N);
+  const double sc = post > 1e-15 ? n_el / post : 1.0;
+  k_scale<<<gb(N), 256, 0, c.stream>>>(N, rho_in.as<double>(), sc);
+  PWK_HIP(hipGetLastError());
+  r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
+
+  // per-k kinetic diagonal and KB projectors
+  std::vector<int32_t> nproj((size_t)cr.nspecies), mesh((size_t)cr.nspecies), pl;
+  std::vector<double> dij, rr, rab, chi;
+  for (int s = 0; s < cr.nspecies; ++s) {
+    const pwk_species& sp = cr.species_data[s];
+    nproj[(size_t)s] = sp.nproj;
+    mesh[(size_t)s] = sp.mesh;
+    for (int i = 0; i < sp.nproj; ++i) pl.push_back(sp.proj_l[i]);
+    for (int i = 0; i < sp.nproj * sp.nproj; ++i) dij.push_back(sp.dij[i]);
+    rr.insert(rr.end(), sp.r, sp.r + sp.mesh);
+    rab.insert(rab.end(), sp.rab, sp.rab + sp.mesh);
+    for (int i = 0; i < sp.nproj * sp.mesh; ++i) chi.push_back(sp.beta[i]);
+  }
+  std::vector<double> gflat((size_t)npw * 3), tk((size_t)npw);
+  for (int i = 0; i < npw; ++i) {
+    gflat[(size_t)3 * i] = gvec[(size_t)i].x;
+    gflat[(size_t)3 * i + 1] = gvec[(size_t)i].y;
+    gflat[(size_t)3 * i + 2] = gvec[(size_t)i].z;
+  }
+  for (int k = 0; k < nk; ++k) {
+    const Vec3 kc = kpts[(size_t)k].cart;
+    for (int i = 0; i < npw; ++i) tk[(size_t)i] = kHbar2Over2m * vnorm2(vadd(kc, gvec[(size_t)i]));
+    KData& kd = c.k(k);
+    kd.kin64.ensure((size_t)npw * 8);
+    kd.kin32.ensure((size_t)npw * 4);
+    PWK_HIP(hipMemcpyAsync(kd.kin64.p, tk.data(), (size_t)npw * 8, hipMemcpyHostToDevice, c.stream));
+    launch_convert(c, kd.kin64.p, kd.kin32.p, (size_t)npw, true);
+    kd.have_kin = true;
+    PWK_HIP(hipStreamSynchronize(c.stream));  // tk is reused
+    const double kv[3] = {kc.x, kc.y, kc.z};
+    build_nonlocal_device(c, k, kv, gflat.data(), cr.natoms, tau.data(), spi.data(), cr.nspecies, nproj.data(),
+                          pl.data(), dij.data(), mesh.data(), rr.data(), rab.data(), chi.data(), omega);
+  }
+  // symmetry tables for the density
+  std::vector<int> rot;
+  std::vector<double> tr;
+  for (const SymOp& o : ops) {
+    rot.insert(rot.end(), o.rot, o.rot + 9);
+    tr.insert(tr.end(), o.tau, o.tau + 3);
+  }
+  sym_rot.ensure(rot.size() * 4);
+  sym_tau.ensure(tr.size() * 8);
+  PWK_HIP(hipMemcpyAsync(sym_rot.p, rot.data(), rot.size() * 4, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(sym_tau.p, tr.data(), tr.size() * 8, hipMemcpyHostToDevice, c.stream));
+  // eigen-system buffers
+  Hb.ensure((size_t)nk * npw * npw * 16);
+  W.ensure((size_t)nk * nb * 8);
+  V.ensure((size_t)nk * npw * nb * 16);
+  if (p.precision != PWK_FP64) V32.ensure((size_t)nk * npw * nb * 8);
+  c.eig_work.ensure(heev_work_doubles(npw, nb, nk) * 8);
+  PWK_HIP(hipStreamSynchronize(c.stream));
+}
+
+// src/ewald.rs:43-143
+double Solve::ewald() {
+  EwaldArgs a{};
+  a.na = cr.natoms;
+  std::vector<double> pos((size_t)cr.natoms * 3), q((size_t)cr.natoms);
+  for (int i = 0; i < cr.natoms; ++i) {
+    const Vec3 t = frac_to_cart(lat, &cr.frac[3 * i]);
+    pos[(size_t)3 * i] = t.x;
+    pos[(size_t)3 * i + 1] = t.y;
+    pos[(size_t)3 * i + 2] = t.z;
+    q[(size_t)i] = cr.species_data[cr.species[i]].z_valence;
+  }
+  a.eta = std::cbrt((double)cr.natoms * kPiC / omega);
+  a.eta2 = a.eta * a.eta;
+  for (int i = 0; i < 3; ++i) {
+    a.nr[i] = (int)std::ceil(10.0 * a.eta / vnorm(rec.v[i]));
+    a.nl[i] = (int)std::ceil(10.0 / a.eta / vnorm(lat.v[i]));
+    a.b[3 * i] = rec.v[i].x;
+    a.b[3 * i + 1] = rec.v[i].y;
+    a.b[3 * i + 2] = rec.v[i].z;
+    a.l[3 * i] = lat.v[i].x;
+    a.l[3 * i + 1] = lat.v[i].y;
+    a.l[3 * i + 2] = lat.v[i].z;
+  }
+  DevBuf dpos, dq, part;
+  dpos.ensure(pos.size() * 8);
+  dq.ensure(q.size() * 8);
+  const int nblk = 64;
+  part.ensure(2 * nblk * 8);
+  PWK_HIP(hipMemcpyAsync(dpos.p, pos.data(), pos.size() * 8, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(dq.p, q.data(), q.size() * 8, hipMemcpyHostToDevice, c.stream));
+  a.pos = dpos.as<double>();
+  a.q = dq.as<double>();
+  k_ewald_recip<<<nblk, 256, 0, c.stream>>>(a, part.as<double>());
+  k_ewald_real<<<nblk, 256, 0, c.stream>>>(a, part.as<double>() + nblk);
+  PWK_HIP(hipGetLastError());
+  std::vector<double> hp(2 * nblk);
+  PWK_HIP(hipMemcpyAsync(hp.data(), part.p, 2 * nblk * 8, hipMemcpyDeviceToHost, c.stream));
+  PWK_HIP(hipStreamSynchronize(c.stream));
+  double er = 0.0, eq = 0.0;
+  for (int i = 0; i < nblk; ++i) er += hp[(size_t)i];
+  for (int i = 0; i < nblk; ++i) eq += hp[(size_t)nblk + i];
+  er *= 2.0 * kPiC * kE2 / omega;
+  eq *= 0.5 * kE2;
+  double z2 = 0.0, zs = 0.0;
+  for (double z : q) {
+    z2 += z * z;
+    zs += z;
+  }
+  const double self = -a.eta / std::sqrt(kPiC) * z2 * kE2;
+  const double bg = -kPiC * zs * zs * kE2 / (2.0 * omega * a.eta2);
+  return eq + er + self + bg;
+}
+
+// V_eff(G) = V_loc + V_H[ρ] + FFT(v_xc[ρ + ρ_core])/N  (src/scf/driver.rs:359-431)
+void Solve::veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out,
+                      double* vxc_out) {
+  k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_r, have_core ? core_r.as<double>() : nullptr, rho_xc_out);
+  PWK_HIP(hipGetLastError());
+  launch_lda(c, rho_xc_out, exc_out, vxc_out, PWK_FP64);
+  r_to_g(vxc_out, vxc_g.as<double2>());
+  launch_hartree_veff(c, rho_g, 4.0 * kPiC * kE2, vloc_g.p, vxc_g.p, veff_g.p, PWK_FP64, true, true, true);
+}
+
+void Solve::run(double* eig_out, double* rho_out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  setup();
+  const double e_ewald = ewald();
+  const double spin = 2.0;
+  const double dvol = omega / (double)N;
+  const int ndim = std::max(1, std::min(p.mixing_ndim, 16));
+  hist.ensure((size_t)2 * ndim * N * 8);
+  std::vector<int> hslot;  // logical history order → slot
+  int next_slot = 0;
+
+  std::vector<double> ev((size_t)nk * nb), evals_final;
+  bool have_prev = false;
+  double e_prev = 0.0;
+  res = pwk_scf_result{};
+  res.nk = nk;
+  res.n_bands = nb;
+  res.npw = npw;
+  for (int d = 0; d < 3; ++d) res.dims[d] = c.dims[d];
+  res.e_ewald = e_ewald;
+  res.n_ops = (int)ops.size();
+
+  for (int iter = 0; iter < p.max_iter; ++iter) {
+    // ---- V_eff from the input density --------------------------------------------------
+    veff_from(rho_in.as<double>(), rho_g_in.as<double2>(), rho_xc.as<double>(), exc.as<double>(), vxc.as<double>());
+    // keep the input-side xc for the Harris energy: ρ_xc,in · ε_xc,in and ρ_in · v_xc,in
+    std::vector<DotTerm> harris = {DotTerm{rho_g_in.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+                                   DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+                                   DotTerm{rho_in.as<double>(), nullptr, vxc.as<double>(), nullptr, 0}};
+    dots(harris, N);
+    const double eh_in = host_small[0] * 4.0 * kPiC * kE2 * 0.5 * omega;
+    const double exc_in = host_small[1] * dvol - host_small[2] * dvol;
+    set_veff_device(c, veff_g.p);
+    // ---- dense H per k, one batched eigensolve ------------------------------------------
+    {
+      TimedScope ts(c, "scf_dense_h");
+      for (int k = 0; k < nk; ++k) launch_dense_h(c, k, 1, Hb.as<double2>() + (size_t)k * npw * npw);
+    }
+    launch_heev_batched(c, npw, nb, nk, Hb.p, c.eig_work.p, W.as<double>(), V.p, nullptr);
+    PWK_HIP(hipMemcpyAsync(ev.data(), W.p, ev.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipStreamSynchronize(c.stream));
+    // ---- occupations (src/scf/smearing.rs) -----------------------------------------------
+    const double ef = fermi_level(ev, nk, nb, wk, n_el, p.smearing_sigma, spin);
+    std::vector<double> occ((size_t)nk * nb);
+    for (size_t i = 0; i < occ.size(); ++i) occ[i] = fd_occ(ev[i], ef, p.smearing_sigma) * spin;
+    // ---- density (src/scf/density.rs:41-111) ----------------------------------------------
+    PWK_HIP(hipMemsetAsync(rho_new.p, 0, N * 8, c.stream));
+    if (p.precision != PWK_FP64) launch_convert(c, V.p, V32.p, (size_t)nk * npw * nb * 2, true);
+    std::vector<std::vector<double>> fw((size_t)nk, std::vector<double>((size_t)nb));
+    {
+      TimedScope ts(c, "scf_density");
+      for (int k = 0; k < nk; ++k) {
+        for (int b = 0; b < nb; ++b) fw[(size_t)k][(size_t)b] = occ[(size_t)k * nb + b] * wk[(size_t)k];
+        if (p.precision == PWK_FP64)
+          launch_density(c, nb, V.as<double2>() + (size_t)k * npw * nb, fw[(size_t)k].data(), rho_new.p, PWK_FP64);
+        else
+          launch_density(c, nb, V32.as<float2>() + (size_t)k * npw * nb, fw[(size_t)k].data(), rho_new.p, PWK_FP32);
+        PWK_HIP(hipStreamSynchronize(c.stream));  // fw staging is host memory
+      }
+    }
+    dots({DotTerm{rho_new.as<double>(), nullptr, nullptr, nullptr, 0}}, N);
+    const double integral = host_small[0] * dvol;
+    if (std::fabs(integral) > 1e-15) {
+      const double sc = n_el / integral;
+      k_scale<<<gb(N), 256, 0, c.stream>>>(N, rho_new.as<double>(), sc);
+      PWK_HIP(hipGetLastError());
+    }
+    // ---- symmetrisation ------------------------------------------------------------------
+    if (ops.size() > 1) {
+      TimedScope ts(c, "scf_symmetrize");
+      k_r2c_scaled<<<gb(N), 256, 0, c.stream>>>(N, rho_new.as<double>(), 1.0, cwork.as<double2>());
+      launch_fft3d(c, cwork.p, 1, -1, PWK_FP64);
+      k_sym_gather<<<gb(N), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
+                                                 sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
+                                                 rho_g_new.as<double2>(), 1.0 / (double)ops.size());
+      launch_fft3d(c, rho_g_new.p, 1, +1, PWK_FP64);
+      k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, rho_g_new.as<double2>(), 1.0 / (double)N, rho_new.as<double>(), 0);
+      PWK_HIP(hipGetLastError());
+    }
+    // ---- output-density energies ---------------------------------------------------------
+    r_to_g(rho_new.as<double>(), rho_g_new.as<double2>());
+    // output-side xc reuses the work arrays (input-side values were reduced above)
+    k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_new.as<double>(), have_core ? core_r.as<double>() : nullptr,
+                                            rho_xc.as<double>());
+    launch_lda(c, rho_xc.p, exc.p, vxc.p, PWK_FP64);
+    std::vector<DotTerm> out_terms = {
+        DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+        DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+        DotTerm{rho_new.as<double>(), nullptr, vxc.as<double>(), nullptr, 0},
+        DotTerm{rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>(), rho_new.as<double>(), 0},
+        DotTerm{rho_new.as<double>(), nullptr, vloc_r.as<double>(), nullptr, 0},
+        DotTerm{rho_new.as<double>(), nullptr, nullptr, nullptr, 0}};
+    dots(out_terms, N);
+    const double e_h_out = host_small[0] * 4.0 * kPiC * kE2 * 0.5 * omega;
+    const double exc_int = host_small[1] * dvol, evxc = host_small[2] * dvol;
+    const double delta = std::sqrt(host_small[3] * dvol / omega);
+    const double e_loc = host_small[4] * dvol;
+    const double charge = host_small[5] * dvol;
+    double e_band = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      double sk = 0.0;
+      for (int b = 0; b < nb; ++b) sk += occ[(size_t)k * nb + b] * wk[(size_t)k] * ev[(size_t)k * nb + b];
+      e_band += sk;
+    }
+    const double ts = smearing_ts(ev, nk, nb, wk, ef, p.smearing_sigma, spin);
+    const double e_xc_out = exc_int - evxc;
+    const double e_total = e_band - e_h_out + e_xc_out + e_ewald + (-ts);
+    const double e_harris = e_band - eh_in + exc_in + e_ewald + (-ts);
+    const double de = have_prev ? std::fabs(e_total - e_prev) : INFINITY;
+    const bool e_conv = have_prev && de < p.energy_threshold;
+    have_prev = true;
+    e_prev = e_total;
+    res.n_iter = iter + 1;
+    res.final_delta = delta;
+    if (p.verbose)
+      std::fprintf(stderr, "[pwk scf] iter %3d  E=%.10f  dE=%.3e  delta=%.3e  EF=%.6f\n", iter, e_total, de, delta,
+                   ef);
+    if (delta < p.conv_threshold && e_conv) {
+      res.converged = 1;
+      res.e_total = e_total;
+      res.e_harris = e_harris;
+      res.fermi = ef;
+      res.ts = ts;
+      res.e_band = e_band;
+      res.e_hartree = e_h_out;
+      res.e_xc = exc_int;
+      res.e_vxc = evxc;
+      res.e_smearing = -ts;
+      res.e_local = e_loc;
+      res.integrated_charge = charge;
+      // kinetic and nonlocal expectation values (src/scf/energy.rs:382-510)
+      nl_x.ensure((size_t)npw * nb * 16);
+      expect.ensure((size_t)2 * nb * 8);
+      std::vector<double> e2((size_t)2 * nb);
+      for (int k = 0; k < nk; ++k) {
+        KData& kd = c.k(k);
+        const double2* psi = V.as<double2>() + (size_t)k * npw * nb;
+        if (kd.nch > 0) {
+          nl_p.ensure((size_t)kd.nch * nb * 16);
+          launch_zgemm_mfma(c, kd.nch, nb, npw, kd.DBH64.p, kd.nch, psi, npw, nl_p.p, kd.nch, false, PWK_FP64);
+          launch_zgemm_mfma(c, npw, nb, kd.nch, kd.B64.p, npw, nl_p.p, kd.nch, nl_x.p, npw, false, PWK_FP64);
+        } else {
+          PWK_HIP(hipMemsetAsync(nl_x.p, 0, (size_t)npw * nb * 16, c.stream));
+        }
+        k_band_expect<<<(unsigned)nb, 256, 0, c.stream>>>(npw, psi, kd.kin64.as<double>(), nl_x.as<double2>(),
+                                                         expect.as<double>());
+        PWK_HIP(hipGetLastError());
+        PWK_HIP(hipMemcpyAsync(e2.data(), expect.p, e2.size() * 8, hipMemcpyDeviceToHost, c.stream));
+        PWK_HIP(hipStreamSynchronize(c.stream));
+        double ek = 0.0, enl = 0.0;
+        for (int b = 0; b < nb; ++b) {
+          const double f = occ[(size_t)k * nb + b];
+          if (f == 0.0) continue;
+          ek += f * wk[(size_t)k] * e2[(size_t)2 * b];
+          enl += f * wk[(size_t)k] * e2[(size_t)2 * b + 1];
+        }
+        res.e_kinetic += ek;
+        res.e_nonlocal += enl;
+      }
+      if (eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);
+      if (rho_out) {
+        PWK_HIP(hipMemcpyAsync(rho_out, rho_new.p, N * 8, hipMemcpyDeviceToHost, c.stream));
+        PWK_HIP(hipStreamSynchronize(c.stream));
+      }
+      break;
+    }
+    // ---- Anderson mixing (src/scf/mixing/anderson.rs:170-266) --------------------------------
+    {
+      TimedScope ts(c, "scf_mix");
+      int slot;
+      if ((int)hslot.size() < ndim) {
+        slot = next_slot++;
+      } else {
+        slot = hslot.front();
+        hslot.erase(hslot.begin());
+      }
+      hslot.push_back(slot);
+      double* hin = hist.as<double>() + (size_t)slot * 2 * N;
+      double* hres = hin + N;
+      PWK_HIP(hipMemcpyAsync(hin, rho_in.p, N * 8, hipMemcpyDeviceToDevice, c.stream));
+      k_sub<<<gb(N), 256, 0, c.stream>>>(N, rho_new.as<double>(), rho_in.as<double>(), hres);
+      PWK_HIP(hipGetLastError());
+      const int m = (int)hslot.size();
+      MixArgs ma{};
+      ma.m = m;
+      ma.beta = p.mixing_beta;
+      for (int j = 0; j < m; ++j) {
+        ma.in[j] = hist.as<double>() + (size_t)hslot[(size_t)j] * 2 * N;
+        ma.res[j] = ma.in[j] + N;
+      }
+      if (m < 2) {
+        ma.coef[0] = 1.0;
+      } else {
+        const int mm = m - 1;
+        const double* rl = ma.res[m - 1];
+        std::vector<DotTerm> t;
+        for (int i = 0; i < mm; ++i) t.push_back(DotTerm{ma.res[i], rl, rl, nullptr, 0});
+        for (int i = 0; i < mm; ++i)
+          for (int j = i; j < mm; ++j) t.push_back(DotTerm{ma.res[i], rl, ma.res[j], rl, 0});
+        dots(t, N);
+        std::vector<double> A((size_t)mm * mm), bvec((size_t)mm);
+        for (int i = 0; i < mm; ++i) bvec[(size_t)i] = -host_small[i];
+        int q = mm;
+        for (int i = 0; i < mm; ++i)
+          for (int j = i; j < mm; ++j) {
+            A[(size_t)i * mm + j] = host_small[q];
+            A[(size_t)j * mm + i] = host_small[q];
+            ++q;
+          }
+        std::vector<double> alpha = small_solve(A, bvec, mm);
+        double asum = 0.0;
+        for (double x : alpha) asum += x;
+        for (int j = 0; j < mm; ++j) ma.coef[j] = alpha[(size_t)j];
+        ma.coef[m - 1] = 1.0 - asum;
+      }
+      k_anderson_combine<<<gb(N), 256, 0, c.stream>>>(N, ma, rho_in.as<double>());
+      PWK_HIP(hipGetLastError());
+      r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
+    }
+  }
+  if (!res.converged && eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);
+  PWK_HIP(hipStreamSynchronize(c.stream));
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
+             double* rho) {
+  Solve s(c, cr, p, out);
+  s.run(eig, rho);
+}
+
+void scf_tables(const pwk_crystal& cr, const pwk_scf_params& p, int* npw, int dims[3], int* nk, int* nops) {
+  Lat lat;
+  for (int i = 0; i < 3; ++i) lat.v[i] = {cr.lattice[3 * i], cr.lattice[3 * i + 1], cr.lattice[3 * i + 2]};
+  std::vector<int32_t> miller;
+  std::vector<Vec3> g;
+  sphere_basis(lat, p.ecut_ev, miller, g);
+  *npw = (int)g.size();
+  grid_dims(miller, p.ecutrho_ratio > 0 ? p.ecutrho_ratio : 4, p.fft_dims, dims);
+  std::vector<AtomSite> sites((size_t)cr.natoms);
+  for (int a = 0; a < cr.natoms; ++a) {
+    sites[(size_t)a].z = cr.species_data[cr.species[a]].atomic_number;
+    for (int d = 0; d < 3; ++d) sites[(size_t)a].frac[d] = cr.frac[3 * a + d];
+  }
+  std::vector<SymOp> ops;
+  if (p.use_symmetry) ops = detect_symmetry(lat, sites, p.symprec > 0.0 ? p.symprec : 1e-5);
+  const bool trev = p.use_symmetry != 0;
+  if (ops.empty()) {
+    SymOp id{};
+    id.rot[0] = id.rot[4] = id.rot[8] = 1;
+    ops.push_back(id);
+  }
+  int grid[3], shift[3];
+  for (int d = 0; d < 3; ++d) {
+    grid[d] = p.kgrid[d] > 0 ? p.kgrid[d] : 1;
+    shift[d] = p.kshift[d];
+  }
+  *nk = (int)kpoints(lat, grid, shift, ops, trev).size();
+  *nops = (int)ops.size();
+}
+
+}  // namespace pwk

@@ -78,6 +78,8 @@ _SIGS = {
     "pwk_dev_hpsi": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.c_int]),
     "pwk_dense_h": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "pwk_dev_dense_h": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+    "pwk_dense_eigh": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "pwk_dev_dense_eigh": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "pwk_density_accumulate": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_int]),
     "pwk_dev_density_accumulate": (C.c_int, [_vp, C.c_int, _vp, _dp, _vp, C.c_int]),
 }
@@ -305,6 +307,20 @@ class GpuAccelerator:
         h = np.empty((self.npw, self.npw), dtype=np.complex128)  # filled column-major
         _check(lib().pwk_dense_h(self.h, ik, int(with_nl), _d(h.view(np.float64))))
         return h.T.copy()
+
+    def dense_eigh(self, h, nb):
+        """Lowest ``nb`` eigenpairs of each Hermitian matrix in ``h`` (nk, n, n):
+        diagonalize_lowest (src/eigensolver/dense.rs:87-98) batched over k.
+        Returns (w (nk, nb) ascending, v (nk, n, nb))."""
+        h = np.asarray(h, dtype=np.complex128)
+        if h.ndim == 2:
+            h = h[None]
+        nk, n, _ = h.shape
+        hc = np.ascontiguousarray(np.transpose(h, (0, 2, 1)))  # column-major blocks
+        w = np.empty((nk, nb))
+        v = np.empty((nk, nb, n), dtype=np.complex128)
+        _check(lib().pwk_dense_eigh(self.h, n, nb, nk, _d(hc.view(np.float64)), _d(w), _d(v.view(np.float64))))
+        return w, np.transpose(v, (0, 2, 1))
 
     def density(self, psi, fw, rho=None, precision="fp64"):
         p = _c128(psi)

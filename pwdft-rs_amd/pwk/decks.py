@@ -9,6 +9,7 @@ G-vector arithmetic repeat the reference's floating-point operation order
 from __future__ import annotations
 
 import math
+import re
 import os
 
 import numpy as np
@@ -129,6 +130,11 @@ def kinetic(gvec, k):
     return HBAR2_OVER_2M * ((q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1]) + q[:, 2] * q[:, 2])
 
 
+ATOMIC_NUMBER = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Ne": 10, "Na": 11,
+                 "Mg": 12, "Al": 13, "Si": 14, "P": 15, "S": 16, "Cl": 17, "Ar": 18, "K": 19, "Ca": 20, "Ti": 22,
+                 "Fe": 26, "Co": 27, "Ni": 28, "Cu": 29, "Zn": 30, "Ga": 31, "Ge": 32, "As": 33, "Mn": 25}
+
+
 def read_upf(path):
     """Minimal UPF v2 reader for the radial tables (src/pseudopotential/upf/parse.rs:20-140)."""
     text = open(path).read()
@@ -163,6 +169,16 @@ def read_upf(path):
         chi.append(block(tag, mesh) / math.sqrt(BOHR_TO_ANG))
     out["l"] = ls
     out["chi"] = np.array(chi)
+    out["v_local"] = block("PP_LOCAL", mesh) * RY_TO_EV
+    out["atomic_number"] = ATOMIC_NUMBER[out["element"]]
+    out["rho_atom"] = None
+    if "<PP_RHOATOM" in text:
+        rho = block("PP_RHOATOM", mesh) / BOHR_TO_ANG
+        if np.any(np.abs(rho) > 1e-20):
+            out["rho_atom"] = rho
+    out["core"] = None
+    if re.search(r'core_correction="[Tt]"', text) or "nlcc=.true." in text:
+        out["core"] = block("PP_NLCC", mesh) / (BOHR_TO_ANG * BOHR_TO_ANG * BOHR_TO_ANG) if "<PP_NLCC" in text else np.zeros(mesh)
     return out
 
 

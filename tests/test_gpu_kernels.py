@@ -213,3 +213,65 @@ def test_density_accumulate(acc, ham, si_sys, prec, tol):
     ref = si_sys.density(psi, fw, rho0)
     out = acc.density(psi, fw, rho0, prec)
     assert rel(out, ref) < tol
+
+
+# ---- dense eigensolver (diagonalize_lowest, src/eigensolver/dense.rs:87-98) -----
+def _hermitian_with_spectrum(rng, lam):
+    n = len(lam)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    return (q * lam) @ q.conj().T
+
+
+def _check_pairs(h, w, v, w_ref, tol_w, tol_res):
+    n = h.shape[0]
+    nb = w.shape[0]
+    assert np.all(np.diff(w) >= -1e-12 * max(1.0, np.abs(w).max()))  # ascending
+    assert np.abs(w - w_ref[:nb]).max() <= tol_w * max(1.0, np.abs(w_ref).max())
+    res = h @ v - v * w
+    assert np.abs(res).max() <= tol_res * max(1.0, np.abs(w_ref).max())
+    assert np.abs(v.conj().T @ v - np.eye(nb)).max() < 1e-10
+    assert v.shape == (n, nb)
+
+
+def test_dense_eigh_random_and_degenerate(gpu):
+    rng = np.random.default_rng(5)
+    mats, refs = [], []
+    # generic spectrum, an exactly degenerate one (3-fold and 2-fold clusters), a
+    # near-degenerate one (splitting 1e-9) and a diagonal matrix (zero reflectors)
+    lam_sets = [
+        np.sort(rng.standard_normal(40)) * 10,
+        np.array([-5.0] * 3 + [-1.0] * 2 + list(np.linspace(0.5, 9.0, 35))),
+        np.array([-2.0, -2.0 + 1e-9, -2.0 + 2e-9] + list(np.linspace(-1.0, 4.0, 37))),
+    ]
+    for lam in lam_sets:
+        mats.append(_hermitian_with_spectrum(rng, lam))
+        refs.append(np.sort(lam))
+    mats.append(np.diag(np.arange(40, dtype=float)[::-1]).astype(np.complex128))
+    refs.append(np.arange(40, dtype=float))
+    h = np.stack(mats)
+    for nb in (1, 8, 40):
+        w, v = gpu.dense_eigh(h, nb)
+        for i in range(len(mats)):
+            _check_pairs(mats[i], w[i], v[i], refs[i], 1e-12, 1e-10)
+
+
+def test_dense_eigh_tiny_sizes(gpu):
+    rng = np.random.default_rng(9)
+    for n in (1, 2, 3):
+        hm = _hermitian_with_spectrum(rng, np.sort(rng.standard_normal(n)))
+        w, v = gpu.dense_eigh(hm, n)
+        _check_pairs(hm, w[0], v[0], np.linalg.eigvalsh(hm), 1e-12, 1e-12)
+
+
+def test_dense_eigh_kohn_sham(gpu, oracle, si_sys):
+    # dense H of Si at two k-points: eigenvalues equal the oracle's FP64 EVD
+    rng = np.random.default_rng(3)
+    n = si_sys.n_grid
+    veff = np.fft.fftn(rng.uniform(-1, 1, si_sys.dims)).ravel() / n
+    ks = [np.zeros(3), np.array([0.1, -0.2, 0.05])]
+    hs = [si_sys.dense_h(k, veff, True) for k in ks]
+    nb = 12
+    w, v = gpu.dense_eigh(np.stack(hs), nb)
+    for i, hm in enumerate(hs):
+        w_ref, _ = oracle.eigh(hm)
+        _check_pairs(hm, w[i], v[i], w_ref, 1e-12, 1e-9)

@@ -1,0 +1,103 @@
+"""Device SCF (pwk_scf_run) against the reference's end-to-end pins and the FP64
+oracle on the same decks.
+
+Bars: the reference's own pins (tests/cpu_scf_baseline.rs:43-74, 1 meV;
+tests/qe_validation.rs:476-513, 1 meV) and, against the oracle on the same deck,
+total energy and every energy term within 1e-5 Ry/atom, integrated charge to 1e-8.
+"""
+import numpy as np
+import pytest
+
+from conftest import SI_UPF
+
+pytestmark = pytest.mark.gpu
+
+RY = 13.605693122994
+BOHR = 0.529177210903
+TOL_TERM = 1e-5 * RY  # eV per atom
+SI_FRAC = [[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]]
+TERMS = ["e_total", "e_band", "e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_ewald", "e_smearing"]
+
+
+def fcc(a):
+    h = a / 2.0
+    return np.array([[0.0, h, h], [h, 0.0, h], [h, h, 0.0]])
+
+
+@pytest.fixture(scope="module")
+def scf_mod(pwk_mod):
+    from pwk import scf
+
+    return scf
+
+
+def _run(scf_mod, gpu, lat, p):
+    deck = scf_mod.CrystalDeck(lat, SI_FRAC, [0, 0], [SI_UPF])
+    print("scf deck", p, flush=True)
+    return deck, scf_mod.run_scf(gpu, deck, p)
+
+
+def test_scf_cpu_baseline_pin(scf_mod, gpu, oracle):
+    # Si 5.431 Å, 100 eV, fixed 16³, Γ, 4 bands, σ 0.05 eV, β 0.3, ndim 4, identity symmetry
+    p = scf_mod.ScfParams(ecut_ev=100.0, fft_dims=(16, 16, 16), use_symmetry=False, n_bands=4, max_iter=40,
+                          conv_threshold=1e-6, energy_threshold=1e-5, mixing_ndim=4, smearing_sigma=0.05)
+    _, r = _run(scf_mod, gpu, fcc(5.431), p)
+    assert r["converged"] == 1 and r["n_iter"] < 40
+    assert abs(r["e_total"] - (-213.028339)) < 1e-3
+    assert abs(r["fermi"] - 8.052064) < 1e-3
+    s = oracle.System(fcc(5.431), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 100.0, dims=[16, 16, 16])
+    ro = s.scf([[0, 0, 0]], [1.0], n_bands=4, max_iter=40, conv=1e-6, ethr=1e-5, beta=0.3, ndim=4, sigma=0.05)
+    print("gpu", r["e_total"], r["n_iter"], "oracle", ro["e_total"], ro["n_iter"], flush=True)
+    assert abs(r["n_iter"] - ro["n_iter"]) <= 1
+    for t in TERMS:
+        assert abs(r[t] - ro[t]) < TOL_TERM * 2, t
+    assert abs(r["integrated_charge"] - 8.0) < 1e-8
+    assert np.abs(r["eigenvalues"][0] - ro["eigenvalues"][0]).max() < 1e-5
+
+
+@pytest.fixture(scope="module")
+def small_deck_results(scf_mod, gpu, oracle):
+    # Si2 FCC a = 10.26 bohr, 12 Ry (15³, 169 PW), 4×4×4 Γ-centred → 8 IBZ k, 8 bands,
+    # FD 0.01 Ry, Anderson β 0.3 ndim 8, conv 1e-8
+    lat = fcc(10.26 * BOHR)
+    kw = dict(ecut_ev=12 * RY, kgrid=(4, 4, 4), n_bands=8, max_iter=100, conv_threshold=1e-8,
+              energy_threshold=1e-8 * RY, mixing_ndim=8, smearing_sigma=0.01 * RY)
+    _, r64 = _run(scf_mod, gpu, lat, scf_mod.ScfParams(**kw))
+    _, r32 = _run(scf_mod, gpu, lat, scf_mod.ScfParams(precision="fp32", **kw))
+    s = oracle.System(lat, [0, 0], [14, 14], SI_FRAC, [SI_UPF], 12 * RY)
+    k, w = oracle.kpoints(lat, [14, 14], SI_FRAC, [4, 4, 4], (0, 0, 0), mode=2)
+    ro = s.scf(k, w, n_bands=8, max_iter=100, conv=1e-8, ethr=1e-8 * RY, beta=0.3, ndim=8, sigma=0.01 * RY,
+               use_sym=True)
+    return r64, r32, ro
+
+
+def test_small_deck_fp64_matches_oracle(small_deck_results):
+    r, _, ro = small_deck_results
+    print({t: (r[t], ro[t]) for t in TERMS}, flush=True)
+    assert r["converged"] == 1 and ro["converged"] == 1
+    assert r["nk"] == 8 and r["npw"] == 169 and tuple(r["dims"]) == (15, 15, 15) and r["n_ops"] == 48
+    for t in TERMS:
+        assert abs(r[t] - ro[t]) / 2 < TOL_TERM, t
+    assert abs(r["fermi"] - ro["fermi"]) < 1e-4
+    assert abs(r["integrated_charge"] - 8.0) < 1e-8
+    assert np.abs(r["eigenvalues"] - ro["eigenvalues"]).max() < 1e-5
+    # the energy split closes (tests/vgc5_per_component_si.rs:326-333)
+    comp = sum(r[t] for t in ("e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_ewald", "e_smearing"))
+    assert abs(comp - r["e_total"]) < 1e-5
+
+
+def test_small_deck_mixed_precision_error(small_deck_results):
+    r64, r32, _ = small_deck_results
+    assert r32["converged"] == 1
+    for t in TERMS:
+        assert abs(r32[t] - r64[t]) / 2 < TOL_TERM, t
+    assert abs(r32["integrated_charge"] - 8.0) < 1e-8
+
+
+def test_qe_gamma_centred_pin(scf_mod, gpu):
+    # Si 5.431 Å, 4×4×4 Γ-centred, 15 Ry, 8 bands, FD 0.01 Ry, conv 1e-8 → -231.6544 eV within 1 meV
+    p = scf_mod.ScfParams(ecut_ev=15 * RY, kgrid=(4, 4, 4), n_bands=8, max_iter=80, conv_threshold=1e-8,
+                          energy_threshold=1e-6, mixing_ndim=8, smearing_sigma=0.01 * RY)
+    _, r = _run(scf_mod, gpu, fcc(5.431), p)
+    assert r["converged"] == 1
+    assert abs(r["e_total"] - (-231.6544)) < 1e-3
