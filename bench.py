@@ -1,16 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: H·ψ at scale on one MI355X.
+"""Benchmark on one MI355X: SCF solves/s with p50/p99 latency, served mixed
+precision against the plain FP64 GPU form, plus H·ψ at scale.
 
-Workload (the configuration the metric is quoted on that fits one GPU):
-64-atom cubic Si supercell a = 20.52 bohr, ecutwfc 20 Ry -> 13133 plane waves
-on a 60³ grid, Γ point, 144 bands; ψ complex with seeded Gaussian real/imag
-parts × exp(-|G|²_bohr/8), normalised; V_eff(r) seeded uniform in [-1, 1] Ry;
-2 s + 2 p Kleinman-Bylander projectors per atom (the first four β of Si.upf
-with their 4×4 D block) -> 512 channels. One step = one full H·ψ over the
-144 bands with inputs resident in HBM, through libpwk's C ABI
-(pwk_dev_hpsi). The mixed-precision form (FP32 FFT/pointwise, FP32
-projector GEMMs) is the headline; the FP64 GPU form is timed beside it and
-the error of the mixed form is reported against it.
+SCF workload (the SCF deck that fits one GPU; the metric is quoted in SCF
+solves): 2-atom diamond Si FCC cell a = 10.26 bohr, Si.upf (nc/lda), PZ-LDA,
+ecutwfc 12 Ry (169 plane waves, 15³ grid), 4×4×4 Γ-centred MP grid reduced by
+the 48 operations + time reversal to 8 k-points, 8 bands, Fermi-Dirac 0.01 Ry,
+Anderson β = 0.3 (8 steps of history), dense eigensolver, converged to
+RMS Δρ < 1e-8 e/Å³ and |ΔE| < 1e-8 Ry. Every solve starts from the atomic
+density (setup included in its time).
+
+  * plain FP64 GPU form: one pwk_scf_run at a time, FP64 throughout;
+  * served mixed form (the headline ``value``): a pwk_server with --slots
+    worker contexts; one step submits --slots solves at once and waits for all
+    (continuous batching: solves overlap on the device); density FFTs in FP32,
+    eigensolver, energies, Ewald and mixing in FP64. Its total energy is
+    checked against the FP64 form: |ΔE| per atom must stay below 1e-5 Ry.
+
+H·ψ at scale (reported beside, --no-hpsi skips it): 64-atom cubic Si supercell
+a = 20.52 bohr, ecutwfc 20 Ry -> 13133 plane waves on 60³, Γ, 144 bands;
+seeded ψ and V_eff; 2 s + 2 p KB projectors per atom -> 512 channels; one
+H·ψ over the 144 bands with inputs resident in HBM (pwk_dev_hpsi), FP32 form
+against FP64.
 
 Prints one JSON line. Only one GPU is used; --gpus N > 1 exits.
 """
@@ -110,32 +121,10 @@ def algorithmic(name, w, nb, sc, sr):
     return None, None
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--bands", type=int, default=144)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "split_bf16"])
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (0: all, max 16)")
-    ap.add_argument("--copy-calibration-mb", type=int, default=0,
-                    help="run one plain device copy of this size first (HBM counter calibration under rocprofv3)")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported (the H·psi path would shard "
-              "by k-point or band across GPUs, which this build does not launch).", file=sys.stderr)
-        sys.exit(2)
-
+def hpsi_bench(args, acc):
     import numpy as np
     import torch
-    import pwk
 
-    if not torch.cuda.is_available():
-        raise SystemExit("no GPU visible")
-    acc = pwk.GpuAccelerator(0)
     nb = args.bands
     if args.copy_calibration_mb > 0:
         n = args.copy_calibration_mb * (1 << 20) // 4
@@ -162,13 +151,13 @@ def main():
     step32 = lambda: acc.hpsi_dev(0, nb, psi32.data_ptr(), out32.data_ptr(), prec)
     step64 = lambda: acc.hpsi_dev(0, nb, psi64.data_ptr(), out64.data_ptr(), "fp64")
 
-    log(f"timing: {args.warmup} warmup + {args.steps} steps ({prec})")
-    t32 = time_steps(acc, step32, args.steps, args.warmup, torch)
-    kern32 = kernel_profile(acc, step32, max(3, min(args.steps, 10)))
+    log(f"timing: H·psi, 3 warmup + {args.hpsi_steps} steps ({prec})")
+    t32 = time_steps(acc, step32, args.hpsi_steps, 3, torch)
+    kern32 = kernel_profile(acc, step32, max(3, min(args.hpsi_steps, 10)))
 
     res64 = None
     if not args.no_fp64:
-        steps64 = max(3, args.steps // 2)
+        steps64 = max(3, args.hpsi_steps // 2)
         log(f"timing: FP64 GPU form, {steps64} steps")
         t64 = time_steps(acc, step64, steps64, 1, torch)
         kern64 = kernel_profile(acc, step64, 3)
@@ -209,7 +198,6 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import pyoracle
 
             threads = pyoracle.set_threads(args.cpu_threads or min(16, os.cpu_count() or 1))
@@ -242,17 +230,11 @@ def main():
             log(f"cpu baseline failed: {e!r}")
             cpu = None
 
-    out = {
-        "metric": METRIC,
+    return {
         "value": 1.0 / t32,
         "unit": "Hpsi/s (one H·psi over 144 bands)",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": args.hpsi_steps,
         "ms_per_step": 1e3 * t32,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
         "dtype": "fp32" if prec == "fp32" else "fp32+split-bf16",
         "data": "synthetic (seeded psi, V_eff; Si.upf 2s+2p projectors)",
         "config": {"workload": "H·psi at scale: Si64 cubic a=20.52 bohr, ecutwfc 20 Ry, Gamma, 144 bands, "
@@ -263,8 +245,245 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+
+
+
+
+# ---------------------------------------------------------------------------------------
+# SCF solves/s
+# ---------------------------------------------------------------------------------------
+RY = 13.605693122994
+BOHR = 0.529177210903
+SI_FRAC = [[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]]
+
+
+def small_deck(precision):
+    from pwk import scf
+
+    a = 10.26 * BOHR
+    h = a / 2.0
+    lat = [[0.0, h, h], [h, 0.0, h], [h, h, 0.0]]
+    deck = scf.CrystalDeck(lat, SI_FRAC, [0, 0], [os.path.join(ROOT, "tests", "golden", "pseudo", "Si.upf")])
+    p = scf.ScfParams(ecut_ev=12 * RY, kgrid=(4, 4, 4), use_symmetry=True, n_bands=8, max_iter=100,
+                      conv_threshold=1e-8, energy_threshold=1e-8 * RY, mixing_beta=0.3, mixing_ndim=8,
+                      smearing_sigma=0.01 * RY, precision=precision)
+    return deck, p
+
+
+def percentile(xs, q):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def scf_algorithmic(name, t):
+    """Algorithmic bytes (HBM-bound kernels) or flops (eigensolver) per launch of the
+    dominant SCF kernels on the small deck (nk matrices of n = npw, nb bands)."""
+    n, nb, nk, N = t["npw"], t["nb"], t["nk"], t["N"]
+    if name == "dense_eigh":
+        # Householder tridiagonalisation of a complex Hermitian matrix: 16/3·n³ real flops,
+        # back-transformation of nb vectors: 8·n²·nb
+        return "flop", nk * (16.0 / 3.0 * n ** 3 + 8.0 * n * n * nb)
+    if name == "scf_dense_h":
+        return "hbm", nk * (n * n * 16 + N * 16)
+    if name == "scf_density":
+        return "hbm", nk * nb * (2 * N * 16 + n * 16)
+    return None, None
+
+
+def scf_bench(args):
+    import numpy as np
+    import pwk
+    from pwk import scf
+
+    deck64, p64 = small_deck("fp64")
+    deckmx, pmx = small_deck(args.scf_precision)
+    tab = scf.scf_tables(deck64, p64)
+    natoms = 2
+    acc = pwk.GpuAccelerator(0)
+
+    # plain FP64 GPU form: one solve at a time
+    log(f"scf: FP64 form, 1 warmup + {args.fp64_solves} solves (npw {tab['npw']}, grid {tab['dims']}, nk {tab['nk']})")
+    scf.run_scf(acc, deck64, p64)
+    lat64, r64 = [], None
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_solves):
+        t1 = time.perf_counter()
+        r64 = scf.run_scf(acc, deck64, p64)
+        lat64.append(time.perf_counter() - t1)
+        if r64["converged"] != 1:
+            raise SystemExit("FP64 solve did not converge")
+    wall64 = time.perf_counter() - t0
+    # kernel profile of one FP64 solve (library HIP-event timing)
+    acc.profile_reset()
+    acc.profile(True)
+    rprof = scf.run_scf(acc, deck64, p64)
+    prof = acc.profile_read()
+    acc.profile(False)
+    acc.profile_reset()
+    kern = {k: {"avg_us": 1e3 * ms / max(n, 1), "launches": n, "total_ms": ms} for k, (ms, n) in prof.items()}
+
+    # served mixed form
+    srv = scf.Server(0, n_slots=args.slots)
+    log(f"scf: served {args.scf_precision} form, {args.slots} slots, {args.warmup} warmup + {args.steps} steps")
+    for _ in range(args.warmup):
+        ts = [srv.submit(deckmx, pmx) for _ in range(args.slots)]
+        for t in ts:
+            srv.wait(t)
+    lat_mx, results = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = [srv.submit(deckmx, pmx) for _ in range(args.slots)]
+        for t in ts:
+            r = srv.wait(t)
+            lat_mx.append(r["latency_s"])
+            results.append(r)
+    wall_mx = time.perf_counter() - t0
+    srv.close()
+    n_mx = len(results)
+    if any(r["converged"] != 1 for r in results):
+        raise SystemExit("a served solve did not converge")
+    de = max(abs(r["e_total"] - r64["e_total"]) for r in results) / RY / natoms
+    terms = ["e_band", "e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_ewald", "e_smearing"]
+    de_terms = {t: max(abs(r[t] - r64[t]) for r in results) / RY / natoms for t in terms}
+
+    # roofline of the dominant kernel (by total time in one FP64 solve)
+    info = {"npw": tab["npw"], "nb": min(8, tab["npw"]), "nk": tab["nk"], "N": int(np.prod(tab["dims"]))}
+    name, kv = max(kern.items(), key=lambda x: x[1]["total_ms"])
+    bound, amount = scf_algorithmic(name, info)
+    dur = kv["avg_us"] * 1e-6
+    if bound == "flop":
+        roofline = {"kernel": name, "bound": "mfma", "achieved": round(amount / dur / 1e12, 5),
+                    "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(amount / dur / 1e12 / FP64_MFMA_PEAK_TFLOPS, 6), "traffic": None,
+                    "algorithmic_per_launch": amount,
+                    "note": "FP64 eigensolver, one workgroup per k-point matrix: latency-bound at this size"}
+    elif bound == "hbm":
+        roofline = {"kernel": name, "bound": "hbm", "achieved": round(amount / dur / 1e9, 3), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(amount / dur / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
+                    "algorithmic_per_launch": amount}
+    else:
+        roofline = {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
+                    "traffic": None}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            import pyoracle
+
+            threads = pyoracle.set_threads(args.cpu_threads or min(16, os.cpu_count() or 1))
+            s = pyoracle.System(deck64.lattice, [0, 0], [14, 14], SI_FRAC,
+                                [os.path.join(ROOT, "tests", "golden", "pseudo", "Si.upf")], 12 * RY)
+            k, w = pyoracle.kpoints(deck64.lattice, [14, 14], SI_FRAC, [4, 4, 4], (0, 0, 0), mode=2)
+            log(f"cpu baseline: oracle FP64 SCF solve, {threads} threads")
+            t1 = time.perf_counter()
+            ro = s.scf(k, w, n_bands=8, max_iter=100, conv=1e-8, ethr=1e-8 * RY, beta=0.3, ndim=8, sigma=0.01 * RY,
+                       use_sym=True)
+            tc = time.perf_counter() - t1
+            cpu = {"value": 1.0 / tc, "unit": "SCF solves/s", "cores": threads, "kind": "port",
+                   "sample": f"one full solve of the same deck by the FP64 oracle ({int(ro['n_iter'])} iterations, "
+                             f"{tc:.2f} s wall)",
+                   "gpu_fp64_vs_cpu_abs_dE_eV": abs(r64["e_total"] - ro["e_total"])}
+        except Exception as e:
+            log(f"cpu baseline failed: {e!r}")
     acc.close()
+    return {
+        "value": n_mx / wall_mx,
+        "ms_per_step": 1e3 * wall_mx / max(args.steps, 1),
+        "p50_latency_ms": 1e3 * percentile(lat_mx, 50),
+        "p99_latency_ms": 1e3 * percentile(lat_mx, 99),
+        "solves_timed": n_mx,
+        "fp64_gpu": {"solves_per_s": args.fp64_solves / wall64, "p50_latency_ms": 1e3 * percentile(lat64, 50),
+                     "p99_latency_ms": 1e3 * percentile(lat64, 99), "n_iter": r64["n_iter"],
+                     "e_total_eV": r64["e_total"]},
+        "speedup_vs_fp64_gpu": (n_mx / wall_mx) / (args.fp64_solves / wall64),
+        "abs_dE_Ry_per_atom_vs_fp64": de,
+        "abs_dE_terms_Ry_per_atom": de_terms,
+        "n_iter": results[0]["n_iter"],
+        "kernels_fp64_solve_us": {k: {"avg_us": round(v["avg_us"], 2), "launches": v["launches"]}
+                                  for k, v in kern.items()},
+        "config": {"workload": "SCF solve: Si2 diamond FCC a=10.26 bohr, ecutwfc 12 Ry (169 PW, 15^3), "
+                               "4x4x4 Gamma-centred MP -> 8 IBZ k, 8 bands, FD 0.01 Ry, Anderson beta 0.3 ndim 8, "
+                               "dense eigensolver, conv 1e-8",
+                   "npw": tab["npw"], "grid": list(tab["dims"]), "nk": tab["nk"], "bands": 8,
+                   "slots": args.slots, "precision_served": args.scf_precision},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4, help="timed steps; one step = --slots solves served at once")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--fp64-solves", type=int, default=5)
+    ap.add_argument("--scf-precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--no-hpsi", action="store_true", help="skip the H·psi at scale section")
+    ap.add_argument("--hpsi-only", action="store_true")
+    ap.add_argument("--hpsi-steps", type=int, default=20)
+    ap.add_argument("--bands", type=int, default=144)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "split_bf16"], help="H·psi precision")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 H·psi comparison")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (0: all, max 16)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run one plain device copy of this size first (HBM counter calibration under rocprofv3)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported (SCF solves would shard by "
+              "k-point across GPUs with one all-reduce of the density per iteration; this build does not launch "
+              "it).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        raise SystemExit("no GPU visible")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pwk
+
+    hp = None
+    if not args.no_hpsi:
+        acc = pwk.GpuAccelerator(0)
+        hp = hpsi_bench(args, acc)
+        acc.close()
+    if args.hpsi_only:
+        out = {"metric": METRIC, "value": hp["value"], "unit": hp["unit"], "n_gpus": 1, "steps": args.hpsi_steps,
+               "warmup": 3, "ms_per_step": hp["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": hp["dtype"], "data": hp["data"], "config": hp["config"],
+               "roofline": hp["roofline"], "cpu_baseline": hp["cpu_baseline"], "hpsi_at_scale": hp}
+        print(json.dumps(out), flush=True)
+        return
+    sc = scf_bench(args)
+    out = {
+        "metric": METRIC,
+        "value": sc["value"],
+        "unit": "SCF solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sc["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 density FFTs + fp64 eigensolver/energies" if args.scf_precision == "fp32" else "fp64",
+        "data": "synthetic deck (Si2 diamond, repo Si.upf); no external data",
+        "config": sc["config"],
+        "p50_latency_ms": sc["p50_latency_ms"],
+        "p99_latency_ms": sc["p99_latency_ms"],
+        "fp64_gpu": sc["fp64_gpu"],
+        "speedup_vs_fp64_gpu": sc["speedup_vs_fp64_gpu"],
+        "abs_dE_Ry_per_atom_vs_fp64": sc["abs_dE_Ry_per_atom_vs_fp64"],
+        "abs_dE_terms_Ry_per_atom": sc["abs_dE_terms_Ry_per_atom"],
+        "kernels_fp64_solve_us": sc["kernels_fp64_solve_us"],
+        "roofline": sc["roofline"],
+        "cpu_baseline": sc["cpu_baseline"],
+        "hpsi_at_scale": hp,
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -257,7 +257,9 @@ typedef struct pwk_scf_params {
   int fft_dims[3];        /* all > 0: explicit grid; else derived */
   int kgrid[3];
   int kshift[3];
-  int use_symmetry;       /* 0: identity only; 1: detected operations + time reversal */
+  int use_symmetry;       /* 0: identity only; 1: detected operations + time reversal,
+                             k-points reduced to the IBZ; 2: the full MP grid with the
+                             density symmetrised by the detected operations */
   double symprec;         /* 1e-5 */
   int n_bands;
   int max_iter;
@@ -267,7 +269,12 @@ typedef struct pwk_scf_params {
   int mixing_ndim;
   double smearing_sigma;  /* Fermi-Dirac width, eV */
   double sad_sigma;       /* width of the Gaussian start density without PP_RHOATOM, Å */
-  int precision;          /* PWK_FP64, or PWK_FP32 for the density FFTs */
+  int precision;          /* PWK_FP64; PWK_FP32 / PWK_SPLIT_BF16: density FFTs and the
+                             Chebyshev filter's H·ψ in that precision, FP64 elsewhere */
+  int eigensolver;        /* 0: dense (diagonalize_lowest per k); 1: Chebyshev-filtered
+                             subspace iteration on H·ψ with FP64 Rayleigh-Ritz; -1: dense
+                             up to 1500 plane waves, filtered above */
+  int cheb_degree;        /* filter polynomial degree (0: 10) */
   int verbose;
 } pwk_scf_params;
 
@@ -294,6 +301,22 @@ int pwk_scf_run(pwk_ctx* ctx, const pwk_crystal* crystal, const pwk_scf_params* 
  * symmetry operations. Host only. */
 int pwk_scf_tables(const pwk_crystal* crystal, const pwk_scf_params* params, int* npw, int dims[3], int* nk,
                    int* nops);
+
+/* ---- Serving: a request loop of SCF solves on one device --------------------------
+ * No counterpart in the reference (it runs one solve per process). n_slots
+ * worker threads each run one solve at a time in a context of their own;
+ * queued requests are admitted in arrival order as slots and device memory
+ * (hbm_fraction of the free HBM at creation) allow, so solves of different
+ * sizes overlap on the device. The crystal's arrays must stay valid until the
+ * request has been waited for. pwk_server_wait blocks until the request has
+ * finished and returns its status (PWK_E_OOM when it can never fit); the
+ * latency is from submission to completion, in seconds. */
+typedef struct pwk_server pwk_server;
+int pwk_server_create(int device, int n_slots, double hbm_fraction, pwk_server** out);
+int pwk_server_submit(pwk_server* srv, const pwk_crystal* crystal, const pwk_scf_params* params, int64_t* ticket);
+int pwk_server_wait(pwk_server* srv, int64_t ticket, pwk_scf_result* result, double* eigenvalues, double* rho,
+                    double* latency_s);
+int pwk_server_destroy(pwk_server* srv);
 
 #ifdef __cplusplus
 }

@@ -207,6 +207,13 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
                            const double* dij, const int32_t* mesh, const double* r, const double* rab,
                            const double* chi, double omega);
 
+// subspace.hip: Chebyshev filter and Rayleigh-Ritz helpers
+void launch_cheb_step(Context& c, size_t n, const void* HY, const void* Y, const void* X, double alpha, double cshift,
+                      double beta, void* out, int precision);
+void launch_conj_transpose(Context& c, int rows, int cols, const void* A, int lda, void* out, int ldo);
+void launch_hermitize(Context& c, int m, void* A);
+void launch_lowdin_scale(Context& c, int m, const double* s, double floor_rel, void* U);
+void launch_start_block(Context& c, int npw, int nb, const int* lowest_dev, void* V);
 // scf.hip
 void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
              double* rho);

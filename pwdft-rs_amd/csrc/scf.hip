@@ -343,6 +343,12 @@ __global__ void k_anderson_combine(size_t n, MixArgs a, double* __restrict__ out
 // ---------------------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------------------
+std::vector<SymOp> identity_ops() {
+  SymOp id{};
+  id.rot[0] = id.rot[4] = id.rot[8] = 1;
+  return {id};
+}
+
 
 // Simpson coefficients of src/numerics.rs:29-58 (odd mesh: 1,4,2,...,4,1; even mesh:
 // the QE/DFTK end correction), scaled by 3 so that Σ c_i f_i rab_i / 3 is the rule.
@@ -501,8 +507,20 @@ struct Solve {
     launch_fft3d(c, g, 1, -1, PWK_FP64);
   }
 
+  // Chebyshev-filtered subspace iteration state
+  bool chebyshev = false;
+  std::vector<double>* ev_host = nullptr;  // nk × nb eigenvalues on the host
+  int fprec = PWK_FP64;                // precision of the filter's H·ψ
+  int degree = 10;
+  std::vector<double> ub;              // per-k upper bound of the spectrum
+  DevBuf F0, F1, F2, HYf, Y64, HY64, YH, sS, sH, sU, sT, sX, sXH, sW, sC, s_ev, s_lam, s_work, lowest;
+
   void setup();
   double ewald();
+  void rayleigh_ritz(int k, const double2* Y, const double2* HY, double2* Vk);
+  void lanczos_bound(int k);
+  void chebyshev_pass(int k);
+  void eig_chebyshev(bool first);
   void veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out, double* vxc_out);
   void run(double* eig_out, double* rho_out);
 };
@@ -573,7 +591,7 @@ void Solve::setup() {
     grid[d] = p.kgrid[d] > 0 ? p.kgrid[d] : 1;
     shift[d] = p.kshift[d];
   }
-  kpts = kpoints(lat, grid, shift, ops, trev);
+  kpts = kpoints(lat, grid, shift, p.use_symmetry == 2 ? identity_ops() : ops, p.use_symmetry == 2 ? false : trev);
   nk = (int)kpts.size();
   wk.resize((size_t)nk);
   for (int k = 0; k < nk; ++k) wk[(size_t)k] = kpts[(size_t)k].weight;
@@ -736,11 +754,39 @@ void Solve::setup() {
   PWK_HIP(hipMemcpyAsync(sym_rot.p, rot.data(), rot.size() * 4, hipMemcpyHostToDevice, c.stream));
   PWK_HIP(hipMemcpyAsync(sym_tau.p, tr.data(), tr.size() * 8, hipMemcpyHostToDevice, c.stream));
   // eigen-system buffers
-  Hb.ensure((size_t)nk * npw * npw * 16);
+  chebyshev = p.eigensolver == 1 || (p.eigensolver < 0 && !(npw <= 1500 && heev_supported(npw, nb)));
   W.ensure((size_t)nk * nb * 8);
   V.ensure((size_t)nk * npw * nb * 16);
   if (p.precision != PWK_FP64) V32.ensure((size_t)nk * npw * nb * 8);
-  c.eig_work.ensure(heev_work_doubles(npw, nb, nk) * 8);
+  if (!chebyshev) {
+    Hb.ensure((size_t)nk * npw * npw * 16);
+    c.eig_work.ensure(heev_work_doubles(npw, nb, nk) * 8);
+  } else {
+    PWK_REQUIRE(heev_supported(nb, nb), PWK_E_UNSUPPORTED, "too many bands for the subspace eigensolver");
+    fprec = p.precision;
+    degree = p.cheb_degree > 0 ? p.cheb_degree : 10;
+    const size_t blk = (size_t)npw * nb;
+    const size_t fsz = fprec == PWK_FP64 ? 16 : 8;
+    for (DevBuf* b : {&F0, &F1, &F2, &HYf}) b->ensure(blk * fsz);
+    for (DevBuf* b : {&Y64, &HY64, &YH}) b->ensure(blk * 16);
+    for (DevBuf* b : {&sS, &sH, &sU, &sT, &sX, &sXH, &sW, &sC}) b->ensure((size_t)nb * nb * 16);
+    s_ev.ensure((size_t)nb * 8);
+    s_lam.ensure((size_t)nb * 8);
+    s_work.ensure(heev_work_doubles(nb, nb, 1) * 8);
+    ub.assign((size_t)nk, 0.0);
+    // start block: the nb lowest-kinetic plane waves of each k (plus a seeded perturbation)
+    lowest.ensure((size_t)nb * nk * 4);
+    std::vector<int> low((size_t)nb * nk);
+    for (int k = 0; k < nk; ++k) {
+      std::vector<std::pair<double, int>> t((size_t)npw);
+      for (int i = 0; i < npw; ++i) t[(size_t)i] = {vnorm2(vadd(kpts[(size_t)k].cart, gvec[(size_t)i])), i};
+      std::partial_sort(t.begin(), t.begin() + nb, t.end());
+      for (int j = 0; j < nb; ++j) low[(size_t)k * nb + j] = t[(size_t)j].second;
+    }
+    PWK_HIP(hipMemcpyAsync(lowest.p, low.data(), low.size() * 4, hipMemcpyHostToDevice, c.stream));
+    for (int k = 0; k < nk; ++k)
+      launch_start_block(c, npw, nb, lowest.as<int>() + (size_t)k * nb, V.as<double2>() + (size_t)k * npw * nb);
+  }
   PWK_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -808,6 +854,138 @@ void Solve::veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_
   launch_hartree_veff(c, rho_g, 4.0 * kPiC * kE2, vloc_g.p, vxc_g.p, veff_g.p, PWK_FP64, true, true, true);
 }
 
+// Rayleigh-Ritz on span(Y) in FP64 with Löwdin orthonormalisation:
+// S = Y^H Y = U s U^H, X = U s^{-1/2}, H' = X^H (Y^H HY) X = W λ W^H, V = Y·X·W.
+void Solve::rayleigh_ritz(int k, const double2* Y, const double2* HY, double2* Vk) {
+  TimedScope ts(c, "scf_rayleigh_ritz");
+  const int m = nb;
+  launch_conj_transpose(c, npw, m, Y, npw, YH.p, m);
+  launch_zgemm_mfma(c, m, m, npw, YH.p, m, Y, npw, sS.p, m, false, PWK_FP64);
+  launch_zgemm_mfma(c, m, m, npw, YH.p, m, HY, npw, sH.p, m, false, PWK_FP64);
+  launch_hermitize(c, m, sS.p);
+  launch_hermitize(c, m, sH.p);
+  launch_heev_batched(c, m, m, 1, sS.p, s_work.p, s_ev.as<double>(), sU.p, nullptr);
+  launch_lowdin_scale(c, m, s_ev.as<double>(), 1e-14, sU.p);  // sU is now X
+  launch_zgemm_mfma(c, m, m, m, sH.p, m, sU.p, m, sT.p, m, false, PWK_FP64);
+  launch_conj_transpose(c, m, m, sU.p, m, sXH.p, m);
+  launch_zgemm_mfma(c, m, m, m, sXH.p, m, sT.p, m, sX.p, m, false, PWK_FP64);
+  launch_hermitize(c, m, sX.p);
+  launch_heev_batched(c, m, m, 1, sX.p, s_work.p, s_lam.as<double>(), sW.p, nullptr);
+  launch_zgemm_mfma(c, m, m, m, sU.p, m, sW.p, m, sC.p, m, false, PWK_FP64);
+  launch_zgemm_mfma(c, npw, m, m, Y, npw, sC.p, m, Vk, npw, false, PWK_FP64);
+  PWK_HIP(hipMemcpyAsync(ev_host->data() + (size_t)k * nb, s_lam.p, (size_t)nb * 8, hipMemcpyDeviceToHost,
+                         c.stream));
+}
+
+// Upper bound of the spectrum of H_k: ten Lanczos steps from a seeded vector;
+// the Gershgorin bound of the Lanczos matrix plus the last off-diagonal.
+void Solve::lanczos_bound(int k) {
+  const int steps = std::min(10, npw);
+  std::vector<int> none(1, -1);
+  DevBuf dn;
+  dn.ensure(4);
+  PWK_HIP(hipMemcpyAsync(dn.p, none.data(), 4, hipMemcpyHostToDevice, c.stream));
+  // Lanczos runs in FP64 on the first column of the (FP64-sized) filter buffers
+  DevBuf a, b0, b1;
+  a.ensure((size_t)npw * 16);
+  b0.ensure((size_t)npw * 16);
+  b1.ensure((size_t)npw * 16);
+  double2* v0 = a.as<double2>();
+  double2* v1 = b0.as<double2>();
+  double2* w = b1.as<double2>();
+  launch_start_block(c, npw, 1, dn.as<int>(), v1);
+  PWK_HIP(hipMemsetAsync(v0, 0, (size_t)npw * 16, c.stream));
+  const size_t n2 = (size_t)npw * 2;
+  dots({DotTerm{(const double*)v1, nullptr, (const double*)v1, nullptr, 0}}, n2);
+  k_scale<<<gb(n2), 256, 0, c.stream>>>(n2, (double*)v1, 1.0 / std::sqrt(host_small[0]));
+  PWK_HIP(hipGetLastError());
+  std::vector<double> al, be;
+  double beta = 0.0;
+  for (int j = 0; j < steps; ++j) {
+    launch_hpsi(c, k, 1, v1, w, PWK_FP64);
+    dots({DotTerm{(const double*)v1, nullptr, (const double*)w, nullptr, 0}}, n2);
+    const double alpha = host_small[0];
+    launch_cheb_step(c, (size_t)npw, w, v1, v0, 1.0, alpha, beta, w, PWK_FP64);
+    dots({DotTerm{(const double*)w, nullptr, (const double*)w, nullptr, 0}}, n2);
+    al.push_back(alpha);
+    beta = std::sqrt(std::max(host_small[0], 0.0));
+    be.push_back(beta);
+    if (beta < 1e-12) break;
+    std::swap(v0, v1);  // v0 <- v_j
+    k_scale<<<gb(n2), 256, 0, c.stream>>>(n2, (double*)w, 1.0 / beta);
+    PWK_HIP(hipGetLastError());
+    std::swap(v1, w);  // v1 <- v_{j+1}; w is free again
+  }
+  double bound = -INFINITY;
+  for (size_t i = 0; i < al.size(); ++i) {
+    const double r = std::fabs(i > 0 ? be[i - 1] : 0.0) + std::fabs(i + 1 < al.size() ? be[i] : 0.0);
+    bound = std::max(bound, al[i] + r);
+  }
+  ub[(size_t)k] = bound + std::fabs(be.back());
+  PWK_HIP(hipStreamSynchronize(c.stream));
+}
+
+// One filtered pass on k: Y = p(H)·V with the scaled Chebyshev recurrence on
+// [a, b] (a = highest Ritz value kept, b = spectrum bound, scaled at the lowest
+// Ritz value), then Rayleigh-Ritz. The filter runs in the solve's precision,
+// the Rayleigh-Ritz step in FP64.
+void Solve::chebyshev_pass(int k) {
+  TimedScope ts(c, "scf_chebyshev_filter");
+  const size_t blk = (size_t)npw * nb;
+  double2* Vk = V.as<double2>() + (size_t)k * blk;
+  const double* lam = ev_host->data() + (size_t)k * nb;
+  const double a0 = lam[0];
+  double a = lam[nb - 1], b = ub[(size_t)k];
+  if (!(b > a + 1e-6)) b = a + std::max(1.0, std::fabs(a));
+  if (!(a > a0)) a = a0 + 1e-3 * (b - a0);
+  const double e = 0.5 * (b - a), cc = 0.5 * (b + a);
+  double sigma = e / (a0 - cc);
+  const double tau = 2.0 / sigma;
+  void* buf[3] = {F0.p, F1.p, F2.p};
+  if (fprec == PWK_FP64)
+    PWK_HIP(hipMemcpyAsync(buf[0], Vk, blk * 16, hipMemcpyDeviceToDevice, c.stream));
+  else
+    launch_convert(c, Vk, buf[0], blk * 2, true);
+  void *X = buf[0], *Y = buf[1], *Z = buf[2];
+  launch_hpsi(c, k, nb, X, HYf.p, fprec);
+  launch_cheb_step(c, blk, HYf.p, X, nullptr, sigma / e, cc, 0.0, Y, fprec);
+  for (int i = 2; i <= degree; ++i) {
+    const double sn = 1.0 / (tau - sigma);
+    launch_hpsi(c, k, nb, Y, HYf.p, fprec);
+    launch_cheb_step(c, blk, HYf.p, Y, X, 2.0 * sn / e, cc, sigma * sn, Z, fprec);
+    void* t = X;
+    X = Y;
+    Y = Z;
+    Z = t;
+    sigma = sn;
+  }
+  if (fprec == PWK_FP64)
+    PWK_HIP(hipMemcpyAsync(Y64.p, Y, blk * 16, hipMemcpyDeviceToDevice, c.stream));
+  else
+    launch_convert(c, Y, Y64.p, blk * 2, false);
+  launch_hpsi(c, k, nb, Y64.p, HY64.p, PWK_FP64);
+  rayleigh_ritz(k, Y64.as<double2>(), HY64.as<double2>(), Vk);
+}
+
+void Solve::eig_chebyshev(bool first) {
+  const size_t blk = (size_t)npw * nb;
+  if (first) {
+    for (int k = 0; k < nk; ++k) {
+      lanczos_bound(k);
+      double2* Vk = V.as<double2>() + (size_t)k * blk;
+      PWK_HIP(hipMemcpyAsync(Y64.p, Vk, blk * 16, hipMemcpyDeviceToDevice, c.stream));
+      launch_hpsi(c, k, nb, Y64.p, HY64.p, PWK_FP64);
+      rayleigh_ritz(k, Y64.as<double2>(), HY64.as<double2>(), Vk);
+      PWK_HIP(hipStreamSynchronize(c.stream));
+    }
+  }
+  const int passes = first ? 4 : 1;
+  for (int pass = 0; pass < passes; ++pass) {
+    for (int k = 0; k < nk; ++k) chebyshev_pass(k);
+    PWK_HIP(hipStreamSynchronize(c.stream));
+  }
+}
+
 void Solve::run(double* eig_out, double* rho_out) {
   const auto t0 = std::chrono::steady_clock::now();
   setup();
@@ -841,14 +1019,20 @@ void Solve::run(double* eig_out, double* rho_out) {
     const double eh_in = host_small[0] * 4.0 * kPiC * kE2 * 0.5 * omega;
     const double exc_in = host_small[1] * dvol - host_small[2] * dvol;
     set_veff_device(c, veff_g.p);
-    // ---- dense H per k, one batched eigensolve ------------------------------------------
-    {
-      TimedScope ts(c, "scf_dense_h");
-      for (int k = 0; k < nk; ++k) launch_dense_h(c, k, 1, Hb.as<double2>() + (size_t)k * npw * npw);
+    if (chebyshev) {
+      // ---- Chebyshev-filtered subspace iteration on H·ψ, FP64 Rayleigh-Ritz -----------
+      ev_host = &ev;
+      eig_chebyshev(iter == 0);
+    } else {
+      // ---- dense H per k, one batched eigensolve ----------------------------------------
+      {
+        TimedScope ts(c, "scf_dense_h");
+        for (int k = 0; k < nk; ++k) launch_dense_h(c, k, 1, Hb.as<double2>() + (size_t)k * npw * npw);
+      }
+      launch_heev_batched(c, npw, nb, nk, Hb.p, c.eig_work.p, W.as<double>(), V.p, nullptr);
+      PWK_HIP(hipMemcpyAsync(ev.data(), W.p, ev.size() * 8, hipMemcpyDeviceToHost, c.stream));
+      PWK_HIP(hipStreamSynchronize(c.stream));
     }
-    launch_heev_batched(c, npw, nb, nk, Hb.p, c.eig_work.p, W.as<double>(), V.p, nullptr);
-    PWK_HIP(hipMemcpyAsync(ev.data(), W.p, ev.size() * 8, hipMemcpyDeviceToHost, c.stream));
-    PWK_HIP(hipStreamSynchronize(c.stream));
     // ---- occupations (src/scf/smearing.rs) -----------------------------------------------
     const double ef = fermi_level(ev, nk, nb, wk, n_el, p.smearing_sigma, spin);
     std::vector<double> occ((size_t)nk * nb);
@@ -1067,7 +1251,8 @@ void scf_tables(const pwk_crystal& cr, const pwk_scf_params& p, int* npw, int di
     grid[d] = p.kgrid[d] > 0 ? p.kgrid[d] : 1;
     shift[d] = p.kshift[d];
   }
-  *nk = (int)kpoints(lat, grid, shift, ops, trev).size();
+  *nk = (int)kpoints(lat, grid, shift, p.use_symmetry == 2 ? identity_ops() : ops, p.use_symmetry == 2 ? false : trev)
+            .size();
   *nops = (int)ops.size();
 }
 

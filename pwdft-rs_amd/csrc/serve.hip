@@ -1,0 +1,241 @@
+// Request loop for SCF solves on one device (continuous batching).
+//
+// A server owns n_slots worker threads, each with its own context and HIP
+// stream. Requests queue in arrival order; a free slot admits the oldest
+// request whose estimated device footprint fits in the free HBM left after
+// the solves already running, so solves of different cell sizes run side by
+// side and overlap on the device while later arrivals wait for memory or a
+// slot. Each solve keeps ψ, ρ and the mixer history resident in its own
+// context for its whole lifetime. A request larger than the whole budget
+// fails with PWK_E_OOM instead of waiting forever.
+//
+// Solves are independent and each is deterministic (fixed-order reductions),
+// so a result does not depend on what else is running.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "common.hpp"
+
+namespace pwk {
+
+struct ServeJob {
+  int64_t id = 0;
+  pwk_crystal cr{};
+  pwk_scf_params p{};
+  size_t bytes = 0;
+  int status = PWK_OK;
+  std::string error;
+  pwk_scf_result res{};
+  std::vector<double> eig, rho;
+  bool done = false;
+  std::chrono::steady_clock::time_point t_submit, t_start, t_end;
+};
+
+struct Server {
+  int device = 0;
+  size_t budget = 0;  // bytes the server may hold for running solves
+  size_t in_use = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<ServeJob>> queue;
+  std::map<int64_t, std::shared_ptr<ServeJob>> jobs;
+  int64_t next_id = 1;
+  bool stopping = false;
+  std::vector<std::thread> workers;
+
+  bool admissible(const ServeJob& j) const { return j.bytes > budget || in_use + j.bytes <= budget; }
+
+  // Runs one solve in a fresh context (own stream, buffers released afterwards,
+  // so the memory accounting above matches what the device holds).
+  void solve(ServeJob& job) {
+    if (job.bytes > budget) {
+      job.status = PWK_E_OOM;
+      job.error = "solve does not fit in the server's device-memory budget";
+      return;
+    }
+    try {
+      PWK_HIP(hipSetDevice(device));
+      Context c;
+      c.device = device;
+      PWK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+      struct StreamGuard {
+        Context& c;
+        ~StreamGuard() {
+          (void)hipStreamSynchronize(c.stream);
+          c.kd.clear();
+          (void)hipStreamDestroy(c.stream);
+        }
+      } guard{c};
+      hipDeviceProp_t prop;
+      PWK_HIP(hipGetDeviceProperties(&prop, device));
+      c.cu_count = prop.multiProcessorCount;
+      int npw, dims[3], nk, nops;
+      scf_tables(job.cr, job.p, &npw, dims, &nk, &nops);
+      job.eig.assign((size_t)nk * (size_t)std::min(job.p.n_bands, npw), 0.0);
+      job.rho.assign((size_t)dims[0] * dims[1] * dims[2], 0.0);
+      scf_run(c, job.cr, job.p, job.res, job.eig.data(), job.rho.data());
+    } catch (const Error& e) {
+      job.status = e.code;
+      job.error = e.what();
+    } catch (const std::exception& e) {
+      job.status = PWK_E_DEVICE;
+      job.error = e.what();
+    }
+  }
+
+  void worker_main() {
+    for (;;) {
+      std::shared_ptr<ServeJob> job;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] {
+          if (stopping && queue.empty()) return true;
+          for (auto& j : queue)
+            if (admissible(*j)) return true;
+          return false;
+        });
+        if (queue.empty()) break;  // only reached when stopping
+        for (auto it = queue.begin(); it != queue.end(); ++it)
+          if (admissible(**it)) {
+            job = *it;
+            queue.erase(it);
+            break;
+          }
+        if (job->bytes <= budget) in_use += job->bytes;
+      }
+      job->t_start = std::chrono::steady_clock::now();
+      solve(*job);
+      job->t_end = std::chrono::steady_clock::now();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (job->bytes <= budget) in_use -= job->bytes;
+        job->done = true;
+      }
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+};
+
+// Device bytes one solve holds at its peak (dominant terms of scf.hip).
+size_t scf_footprint(const pwk_crystal& cr, const pwk_scf_params& p) {
+  int npw, dims[3], nk, nops;
+  scf_tables(cr, p, &npw, dims, &nk, &nops);
+  const size_t n = (size_t)npw, nb = (size_t)std::min(p.n_bands, npw), N = (size_t)dims[0] * dims[1] * dims[2];
+  size_t nch = 0;
+  for (int a = 0; a < cr.natoms; ++a) {
+    const pwk_species& s = cr.species_data[cr.species[a]];
+    for (int i = 0; i < s.nproj; ++i) nch += (size_t)(2 * s.proj_l[i] + 1);
+  }
+  size_t b = (size_t)nk * n * n * 16;                   // dense H batch
+  b += heev_work_doubles(npw, (int)nb, nk) * 8;         // eigensolver scratch
+  b += (size_t)nk * n * nb * (16 + 8);                  // eigenvectors (FP64 + FP32 copy)
+  b += (size_t)nk * n * nch * 16 * 3;                   // B, D·B^H (FP64) and FP32 copies
+  b += N * (16 * 10 + 8 * 12) + (size_t)2 * std::max(1, p.mixing_ndim) * N * 8;  // grids, mixer history
+  b += (size_t)nb * dims[2] * n * 16 + (64u << 20);     // stick buffers, slack
+  return b;
+}
+
+}  // namespace pwk
+
+struct pwk_server {
+  pwk::Server s;
+};
+
+namespace {
+template <class F>
+int serve_guarded(F&& f) {
+  try {
+    f();
+    return PWK_OK;
+  } catch (const pwk::Error& e) {
+    pwk::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    pwk::set_last_error(e.what());
+    return PWK_E_DEVICE;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int pwk_server_create(int device, int n_slots, double hbm_fraction, pwk_server** out) {
+  return serve_guarded([&] {
+    PWK_REQUIRE(out && n_slots >= 1 && n_slots <= 64 && hbm_fraction > 0.0 && hbm_fraction <= 1.0, PWK_E_INVALID,
+                "bad server arguments (1..64 slots, 0 < hbm_fraction <= 1)");
+    *out = nullptr;
+    int nd = 0;
+    PWK_REQUIRE(hipGetDeviceCount(&nd) == hipSuccess && device >= 0 && device < nd, PWK_E_NO_DEVICE,
+                "no such HIP device");
+    PWK_HIP(hipSetDevice(device));
+    size_t free_b = 0, total_b = 0;
+    PWK_HIP(hipMemGetInfo(&free_b, &total_b));
+    auto srv = std::make_unique<pwk_server>();
+    srv->s.device = device;
+    srv->s.budget = (size_t)((double)free_b * hbm_fraction);
+    for (int i = 0; i < n_slots; ++i) srv->s.workers.emplace_back([p = &srv->s] { p->worker_main(); });
+    *out = srv.release();
+  });
+}
+
+int pwk_server_submit(pwk_server* srv, const pwk_crystal* crystal, const pwk_scf_params* params, int64_t* ticket) {
+  return serve_guarded([&] {
+    PWK_REQUIRE(srv && crystal && params && ticket, PWK_E_INVALID, "null argument");
+    auto job = std::make_shared<pwk::ServeJob>();
+    job->cr = *crystal;
+    job->p = *params;
+    job->bytes = pwk::scf_footprint(*crystal, *params);
+    job->t_submit = std::chrono::steady_clock::now();
+    {
+      std::lock_guard<std::mutex> lk(srv->s.mu);
+      PWK_REQUIRE(!srv->s.stopping, PWK_E_STATE, "server is shutting down");
+      job->id = srv->s.next_id++;
+      srv->s.jobs[job->id] = job;
+      srv->s.queue.push_back(job);
+    }
+    srv->s.cv_work.notify_all();
+    *ticket = job->id;
+  });
+}
+
+int pwk_server_wait(pwk_server* srv, int64_t ticket, pwk_scf_result* result, double* eigenvalues, double* rho,
+                    double* latency_s) {
+  std::shared_ptr<pwk::ServeJob> job;
+  int st = serve_guarded([&] {
+    PWK_REQUIRE(srv, PWK_E_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(srv->s.mu);
+    auto it = srv->s.jobs.find(ticket);
+    PWK_REQUIRE(it != srv->s.jobs.end(), PWK_E_INVALID, "unknown ticket");
+    job = it->second;
+    srv->s.cv_done.wait(lk, [&] { return job->done; });
+    srv->s.jobs.erase(it);
+  });
+  if (st != PWK_OK) return st;
+  if (result) *result = job->res;
+  if (eigenvalues) std::copy(job->eig.begin(), job->eig.end(), eigenvalues);
+  if (rho) std::copy(job->rho.begin(), job->rho.end(), rho);
+  if (latency_s) *latency_s = std::chrono::duration<double>(job->t_end - job->t_submit).count();
+  if (job->status != PWK_OK) pwk::set_last_error(job->error);
+  return job->status;
+}
+
+int pwk_server_destroy(pwk_server* srv) {
+  return serve_guarded([&] {
+    if (!srv) return;
+    {
+      std::lock_guard<std::mutex> lk(srv->s.mu);
+      srv->s.stopping = true;
+    }
+    srv->s.cv_work.notify_all();
+    for (auto& t : srv->s.workers) t.join();
+    delete srv;
+  });
+}
+
+}  // extern "C"

@@ -37,7 +37,7 @@ class Params(C.Structure):
                 ("n_bands", C.c_int), ("max_iter", C.c_int), ("conv_threshold", C.c_double),
                 ("energy_threshold", C.c_double), ("mixing_beta", C.c_double), ("mixing_ndim", C.c_int),
                 ("smearing_sigma", C.c_double), ("sad_sigma", C.c_double), ("precision", C.c_int),
-                ("verbose", C.c_int)]
+                ("eigensolver", C.c_int), ("cheb_degree", C.c_int), ("verbose", C.c_int)]
 
 
 class Result(C.Structure):
@@ -58,18 +58,17 @@ _SIGS = {
     "pwk_scf_tables": (C.c_int, [C.POINTER(Crystal), C.POINTER(Params), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
-_bound = False
+_bound = set()
 
 
 def _lib():
-    global _bound
     L = lib()
-    if not _bound:
-        for name, (res, args) in _SIGS.items():
+    for name, (res, args) in _SIGS.items():
+        if name not in _bound:
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _bound = True
+            _bound.add(name)
     return L
 
 
@@ -154,6 +153,8 @@ class ScfParams:
     ecutrho_ratio: int = 4
     fft_dims: tuple = (0, 0, 0)
     precision: str = "fp64"
+    eigensolver: int = -1  # 0 dense, 1 Chebyshev-filtered subspace iteration, -1 by size
+    cheb_degree: int = 0
     verbose: bool = False
 
 
@@ -190,7 +191,7 @@ def _params(p: ScfParams):
     return Params(p.ecut_ev, p.ecutrho_ratio, (C.c_int * 3)(*p.fft_dims), (C.c_int * 3)(*p.kgrid),
                   (C.c_int * 3)(*p.kshift), int(p.use_symmetry), p.symprec, p.n_bands, p.max_iter, p.conv_threshold,
                   p.energy_threshold, p.mixing_beta, p.mixing_ndim, p.smearing_sigma, p.sad_sigma, _prec(p.precision),
-                  int(p.verbose))
+                  int(p.eigensolver), int(p.cheb_degree), int(p.verbose))
 
 
 def scf_tables(deck: CrystalDeck, p: ScfParams):
@@ -218,5 +219,56 @@ def run_scf(acc, deck: CrystalDeck, p: ScfParams):
     return out
 
 
-__all__ = ["basis", "fft_dims", "symmetry", "kpoints", "ScfParams", "CrystalDeck", "scf_tables", "run_scf",
-           "PwkError"]
+
+
+# ---- serving --------------------------------------------------------------------------
+_SIGS.update({
+    "pwk_server_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.POINTER(C.c_void_p)]),
+    "pwk_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Crystal), C.POINTER(Params), C.POINTER(C.c_int64)]),
+    "pwk_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Result), _dp, _dp, C.POINTER(C.c_double)]),
+    "pwk_server_destroy": (C.c_int, [C.c_void_p]),
+})
+
+
+class Server:
+    """Request loop of SCF solves on one device (``pwk_server_*``): ``submit``
+    returns a ticket at once; ``wait`` blocks until that solve is done and
+    returns the same dict as :func:`run_scf` plus ``latency_s``."""
+
+    def __init__(self, device=0, n_slots=8, hbm_fraction=0.8):
+        h = C.c_void_p()
+        _check(_lib().pwk_server_create(device, n_slots, hbm_fraction, C.byref(h)))
+        self.h = h
+        self._pending = {}
+
+    def submit(self, deck: CrystalDeck, p: ScfParams):
+        prm = _params(p)
+        t = C.c_int64()
+        _check(_lib().pwk_server_submit(self.h, C.byref(deck.c), C.byref(prm), C.byref(t)))
+        self._pending[t.value] = (deck, prm, scf_tables(deck, p), min(p.n_bands, scf_tables(deck, p)["npw"]))
+        return t.value
+
+    def wait(self, ticket):
+        deck, prm, t, nb = self._pending.pop(ticket)
+        ev = np.zeros((t["nk"], nb))
+        rho = np.zeros(int(np.prod(t["dims"])))
+        res = Result()
+        lat = C.c_double()
+        _check(_lib().pwk_server_wait(self.h, ticket, C.byref(res), _pd(ev), _pd(rho), C.byref(lat)))
+        out = {name: getattr(res, name) for name, _ in Result._fields_ if name != "dims"}
+        out["dims"] = tuple(res.dims)
+        out["eigenvalues"] = ev
+        out["rho"] = rho
+        out["latency_s"] = lat.value
+        return out
+
+    def close(self):
+        if self.h:
+            _check(_lib().pwk_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

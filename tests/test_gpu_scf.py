@@ -94,10 +94,63 @@ def test_small_deck_mixed_precision_error(small_deck_results):
     assert abs(r32["integrated_charge"] - 8.0) < 1e-8
 
 
-def test_qe_gamma_centred_pin(scf_mod, gpu):
-    # Si 5.431 Å, 4×4×4 Γ-centred, 15 Ry, 8 bands, FD 0.01 Ry, conv 1e-8 → -231.6544 eV within 1 meV
-    p = scf_mod.ScfParams(ecut_ev=15 * RY, kgrid=(4, 4, 4), n_bands=8, max_iter=80, conv_threshold=1e-8,
-                          energy_threshold=1e-6, mixing_ndim=8, smearing_sigma=0.01 * RY)
-    _, r = _run(scf_mod, gpu, fcc(5.431), p)
-    assert r["converged"] == 1
+def test_qe_gamma_centred_pin(scf_mod, gpu, oracle):
+    # tests/qe_validation.rs:476-513 passes the full 4×4×4 Γ-centred list to run_scf with the
+    # crystal's symmetry for the density: Si 5.431 Å, 15 Ry, 8 bands, FD 0.01 Ry, conv 1e-8
+    # → -231.6544 eV within 1 meV
+    kw = dict(ecut_ev=15 * RY, kgrid=(4, 4, 4), n_bands=8, max_iter=80, conv_threshold=1e-8, energy_threshold=1e-6,
+              mixing_ndim=8, smearing_sigma=0.01 * RY)
+    _, r = _run(scf_mod, gpu, fcc(5.431), scf_mod.ScfParams(use_symmetry=2, **kw))
+    assert r["converged"] == 1 and r["nk"] == 64 and r["n_ops"] == 48
     assert abs(r["e_total"] - (-231.6544)) < 1e-3
+    # the IBZ-reduced solve of the same deck equals the oracle's IBZ solve (the reduced and
+    # the full list differ by 5 meV in the reference's algorithm itself)
+    _, ri = _run(scf_mod, gpu, fcc(5.431), scf_mod.ScfParams(use_symmetry=1, **kw))
+    s = oracle.System(fcc(5.431), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 15 * RY)
+    k, w = oracle.kpoints(fcc(5.431), [14, 14], SI_FRAC, [4, 4, 4], (0, 0, 0), mode=2)
+    ro = s.scf(k, w, n_bands=8, max_iter=80, conv=1e-8, ethr=1e-6, beta=0.3, ndim=8, sigma=0.01 * RY, use_sym=True)
+    assert ri["nk"] == 8
+    assert abs(ri["e_total"] - ro["e_total"]) < 1e-5
+
+
+def test_served_results_equal_alone(scf_mod, gpu):
+    # serving: solves of different sizes submitted together overlap on the device;
+    # each result equals the same solve run alone (solves are deterministic)
+    decks, params = [], []
+    for a_scale, ecut_ry, kg in ((1.00, 12, 2), (1.02, 14, 2), (0.98, 12, 3), (1.01, 16, 1)):
+        lat = fcc(10.26 * BOHR * a_scale)
+        decks.append(scf_mod.CrystalDeck(lat, SI_FRAC, [0, 0], [SI_UPF]))
+        params.append(scf_mod.ScfParams(ecut_ev=ecut_ry * RY, kgrid=(kg, kg, kg), n_bands=8, max_iter=60,
+                                        conv_threshold=1e-7, energy_threshold=1e-7, smearing_sigma=0.01 * RY))
+    alone = [scf_mod.run_scf(gpu, d, p) for d, p in zip(decks, params)]
+    srv = scf_mod.Server(0, n_slots=4)
+    try:
+        tickets = [srv.submit(d, p) for d, p in zip(decks, params)] * 1
+        mixed = [srv.wait(t) for t in tickets]
+    finally:
+        srv.close()
+    for a, m in zip(alone, mixed):
+        assert m["converged"] == 1 and a["converged"] == 1
+        assert m["n_iter"] == a["n_iter"]
+        for t in TERMS:
+            assert abs(m[t] - a[t]) < 1e-9, t
+        assert np.array_equal(m["eigenvalues"], a["eigenvalues"])
+        assert m["latency_s"] > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_small_deck_chebyshev_filtered(scf_mod, gpu, small_deck_results, precision):
+    # the H·ψ-based eigensolver (FP64 Rayleigh-Ritz; filter H·ψ in `precision`) reaches the
+    # same fixed point as the dense eigensolver
+    r64, _, ro = small_deck_results
+    lat = fcc(10.26 * BOHR)
+    p = scf_mod.ScfParams(ecut_ev=12 * RY, kgrid=(4, 4, 4), n_bands=8, max_iter=100, conv_threshold=1e-8,
+                          energy_threshold=1e-8 * RY, mixing_ndim=8, smearing_sigma=0.01 * RY, eigensolver=1,
+                          precision=precision)
+    _, r = _run(scf_mod, gpu, lat, p)
+    print("chebyshev", precision, r["n_iter"], r["e_total"], "dense", r64["n_iter"], r64["e_total"], flush=True)
+    assert r["converged"] == 1
+    for t in TERMS:
+        assert abs(r[t] - ro[t]) / 2 < TOL_TERM, t
+    # occupied bands (4 of 8) agree with the dense solve
+    assert np.abs(r["eigenvalues"][:, :4] - r64["eigenvalues"][:, :4]).max() < 1e-5

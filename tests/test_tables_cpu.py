@@ -119,6 +119,9 @@ def test_scf_tables_of_small_deck(scf_mod):
     assert t == {"npw": 169, "dims": (15, 15, 15), "nk": 8, "nops": 48}
     p2 = scf_mod.ScfParams(ecut_ev=12 * 13.605693122994, kgrid=(2, 2, 2), use_symmetry=False)
     assert scf_mod.scf_tables(deck, p2)["nk"] == 8 and scf_mod.scf_tables(deck, p2)["nops"] == 1
+    # full list with a symmetrised density (the reference's run_scf fed an unreduced MP grid)
+    p3 = scf_mod.ScfParams(ecut_ev=12 * 13.605693122994, kgrid=(4, 4, 4), use_symmetry=2)
+    assert scf_mod.scf_tables(deck, p3)["nk"] == 64 and scf_mod.scf_tables(deck, p3)["nops"] == 48
 
 
 def test_upf_reader_fields(pwk_mod):
