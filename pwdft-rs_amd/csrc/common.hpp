@@ -189,6 +189,8 @@ void setup_basis(Context& c, int npw, const int32_t* miller_host);
 void set_veff_device(Context& c, const void* veff_g_dev);
 void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int precision);
 void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision);
+void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, int precision, const void* kin,
+                       int nch, const void* B, const void* DBH, void* proj);
 // dense.hip
 void launch_dense_h(Context& c, int ik, int with_nl, void* h);
 // gemm.hip: complex C = alpha·op(A)·B + beta·C, column-major, op = N only
@@ -197,6 +199,10 @@ void launch_zgemm(Context& c, int M, int N, int K, const void* A, int lda, const
 // gemm_mfma.hip: the same product on the matrix cores (FP32, FP64 or 3-term split-bf16)
 void launch_zgemm_mfma(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C,
                        int ldc, bool accumulate, int precision);
+// `batch` independent products with element strides sA, sB, sC between them
+void launch_zgemm_mfma_batched(Context& c, int M, int N, int K, const void* A, int lda, long long sA, const void* B,
+                               int ldb, long long sB, void* C, int ldc, long long sC, int batch, bool accumulate,
+                               int precision);
 // eig.hip: lowest nb eigenpairs of nk Hermitian n×n matrices (H overwritten)
 size_t heev_work_doubles(int n, int nb, int nk);
 bool heev_supported(int n, int nb);
@@ -207,13 +213,18 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
                            const double* dij, const int32_t* mesh, const double* r, const double* rab,
                            const double* chi, double omega);
 
-// subspace.hip: Chebyshev filter and Rayleigh-Ritz helpers
-void launch_cheb_step(Context& c, size_t n, const void* HY, const void* Y, const void* X, double alpha, double cshift,
-                      double beta, void* out, int precision);
-void launch_conj_transpose(Context& c, int rows, int cols, const void* A, int lda, void* out, int ldo);
-void launch_hermitize(Context& c, int m, void* A);
-void launch_lowdin_scale(Context& c, int m, const double* s, double floor_rel, void* U);
-void launch_start_block(Context& c, int npw, int nb, const int* lowest_dev, void* V);
+// subspace.hip: Chebyshev filter and Rayleigh-Ritz helpers, batched over blocks
+void launch_cheb_step(Context& c, size_t n, size_t per, const void* HY, const void* Y, const void* X,
+                      const double* coef_dev, void* out, int precision);
+void launch_conj_transpose(Context& c, int rows, int cols, const void* A, int lda, long long sA, void* out, int ldo,
+                           long long sO, int batch);
+void launch_hermitize(Context& c, int m, void* A, int batch);
+void launch_lowdin_scale(Context& c, int m, const double* s, double floor_rel, void* U, int batch);
+void launch_block_dot(Context& c, size_t per, int batch, const double* x, const double* y, double* out);
+void launch_block_scale(Context& c, size_t per, int batch, const double* s, int mode, double* x);
+void launch_start_block(Context& c, int npw, int nb, int nk, const int* lowest_dev, void* V);
+void launch_resid_norms(Context& c, int npw, int nbs, int nres, int nk, const void* V, const void* HV,
+                        const double* lam, double* out);
 // scf.hip
 void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
              double* rho);

@@ -55,7 +55,8 @@ struct Lds {
 };
 
 template <int MODE, int NB16>
-__global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kchunk,
+__global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kchunk, int nsplit, long long sA,
+                                                    long long sB, long long sC,
                                                     const typename Traits<MODE>::V2* __restrict__ A, int lda,
                                                     const typename Traits<MODE>::V2* __restrict__ B, int ldb,
                                                     typename Traits<MODE>::V2* __restrict__ C, int ldc, int accumulate,
@@ -74,7 +75,13 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
   const int m0 = blockIdx.x * BMC, n0 = blockIdx.y * BN;
-  const int kbeg = blockIdx.z * kchunk;
+  // blockIdx.z = batch · nsplit + split
+  const int zb = (int)blockIdx.z / nsplit, zs = (int)blockIdx.z % nsplit;
+  A += (size_t)zb * sA;
+  B += (size_t)zb * sB;
+  C += (size_t)zb * sC;
+  if (partial) partial += (size_t)zb * nsplit * M * N;
+  const int kbeg = zs * kchunk;
   const int kend = min(K, kbeg + kchunk);
 
   acc_t acc[2][NB16];
@@ -195,7 +202,7 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
           o.x = acc[rb][cb][q];
           o.y = partner[q];
           if (partial) {
-            partial[((size_t)blockIdx.z * N + gn) * M + gm] = o;
+            partial[((size_t)zs * N + gn) * M + gm] = o;
           } else {
             V2* c = &C[(size_t)gn * ldc + gm];
             if (accumulate) {
@@ -224,7 +231,7 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
         o.x = acc[rb][cb][2 * h];
         o.y = acc[rb][cb][2 * h + 1];
         if (partial) {
-          partial[((size_t)blockIdx.z * N + gn) * M + gm] = o;
+          partial[((size_t)zs * N + gn) * M + gm] = o;
         } else {
           V2* c = &C[(size_t)gn * ldc + gm];
           if (accumulate) {
@@ -240,9 +247,11 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
 
 template <typename V2>
 __global__ void __launch_bounds__(256) k_splitk_sum(int M, int N, int nsplit, const V2* __restrict__ partial,
-                                                    V2* __restrict__ C, int ldc, int accumulate) {
+                                                    V2* __restrict__ C, int ldc, long long sC, int accumulate) {
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)M * N) return;
+  partial += (size_t)blockIdx.y * nsplit * M * N;
+  C += (size_t)blockIdx.y * sC;
   int gm = (int)(idx % M), gn = (int)(idx / M);
   V2 s;
   s.x = 0;
@@ -261,50 +270,53 @@ __global__ void __launch_bounds__(256) k_splitk_sum(int M, int N, int nsplit, co
   *c = s;
 }
 
+struct Batch {
+  int count = 1;
+  long long sA = 0, sB = 0, sC = 0;  // element strides between consecutive products
+};
+
 template <int MODE, int NB16>
-void launch_nb(Context& c, dim3 grid, int M, int N, int K, int kchunk, const void* A, int lda, const void* B, int ldb,
-               void* C, int ldc, int acc, void* part) {
+void launch_nb(Context& c, dim3 grid, int M, int N, int K, int kchunk, int nsplit, const Batch& bt, const void* A,
+               int lda, const void* B, int ldb, void* C, int ldc, int acc, void* part) {
   using V2 = typename Traits<MODE>::V2;
-  k_zgemm_mfma<MODE, NB16><<<grid, 256, 0, c.stream>>>(M, N, K, kchunk, (const V2*)A, lda, (const V2*)B, ldb,
-                                                        (V2*)C, ldc, acc, (V2*)part);
+  k_zgemm_mfma<MODE, NB16><<<grid, 256, 0, c.stream>>>(M, N, K, kchunk, nsplit, bt.sA, bt.sB, bt.sC, (const V2*)A,
+                                                        lda, (const V2*)B, ldb, (V2*)C, ldc, acc, (V2*)part);
 }
 
 template <int MODE>
 void zgemm_mfma_t(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
-                  bool accumulate) {
+                  bool accumulate, const Batch& bt) {
   using V2 = typename Traits<MODE>::V2;
   int nb16 = std::min(9, (N + 15) / 16);
   int BN = 16 * nb16;
   int mt = (M + BMC - 1) / BMC, nt = (N + BN - 1) / BN;
-  int tiles = mt * nt;
+  long tiles = (long)mt * nt * bt.count;
   int target = 2 * (c.cu_count > 0 ? c.cu_count : 256);
   int nsplit = 1;
-  if (tiles < target) nsplit = std::min((target + tiles - 1) / tiles, std::max(1, K / (4 * BKC)));
+  if (tiles < target) nsplit = (int)std::min<long>((target + tiles - 1) / tiles, std::max(1, K / (4 * BKC)));
   int kchunk = (K + nsplit - 1) / nsplit;
   kchunk = ((kchunk + BKC - 1) / BKC) * BKC;
   nsplit = (K + kchunk - 1) / kchunk;
-  dim3 grid((unsigned)mt, (unsigned)nt, (unsigned)nsplit);
+  PWK_REQUIRE((long)nsplit * bt.count <= 65535, PWK_E_UNSUPPORTED, "GEMM batch too large");
+  dim3 grid((unsigned)mt, (unsigned)nt, (unsigned)(nsplit * bt.count));
   void* part = nullptr;
   if (nsplit > 1) {
-    c.gemm_ws.ensure((size_t)nsplit * M * N * sizeof(V2));
+    c.gemm_ws.ensure((size_t)bt.count * nsplit * M * N * sizeof(V2));
     part = c.gemm_ws.p;
   }
   int acc = (nsplit == 1 && accumulate) ? 1 : 0;
   switch (nb16) {
-    case 1: launch_nb<MODE, 1>(c, grid, M, N, K, kchunk, A, lda, B, ldb, C, ldc, acc, part); break;
-    case 2: launch_nb<MODE, 2>(c, grid, M, N, K, kchunk, A, lda, B, ldb, C, ldc, acc, part); break;
-    case 3: launch_nb<MODE, 3>(c, grid, M, N, K, kchunk, A, lda, B, ldb, C, ldc, acc, part); break;
-    case 4: launch_nb<MODE, 4>(c, grid, M, N, K, kchunk, A, lda, B, ldb, C, ldc, acc, part); break;
-    case 5: launch_nb<MODE, 5>(c, grid, M, N, K, kchunk, A, lda, B, ldb, C, ldc, acc, part); break;
-    case 6: launch_nb<MODE, 6>(c, grid, M, N, K, kchunk, A, lda, B, ldb, C, ldc, acc, part); break;
-    case 7: launch_nb<MODE, 7>(c, grid, M, N, K, kchunk, A, lda, B, ldb, C, ldc, acc, part); break;
-    case 8: launch_nb<MODE, 8>(c, grid, M, N, K, kchunk, A, lda, B, ldb, C, ldc, acc, part); break;
-    default: launch_nb<MODE, 9>(c, grid, M, N, K, kchunk, A, lda, B, ldb, C, ldc, acc, part); break;
+#define PWK_NB(X) \
+  case X: launch_nb<MODE, X>(c, grid, M, N, K, kchunk, nsplit, bt, A, lda, B, ldb, C, ldc, acc, part); break;
+    PWK_NB(1) PWK_NB(2) PWK_NB(3) PWK_NB(4) PWK_NB(5) PWK_NB(6) PWK_NB(7) PWK_NB(8)
+#undef PWK_NB
+    default: launch_nb<MODE, 9>(c, grid, M, N, K, kchunk, nsplit, bt, A, lda, B, ldb, C, ldc, acc, part); break;
   }
   PWK_HIP(hipGetLastError());
   if (nsplit > 1) {
-    k_splitk_sum<V2><<<blocks_for((size_t)M * N, 256), 256, 0, c.stream>>>(M, N, nsplit, (const V2*)part, (V2*)C,
-                                                                           ldc, accumulate ? 1 : 0);
+    dim3 g(blocks_for((size_t)M * N, 256), (unsigned)bt.count);
+    k_splitk_sum<V2><<<g, 256, 0, c.stream>>>(M, N, nsplit, (const V2*)part, (V2*)C, ldc, bt.sC,
+                                              accumulate ? 1 : 0);
     PWK_HIP(hipGetLastError());
   }
 }
@@ -313,17 +325,28 @@ void zgemm_mfma_t(Context& c, int M, int N, int K, const void* A, int lda, const
 
 void launch_zgemm_mfma(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C,
                        int ldc, bool accumulate, int precision) {
-  if (M <= 0 || N <= 0) return;
+  launch_zgemm_mfma_batched(c, M, N, K, A, lda, 0, B, ldb, 0, C, ldc, 0, 1, accumulate, precision);
+}
+
+void launch_zgemm_mfma_batched(Context& c, int M, int N, int K, const void* A, int lda, long long sA, const void* B,
+                               int ldb, long long sB, void* C, int ldc, long long sC, int batch, bool accumulate,
+                               int precision) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
   if (K <= 0) {
     PWK_REQUIRE(accumulate, PWK_E_INVALID, "zgemm with K = 0 and no accumulation");
     return;
   }
+  Batch bt;
+  bt.count = batch;
+  bt.sA = sA;
+  bt.sB = sB;
+  bt.sC = sC;
   if (precision == PWK_FP64)
-    zgemm_mfma_t<MODE_F64>(c, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+    zgemm_mfma_t<MODE_F64>(c, M, N, K, A, lda, B, ldb, C, ldc, accumulate, bt);
   else if (precision == PWK_SPLIT_BF16)
-    zgemm_mfma_t<MODE_BF16X3>(c, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+    zgemm_mfma_t<MODE_BF16X3>(c, M, N, K, A, lda, B, ldb, C, ldc, accumulate, bt);
   else
-    zgemm_mfma_t<MODE_F32>(c, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+    zgemm_mfma_t<MODE_F32>(c, M, N, K, A, lda, B, ldb, C, ldc, accumulate, bt);
 }
 
 }  // namespace pwk

@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(64) k_zfwd(const typename V2t<T>::t* __restric
                                              typename V2t<T>::t* __restrict__ hpsi, const T* __restrict__ kin,
                                              int npw, int S, const int* __restrict__ st_pw0,
                                              const int* __restrict__ st_cnt, const int* __restrict__ st_lo,
-                                             T inv_n) {
+                                             T inv_n, int nbk) {
   int s = blockIdx.x * 64 + threadIdx.x;
   int b = blockIdx.y;
   if (s >= S) return;
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(64) k_zfwd(const typename V2t<T>::t* __restric
   fft_dif<T, NZ, -1>(re, im);
   const typename V2t<T>::t* p = psi + (size_t)b * npw + pw0;
   typename V2t<T>::t* h = hpsi + (size_t)b * npw + pw0;
-  const T* t = kin + pw0;
+  const T* t = kin + (size_t)(b / nbk) * npw + pw0;  // bands of k-point b / nbk
 #pragma unroll
   for (int i3 = 0; i3 < NZ; ++i3) {
     int j = stick_slot(i3, lo, NZ);
@@ -263,19 +263,19 @@ __global__ void __launch_bounds__(256) k_vr_reorder(const double2* __restrict__ 
 }
 
 // ---- dispatch ------------------------------------------------------------------
+// Local part of H·ψ for nb bands; band b uses the kinetic diagonal kin + (b / nbk)·npw,
+// so consecutive groups of nbk bands may belong to different k-points.
 template <typename T>
-void hpsi_local(Context& c, int ik, int nb, const void* psi, void* hpsi) {
+void hpsi_local(Context& c, const T* kin, int nb, int nbk, const void* psi, void* hpsi) {
   using V2 = typename V2t<T>::t;
   int nx = c.dims[0], ny = c.dims[1], nz = c.dims[2];
   PWK_REQUIRE(nx == ny, PWK_E_UNSUPPORTED, "H·psi plane kernels need nx == ny");
-  KData& kd = c.k(ik);
-  PWK_REQUIRE(kd.have_kin, PWK_E_STATE, "kinetic diagonal of this k-point not set (pwk_set_kinetic)");
   PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
+  PWK_REQUIRE(nb <= 65535, PWK_E_UNSUPPORTED, "too many bands in one H·psi launch");
   size_t tb = (size_t)nb * nz * c.nsticks * sizeof(V2);
   c.work_a.ensure(tb);
   V2* T_ = c.work_a.as<V2>();
   const T* vr = std::is_same<T, double>::value ? (const T*)c.vr64.p : (const T*)c.vr32.p;
-  const T* kin = std::is_same<T, double>::value ? (const T*)kd.kin64.p : (const T*)kd.kin32.p;
   dim3 gz(blocks_for((size_t)c.nsticks, 64), (unsigned)nb);
   {
     TimedScope ts(c, "hpsi_zinv");
@@ -317,7 +317,7 @@ void hpsi_local(Context& c, int ik, int nb, const void* psi, void* hpsi) {
   case L:                                                                                               \
     k_zfwd<T, L><<<gz, 64, 0, c.stream>>>(T_, (const V2*)psi, (V2*)hpsi, kin, c.npw, c.nsticks,         \
                                           c.st_pw0.as<int>(), c.st_cnt.as<int>(), c.st_n3lo.as<int>(),  \
-                                          inv_n);                                                       \
+                                          inv_n, nbk);                                                  \
     break;
     PWK_FOR_EACH_FFT_SIZE(PWK_CASE)
 #undef PWK_CASE
@@ -443,10 +443,11 @@ void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int pr
   c.check_basis();
   PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
   KData& kd = c.k(ik);
+  PWK_REQUIRE(kd.have_kin, PWK_E_STATE, "kinetic diagonal of this k-point not set (pwk_set_kinetic)");
   if (precision == PWK_FP64) {
-    hpsi_local<double>(c, ik, nb, psi, hpsi);
+    hpsi_local<double>(c, kd.kin64.as<double>(), nb, nb, psi, hpsi);
   } else {
-    hpsi_local<float>(c, ik, nb, psi, hpsi);
+    hpsi_local<float>(c, kd.kin32.as<float>(), nb, nb, psi, hpsi);
   }
   if (kd.nch > 0) {
     // P = (D·B^H)·ψ  (nch × nb), then Hψ += B·P   (src/potential/nonlocal.rs:432-450)
@@ -460,6 +461,31 @@ void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int pr
     }
     TimedScope ts(c, "hpsi_nl_expand");
     launch_zgemm_mfma(c, c.npw, nb, kd.nch, bm, c.npw, c.proj.p, kd.nch, hpsi, c.npw, true, precision);
+  }
+}
+
+// H·ψ of nb bands at each of nk k-points in one pass: psi and hpsi hold nk blocks of
+// npw × nb (band-major), kin nk × npw in the precision of psi, B / D·B^H nk blocks of
+// npw × nch / nch × npw. The local part runs as one launch sequence over nk·nb bands,
+// the projector GEMMs as strided batches.
+void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, int precision, const void* kin,
+                       int nch, const void* B, const void* DBH, void* proj) {
+  c.check_basis();
+  PWK_REQUIRE(nk > 0 && nb > 0, PWK_E_INVALID, "nk and nb must be positive");
+  const long long npw = c.npw;
+  if (precision == PWK_FP64)
+    hpsi_local<double>(c, (const double*)kin, nk * nb, nb, psi, hpsi);
+  else
+    hpsi_local<float>(c, (const float*)kin, nk * nb, nb, psi, hpsi);
+  if (nch > 0) {
+    {
+      TimedScope ts(c, "hpsi_nl_project");
+      launch_zgemm_mfma_batched(c, nch, nb, (int)npw, DBH, nch, (long long)nch * npw, psi, (int)npw, npw * nb, proj,
+                                nch, (long long)nch * nb, nk, false, precision);
+    }
+    TimedScope ts(c, "hpsi_nl_expand");
+    launch_zgemm_mfma_batched(c, (int)npw, nb, nch, B, (int)npw, npw * nch, proj, nch, (long long)nch * nb, hpsi,
+                              (int)npw, npw * nb, nk, true, precision);
   }
 }
 

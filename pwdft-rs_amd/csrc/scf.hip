@@ -509,17 +509,39 @@ struct Solve {
 
   // Chebyshev-filtered subspace iteration state
   bool chebyshev = false;
-  std::vector<double>* ev_host = nullptr;  // nk × nb eigenvalues on the host
+  std::vector<double> ev_sub;              // nk × nbs Ritz values of the filtered subspace
+  std::vector<double>* ev_host = nullptr;  // nk × nb eigenvalues used for occupations
+  int nbs = 0;                             // bands per k held in V (nb, plus a buffer when filtering)
   int fprec = PWK_FP64;                // precision of the filter's H·ψ
   int degree = 10;
   std::vector<double> ub;              // per-k upper bound of the spectrum
   DevBuf F0, F1, F2, HYf, Y64, HY64, YH, sS, sH, sU, sT, sX, sXH, sW, sC, s_ev, s_lam, s_work, lowest;
+  DevBuf kin_all64, kin_all32;              // nk × npw kinetic diagonals
+  DevBuf R64, resid;                        // H·V after Rayleigh-Ritz, residual norms
+  std::vector<double> resid_host;
+  double last_delta = INFINITY;             // density change of the previous SCF iteration
+  int passes_total = 0;
+  DevBuf Ball64, DBHall64, Ball32, DBHall32, projall, coef;
+  int nch_all = 0;
+  std::vector<double> coef_host;
+
+  const void* B_of(int k) {
+    return chebyshev ? (const void*)(Ball64.as<double2>() + (size_t)k * npw * nch_all) : c.k(k).B64.p;
+  }
+  const void* DBH_of(int k) {
+    return chebyshev ? (const void*)(DBHall64.as<double2>() + (size_t)k * npw * nch_all) : c.k(k).DBH64.p;
+  }
+  void hpsi_all(const void* psi, void* hpsi, int nbands, int prec) {
+    const bool f64 = prec == PWK_FP64;
+    launch_hpsi_multi(c, nk, nbands, psi, hpsi, prec, f64 ? kin_all64.p : kin_all32.p, nch_all,
+                      f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, projall.p);
+  }
 
   void setup();
   double ewald();
-  void rayleigh_ritz(int k, const double2* Y, const double2* HY, double2* Vk);
-  void lanczos_bound(int k);
-  void chebyshev_pass(int k);
+  void rayleigh_ritz(const double2* Y, const double2* HY);
+  void lanczos_bounds();
+  double chebyshev_pass(int prec);
   void eig_chebyshev(bool first);
   void veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out, double* vxc_out);
   void run(double* eig_out, double* rho_out);
@@ -547,7 +569,8 @@ void Solve::setup() {
   for (int d = 0; d < 3; ++d) PWK_REQUIRE(dims[d] > 0 && dims[d] <= 1024, PWK_E_INVALID, "FFT grid exceeds 1024");
   nb = std::min(p.n_bands, npw);
   PWK_REQUIRE(nb >= 1, PWK_E_INVALID, "n_bands must be positive");
-  PWK_REQUIRE(heev_supported(npw, nb), PWK_E_UNSUPPORTED, "basis too large for the dense eigensolver");
+  PWK_REQUIRE(p.eigensolver != 0 || heev_supported(npw, nb), PWK_E_UNSUPPORTED,
+              "basis too large for the dense eigensolver");
   // |G|² per grid point (src/scf/context.rs:146-156)
   c.dims[0] = dims[0];
   c.dims[1] = dims[1];
@@ -728,6 +751,9 @@ void Solve::setup() {
     gflat[(size_t)3 * i + 1] = gvec[(size_t)i].y;
     gflat[(size_t)3 * i + 2] = gvec[(size_t)i].z;
   }
+  chebyshev = p.eigensolver == 1 || (p.eigensolver < 0 && !(npw <= 1500 && heev_supported(npw, nb)));
+  kin_all64.ensure((size_t)nk * npw * 8);
+  kin_all32.ensure((size_t)nk * npw * 4);
   for (int k = 0; k < nk; ++k) {
     const Vec3 kc = kpts[(size_t)k].cart;
     for (int i = 0; i < npw; ++i) tk[(size_t)i] = kHbar2Over2m * vnorm2(vadd(kc, gvec[(size_t)i]));
@@ -736,11 +762,39 @@ void Solve::setup() {
     kd.kin32.ensure((size_t)npw * 4);
     PWK_HIP(hipMemcpyAsync(kd.kin64.p, tk.data(), (size_t)npw * 8, hipMemcpyHostToDevice, c.stream));
     launch_convert(c, kd.kin64.p, kd.kin32.p, (size_t)npw, true);
+    PWK_HIP(hipMemcpyAsync(kin_all64.as<double>() + (size_t)k * npw, kd.kin64.p, (size_t)npw * 8,
+                           hipMemcpyDeviceToDevice, c.stream));
+    PWK_HIP(hipMemcpyAsync(kin_all32.as<float>() + (size_t)k * npw, kd.kin32.p, (size_t)npw * 4,
+                           hipMemcpyDeviceToDevice, c.stream));
     kd.have_kin = true;
     PWK_HIP(hipStreamSynchronize(c.stream));  // tk is reused
     const double kv[3] = {kc.x, kc.y, kc.z};
     build_nonlocal_device(c, k, kv, gflat.data(), cr.natoms, tau.data(), spi.data(), cr.nspecies, nproj.data(),
                           pl.data(), dij.data(), mesh.data(), rr.data(), rab.data(), chi.data(), omega);
+    if (chebyshev) {
+      // projectors of all k-points in contiguous blocks for the batched GEMMs
+      if (k == 0) {
+        nch_all = kd.nch;
+        const size_t e = (size_t)nk * npw * std::max(1, nch_all);
+        Ball64.ensure(e * 16);
+        DBHall64.ensure(e * 16);
+        Ball32.ensure(e * 8);
+        DBHall32.ensure(e * 8);
+      }
+      PWK_REQUIRE(kd.nch == nch_all, PWK_E_STATE, "projector count differs between k-points");
+      const size_t e = (size_t)npw * nch_all, o = (size_t)k * e;
+      if (nch_all > 0) {
+        PWK_HIP(hipMemcpyAsync(Ball64.as<double2>() + o, kd.B64.p, e * 16, hipMemcpyDeviceToDevice, c.stream));
+        PWK_HIP(hipMemcpyAsync(DBHall64.as<double2>() + o, kd.DBH64.p, e * 16, hipMemcpyDeviceToDevice, c.stream));
+        PWK_HIP(hipMemcpyAsync(Ball32.as<float2>() + o, kd.B32.p, e * 8, hipMemcpyDeviceToDevice, c.stream));
+        PWK_HIP(hipMemcpyAsync(DBHall32.as<float2>() + o, kd.DBH32.p, e * 8, hipMemcpyDeviceToDevice, c.stream));
+        PWK_HIP(hipStreamSynchronize(c.stream));
+      }
+      kd.B64.release();
+      kd.DBH64.release();
+      kd.B32.release();
+      kd.DBH32.release();
+    }
   }
   // symmetry tables for the density
   std::vector<int> rot;
@@ -753,39 +807,46 @@ void Solve::setup() {
   sym_tau.ensure(tr.size() * 8);
   PWK_HIP(hipMemcpyAsync(sym_rot.p, rot.data(), rot.size() * 4, hipMemcpyHostToDevice, c.stream));
   PWK_HIP(hipMemcpyAsync(sym_tau.p, tr.data(), tr.size() * 8, hipMemcpyHostToDevice, c.stream));
-  // eigen-system buffers
-  chebyshev = p.eigensolver == 1 || (p.eigensolver < 0 && !(npw <= 1500 && heev_supported(npw, nb)));
+  // eigen-system buffers; the filtered solver carries buffer bands above the nb used for
+  // occupations so that the filter's damped interval starts well above them
+  nbs = chebyshev ? std::min(npw, nb + std::max(8, nb / 4)) : nb;
   W.ensure((size_t)nk * nb * 8);
-  V.ensure((size_t)nk * npw * nb * 16);
-  if (p.precision != PWK_FP64) V32.ensure((size_t)nk * npw * nb * 8);
+  V.ensure((size_t)nk * npw * nbs * 16);
+  if (p.precision != PWK_FP64) V32.ensure((size_t)nk * npw * nbs * 8);
   if (!chebyshev) {
     Hb.ensure((size_t)nk * npw * npw * 16);
     c.eig_work.ensure(heev_work_doubles(npw, nb, nk) * 8);
   } else {
-    PWK_REQUIRE(heev_supported(nb, nb), PWK_E_UNSUPPORTED, "too many bands for the subspace eigensolver");
+    PWK_REQUIRE(heev_supported(nbs, nbs), PWK_E_UNSUPPORTED, "too many bands for the subspace eigensolver");
     fprec = p.precision;
     degree = p.cheb_degree > 0 ? p.cheb_degree : 10;
-    const size_t blk = (size_t)npw * nb;
-    const size_t fsz = fprec == PWK_FP64 ? 16 : 8;
-    for (DevBuf* b : {&F0, &F1, &F2, &HYf}) b->ensure(blk * fsz);
-    for (DevBuf* b : {&Y64, &HY64, &YH}) b->ensure(blk * 16);
-    for (DevBuf* b : {&sS, &sH, &sU, &sT, &sX, &sXH, &sW, &sC}) b->ensure((size_t)nb * nb * 16);
-    s_ev.ensure((size_t)nb * 8);
-    s_lam.ensure((size_t)nb * 8);
-    s_work.ensure(heev_work_doubles(nb, nb, 1) * 8);
+    const size_t blk = (size_t)nk * npw * nbs;
+    // FP64-sized filter buffers: a filtered solve switches its H·ψ to FP64 once the residual
+    // target falls below what FP32 resolves
+    for (DevBuf* b : {&F0, &F1, &F2, &HYf}) b->ensure(blk * 16);
+    for (DevBuf* b : {&Y64, &HY64, &YH, &R64}) b->ensure(blk * 16);
+    resid.ensure((size_t)nk * nbs * 8);
+    resid_host.assign((size_t)nk * nbs, 0.0);
+    for (DevBuf* b : {&sS, &sH, &sU, &sT, &sX, &sXH, &sW, &sC}) b->ensure((size_t)nk * nbs * nbs * 16);
+    s_ev.ensure((size_t)nk * nbs * 8);
+    s_lam.ensure((size_t)nk * nbs * 8);
+    s_work.ensure(heev_work_doubles(nbs, nbs, nk) * 8);
+    projall.ensure((size_t)nk * std::max(1, nch_all) * nbs * 16);
+    ev_sub.assign((size_t)nk * nbs, 0.0);
+    coef.ensure((size_t)std::max(degree, 1) * nk * 3 * 8);
     ub.assign((size_t)nk, 0.0);
     // start block: the nb lowest-kinetic plane waves of each k (plus a seeded perturbation)
-    lowest.ensure((size_t)nb * nk * 4);
-    std::vector<int> low((size_t)nb * nk);
+    lowest.ensure((size_t)nbs * nk * 4);
+    std::vector<int> low((size_t)nbs * nk);
     for (int k = 0; k < nk; ++k) {
       std::vector<std::pair<double, int>> t((size_t)npw);
       for (int i = 0; i < npw; ++i) t[(size_t)i] = {vnorm2(vadd(kpts[(size_t)k].cart, gvec[(size_t)i])), i};
-      std::partial_sort(t.begin(), t.begin() + nb, t.end());
-      for (int j = 0; j < nb; ++j) low[(size_t)k * nb + j] = t[(size_t)j].second;
+      std::partial_sort(t.begin(), t.begin() + nbs, t.end());
+      for (int j = 0; j < nbs; ++j) low[(size_t)k * nbs + j] = t[(size_t)j].second;
     }
     PWK_HIP(hipMemcpyAsync(lowest.p, low.data(), low.size() * 4, hipMemcpyHostToDevice, c.stream));
-    for (int k = 0; k < nk; ++k)
-      launch_start_block(c, npw, nb, lowest.as<int>() + (size_t)k * nb, V.as<double2>() + (size_t)k * npw * nb);
+    launch_start_block(c, npw, nbs, nk, lowest.as<int>(), V.p);
+    PWK_HIP(hipStreamSynchronize(c.stream));
   }
   PWK_HIP(hipStreamSynchronize(c.stream));
 }
@@ -854,136 +915,185 @@ void Solve::veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_
   launch_hartree_veff(c, rho_g, 4.0 * kPiC * kE2, vloc_g.p, vxc_g.p, veff_g.p, PWK_FP64, true, true, true);
 }
 
-// Rayleigh-Ritz on span(Y) in FP64 with Löwdin orthonormalisation:
+// Rayleigh-Ritz on span(Y_k) for every k at once, FP64, Löwdin orthonormalisation:
 // S = Y^H Y = U s U^H, X = U s^{-1/2}, H' = X^H (Y^H HY) X = W λ W^H, V = Y·X·W.
-void Solve::rayleigh_ritz(int k, const double2* Y, const double2* HY, double2* Vk) {
+void Solve::rayleigh_ritz(const double2* Y, const double2* HY) {
   TimedScope ts(c, "scf_rayleigh_ritz");
-  const int m = nb;
-  launch_conj_transpose(c, npw, m, Y, npw, YH.p, m);
-  launch_zgemm_mfma(c, m, m, npw, YH.p, m, Y, npw, sS.p, m, false, PWK_FP64);
-  launch_zgemm_mfma(c, m, m, npw, YH.p, m, HY, npw, sH.p, m, false, PWK_FP64);
-  launch_hermitize(c, m, sS.p);
-  launch_hermitize(c, m, sH.p);
-  launch_heev_batched(c, m, m, 1, sS.p, s_work.p, s_ev.as<double>(), sU.p, nullptr);
-  launch_lowdin_scale(c, m, s_ev.as<double>(), 1e-14, sU.p);  // sU is now X
-  launch_zgemm_mfma(c, m, m, m, sH.p, m, sU.p, m, sT.p, m, false, PWK_FP64);
-  launch_conj_transpose(c, m, m, sU.p, m, sXH.p, m);
-  launch_zgemm_mfma(c, m, m, m, sXH.p, m, sT.p, m, sX.p, m, false, PWK_FP64);
-  launch_hermitize(c, m, sX.p);
-  launch_heev_batched(c, m, m, 1, sX.p, s_work.p, s_lam.as<double>(), sW.p, nullptr);
-  launch_zgemm_mfma(c, m, m, m, sU.p, m, sW.p, m, sC.p, m, false, PWK_FP64);
-  launch_zgemm_mfma(c, npw, m, m, Y, npw, sC.p, m, Vk, npw, false, PWK_FP64);
-  PWK_HIP(hipMemcpyAsync(ev_host->data() + (size_t)k * nb, s_lam.p, (size_t)nb * 8, hipMemcpyDeviceToHost,
-                         c.stream));
-}
-
-// Upper bound of the spectrum of H_k: ten Lanczos steps from a seeded vector;
-// the Gershgorin bound of the Lanczos matrix plus the last off-diagonal.
-void Solve::lanczos_bound(int k) {
-  const int steps = std::min(10, npw);
-  std::vector<int> none(1, -1);
-  DevBuf dn;
-  dn.ensure(4);
-  PWK_HIP(hipMemcpyAsync(dn.p, none.data(), 4, hipMemcpyHostToDevice, c.stream));
-  // Lanczos runs in FP64 on the first column of the (FP64-sized) filter buffers
-  DevBuf a, b0, b1;
-  a.ensure((size_t)npw * 16);
-  b0.ensure((size_t)npw * 16);
-  b1.ensure((size_t)npw * 16);
-  double2* v0 = a.as<double2>();
-  double2* v1 = b0.as<double2>();
-  double2* w = b1.as<double2>();
-  launch_start_block(c, npw, 1, dn.as<int>(), v1);
-  PWK_HIP(hipMemsetAsync(v0, 0, (size_t)npw * 16, c.stream));
-  const size_t n2 = (size_t)npw * 2;
-  dots({DotTerm{(const double*)v1, nullptr, (const double*)v1, nullptr, 0}}, n2);
-  k_scale<<<gb(n2), 256, 0, c.stream>>>(n2, (double*)v1, 1.0 / std::sqrt(host_small[0]));
-  PWK_HIP(hipGetLastError());
-  std::vector<double> al, be;
-  double beta = 0.0;
-  for (int j = 0; j < steps; ++j) {
-    launch_hpsi(c, k, 1, v1, w, PWK_FP64);
-    dots({DotTerm{(const double*)v1, nullptr, (const double*)w, nullptr, 0}}, n2);
-    const double alpha = host_small[0];
-    launch_cheb_step(c, (size_t)npw, w, v1, v0, 1.0, alpha, beta, w, PWK_FP64);
-    dots({DotTerm{(const double*)w, nullptr, (const double*)w, nullptr, 0}}, n2);
-    al.push_back(alpha);
-    beta = std::sqrt(std::max(host_small[0], 0.0));
-    be.push_back(beta);
-    if (beta < 1e-12) break;
-    std::swap(v0, v1);  // v0 <- v_j
-    k_scale<<<gb(n2), 256, 0, c.stream>>>(n2, (double*)w, 1.0 / beta);
-    PWK_HIP(hipGetLastError());
-    std::swap(v1, w);  // v1 <- v_{j+1}; w is free again
-  }
-  double bound = -INFINITY;
-  for (size_t i = 0; i < al.size(); ++i) {
-    const double r = std::fabs(i > 0 ? be[i - 1] : 0.0) + std::fabs(i + 1 < al.size() ? be[i] : 0.0);
-    bound = std::max(bound, al[i] + r);
-  }
-  ub[(size_t)k] = bound + std::fabs(be.back());
+  const int m = nbs;
+  const long long blk = (long long)npw * m, sm = (long long)m * m;
+  launch_conj_transpose(c, npw, m, Y, npw, blk, YH.p, m, blk, nk);
+  launch_zgemm_mfma_batched(c, m, m, npw, YH.p, m, blk, Y, npw, blk, sS.p, m, sm, nk, false, PWK_FP64);
+  launch_zgemm_mfma_batched(c, m, m, npw, YH.p, m, blk, HY, npw, blk, sH.p, m, sm, nk, false, PWK_FP64);
+  launch_hermitize(c, m, sS.p, nk);
+  launch_hermitize(c, m, sH.p, nk);
+  launch_heev_batched(c, m, m, nk, sS.p, s_work.p, s_ev.as<double>(), sU.p, nullptr);
+  launch_lowdin_scale(c, m, s_ev.as<double>(), 1e-14, sU.p, nk);  // sU is now X
+  launch_zgemm_mfma_batched(c, m, m, m, sH.p, m, sm, sU.p, m, sm, sT.p, m, sm, nk, false, PWK_FP64);
+  launch_conj_transpose(c, m, m, sU.p, m, sm, sXH.p, m, sm, nk);
+  launch_zgemm_mfma_batched(c, m, m, m, sXH.p, m, sm, sT.p, m, sm, sX.p, m, sm, nk, false, PWK_FP64);
+  launch_hermitize(c, m, sX.p, nk);
+  launch_heev_batched(c, m, m, nk, sX.p, s_work.p, s_lam.as<double>(), sW.p, nullptr);
+  launch_zgemm_mfma_batched(c, m, m, m, sU.p, m, sm, sW.p, m, sm, sC.p, m, sm, nk, false, PWK_FP64);
+  launch_zgemm_mfma_batched(c, npw, m, m, Y, npw, blk, sC.p, m, sm, V.p, npw, blk, nk, false, PWK_FP64);
+  PWK_HIP(hipMemcpyAsync(ev_sub.data(), s_lam.p, (size_t)nk * nbs * 8, hipMemcpyDeviceToHost, c.stream));
   PWK_HIP(hipStreamSynchronize(c.stream));
 }
 
-// One filtered pass on k: Y = p(H)·V with the scaled Chebyshev recurrence on
-// [a, b] (a = highest Ritz value kept, b = spectrum bound, scaled at the lowest
-// Ritz value), then Rayleigh-Ritz. The filter runs in the solve's precision,
-// the Rayleigh-Ritz step in FP64.
-void Solve::chebyshev_pass(int k) {
+// Upper bound of the spectrum of every H_k: ten Lanczos steps from a seeded
+// vector per k (all k in one batch); the Gershgorin bound of each Lanczos matrix
+// plus its last off-diagonal element.
+void Solve::lanczos_bounds() {
+  TimedScope ts(c, "scf_lanczos");
+  const int steps = std::min(10, npw);
+  const size_t per = (size_t)npw * 2;  // reals per k
+  DevBuf a, b0, b1, none, red, cf;
+  a.ensure((size_t)nk * npw * 16);
+  b0.ensure((size_t)nk * npw * 16);
+  b1.ensure((size_t)nk * npw * 16);
+  none.ensure((size_t)nk * 4);
+  red.ensure((size_t)nk * 8);
+  cf.ensure((size_t)nk * 3 * 8);
+  double2* v0 = a.as<double2>();
+  double2* v1 = b0.as<double2>();
+  double2* w = b1.as<double2>();
+  const std::vector<int> minus((size_t)nk, -1);
+  PWK_HIP(hipMemcpyAsync(none.p, minus.data(), minus.size() * 4, hipMemcpyHostToDevice, c.stream));
+  launch_start_block(c, npw, 1, nk, none.as<int>(), v1);
+  PWK_HIP(hipMemsetAsync(v0, 0, (size_t)nk * npw * 16, c.stream));
+  launch_block_dot(c, per, nk, (const double*)v1, (const double*)v1, red.as<double>());
+  launch_block_scale(c, per, nk, red.as<double>(), 1, (double*)v1);
+  std::vector<std::vector<double>> al((size_t)nk), be((size_t)nk);
+  std::vector<double> hv((size_t)nk), cfh((size_t)nk * 3), beta((size_t)nk, 0.0), inv((size_t)nk);
+  for (int j = 0; j < steps; ++j) {
+    hpsi_all(v1, w, 1, PWK_FP64);
+    launch_block_dot(c, per, nk, (const double*)v1, (const double*)w, red.as<double>());
+    PWK_HIP(hipMemcpyAsync(hv.data(), red.p, (size_t)nk * 8, hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < nk; ++k) {
+      al[(size_t)k].push_back(hv[(size_t)k]);
+      cfh[(size_t)3 * k] = 1.0;
+      cfh[(size_t)3 * k + 1] = hv[(size_t)k];
+      cfh[(size_t)3 * k + 2] = beta[(size_t)k];
+    }
+    PWK_HIP(hipMemcpyAsync(cf.p, cfh.data(), cfh.size() * 8, hipMemcpyHostToDevice, c.stream));
+    launch_cheb_step(c, (size_t)nk * npw, (size_t)npw, w, v1, v0, cf.as<double>(), w, PWK_FP64);
+    launch_block_dot(c, per, nk, (const double*)w, (const double*)w, red.as<double>());
+    PWK_HIP(hipMemcpyAsync(hv.data(), red.p, (size_t)nk * 8, hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipStreamSynchronize(c.stream));
+    bool stop = false;
+    for (int k = 0; k < nk; ++k) {
+      beta[(size_t)k] = std::sqrt(std::max(hv[(size_t)k], 0.0));
+      be[(size_t)k].push_back(beta[(size_t)k]);
+      inv[(size_t)k] = beta[(size_t)k] > 1e-300 ? 1.0 / beta[(size_t)k] : 0.0;
+      stop |= beta[(size_t)k] < 1e-12;
+    }
+    if (stop || j + 1 == steps) break;
+    std::swap(v0, v1);  // v0 <- v_j
+    PWK_HIP(hipMemcpyAsync(red.p, inv.data(), (size_t)nk * 8, hipMemcpyHostToDevice, c.stream));
+    launch_block_scale(c, per, nk, red.as<double>(), 0, (double*)w);
+    std::swap(v1, w);  // v1 <- v_{j+1}
+    PWK_HIP(hipStreamSynchronize(c.stream));  // inv is host memory reused next step
+  }
+  for (int k = 0; k < nk; ++k) {
+    const auto& A = al[(size_t)k];
+    const auto& Bv = be[(size_t)k];
+    double bound = -INFINITY;
+    for (size_t i = 0; i < A.size(); ++i) {
+      const double r = std::fabs(i > 0 ? Bv[i - 1] : 0.0) + std::fabs(i + 1 < A.size() ? Bv[i] : 0.0);
+      bound = std::max(bound, A[i] + r);
+    }
+    ub[(size_t)k] = bound + std::fabs(Bv.back());
+  }
+  PWK_HIP(hipStreamSynchronize(c.stream));
+}
+
+// One filtered pass over all k: Y = p_k(H_k)·V_k with the scaled Chebyshev
+// recurrence on [a_k, b_k] (a_k = highest Ritz value kept, b_k = spectrum bound,
+// scaled at the lowest Ritz value), then Rayleigh-Ritz. The filter's H·ψ runs in
+// the solve's precision, the Rayleigh-Ritz step in FP64.
+double Solve::chebyshev_pass(int fprec) {
   TimedScope ts(c, "scf_chebyshev_filter");
-  const size_t blk = (size_t)npw * nb;
-  double2* Vk = V.as<double2>() + (size_t)k * blk;
-  const double* lam = ev_host->data() + (size_t)k * nb;
-  const double a0 = lam[0];
-  double a = lam[nb - 1], b = ub[(size_t)k];
-  if (!(b > a + 1e-6)) b = a + std::max(1.0, std::fabs(a));
-  if (!(a > a0)) a = a0 + 1e-3 * (b - a0);
-  const double e = 0.5 * (b - a), cc = 0.5 * (b + a);
-  double sigma = e / (a0 - cc);
-  const double tau = 2.0 / sigma;
+  ++passes_total;
+  const size_t blk = (size_t)nk * npw * nbs;
+  coef_host.assign((size_t)degree * nk * 3, 0.0);
+  for (int k = 0; k < nk; ++k) {
+    const double* lam = ev_sub.data() + (size_t)k * nbs;
+    const double a0 = lam[0];
+    double a = lam[nbs - 1], b = ub[(size_t)k];
+    if (!(b > a + 1e-6)) b = a + std::max(1.0, std::fabs(a));
+    if (!(a > a0)) a = a0 + 1e-3 * (b - a0);
+    const double e = 0.5 * (b - a), cc = 0.5 * (b + a);
+    double sigma = e / (a0 - cc);
+    const double tau = 2.0 / sigma;
+    coef_host[(size_t)3 * k] = sigma / e;
+    coef_host[(size_t)3 * k + 1] = cc;
+    coef_host[(size_t)3 * k + 2] = 0.0;
+    for (int i = 1; i < degree; ++i) {
+      const double sn = 1.0 / (tau - sigma);
+      double* cf = &coef_host[((size_t)i * nk + k) * 3];
+      cf[0] = 2.0 * sn / e;
+      cf[1] = cc;
+      cf[2] = sigma * sn;
+      sigma = sn;
+    }
+  }
+  PWK_HIP(hipMemcpyAsync(coef.p, coef_host.data(), coef_host.size() * 8, hipMemcpyHostToDevice, c.stream));
   void* buf[3] = {F0.p, F1.p, F2.p};
   if (fprec == PWK_FP64)
-    PWK_HIP(hipMemcpyAsync(buf[0], Vk, blk * 16, hipMemcpyDeviceToDevice, c.stream));
+    PWK_HIP(hipMemcpyAsync(buf[0], V.p, blk * 16, hipMemcpyDeviceToDevice, c.stream));
   else
-    launch_convert(c, Vk, buf[0], blk * 2, true);
+    launch_convert(c, V.p, buf[0], blk * 2, true);
   void *X = buf[0], *Y = buf[1], *Z = buf[2];
-  launch_hpsi(c, k, nb, X, HYf.p, fprec);
-  launch_cheb_step(c, blk, HYf.p, X, nullptr, sigma / e, cc, 0.0, Y, fprec);
-  for (int i = 2; i <= degree; ++i) {
-    const double sn = 1.0 / (tau - sigma);
-    launch_hpsi(c, k, nb, Y, HYf.p, fprec);
-    launch_cheb_step(c, blk, HYf.p, Y, X, 2.0 * sn / e, cc, sigma * sn, Z, fprec);
+  const size_t per = (size_t)npw * nbs;
+  hpsi_all(X, HYf.p, nbs, fprec);
+  launch_cheb_step(c, blk, per, HYf.p, X, nullptr, coef.as<double>(), Y, fprec);
+  for (int i = 1; i < degree; ++i) {
+    hpsi_all(Y, HYf.p, nbs, fprec);
+    launch_cheb_step(c, blk, per, HYf.p, Y, X, coef.as<double>() + (size_t)i * nk * 3, Z, fprec);
     void* t = X;
     X = Y;
     Y = Z;
     Z = t;
-    sigma = sn;
   }
   if (fprec == PWK_FP64)
     PWK_HIP(hipMemcpyAsync(Y64.p, Y, blk * 16, hipMemcpyDeviceToDevice, c.stream));
   else
     launch_convert(c, Y, Y64.p, blk * 2, false);
-  launch_hpsi(c, k, nb, Y64.p, HY64.p, PWK_FP64);
-  rayleigh_ritz(k, Y64.as<double2>(), HY64.as<double2>(), Vk);
+  hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
+  rayleigh_ritz(Y64.as<double2>(), HY64.as<double2>());
+  // residuals of the nb bands used for occupations: H·V = (H·Y)·C
+  const long long bl = (long long)npw * nbs, sm = (long long)nbs * nbs;
+  launch_zgemm_mfma_batched(c, npw, nbs, nbs, HY64.p, npw, bl, sC.p, nbs, sm, R64.p, npw, bl, nk, false, PWK_FP64);
+  launch_resid_norms(c, npw, nbs, nb, nk, V.p, R64.p, s_lam.as<double>(), resid.as<double>());
+  PWK_HIP(hipMemcpyAsync(resid_host.data(), resid.p, resid_host.size() * 8, hipMemcpyDeviceToHost, c.stream));
+  PWK_HIP(hipStreamSynchronize(c.stream));
+  double worst = 0.0;
+  for (int k = 0; k < nk; ++k)
+    for (int b = 0; b < nb; ++b) worst = std::max(worst, resid_host[(size_t)k * nbs + b]);
+  return worst;
 }
 
 void Solve::eig_chebyshev(bool first) {
-  const size_t blk = (size_t)npw * nb;
   if (first) {
-    for (int k = 0; k < nk; ++k) {
-      lanczos_bound(k);
-      double2* Vk = V.as<double2>() + (size_t)k * blk;
-      PWK_HIP(hipMemcpyAsync(Y64.p, Vk, blk * 16, hipMemcpyDeviceToDevice, c.stream));
-      launch_hpsi(c, k, nb, Y64.p, HY64.p, PWK_FP64);
-      rayleigh_ritz(k, Y64.as<double2>(), HY64.as<double2>(), Vk);
-      PWK_HIP(hipStreamSynchronize(c.stream));
-    }
+    lanczos_bounds();
+    PWK_HIP(hipMemcpyAsync(Y64.p, V.p, (size_t)nk * npw * nbs * 16, hipMemcpyDeviceToDevice, c.stream));
+    hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
+    rayleigh_ritz(Y64.as<double2>(), HY64.as<double2>());
   }
-  const int passes = first ? 4 : 1;
-  for (int pass = 0; pass < passes; ++pass) {
-    for (int k = 0; k < nk; ++k) chebyshev_pass(k);
-    PWK_HIP(hipStreamSynchronize(c.stream));
+  // passes until the residual of the occupied-range bands is below a target that tightens with
+  // the SCF: 10% of the previous density change (eV per unit vector), between 1e-9 and 1e-3.
+  // A reduced-precision filter hands over to FP64 below 3e-6, where FP32 noise dominates.
+  const double target = std::max(1e-9, std::min(1e-3, 0.1 * last_delta));
+  const int max_passes = first ? 12 : 6;
+  for (int pass = 0; pass < max_passes; ++pass) {
+    const int prec = (fprec != PWK_FP64 && target < 3e-6) ? PWK_FP64 : fprec;
+    const double worst = chebyshev_pass(prec);
+    if (p.verbose) std::fprintf(stderr, "[pwk scf]   filter pass %d (%s): max residual %.3e (target %.1e)\n", pass,
+                                prec == PWK_FP64 ? "fp64" : "reduced", worst, target);
+    if (worst < target) break;
   }
+  for (int k = 0; k < nk; ++k)
+    for (int b = 0; b < nb; ++b) (*ev_host)[(size_t)k * nb + b] = ev_sub[(size_t)k * nbs + b];
 }
 
 void Solve::run(double* eig_out, double* rho_out) {
@@ -1039,18 +1149,17 @@ void Solve::run(double* eig_out, double* rho_out) {
     for (size_t i = 0; i < occ.size(); ++i) occ[i] = fd_occ(ev[i], ef, p.smearing_sigma) * spin;
     // ---- density (src/scf/density.rs:41-111) ----------------------------------------------
     PWK_HIP(hipMemsetAsync(rho_new.p, 0, N * 8, c.stream));
-    if (p.precision != PWK_FP64) launch_convert(c, V.p, V32.p, (size_t)nk * npw * nb * 2, true);
-    std::vector<std::vector<double>> fw((size_t)nk, std::vector<double>((size_t)nb));
+    if (p.precision != PWK_FP64) launch_convert(c, V.p, V32.p, (size_t)nk * npw * nbs * 2, true);
+    // the basis is the same G-sphere at every k, so all nk·nb bands go through one launch
+    std::vector<double> fw((size_t)nk * nbs, 0.0);  // buffer bands carry no weight
+    for (int k = 0; k < nk; ++k)
+      for (int b = 0; b < nb; ++b) fw[(size_t)k * nbs + b] = occ[(size_t)k * nb + b] * wk[(size_t)k];
     {
       TimedScope ts(c, "scf_density");
-      for (int k = 0; k < nk; ++k) {
-        for (int b = 0; b < nb; ++b) fw[(size_t)k][(size_t)b] = occ[(size_t)k * nb + b] * wk[(size_t)k];
-        if (p.precision == PWK_FP64)
-          launch_density(c, nb, V.as<double2>() + (size_t)k * npw * nb, fw[(size_t)k].data(), rho_new.p, PWK_FP64);
-        else
-          launch_density(c, nb, V32.as<float2>() + (size_t)k * npw * nb, fw[(size_t)k].data(), rho_new.p, PWK_FP32);
-        PWK_HIP(hipStreamSynchronize(c.stream));  // fw staging is host memory
-      }
+      if (p.precision == PWK_FP64)
+        launch_density(c, nk * nbs, V.p, fw.data(), rho_new.p, PWK_FP64);
+      else
+        launch_density(c, nk * nbs, V32.p, fw.data(), rho_new.p, PWK_FP32);
     }
     dots({DotTerm{rho_new.as<double>(), nullptr, nullptr, nullptr, 0}}, N);
     const double integral = host_small[0] * dvol;
@@ -1106,6 +1215,7 @@ void Solve::run(double* eig_out, double* rho_out) {
     e_prev = e_total;
     res.n_iter = iter + 1;
     res.final_delta = delta;
+    last_delta = delta;
     if (p.verbose)
       std::fprintf(stderr, "[pwk scf] iter %3d  E=%.10f  dE=%.3e  delta=%.3e  EF=%.6f\n", iter, e_total, de, delta,
                    ef);
@@ -1128,11 +1238,11 @@ void Solve::run(double* eig_out, double* rho_out) {
       std::vector<double> e2((size_t)2 * nb);
       for (int k = 0; k < nk; ++k) {
         KData& kd = c.k(k);
-        const double2* psi = V.as<double2>() + (size_t)k * npw * nb;
+        const double2* psi = V.as<double2>() + (size_t)k * npw * nbs;
         if (kd.nch > 0) {
           nl_p.ensure((size_t)kd.nch * nb * 16);
-          launch_zgemm_mfma(c, kd.nch, nb, npw, kd.DBH64.p, kd.nch, psi, npw, nl_p.p, kd.nch, false, PWK_FP64);
-          launch_zgemm_mfma(c, npw, nb, kd.nch, kd.B64.p, npw, nl_p.p, kd.nch, nl_x.p, npw, false, PWK_FP64);
+          launch_zgemm_mfma(c, kd.nch, nb, npw, DBH_of(k), kd.nch, psi, npw, nl_p.p, kd.nch, false, PWK_FP64);
+          launch_zgemm_mfma(c, npw, nb, kd.nch, B_of(k), npw, nl_p.p, kd.nch, nl_x.p, npw, false, PWK_FP64);
         } else {
           PWK_HIP(hipMemsetAsync(nl_x.p, 0, (size_t)npw * nb * 16, c.stream));
         }
