@@ -257,7 +257,7 @@ BOHR = 0.529177210903
 SI_FRAC = [[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]]
 
 
-def small_deck(precision):
+def small_deck(precision, eigensolver=0):
     from pwk import scf
 
     a = 10.26 * BOHR
@@ -266,7 +266,7 @@ def small_deck(precision):
     deck = scf.CrystalDeck(lat, SI_FRAC, [0, 0], [os.path.join(ROOT, "tests", "golden", "pseudo", "Si.upf")])
     p = scf.ScfParams(ecut_ev=12 * RY, kgrid=(4, 4, 4), use_symmetry=True, n_bands=8, max_iter=100,
                       conv_threshold=1e-8, energy_threshold=1e-8 * RY, mixing_beta=0.3, mixing_ndim=8,
-                      smearing_sigma=0.01 * RY, precision=precision)
+                      smearing_sigma=0.01 * RY, precision=precision, eigensolver=eigensolver)
     return deck, p
 
 
@@ -297,7 +297,7 @@ def scf_bench(args):
     from pwk import scf
 
     deck64, p64 = small_deck("fp64")
-    deckmx, pmx = small_deck(args.scf_precision)
+    deckmx, pmx = small_deck(args.scf_precision, {"dense": 0, "chebyshev": 1}[args.scf_eigensolver])
     tab = scf.scf_tables(deck64, p64)
     natoms = 2
     acc = pwk.GpuAccelerator(0)
@@ -404,9 +404,11 @@ def scf_bench(args):
                                   for k, v in kern.items()},
         "config": {"workload": "SCF solve: Si2 diamond FCC a=10.26 bohr, ecutwfc 12 Ry (169 PW, 15^3), "
                                "4x4x4 Gamma-centred MP -> 8 IBZ k, 8 bands, FD 0.01 Ry, Anderson beta 0.3 ndim 8, "
-                               "dense eigensolver, conv 1e-8",
+                               "conv 1e-8 e/A^3 and 1e-8 Ry",
                    "npw": tab["npw"], "grid": list(tab["dims"]), "nk": tab["nk"], "bands": 8,
-                   "slots": args.slots, "precision_served": args.scf_precision},
+                   "slots": args.slots, "precision_served": args.scf_precision,
+                   "eigensolver_served": args.scf_eigensolver, "eigensolver_fp64": "dense",
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "0")) or None},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -419,7 +421,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--fp64-solves", type=int, default=5)
-    ap.add_argument("--scf-precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--scf-precision", default="fp32", choices=["fp32", "split_bf16", "fp64"])
+    ap.add_argument("--scf-eigensolver", default="chebyshev", choices=["dense", "chebyshev"],
+                    help="eigensolver of the served form (the plain FP64 form always uses the dense one)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for this process (0: leave the runtime default); at most 32")
     ap.add_argument("--no-hpsi", action="store_true", help="skip the H·psi at scale section")
     ap.add_argument("--hpsi-only", action="store_true")
     ap.add_argument("--hpsi-steps", type=int, default=20)
@@ -438,6 +444,10 @@ def main():
               "it).", file=sys.stderr)
         sys.exit(2)
 
+    if args.hw_queues:
+        if not 1 <= args.hw_queues <= 32:
+            raise SystemExit("--hw-queues must be in 1..32")
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)  # before the HIP runtime starts
     import torch
 
     if not torch.cuda.is_available():

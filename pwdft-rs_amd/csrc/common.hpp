@@ -193,6 +193,7 @@ void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, 
                        int nch, const void* B, const void* DBH, void* proj);
 // dense.hip
 void launch_dense_h(Context& c, int ik, int with_nl, void* h);
+void launch_dense_h_batched(Context& c, int nk, const double* kin, int nch, const void* B, const void* DBH, void* h);
 // gemm.hip: complex C = alpha·op(A)·B + beta·C, column-major, op = N only
 void launch_zgemm(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
                   bool accumulate, int precision);

@@ -7,11 +7,12 @@
 //      kept below the diagonal of the (overwritten) input, the trailing
 //      rank-2 updates are spread over the workgroup.
 //   2. A diagonal unitary D makes the sub-diagonal real: T = D·T_r·D^H.
-//   3. The nb lowest eigenvalues of T_r by Sturm-count multisection, one
-//      wavefront per eigenvalue, 64 probe points per step.
+//   3. The nb lowest eigenvalues of T_r by Sturm-count multisection, lane
+//      groups of one wavefront per eigenvalue, one probe point per lane.
 //   4. Their eigenvectors by inverse iteration (tridiagonal LU with partial
 //      pivoting, one lane per vector, three solves from a fixed pseudo-random
-//      start), then modified Gram-Schmidt inside clusters of close eigenvalues.
+//      start; factors in LDS when they fit), then modified Gram-Schmidt inside
+//      clusters of close eigenvalues.
 //   5. Back-transformation V = Q·D·Z through the reflectors in reverse order.
 // Every loop has a fixed bound; nothing waits on data.
 #include "common.hpp"
@@ -20,8 +21,11 @@ namespace pwk {
 
 namespace {
 
-constexpr int EIG_NT = 256;
-constexpr int EIG_WAVES = EIG_NT / 64;
+// Two workgroup shapes: 16 waves (memory-level parallelism for the latency-bound passes
+// of larger matrices) and one wave for matrices of at most 64 rows, where barriers
+// dominate and a single wave needs none of its reductions in LDS.
+constexpr int EIG_NT = 1024;
+constexpr int EIG_WAVES = EIG_NT / 64;  // LDS is sized for the larger shape
 
 __host__ __device__ inline size_t eig_stride(int n, int nb) {
   // ecx (2n) | dph (2n) | z (n·nb) | LU factors (5·n·nb)
@@ -29,9 +33,12 @@ __host__ __device__ inline size_t eig_stride(int n, int nb) {
   return (s + 1) & ~(size_t)1;
 }
 
+template <int NT>
 __device__ double block_sum(double v, double* red) {
+  constexpr int EIG_WAVES = NT / 64;
   const int t = threadIdx.x;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (NT == 64) return __shfl(v, 0, 64);
   __syncthreads();
   if ((t & 63) == 0) red[t >> 6] = v;
   __syncthreads();
@@ -69,9 +76,12 @@ __device__ double start_entry(int i, int j) {
   return (double)h * (2.0 / 4294967296.0) - 1.0;
 }
 
-__global__ void __launch_bounds__(EIG_NT) k_heev(int n, int nb, double2* __restrict__ Hb, double* __restrict__ work,
-                                                 double* __restrict__ w_out, double2* __restrict__ V_out,
-                                                 int* __restrict__ status) {
+template <int NT>
+__global__ void __launch_bounds__(NT) k_heev(int n, int nb, int a_in_lds, int lu_in_lds, double2* __restrict__ Hb,
+                                             double* __restrict__ work, double* __restrict__ w_out,
+                                             double2* __restrict__ V_out, int* __restrict__ status) {
+  constexpr int EIG_NT = NT;
+  constexpr int EIG_WAVES = NT / 64;
   extern __shared__ unsigned char smem[];
   double2* v = reinterpret_cast<double2*>(smem);      // n
   double2* wv = v + n;                                // n
@@ -80,14 +90,24 @@ __global__ void __launch_bounds__(EIG_NT) k_heev(int n, int nb, double2* __restr
   double* e2 = e + n;                                 // n: e²
   double* lam = e2 + n;                               // nb: eigenvalues
   double* red = lam + nb;                             // EIG_WAVES + 2
+  // EIG_WAVES × 64 matvec partials, 16-byte aligned
+  const size_t pp_off = ((size_t)(reinterpret_cast<unsigned char*>(red + EIG_WAVES + 2) - smem) + 15) & ~(size_t)15;
+  double2* pp = reinterpret_cast<double2*>(smem + pp_off);
+  double2* lds_a = pp + 16 * 64;                                          // n·n when a_in_lds
+  double* lds_lu = reinterpret_cast<double*>(lds_a + (a_in_lds ? (size_t)n * n : 0));  // 6·n·nb when lu_in_lds
   __shared__ double s_bounds[3];
 
   const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
   double2* A = Hb + (size_t)b * n * n;
+  if (a_in_lds) {  // small matrices live in LDS for the whole solve (the input is not written back)
+    for (int i = t; i < n * n; i += EIG_NT) lds_a[i] = A[i];
+    __syncthreads();
+    A = lds_a;
+  }
   double* base = work + (size_t)b * eig_stride(n, nb);
   double2* ecx = reinterpret_cast<double2*>(base);    // n
   double2* dph = ecx + n;                              // n
-  double* Z = reinterpret_cast<double*>(dph + n);      // Z[i*nb + j]
+  double* Z = lu_in_lds ? lds_lu : reinterpret_cast<double*>(dph + n);  // Z[i*nb + j]
   double* U0 = Z + (size_t)n * nb;                     // LU factors, interleaved by vector like Z
   double* U1 = U0 + (size_t)n * nb;
   double* U2 = U1 + (size_t)n * nb;
@@ -100,7 +120,7 @@ __global__ void __launch_bounds__(EIG_NT) k_heev(int n, int nb, double2* __restr
     double2* col = A + (size_t)k * n + k + 1;
     double part = 0.0;
     for (int i = t; i < m; i += EIG_NT) part += col[i].x * col[i].x + col[i].y * col[i].y;
-    const double xn = sqrt(block_sum(part, red));
+    const double xn = sqrt(block_sum<NT>(part, red));
     if (xn == 0.0) {
       for (int i = t; i < m; i += EIG_NT) col[i] = make_double2(0.0, 0.0);  // identity reflector
       if (t == 0) ecx[k] = make_double2(0.0, 0.0);
@@ -122,22 +142,35 @@ __global__ void __launch_bounds__(EIG_NT) k_heev(int n, int nb, double2* __restr
       v[i] = make_double2(x.x / vn, x.y / vn);
     }
     __syncthreads();
-    // p = A_sub v
-    for (int i = t; i < m; i += EIG_NT) {
+    // p = A_sub v: rows in chunks of 64 lanes, the columns split over the waves, then a
+    // fixed-order sum of the per-wave partials
+    for (int i0 = 0; i0 < m; i0 += 64) {
+      const int i = i0 + lane;
       double pr = 0.0, pi = 0.0;
-      const double2* arow = A + (size_t)(k + 1) * n + k + 1 + i;
-      for (int j = 0; j < m; ++j) {
-        const double2 a = arow[(size_t)j * n];
-        const double2 vj = v[j];
-        pr += a.x * vj.x - a.y * vj.y;
-        pi += a.x * vj.y + a.y * vj.x;
+      if (i < m) {
+        const double2* arow = A + (size_t)(k + 1) * n + k + 1 + i;
+        for (int j = wave; j < m; j += EIG_WAVES) {
+          const double2 a = arow[(size_t)j * n];
+          const double2 vj = v[j];
+          pr += a.x * vj.x - a.y * vj.y;
+          pi += a.x * vj.y + a.y * vj.x;
+        }
       }
-      wv[i] = make_double2(pr, pi);
+      pp[wave * 64 + lane] = make_double2(pr, pi);
+      __syncthreads();
+      if (wave == 0 && i < m) {
+        double sr = 0.0, si = 0.0;
+        for (int w = 0; w < EIG_WAVES; ++w) {
+          sr += pp[w * 64 + lane].x;
+          si += pp[w * 64 + lane].y;
+        }
+        wv[i] = make_double2(sr, si);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     part = 0.0;
     for (int i = t; i < m; i += EIG_NT) part += v[i].x * wv[i].x + v[i].y * wv[i].y;  // v^H p (real)
-    const double K = block_sum(part, red);
+    const double K = block_sum<NT>(part, red);
     for (int i = t; i < m; i += EIG_NT) {
       wv[i].x -= K * v[i].x;
       wv[i].y -= K * v[i].y;
@@ -193,22 +226,40 @@ __global__ void __launch_bounds__(EIG_NT) k_heev(int n, int nb, double2* __restr
   const double tnorm = s_bounds[2];
   const double pivmin = 1e-290 * fmax(1.0, tnorm * tnorm);
 
-  // ---- 3. eigenvalues by multisection: one wavefront per eigenvalue --------------------
-  for (int j = wave; j < nb; j += EIG_WAVES) {
-    double a = s_bounds[0], bb = s_bounds[1];
-    for (int it = 0; it < 24; ++it) {
-      if (bb - a <= 2.220446049250313e-16 * fmax(fabs(a), fabs(bb)) + 1e-300) break;
-      const double x = a + (bb - a) * (double)(lane + 1) * (1.0 / 65.0);
-      const int c = sturm_count(n, d, e2, x, pivmin);
-      const unsigned long long le = __ballot(c <= j);
-      // first lane whose probe lies above the j-th eigenvalue
-      const int k = (~le == 0ull) ? 64 : __builtin_ctzll(~le);
-      const double na = k > 0 ? __shfl(x, k - 1, 64) : a;
-      const double nbnd = k < 64 ? __shfl(x, k, 64) : bb;
-      a = na;
-      bb = nbnd;
+  // ---- 3. eigenvalues by multisection: groups of G lanes per eigenvalue, each lane one
+  // probe; the groups of all waves cover as many eigenvalues at once as there are groups
+  {
+    int gpw = 1;  // groups per wave: the smallest power of two with NW·gpw >= nb (at most 64)
+    while (gpw < 64 && gpw * EIG_WAVES < nb) gpw <<= 1;
+    const int G = 64 / gpw, g = lane / G, sub = lane % G;
+    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const int per_round = EIG_WAVES * gpw;
+    for (int j0 = 0; j0 < nb; j0 += per_round) {
+      const int j = j0 + wave * gpw + g;
+      const bool active = j < nb;
+      double a = s_bounds[0], bb = s_bounds[1];
+      bool done = !active;
+      for (int it = 0; it < 64; ++it) {
+        if (__ballot(!done) == 0ull) break;  // wave-uniform exit
+        const double x = a + (bb - a) * ((double)(sub + 1) / (double)(G + 1));
+        const int cnt = done ? 0 : sturm_count(n, d, e2, x, pivmin);
+        const unsigned long long le = (__ballot(cnt <= j) >> (g * G)) & gmask;
+        // first probe of the group above the j-th eigenvalue
+        const int k = (~le & gmask) == 0ull ? G : __builtin_ctzll(~le & gmask);
+        const double na = __shfl(x, g * G + (k > 0 ? k - 1 : 0), 64);
+        const double nbv = __shfl(x, g * G + (k < G ? k : G - 1), 64);
+        if (!done) {
+          const double a2 = k > 0 ? na : a, b2 = k < G ? nbv : bb;
+          // converged to the precision the tridiagonal form carries: eps relative to the
+          // eigenvalue or eps·|T| absolute, whichever is larger
+          done = (a2 == a && b2 == bb) ||
+                 b2 - a2 <= 4.440892098500626e-16 * fmax(fmax(fabs(a2), fabs(b2)), tnorm);
+          a = a2;
+          bb = b2;
+        }
+      }
+      if (active && sub == 0) lam[j] = 0.5 * (a + bb);
     }
-    if (lane == 0) lam[j] = 0.5 * (a + bb);
   }
   __syncthreads();
 
@@ -306,13 +357,13 @@ __global__ void __launch_bounds__(EIG_NT) k_heev(int n, int nb, double2* __restr
     for (int i = first; i < j; ++i) {
       double part = 0.0;
       for (int r = t; r < n; r += EIG_NT) part += Z[(size_t)r * nb + i] * Z[(size_t)r * nb + j];
-      const double dp = block_sum(part, red);
+      const double dp = block_sum<NT>(part, red);
       for (int r = t; r < n; r += EIG_NT) Z[(size_t)r * nb + j] -= dp * Z[(size_t)r * nb + i];
       __syncthreads();
     }
     double part = 0.0;
     for (int r = t; r < n; r += EIG_NT) part += Z[(size_t)r * nb + j] * Z[(size_t)r * nb + j];
-    const double nn = block_sum(part, red);
+    const double nn = block_sum<NT>(part, red);
     const double sc = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
     for (int r = t; r < n; r += EIG_NT) Z[(size_t)r * nb + j] *= sc;
     __syncthreads();
@@ -332,15 +383,20 @@ __global__ void __launch_bounds__(EIG_NT) k_heev(int n, int nb, double2* __restr
   for (int k = n - 3; k >= 0; --k) {
     const int m = n - k - 1;
     const double2* vk = A + (size_t)k * n + k + 1;
-    for (int jj = t; jj < nb; jj += EIG_NT) {
+    // one wave per vector, lanes over the reflector
+    for (int jj = wave; jj < nb; jj += EIG_WAVES) {
       double sr = 0.0, si = 0.0;
       const double2* x = V + (size_t)jj * n + k + 1;
-      for (int i = 0; i < m; ++i) {
+      for (int i = lane; i < m; i += 64) {
         const double2 a = vk[i], xi = x[i];
         sr += a.x * xi.x + a.y * xi.y;  // conj(v)·x
         si += a.x * xi.y - a.y * xi.x;
       }
-      sv[jj] = make_double2(sr, si);
+      for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_down(sr, off, 64);
+        si += __shfl_down(si, off, 64);
+      }
+      if (lane == 0) sv[jj] = make_double2(sr, si);
     }
     __syncthreads();
     for (int idx = t; idx < m * nb; idx += EIG_NT) {
@@ -355,23 +411,43 @@ __global__ void __launch_bounds__(EIG_NT) k_heev(int n, int nb, double2* __restr
   if (t == 0 && status) status[b] = 0;
 }
 
+size_t heev_lds_base(int n, int nb) {
+  size_t b = (size_t)2 * n * sizeof(double2) + ((size_t)3 * n + nb + EIG_WAVES + 2) * sizeof(double);
+  b = (b + 15) & ~(size_t)15;
+  return b + (size_t)16 * 64 * sizeof(double2);
+}
+constexpr size_t kLdsCap = 150 * 1024;
+size_t a_bytes(int n) { return (size_t)n * n * sizeof(double2); }
+size_t lu_bytes(int n, int nb) { return (size_t)6 * n * nb * sizeof(double); }
+bool a_fits_lds(int n, int nb) { return heev_lds_base(n, nb) + a_bytes(n) <= kLdsCap; }
+bool lu_fits_lds(int n, int nb) {
+  return heev_lds_base(n, nb) + (a_fits_lds(n, nb) ? a_bytes(n) : 0) + lu_bytes(n, nb) <= kLdsCap;
+}
 size_t heev_lds_bytes(int n, int nb) {
-  return (size_t)2 * n * sizeof(double2) + ((size_t)3 * n + nb + EIG_WAVES + 2) * sizeof(double);
+  return heev_lds_base(n, nb) + (a_fits_lds(n, nb) ? a_bytes(n) : 0) + (lu_fits_lds(n, nb) ? lu_bytes(n, nb) : 0);
 }
 
 }  // namespace
 
 size_t heev_work_doubles(int n, int nb, int nk) { return (size_t)nk * eig_stride(n, nb); }
 
-bool heev_supported(int n, int nb) { return n >= 1 && nb >= 1 && nb <= n && heev_lds_bytes(n, nb) <= 150 * 1024; }
+bool heev_supported(int n, int nb) { return n >= 1 && nb >= 1 && nb <= n && heev_lds_base(n, nb) <= kLdsCap; }
 
 void launch_heev_batched(Context& c, int n, int nb, int nk, void* H, void* work, double* w, void* V, int* status) {
   PWK_REQUIRE(n >= 1 && nb >= 1 && nb <= n && nk >= 1, PWK_E_INVALID, "bad eigensolver shape");
   PWK_REQUIRE(heev_supported(n, nb), PWK_E_UNSUPPORTED, "matrix too large for the batched eigensolver");
   const size_t sh = heev_lds_bytes(n, nb);
-  PWK_HIP(hipFuncSetAttribute((const void*)k_heev, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   TimedScope ts(c, "dense_eigh");
-  k_heev<<<(unsigned)nk, EIG_NT, sh, c.stream>>>(n, nb, (double2*)H, (double*)work, w, (double2*)V, status);
+  const int al = a_fits_lds(n, nb) ? 1 : 0, lu = lu_fits_lds(n, nb) ? 1 : 0;
+  if (n <= 64) {
+    PWK_HIP(hipFuncSetAttribute((const void*)k_heev<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_heev<64><<<(unsigned)nk, 64, sh, c.stream>>>(n, nb, al, lu, (double2*)H, (double*)work, w, (double2*)V,
+                                                   status);
+  } else {
+    PWK_HIP(hipFuncSetAttribute((const void*)k_heev<EIG_NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_heev<EIG_NT><<<(unsigned)nk, EIG_NT, sh, c.stream>>>(n, nb, al, lu, (double2*)H, (double*)work, w,
+                                                           (double2*)V, status);
+  }
   PWK_HIP(hipGetLastError());
 }
 

@@ -29,44 +29,41 @@ __device__ double sph_bessel(int l, double x) {
   return j;
 }
 
-// F[p][ig], p over all (species, projector) pairs
+// F[p][ig] = 4π ∫ χ_p(r) j_l(|k+G| r) r dr (Simpson with the even-mesh end correction,
+// src/numerics.rs:29-58). One wavefront per (G, projector): lanes stride over the mesh,
+// then a wave reduction, so every pair integrates in parallel.
+__device__ inline double simpson_weight(int i, int n) {
+  if (n < 3) return 3.0;  // plain sum, divided by 3 below
+  if (i == 0) return 1.0;
+  double w;
+  if (i == n - 1) {
+    w = (n % 2 == 1) ? 1.0 : 1.25;
+  } else {
+    w = (i % 2 == 1) ? 4.0 : 2.0;
+    if (n % 2 == 0 && i == n - 3) w -= 0.25;
+    if (n % 2 == 0 && i == n - 2) w += 1.0;
+  }
+  return w;
+}
+
 __global__ void __launch_bounds__(256) k_form_factors(int npw, int nprojtot, const double* __restrict__ qn,
                                                       const int* __restrict__ p_l, const int* __restrict__ p_mesh,
                                                       const long* __restrict__ p_rofs, const long* __restrict__ p_cofs,
                                                       const double* __restrict__ r, const double* __restrict__ rab,
                                                       const double* __restrict__ chi, double* __restrict__ F) {
-  int ig = blockIdx.x * blockDim.x + threadIdx.x;
-  int p = blockIdx.y;
-  if (ig >= npw || p >= nprojtot) return;
-  double q = qn[ig];
-  int l = p_l[p], n = p_mesh[p];
+  const int lane = threadIdx.x & 63;
+  const long pair = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pair >= (long)npw * nprojtot) return;  // wave-uniform
+  const int ig = (int)(pair % npw), p = (int)(pair / npw);
+  const double q = qn[ig];
+  const int l = p_l[p], n = p_mesh[p];
   const double* rr = r + p_rofs[p];
   const double* ra = rab + p_rofs[p];
   const double* ch = chi + p_cofs[p];
-  // integrand f_i = χ_i j_l(q r_i) r_i ; Simpson with the even-mesh correction
-  auto f = [&](int i) { return ch[i] * sph_bessel(l, q * rr[i]) * rr[i]; };
-  double res;
-  if (n < 3) {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += f(i) * ra[i];
-    res = s;
-  } else {
-    double sum = 0.0;
-    for (int i = 1; i < n - 1; ++i) {
-      double w = (i % 2 == 1) ? 4.0 : 2.0;
-      sum += w * f(i) * ra[i];
-    }
-    if (n % 2 == 1) {
-      res = fma(f(n - 1), ra[n - 1], fma(f(0), ra[0], sum)) / 3.0;
-    } else {
-      double acc = fma(f(0), ra[0], sum);
-      acc = fma(f(n - 3) * ra[n - 3], -0.25, acc);
-      acc += f(n - 2) * ra[n - 2];
-      acc = fma(f(n - 1) * ra[n - 1], 1.25, acc);
-      res = acc / 3.0;
-    }
-  }
-  F[(size_t)p * npw + ig] = 4.0 * kPi * res;
+  double acc = 0.0;
+  for (int i = lane; i < n; i += 64) acc += simpson_weight(i, n) * (ch[i] * sph_bessel(l, q * rr[i]) * rr[i]) * ra[i];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) F[(size_t)p * npw + ig] = 4.0 * kPi * (acc / 3.0);
 }
 
 // Real Y_lm for l <= 3 in the order (0,0),(1,0),(1,+1),(1,-1),(2,0),(2,+1),(2,-1),... (nonlocal.rs:469-552)
@@ -270,7 +267,7 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
   k_qnorm<<<blocks_for((size_t)npw, 256), 256, 0, st>>>(npw, d_k.as<double>(), d_g.as<double>(), d_qn.as<double>());
   PWK_HIP(hipGetLastError());
   if (nprojtot > 0) {
-    dim3 g(blocks_for((size_t)npw, 256), (unsigned)nprojtot);
+    const unsigned g = blocks_for((size_t)npw * nprojtot, 4);
     k_form_factors<<<g, 256, 0, st>>>(npw, nprojtot, d_qn.as<double>(), d_pl.as<int>(), d_pm.as<int>(),
                                       d_pro.as<long>(), d_pco.as<long>(), d_r.as<double>(), d_rab.as<double>(),
                                       d_chi.as<double>(), d_F.as<double>());

@@ -525,12 +525,8 @@ struct Solve {
   int nch_all = 0;
   std::vector<double> coef_host;
 
-  const void* B_of(int k) {
-    return chebyshev ? (const void*)(Ball64.as<double2>() + (size_t)k * npw * nch_all) : c.k(k).B64.p;
-  }
-  const void* DBH_of(int k) {
-    return chebyshev ? (const void*)(DBHall64.as<double2>() + (size_t)k * npw * nch_all) : c.k(k).DBH64.p;
-  }
+  const void* B_of(int k) { return Ball64.as<double2>() + (size_t)k * npw * nch_all; }
+  const void* DBH_of(int k) { return DBHall64.as<double2>() + (size_t)k * npw * nch_all; }
   void hpsi_all(const void* psi, void* hpsi, int nbands, int prec) {
     const bool f64 = prec == PWK_FP64;
     launch_hpsi_multi(c, nk, nbands, psi, hpsi, prec, f64 ? kin_all64.p : kin_all32.p, nch_all,
@@ -771,7 +767,7 @@ void Solve::setup() {
     const double kv[3] = {kc.x, kc.y, kc.z};
     build_nonlocal_device(c, k, kv, gflat.data(), cr.natoms, tau.data(), spi.data(), cr.nspecies, nproj.data(),
                           pl.data(), dij.data(), mesh.data(), rr.data(), rab.data(), chi.data(), omega);
-    if (chebyshev) {
+    {
       // projectors of all k-points in contiguous blocks for the batched GEMMs
       if (k == 0) {
         nch_all = kd.nch;
@@ -1137,7 +1133,7 @@ void Solve::run(double* eig_out, double* rho_out) {
       // ---- dense H per k, one batched eigensolve ----------------------------------------
       {
         TimedScope ts(c, "scf_dense_h");
-        for (int k = 0; k < nk; ++k) launch_dense_h(c, k, 1, Hb.as<double2>() + (size_t)k * npw * npw);
+        launch_dense_h_batched(c, nk, kin_all64.as<double>(), nch_all, Ball64.p, DBHall64.p, Hb.p);
       }
       launch_heev_batched(c, npw, nb, nk, Hb.p, c.eig_work.p, W.as<double>(), V.p, nullptr);
       PWK_HIP(hipMemcpyAsync(ev.data(), W.p, ev.size() * 8, hipMemcpyDeviceToHost, c.stream));
