@@ -293,9 +293,10 @@ void zgemm_mfma_t(Context& c, int M, int N, int K, const void* A, int lda, const
   long tiles = (long)mt * nt * bt.count;
   int target = 2 * (c.cu_count > 0 ? c.cu_count : 256);
   int nsplit = 1;
-  // split K only when each chunk keeps at least 16 stages: short reductions are cheaper than
-  // the extra partial-sum pass
-  if (tiles < target) nsplit = (int)std::min<long>((target + tiles - 1) / tiles, std::max(1, K / (16 * BKC)));
+  // split K only for K >= 256 (short reductions are cheaper than the extra partial-sum pass),
+  // then into chunks of at least 4 stages
+  if (tiles < target && K >= 16 * BKC)
+    nsplit = (int)std::min<long>((target + tiles - 1) / tiles, std::max(1, K / (4 * BKC)));
   int kchunk = (K + nsplit - 1) / nsplit;
   kchunk = ((kchunk + BKC - 1) / BKC) * BKC;
   nsplit = (K + kchunk - 1) / kchunk;
