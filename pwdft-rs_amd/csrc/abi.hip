@@ -137,6 +137,7 @@ int pwk_profile_enable(pwk_ctx* ctx, int on) {
 int pwk_profile_reset(pwk_ctx* ctx) {
   return guarded([&] {
     Context& c = ctx_of(ctx);
+    PWK_HIP(hipStreamSynchronize(c.stream));
     c.prof.collect();
     c.prof.totals.clear();
   });
@@ -146,6 +147,7 @@ int pwk_profile_read(pwk_ctx* ctx, int cap, char* names, double* total_ms, int64
   int n = 0;
   int st = guarded([&] {
     Context& c = ctx_of(ctx);
+    PWK_HIP(hipStreamSynchronize(c.stream));
     c.prof.collect();
     for (auto& kv : c.prof.totals) {
       if (n < cap) {

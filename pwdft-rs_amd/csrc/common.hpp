@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -82,11 +83,16 @@ struct Profiler {
       pool.pop_back();
       return e;
     }
+    // timing-only events without the system-scope fence (cache writeback/invalidate) on record,
+    // so short kernels are not inflated by the event itself; plain events where unsupported
     hipEvent_t e;
-    PWK_HIP(hipEventCreate(&e));
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
+      (void)hipGetLastError();
+      PWK_HIP(hipEventCreate(&e));
+    }
     return e;
   }
-  void collect() {
+  void collect() {  // callers synchronise the stream first
     for (auto& p : pending) {
       PWK_HIP(hipEventSynchronize(p.b));
       float ms = 0;
@@ -230,6 +236,18 @@ void launch_resid_norms(Context& c, int npw, int nbs, int nres, int nk, const vo
 void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
              double* rho);
 void scf_tables(const pwk_crystal& cr, const pwk_scf_params& p, int* npw, int dims[3], int* nk, int* nops);
+
+// Raises a kernel's dynamic-LDS limit once (the attribute call costs host time that
+// would otherwise sit between back-to-back launches).
+inline void ensure_lds_limit(const void* fn, size_t bytes) {
+  static std::mutex mu;
+  static std::map<const void*, size_t> set;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& cur = set[fn];
+  if (bytes <= cur) return;
+  PWK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  cur = bytes;
+}
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 

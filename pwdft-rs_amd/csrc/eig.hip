@@ -569,11 +569,11 @@ void launch_heev_batched(Context& c, int n, int nb, int nk, void* H, void* work,
   }
   const int al = a_fits_lds(n, nb) ? 1 : 0, lu = lu_fits_lds(n, nb) ? 1 : 0;
   if (n <= 64) {
-    PWK_HIP(hipFuncSetAttribute((const void*)k_heev<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    ensure_lds_limit((const void*)k_heev<64>, sh);
     k_heev<64><<<(unsigned)nk, 64, sh, c.stream>>>(n, nb, al, lu, (double2*)H, (double*)work, w, (double2*)V,
                                                    status);
   } else {
-    PWK_HIP(hipFuncSetAttribute((const void*)k_heev<EIG_NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    ensure_lds_limit((const void*)k_heev<EIG_NT>, sh);
     k_heev<EIG_NT><<<(unsigned)nk, EIG_NT, sh, c.stream>>>(n, nb, al, lu, (double2*)H, (double*)work, w,
                                                            (double2*)V, status);
   }

@@ -293,13 +293,13 @@ void hpsi_local(Context& c, const T* kin, int nb, int nbk, const void* psi, void
     PWK_HIP(hipGetLastError());
   }
   dim3 gp((unsigned)nz, (unsigned)nb);
+  {
   TimedScope ts_plane(c, "hpsi_plane");
   switch (nx) {
 #define PWK_CASE(L)                                                                                      \
   case L: {                                                                                              \
     size_t sh = Plane<T, L, L>::bytes;                                                                   \
-    PWK_HIP(hipFuncSetAttribute((const void*)k_plane_hpsi<T, L, L>,                                      \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));                  \
+    ensure_lds_limit((const void*)k_plane_hpsi<T, L, L>, sh);                                           \
     k_plane_hpsi<T, L, L><<<gp, 64, sh, c.stream>>>(T_, vr, c.nsticks, nz, c.st_xy.as<int>(), c.nx1,     \
                                                     c.x1list.as<int>());                                \
     break;                                                                                               \
@@ -310,6 +310,7 @@ void hpsi_local(Context& c, const T* kin, int nb, int nbk, const void* psi, void
       throw Error(PWK_E_UNSUPPORTED, "nx has no compiled kernel");
   }
   PWK_HIP(hipGetLastError());
+  }  // the plane scope ends here, before the z-forward launch
   T inv_n = (T)(1.0 / (double)c.N);
   TimedScope ts_zfwd(c, "hpsi_zfwd");
   switch (nz) {
@@ -354,8 +355,7 @@ void density_t(Context& c, int nb, const void* psi, const double* fw_host, void*
 #define PWK_CASE(L)                                                                                      \
   case L: {                                                                                              \
     size_t sh = Plane<T, L, L>::bytes + (size_t)L * L * sizeof(double);                                  \
-    PWK_HIP(hipFuncSetAttribute((const void*)k_plane_density<T, L, L>,                                   \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));                  \
+    ensure_lds_limit((const void*)k_plane_density<T, L, L>, sh);                                        \
     k_plane_density<T, L, L><<<(unsigned)nz, 64, sh, c.stream>>>(T_, nb, c.work_b.as<double>(),          \
                                                                  (double*)rho, c.nsticks, nz,            \
                                                                  c.st_xy.as<int>(), c.nx1,              \
