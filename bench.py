@@ -488,6 +488,57 @@ def scf_bench(args):
     }
 
 
+def diamond_cubic(a_ang, n):
+    base = [[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0], [.25, .25, .25], [.25, .75, .75], [.75, .25, .75],
+            [.75, .75, .25]]
+    frac = []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                frac += [[(f[0] + i) / n, (f[1] + j) / n, (f[2] + k) / n] for f in base]
+    import numpy as np
+
+    return np.eye(3) * a_ang * n, np.array(frac)
+
+
+def precision_study():
+    """Full SCF of the 64-atom C (a = 13.48 bohr, 30 Ry) and Si (a = 20.52 bohr, 20 Ry) supercells,
+    atoms displaced by N(0, 0.05 bohr) (seed 20261202), 2×2×2 Γ-centred MP reduced by the detected
+    symmetry, 160 bands, filtered eigensolver: FP32 filter H·psi against FP64, error per energy term."""
+    import numpy as np
+    import pwk
+    from pwk import scf
+
+    acc = pwk.GpuAccelerator(0)
+    out = {}
+    terms = ["e_total", "e_band", "e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_ewald", "e_smearing"]
+    for el, a_bohr, ecut in (("C", 13.48, 30), ("Si", 20.52, 20)):
+        rng = np.random.default_rng(20261202)
+        lat, frac = diamond_cubic(a_bohr / 2 * BOHR, 2)
+        frac = frac + rng.normal(0.0, 0.05 * BOHR, frac.shape) @ np.linalg.inv(lat)
+        deck = scf.CrystalDeck(lat, frac, [0] * 64,
+                               [os.path.join(ROOT, "tests", "golden", "pseudo", f"{el}.upf")])
+        res = {}
+        for prec in ("fp64", "fp32"):
+            p = scf.ScfParams(ecut_ev=ecut * RY, kgrid=(2, 2, 2), n_bands=160, max_iter=40, conv_threshold=1e-7,
+                              energy_threshold=1e-6, smearing_sigma=0.01 * RY, eigensolver=1, precision=prec)
+            log(f"precision study: {el}64 {prec}")
+            t0 = time.perf_counter()
+            res[prec] = scf.run_scf(acc, deck, p)
+            res[prec]["wall_s"] = time.perf_counter() - t0
+        t = scf.scf_tables(deck, p)
+        out[f"{el}64"] = {
+            "npw": t["npw"], "grid": list(t["dims"]), "nk": t["nk"], "bands": 160,
+            "converged": [res["fp64"]["converged"], res["fp32"]["converged"]],
+            "iterations": [res["fp64"]["n_iter"], res["fp32"]["n_iter"]],
+            "wall_s": [round(res["fp64"]["wall_s"], 2), round(res["fp32"]["wall_s"], 2)],
+            "e_total_eV": res["fp64"]["e_total"],
+            "abs_err_Ry_per_atom": {k: abs(res["fp32"][k] - res["fp64"][k]) / RY / 64 for k in terms},
+        }
+    acc.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -502,6 +553,8 @@ def main():
                     help="GPU_MAX_HW_QUEUES for this process (0: leave the runtime default); at most 32")
     ap.add_argument("--no-hpsi", action="store_true", help="skip the H·psi at scale section")
     ap.add_argument("--hpsi-only", action="store_true")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="also run the 64-atom C/Si FP32-vs-FP64 SCF study (about two minutes)")
     ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--bands", type=int, default=144)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "split_bf16"], help="H·psi precision")
@@ -578,6 +631,8 @@ def main():
         "cpu_baseline": sc["cpu_baseline"],
         "hpsi_at_scale": hp,
     }
+    if args.precision_study:
+        out["precision_study"] = precision_study()
     print(json.dumps(out), flush=True)
 
 
