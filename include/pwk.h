@@ -239,8 +239,11 @@ typedef struct pwk_species {
   const double* rho_core;
 } pwk_species;
 
+#define PWK_MAX_SPECIES 8
+
 /* Crystal (src/crystal.rs:83-121): lattice rows a, b, c in Å, fractional
- * positions, a species index per atom into species_data (at most 8 species). */
+ * positions, a species index per atom into species_data (at most
+ * PWK_MAX_SPECIES species). */
 typedef struct pwk_crystal {
   double lattice[9];
   int natoms;
@@ -250,7 +253,9 @@ typedef struct pwk_crystal {
   const pwk_species* species_data;
 } pwk_crystal;
 
-/* Parameters of ScfParams (src/scf/mod.rs:72-402) used by the nspin = 1 driver. */
+/* Parameters of ScfParams (src/scf/mod.rs:72-402) used by the drivers. A
+ * zero-initialised struct with ecut, bands and thresholds filled in is a plain
+ * nspin = 1 solve. */
 typedef struct pwk_scf_params {
   double ecut_ev;
   int ecutrho_ratio;      /* 4 in the reference */
@@ -276,6 +281,17 @@ typedef struct pwk_scf_params {
                              up to 1500 plane waves, filtered above */
   int cheb_degree;        /* filter polynomial degree (0: 10) */
   int verbose;
+  /* spin and mixing (src/scf/mod.rs:90-107) */
+  int nspin;              /* 1 (0 reads as 1) or 2: LSDA, src/scf/driver_spin.rs */
+  double starting_magnetization[PWK_MAX_SPECIES]; /* per species, in [-1, 1]; the start
+                             density splits as rho (1 +- m)/2 with m their mean over atoms */
+  int fixed_magnetization;/* nonzero: hold the total moment at tot_magnetization with one
+                             Fermi level per spin channel */
+  double tot_magnetization;
+  int mixing_mode;        /* 0: plain Anderson; 1: Anderson on Kerker-preconditioned residuals
+                             (the magnetisation channel of nspin = 2 always mixes plain) */
+  double kerker_q_tf;     /* Thomas-Fermi wave vector, 1/Angstrom; <= 0: estimated from the
+                             mean valence density (mixing/kerker.rs:50-62) */
 } pwk_scf_params;
 
 /* ScfResult + EnergyComponents (src/scf/energy.rs:597-670), eV. */
@@ -285,14 +301,19 @@ typedef struct pwk_scf_result {
   double e_total, e_harris, fermi, ts, final_delta, integrated_charge;
   double e_band, e_kinetic, e_local, e_nonlocal, e_hartree, e_xc, e_vxc, e_ewald, e_smearing;
   double seconds;
+  int nspin;
+  double magnetization;   /* |N_up - N_down| (nspin = 2) */
+  double n_up, n_down;    /* electrons per spin channel (nspin = 2) */
 } pwk_scf_result;
 
-/* run_scf for nspin = 1 (src/scf/driver.rs:353-735) with the dense
- * eigensolver: setup, V_eff, dense H per k, batched eigensolve, Fermi-Dirac
- * occupations, density, symmetrisation, energies and Anderson mixing, all on
- * the context's device. eigenvalues (nk × n_bands) and rho (the converged
- * density on the FFT grid) may be NULL; their sizes come from
- * pwk_scf_tables. Returns PWK_OK also when the solve did not converge
+/* run_scf (src/scf/mod.rs:326-400): nspin = 1 follows run_scf_unpolarized
+ * (src/scf/driver.rs:353-735), nspin = 2 run_scf_spin (src/scf/driver_spin.rs:37-839).
+ * Setup, V_eff, the eigensolve (dense H per k with a batched eigensolver, or
+ * Chebyshev-filtered subspace iteration), Fermi-Dirac occupations, density,
+ * symmetrisation, energies and Anderson mixing all run on the context's
+ * device. eigenvalues (nspin × nk × n_bands, spin-up channel first) and rho
+ * (the converged total density on the FFT grid) may be NULL; their sizes come
+ * from pwk_scf_tables. Returns PWK_OK also when the solve did not converge
  * (result->converged = 0). */
 int pwk_scf_run(pwk_ctx* ctx, const pwk_crystal* crystal, const pwk_scf_params* params, pwk_scf_result* result,
                 double* eigenvalues, double* rho);

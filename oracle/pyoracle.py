@@ -326,20 +326,27 @@ class System:
         return acc
 
     def scf(self, kpts, weights, n_bands=8, max_iter=100, conv=1e-6, ethr=1e-5, beta=0.3, ndim=8, sigma=0.01,
-            gaussian_sigma=1.0, use_sym=False, sym_tol=1e-5, verbose=False):
+            gaussian_sigma=1.0, use_sym=False, sym_tol=1e-5, verbose=False, nspin=1, start_mag=0.0,
+            tot_mag=None, kerker=False, q_tf=0.0):
+        """nspin=2: LSDA; start_mag is the mean per-atom starting magnetisation, tot_mag a fixed
+        total moment (one Fermi level per channel). Eigenvalues are (nspin·nk, nb), up channel first.
+        kerker: Kerker-preconditioned Anderson (q_tf ≤ 0: Thomas-Fermi estimate)."""
         k = f64(kpts).reshape(-1)
         w = f64(weights)
         nk = len(w)
-        params = f64([n_bands, max_iter, conv, ethr, beta, ndim, sigma, gaussian_sigma, float(use_sym), sym_tol, float(verbose)])
-        out = np.zeros(16)
+        params = f64([n_bands, max_iter, conv, ethr, beta, ndim, sigma, gaussian_sigma, float(use_sym), sym_tol,
+                      float(verbose), nspin, start_mag, float(tot_mag is not None), tot_mag or 0.0,
+                      float(kerker), q_tf])
+        out = np.zeros(19)
         nb = min(n_bands, self.npw)
-        evals = np.zeros((nk, nb))
+        evals = np.zeros((nspin * nk, nb))
         rho = np.zeros(self.n_grid)
         rc = lib().orc_scf(self.h, nk, _d(k), _d(w), _d(params), _d(out), _d(evals), _d(rho))
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         keys = ["e_total", "e_harris", "fermi", "ts", "n_iter", "converged", "final_delta", "e_band", "e_kinetic",
-                "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_vxc", "e_ewald", "e_smearing"]
+                "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_vxc", "e_ewald", "e_smearing", "magnetization",
+                "n_up", "n_dn"]
         res = dict(zip(keys, out.tolist()))
         res["eigenvalues"] = evals
         res["rho"] = rho

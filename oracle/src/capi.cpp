@@ -323,12 +323,18 @@ int orc_scf(void* sp, int nk, const double* k, const double* w, const double* pa
     p.sigma = params[6];
     p.gaussian_sigma = params[7];
     p.verbose = (int)params[10];
+    p.nspin = (int)params[11];
+    p.start_mag = params[12];
+    p.fixed_mag = params[13] != 0.0;
+    p.tot_mag = params[14];
+    p.kerker = (int)params[15];
+    p.q_tf = params[16];
     Symmetry sym = params[8] != 0.0 ? find_symmetry(s->crystal, params[9]) : identity_symmetry();
-    ScfResult r = run_scf(*s, kp, p, sym);
+    ScfResult r = p.nspin == 2 ? run_scf_spin(*s, kp, p, sym) : run_scf(*s, kp, p, sym);
     const EnergyComponents& c = r.comp;
-    double vals[16] = {r.e_total, r.e_harris, r.fermi, r.ts, (double)r.n_iter, r.converged ? 1.0 : 0.0,
+    double vals[19] = {r.e_total, r.e_harris, r.fermi, r.ts, (double)r.n_iter, r.converged ? 1.0 : 0.0,
                        r.final_delta, c.e_band, c.e_kinetic, c.e_local, c.e_nonlocal, c.e_hartree, c.e_xc,
-                       c.e_vxc, c.e_ewald, c.e_smearing};
+                       c.e_vxc, c.e_ewald, c.e_smearing, r.magnetization, r.n_up, r.n_dn};
     std::memcpy(out, vals, sizeof vals);
     if (evals && r.converged)
       for (size_t i = 0; i < r.eigenvalues.size(); ++i)

@@ -235,10 +235,16 @@ double find_fermi_energy(const std::vector<std::vector<double>>& evs, const std:
                          double n_el, double sigma, double spin_factor);
 double entropy_ts(const std::vector<std::vector<double>>& evs, const std::vector<double>& w, double ef,
                   double sigma, double spin_factor);
+// q_TF² in Å⁻² from the mean valence density (mixing/kerker.rs:50-62)
+double kerker_auto_qtf2(double n_electrons, double omega);
 
+// Plain Anderson, or Kerker-preconditioned when kerker_w holds the per-grid-point weights
+// |G|²/(|G|² + q_TF²) (0 at G = 0) of mixing/anderson.rs:34-97 and mixing/kerker.rs:31-48
 struct Anderson {
   double beta = 0.3;
   size_t max_history = 8;
+  std::vector<double> kerker_w;
+  std::array<size_t, 3> dims{};
   std::vector<std::vector<double>> hist_in, hist_res;
   std::vector<double> mix(const std::vector<double>& rho_in, const std::vector<double>& rho_out);
 };
@@ -256,6 +262,17 @@ struct ScfParams {
   double sigma = 0.01;
   double gaussian_sigma = 1.0;
   int verbose = 0;
+  // nspin = 2 (src/scf/driver_spin.rs): the start density splits as ρ(1 ± m)/2 with m the
+  // mean of the per-atom starting magnetisations; a fixed total moment uses one Fermi
+  // level per channel
+  int nspin = 1;
+  double start_mag = 0.0;
+  bool fixed_mag = false;
+  double tot_mag = 0.0;
+  // mixing_mode Plain (0) or Kerker (1); q_tf ≤ 0 selects the automatic Thomas-Fermi estimate.
+  // The magnetisation channel of nspin = 2 always mixes plain.
+  int kerker = 0;
+  double q_tf = 0.0;
 };
 struct EnergyComponents {
   double e_band = 0, e_kinetic = 0, e_local = 0, e_nonlocal = 0, e_hartree = 0, e_xc = 0, e_vxc = 0,
@@ -265,10 +282,14 @@ struct ScfResult {
   bool converged = false;
   size_t n_iter = 0;
   double e_total = 0, e_harris = 0, fermi = 0, ts = 0, final_delta = 0;
+  double magnetization = 0, n_up = 0, n_dn = 0;  // nspin = 2
   EnergyComponents comp;
   std::vector<std::vector<double>> eigenvalues;
   std::vector<double> rho_r;
 };
 ScfResult run_scf(const System& s, const std::vector<KPoint>& kpts, const ScfParams& p, const Symmetry& sym);
+// nspin = 2, LSDA with coupled-channel (ρ_total, m) Anderson mixing — src/scf/driver_spin.rs:37-839.
+// Eigenvalues come back as [up k0.., down k0..].
+ScfResult run_scf_spin(const System& s, const std::vector<KPoint>& kpts, const ScfParams& p, const Symmetry& sym);
 
 }  // namespace orc

@@ -243,10 +243,28 @@ static std::vector<double> solve_small(std::vector<double> a, std::vector<double
 }
 
 // src/scf/mixing/anderson.rs:170-266 (plain mode, no Kerker, fixed β)
+double kerker_auto_qtf2(double n_electrons, double omega) {
+  const double bohr = 0.529177210903;
+  const double rho_bohr = n_electrons / omega * bohr * bohr * bohr;
+  const double pi = 3.14159265358979323846;
+  const double q2_bohr = 4.0 * std::cbrt(3.0 * pi * pi * rho_bohr) / pi;
+  return q2_bohr / (bohr * bohr);
+}
+
 std::vector<double> Anderson::mix(const std::vector<double>& rho_in, const std::vector<double>& rho_out) {
   size_t n = rho_in.size();
   std::vector<double> res(n);
   for (size_t i = 0; i < n; ++i) res[i] = rho_out[i] - rho_in[i];
+  if (!kerker_w.empty()) {
+    // mixing/kerker.rs:5-29: forward FFT, weight per G, inverse FFT, 1/N
+    std::vector<cplx> rg(n);
+    for (size_t i = 0; i < n; ++i) rg[i] = cplx(res[i], 0.0);
+    fft3d(rg, dims, -1);
+    for (size_t i = 0; i < n; ++i) rg[i] *= kerker_w[i];
+    fft3d(rg, dims, +1);
+    const double inv = 1.0 / (double)n;
+    for (size_t i = 0; i < n; ++i) res[i] = rg[i].real() * inv;
+  }
   hist_in.push_back(rho_in);
   hist_res.push_back(res);
   if (hist_in.size() > max_history) {

@@ -86,7 +86,7 @@ extern "C" {
 
 const char* pwk_last_error(void) { return pwk::g_last_error.c_str(); }
 
-int pwk_version(void) { return 100; }
+int pwk_version(void) { return 101; }
 
 int pwk_ctx_create(int device, pwk_ctx** out) {
   return guarded([&] {

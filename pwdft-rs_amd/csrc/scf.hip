@@ -205,12 +205,52 @@ __global__ void k_c2r_scaled(size_t n, const double2* __restrict__ c, double s, 
   r[i] = v;
 }
 
-// src/scf/energy.rs:314-324: ρ_xc = max(ρ + ρ_core, 0) (no clamp without a core)
-__global__ void k_add_core(size_t n, const double* __restrict__ rho, const double* __restrict__ core,
+// src/scf/energy.rs:314-324: ρ_xc = max(ρ + cs·ρ_core, 0) (no clamp without a core); cs is
+// 1, or 1/2 for one spin channel (src/scf/driver_spin.rs:260)
+__global__ void k_add_core(size_t n, const double* __restrict__ rho, const double* __restrict__ core, double cs,
                            double* __restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = core ? fmax(rho[i] + core[i], 0.0) : rho[i];
+  out[i] = core ? fmax(rho[i] + core[i] * cs, 0.0) : rho[i];
+}
+
+// ρ_total = ρ↑ + ρ↓ and m = ρ↑ − ρ↓ (m may be null)
+__global__ void k_spin_combine(size_t n, const double* __restrict__ up, const double* __restrict__ dn,
+                               double* __restrict__ tot, double* __restrict__ mag) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double u = up[i], d = dn[i];
+  tot[i] = u + d;
+  if (mag) mag[i] = u - d;
+}
+
+// ρ↑ = (ρ_total + m)/2, ρ↓ = (ρ_total − m)/2  (src/scf/driver_spin.rs:820-831)
+__global__ void k_spin_split(size_t n, const double* __restrict__ tot, const double* __restrict__ mag,
+                             double* __restrict__ up, double* __restrict__ dn) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double t = tot[i], m = mag[i];
+  up[i] = 0.5 * (t + m);
+  dn[i] = 0.5 * (t - m);
+}
+
+// start split ρ↑ = ρ(1 + m)/2, ρ↓ = ρ(1 − m)/2 (src/scf/driver_spin.rs:74-75); rho may alias up
+__global__ void k_spin_start(size_t n, const double* rho, double mag, double* up, double* __restrict__ dn) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double r = rho[i];
+  up[i] = r * (1.0 + mag) / 2.0;
+  dn[i] = r * (1.0 - mag) / 2.0;
+}
+
+// Kerker weight |G|²/(|G|² + q_TF²), zero at G = 0 (src/scf/mixing/anderson.rs:64-76)
+__global__ void k_kerker(size_t n, const double* __restrict__ g2, double q2, double2* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double g = g2[i];
+  const double w = g < 1e-12 ? 0.0 : g / (g + q2);
+  x[i].x *= w;
+  x[i].y *= w;
 }
 
 __global__ void k_scale(size_t n, double* __restrict__ x, double s) {
@@ -473,10 +513,28 @@ struct Solve {
   std::vector<SymOp> ops;
   bool have_core = false;
 
-  // device arrays
+  // spin channels: 1, or 2 for LSDA (src/scf/driver_spin.rs); per-channel arrays hold the
+  // spin-up channel first
+  int ns = 1;
+  int sc = 0;  // channel the eigensolver works on
+
+  // device arrays: rho_in / rho_new / rho_xc / vxc / veff_g hold ns channels of N points;
+  // t_in, m_in, t_out, m_out are ρ_total and m = ρ↑ − ρ↓ of the input and output density
+  // (nspin = 2), xc_tot the clamped ρ_total + ρ_core
   DevBuf vloc_g, vloc_r, core_r, rho_in, rho_g_in, rho_new, rho_g_new, rho_xc, exc, vxc, vxc_g, veff_g, cwork;
-  DevBuf Hb, W, V, V32, sym_rot, sym_tau, partial, dots_out, hist, nl_p, nl_x, expect;
+  DevBuf t_in, m_in, t_out, m_out, xc_tot;
+  DevBuf Hb, W, V, V32, sym_rot, sym_tau, partial, dots_out, nl_p, nl_x, expect;
   double* host_small = nullptr;  // pinned
+
+  // Anderson history of one mixed quantity (src/scf/mixing/anderson.rs:20-28)
+  struct Mixer {
+    DevBuf hist;             // ndim slots of (input, residual)
+    std::vector<int> hslot;  // logical history order → slot
+    int next_slot = 0, ndim = 1;
+    bool kerker = false;
+  };
+  Mixer mix_tot, mix_mag;
+  double kerker_q2 = 0.0;
 
   Solve(Context& cc, const pwk_crystal& a, const pwk_scf_params& b, pwk_scf_result& r) : c(cc), cr(a), p(b), res(r) {}
   ~Solve() {
@@ -509,12 +567,12 @@ struct Solve {
 
   // Chebyshev-filtered subspace iteration state
   bool chebyshev = false;
-  std::vector<double> ev_sub;              // nk × nbs Ritz values of the filtered subspace
-  std::vector<double>* ev_host = nullptr;  // nk × nb eigenvalues used for occupations
+  std::vector<double> ev_sub;              // ns × nk × nbs Ritz values of the filtered subspace
+  double* ev_host = nullptr;               // nk × nb eigenvalues of the current channel
   int nbs = 0;                             // bands per k held in V (nb, plus a buffer when filtering)
   int fprec = PWK_FP64;                // precision of the filter's H·ψ
   int degree = 10;
-  std::vector<double> ub;              // per-k upper bound of the spectrum
+  std::vector<double> ub;              // per-(channel, k) upper bound of the spectrum
   DevBuf F0, F1, F2, HYf, Y64, HY64, YH, sS, sH, sU, sT, sX, sXH, sW, sC, s_ev, s_lam, s_work, lowest;
   DevBuf kin_all64, kin_all32;              // nk × npw kinetic diagonals
   DevBuf R64, resid;                        // H·V after Rayleigh-Ritz, residual norms
@@ -524,6 +582,13 @@ struct Solve {
   DevBuf Ball64, DBHall64, Ball32, DBHall32, projall, coef;
   int nch_all = 0;
   std::vector<double> coef_host;
+
+  size_t vblk() const { return (size_t)nk * npw * nbs; }  // eigenvector elements per channel
+  double2* Vc() { return V.as<double2>() + (size_t)sc * vblk(); }
+  double* ev_sub_c() { return ev_sub.data() + (size_t)sc * nk * nbs; }
+  double* ub_c() { return ub.data() + (size_t)sc * nk; }
+  double* rho_in_s(int s) { return rho_in.as<double>() + (size_t)s * N; }
+  double* rho_new_s(int s) { return rho_new.as<double>() + (size_t)s * N; }
 
   const void* B_of(int k) { return Ball64.as<double2>() + (size_t)k * npw * nch_all; }
   const void* DBH_of(int k) { return DBHall64.as<double2>() + (size_t)k * npw * nch_all; }
@@ -540,6 +605,8 @@ struct Solve {
   double chebyshev_pass(int prec);
   void eig_chebyshev(bool first);
   void veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out, double* vxc_out);
+  void xc_spin(const double* up, const double* dn);
+  void mix(Mixer& m, const double* in, const double* out, double* result);
   void run(double* eig_out, double* rho_out);
 };
 
@@ -565,6 +632,9 @@ void Solve::setup() {
   for (int d = 0; d < 3; ++d) PWK_REQUIRE(dims[d] > 0 && dims[d] <= 1024, PWK_E_INVALID, "FFT grid exceeds 1024");
   nb = std::min(p.n_bands, npw);
   PWK_REQUIRE(nb >= 1, PWK_E_INVALID, "n_bands must be positive");
+  PWK_REQUIRE(p.nspin >= 0 && p.nspin <= 2, PWK_E_INVALID, "nspin must be 1 or 2");
+  ns = p.nspin == 2 ? 2 : 1;
+  PWK_REQUIRE(p.mixing_mode == 0 || p.mixing_mode == 1, PWK_E_INVALID, "mixing_mode must be 0 (plain) or 1 (Kerker)");
   PWK_REQUIRE(p.eigensolver != 0 || heev_supported(npw, nb), PWK_E_UNSUPPORTED,
               "basis too large for the dense eigensolver");
   // |G|² per grid point (src/scf/context.rs:146-156)
@@ -704,8 +774,12 @@ void Solve::setup() {
     PWK_HIP(hipGetLastError());
   }
   // working arrays
-  for (DevBuf* b : {&vloc_r, &core_r, &rho_in, &rho_new, &rho_xc, &exc, &vxc}) b->ensure(N * 8);
-  for (DevBuf* b : {&rho_g_in, &rho_g_new, &vxc_g, &veff_g, &cwork}) b->ensure(N * 16);
+  for (DevBuf* b : {&vloc_r, &core_r, &exc}) b->ensure(N * 8);
+  for (DevBuf* b : {&rho_in, &rho_new, &rho_xc, &vxc}) b->ensure((size_t)ns * N * 8);
+  for (DevBuf* b : {&rho_g_in, &rho_g_new, &vxc_g, &cwork}) b->ensure(N * 16);
+  veff_g.ensure((size_t)ns * N * 16);
+  if (ns == 2)
+    for (DevBuf* b : {&t_in, &m_in, &t_out, &m_out, &xc_tot}) b->ensure(N * 8);
   partial.ensure((size_t)MAX_TERMS * RED_BLOCKS * 8);
   dots_out.ensure(MAX_TERMS * 8);
   PWK_HIP(hipHostMalloc((void**)&host_small, 64 * sizeof(double), hipHostMallocDefault));
@@ -723,10 +797,31 @@ void Solve::setup() {
   // renormalise to N_el (initial_density.rs:150-166)
   dots({DotTerm{rho_in.as<double>(), nullptr, nullptr, nullptr, 0}}, N);
   const double post = host_small[0] * (omega / (double)N);
-  const double sc = post > 1e-15 ? n_el / post : 1.0;
-  k_scale<<<gb(N), 256, 0, c.stream>>>(N, rho_in.as<double>(), sc);
+  const double scale = post > 1e-15 ? n_el / post : 1.0;
+  k_scale<<<gb(N), 256, 0, c.stream>>>(N, rho_in.as<double>(), scale);
   PWK_HIP(hipGetLastError());
-  r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
+  if (ns == 2) {
+    // split by the mean starting magnetisation over atoms (src/scf/driver_spin.rs:48-75)
+    double msum = 0.0;
+    for (int a = 0; a < cr.natoms; ++a) {
+      const double m = p.starting_magnetization[cr.species[a]];
+      PWK_REQUIRE(std::isfinite(m) && std::fabs(m) <= 1.0, PWK_E_INVALID, "starting magnetisation outside [-1, 1]");
+      msum += m;
+    }
+    k_spin_start<<<gb(N), 256, 0, c.stream>>>(N, rho_in.as<double>(), msum / (double)cr.natoms, rho_in_s(0),
+                                              rho_in_s(1));
+    PWK_HIP(hipGetLastError());
+  }
+  // Kerker q_TF² (src/scf/mixing/kerker.rs:50-62)
+  if (p.mixing_mode == 1) {
+    if (p.kerker_q_tf > 0.0) {
+      kerker_q2 = p.kerker_q_tf * p.kerker_q_tf;
+    } else {
+      const double bohr = 0.529177210903;
+      const double rho_bohr = (n_el / omega) * (bohr * bohr * bohr);
+      kerker_q2 = 4.0 * std::cbrt(3.0 * kPiC * kPiC * rho_bohr) / kPiC / (bohr * bohr);
+    }
+  }
 
   // per-k kinetic diagonal and KB projectors
   std::vector<int32_t> nproj((size_t)cr.nspecies), mesh((size_t)cr.nspecies), pl;
@@ -806,9 +901,9 @@ void Solve::setup() {
   // eigen-system buffers; the filtered solver carries buffer bands above the nb used for
   // occupations so that the filter's damped interval starts well above them
   nbs = chebyshev ? std::min(npw, nb + std::max(8, nb / 4)) : nb;
-  W.ensure((size_t)nk * nb * 8);
-  V.ensure((size_t)nk * npw * nbs * 16);
-  if (p.precision != PWK_FP64) V32.ensure((size_t)nk * npw * nbs * 8);
+  W.ensure((size_t)ns * nk * nb * 8);
+  V.ensure((size_t)ns * nk * npw * nbs * 16);
+  if (p.precision != PWK_FP64) V32.ensure((size_t)ns * nk * npw * nbs * 8);
   if (!chebyshev) {
     Hb.ensure((size_t)nk * npw * npw * 16);
     c.eig_work.ensure(heev_work_doubles(npw, nb, nk) * 8);
@@ -828,9 +923,9 @@ void Solve::setup() {
     s_lam.ensure((size_t)nk * nbs * 8);
     s_work.ensure(heev_work_doubles(nbs, nbs, nk) * 8);
     projall.ensure((size_t)nk * std::max(1, nch_all) * nbs * 16);
-    ev_sub.assign((size_t)nk * nbs, 0.0);
+    ev_sub.assign((size_t)ns * nk * nbs, 0.0);
     coef.ensure((size_t)std::max(degree, 1) * nk * 3 * 8);
-    ub.assign((size_t)nk, 0.0);
+    ub.assign((size_t)ns * nk, 0.0);
     // start block: the nb lowest-kinetic plane waves of each k (plus a seeded perturbation)
     lowest.ensure((size_t)nbs * nk * 4);
     std::vector<int> low((size_t)nbs * nk);
@@ -842,6 +937,8 @@ void Solve::setup() {
     }
     PWK_HIP(hipMemcpyAsync(lowest.p, low.data(), low.size() * 4, hipMemcpyHostToDevice, c.stream));
     launch_start_block(c, npw, nbs, nk, lowest.as<int>(), V.p);
+    if (ns == 2)  // the same start block for the spin-down channel
+      PWK_HIP(hipMemcpyAsync(V.as<double2>() + vblk(), V.p, vblk() * 16, hipMemcpyDeviceToDevice, c.stream));
     PWK_HIP(hipStreamSynchronize(c.stream));
   }
   PWK_HIP(hipStreamSynchronize(c.stream));
@@ -904,11 +1001,86 @@ double Solve::ewald() {
 // V_eff(G) = V_loc + V_H[ρ] + FFT(v_xc[ρ + ρ_core])/N  (src/scf/driver.rs:359-431)
 void Solve::veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out,
                       double* vxc_out) {
-  k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_r, have_core ? core_r.as<double>() : nullptr, rho_xc_out);
+  k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_r, have_core ? core_r.as<double>() : nullptr, 1.0, rho_xc_out);
   PWK_HIP(hipGetLastError());
   launch_lda(c, rho_xc_out, exc_out, vxc_out, PWK_FP64);
   r_to_g(vxc_out, vxc_g.as<double2>());
   launch_hartree_veff(c, rho_g, 4.0 * kPiC * kE2, vloc_g.p, vxc_g.p, veff_g.p, PWK_FP64, true, true, true);
+}
+
+// LSDA on ρ_σ + ρ_core/2 per channel (src/scf/driver_spin.rs:273-318): ε_xc into exc, v_xc↑
+// and v_xc↓ into the two halves of vxc, and the clamped ρ_total + ρ_core into xc_tot (the
+// density the energy integrals weight ε_xc with)
+void Solve::xc_spin(const double* up, const double* dn) {
+  const double* core = have_core ? core_r.as<double>() : nullptr;
+  k_add_core<<<gb(N), 256, 0, c.stream>>>(N, up, core, 0.5, rho_xc.as<double>());
+  k_add_core<<<gb(N), 256, 0, c.stream>>>(N, dn, core, 0.5, rho_xc.as<double>() + N);
+  PWK_HIP(hipGetLastError());
+  launch_lsda(c, rho_xc.p, rho_xc.as<double>() + N, exc.p, vxc.p, vxc.as<double>() + N, PWK_FP64);
+}
+
+// One Anderson step (src/scf/mixing/anderson.rs:112-203): the input and its residual (Kerker-
+// preconditioned when enabled, mixing/kerker.rs:5-29) enter the history, the oldest entry
+// leaves past ndim, and result = c_last·(in_l + β r_l) + Σ_j α_j (in_j + β r_j). result may
+// alias in.
+void Solve::mix(Mixer& mx, const double* in, const double* out, double* result) {
+  TimedScope ts(c, "scf_mix");
+  int slot;
+  if ((int)mx.hslot.size() < mx.ndim) {
+    slot = mx.next_slot++;
+  } else {
+    slot = mx.hslot.front();
+    mx.hslot.erase(mx.hslot.begin());
+  }
+  mx.hslot.push_back(slot);
+  double* hin = mx.hist.as<double>() + (size_t)slot * 2 * N;
+  double* hres = hin + N;
+  PWK_HIP(hipMemcpyAsync(hin, in, N * 8, hipMemcpyDeviceToDevice, c.stream));
+  k_sub<<<gb(N), 256, 0, c.stream>>>(N, out, in, hres);
+  PWK_HIP(hipGetLastError());
+  if (mx.kerker) {
+    k_r2c_scaled<<<gb(N), 256, 0, c.stream>>>(N, hres, 1.0, cwork.as<double2>());
+    launch_fft3d(c, cwork.p, 1, -1, PWK_FP64);
+    k_kerker<<<gb(N), 256, 0, c.stream>>>(N, c.g2_64.as<double>(), kerker_q2, cwork.as<double2>());
+    launch_fft3d(c, cwork.p, 1, +1, PWK_FP64);
+    k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, cwork.as<double2>(), 1.0 / (double)N, hres, 0);
+    PWK_HIP(hipGetLastError());
+  }
+  const int m = (int)mx.hslot.size();
+  MixArgs ma{};
+  ma.m = m;
+  ma.beta = p.mixing_beta;
+  for (int j = 0; j < m; ++j) {
+    ma.in[j] = mx.hist.as<double>() + (size_t)mx.hslot[(size_t)j] * 2 * N;
+    ma.res[j] = ma.in[j] + N;
+  }
+  if (m < 2) {
+    ma.coef[0] = 1.0;
+  } else {
+    const int mm = m - 1;
+    const double* rl = ma.res[m - 1];
+    std::vector<DotTerm> t;
+    for (int i = 0; i < mm; ++i) t.push_back(DotTerm{ma.res[i], rl, rl, nullptr, 0});
+    for (int i = 0; i < mm; ++i)
+      for (int j = i; j < mm; ++j) t.push_back(DotTerm{ma.res[i], rl, ma.res[j], rl, 0});
+    dots(t, N);
+    std::vector<double> A((size_t)mm * mm), bvec((size_t)mm);
+    for (int i = 0; i < mm; ++i) bvec[(size_t)i] = -host_small[i];
+    int q = mm;
+    for (int i = 0; i < mm; ++i)
+      for (int j = i; j < mm; ++j) {
+        A[(size_t)i * mm + j] = host_small[q];
+        A[(size_t)j * mm + i] = host_small[q];
+        ++q;
+      }
+    std::vector<double> alpha = small_solve(A, bvec, mm);
+    double asum = 0.0;
+    for (double x : alpha) asum += x;
+    for (int j = 0; j < mm; ++j) ma.coef[j] = alpha[(size_t)j];
+    ma.coef[m - 1] = 1.0 - asum;
+  }
+  k_anderson_combine<<<gb(N), 256, 0, c.stream>>>(N, ma, result);
+  PWK_HIP(hipGetLastError());
 }
 
 // Rayleigh-Ritz on span(Y_k) for every k at once, FP64, Löwdin orthonormalisation:
@@ -930,8 +1102,8 @@ void Solve::rayleigh_ritz(const double2* Y, const double2* HY) {
   launch_hermitize(c, m, sX.p, nk);
   launch_heev_batched(c, m, m, nk, sX.p, s_work.p, s_lam.as<double>(), sW.p, nullptr);
   launch_zgemm_mfma_batched(c, m, m, m, sU.p, m, sm, sW.p, m, sm, sC.p, m, sm, nk, false, PWK_FP64);
-  launch_zgemm_mfma_batched(c, npw, m, m, Y, npw, blk, sC.p, m, sm, V.p, npw, blk, nk, false, PWK_FP64);
-  PWK_HIP(hipMemcpyAsync(ev_sub.data(), s_lam.p, (size_t)nk * nbs * 8, hipMemcpyDeviceToHost, c.stream));
+  launch_zgemm_mfma_batched(c, npw, m, m, Y, npw, blk, sC.p, m, sm, Vc(), npw, blk, nk, false, PWK_FP64);
+  PWK_HIP(hipMemcpyAsync(ev_sub_c(), s_lam.p, (size_t)nk * nbs * 8, hipMemcpyDeviceToHost, c.stream));
   PWK_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -998,7 +1170,7 @@ void Solve::lanczos_bounds() {
       const double r = std::fabs(i > 0 ? Bv[i - 1] : 0.0) + std::fabs(i + 1 < A.size() ? Bv[i] : 0.0);
       bound = std::max(bound, A[i] + r);
     }
-    ub[(size_t)k] = bound + std::fabs(Bv.back());
+    ub_c()[k] = bound + std::fabs(Bv.back());
   }
   PWK_HIP(hipStreamSynchronize(c.stream));
 }
@@ -1013,9 +1185,9 @@ double Solve::chebyshev_pass(int fprec) {
   const size_t blk = (size_t)nk * npw * nbs;
   coef_host.assign((size_t)degree * nk * 3, 0.0);
   for (int k = 0; k < nk; ++k) {
-    const double* lam = ev_sub.data() + (size_t)k * nbs;
+    const double* lam = ev_sub_c() + (size_t)k * nbs;
     const double a0 = lam[0];
-    double a = lam[nbs - 1], b = ub[(size_t)k];
+    double a = lam[nbs - 1], b = ub_c()[k];
     if (!(b > a + 1e-6)) b = a + std::max(1.0, std::fabs(a));
     if (!(a > a0)) a = a0 + 1e-3 * (b - a0);
     const double e = 0.5 * (b - a), cc = 0.5 * (b + a);
@@ -1036,9 +1208,9 @@ double Solve::chebyshev_pass(int fprec) {
   PWK_HIP(hipMemcpyAsync(coef.p, coef_host.data(), coef_host.size() * 8, hipMemcpyHostToDevice, c.stream));
   void* buf[3] = {F0.p, F1.p, F2.p};
   if (fprec == PWK_FP64)
-    PWK_HIP(hipMemcpyAsync(buf[0], V.p, blk * 16, hipMemcpyDeviceToDevice, c.stream));
+    PWK_HIP(hipMemcpyAsync(buf[0], Vc(), blk * 16, hipMemcpyDeviceToDevice, c.stream));
   else
-    launch_convert(c, V.p, buf[0], blk * 2, true);
+    launch_convert(c, Vc(), buf[0], blk * 2, true);
   void *X = buf[0], *Y = buf[1], *Z = buf[2];
   const size_t per = (size_t)npw * nbs;
   hpsi_all(X, HYf.p, nbs, fprec);
@@ -1060,7 +1232,7 @@ double Solve::chebyshev_pass(int fprec) {
   // residuals of the nb bands used for occupations: H·V = (H·Y)·C
   const long long bl = (long long)npw * nbs, sm = (long long)nbs * nbs;
   launch_zgemm_mfma_batched(c, npw, nbs, nbs, HY64.p, npw, bl, sC.p, nbs, sm, R64.p, npw, bl, nk, false, PWK_FP64);
-  launch_resid_norms(c, npw, nbs, nb, nk, V.p, R64.p, s_lam.as<double>(), resid.as<double>());
+  launch_resid_norms(c, npw, nbs, nb, nk, Vc(), R64.p, s_lam.as<double>(), resid.as<double>());
   PWK_HIP(hipMemcpyAsync(resid_host.data(), resid.p, resid_host.size() * 8, hipMemcpyDeviceToHost, c.stream));
   PWK_HIP(hipStreamSynchronize(c.stream));
   double worst = 0.0;
@@ -1072,7 +1244,7 @@ double Solve::chebyshev_pass(int fprec) {
 void Solve::eig_chebyshev(bool first) {
   if (first) {
     lanczos_bounds();
-    PWK_HIP(hipMemcpyAsync(Y64.p, V.p, (size_t)nk * npw * nbs * 16, hipMemcpyDeviceToDevice, c.stream));
+    PWK_HIP(hipMemcpyAsync(Y64.p, Vc(), vblk() * 16, hipMemcpyDeviceToDevice, c.stream));
     hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
     rayleigh_ritz(Y64.as<double2>(), HY64.as<double2>());
   }
@@ -1089,119 +1261,213 @@ void Solve::eig_chebyshev(bool first) {
     if (worst < target) break;
   }
   for (int k = 0; k < nk; ++k)
-    for (int b = 0; b < nb; ++b) (*ev_host)[(size_t)k * nb + b] = ev_sub[(size_t)k * nbs + b];
+    for (int b = 0; b < nb; ++b) ev_host[(size_t)k * nb + b] = ev_sub_c()[(size_t)k * nbs + b];
 }
 
 void Solve::run(double* eig_out, double* rho_out) {
   const auto t0 = std::chrono::steady_clock::now();
   setup();
   const double e_ewald = ewald();
-  const double spin = 2.0;
+  const double spin = ns == 2 ? 1.0 : 2.0;  // spin_factor = 2/nspin (src/scf/context.rs:188)
   const double dvol = omega / (double)N;
+  const double fourpi_e2_half_omega = 4.0 * kPiC * kE2 * 0.5 * omega;
   const int ndim = std::max(1, std::min(p.mixing_ndim, 16));
-  hist.ensure((size_t)2 * ndim * N * 8);
-  std::vector<int> hslot;  // logical history order → slot
-  int next_slot = 0;
+  for (Mixer* m : {&mix_tot, &mix_mag}) {
+    m->ndim = ndim;
+    m->hslot.clear();
+    m->next_slot = 0;
+  }
+  mix_tot.hist.ensure((size_t)2 * ndim * N * 8);
+  mix_tot.kerker = p.mixing_mode == 1;
+  if (ns == 2) mix_mag.hist.ensure((size_t)2 * ndim * N * 8);  // always plain (driver_spin.rs:76-98)
+  const double* core = have_core ? core_r.as<double>() : nullptr;
+  const int nkb = ns * nk;  // (channel, k) entries, spin-up first
+  std::vector<double> wk_all((size_t)nkb);
+  for (int e = 0; e < nkb; ++e) wk_all[(size_t)e] = wk[(size_t)(e % nk)];
 
-  std::vector<double> ev((size_t)nk * nb), evals_final;
+  std::vector<double> ev((size_t)nkb * nb);
   bool have_prev = false;
   double e_prev = 0.0;
   res = pwk_scf_result{};
   res.nk = nk;
   res.n_bands = nb;
   res.npw = npw;
+  res.nspin = ns;
   for (int d = 0; d < 3; ++d) res.dims[d] = c.dims[d];
   res.e_ewald = e_ewald;
   res.n_ops = (int)ops.size();
 
   for (int iter = 0; iter < p.max_iter; ++iter) {
-    // ---- V_eff from the input density --------------------------------------------------
-    veff_from(rho_in.as<double>(), rho_g_in.as<double2>(), rho_xc.as<double>(), exc.as<double>(), vxc.as<double>());
-    // keep the input-side xc for the Harris energy: ρ_xc,in · ε_xc,in and ρ_in · v_xc,in
-    std::vector<DotTerm> harris = {DotTerm{rho_g_in.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
-                                   DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
-                                   DotTerm{rho_in.as<double>(), nullptr, vxc.as<double>(), nullptr, 0}};
-    dots(harris, N);
-    const double eh_in = host_small[0] * 4.0 * kPiC * kE2 * 0.5 * omega;
-    const double exc_in = host_small[1] * dvol - host_small[2] * dvol;
-    set_veff_device(c, veff_g.p);
-    if (chebyshev) {
-      // ---- Chebyshev-filtered subspace iteration on H·ψ, FP64 Rayleigh-Ritz -----------
-      ev_host = &ev;
-      eig_chebyshev(iter == 0);
+    // ---- V_eff per channel from the input density ---------------------------------------
+    double eh_in, exc_in;
+    if (ns == 1) {
+      r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
+      veff_from(rho_in.as<double>(), rho_g_in.as<double2>(), rho_xc.as<double>(), exc.as<double>(),
+                vxc.as<double>());
+      // input-side xc for the Harris energy: ρ_xc,in · ε_xc,in and ρ_in · v_xc,in
+      dots({DotTerm{rho_g_in.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+            DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+            DotTerm{rho_in.as<double>(), nullptr, vxc.as<double>(), nullptr, 0}},
+           N);
+      eh_in = host_small[0] * fourpi_e2_half_omega;
+      exc_in = host_small[1] * dvol - host_small[2] * dvol;
     } else {
-      // ---- dense H per k, one batched eigensolve ----------------------------------------
-      {
-        TimedScope ts(c, "scf_dense_h");
-        launch_dense_h_batched(c, nk, kin_all64.as<double>(), nch_all, Ball64.p, DBHall64.p, Hb.p);
+      // common Hartree from ρ↑ + ρ↓, LSDA per channel (src/scf/driver_spin.rs:263-339)
+      k_spin_combine<<<gb(N), 256, 0, c.stream>>>(N, rho_in_s(0), rho_in_s(1), t_in.as<double>(), m_in.as<double>());
+      PWK_HIP(hipGetLastError());
+      r_to_g(t_in.as<double>(), rho_g_in.as<double2>());
+      xc_spin(rho_in_s(0), rho_in_s(1));
+      k_add_core<<<gb(N), 256, 0, c.stream>>>(N, t_in.as<double>(), core, 1.0, xc_tot.as<double>());
+      PWK_HIP(hipGetLastError());
+      for (int s = 0; s < 2; ++s) {
+        r_to_g(vxc.as<double>() + (size_t)s * N, vxc_g.as<double2>());
+        launch_hartree_veff(c, rho_g_in.p, 4.0 * kPiC * kE2, vloc_g.p, vxc_g.p, veff_g.as<double2>() + (size_t)s * N,
+                            PWK_FP64, true, true, true);
       }
-      launch_heev_batched(c, npw, nb, nk, Hb.p, c.eig_work.p, W.as<double>(), V.p, nullptr);
-      PWK_HIP(hipMemcpyAsync(ev.data(), W.p, ev.size() * 8, hipMemcpyDeviceToHost, c.stream));
-      PWK_HIP(hipStreamSynchronize(c.stream));
+      // Harris side (driver_spin.rs:634-651): ∫(ρ_tot,in + ρ_core)·ε_xc,in − ∫Σ_σ ρ_σ,in·v_σ,in
+      dots({DotTerm{rho_g_in.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+            DotTerm{xc_tot.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+            DotTerm{rho_in_s(0), nullptr, vxc.as<double>(), nullptr, 0},
+            DotTerm{rho_in_s(1), nullptr, vxc.as<double>() + N, nullptr, 0}},
+           N);
+      eh_in = host_small[0] * fourpi_e2_half_omega;
+      exc_in = host_small[1] * dvol - (host_small[2] + host_small[3]) * dvol;
     }
-    // ---- occupations (src/scf/smearing.rs) -----------------------------------------------
-    const double ef = fermi_level(ev, nk, nb, wk, n_el, p.smearing_sigma, spin);
-    std::vector<double> occ((size_t)nk * nb);
-    for (size_t i = 0; i < occ.size(); ++i) occ[i] = fd_occ(ev[i], ef, p.smearing_sigma) * spin;
-    // ---- density (src/scf/density.rs:41-111) ----------------------------------------------
-    PWK_HIP(hipMemsetAsync(rho_new.p, 0, N * 8, c.stream));
-    if (p.precision != PWK_FP64) launch_convert(c, V.p, V32.p, (size_t)nk * npw * nbs * 2, true);
-    // the basis is the same G-sphere at every k, so all nk·nb bands go through one launch
-    std::vector<double> fw((size_t)nk * nbs, 0.0);  // buffer bands carry no weight
-    for (int k = 0; k < nk; ++k)
-      for (int b = 0; b < nb; ++b) fw[(size_t)k * nbs + b] = occ[(size_t)k * nb + b] * wk[(size_t)k];
-    {
-      TimedScope ts(c, "scf_density");
-      if (p.precision == PWK_FP64)
-        launch_density(c, nk * nbs, V.p, fw.data(), rho_new.p, PWK_FP64);
-      else
-        launch_density(c, nk * nbs, V32.p, fw.data(), rho_new.p, PWK_FP32);
+
+    // ---- eigenpairs per channel ------------------------------------------------------------
+    for (sc = 0; sc < ns; ++sc) {
+      set_veff_device(c, veff_g.as<double2>() + (size_t)sc * N);
+      double* evc = ev.data() + (size_t)sc * nk * nb;
+      if (chebyshev) {
+        // Chebyshev-filtered subspace iteration on H·ψ, FP64 Rayleigh-Ritz
+        ev_host = evc;
+        eig_chebyshev(iter == 0);
+      } else {
+        // dense H per k, one batched eigensolve
+        {
+          TimedScope ts(c, "scf_dense_h");
+          launch_dense_h_batched(c, nk, kin_all64.as<double>(), nch_all, Ball64.p, DBHall64.p, Hb.p);
+        }
+        launch_heev_batched(c, npw, nb, nk, Hb.p, c.eig_work.p, W.as<double>() + (size_t)sc * nk * nb, Vc(),
+                            nullptr);
+        PWK_HIP(hipMemcpyAsync(evc, W.as<double>() + (size_t)sc * nk * nb, (size_t)nk * nb * 8,
+                               hipMemcpyDeviceToHost, c.stream));
+        PWK_HIP(hipStreamSynchronize(c.stream));
+      }
     }
-    dots({DotTerm{rho_new.as<double>(), nullptr, nullptr, nullptr, 0}}, N);
-    const double integral = host_small[0] * dvol;
-    if (std::fabs(integral) > 1e-15) {
-      const double sc = n_el / integral;
-      k_scale<<<gb(N), 256, 0, c.stream>>>(N, rho_new.as<double>(), sc);
-      PWK_HIP(hipGetLastError());
+    sc = 0;
+
+    // ---- Fermi level(s) and occupations (src/scf/smearing.rs, driver_spin.rs:407-474) ------
+    double ef, ef_s[2];
+    if (ns == 2 && p.fixed_magnetization) {
+      const std::vector<double> eu(ev.begin(), ev.begin() + (long)nk * nb), ed(ev.begin() + (long)nk * nb, ev.end());
+      ef_s[0] = fermi_level(eu, nk, nb, wk, 0.5 * (n_el + p.tot_magnetization), p.smearing_sigma, spin);
+      ef_s[1] = fermi_level(ed, nk, nb, wk, (n_el - p.tot_magnetization) / 2.0, p.smearing_sigma, spin);
+      ef = 0.5 * (ef_s[0] + ef_s[1]);
+    } else {
+      ef = fermi_level(ev, nkb, nb, wk_all, n_el, p.smearing_sigma, spin);
+      ef_s[0] = ef_s[1] = ef;
     }
-    // ---- symmetrisation ------------------------------------------------------------------
-    if (ops.size() > 1) {
-      TimedScope ts(c, "scf_symmetrize");
-      k_r2c_scaled<<<gb(N), 256, 0, c.stream>>>(N, rho_new.as<double>(), 1.0, cwork.as<double2>());
-      launch_fft3d(c, cwork.p, 1, -1, PWK_FP64);
-      k_sym_gather<<<gb(N), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
-                                                 sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
-                                                 rho_g_new.as<double2>(), 1.0 / (double)ops.size());
-      launch_fft3d(c, rho_g_new.p, 1, +1, PWK_FP64);
-      k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, rho_g_new.as<double2>(), 1.0 / (double)N, rho_new.as<double>(), 0);
-      PWK_HIP(hipGetLastError());
+    std::vector<double> occ((size_t)nkb * nb);
+    for (size_t i = 0; i < occ.size(); ++i) occ[i] = fd_occ(ev[i], ef_s[i / ((size_t)nk * nb)], p.smearing_sigma) * spin;
+    double n_s[2] = {0.0, 0.0};  // electrons per channel
+    for (int s = 0; s < ns; ++s)
+      for (int k = 0; k < nk; ++k)
+        for (int b = 0; b < nb; ++b) n_s[s] += occ[((size_t)s * nk + k) * nb + b] * wk[(size_t)k];
+
+    // ---- density per channel (src/scf/density.rs:41-111) -----------------------------------
+    if (p.precision != PWK_FP64) launch_convert(c, V.p, V32.p, (size_t)ns * vblk() * 2, true);
+    for (int s = 0; s < ns; ++s) {
+      double* rn = rho_new_s(s);
+      PWK_HIP(hipMemsetAsync(rn, 0, N * 8, c.stream));
+      // the basis is the same G-sphere at every k, so all nk·nb bands go through one launch
+      std::vector<double> fw((size_t)nk * nbs, 0.0);  // buffer bands carry no weight
+      for (int k = 0; k < nk; ++k)
+        for (int b = 0; b < nb; ++b)
+          fw[(size_t)k * nbs + b] = occ[((size_t)s * nk + k) * nb + b] * wk[(size_t)k];
+      {
+        TimedScope ts(c, "scf_density");
+        if (p.precision == PWK_FP64)
+          launch_density(c, nk * nbs, V.as<double2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP64);
+        else
+          launch_density(c, nk * nbs, V32.as<float2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP32);
+      }
+      dots({DotTerm{rn, nullptr, nullptr, nullptr, 0}}, N);
+      const double integral = host_small[0] * dvol;
+      if (std::fabs(integral) > 1e-15) {
+        const double target = ns == 2 ? n_s[s] : n_el;
+        k_scale<<<gb(N), 256, 0, c.stream>>>(N, rn, target / integral);
+        PWK_HIP(hipGetLastError());
+      }
+      // ---- symmetrisation (src/symmetry/density/g_space.rs) --------------------------------
+      if (ops.size() > 1) {
+        TimedScope ts(c, "scf_symmetrize");
+        k_r2c_scaled<<<gb(N), 256, 0, c.stream>>>(N, rn, 1.0, cwork.as<double2>());
+        launch_fft3d(c, cwork.p, 1, -1, PWK_FP64);
+        k_sym_gather<<<gb(N), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
+                                                   sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
+                                                   rho_g_new.as<double2>(), 1.0 / (double)ops.size());
+        launch_fft3d(c, rho_g_new.p, 1, +1, PWK_FP64);
+        k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, rho_g_new.as<double2>(), 1.0 / (double)N, rn, 0);
+        PWK_HIP(hipGetLastError());
+      }
     }
-    // ---- output-density energies ---------------------------------------------------------
-    r_to_g(rho_new.as<double>(), rho_g_new.as<double2>());
+
+    // ---- output-density energies -------------------------------------------------------------
     // output-side xc reuses the work arrays (input-side values were reduced above)
-    k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_new.as<double>(), have_core ? core_r.as<double>() : nullptr,
-                                            rho_xc.as<double>());
-    launch_lda(c, rho_xc.p, exc.p, vxc.p, PWK_FP64);
-    std::vector<DotTerm> out_terms = {
-        DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
-        DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
-        DotTerm{rho_new.as<double>(), nullptr, vxc.as<double>(), nullptr, 0},
-        DotTerm{rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>(), rho_new.as<double>(), 0},
-        DotTerm{rho_new.as<double>(), nullptr, vloc_r.as<double>(), nullptr, 0},
-        DotTerm{rho_new.as<double>(), nullptr, nullptr, nullptr, 0}};
-    dots(out_terms, N);
-    const double e_h_out = host_small[0] * 4.0 * kPiC * kE2 * 0.5 * omega;
-    const double exc_int = host_small[1] * dvol, evxc = host_small[2] * dvol;
-    const double delta = std::sqrt(host_small[3] * dvol / omega);
-    const double e_loc = host_small[4] * dvol;
-    const double charge = host_small[5] * dvol;
+    double e_h_out, exc_int, evxc, delta, e_loc, charge;
+    const double* rho_tot_new;
+    if (ns == 1) {
+      rho_tot_new = rho_new.as<double>();
+      r_to_g(rho_tot_new, rho_g_new.as<double2>());
+      k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_tot_new, core, 1.0, rho_xc.as<double>());
+      launch_lda(c, rho_xc.p, exc.p, vxc.p, PWK_FP64);
+      dots({DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+            DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+            DotTerm{rho_new.as<double>(), nullptr, vxc.as<double>(), nullptr, 0},
+            DotTerm{rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>(), rho_new.as<double>(), 0},
+            DotTerm{rho_new.as<double>(), nullptr, vloc_r.as<double>(), nullptr, 0},
+            DotTerm{rho_new.as<double>(), nullptr, nullptr, nullptr, 0}},
+           N);
+      e_h_out = host_small[0] * fourpi_e2_half_omega;
+      exc_int = host_small[1] * dvol;
+      evxc = host_small[2] * dvol;
+      delta = std::sqrt(host_small[3] * dvol / omega);
+      e_loc = host_small[4] * dvol;
+      charge = host_small[5] * dvol;
+    } else {
+      // driver_spin.rs:525-604: Hartree of ρ_total, LSDA on the output channels, Δρ per channel
+      rho_tot_new = t_out.as<double>();
+      k_spin_combine<<<gb(N), 256, 0, c.stream>>>(N, rho_new_s(0), rho_new_s(1), t_out.as<double>(),
+                                                  m_out.as<double>());
+      PWK_HIP(hipGetLastError());
+      r_to_g(rho_tot_new, rho_g_new.as<double2>());
+      xc_spin(rho_new_s(0), rho_new_s(1));
+      k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_tot_new, core, 1.0, xc_tot.as<double>());
+      PWK_HIP(hipGetLastError());
+      dots({DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+            DotTerm{xc_tot.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+            DotTerm{rho_new_s(0), nullptr, vxc.as<double>(), nullptr, 0},
+            DotTerm{rho_new_s(1), nullptr, vxc.as<double>() + N, nullptr, 0},
+            DotTerm{rho_in_s(0), rho_new_s(0), rho_in_s(0), rho_new_s(0), 0},
+            DotTerm{rho_in_s(1), rho_new_s(1), rho_in_s(1), rho_new_s(1), 0},
+            DotTerm{rho_tot_new, nullptr, vloc_r.as<double>(), nullptr, 0},
+            DotTerm{rho_tot_new, nullptr, nullptr, nullptr, 0}},
+           N);
+      e_h_out = host_small[0] * fourpi_e2_half_omega;
+      exc_int = host_small[1] * dvol;
+      evxc = (host_small[2] + host_small[3]) * dvol;
+      delta = std::max(std::sqrt(host_small[4] * dvol / omega), std::sqrt(host_small[5] * dvol / omega));
+      e_loc = host_small[6] * dvol;
+      charge = host_small[7] * dvol;
+    }
     double e_band = 0.0;
-    for (int k = 0; k < nk; ++k) {
+    for (int e = 0; e < nkb; ++e) {
       double sk = 0.0;
-      for (int b = 0; b < nb; ++b) sk += occ[(size_t)k * nb + b] * wk[(size_t)k] * ev[(size_t)k * nb + b];
+      for (int b = 0; b < nb; ++b) sk += occ[(size_t)e * nb + b] * wk_all[(size_t)e] * ev[(size_t)e * nb + b];
       e_band += sk;
     }
-    const double ts = smearing_ts(ev, nk, nb, wk, ef, p.smearing_sigma, spin);
+    const double ts = smearing_ts(ev, nkb, nb, wk_all, ef, p.smearing_sigma, spin);
     const double e_xc_out = exc_int - evxc;
     const double e_total = e_band - e_h_out + e_xc_out + e_ewald + (-ts);
     const double e_harris = e_band - eh_in + exc_in + e_ewald + (-ts);
@@ -1213,8 +1479,8 @@ void Solve::run(double* eig_out, double* rho_out) {
     res.final_delta = delta;
     last_delta = delta;
     if (p.verbose)
-      std::fprintf(stderr, "[pwk scf] iter %3d  E=%.10f  dE=%.3e  delta=%.3e  EF=%.6f\n", iter, e_total, de, delta,
-                   ef);
+      std::fprintf(stderr, "[pwk scf] iter %3d  E=%.10f  dE=%.3e  delta=%.3e  EF=%.6f%s\n", iter, e_total, de, delta,
+                   ef, ns == 2 ? (std::string("  M=") + std::to_string(n_s[0] - n_s[1])).c_str() : "");
     if (delta < p.conv_threshold && e_conv) {
       res.converged = 1;
       res.e_total = e_total;
@@ -1228,94 +1494,57 @@ void Solve::run(double* eig_out, double* rho_out) {
       res.e_smearing = -ts;
       res.e_local = e_loc;
       res.integrated_charge = charge;
-      // kinetic and nonlocal expectation values (src/scf/energy.rs:382-510)
+      if (ns == 2) {
+        res.n_up = n_s[0];
+        res.n_down = n_s[1];
+        res.magnetization = std::fabs(n_s[0] - n_s[1]);
+      }
+      // kinetic and nonlocal expectation values, summed over channels (src/scf/energy.rs:382-510)
       nl_x.ensure((size_t)npw * nb * 16);
       expect.ensure((size_t)2 * nb * 8);
       std::vector<double> e2((size_t)2 * nb);
-      for (int k = 0; k < nk; ++k) {
-        KData& kd = c.k(k);
-        const double2* psi = V.as<double2>() + (size_t)k * npw * nbs;
-        if (kd.nch > 0) {
-          nl_p.ensure((size_t)kd.nch * nb * 16);
-          launch_zgemm_mfma(c, kd.nch, nb, npw, DBH_of(k), kd.nch, psi, npw, nl_p.p, kd.nch, false, PWK_FP64);
-          launch_zgemm_mfma(c, npw, nb, kd.nch, B_of(k), npw, nl_p.p, kd.nch, nl_x.p, npw, false, PWK_FP64);
-        } else {
-          PWK_HIP(hipMemsetAsync(nl_x.p, 0, (size_t)npw * nb * 16, c.stream));
+      for (int s = 0; s < ns; ++s)
+        for (int k = 0; k < nk; ++k) {
+          KData& kd = c.k(k);
+          const double2* psi = V.as<double2>() + (size_t)s * vblk() + (size_t)k * npw * nbs;
+          if (kd.nch > 0) {
+            nl_p.ensure((size_t)kd.nch * nb * 16);
+            launch_zgemm_mfma(c, kd.nch, nb, npw, DBH_of(k), kd.nch, psi, npw, nl_p.p, kd.nch, false, PWK_FP64);
+            launch_zgemm_mfma(c, npw, nb, kd.nch, B_of(k), npw, nl_p.p, kd.nch, nl_x.p, npw, false, PWK_FP64);
+          } else {
+            PWK_HIP(hipMemsetAsync(nl_x.p, 0, (size_t)npw * nb * 16, c.stream));
+          }
+          k_band_expect<<<(unsigned)nb, 256, 0, c.stream>>>(npw, psi, kd.kin64.as<double>(), nl_x.as<double2>(),
+                                                           expect.as<double>());
+          PWK_HIP(hipGetLastError());
+          PWK_HIP(hipMemcpyAsync(e2.data(), expect.p, e2.size() * 8, hipMemcpyDeviceToHost, c.stream));
+          PWK_HIP(hipStreamSynchronize(c.stream));
+          double ek = 0.0, enl = 0.0;
+          for (int b = 0; b < nb; ++b) {
+            const double f = occ[((size_t)s * nk + k) * nb + b];
+            if (f == 0.0) continue;
+            ek += f * wk[(size_t)k] * e2[(size_t)2 * b];
+            enl += f * wk[(size_t)k] * e2[(size_t)2 * b + 1];
+          }
+          res.e_kinetic += ek;
+          res.e_nonlocal += enl;
         }
-        k_band_expect<<<(unsigned)nb, 256, 0, c.stream>>>(npw, psi, kd.kin64.as<double>(), nl_x.as<double2>(),
-                                                         expect.as<double>());
-        PWK_HIP(hipGetLastError());
-        PWK_HIP(hipMemcpyAsync(e2.data(), expect.p, e2.size() * 8, hipMemcpyDeviceToHost, c.stream));
-        PWK_HIP(hipStreamSynchronize(c.stream));
-        double ek = 0.0, enl = 0.0;
-        for (int b = 0; b < nb; ++b) {
-          const double f = occ[(size_t)k * nb + b];
-          if (f == 0.0) continue;
-          ek += f * wk[(size_t)k] * e2[(size_t)2 * b];
-          enl += f * wk[(size_t)k] * e2[(size_t)2 * b + 1];
-        }
-        res.e_kinetic += ek;
-        res.e_nonlocal += enl;
-      }
       if (eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);
       if (rho_out) {
-        PWK_HIP(hipMemcpyAsync(rho_out, rho_new.p, N * 8, hipMemcpyDeviceToHost, c.stream));
+        PWK_HIP(hipMemcpyAsync(rho_out, rho_tot_new, N * 8, hipMemcpyDeviceToHost, c.stream));
         PWK_HIP(hipStreamSynchronize(c.stream));
       }
       break;
     }
-    // ---- Anderson mixing (src/scf/mixing/anderson.rs:170-266) --------------------------------
-    {
-      TimedScope ts(c, "scf_mix");
-      int slot;
-      if ((int)hslot.size() < ndim) {
-        slot = next_slot++;
-      } else {
-        slot = hslot.front();
-        hslot.erase(hslot.begin());
-      }
-      hslot.push_back(slot);
-      double* hin = hist.as<double>() + (size_t)slot * 2 * N;
-      double* hres = hin + N;
-      PWK_HIP(hipMemcpyAsync(hin, rho_in.p, N * 8, hipMemcpyDeviceToDevice, c.stream));
-      k_sub<<<gb(N), 256, 0, c.stream>>>(N, rho_new.as<double>(), rho_in.as<double>(), hres);
+    // ---- mixing: ρ (nspin = 1), or ρ_total and m with separate histories (CCMX,
+    // src/scf/driver_spin.rs:800-832) ---------------------------------------------------------
+    if (ns == 1) {
+      mix(mix_tot, rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>());
+    } else {
+      mix(mix_tot, t_in.as<double>(), t_out.as<double>(), t_in.as<double>());
+      mix(mix_mag, m_in.as<double>(), m_out.as<double>(), m_in.as<double>());
+      k_spin_split<<<gb(N), 256, 0, c.stream>>>(N, t_in.as<double>(), m_in.as<double>(), rho_in_s(0), rho_in_s(1));
       PWK_HIP(hipGetLastError());
-      const int m = (int)hslot.size();
-      MixArgs ma{};
-      ma.m = m;
-      ma.beta = p.mixing_beta;
-      for (int j = 0; j < m; ++j) {
-        ma.in[j] = hist.as<double>() + (size_t)hslot[(size_t)j] * 2 * N;
-        ma.res[j] = ma.in[j] + N;
-      }
-      if (m < 2) {
-        ma.coef[0] = 1.0;
-      } else {
-        const int mm = m - 1;
-        const double* rl = ma.res[m - 1];
-        std::vector<DotTerm> t;
-        for (int i = 0; i < mm; ++i) t.push_back(DotTerm{ma.res[i], rl, rl, nullptr, 0});
-        for (int i = 0; i < mm; ++i)
-          for (int j = i; j < mm; ++j) t.push_back(DotTerm{ma.res[i], rl, ma.res[j], rl, 0});
-        dots(t, N);
-        std::vector<double> A((size_t)mm * mm), bvec((size_t)mm);
-        for (int i = 0; i < mm; ++i) bvec[(size_t)i] = -host_small[i];
-        int q = mm;
-        for (int i = 0; i < mm; ++i)
-          for (int j = i; j < mm; ++j) {
-            A[(size_t)i * mm + j] = host_small[q];
-            A[(size_t)j * mm + i] = host_small[q];
-            ++q;
-          }
-        std::vector<double> alpha = small_solve(A, bvec, mm);
-        double asum = 0.0;
-        for (double x : alpha) asum += x;
-        for (int j = 0; j < mm; ++j) ma.coef[j] = alpha[(size_t)j];
-        ma.coef[m - 1] = 1.0 - asum;
-      }
-      k_anderson_combine<<<gb(N), 256, 0, c.stream>>>(N, ma, rho_in.as<double>());
-      PWK_HIP(hipGetLastError());
-      r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
     }
   }
   if (!res.converged && eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);

@@ -76,7 +76,8 @@ struct Server {
       c.cu_count = prop.multiProcessorCount;
       int npw, dims[3], nk, nops;
       scf_tables(job.cr, job.p, &npw, dims, &nk, &nops);
-      job.eig.assign((size_t)nk * (size_t)std::min(job.p.n_bands, npw), 0.0);
+      const size_t ns = job.p.nspin == 2 ? 2 : 1;  // eigenvalues of both spin channels
+      job.eig.assign(ns * (size_t)nk * (size_t)std::min(job.p.n_bands, npw), 0.0);
       job.rho.assign((size_t)dims[0] * dims[1] * dims[2], 0.0);
       scf_run(c, job.cr, job.p, job.res, job.eig.data(), job.rho.data());
     } catch (const Error& e) {
@@ -132,11 +133,12 @@ size_t scf_footprint(const pwk_crystal& cr, const pwk_scf_params& p) {
     const pwk_species& s = cr.species_data[cr.species[a]];
     for (int i = 0; i < s.nproj; ++i) nch += (size_t)(2 * s.proj_l[i] + 1);
   }
+  const size_t ns = p.nspin == 2 ? 2 : 1;
   size_t b = (size_t)nk * n * n * 16;                   // dense H batch
   b += heev_work_doubles(npw, (int)nb, nk) * 8;         // eigensolver scratch
-  b += (size_t)nk * n * nb * (16 + 8);                  // eigenvectors (FP64 + FP32 copy)
+  b += ns * nk * n * nb * (16 + 8);                     // eigenvectors (FP64 + FP32 copy) per channel
   b += (size_t)nk * n * nch * 16 * 3;                   // B, D·B^H (FP64) and FP32 copies
-  b += N * (16 * 10 + 8 * 12) + (size_t)2 * std::max(1, p.mixing_ndim) * N * 8;  // grids, mixer history
+  b += N * (16 * 10 + 8 * 12) * ns + ns * 2 * std::max(1, p.mixing_ndim) * N * 8;  // grids, mixer histories
   b += (size_t)nb * dims[2] * n * 16 + (64u << 20);     // stick buffers, slack
   return b;
 }

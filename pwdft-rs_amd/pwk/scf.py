@@ -1,5 +1,6 @@
 """SCF and integer-table entry points of libpwk, mirroring the reference's
-``run_scf`` (src/scf/driver.rs:353-735), ``BasisSet::new`` (src/basis.rs:36-63),
+``run_scf`` (src/scf/mod.rs:326-400, with src/scf/driver.rs:353-735 for nspin = 1 and
+src/scf/driver_spin.rs:37-839 for nspin = 2), ``BasisSet::new`` (src/basis.rs:36-63),
 ``find_symmetry_operations`` (src/symmetry/detect.rs:22-178) and
 ``reduce_kpoints`` (src/symmetry/kpoints.rs:31-143).
 
@@ -18,6 +19,7 @@ from .decks import read_upf
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
+MAX_SPECIES = 8  # PWK_MAX_SPECIES
 
 
 class Species(C.Structure):
@@ -37,7 +39,9 @@ class Params(C.Structure):
                 ("n_bands", C.c_int), ("max_iter", C.c_int), ("conv_threshold", C.c_double),
                 ("energy_threshold", C.c_double), ("mixing_beta", C.c_double), ("mixing_ndim", C.c_int),
                 ("smearing_sigma", C.c_double), ("sad_sigma", C.c_double), ("precision", C.c_int),
-                ("eigensolver", C.c_int), ("cheb_degree", C.c_int), ("verbose", C.c_int)]
+                ("eigensolver", C.c_int), ("cheb_degree", C.c_int), ("verbose", C.c_int), ("nspin", C.c_int),
+                ("starting_magnetization", C.c_double * MAX_SPECIES), ("fixed_magnetization", C.c_int),
+                ("tot_magnetization", C.c_double), ("mixing_mode", C.c_int), ("kerker_q_tf", C.c_double)]
 
 
 class Result(C.Structure):
@@ -45,7 +49,8 @@ class Result(C.Structure):
                 ("n_ops", C.c_int), ("dims", C.c_int * 3)] + [
         (n, C.c_double) for n in ("e_total", "e_harris", "fermi", "ts", "final_delta", "integrated_charge", "e_band",
                                   "e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_vxc", "e_ewald",
-                                  "e_smearing", "seconds")]
+                                  "e_smearing", "seconds")] + [
+        ("nspin", C.c_int), ("magnetization", C.c_double), ("n_up", C.c_double), ("n_down", C.c_double)]
 
 
 _SIGS = {
@@ -136,7 +141,12 @@ def kpoints(lattice, z, frac, grid, shift=(0, 0, 0), use_symmetry=True, symprec=
 # ---- SCF ------------------------------------------------------------------------------
 @dataclass
 class ScfParams:
-    """The ScfParams fields the nspin = 1 dense path reads (src/scf/mod.rs:72-402)."""
+    """The ScfParams fields the drivers read (src/scf/mod.rs:72-402).
+
+    nspin = 2 runs LSDA; ``starting_magnetization`` maps a species index to its
+    starting moment fraction (the reference keys it by element), and
+    ``tot_magnetization`` (not None) fixes the total moment. ``mixing_mode``
+    is "plain" or "kerker" (``kerker_q_tf`` ≤ 0: Thomas-Fermi estimate)."""
     ecut_ev: float
     kgrid: tuple = (1, 1, 1)
     kshift: tuple = (0, 0, 0)
@@ -156,6 +166,11 @@ class ScfParams:
     eigensolver: int = -1  # 0 dense, 1 Chebyshev-filtered subspace iteration, -1 by size
     cheb_degree: int = 0
     verbose: bool = False
+    nspin: int = 1
+    starting_magnetization: dict = field(default_factory=dict)
+    tot_magnetization: float | None = None
+    mixing_mode: str = "plain"
+    kerker_q_tf: float = 0.0
 
 
 class CrystalDeck:
@@ -187,11 +202,23 @@ class CrystalDeck:
         return np.array([self.upfs[s]["atomic_number"] for s in self.species], dtype=np.int32)
 
 
+_MIXING = {"plain": 0, "kerker": 1}
+
+
 def _params(p: ScfParams):
+    if p.mixing_mode not in _MIXING:
+        raise PwkError(1, f"mixing_mode must be one of {sorted(_MIXING)}")
+    mag = [0.0] * MAX_SPECIES
+    for sp, m in p.starting_magnetization.items():
+        if not 0 <= int(sp) < MAX_SPECIES:
+            raise PwkError(1, f"species index {sp} out of range")
+        mag[int(sp)] = float(m)
     return Params(p.ecut_ev, p.ecutrho_ratio, (C.c_int * 3)(*p.fft_dims), (C.c_int * 3)(*p.kgrid),
                   (C.c_int * 3)(*p.kshift), int(p.use_symmetry), p.symprec, p.n_bands, p.max_iter, p.conv_threshold,
                   p.energy_threshold, p.mixing_beta, p.mixing_ndim, p.smearing_sigma, p.sad_sigma, _prec(p.precision),
-                  int(p.eigensolver), int(p.cheb_degree), int(p.verbose))
+                  int(p.eigensolver), int(p.cheb_degree), int(p.verbose), int(p.nspin),
+                  (C.c_double * MAX_SPECIES)(*mag), int(p.tot_magnetization is not None),
+                  float(p.tot_magnetization or 0.0), _MIXING[p.mixing_mode], float(p.kerker_q_tf))
 
 
 def scf_tables(deck: CrystalDeck, p: ScfParams):
@@ -204,10 +231,11 @@ def scf_tables(deck: CrystalDeck, p: ScfParams):
 
 def run_scf(acc, deck: CrystalDeck, p: ScfParams):
     """One SCF solve on ``acc``'s device. Returns a dict of the ScfResult fields
-    plus ``eigenvalues`` (nk, n_bands) and ``rho`` (the converged density, grid order)."""
+    plus ``eigenvalues`` (nspin·nk, n_bands; spin-up k-points first) and ``rho``
+    (the converged total density, grid order)."""
     t = scf_tables(deck, p)
     nb = min(p.n_bands, t["npw"])
-    ev = np.zeros((t["nk"], nb))
+    ev = np.zeros(((2 if p.nspin == 2 else 1) * t["nk"], nb))
     rho = np.zeros(int(np.prod(t["dims"])))
     prm = _params(p)
     res = Result()
@@ -245,12 +273,13 @@ class Server:
         prm = _params(p)
         t = C.c_int64()
         _check(_lib().pwk_server_submit(self.h, C.byref(deck.c), C.byref(prm), C.byref(t)))
-        self._pending[t.value] = (deck, prm, scf_tables(deck, p), min(p.n_bands, scf_tables(deck, p)["npw"]))
+        tab = scf_tables(deck, p)
+        self._pending[t.value] = (deck, prm, tab, min(p.n_bands, tab["npw"]), 2 if p.nspin == 2 else 1)
         return t.value
 
     def wait(self, ticket):
-        deck, prm, t, nb = self._pending.pop(ticket)
-        ev = np.zeros((t["nk"], nb))
+        deck, prm, t, nb, ns = self._pending.pop(ticket)
+        ev = np.zeros((ns * t["nk"], nb))
         rho = np.zeros(int(np.prod(t["dims"])))
         res = Result()
         lat = C.c_double()

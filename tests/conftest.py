@@ -12,6 +12,7 @@ for p in (os.path.join(ROOT, "pwdft-rs_amd"), os.path.join(ROOT, "oracle")):
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 SI_UPF = os.path.join(GOLDEN, "pseudo", "Si.upf")
 C_UPF = os.path.join(GOLDEN, "pseudo", "C.upf")
+FE_UPF = os.path.join(GOLDEN, "pseudo", "Fe.upf")
 
 
 def pytest_configure(config):

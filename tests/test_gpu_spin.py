@@ -1,0 +1,177 @@
+"""Device LSDA SCF (pwk_scf_run with nspin = 2) and Kerker mixing against the
+reference's spin pins (tests/spin_polarization.rs, tests/qe_validation.rs:710-765)
+and the FP64 oracle on the same decks.
+
+Bars: the reference's own assertions for each deck, and against the oracle on
+the same inputs every energy term within 1e-5 Ry/atom, the iteration count
+within one, the magnetisation within 1e-6.
+"""
+import numpy as np
+import pytest
+
+from conftest import FE_UPF, SI_UPF
+
+pytestmark = pytest.mark.gpu
+
+RY = 13.605693122994
+TOL_TERM = 1e-5 * RY  # eV per atom
+SI_FRAC = [[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]]
+TERMS = ["e_total", "e_harris", "e_band", "e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_vxc",
+         "e_ewald", "e_smearing"]
+
+
+def fcc(a):
+    h = a / 2.0
+    return np.array([[0.0, h, h], [h, 0.0, h], [h, h, 0.0]])
+
+
+def fe_bcc():
+    a = 2.87
+    return a / 2 * np.array([[-1.0, 1.0, 1.0], [1.0, -1.0, 1.0], [1.0, 1.0, -1.0]]), np.zeros((1, 3))
+
+
+@pytest.fixture(scope="module")
+def scf_mod(pwk_mod):
+    from pwk import scf
+
+    return scf
+
+
+def _si_params(scf_mod, **kw):
+    base = dict(ecut_ev=100.0, fft_dims=(16, 16, 16), use_symmetry=False, n_bands=4, max_iter=40,
+                conv_threshold=1e-6, energy_threshold=1e-5, mixing_ndim=4, smearing_sigma=0.05)
+    base.update(kw)
+    return scf_mod.ScfParams(**base)
+
+
+def _si_oracle(oracle):
+    return oracle.System(fcc(5.431), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 100.0, dims=[16, 16, 16])
+
+
+def _compare(r, ro, natoms, iters=1):
+    print("gpu", r["e_total"], r["n_iter"], r["magnetization"], "oracle", ro["e_total"], ro["n_iter"],
+          ro["magnetization"], flush=True)
+    assert r["converged"] == 1 and ro["converged"] == 1.0
+    assert abs(r["n_iter"] - ro["n_iter"]) <= iters
+    for t in TERMS:
+        assert abs(r[t] - ro[t]) < TOL_TERM * natoms, t
+    assert abs(r["magnetization"] - ro["magnetization"]) < 1e-6
+
+
+def test_spin_unpolarised_limit(scf_mod, gpu, oracle):
+    # tests/spin_polarization.rs:54-139: nspin = 2 without a starting moment equals nspin = 1
+    # within 1e-5 eV, |M| < 1e-4
+    deck = scf_mod.CrystalDeck(fcc(5.431), SI_FRAC, [0, 0], [SI_UPF])
+    print("spin: Si Γ nspin 1 vs 2", flush=True)
+    r1 = scf_mod.run_scf(gpu, deck, _si_params(scf_mod))
+    r2 = scf_mod.run_scf(gpu, deck, _si_params(scf_mod, nspin=2))
+    assert r1["converged"] == 1 and r2["converged"] == 1 and r2["n_iter"] < 40
+    assert r2["nspin"] == 2 and r2["eigenvalues"].shape == (2, 4)
+    assert abs(r1["e_total"] - r2["e_total"]) < 1e-5
+    assert abs(r2["magnetization"]) < 1e-4
+    assert abs(r2["n_up"] + r2["n_down"] - 8.0) < 1e-6  # FD tails across the gap
+    assert np.abs(r2["eigenvalues"][0] - r2["eigenvalues"][1]).max() < 1e-6
+    ro = _si_oracle(oracle).scf([[0, 0, 0]], [1.0], n_bands=4, max_iter=40, conv=1e-6, ethr=1e-5, beta=0.3, ndim=4,
+                                sigma=0.05, nspin=2)
+    _compare(r2, ro, 2)
+    assert np.abs(r2["eigenvalues"] - ro["eigenvalues"]).max() < 1e-5
+
+
+@pytest.mark.parametrize("eigensolver,precision", [(0, "fp64"), (1, "fp64"), (1, "fp32")])
+def test_spin_xc_consistency(scf_mod, gpu, oracle, eigensolver, precision):
+    # tests/spin_polarization.rs:143-225: starting magnetisation 0.2, thresholds 1e-6:
+    # |E_HF - E_KS| < 1e-5 eV, M < 0.05; also through the filtered eigensolver, whose
+    # subspace state is per spin channel
+    deck = scf_mod.CrystalDeck(fcc(5.431), SI_FRAC, [0, 0], [SI_UPF])
+    p = _si_params(scf_mod, nspin=2, starting_magnetization={0: 0.2}, max_iter=60, energy_threshold=1e-6,
+                   eigensolver=eigensolver, precision=precision)
+    print("spin: Si Γ start 0.2", eigensolver, precision, flush=True)
+    r = scf_mod.run_scf(gpu, deck, p)
+    assert r["converged"] == 1
+    assert abs(r["e_harris"] - r["e_total"]) < 1e-5
+    assert r["magnetization"] < 0.05
+    ro = _si_oracle(oracle).scf([[0, 0, 0]], [1.0], n_bands=4, max_iter=60, conv=1e-6, ethr=1e-6, beta=0.3, ndim=4,
+                                sigma=0.05, nspin=2, start_mag=0.2)
+    _compare(r, ro, 2, iters=1 if eigensolver == 0 else 3)
+
+
+def test_kerker_mixing_nspin1_matches_oracle(scf_mod, gpu, oracle):
+    # Kerker-preconditioned Anderson (src/scf/mixing/anderson.rs:34-97, kerker.rs) on the Si deck
+    deck = scf_mod.CrystalDeck(fcc(5.431), SI_FRAC, [0, 0], [SI_UPF])
+    print("spin: Si Γ Kerker nspin 1", flush=True)
+    r = scf_mod.run_scf(gpu, deck, _si_params(scf_mod, mixing_mode="kerker"))
+    ro = _si_oracle(oracle).scf([[0, 0, 0]], [1.0], n_bands=4, max_iter=40, conv=1e-6, ethr=1e-5, beta=0.3, ndim=4,
+                                sigma=0.05, kerker=True)
+    _compare(r, ro, 2)
+    assert abs(r["e_total"] - (-213.028339)) < 1e-3
+
+
+def _fe_deck(scf_mod):
+    lat, frac = fe_bcc()
+    return scf_mod.CrystalDeck(lat, frac, [0], [FE_UPF])
+
+
+def test_fe_ccmx_kerker(scf_mod, gpu, oracle):
+    # tests/spin_polarization.rs:229-336: Fe BCC 15 Ry, 4×4×4 MP-1976 (full list, 48-op density
+    # symmetrisation), 8 bands, FD 0.02 Ry, Kerker, starting magnetisation 0.5, conv 1e-3
+    p = scf_mod.ScfParams(ecut_ev=15 * RY, kgrid=(4, 4, 4), kshift=(1, 1, 1), use_symmetry=2, n_bands=8, max_iter=80,
+                          conv_threshold=1e-3, energy_threshold=1e-3, mixing_ndim=8, smearing_sigma=0.02 * RY,
+                          nspin=2, starting_magnetization={0: 0.5}, mixing_mode="kerker")
+    print("spin: Fe BCC CCMX Kerker", flush=True)
+    r = scf_mod.run_scf(gpu, _fe_deck(scf_mod), p)
+    assert r["nk"] == 64 and r["n_ops"] == 48
+    assert r["converged"] == 1 and r["n_iter"] < 80
+    assert abs(r["e_harris"] - r["e_total"]) < 1e-3
+    assert r["magnetization"] < 0.05
+    assert r["final_delta"] < 1e-2
+    lat, frac = fe_bcc()
+    s = oracle.System(lat, [0], [26], frac, [FE_UPF], 15 * RY)
+    k, w = oracle.kpoints(lat, [26], frac, [4, 4, 4], (1, 1, 1), mode=0)
+    ro = s.scf(k, w, n_bands=8, max_iter=80, conv=1e-3, ethr=1e-3, beta=0.3, ndim=8, sigma=0.02 * RY, use_sym=True,
+               nspin=2, start_mag=0.5, kerker=True)
+    _compare(r, ro, 1)
+
+
+def test_fe_fixed_moment_does_not_converge(scf_mod, gpu):
+    # tests/spin_polarization.rs:340-430 (inverted detector): a fixed total moment of 2 has no SCF
+    # fixed point on this pseudopotential; 100 iterations end unconverged
+    p = scf_mod.ScfParams(ecut_ev=15 * RY, kgrid=(4, 4, 4), kshift=(1, 1, 1), use_symmetry=2, n_bands=8,
+                          max_iter=100, conv_threshold=1e-6, energy_threshold=1e-5, mixing_beta=0.2, mixing_ndim=8,
+                          smearing_sigma=0.02 * RY, nspin=2, tot_magnetization=2.0, mixing_mode="kerker")
+    print("spin: Fe BCC fixed moment 2", flush=True)
+    r = scf_mod.run_scf(gpu, _fe_deck(scf_mod), p)
+    assert r["converged"] == 0 and r["n_iter"] == 100
+    assert r["final_delta"] > 1e-6
+
+
+def test_fe_qe_config(scf_mod, gpu):
+    # tests/qe_validation.rs:710-765: 8×8×8 Γ-centred full grid, 12 bands, Kerker, conv 1e-8; the
+    # reference records E = -3049.016 eV with a non-magnetic collapse
+    p = scf_mod.ScfParams(ecut_ev=15 * RY, kgrid=(8, 8, 8), use_symmetry=2, n_bands=12, max_iter=80,
+                          conv_threshold=1e-8, energy_threshold=1e-6, mixing_ndim=8, smearing_sigma=0.02 * RY,
+                          nspin=2, starting_magnetization={0: 0.5}, mixing_mode="kerker")
+    print("spin: Fe BCC 8x8x8", flush=True)
+    r = scf_mod.run_scf(gpu, _fe_deck(scf_mod), p)
+    print("gpu", r["e_total"], r["n_iter"], r["magnetization"], r["seconds"], flush=True)
+    assert r["nk"] == 512 and r["converged"] == 1
+    assert abs(r["e_total"] - (-3049.016)) < 1e-3
+    assert r["magnetization"] < 1e-3
+
+
+def test_served_spin_equals_alone(scf_mod, gpu):
+    # an LSDA request in a served mix gives the result it gives alone
+    deck = scf_mod.CrystalDeck(fcc(5.431), SI_FRAC, [0, 0], [SI_UPF])
+    p_spin = _si_params(scf_mod, nspin=2, starting_magnetization={0: 0.2})
+    p_plain = _si_params(scf_mod)
+    alone = scf_mod.run_scf(gpu, deck, p_spin)
+    srv = scf_mod.Server(0, n_slots=3)
+    try:
+        tickets = [srv.submit(deck, p) for p in (p_spin, p_plain, p_spin)]
+        outs = [srv.wait(t) for t in tickets]
+    finally:
+        srv.close()
+    for o in (outs[0], outs[2]):
+        assert o["converged"] == 1 and o["nspin"] == 2
+        assert abs(o["e_total"] - alone["e_total"]) < 1e-9
+        assert np.abs(o["eigenvalues"] - alone["eigenvalues"]).max() < 1e-9
+    assert outs[1]["nspin"] == 1 and abs(outs[1]["e_total"] - (-213.028339)) < 1e-3

@@ -14,7 +14,34 @@ def test_library_exports_every_header_symbol(pwk_mod):
     L = ctypes.CDLL(pwk_mod.LIB_PATH)
     missing = [s for s in syms if not hasattr(L, s)]
     assert missing == []
-    assert pwk_mod.lib().pwk_version() == 100
+    assert pwk_mod.lib().pwk_version() == 101
+
+
+def test_struct_layout_matches_header(pwk_mod, tmp_path):
+    """The ctypes mirrors of the ABI structs have the C compiler's offsets and sizes."""
+    import subprocess
+    from pwk import scf
+
+    structs = {"pwk_species": scf.Species, "pwk_crystal": scf.Crystal, "pwk_scf_params": scf.Params,
+               "pwk_scf_result": scf.Result}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pwk.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(pwk_mod.HEADER), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, val = line.split()
+        got[(cname, field)] = int(val)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
 
 def test_status_codes_match_header(pwk_mod):

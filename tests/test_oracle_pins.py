@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, SI_UPF, C_UPF
+from conftest import GOLDEN, SI_UPF, C_UPF, FE_UPF
 
 A_SI = 5.431
 
@@ -315,6 +315,92 @@ def test_scf_qe_gamma_centred_pin(oracle):
     assert abs(r["e_total"] - (-231.6544)) < 1e-3
     # QE reference (data/qe/reference_data.toml) within the reference's 60 meV
     assert abs(r["e_total"] - (-17.02299344 * ry)) < 0.060
+
+
+# ---- nspin = 2 (src/scf/driver_spin.rs) ---------------------------------------------------
+def _si_gamma(oracle):
+    return oracle.System(si_lat(), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 100.0, dims=[16, 16, 16])
+
+
+def test_scf_spin_unpolarised_limit_pin(oracle):
+    # tests/spin_polarization.rs:54-139: Si nspin = 2 with no starting moment matches nspin = 1
+    # within 1e-5 eV and relaxes to |M| < 1e-4
+    s = _si_gamma(oracle)
+    kw = dict(n_bands=4, max_iter=40, conv=1e-6, ethr=1e-5, beta=0.3, ndim=4, sigma=0.05)
+    r1 = s.scf([[0, 0, 0]], [1.0], **kw)
+    r2 = s.scf([[0, 0, 0]], [1.0], nspin=2, **kw)
+    assert r1["converged"] == 1.0 and r2["converged"] == 1.0 and r2["n_iter"] < 40
+    assert abs(r1["e_total"] - r2["e_total"]) < 1e-5
+    assert abs(r2["magnetization"]) < 1e-4
+    # both channels carry the unpolarised eigenvalues
+    assert np.allclose(r2["eigenvalues"][0], r1["eigenvalues"][0], atol=1e-6)
+    assert np.allclose(r2["eigenvalues"][1], r1["eigenvalues"][0], atol=1e-6)
+
+
+def test_scf_spin_xc_consistency_pin(oracle):
+    # tests/spin_polarization.rs:143-225: Si nspin = 2 from starting magnetisation 0.2, conv and
+    # energy thresholds 1e-6: |E_HF - E_KS| < 1e-5 eV (O(Δρ²)), M < 0.05
+    s = _si_gamma(oracle)
+    r = s.scf([[0, 0, 0]], [1.0], n_bands=4, max_iter=60, conv=1e-6, ethr=1e-6, beta=0.3, ndim=4, sigma=0.05,
+              nspin=2, start_mag=0.2)
+    assert r["converged"] == 1.0
+    assert abs(r["e_harris"] - r["e_total"]) < 1e-5
+    assert r["magnetization"] < 0.05
+    # a wrong channel weighting (spin factor 2, or ρ_core added in full per channel) moves E by eV
+    assert abs(r["e_total"] - (-213.028339)) < 1e-3
+
+
+def _fe_bcc():
+    a = 2.87
+    return a / 2 * np.array([[-1.0, 1.0, 1.0], [1.0, -1.0, 1.0], [1.0, 1.0, -1.0]]), np.zeros((1, 3))
+
+
+def test_scf_fe_ccmx_kerker_pin(oracle):
+    # tests/spin_polarization.rs:229-336: Fe BCC 15 Ry, 4×4×4 MP-1976 (full list, 48-op density
+    # symmetrisation), 8 bands, FD 0.02 Ry, Kerker, starting magnetisation 0.5, conv 1e-3:
+    # converges (recorded: ~14 iterations, |HF-KS| ~ 1e-4 eV, M ~ 0), |HF-KS| < 1e-3, M < 0.05,
+    # final Δρ < 1e-2 (the pre-coupled-channel mixer locked at 0.254)
+    ry = oracle.const("RY_TO_EV")
+    lat, frac = _fe_bcc()
+    s = oracle.System(lat, [0], [26], frac, [FE_UPF], 15 * ry)
+    k, w = oracle.kpoints(lat, [26], frac, [4, 4, 4], (1, 1, 1), mode=0)
+    assert len(w) == 64
+    r = s.scf(k, w, n_bands=8, max_iter=80, conv=1e-3, ethr=1e-3, beta=0.3, ndim=8, sigma=0.02 * ry, use_sym=True,
+              nspin=2, start_mag=0.5, kerker=True)
+    assert r["converged"] == 1.0 and r["n_iter"] < 80
+    assert 10 <= r["n_iter"] <= 18
+    assert abs(r["e_harris"] - r["e_total"]) < 1e-3
+    assert r["magnetization"] < 0.05
+    assert r["final_delta"] < 1e-2
+
+
+def test_scf_fe_fixed_moment_does_not_converge(oracle):
+    # tests/spin_polarization.rs:340-430 (inverted detector): Fe BCC fixed total moment 2 on this
+    # pseudopotential has no SCF fixed point; 100 iterations end unconverged
+    ry = oracle.const("RY_TO_EV")
+    lat, frac = _fe_bcc()
+    s = oracle.System(lat, [0], [26], frac, [FE_UPF], 15 * ry)
+    k, w = oracle.kpoints(lat, [26], frac, [4, 4, 4], (1, 1, 1), mode=0)
+    r = s.scf(k, w, n_bands=8, max_iter=100, conv=1e-6, ethr=1e-5, beta=0.2, ndim=8, sigma=0.02 * ry, use_sym=True,
+              nspin=2, tot_mag=2.0, kerker=True)
+    assert r["converged"] == 0.0
+    assert r["final_delta"] > 1e-6
+
+
+def test_scf_fe_qe_config_pin(oracle):
+    # tests/qe_validation.rs:710-765: Fe BCC nspin = 2, 8×8×8 Γ-centred full grid, 15 Ry, 12 bands,
+    # Kerker, FD 0.02 Ry, conv 1e-8; the reference records E = -3049.016 eV (its 11.14 eV gap
+    # to QE is the reference's own, tracked there) and a non-magnetic collapse
+    ry = oracle.const("RY_TO_EV")
+    lat, frac = _fe_bcc()
+    s = oracle.System(lat, [0], [26], frac, [FE_UPF], 15 * ry)
+    k, w = oracle.kpoints(lat, [26], frac, [8, 8, 8], (0, 0, 0), mode=0)
+    r = s.scf(k, w, n_bands=12, max_iter=80, conv=1e-8, ethr=1e-6, beta=0.3, ndim=8, sigma=0.02 * ry, use_sym=True,
+              nspin=2, start_mag=0.5, kerker=True)
+    assert r["converged"] == 1.0
+    assert abs(r["e_total"] - (-3049.016)) < 1e-3
+    assert r["magnetization"] < 1e-3
+    assert abs(r["e_total"] - (-224.91744934 * ry)) < 12.0  # the reference's own QE tolerance
 
 
 def test_ewald_closed_form_independent(oracle):
