@@ -97,7 +97,7 @@ def test_lsda_xc(acc, si_sys, oracle):
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 3e-6)])
 def test_fft3d_matches_dft(gpu, prec, tol):
     rng = np.random.default_rng(4)
-    for dims in [(15, 15, 15), (16, 18, 20), (24, 25, 27), (60, 60, 60)]:
+    for dims in [(15, 15, 15), (16, 18, 20), (24, 25, 27), (60, 60, 60), (72, 75, 80), (81, 81, 64)]:
         gpu.prepare_buffers(dims, np.zeros(int(np.prod(dims))))
         x = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
         f = gpu.fft3d(x.reshape(-1), -1, prec).reshape(dims)
@@ -127,6 +127,28 @@ def ham(acc, si_sys, oracle):
     acc.set_veff(veff)
     B, DBH, _ = si_sys.nonlocal_(k)
     return dict(veff=veff, k=k, B=B, DBH=DBH)
+
+
+@pytest.mark.parametrize("dims", [(72, 72, 75), (80, 80, 81), (81, 81, 72)])
+def test_hpsi_on_large_grids(gpu, oracle, dims):
+    # the largest compiled line lengths through the z-stick and plane kernels: Si FCC at 150 eV
+    # on a forced grid (no aliasing: the product is the same as on the rule's grid)
+    s = oracle.System(si_lat(), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 150.0, dims=list(dims))
+    print("hpsi grid", dims, "npw", s.npw, flush=True)
+    gpu.prepare_buffers(s.dims, s.g2())
+    gpu.set_basis(s.miller())
+    gpu.set_nk(1)
+    k = np.array([0.07, 0.02, -0.11])
+    gpu.set_kinetic(0, s.kinetic(k))
+    rng = np.random.default_rng(11)
+    veff = np.fft.fftn(rng.uniform(-1, 1, s.dims) * 13.6).reshape(-1) / s.n_grid
+    gpu.set_veff(veff)
+    B, DBH, _ = s.nonlocal_(k)
+    gpu.set_nonlocal(0, B, DBH)
+    psi = rng.standard_normal((3, s.npw)) + 1j * rng.standard_normal((3, s.npw))
+    ref = s.hpsi_with_nl(k, veff, B, DBH, psi)
+    assert rel(gpu.hpsi(0, psi, "fp64"), ref) < 1e-12
+    assert rel(gpu.hpsi(0, psi, "fp32"), ref) < 2e-5
 
 
 def test_state_errors(gpu, si_sys, pwk_mod):
