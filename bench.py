@@ -275,13 +275,14 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
-def scf_algorithmic(name, t, precision):
+def scf_algorithmic(name, t, precision, bands=None, nkp=None):
     """Algorithmic bytes (HBM-bound kernels) or flops (MFMA kernels) per launch of the SCF
     kernels on the small deck. The filtered eigensolver applies H·psi to B = nk·nbs bands
     at once (nbs = nb plus buffer bands); s_c / s_r are the complex / real sizes."""
     n, nk, N, nch = t["npw"], t["nk"], t["N"], t["nch"]
     nbs = min(n, t["nb"] + max(8, t["nb"] // 4))
-    B = nk * nbs
+    B = nk * nbs if bands is None else bands  # bands per launch
+    nk = nk if nkp is None else nkp  # k-points per launch
     sc, sr = (16, 8) if precision == "fp64" else (8, 4)
     if name == "hpsi_zinv":
         return "hbm", B * (n + N) * sc
@@ -293,13 +294,17 @@ def scf_algorithmic(name, t, precision):
         return "flop", 8.0 * n * nch * B
     if name == "scf_density":
         return "hbm", B * (2 * N * sc + n * sc)
+    if name == "hpsi_small":
+        # fused H·psi: psi in and H·psi out once per band; kinetic diagonal and projector blocks
+        # once per k-point and V(r) once per launch (L2-resident across the bands)
+        return "hbm", B * 2 * n * sc + nk * (n * sr + 2 * n * nch * sc) + N * sr
     return None, None
 
 
-def local_kernel_probe(deck, p, nbands, precision, reps=60):
-    """Per-launch device time of the local H·psi kernels at the served solve's launch shape
-    (nbands = nk·nbs bands on the deck's grid), with the stream kept busy so the HIP events
-    bracket the kernels rather than host launch gaps."""
+def local_kernel_probe(deck, p, nbands, precision, nch=0, reps=60):
+    """Per-launch device time of the H·psi kernels at the served solve's launch shape
+    (nbands = nk·nbs bands on the deck's grid, nch seeded projector channels), with the stream
+    kept busy so the HIP events bracket the kernels rather than host launch gaps."""
     import numpy as np
     import torch
     import pwk
@@ -315,6 +320,10 @@ def local_kernel_probe(deck, p, nbands, precision, reps=60):
     gvec = miller @ decks.reciprocal(lat)
     acc.set_kinetic(0, decks.kinetic(gvec, [0.0, 0.0, 0.0]))
     acc.set_veff(decks.seeded_veff_g(dims, 7))
+    if nch:
+        rng = np.random.default_rng(5)
+        Bm = (rng.standard_normal((len(miller), nch)) + 1j * rng.standard_normal((len(miller), nch))) * 0.1
+        acc.set_nonlocal(0, Bm, Bm.conj().T.copy())
     dt = torch.complex64 if precision != "fp64" else torch.complex128
     psi = torch.randn(nbands, len(miller), dtype=dt, device="cuda:0")
     out = torch.empty_like(psi)
@@ -409,12 +418,15 @@ def scf_bench(args):
     bound, amount = scf_algorithmic(name, info, args.scf_precision)
     dur = kv["avg_us"] * 1e-6
     probe_us = None
-    if name in ("hpsi_zinv", "hpsi_plane", "hpsi_zfwd"):
+    if name in ("hpsi_zinv", "hpsi_plane", "hpsi_zfwd", "hpsi_small"):
         nbs = min(info["npw"], info["nb"] + max(8, info["nb"] // 4))
-        probe = local_kernel_probe(deckmx, pmx, info["nk"] * nbs, args.scf_precision)
+        probe = local_kernel_probe(deckmx, pmx, info["nk"] * nbs, args.scf_precision,
+                                   nch=info["nch"] if name == "hpsi_small" else 0)
         probe_us = probe.get(name)
         if probe_us:
             dur = probe_us * 1e-6
+            if name == "hpsi_small":  # the probe's bands all sit at one k-point
+                bound, amount = scf_algorithmic(name, info, args.scf_precision, bands=info["nk"] * nbs, nkp=1)
     if bound == "flop":
         peak = FP64_MFMA_PEAK_TFLOPS if args.scf_precision == "fp64" else FP32_MFMA_PEAK_TFLOPS
         roofline = {"kernel": name, "bound": "mfma", "achieved": round(amount / dur / 1e12, 5), "peak": peak,
@@ -549,7 +561,8 @@ def precision_study():
 
 # cell sizes of the serving stream: atoms -> (supercell of the 8-atom cube, or None for the
 # 2-atom FCC primitive cell, (fewest, most) MP divisions drawn for this size, draw weight).
-# The 16-atom cell is doubled along z (the plane kernels of H·psi take nx == ny grids). 2-atom
+# The 16-atom cell is doubled along z, where the stick kernels of H·psi serve it (grids with
+# nx != ny take the slower full-grid path). 2-atom
 # cells draw at least 2 divisions: at Γ alone the 2-atom LSDA decks fall into a magnetised limit
 # cycle (M ~ 6) that the reference algorithm does not leave either.
 STREAM_CELLS = {2: (None, (2, 4), 0.45), 8: ((1, 1, 1), (1, 2), 0.3), 16: ((1, 1, 2), (1, 2), 0.2),

@@ -127,6 +127,10 @@ int pwk_ctx_sync(pwk_ctx* ctx) {
 
 void* pwk_ctx_stream(pwk_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
 
+int pwk_ctx_set_fused_hpsi(pwk_ctx* ctx, int on) {
+  return guarded([&] { ctx_of(ctx).small_hpsi = on != 0; });
+}
+
 int pwk_profile_enable(pwk_ctx* ctx, int on) {
   return guarded([&] {
     Context& c = ctx_of(ctx);

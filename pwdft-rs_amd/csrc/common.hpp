@@ -153,6 +153,8 @@ struct Context {
   bool have_v = false;
   DevBuf veffg64;      // double2 N, V_eff(G)
   DevBuf vr64, vr32;   // real N, V(r) in plane-major order [i3][i1][i2]
+  DevBuf vrn64, vrn32; // real N, V(r) in grid order [i1][i2][i3]
+  bool small_hpsi = true;  // fused one-launch H·ψ on small cubic grids (hpsi_small.hip)
 
   std::vector<KData> kd;
 
@@ -197,6 +199,12 @@ void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int pr
 void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision);
 void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, int precision, const void* kin,
                        int nch, const void* B, const void* DBH, void* proj);
+// hpsi_small.hip: the whole H·ψ of nbands bands (band b at k-point b / nbk) in one launch;
+// with coef (α, c, β per k-point, device) the output is α(Hψ − cψ) − βX instead
+bool hpsi_small_supported(const Context& c, int precision, int nch);
+void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
+                       int nch, const void* B, const void* DBH, const double* coef = nullptr,
+                       const void* X = nullptr);
 // dense.hip
 void launch_dense_h(Context& c, int ik, int with_nl, void* h);
 void launch_dense_h_batched(Context& c, int nk, const double* kin, int nch, const void* B, const void* DBH, void* h);

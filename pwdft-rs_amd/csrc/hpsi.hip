@@ -247,9 +247,11 @@ __global__ void __launch_bounds__(64) k_plane_density(const typename V2t<T>::t* 
   }
 }
 
-// V(r) = Re IFFT_unnorm(V_eff(G)) re-ordered to plane-major [i3][i1][i2]
+// V(r) = Re IFFT_unnorm(V_eff(G)) re-ordered to plane-major [i3][i1][i2] (stick path), and in
+// grid order (fused small-grid path)
 __global__ void __launch_bounds__(256) k_vr_reorder(const double2* __restrict__ vg_r, double* __restrict__ v64,
-                                                    float* __restrict__ v32, int nx, int ny, int nz) {
+                                                    float* __restrict__ v32, double* __restrict__ n64,
+                                                    float* __restrict__ n32, int nx, int ny, int nz) {
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t n = (size_t)nx * ny * nz;
   if (idx >= n) return;
@@ -257,9 +259,139 @@ __global__ void __launch_bounds__(256) k_vr_reorder(const double2* __restrict__ 
   int i2 = (int)(idx % ny);
   int i1 = (int)((idx / ny) % nx);
   int i3 = (int)(idx / ((size_t)nx * ny));
-  double v = vg_r[((size_t)i1 * ny + i2) * nz + i3].x;
+  const size_t g = ((size_t)i1 * ny + i2) * nz + i3;
+  double v = vg_r[g].x;
   v64[idx] = v;
   v32[idx] = (float)v;
+  n64[g] = v;
+  n32[g] = (float)v;
+}
+
+// ---- full-grid path (grids with nx != ny) ----------------------------------------
+// ψ scattered on the whole grid of each band of a chunk, batched 3D FFTs (one compiled
+// line length per axis), V(r) multiply and gather. Slower than the stick path but valid
+// for every grid shape the line kernels compile.
+template <typename T>
+__global__ void __launch_bounds__(256) k_grid_zero(size_t n, typename V2t<T>::t* __restrict__ grid) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  typename V2t<T>::t z;
+  z.x = 0;
+  z.y = 0;
+  grid[i] = z;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_grid_put(size_t n, size_t N, int npw, const typename V2t<T>::t* psi,
+                                                  const int* __restrict__ g2fft, typename V2t<T>::t* __restrict__ grid) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;  // band · npw + g
+  if (i >= n) return;
+  const size_t b = i / npw, g = i % npw;
+  grid[b * N + g2fft[g]] = psi[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_grid_mulv(size_t n, size_t N, const T* __restrict__ v,
+                                                   typename V2t<T>::t* __restrict__ grid) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const T vv = v[i % N];
+  grid[i].x *= vv;
+  grid[i].y *= vv;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_grid_get(size_t n, size_t N, int npw, int nbk, int b0,
+                                                  const typename V2t<T>::t* __restrict__ grid,
+                                                  const int* __restrict__ g2fft,
+                                                  const typename V2t<T>::t* __restrict__ psi, const T* __restrict__ kin,
+                                                  typename V2t<T>::t* __restrict__ hpsi, T inv_n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;  // chunk band · npw + g
+  if (i >= n) return;
+  const size_t bl = i / npw, g = i % npw;
+  const size_t b = (size_t)b0 + bl;
+  const typename V2t<T>::t w = grid[bl * N + g2fft[g]], ps = psi[b * npw + g];
+  const T tk = kin[(b / nbk) * npw + g];
+  typename V2t<T>::t o;
+  o.x = tk * ps.x + w.x * inv_n;
+  o.y = tk * ps.y + w.y * inv_n;
+  hpsi[b * npw + g] = o;
+}
+
+// ρ(r) += Σ_b fw_b |ψ_b(r)|² over the bands of a chunk in band order (src/scf/density.rs:59-97)
+template <typename T>
+__global__ void __launch_bounds__(256) k_grid_density(size_t N, int nbc, const typename V2t<T>::t* __restrict__ grid,
+                                                      const double* __restrict__ fw, double* __restrict__ rho) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  double acc = rho[i];
+  for (int b = 0; b < nbc; ++b) {
+    const double f = fw[b];
+    if (f < 1e-15) continue;  // src/scf/density.rs:14,72
+    const typename V2t<T>::t v = grid[(size_t)b * N + i];
+    const double x = (double)v.x, y = (double)v.y;
+    acc += f * (x * x + y * y);
+  }
+  rho[i] = acc;
+}
+
+// bands per chunk of the full-grid path: at most 256 MiB of grids
+template <typename T>
+int grid_chunk(const Context& c, int nb) {
+  const size_t per = c.N * sizeof(typename V2t<T>::t);
+  return (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>((size_t)nb, 32768), ((size_t)256 << 20) / per));
+}
+
+template <typename T>
+void hpsi_fullgrid(Context& c, const T* kin, int nb, int nbk, const void* psi, void* hpsi) {
+  using V2 = typename V2t<T>::t;
+  const int prec = std::is_same<T, double>::value ? PWK_FP64 : PWK_FP32;
+  const T* vr = std::is_same<T, double>::value ? (const T*)c.vrn64.p : (const T*)c.vrn32.p;
+  const size_t N = c.N;
+  const int chunk = grid_chunk<T>(c, nb);
+  c.work_a.ensure((size_t)chunk * N * sizeof(V2));
+  V2* grid = c.work_a.as<V2>();
+  TimedScope ts(c, "hpsi_fullgrid");
+  for (int b0 = 0; b0 < nb; b0 += chunk) {
+    const int nbc = std::min(chunk, nb - b0);
+    const size_t ng = (size_t)nbc * N, np = (size_t)nbc * c.npw;
+    k_grid_zero<T><<<blocks_for(ng, 256), 256, 0, c.stream>>>(ng, grid);
+    k_grid_put<T><<<blocks_for(np, 256), 256, 0, c.stream>>>(np, N, c.npw, (const V2*)psi + (size_t)b0 * c.npw,
+                                                             c.g2fft.as<int>(), grid);
+    PWK_HIP(hipGetLastError());
+    launch_fft3d(c, grid, nbc, +1, prec);
+    k_grid_mulv<T><<<blocks_for(ng, 256), 256, 0, c.stream>>>(ng, N, vr, grid);
+    PWK_HIP(hipGetLastError());
+    launch_fft3d(c, grid, nbc, -1, prec);
+    k_grid_get<T><<<blocks_for(np, 256), 256, 0, c.stream>>>(np, N, c.npw, nbk, b0, grid, c.g2fft.as<int>(),
+                                                             (const V2*)psi, kin, (V2*)hpsi,
+                                                             (T)(1.0 / (double)N));
+    PWK_HIP(hipGetLastError());
+  }
+}
+
+template <typename T>
+void density_fullgrid(Context& c, int nb, const void* psi, const double* fw_host, void* rho) {
+  using V2 = typename V2t<T>::t;
+  const int prec = std::is_same<T, double>::value ? PWK_FP64 : PWK_FP32;
+  const size_t N = c.N;
+  const int chunk = grid_chunk<T>(c, nb);
+  c.work_a.ensure((size_t)chunk * N * sizeof(V2));
+  c.work_b.ensure((size_t)nb * sizeof(double));
+  PWK_HIP(hipMemcpyAsync(c.work_b.p, fw_host, (size_t)nb * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  V2* grid = c.work_a.as<V2>();
+  for (int b0 = 0; b0 < nb; b0 += chunk) {
+    const int nbc = std::min(chunk, nb - b0);
+    const size_t ng = (size_t)nbc * N, np = (size_t)nbc * c.npw;
+    k_grid_zero<T><<<blocks_for(ng, 256), 256, 0, c.stream>>>(ng, grid);
+    k_grid_put<T><<<blocks_for(np, 256), 256, 0, c.stream>>>(np, N, c.npw, (const V2*)psi + (size_t)b0 * c.npw,
+                                                             c.g2fft.as<int>(), grid);
+    PWK_HIP(hipGetLastError());
+    launch_fft3d(c, grid, nbc, +1, prec);
+    k_grid_density<T><<<blocks_for(N, 256), 256, 0, c.stream>>>(N, nbc, grid, c.work_b.as<double>() + b0,
+                                                                (double*)rho);
+    PWK_HIP(hipGetLastError());
+  }
 }
 
 // ---- dispatch ------------------------------------------------------------------
@@ -269,8 +401,11 @@ template <typename T>
 void hpsi_local(Context& c, const T* kin, int nb, int nbk, const void* psi, void* hpsi) {
   using V2 = typename V2t<T>::t;
   int nx = c.dims[0], ny = c.dims[1], nz = c.dims[2];
-  PWK_REQUIRE(nx == ny, PWK_E_UNSUPPORTED, "H·psi plane kernels need nx == ny");
   PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
+  if (nx != ny) {
+    hpsi_fullgrid<T>(c, kin, nb, nbk, psi, hpsi);
+    return;
+  }
   PWK_REQUIRE(nb <= 65535, PWK_E_UNSUPPORTED, "too many bands in one H·psi launch");
   size_t tb = (size_t)nb * nz * c.nsticks * sizeof(V2);
   c.work_a.ensure(tb);
@@ -332,7 +467,10 @@ template <typename T>
 void density_t(Context& c, int nb, const void* psi, const double* fw_host, void* rho) {
   using V2 = typename V2t<T>::t;
   int nx = c.dims[0], ny = c.dims[1], nz = c.dims[2];
-  PWK_REQUIRE(nx == ny, PWK_E_UNSUPPORTED, "density plane kernels need nx == ny");
+  if (nx != ny) {
+    density_fullgrid<T>(c, nb, psi, fw_host, rho);
+    return;
+  }
   size_t tb = (size_t)nb * nz * c.nsticks * sizeof(V2);
   c.work_a.ensure(tb);
   c.work_b.ensure((size_t)nb * sizeof(double));
@@ -429,12 +567,15 @@ void set_veff_device(Context& c, const void* veff_g_dev) {
   c.veffg64.ensure(n * sizeof(double2));
   c.vr64.ensure(n * sizeof(double));
   c.vr32.ensure(n * sizeof(float));
+  c.vrn64.ensure(n * sizeof(double));
+  c.vrn32.ensure(n * sizeof(float));
   c.work_b.ensure(n * sizeof(double2));
   PWK_HIP(hipMemcpyAsync(c.veffg64.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
   PWK_HIP(hipMemcpyAsync(c.work_b.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
   launch_fft3d(c, c.work_b.p, 1, +1, PWK_FP64);
   k_vr_reorder<<<blocks_for(n, 256), 256, 0, c.stream>>>(c.work_b.as<double2>(), c.vr64.as<double>(),
-                                                         c.vr32.as<float>(), c.dims[0], c.dims[1], c.dims[2]);
+                                                         c.vr32.as<float>(), c.vrn64.as<double>(),
+                                                         c.vrn32.as<float>(), c.dims[0], c.dims[1], c.dims[2]);
   PWK_HIP(hipGetLastError());
   c.have_v = true;
 }
@@ -444,6 +585,13 @@ void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int pr
   PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
   KData& kd = c.k(ik);
   PWK_REQUIRE(kd.have_kin, PWK_E_STATE, "kinetic diagonal of this k-point not set (pwk_set_kinetic)");
+  if (precision != PWK_SPLIT_BF16 && c.have_v && hpsi_small_supported(c, precision, kd.nch) &&
+      (kd.nch == 0 || (precision == PWK_FP64 ? kd.B64.p && kd.DBH64.p : kd.B32.p && kd.DBH32.p))) {
+    const bool f64 = precision == PWK_FP64;
+    launch_hpsi_small(c, nb, nb, psi, hpsi, precision, f64 ? kd.kin64.p : kd.kin32.p, kd.nch,
+                      f64 ? kd.B64.p : kd.B32.p, f64 ? kd.DBH64.p : kd.DBH32.p);
+    return;
+  }
   if (precision == PWK_FP64) {
     hpsi_local<double>(c, kd.kin64.as<double>(), nb, nb, psi, hpsi);
   } else {
@@ -472,6 +620,10 @@ void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, 
                        int nch, const void* B, const void* DBH, void* proj) {
   c.check_basis();
   PWK_REQUIRE(nk > 0 && nb > 0, PWK_E_INVALID, "nk and nb must be positive");
+  if (precision != PWK_SPLIT_BF16 && c.have_v && hpsi_small_supported(c, precision, nch)) {
+    launch_hpsi_small(c, nk * nb, nb, psi, hpsi, precision, kin, nch, B, DBH);
+    return;
+  }
   const long long npw = c.npw;
   if (precision == PWK_FP64)
     hpsi_local<double>(c, (const double*)kin, nk * nb, nb, psi, hpsi);

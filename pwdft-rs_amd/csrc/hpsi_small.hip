@@ -1,0 +1,241 @@
+// Fused H·ψ for small cubic grids: one workgroup per band holds the band's
+// whole N³ grid in LDS and runs the complete product in one launch:
+//   scatter ψ(G) -> inverse FFT z, y, x (the x pass multiplies V(r) between its
+//   inverse and forward transforms) -> forward FFT y, z -> gather the sphere,
+//   Hψ = T_kin ψ + (1/N)·FFT[V·IFFT ψ] + B·((D·B^H)·ψ)
+// (src/scf/potentials.rs:163-190 and src/potential/nonlocal.rs:432-450, the
+// same product as the three-kernel stick path plus the two projector GEMMs).
+//
+// Small decks (a few hundred plane waves, 15³-24³ grids) are launch- and
+// latency-bound on the stick path: five launches and two HBM round trips of
+// the stick buffer per H·ψ. Here ψ is read once, Hψ written once, and the
+// projectors and V(r) stay in L2 across the bands of a launch.
+#include "common.hpp"
+#include "fft_reg.hpp"
+
+namespace pwk {
+
+namespace {
+
+template <typename T>
+struct V2s;
+template <>
+struct V2s<double> {
+  using t = double2;
+};
+template <>
+struct V2s<float> {
+  using t = float2;
+};
+
+constexpr int SMALL_NT = 256;
+
+// One set of N² lines through the LDS grid G[x][y][z] (z fastest). AXIS 2: z-lines
+// (stride 1), 1: y-lines (stride N), 0: x-lines (stride N²). With vr, the x pass runs
+// inverse -> × V(r) -> forward on each line while it is in registers.
+template <typename T, int N, int AXIS, int SIGN>
+__device__ __forceinline__ void lds_lines(typename V2s<T>::t* G, const T* __restrict__ vr) {
+  constexpr int es = AXIS == 2 ? 1 : AXIS == 1 ? N : N * N;
+  for (int l = threadIdx.x; l < N * N; l += SMALL_NT) {
+    int base;
+    if constexpr (AXIS == 2)
+      base = l * N;  // l = x·N + y
+    else if constexpr (AXIS == 1)
+      base = (l / N) * N * N + (l % N);  // l = x·N + z
+    else
+      base = l;  // l = y·N + z
+    T re[N], im[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      re[j] = G[base + j * es].x;
+      im[j] = G[base + j * es].y;
+    }
+    fft_dif<T, N, SIGN>(re, im);
+    if constexpr (AXIS == 0) {
+      if (vr) {
+        T r2[N], m2[N];
+#pragma unroll
+        for (int f = 0; f < N; ++f) {
+          const T v = vr[base + f * es];
+          r2[f] = re[FftOrder<N>::pos(f)] * v;
+          m2[f] = im[FftOrder<N>::pos(f)] * v;
+        }
+        fft_dif<T, N, -SIGN>(r2, m2);
+#pragma unroll
+        for (int f = 0; f < N; ++f) {
+          G[base + f * es].x = r2[FftOrder<N>::pos(f)];
+          G[base + f * es].y = m2[FftOrder<N>::pos(f)];
+        }
+        continue;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < N; ++f) {
+      G[base + f * es].x = re[FftOrder<N>::pos(f)];
+      G[base + f * es].y = im[FftOrder<N>::pos(f)];
+    }
+  }
+}
+
+// Band b (k-point b / nbk) of psi -> hpsi. vr is V(r) in natural grid order; g2fft the flat
+// grid index of each plane wave; B (npw × nch) and DBH (nch × npw) column-major blocks per k.
+// With coef (α, c, β per k-point) the output is the Chebyshev recurrence step
+// α_k (Hψ − c_k ψ) − β_k X (X may be null) instead of Hψ.
+template <typename T, int N>
+__global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::t* __restrict__ psi,
+                                                         typename V2s<T>::t* __restrict__ hpsi,
+                                                         const T* __restrict__ kin, const T* __restrict__ vr,
+                                                         const int* __restrict__ g2fft, int npw, int nbk, int nch,
+                                                         const typename V2s<T>::t* __restrict__ B,
+                                                         const typename V2s<T>::t* __restrict__ DBH, T inv_n,
+                                                         const double* __restrict__ coef,
+                                                         const typename V2s<T>::t* __restrict__ X) {
+  using V2 = typename V2s<T>::t;
+  extern __shared__ unsigned char smem_raw[];
+  V2* G = reinterpret_cast<V2*>(smem_raw);
+  V2* proj = G + N * N * N;  // nch projections
+  const int b = blockIdx.x, k = b / nbk;
+  const V2* p = psi + (size_t)b * npw;
+  for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
+    G[i].x = 0;
+    G[i].y = 0;
+  }
+  __syncthreads();
+  for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = p[g];
+  // projections P_c = Σ_G (D·B^H)[c, G] ψ(G): one wave per channel, lanes over G
+  if (nch > 0) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const V2* dk = DBH + (size_t)k * nch * npw;
+    for (int ch = wave; ch < nch; ch += SMALL_NT / 64) {
+      T ar = 0, ai = 0;
+      for (int g = lane; g < npw; g += 64) {
+        const V2 d = dk[(size_t)g * nch + ch], v = p[g];
+        ar += d.x * v.x - d.y * v.y;
+        ai += d.x * v.y + d.y * v.x;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        ar += __shfl_xor(ar, off, 64);
+        ai += __shfl_xor(ai, off, 64);
+      }
+      if (lane == 0) {
+        proj[ch].x = ar;
+        proj[ch].y = ai;
+      }
+    }
+  }
+  __syncthreads();
+  lds_lines<T, N, 2, 1>(G, nullptr);
+  __syncthreads();
+  lds_lines<T, N, 1, 1>(G, nullptr);
+  __syncthreads();
+  lds_lines<T, N, 0, 1>(G, vr);
+  __syncthreads();
+  lds_lines<T, N, 1, -1>(G, nullptr);
+  __syncthreads();
+  lds_lines<T, N, 2, -1>(G, nullptr);
+  __syncthreads();
+  const T* t = kin + (size_t)k * npw;
+  const V2* bk = B + (size_t)k * npw * nch;
+  V2* h = hpsi + (size_t)b * npw;
+  T alpha = 0, cshift = 0, beta = 0;
+  if (coef) {
+    alpha = (T)coef[3 * k];
+    cshift = (T)coef[3 * k + 1];
+    beta = (T)coef[3 * k + 2];
+  }
+  const V2* xb = X ? X + (size_t)b * npw : nullptr;
+  for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
+    const V2 v = p[g], w = G[g2fft[g]];
+    const T tk = t[g];
+    T ox = tk * v.x + w.x * inv_n, oy = tk * v.y + w.y * inv_n;
+    for (int ch = 0; ch < nch; ++ch) {
+      const V2 bb = bk[(size_t)ch * npw + g], pc = proj[ch];
+      ox += bb.x * pc.x - bb.y * pc.y;
+      oy += bb.x * pc.y + bb.y * pc.x;
+    }
+    V2 o;
+    if (coef) {
+      // the elementwise step of the filter (subspace.hip k_cheb_step), same operation order
+      o.x = alpha * (ox - cshift * v.x);
+      o.y = alpha * (oy - cshift * v.y);
+      if (xb) {
+        const V2 xv = xb[g];
+        o.x -= beta * xv.x;
+        o.y -= beta * xv.y;
+      }
+    } else {
+      o.x = ox;
+      o.y = oy;
+    }
+    h[g] = o;
+  }
+}
+
+template <typename T>
+size_t small_lds_bytes(int n, int nch) {
+  return ((size_t)n * n * n + (size_t)nch) * sizeof(typename V2s<T>::t);
+}
+
+constexpr size_t kLdsMax = 160 * 1024;
+
+#define PWK_SMALL_SIZES(X) X(8) X(9) X(10) X(12) X(15) X(16) X(18) X(20) X(24)
+
+bool small_len(int n) {
+  switch (n) {
+#define PWK_CASE(L) case L:
+    PWK_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <typename T>
+void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi, const void* kin, int nch,
+                    const void* B, const void* DBH, const double* coef, const void* X) {
+  using V2 = typename V2s<T>::t;
+  const int n = c.dims[0];
+  const size_t sh = small_lds_bytes<T>(n, nch);
+  const T* vr = std::is_same<T, double>::value ? (const T*)c.vrn64.p : (const T*)c.vrn32.p;
+  const T inv_n = (T)(1.0 / (double)c.N);
+  TimedScope ts(c, "hpsi_small");
+  switch (n) {
+#define PWK_CASE(L)                                                                                            \
+  case L:                                                                                                      \
+    ensure_lds_limit((const void*)k_hpsi_small<T, L>, sh);                                                    \
+    k_hpsi_small<T, L><<<(unsigned)nbands, SMALL_NT, sh, c.stream>>>(                                          \
+        (const V2*)psi, (V2*)hpsi, (const T*)kin, vr, c.g2fft.as<int>(), c.npw, nbk, nch, (const V2*)B,        \
+        (const V2*)DBH, inv_n, coef, (const V2*)X);                                                            \
+    break;
+    PWK_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "grid has no fused small-grid kernel");
+  }
+  PWK_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+bool hpsi_small_supported(const Context& c, int precision, int nch) {
+  if (!c.small_hpsi || c.dims[0] != c.dims[1] || c.dims[1] != c.dims[2] || !small_len(c.dims[0])) return false;
+  if (c.vrn64.p == nullptr || c.vrn32.p == nullptr) return false;
+  const size_t sh = precision == PWK_FP64 ? small_lds_bytes<double>(c.dims[0], nch)
+                                          : small_lds_bytes<float>(c.dims[0], nch);
+  return sh <= kLdsMax;
+}
+
+void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
+                       int nch, const void* B, const void* DBH, const double* coef, const void* X) {
+  c.check_basis();
+  PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
+  PWK_REQUIRE(nbands > 0 && nbk > 0, PWK_E_INVALID, "band counts must be positive");
+  PWK_REQUIRE(hpsi_small_supported(c, precision, nch), PWK_E_UNSUPPORTED, "grid too large for the fused kernel");
+  if (precision == PWK_FP64)
+    launch_small_t<double>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X);
+  else
+    launch_small_t<float>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X);
+}
+
+}  // namespace pwk

@@ -597,6 +597,18 @@ struct Solve {
     launch_hpsi_multi(c, nk, nbands, psi, hpsi, prec, f64 ? kin_all64.p : kin_all32.p, nch_all,
                       f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, projall.p);
   }
+  // out = α_k (H·Y − c_k Y) − β_k X for all k: one fused launch on small grids, else H·ψ
+  // into HYf and the elementwise step
+  void hpsi_cheb(const void* Y, const void* X, const double* cf, void* out, int prec) {
+    const bool f64 = prec == PWK_FP64;
+    if (prec != PWK_SPLIT_BF16 && hpsi_small_supported(c, prec, nch_all)) {
+      launch_hpsi_small(c, nk * nbs, nbs, Y, out, prec, f64 ? kin_all64.p : kin_all32.p, nch_all,
+                        f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, cf, X);
+      return;
+    }
+    hpsi_all(Y, HYf.p, nbs, prec);
+    launch_cheb_step(c, vblk(), (size_t)npw * nbs, HYf.p, Y, X, cf, out, prec);
+  }
 
   void setup();
   double ewald();
@@ -1212,12 +1224,9 @@ double Solve::chebyshev_pass(int fprec) {
   else
     launch_convert(c, Vc(), buf[0], blk * 2, true);
   void *X = buf[0], *Y = buf[1], *Z = buf[2];
-  const size_t per = (size_t)npw * nbs;
-  hpsi_all(X, HYf.p, nbs, fprec);
-  launch_cheb_step(c, blk, per, HYf.p, X, nullptr, coef.as<double>(), Y, fprec);
+  hpsi_cheb(X, nullptr, coef.as<double>(), Y, fprec);
   for (int i = 1; i < degree; ++i) {
-    hpsi_all(Y, HYf.p, nbs, fprec);
-    launch_cheb_step(c, blk, per, HYf.p, Y, X, coef.as<double>() + (size_t)i * nk * 3, Z, fprec);
+    hpsi_cheb(Y, X, coef.as<double>() + (size_t)i * nk * 3, Z, fprec);
     void* t = X;
     X = Y;
     Y = Z;

@@ -51,6 +51,7 @@ _SIGS = {
     "pwk_ctx_destroy": (C.c_int, [_vp]),
     "pwk_ctx_sync": (C.c_int, [_vp]),
     "pwk_ctx_stream": (_vp, [_vp]),
+    "pwk_ctx_set_fused_hpsi": (C.c_int, [_vp, C.c_int]),
     "pwk_profile_enable": (C.c_int, [_vp, C.c_int]),
     "pwk_profile_reset": (C.c_int, [_vp]),
     "pwk_profile_read": (C.c_int, [_vp, C.c_int, C.c_char_p, _dp, C.POINTER(C.c_int64)]),
@@ -162,6 +163,10 @@ class GpuAccelerator:
 
     def sync(self):
         _check(lib().pwk_ctx_sync(self.h))
+
+    def set_fused_hpsi(self, on=True):
+        """Fused one-launch H·psi on small cubic grids (default on) or the z-stick path."""
+        _check(lib().pwk_ctx_set_fused_hpsi(self.h, int(on)))
 
     # ---- kernel timing -----------------------------------------------------------
     def profile(self, on=True):

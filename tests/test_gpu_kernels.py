@@ -151,6 +151,32 @@ def test_hpsi_on_large_grids(gpu, oracle, dims):
     assert rel(gpu.hpsi(0, psi, "fp32"), ref) < 2e-5
 
 
+@pytest.mark.parametrize("dims", [(15, 18, 20), (24, 18, 18), (20, 16, 25)])
+def test_noncubic_grid_hpsi_and_density(gpu, oracle, dims):
+    # nx != ny: the full-grid path (batched 3D FFTs) for H·psi and the density
+    s = oracle.System(si_lat(), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 120.0, dims=list(dims))
+    print("noncubic grid", dims, "npw", s.npw, flush=True)
+    gpu.prepare_buffers(s.dims, s.g2())
+    gpu.set_basis(s.miller())
+    gpu.set_nk(1)
+    k = np.array([0.05, -0.08, 0.13])
+    gpu.set_kinetic(0, s.kinetic(k))
+    rng = np.random.default_rng(15)
+    veff = np.fft.fftn(rng.uniform(-1, 1, s.dims) * 13.6).reshape(-1) / s.n_grid
+    gpu.set_veff(veff)
+    B, DBH, _ = s.nonlocal_(k)
+    gpu.set_nonlocal(0, B, DBH)
+    psi = rng.standard_normal((5, s.npw)) + 1j * rng.standard_normal((5, s.npw))
+    ref = s.hpsi_with_nl(k, veff, B, DBH, psi)
+    assert rel(gpu.hpsi(0, psi, "fp64"), ref) < 1e-12
+    assert rel(gpu.hpsi(0, psi, "fp32"), ref) < 2e-5
+    fw = np.array([2.0, 0.5, 1e-16, 1.0, 0.25])
+    rho0 = rng.uniform(0, 1, s.n_grid)
+    dref = s.density(psi, fw, rho0)
+    assert rel(gpu.density(psi, fw, rho0, "fp64"), dref) < 1e-12
+    assert rel(gpu.density(psi, fw, rho0, "fp32"), dref) < 1e-5
+
+
 def test_state_errors(gpu, si_sys, pwk_mod):
     gpu.prepare_buffers(si_sys.dims, si_sys.g2())
     gpu.set_basis(si_sys.miller())
@@ -215,6 +241,30 @@ def test_hpsi_single_precision(acc, ham, si_sys, prec, tol):
     ref = si_sys.hpsi_with_nl(ham["k"], ham["veff"], ham["B"], ham["DBH"], psi)
     out = acc.hpsi(0, psi, prec)
     assert rel(out, ref) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_hpsi_fused_and_stick_paths_agree(acc, ham, si_sys, prec, tol):
+    # the fused one-launch kernel (small cubic grids) and the z-stick kernels + projector GEMMs
+    # compute the same product; both against the oracle, with and without projectors
+    rng = np.random.default_rng(14)
+    psi = rng.standard_normal((7, si_sys.npw)) + 1j * rng.standard_normal((7, si_sys.npw))
+    for with_nl in (True, False):
+        if with_nl:
+            acc.set_nonlocal(0, ham["B"], ham["DBH"])
+            ref = si_sys.hpsi_with_nl(ham["k"], ham["veff"], ham["B"], ham["DBH"], psi)
+        else:
+            acc.set_nonlocal(0, None, None)
+            ref = (si_sys.dense_h(ham["k"], ham["veff"], with_nl=False) @ psi.T).T
+        try:
+            acc.set_fused_hpsi(True)
+            fused = acc.hpsi(0, psi, prec)
+            acc.set_fused_hpsi(False)
+            stick = acc.hpsi(0, psi, prec)
+        finally:
+            acc.set_fused_hpsi(True)
+        assert rel(fused, ref) < tol, with_nl
+        assert rel(stick, ref) < tol, with_nl
 
 
 def test_hpsi_second_kpoint_and_single_band(acc, ham, si_sys):

@@ -160,6 +160,52 @@ def test_small_deck_chebyshev_filtered(scf_mod, gpu, small_deck_results, precisi
     assert np.abs(r["eigenvalues"][:, :4] - r64["eigenvalues"][:, :4]).max() < 1e-5
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_filtered_solve_fused_and_stick_hpsi_agree(scf_mod, gpu, small_deck_results, precision):
+    # the filtered solve on the small deck (15³ grid) with the fused H·psi and with the stick path
+    r64, _, _ = small_deck_results
+    lat = fcc(10.26 * BOHR)
+    p = scf_mod.ScfParams(ecut_ev=12 * RY, kgrid=(4, 4, 4), n_bands=8, max_iter=100, conv_threshold=1e-8,
+                          energy_threshold=1e-8 * RY, mixing_ndim=8, smearing_sigma=0.01 * RY, eigensolver=1,
+                          precision=precision)
+    print("scf fused vs stick", precision, flush=True)
+    try:
+        gpu.set_fused_hpsi(True)
+        _, rf = _run(scf_mod, gpu, lat, p)
+        gpu.set_fused_hpsi(False)
+        _, rs = _run(scf_mod, gpu, lat, p)
+    finally:
+        gpu.set_fused_hpsi(True)
+    assert rf["converged"] == 1 and rs["converged"] == 1
+    tol = 1e-9 if precision == "fp64" else TOL_TERM
+    for t in TERMS:
+        assert abs(rf[t] - rs[t]) < tol, t
+        assert abs(rf[t] - r64[t]) < TOL_TERM * 2, t
+
+
+@pytest.mark.parametrize("eigensolver,precision", [(0, "fp64"), (1, "fp32")])
+def test_orthorhombic_cell_matches_oracle(scf_mod, gpu, oracle, eigensolver, precision):
+    # an 8-atom Si cell strained to a = (1.12, 0.94, 1.00)·5.431 Å at 8 Ry: a (24, 18, 18) grid,
+    # so the density and the filter's H·psi take the full-grid path; Γ, 20 bands
+    from pwk import decks
+
+    lat, frac = decks.diamond_supercell(5.431)
+    lat = lat * np.array([1.12, 0.94, 1.0])[:, None]
+    deck = scf_mod.CrystalDeck(lat, frac, [0] * 8, [SI_UPF])
+    p = scf_mod.ScfParams(ecut_ev=8 * RY, use_symmetry=False, n_bands=20, max_iter=60, conv_threshold=1e-7,
+                          energy_threshold=1e-6, smearing_sigma=0.01 * RY, eigensolver=eigensolver,
+                          precision=precision)
+    print("orthorhombic", eigensolver, precision, flush=True)
+    r = scf_mod.run_scf(gpu, deck, p)
+    assert tuple(r["dims"]) == (24, 18, 18)
+    s = oracle.System(lat, [0] * 8, [14] * 8, frac, [SI_UPF], 8 * RY)
+    ro = s.scf([[0, 0, 0]], [1.0], n_bands=20, max_iter=60, conv=1e-7, ethr=1e-6, beta=0.3, ndim=8, sigma=0.01 * RY)
+    print("gpu", r["e_total"], r["n_iter"], "oracle", ro["e_total"], ro["n_iter"], flush=True)
+    assert r["converged"] == 1 and ro["converged"] == 1.0
+    for t in TERMS:
+        assert abs(r[t] - ro[t]) / 8 < TOL_TERM, t
+
+
 def test_larger_basis_chebyshev_matches_dense_and_oracle(scf_mod, gpu, oracle):
     # Si2 FCC at 25 Ry (537 PW, 24³), 4×4×4 → 8 IBZ k, 8 bands, conv 1e-6: the batched H·ψ
     # eigensolver (all k in one launch sequence, FP64 and FP32 filters) against the dense
