@@ -205,6 +205,9 @@ bool hpsi_small_supported(const Context& c, int precision, int nch);
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
                        int nch, const void* B, const void* DBH, const double* coef = nullptr,
                        const void* X = nullptr);
+// ρ += Σ_b fw_b |ψ_b(r)|² on small cubic grids (fw on the device)
+bool density_small_supported(const Context& c, int precision);
+void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision);
 // dense.hip
 void launch_dense_h(Context& c, int ik, int with_nl, void* h);
 void launch_dense_h_batched(Context& c, int nk, const double* kin, int nch, const void* B, const void* DBH, void* h);

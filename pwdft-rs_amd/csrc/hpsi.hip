@@ -644,6 +644,13 @@ void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, 
 void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision) {
   c.check_basis();
   PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
+  const int dprec = precision == PWK_FP64 ? PWK_FP64 : PWK_FP32;
+  if (density_small_supported(c, dprec)) {
+    c.work_b.ensure((size_t)nb * sizeof(double));
+    PWK_HIP(hipMemcpyAsync(c.work_b.p, fw_host, (size_t)nb * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    launch_density_small(c, nb, psi, c.work_b.as<double>(), rho, dprec);
+    return;
+  }
   if (precision == PWK_FP64)
     density_t<double>(c, nb, psi, fw_host, rho);
   else

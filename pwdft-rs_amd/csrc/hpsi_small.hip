@@ -171,6 +171,81 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
   }
 }
 
+// Band density on small cubic grids: workgroup g takes bands [g·gsize, (g+1)·gsize), runs the
+// inverse transform of each in LDS and keeps Σ_b fw_b |ψ_b(r)|² of its x-lines in FP64
+// registers (bands with fw < 1e-15 skipped, src/scf/density.rs:14,72); the partial sums of
+// the groups are added in group order by k_density_reduce, so the result does not depend on
+// scheduling.
+template <typename T, int N>
+__global__ void __launch_bounds__(SMALL_NT) k_density_small(const typename V2s<T>::t* __restrict__ psi,
+                                                            const int* __restrict__ g2fft, int npw, int nb, int gsize,
+                                                            const double* __restrict__ fw,
+                                                            double* __restrict__ partial) {
+  using V2 = typename V2s<T>::t;
+  extern __shared__ unsigned char smem_raw[];
+  V2* G = reinterpret_cast<V2*>(smem_raw);
+  constexpr int LPT = (N * N + SMALL_NT - 1) / SMALL_NT;  // x-lines per thread
+  double acc[LPT][N];
+#pragma unroll
+  for (int li = 0; li < LPT; ++li)
+#pragma unroll
+    for (int x = 0; x < N; ++x) acc[li][x] = 0.0;
+  const int b0 = blockIdx.x * gsize, b1 = min(nb, b0 + gsize);
+  for (int b = b0; b < b1; ++b) {
+    const double f = fw[b];
+    if (f < 1e-15) continue;  // uniform over the workgroup
+    __syncthreads();
+    for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
+      G[i].x = 0;
+      G[i].y = 0;
+    }
+    __syncthreads();
+    const V2* p = psi + (size_t)b * npw;
+    for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = p[g];
+    __syncthreads();
+    lds_lines<T, N, 2, 1>(G, nullptr);
+    __syncthreads();
+    lds_lines<T, N, 1, 1>(G, nullptr);
+    __syncthreads();
+#pragma unroll
+    for (int li = 0; li < LPT; ++li) {
+      const int l = threadIdx.x + li * SMALL_NT;  // l = y·N + z
+      if (l < N * N) {
+        T re[N], im[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          re[j] = G[l + j * N * N].x;
+          im[j] = G[l + j * N * N].y;
+        }
+        fft_dif<T, N, 1>(re, im);
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+          const double vr = (double)re[FftOrder<N>::pos(x)], vi = (double)im[FftOrder<N>::pos(x)];
+          acc[li][x] += f * (vr * vr + vi * vi);
+        }
+      }
+    }
+  }
+  double* out = partial + (size_t)blockIdx.x * N * N * N;
+#pragma unroll
+  for (int li = 0; li < LPT; ++li) {
+    const int l = threadIdx.x + li * SMALL_NT;
+    if (l < N * N) {
+#pragma unroll
+      for (int x = 0; x < N; ++x) out[(size_t)x * N * N + l] = acc[li][x];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_density_reduce(size_t n, int ngroups, const double* __restrict__ partial,
+                                                        double* __restrict__ rho) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double s = rho[i];
+  for (int g = 0; g < ngroups; ++g) s += partial[(size_t)g * n + i];
+  rho[i] = s;
+}
+
 template <typename T>
 size_t small_lds_bytes(int n, int nch) {
   return ((size_t)n * n * n + (size_t)nch) * sizeof(typename V2s<T>::t);
@@ -216,10 +291,53 @@ void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi
   PWK_HIP(hipGetLastError());
 }
 
+template <typename T>
+void launch_density_small_t(Context& c, int nb, const void* psi, const double* fw_dev, void* rho) {
+  using V2 = typename V2s<T>::t;
+  const int n = c.dims[0];
+  const size_t sh = small_lds_bytes<T>(n, 0);
+  const int gsize = std::max(1, (nb + 127) / 128);  // at most 128 groups
+  const int ng = (nb + gsize - 1) / gsize;
+  c.work_a.ensure((size_t)ng * c.N * sizeof(double));
+  TimedScope ts(c, "density_small");
+  switch (n) {
+#define PWK_CASE(L)                                                                                            \
+  case L:                                                                                                      \
+    ensure_lds_limit((const void*)k_density_small<T, L>, sh);                                                 \
+    k_density_small<T, L><<<(unsigned)ng, SMALL_NT, sh, c.stream>>>((const V2*)psi, c.g2fft.as<int>(), c.npw,  \
+                                                                    nb, gsize, fw_dev, c.work_a.as<double>());  \
+    break;
+    PWK_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "grid has no fused small-grid kernel");
+  }
+  PWK_HIP(hipGetLastError());
+  k_density_reduce<<<blocks_for(c.N, 256), 256, 0, c.stream>>>(c.N, ng, c.work_a.as<double>(), (double*)rho);
+  PWK_HIP(hipGetLastError());
+}
+
 }  // namespace
 
-bool hpsi_small_supported(const Context& c, int precision, int nch) {
+void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision) {
+  c.check_basis();
+  PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
+  PWK_REQUIRE(density_small_supported(c, precision), PWK_E_UNSUPPORTED, "grid too large for the fused kernel");
+  if (precision == PWK_FP64)
+    launch_density_small_t<double>(c, nb, psi, fw_dev, rho);
+  else
+    launch_density_small_t<float>(c, nb, psi, fw_dev, rho);
+}
+
+bool density_small_supported(const Context& c, int precision) {
   if (!c.small_hpsi || c.dims[0] != c.dims[1] || c.dims[1] != c.dims[2] || !small_len(c.dims[0])) return false;
+  const size_t sh =
+      precision == PWK_FP64 ? small_lds_bytes<double>(c.dims[0], 0) : small_lds_bytes<float>(c.dims[0], 0);
+  return sh <= kLdsMax;
+}
+
+bool hpsi_small_supported(const Context& c, int precision, int nch) {
+  if (!density_small_supported(c, precision)) return false;
   if (c.vrn64.p == nullptr || c.vrn32.p == nullptr) return false;
   const size_t sh = precision == PWK_FP64 ? small_lds_bytes<double>(c.dims[0], nch)
                                           : small_lds_bytes<float>(c.dims[0], nch);

@@ -265,6 +265,20 @@ def test_hpsi_fused_and_stick_paths_agree(acc, ham, si_sys, prec, tol):
             acc.set_fused_hpsi(True)
         assert rel(fused, ref) < tol, with_nl
         assert rel(stick, ref) < tol, with_nl
+    # the density: fused kernel (band groups, FP64 partial sums) and the plane kernels
+    fw = np.array([2.0, 1e-16, 0.5, 1.0, 0.0, 0.3, 1.2])
+    rho0 = rng.uniform(0, 1, si_sys.n_grid)
+    dref = si_sys.density(psi, fw, rho0)
+    try:
+        acc.set_fused_hpsi(True)
+        dfused = acc.density(psi, fw, rho0, prec)
+        acc.set_fused_hpsi(False)
+        dstick = acc.density(psi, fw, rho0, prec)
+    finally:
+        acc.set_fused_hpsi(True)
+    dtol = 1e-12 if prec == "fp64" else 1e-5
+    assert rel(dfused, dref) < dtol
+    assert rel(dstick, dref) < dtol
 
 
 def test_hpsi_second_kpoint_and_single_band(acc, ham, si_sys):
