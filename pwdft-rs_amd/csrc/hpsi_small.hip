@@ -94,7 +94,11 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
   extern __shared__ unsigned char smem_raw[];
   V2* G = reinterpret_cast<V2*>(smem_raw);
   V2* proj = G + N * N * N;  // nch projections
-  const int b = blockIdx.x, k = b / nbk;
+  // XCD-aware band order: workgroups are dealt to the 8 XCDs round-robin, so workgroup i runs
+  // on XCD i % 8; each XCD takes a contiguous range of bands, which keeps the bands of one
+  // k-point (and its projector blocks) in one XCD's L2
+  const int q = (int)gridDim.x >> 3, r = (int)gridDim.x & 7, xcd = (int)blockIdx.x & 7;
+  const int b = xcd * q + min(xcd, r) + ((int)blockIdx.x >> 3), k = b / nbk;
   const V2* p = psi + (size_t)b * npw;
   for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
     G[i].x = 0;
