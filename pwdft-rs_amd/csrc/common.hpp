@@ -243,6 +243,11 @@ void launch_block_scale(Context& c, size_t per, int batch, const double* s, int 
 void launch_start_block(Context& c, int npw, int nb, int nk, const int* lowest_dev, void* V);
 void launch_resid_norms(Context& c, int npw, int nbs, int nres, int nk, const void* V, const void* HV,
                         const double* lam, double* out);
+// whole Rayleigh-Ritz step per k-point in one workgroup (m <= 32): V = Y·C, Ritz values lam
+// (nk × m) and residual norms res (nk × m, first nres columns)
+bool rr_small_supported(int m);
+void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y, const void* HY, void* V,
+                     double* lam, double* res, double floor_rel);
 // scf.hip
 void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
              double* rho);

@@ -20,6 +20,7 @@
 // disjoint complex rotations per round, at most 30 sweeps).
 // Every loop has a fixed bound; nothing waits on data.
 #include "common.hpp"
+#include "jacobi.hpp"
 
 namespace pwk {
 
@@ -415,122 +416,23 @@ __global__ void __launch_bounds__(NT) k_heev(int n, int nb, int a_in_lds, int lu
   if (t == 0 && status) status[b] = 0;
 }
 
-// One wavefront per matrix, n <= 32. A_l[row][col] and V_l[row][col] in LDS.
-constexpr int JMAX = 32;
+// One wavefront per matrix, n <= JMAX: cyclic Jacobi with the matrix and the vectors in LDS.
 __global__ void __launch_bounds__(64) k_jacobi(int n, int nb, const double2* __restrict__ Hb, double* __restrict__ w_out,
                                                double2* __restrict__ V_out) {
-  __shared__ double2 Al[JMAX][JMAX + 1];
-  __shared__ double2 Vl[JMAX][JMAX + 1];
-  __shared__ double rc[JMAX / 2], rs[JMAX / 2];
-  __shared__ double2 rph[JMAX / 2];
-  __shared__ int rp[JMAX / 2], rq[JMAX / 2], idx[JMAX], order[JMAX];
+  __shared__ JacobiLds J;
   const int b = blockIdx.x, lane = threadIdx.x;
   const double2* H = Hb + (size_t)b * n * n;
   for (int e = lane; e < n * n; e += 64) {
     const int r = e % n, col = e / n;
-    Al[r][col] = H[e];
-    Vl[r][col] = make_double2(r == col ? 1.0 : 0.0, 0.0);
-  }
-  const int N = n + (n & 1);
-  if (lane < N) idx[lane] = lane;
-  __syncthreads();
-  for (int sweep = 0; sweep < 30; ++sweep) {
-    // off-diagonal and total Frobenius weight
-    double off = 0.0, tot = 0.0;
-    for (int e = lane; e < n * n; e += 64) {
-      const int r = e % n, col = e / n;
-      const double2 a = Al[r][col];
-      const double m2 = a.x * a.x + a.y * a.y;
-      tot += m2;
-      if (r != col) off += m2;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      off += __shfl_xor(off, o, 64);
-      tot += __shfl_xor(tot, o, 64);
-    }
-    if (off <= 1e-30 * tot || off < 1e-300) break;  // wave-uniform
-    for (int round = 0; round < N - 1; ++round) {
-      if (lane < N / 2) {
-        int p = idx[lane], q = idx[N - 1 - lane];
-        if (p > q) {
-          const int tt = p;
-          p = q;
-          q = tt;
-        }
-        double c = 1.0, sn = 0.0;
-        double2 ph = make_double2(1.0, 0.0);
-        if (q < n) {
-          const double a = Al[p][p].x, bq = Al[q][q].x;
-          const double2 apq = Al[p][q];
-          const double g = sqrt(apq.x * apq.x + apq.y * apq.y);
-          if (g > 1e-300) {
-            ph = make_double2(apq.x / g, apq.y / g);
-            const double tau = (bq - a) / (2.0 * g);
-            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
-            sn = t * c;
-          }
-        }
-        rp[lane] = p;
-        rq[lane] = q < n ? q : -1;
-        rc[lane] = c;
-        rs[lane] = sn;
-        rph[lane] = ph;
-      }
-      __syncthreads();
-      // columns: A ← A·W, V ← V·W with W = [[c, s], [-s e, c e]], e = conj(ph)
-      for (int it = lane; it < (N / 2) * n; it += 64) {
-        const int k = it / n, r = it % n, q = rq[k];
-        if (q < 0) continue;
-        const int p = rp[k];
-        const double c = rc[k], sn = rs[k];
-        const double2 e = make_double2(rph[k].x, -rph[k].y);
-        for (int mtx = 0; mtx < 2; ++mtx) {
-          double2* P = mtx == 0 ? &Al[r][p] : &Vl[r][p];
-          double2* Q = mtx == 0 ? &Al[r][q] : &Vl[r][q];
-          const double2 x = *P, y = *Q;
-          const double2 ey = make_double2(e.x * y.x - e.y * y.y, e.x * y.y + e.y * y.x);
-          *P = make_double2(c * x.x - sn * ey.x, c * x.y - sn * ey.y);
-          *Q = make_double2(sn * x.x + c * ey.x, sn * x.y + c * ey.y);
-        }
-      }
-      __syncthreads();
-      // rows: A ← W^H·A, W^H = [[c, -s conj(e)], [s, c conj(e)]]
-      for (int it = lane; it < (N / 2) * n; it += 64) {
-        const int k = it / n, col = it % n, q = rq[k];
-        if (q < 0) continue;
-        const int p = rp[k];
-        const double c = rc[k], sn = rs[k];
-        const double2 ce = rph[k];  // conj(e) = ph
-        const double2 x = Al[p][col], y = Al[q][col];
-        const double2 ey = make_double2(ce.x * y.x - ce.y * y.y, ce.x * y.y + ce.y * y.x);
-        Al[p][col] = make_double2(c * x.x - sn * ey.x, c * x.y - sn * ey.y);
-        Al[q][col] = make_double2(sn * x.x + c * ey.x, sn * x.y + c * ey.y);
-      }
-      __syncthreads();
-      // round-robin: position 0 stays, the others rotate by one
-      int nv = 0;
-      if (lane < N) nv = lane == 0 ? idx[0] : (lane == 1 ? idx[N - 1] : idx[lane - 1]);
-      __syncthreads();
-      if (lane < N) idx[lane] = nv;
-      __syncthreads();
-    }
-  }
-  // ascending order by rank (ties by index)
-  if (lane < n) {
-    const double di = Al[lane][lane].x;
-    int rank = 0;
-    for (int j = 0; j < n; ++j) {
-      const double dj = Al[j][j].x;
-      rank += (dj < di || (dj == di && j < lane)) ? 1 : 0;
-    }
-    order[rank] = lane;
+    J.A[r][col] = H[e];
+    J.V[r][col] = make_double2(r == col ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
-  for (int jj = lane; jj < nb; jj += 64) w_out[(size_t)b * nb + jj] = Al[order[jj]][order[jj]].x;
+  jacobi_lds(n, J);
+  for (int jj = lane; jj < nb; jj += 64) w_out[(size_t)b * nb + jj] = J.A[J.order[jj]][J.order[jj]].x;
   for (int e = lane; e < n * nb; e += 64) {
     const int r = e % n, jj = e / n;
-    V_out[(size_t)b * n * nb + e] = Vl[r][order[jj]];
+    V_out[(size_t)b * n * nb + e] = J.V[r][J.order[jj]];
   }
 }
 

@@ -576,7 +576,7 @@ struct Solve {
   DevBuf F0, F1, F2, HYf, Y64, HY64, YH, sS, sH, sU, sT, sX, sXH, sW, sC, s_ev, s_lam, s_work, lowest;
   DevBuf kin_all64, kin_all32;              // nk × npw kinetic diagonals
   DevBuf R64, resid;                        // H·V after Rayleigh-Ritz, residual norms
-  std::vector<double> resid_host;
+  std::vector<double> resid_host, rr_host;
   double last_delta = INFINITY;             // density change of the previous SCF iteration
   int passes_total = 0;
   DevBuf Ball64, DBHall64, Ball32, DBHall32, projall, coef;
@@ -932,7 +932,7 @@ void Solve::setup() {
     resid_host.assign((size_t)nk * nbs, 0.0);
     for (DevBuf* b : {&sS, &sH, &sU, &sT, &sX, &sXH, &sW, &sC}) b->ensure((size_t)nk * nbs * nbs * 16);
     s_ev.ensure((size_t)nk * nbs * 8);
-    s_lam.ensure((size_t)nk * nbs * 8);
+    s_lam.ensure((size_t)2 * nk * nbs * 8);  // Ritz values, then residual norms (fused path)
     s_work.ensure(heev_work_doubles(nbs, nbs, nk) * 8);
     projall.ensure((size_t)nk * std::max(1, nch_all) * nbs * 16);
     ev_sub.assign((size_t)ns * nk * nbs, 0.0);
@@ -1096,10 +1096,22 @@ void Solve::mix(Mixer& mx, const double* in, const double* out, double* result) 
 }
 
 // Rayleigh-Ritz on span(Y_k) for every k at once, FP64, Löwdin orthonormalisation:
-// S = Y^H Y = U s U^H, X = U s^{-1/2}, H' = X^H (Y^H HY) X = W λ W^H, V = Y·X·W.
+// S = Y^H Y = U s U^H, X = U s^{-1/2}, H' = X^H (Y^H HY) X = W λ W^H, V = Y·X·W; the Ritz values
+// go to ev_sub, the residual norms |HV − λV| of the nb occupied-range bands to resid_host.
+// Blocks of at most 32 bands per k take one fused launch per call.
 void Solve::rayleigh_ritz(const double2* Y, const double2* HY) {
   TimedScope ts(c, "scf_rayleigh_ritz");
   const int m = nbs;
+  if (rr_small_supported(m)) {
+    const size_t nkm = (size_t)nk * m;
+    launch_rr_small(c, npw, m, nb, nk, Y, HY, Vc(), s_lam.as<double>(), s_lam.as<double>() + nkm, 1e-14);
+    rr_host.resize(2 * nkm);
+    PWK_HIP(hipMemcpyAsync(rr_host.data(), s_lam.p, 2 * nkm * 8, hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipStreamSynchronize(c.stream));
+    std::memcpy(ev_sub_c(), rr_host.data(), nkm * 8);
+    std::memcpy(resid_host.data(), rr_host.data() + nkm, nkm * 8);
+    return;
+  }
   const long long blk = (long long)npw * m, sm = (long long)m * m;
   launch_conj_transpose(c, npw, m, Y, npw, blk, YH.p, m, blk, nk);
   launch_zgemm_mfma_batched(c, m, m, npw, YH.p, m, blk, Y, npw, blk, sS.p, m, sm, nk, false, PWK_FP64);
@@ -1115,7 +1127,11 @@ void Solve::rayleigh_ritz(const double2* Y, const double2* HY) {
   launch_heev_batched(c, m, m, nk, sX.p, s_work.p, s_lam.as<double>(), sW.p, nullptr);
   launch_zgemm_mfma_batched(c, m, m, m, sU.p, m, sm, sW.p, m, sm, sC.p, m, sm, nk, false, PWK_FP64);
   launch_zgemm_mfma_batched(c, npw, m, m, Y, npw, blk, sC.p, m, sm, Vc(), npw, blk, nk, false, PWK_FP64);
+  // residuals of the nb bands used for occupations: H·V = (H·Y)·C
+  launch_zgemm_mfma_batched(c, npw, m, m, HY, npw, blk, sC.p, m, sm, R64.p, npw, blk, nk, false, PWK_FP64);
+  launch_resid_norms(c, npw, m, nb, nk, Vc(), R64.p, s_lam.as<double>(), resid.as<double>());
   PWK_HIP(hipMemcpyAsync(ev_sub_c(), s_lam.p, (size_t)nk * nbs * 8, hipMemcpyDeviceToHost, c.stream));
+  PWK_HIP(hipMemcpyAsync(resid_host.data(), resid.p, resid_host.size() * 8, hipMemcpyDeviceToHost, c.stream));
   PWK_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -1238,12 +1254,6 @@ double Solve::chebyshev_pass(int fprec) {
     launch_convert(c, Y, Y64.p, blk * 2, false);
   hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
   rayleigh_ritz(Y64.as<double2>(), HY64.as<double2>());
-  // residuals of the nb bands used for occupations: H·V = (H·Y)·C
-  const long long bl = (long long)npw * nbs, sm = (long long)nbs * nbs;
-  launch_zgemm_mfma_batched(c, npw, nbs, nbs, HY64.p, npw, bl, sC.p, nbs, sm, R64.p, npw, bl, nk, false, PWK_FP64);
-  launch_resid_norms(c, npw, nbs, nb, nk, Vc(), R64.p, s_lam.as<double>(), resid.as<double>());
-  PWK_HIP(hipMemcpyAsync(resid_host.data(), resid.p, resid_host.size() * 8, hipMemcpyDeviceToHost, c.stream));
-  PWK_HIP(hipStreamSynchronize(c.stream));
   double worst = 0.0;
   for (int k = 0; k < nk; ++k)
     for (int b = 0; b < nb; ++b) worst = std::max(worst, resid_host[(size_t)k * nbs + b]);

@@ -4,6 +4,7 @@
 // column scaling, per-block dot products and the seeded start block. All are
 // HBM/L2-bound elementwise passes over nk blocks of npw × nb.
 #include "common.hpp"
+#include "jacobi.hpp"
 
 namespace pwk {
 
@@ -155,7 +156,211 @@ __global__ void __launch_bounds__(256) k_start_block(int npw, int nb, const int*
   V[(size_t)k * npw * nb + idx] = v;
 }
 
+// Rayleigh-Ritz of one k-point's block in one workgroup, for m = nbs <= JMAX: overlap S = YᴴY and
+// projected Hs = Yᴴ(HY), S = U s Uᴴ (Jacobi), X = U s^{-1/2} (Löwdin, s floored at floor_rel·s_max),
+// H' = Xᴴ Hs X = W λ Wᴴ (Jacobi), C = X·W in ascending λ, then V = Y·C and the residual norms
+// |(HY)·C_j − λ_j V_j| of the first nres columns. One launch replaces the transposes, GEMMs,
+// symmetrisations, two batched eigensolves and the residual pass of the general path.
+constexpr int RR_NT = 256;
+constexpr int RR_ROWS = 32;  // plane-wave rows staged in LDS per chunk of the Gram products
+
+struct RrLds {
+  JacobiLds J;
+  double2 Hs[JMAX][JMAX + 1];
+  double2 X[JMAX][JMAX + 1];
+  double2 C[JMAX][JMAX + 1];
+  double2 Ys[RR_ROWS][JMAX + 1], HYs[RR_ROWS][JMAX + 1];
+  double red[16][JMAX];
+};
+
+__global__ void __launch_bounds__(RR_NT) k_rr_small(int npw, int m, int nres, const double2* __restrict__ Yb,
+                                                    const double2* __restrict__ HYb, double2* __restrict__ Vb,
+                                                    double* __restrict__ lam_out, double* __restrict__ res_out,
+                                                    double floor_rel) {
+  extern __shared__ unsigned char smem_raw[];
+  RrLds& L = *reinterpret_cast<RrLds*>(smem_raw);
+  JacobiLds& J = L.J;
+  const int k = blockIdx.x, t = threadIdx.x;
+  const double2* Y = Yb + (size_t)k * npw * m;
+  const double2* HY = HYb + (size_t)k * npw * m;
+  double2* V = Vb + (size_t)k * npw * m;
+  // S(i, j) = Σ_G conj(Y_i) Y_j and Hs(i, j) = Σ_G conj(Y_i) (HY)_j, summed in G order over
+  // row chunks staged in LDS (coalesced loads); a thread owns up to JMAX²/RR_NT entries
+  constexpr int EPT = JMAX * JMAX / RR_NT;
+  double sr[EPT], si[EPT], hr[EPT], hi[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) sr[q] = si[q] = hr[q] = hi[q] = 0.0;
+  for (int g0 = 0; g0 < npw; g0 += RR_ROWS) {
+    for (int e = t; e < RR_ROWS * m; e += RR_NT) {
+      const int gg = e % RR_ROWS, i = e / RR_ROWS, g = g0 + gg;
+      L.Ys[gg][i] = g < npw ? Y[(size_t)i * npw + g] : make_double2(0.0, 0.0);
+      L.HYs[gg][i] = g < npw ? HY[(size_t)i * npw + g] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int rows = min(RR_ROWS, npw - g0);
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = t + q * RR_NT;
+      if (e < m * m) {
+        const int i = e % m, j = e / m;
+        for (int gg = 0; gg < rows; ++gg) {
+          const double2 a = L.Ys[gg][i], b = L.Ys[gg][j], h = L.HYs[gg][j];
+          sr[q] += a.x * b.x + a.y * b.y;
+          si[q] += a.x * b.y - a.y * b.x;
+          hr[q] += a.x * h.x + a.y * h.y;
+          hi[q] += a.x * h.y - a.y * h.x;
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int e = t + q * RR_NT;
+    if (e < m * m) {
+      J.A[e % m][e / m] = make_double2(sr[q], si[q]);
+      L.Hs[e % m][e / m] = make_double2(hr[q], hi[q]);
+    }
+  }
+  __syncthreads();
+  // Hermitian parts (subspace.hip k_hermitize), V = I
+  for (int e = t; e < m * m; e += RR_NT) {
+    const int i = e % m, j = e / m;
+    if (i <= j) {
+      const double2 a = J.A[i][j], b = J.A[j][i];
+      const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+      J.A[i][j] = h;
+      J.A[j][i] = make_double2(h.x, -h.y);
+      const double2 c = L.Hs[i][j], d = L.Hs[j][i];
+      const double2 g = make_double2(0.5 * (c.x + d.x), 0.5 * (c.y - d.y));
+      L.Hs[i][j] = g;
+      L.Hs[j][i] = make_double2(g.x, -g.y);
+    }
+  }
+  for (int e = t; e < m * m; e += RR_NT) J.V[e % m][e / m] = make_double2(e % m == e / m ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  jacobi_lds(m, J);
+  // X = U diag(1/sqrt(max(s_j, floor_rel·s_max)))
+  double smax = J.A[J.order[m - 1]][J.order[m - 1]].x;
+  for (int e = t; e < m * m; e += RR_NT) {
+    const int r = e % m, j = e / m;
+    const double f = 1.0 / sqrt(fmax(J.A[j][j].x, floor_rel * smax));
+    L.X[r][j] = make_double2(J.V[r][j].x * f, J.V[r][j].y * f);
+  }
+  __syncthreads();
+  // T = Hs·X into C, then H' = Xᴴ·T into J.A (Hermitian part), V = I
+  for (int e = t; e < m * m; e += RR_NT) {
+    const int i = e % m, j = e / m;
+    double re = 0.0, im = 0.0;
+    for (int l = 0; l < m; ++l) {
+      const double2 h = L.Hs[i][l], x = L.X[l][j];
+      re += h.x * x.x - h.y * x.y;
+      im += h.x * x.y + h.y * x.x;
+    }
+    L.C[i][j] = make_double2(re, im);
+  }
+  __syncthreads();
+  for (int e = t; e < m * m; e += RR_NT) {
+    const int i = e % m, j = e / m;
+    double re = 0.0, im = 0.0;
+    for (int l = 0; l < m; ++l) {
+      const double2 x = L.X[l][i], c = L.C[l][j];  // conj(x)·c
+      re += x.x * c.x + x.y * c.y;
+      im += x.x * c.y - x.y * c.x;
+    }
+    J.A[i][j] = make_double2(re, im);
+  }
+  __syncthreads();
+  for (int e = t; e < m * m; e += RR_NT) {
+    const int i = e % m, j = e / m;
+    if (i <= j) {
+      const double2 a = J.A[i][j], b = J.A[j][i];
+      const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+      J.A[i][j] = h;
+      J.A[j][i] = make_double2(h.x, -h.y);
+    }
+    J.V[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  jacobi_lds(m, J);
+  // C[:, jj] = X·W[:, order[jj]] (ascending Ritz values)
+  for (int e = t; e < m * m; e += RR_NT) {
+    const int i = e % m, jj = e / m, w = J.order[jj];
+    double re = 0.0, im = 0.0;
+    for (int l = 0; l < m; ++l) {
+      const double2 x = L.X[i][l], v = J.V[l][w];
+      re += x.x * v.x - x.y * v.y;
+      im += x.x * v.y + x.y * v.x;
+    }
+    L.C[i][jj] = make_double2(re, im);
+  }
+  if (t < m) lam_out[(size_t)k * m + t] = J.A[J.order[t]][J.order[t]].x;
+  __syncthreads();
+  // V = Y·C and the residual weights, one row G per thread, eight columns of C at a time (each
+  // row of Y and HY is read m/8 times, coalesced across the wave)
+  constexpr int CB = 8;
+  double acc[JMAX];
+#pragma unroll
+  for (int jj = 0; jj < JMAX; ++jj) acc[jj] = 0.0;
+  for (int g = t; g < npw; g += RR_NT) {
+#pragma unroll
+    for (int jc = 0; jc < JMAX; jc += CB) {
+      if (jc >= m) continue;  // fully unrolled: acc stays in registers
+      double vr[CB], vi[CB], rr[CB], ri[CB];
+#pragma unroll
+      for (int q = 0; q < CB; ++q) vr[q] = vi[q] = rr[q] = ri[q] = 0.0;
+      for (int i = 0; i < m; ++i) {
+        const double2 y = Y[(size_t)i * npw + g], h = HY[(size_t)i * npw + g];
+#pragma unroll
+        for (int q = 0; q < CB; ++q) {
+          const double2 c = L.C[i][min(jc + q, m - 1)];
+          vr[q] += y.x * c.x - y.y * c.y;
+          vi[q] += y.x * c.y + y.y * c.x;
+          rr[q] += h.x * c.x - h.y * c.y;
+          ri[q] += h.x * c.y + h.y * c.x;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < CB; ++q) {
+        const int jj = jc + q;
+        if (jj < m) V[(size_t)jj * npw + g] = make_double2(vr[q], vi[q]);
+        if (jj < nres) {
+          const double l = J.A[J.order[jj]][J.order[jj]].x;
+          const double dx = rr[q] - l * vr[q], dy = ri[q] - l * vi[q];
+          acc[jj] += dx * dx + dy * dy;
+        }
+      }
+    }
+  }
+  const int wave = t >> 6;
+#pragma unroll
+  for (int jj = 0; jj < JMAX; ++jj) {
+    if (jj >= nres) continue;
+    double v = acc[jj];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((t & 63) == 0) L.red[wave][jj] = v;
+  }
+  __syncthreads();
+  if (t < nres) {
+    double v = 0.0;
+    for (int w = 0; w < RR_NT / 64; ++w) v += L.red[w][t];
+    res_out[(size_t)k * m + t] = sqrt(v);
+  }
+}
+
 }  // namespace
+
+bool rr_small_supported(int m) { return m >= 1 && m <= JMAX; }
+
+void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y, const void* HY, void* V,
+                     double* lam, double* res, double floor_rel) {
+  PWK_REQUIRE(rr_small_supported(m) && nres <= m && npw >= m && nk >= 1, PWK_E_INVALID, "bad Rayleigh-Ritz shape");
+  const size_t sh = sizeof(RrLds);
+  ensure_lds_limit((const void*)k_rr_small, sh);
+  k_rr_small<<<(unsigned)nk, RR_NT, sh, c.stream>>>(npw, m, nres, (const double2*)Y, (const double2*)HY,
+                                                   (double2*)V, lam, res, floor_rel);
+  PWK_HIP(hipGetLastError());
+}
 
 void launch_cheb_step(Context& c, size_t n, size_t per, const void* HY, const void* Y, const void* X,
                       const double* coef_dev, void* out, int precision) {

@@ -183,23 +183,25 @@ def test_filtered_solve_fused_and_stick_hpsi_agree(scf_mod, gpu, small_deck_resu
         assert abs(rf[t] - r64[t]) < TOL_TERM * 2, t
 
 
-@pytest.mark.parametrize("eigensolver,precision", [(0, "fp64"), (1, "fp32")])
-def test_orthorhombic_cell_matches_oracle(scf_mod, gpu, oracle, eigensolver, precision):
+@pytest.mark.parametrize("eigensolver,precision,bands", [(0, "fp64", 20), (1, "fp32", 20), (1, "fp64", 30)])
+def test_orthorhombic_cell_matches_oracle(scf_mod, gpu, oracle, eigensolver, precision, bands):
     # an 8-atom Si cell strained to a = (1.12, 0.94, 1.00)·5.431 Å at 8 Ry: a (24, 18, 18) grid,
-    # so the density and the filter's H·psi take the full-grid path; Γ, 20 bands
+    # so the density and the filter's H·psi take the full-grid path; Γ. With 30 bands the filter
+    # holds 38 per k, above the fused Rayleigh-Ritz kernel's 32: the general batched path runs
     from pwk import decks
 
     lat, frac = decks.diamond_supercell(5.431)
     lat = lat * np.array([1.12, 0.94, 1.0])[:, None]
     deck = scf_mod.CrystalDeck(lat, frac, [0] * 8, [SI_UPF])
-    p = scf_mod.ScfParams(ecut_ev=8 * RY, use_symmetry=False, n_bands=20, max_iter=60, conv_threshold=1e-7,
+    p = scf_mod.ScfParams(ecut_ev=8 * RY, use_symmetry=False, n_bands=bands, max_iter=60, conv_threshold=1e-7,
                           energy_threshold=1e-6, smearing_sigma=0.01 * RY, eigensolver=eigensolver,
                           precision=precision)
-    print("orthorhombic", eigensolver, precision, flush=True)
+    print("orthorhombic", eigensolver, precision, bands, flush=True)
     r = scf_mod.run_scf(gpu, deck, p)
     assert tuple(r["dims"]) == (24, 18, 18)
     s = oracle.System(lat, [0] * 8, [14] * 8, frac, [SI_UPF], 8 * RY)
-    ro = s.scf([[0, 0, 0]], [1.0], n_bands=20, max_iter=60, conv=1e-7, ethr=1e-6, beta=0.3, ndim=8, sigma=0.01 * RY)
+    ro = s.scf([[0, 0, 0]], [1.0], n_bands=bands, max_iter=60, conv=1e-7, ethr=1e-6, beta=0.3, ndim=8,
+               sigma=0.01 * RY)
     print("gpu", r["e_total"], r["n_iter"], "oracle", ro["e_total"], ro["n_iter"], flush=True)
     assert r["converged"] == 1 and ro["converged"] == 1.0
     for t in TERMS:
