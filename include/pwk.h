@@ -174,6 +174,16 @@ int pwk_hpsi(pwk_ctx* ctx, int ik, int nb, const double* psi, double* hpsi, int 
  * PWK_SPLIT_BF16) device arrays of npw×nb. */
 int pwk_dev_hpsi(pwk_ctx* ctx, int ik, int nb, const void* psi, void* hpsi, int precision);
 
+/* No counterpart in the reference (its projector products are faer matmuls inside
+ * NonlocalPotential, src/potential/nonlocal.rs:432-450): the complex GEMM on the
+ * matrix cores that libpwk uses for the projectors and the subspace products,
+ * C (+)= A·B column-major on device pointers, batch products with element strides
+ * between them. complex128 for PWK_FP64, complex64 otherwise (PWK_SPLIT_BF16
+ * computes with split-bf16 products). Enqueued on the context's stream. */
+int pwk_dev_zgemm(pwk_ctx* ctx, int m, int n, int k, const void* a, int lda, long long stride_a, const void* b,
+                  int ldb, long long stride_b, void* c, int ldc, long long stride_c, int batch, int accumulate,
+                  int precision);
+
 /* Dense Hamiltonian of k-point ik (src/scf/potentials.rs:163-190, then
  * nonlocal.rs:432-450 when with_nl): h is npw×npw column-major complex128. */
 int pwk_dense_h(pwk_ctx* ctx, int ik, int with_nl, double* h);

@@ -3,6 +3,8 @@
 #include <cstring>
 #include <memory>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "fft_sizes.hpp"
 #include "tables.hpp"
@@ -430,6 +432,21 @@ int pwk_dev_hpsi(pwk_ctx* ctx, int ik, int nb, const void* psi, void* hpsi, int 
     Context& c = ctx_of(ctx);
     check_precision(precision);
     pwk::launch_hpsi(c, ik, nb, psi, hpsi, precision);
+  });
+}
+
+int pwk_dev_zgemm(pwk_ctx* ctx, int m, int n, int k, const void* a, int lda, long long stride_a, const void* b,
+                  int ldb, long long stride_b, void* cmat, int ldc, long long stride_c, int batch, int accumulate,
+                  int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    PWK_REQUIRE(m >= 0 && n >= 0 && k >= 0 && batch >= 1, PWK_E_INVALID, "bad GEMM shape");
+    PWK_REQUIRE(lda >= std::max(1, m) && ldb >= std::max(1, k) && ldc >= std::max(1, m), PWK_E_INVALID,
+                "leading dimension smaller than the matrix");
+    PWK_REQUIRE(m == 0 || n == 0 || (a && b && cmat), PWK_E_INVALID, "null GEMM operand");
+    pwk::launch_zgemm_mfma_batched(c, m, n, k, a, lda, stride_a, b, ldb, stride_b, cmat, ldc, stride_c, batch,
+                                   accumulate != 0, precision);
   });
 }
 

@@ -47,11 +47,14 @@ struct Traits<MODE_F64> {
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 
-// LDS row strides (elements); +pad keeps the 16 lanes of a column group on distinct banks
+// LDS row strides (elements). A rows keep one padding element (the expanded A tile is written
+// column-wise, rows 33 apart spread the lanes over the banks); FP32 B rows are padded to 36
+// floats so each lane's 8 k values are two aligned 16-byte reads (B is written along k, so
+// its stores stay conflict-free); bf16 rows carry eight.
 template <int MODE>
 struct Lds {
-  static constexpr int AK = (MODE == MODE_BF16X3) ? BKR + 8 : BKR + 1;  // A stored [row][k]
-  static constexpr int BK = (MODE == MODE_BF16X3) ? BKR + 8 : BKR + 1;  // B stored [col][k]
+  static constexpr int AK = (MODE == MODE_BF16X3) ? BKR + 8 : BKR + 1;                                // A [row][k]
+  static constexpr int BK = (MODE == MODE_BF16X3) ? BKR + 8 : (MODE == MODE_F32 ? BKR + 4 : BKR + 1);  // B [col][k]
 };
 
 template <int MODE, int NB16>
@@ -92,21 +95,47 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[rb][cb][q] = 0;
 
-  for (int k0 = kbeg; k0 < kend; k0 += BKC) {
-    // ---- A tile: 64 complex rows × 16 complex k, expanded to 128 × 32 real
+  // global -> register prefetch of the next K stage overlaps the MFMA work of the current one
+  constexpr int NA = (BMC * BKC) / 256, NBV = (BKC * BN + 255) / 256;
+  V2 ra[NA], rbv[NBV];
+  auto gload = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < (BMC * BKC) / 256; ++e) {
-      int idx = tid + 256 * e;
-      int mm = idx & (BMC - 1), kk = idx / BMC;
-      int gm = m0 + mm, gk = k0 + kk;
+    for (int e = 0; e < NA; ++e) {
+      const int idx = tid + 256 * e;
+      const int mm = idx & (BMC - 1), kk = idx / BMC;
+      const int gm = m0 + mm, gk = k0 + kk;
       V2 v;
       v.x = 0;
       v.y = 0;
       if (gm < M && gk < kend) v = A[(size_t)gk * lda + gm];
+      ra[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < NBV; ++e) {
+      const int idx = tid + 256 * e;
+      V2 v;
+      v.x = 0;
+      v.y = 0;
+      if (idx < BKC * BN) {
+        const int kk = idx % BKC, nn = idx / BKC;
+        const int gk = k0 + kk, gn = n0 + nn;
+        if (gn < N && gk < kend) v = B[(size_t)gn * ldb + gk];
+      }
+      rbv[e] = v;
+    }
+  };
+  // A tile: 64 complex rows × 16 complex k, expanded to 128 × 32 real; B tile: 16 complex k ×
+  // BN columns, interleaved -> [col][2k+part]
+  auto lstore = [&]() {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      const int idx = tid + 256 * e;
+      const int mm = idx & (BMC - 1), kk = idx / BMC;
+      const V2 v = ra[e];
       T vals[4] = {v.x, -v.y, v.y, v.x};  // (2m,2k) (2m,2k+1) (2m+1,2k) (2m+1,2k+1)
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        int r = 2 * mm + (t >> 1), c = 2 * kk + (t & 1);
+        const int r = 2 * mm + (t >> 1), c = 2 * kk + (t & 1);
         if constexpr (SPLIT) {
           __bf16 hi = to_bf16(vals[t]);
           As[0][r * AK + c] = hi;
@@ -116,17 +145,12 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
         }
       }
     }
-    // ---- B tile: 16 complex k × BN columns, interleaved -> [col][2k+part]
 #pragma unroll
-    for (int e = 0; e < (BKC * BN + 255) / 256; ++e) {
-      int idx = tid + 256 * e;
+    for (int e = 0; e < NBV; ++e) {
+      const int idx = tid + 256 * e;
       if (idx < BKC * BN) {
-        int kk = idx % BKC, nn = idx / BKC;
-        int gk = k0 + kk, gn = n0 + nn;
-        V2 v;
-        v.x = 0;
-        v.y = 0;
-        if (gn < N && gk < kend) v = B[(size_t)gn * ldb + gk];
+        const int kk = idx % BKC, nn = idx / BKC;
+        const V2 v = rbv[e];
         if constexpr (SPLIT) {
           __bf16 hx = to_bf16(v.x), hy = to_bf16(v.y);
           Bs[0][nn * BKS + 2 * kk] = hx;
@@ -139,7 +163,12 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
         }
       }
     }
+  };
+  if (kbeg < kend) gload(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BKC) {
+    lstore();
     __syncthreads();
+    if (k0 + BKC < kend) gload(k0 + BKC);
     const int li = lane & 15, lk = lane >> 4;
     if constexpr (SPLIT) {
       // one 16x16x32 step covers the 32 real k of the stage; lane holds k = lk*8 .. lk*8+7
@@ -158,6 +187,30 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[rb][cb], 0, 0, 0);
         }
       }
+    } else if constexpr (MODE == MODE_F32) {
+      // 8 MFMA k-steps; in step ks lane group lk supplies real k = lk*8 + ks (the sum over the
+      // stage's 32 k values is split differently from the natural order, A and B alike), so
+      // each lane reads its 8 k values of a B column with two 16-byte LDS reads
+      f4v b[NB16][2];
+#pragma unroll
+      for (int cb = 0; cb < NB16; ++cb) {
+        const float* p = &Bs[0][(cb * 16 + li) * BKS + lk * 8];
+        b[cb][0] = *reinterpret_cast<const f4v*>(p);
+        b[cb][1] = *reinterpret_cast<const f4v*>(p + 4);
+      }
+      float a[2][8];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) a[rb][ks] = As[0][(w * 32 + rb * 16 + li) * AK + lk * 8 + ks];
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+#pragma unroll
+        for (int cb = 0; cb < NB16; ++cb)
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][ks], b[cb][ks >> 2][ks & 3], acc[rb][cb], 0, 0, 0);
+      }
     } else {
 #pragma unroll
       for (int ks = 0; ks < BKR / 4; ++ks) {
@@ -169,12 +222,7 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
         for (int cb = 0; cb < NB16; ++cb) {
           T b = Bs[0][(cb * 16 + li) * BKS + kk];
 #pragma unroll
-          for (int rb = 0; rb < 2; ++rb) {
-            if constexpr (MODE == MODE_F64)
-              acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b, acc[rb][cb], 0, 0, 0);
-            else
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb], b, acc[rb][cb], 0, 0, 0);
-          }
+          for (int rb = 0; rb < 2; ++rb) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b, acc[rb][cb], 0, 0, 0);
         }
       }
     }
@@ -287,15 +335,22 @@ template <int MODE>
 void zgemm_mfma_t(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
                   bool accumulate, const Batch& bt) {
   using V2 = typename Traits<MODE>::V2;
+  const int cus = c.cu_count > 0 ? c.cu_count : 256;
   int nb16 = std::min(9, (N + 15) / 16);
+  int mt = (M + BMC - 1) / BMC;
+  // fewer tiles than CUs with wide column blocks: halve the block width so that every CU gets
+  // work (the A tile is then read by two column blocks)
+  if ((long)mt * ((N + 16 * nb16 - 1) / (16 * nb16)) * bt.count < cus && nb16 > 4 && (long)mt * bt.count >= cus / 4)
+    nb16 = (nb16 + 1) / 2;
   int BN = 16 * nb16;
-  int mt = (M + BMC - 1) / BMC, nt = (N + BN - 1) / BN;
+  int nt = (N + BN - 1) / BN;
   long tiles = (long)mt * nt * bt.count;
-  int target = 2 * (c.cu_count > 0 ? c.cu_count : 256);
+  int target = 2 * cus;
   int nsplit = 1;
-  // split K only for K >= 256 (short reductions are cheaper than the extra partial-sum pass),
-  // then into chunks of at least 4 stages
-  if (tiles < target && K >= 16 * BKC)
+  // split K only for K >= 256 (short reductions are cheaper than the extra partial-sum pass) and
+  // when the tiles fill less than half the CUs (above that the partial-sum round trip costs
+  // more than the idle CUs), then into chunks of at least 4 stages
+  if (tiles < target / 4 && K >= 16 * BKC)
     nsplit = (int)std::min<long>((target + tiles - 1) / tiles, std::max(1, K / (4 * BKC)));
   int kchunk = (K + nsplit - 1) / nsplit;
   kchunk = ((kchunk + BKC - 1) / BKC) * BKC;

@@ -52,6 +52,8 @@ _SIGS = {
     "pwk_ctx_sync": (C.c_int, [_vp]),
     "pwk_ctx_stream": (_vp, [_vp]),
     "pwk_ctx_set_fused_hpsi": (C.c_int, [_vp, C.c_int]),
+    "pwk_dev_zgemm": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_longlong, _vp, C.c_int,
+                                C.c_longlong, _vp, C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "pwk_profile_enable": (C.c_int, [_vp, C.c_int]),
     "pwk_profile_reset": (C.c_int, [_vp]),
     "pwk_profile_read": (C.c_int, [_vp, C.c_int, C.c_char_p, _dp, C.POINTER(C.c_int64)]),
@@ -163,6 +165,12 @@ class GpuAccelerator:
 
     def sync(self):
         _check(lib().pwk_ctx_sync(self.h))
+
+    def zgemm_dev(self, m, n, k, a_ptr, lda, b_ptr, ldb, c_ptr, ldc, precision="fp64", accumulate=False, batch=1,
+                  strides=(0, 0, 0)):
+        """C (+)= A·B on device pointers (column-major), pwk_dev_zgemm."""
+        _check(lib().pwk_dev_zgemm(self.h, m, n, k, a_ptr, lda, strides[0], b_ptr, ldb, strides[1], c_ptr, ldc,
+                                   strides[2], batch, int(accumulate), _prec(precision)))
 
     def set_fused_hpsi(self, on=True):
         """Fused one-launch H·psi on small cubic grids (default on) or the z-stick path."""

@@ -36,6 +36,14 @@ def pwk_mod():
 
 @pytest.fixture(scope="session")
 def gpu(pwk_mod):
+    # torch (used by a few tests for device buffers) ships its own HIP runtime; it has to
+    # initialise the device before libpwk's runtime does
+    try:
+        import torch
+
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     acc = pwk_mod.GpuAccelerator(0)
     yield acc
     acc.close()

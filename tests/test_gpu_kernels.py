@@ -93,6 +93,36 @@ def test_lsda_xc(acc, si_sys, oracle):
         assert np.abs(a - b).max() < 0.015
 
 
+# ---- MFMA complex GEMM (the projector and subspace products) -----------------------
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 2e-5), ("split_bf16", 5e-5)])
+def test_zgemm_mfma_shapes(gpu, prec, tol):
+    # edge tiles (M, N, K not multiples of 64 / 16 / 16), split-K (long K, few tiles), 144-wide
+    # column blocks, accumulation and a strided batch, against numpy complex128
+    import torch
+
+    rng = np.random.default_rng(21)
+    dt = torch.complex128 if prec == "fp64" else torch.complex64
+    for (m, n, k, acc, batch) in [(1, 1, 1, False, 1), (17, 5, 3, True, 1), (64, 144, 16, False, 1),
+                                  (65, 145, 300, True, 1), (40, 24, 5000, False, 1), (700, 144, 512, True, 1),
+                                  (33, 20, 169, False, 8)]:
+        A = rng.standard_normal((batch, m, k)) + 1j * rng.standard_normal((batch, m, k))
+        B = rng.standard_normal((batch, k, n)) + 1j * rng.standard_normal((batch, k, n))
+        C0 = rng.standard_normal((batch, m, n)) + 1j * rng.standard_normal((batch, m, n))
+        ref = A @ B + (C0 if acc else 0)
+        # column-major storage of each product = row-major transposes, batches back to back
+        dA = torch.from_numpy(np.ascontiguousarray(A.transpose(0, 2, 1))).to(dt).cuda()
+        dB = torch.from_numpy(np.ascontiguousarray(B.transpose(0, 2, 1))).to(dt).cuda()
+        dC = torch.from_numpy(np.ascontiguousarray(C0.transpose(0, 2, 1))).to(dt).cuda()
+        torch.cuda.synchronize()
+        gpu.zgemm_dev(m, n, k, dA.data_ptr(), m, dB.data_ptr(), k, dC.data_ptr(), m, precision=prec,
+                      accumulate=acc, batch=batch, strides=(m * k, k * n, m * n))
+        gpu.sync()
+        out = dC.cpu().numpy().astype(np.complex128).transpose(0, 2, 1)
+        err = np.abs(out - ref).max() / np.abs(ref).max()
+        print("zgemm", prec, (m, n, k, acc, batch), f"{err:.2e}", flush=True)
+        assert err < tol, (m, n, k, acc, batch)
+
+
 # ---- FFT ------------------------------------------------------------------------
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 3e-6)])
 def test_fft3d_matches_dft(gpu, prec, tol):
