@@ -192,6 +192,8 @@ void launch_convert(Context& c, const void* src, void* dst, size_t n, bool to_fp
 // fft3d.hip
 bool fft_size_supported(int n);
 void launch_fft3d(Context& c, void* data, int batch, int sign, int precision);
+// out (double2 N) = FFT_sign(r·scale + 0i): one launch on small cubic grids
+void launch_fft3d_r2c(Context& c, const double* r, double scale, void* out, int sign);
 // hpsi.hip
 void setup_basis(Context& c, int npw, const int32_t* miller_host);
 void set_veff_device(Context& c, const void* veff_g_dev);

@@ -1,0 +1,69 @@
+// Line passes of a 3D FFT held in LDS (grid G[x][y][z], z fastest), shared by the fused
+// small-grid kernels: one lane per line, the line in registers (fft_reg.hpp).
+#pragma once
+
+#include "fft_reg.hpp"
+
+namespace pwk {
+
+template <typename T>
+struct V2s;
+template <>
+struct V2s<double> {
+  using t = double2;
+};
+template <>
+struct V2s<float> {
+  using t = float2;
+};
+
+constexpr int SMALL_NT = 256;
+
+// One set of N² lines through the LDS grid G[x][y][z] (z fastest). AXIS 2: z-lines
+// (stride 1), 1: y-lines (stride N), 0: x-lines (stride N²). With vr, the x pass runs
+// inverse -> × V(r) -> forward on each line while it is in registers.
+template <typename T, int N, int AXIS, int SIGN>
+__device__ __forceinline__ void lds_lines(typename V2s<T>::t* G, const T* __restrict__ vr) {
+  constexpr int es = AXIS == 2 ? 1 : AXIS == 1 ? N : N * N;
+  for (int l = threadIdx.x; l < N * N; l += SMALL_NT) {
+    int base;
+    if constexpr (AXIS == 2)
+      base = l * N;  // l = x·N + y
+    else if constexpr (AXIS == 1)
+      base = (l / N) * N * N + (l % N);  // l = x·N + z
+    else
+      base = l;  // l = y·N + z
+    T re[N], im[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      re[j] = G[base + j * es].x;
+      im[j] = G[base + j * es].y;
+    }
+    fft_dif<T, N, SIGN>(re, im);
+    if constexpr (AXIS == 0) {
+      if (vr) {
+        T r2[N], m2[N];
+#pragma unroll
+        for (int f = 0; f < N; ++f) {
+          const T v = vr[base + f * es];
+          r2[f] = re[FftOrder<N>::pos(f)] * v;
+          m2[f] = im[FftOrder<N>::pos(f)] * v;
+        }
+        fft_dif<T, N, -SIGN>(r2, m2);
+#pragma unroll
+        for (int f = 0; f < N; ++f) {
+          G[base + f * es].x = r2[FftOrder<N>::pos(f)];
+          G[base + f * es].y = m2[FftOrder<N>::pos(f)];
+        }
+        continue;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < N; ++f) {
+      G[base + f * es].x = re[FftOrder<N>::pos(f)];
+      G[base + f * es].y = im[FftOrder<N>::pos(f)];
+    }
+  }
+}
+
+}  // namespace pwk

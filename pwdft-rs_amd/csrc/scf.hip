@@ -191,11 +191,6 @@ __global__ void __launch_bounds__(256) k_ewald_real(EwaldArgs a, double* partial
 // ---------------------------------------------------------------------------------------
 // grid helpers
 // ---------------------------------------------------------------------------------------
-__global__ void k_r2c_scaled(size_t n, const double* __restrict__ r, double s, double2* __restrict__ c) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) c[i] = make_double2(r[i] * s, 0.0);
-}
-
 __global__ void k_c2r_scaled(size_t n, const double2* __restrict__ c, double s, double* __restrict__ r,
                              int clamp_nonneg) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -559,11 +554,7 @@ struct Solve {
   }
 
   // ρ(r) → ρ(G) = FFT_fwd(ρ)/N  (src/scf/energy.rs:250-259)
-  void r_to_g(const double* r, double2* g) {
-    k_r2c_scaled<<<gb(N), 256, 0, c.stream>>>(N, r, 1.0 / (double)N, g);
-    PWK_HIP(hipGetLastError());
-    launch_fft3d(c, g, 1, -1, PWK_FP64);
-  }
+  void r_to_g(const double* r, double2* g) { launch_fft3d_r2c(c, r, 1.0 / (double)N, g, -1); }
 
   // Chebyshev-filtered subspace iteration state
   bool chebyshev = false;
@@ -1051,8 +1042,7 @@ void Solve::mix(Mixer& mx, const double* in, const double* out, double* result) 
   k_sub<<<gb(N), 256, 0, c.stream>>>(N, out, in, hres);
   PWK_HIP(hipGetLastError());
   if (mx.kerker) {
-    k_r2c_scaled<<<gb(N), 256, 0, c.stream>>>(N, hres, 1.0, cwork.as<double2>());
-    launch_fft3d(c, cwork.p, 1, -1, PWK_FP64);
+    launch_fft3d_r2c(c, hres, 1.0, cwork.p, -1);
     k_kerker<<<gb(N), 256, 0, c.stream>>>(N, c.g2_64.as<double>(), kerker_q2, cwork.as<double2>());
     launch_fft3d(c, cwork.p, 1, +1, PWK_FP64);
     k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, cwork.as<double2>(), 1.0 / (double)N, hres, 0);
@@ -1421,8 +1411,7 @@ void Solve::run(double* eig_out, double* rho_out) {
       // ---- symmetrisation (src/symmetry/density/g_space.rs) --------------------------------
       if (ops.size() > 1) {
         TimedScope ts(c, "scf_symmetrize");
-        k_r2c_scaled<<<gb(N), 256, 0, c.stream>>>(N, rn, 1.0, cwork.as<double2>());
-        launch_fft3d(c, cwork.p, 1, -1, PWK_FP64);
+        launch_fft3d_r2c(c, rn, 1.0, cwork.p, -1);
         k_sym_gather<<<gb(N), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
                                                    sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
                                                    rho_g_new.as<double2>(), 1.0 / (double)ops.size());
