@@ -196,7 +196,10 @@ void launch_fft3d(Context& c, void* data, int batch, int sign, int precision);
 void launch_fft3d_r2c(Context& c, const double* r, double scale, void* out, int sign);
 // hpsi.hip
 void setup_basis(Context& c, int npw, const int32_t* miller_host);
-void set_veff_device(Context& c, const void* veff_g_dev);
+// V_eff(G) on the device -> V(r) for H·ψ; slot 1 holds a second channel (LSDA) for the fused
+// small-grid H·ψ and needs reserve_veff_slots(c, 2) first
+void reserve_veff_slots(Context& c, int nslots);
+void set_veff_device(Context& c, const void* veff_g_dev, int slot = 0);
 void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int precision);
 void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision);
 void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, int precision, const void* kin,
@@ -204,9 +207,11 @@ void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, 
 // hpsi_small.hip: the whole H·ψ of nbands bands (band b at k-point b / nbk) in one launch;
 // with coef (α, c, β per k-point, device) the output is α(Hψ − cψ) − βX instead
 bool hpsi_small_supported(const Context& c, int precision, int nch);
+// nkv > 0: the band blocks of consecutive groups of nkv k-points use V(r) slot 0, 1, ...
+// (spin channels of LSDA; kinetic and projector blocks are indexed by block mod nkv)
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
                        int nch, const void* B, const void* DBH, const double* coef = nullptr,
-                       const void* X = nullptr);
+                       const void* X = nullptr, int nkv = 0);
 // ρ += Σ_b fw_b |ψ_b(r)|² on small cubic grids (fw on the device)
 bool density_small_supported(const Context& c, int precision);
 void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision);

@@ -561,23 +561,35 @@ void setup_basis(Context& c, int npw, const int32_t* m) {
   up(c.st_x1, sx1.data(), sx1.size() * sizeof(int32_t));
 }
 
-void set_veff_device(Context& c, const void* veff_g_dev) {
+void reserve_veff_slots(Context& c, int nslots) {
   c.check_grid();
-  size_t n = c.N;
-  c.veffg64.ensure(n * sizeof(double2));
+  const size_t n = c.N * (size_t)std::max(1, nslots);
   c.vr64.ensure(n * sizeof(double));
   c.vr32.ensure(n * sizeof(float));
   c.vrn64.ensure(n * sizeof(double));
   c.vrn32.ensure(n * sizeof(float));
+}
+
+void set_veff_device(Context& c, const void* veff_g_dev, int slot) {
+  c.check_grid();
+  size_t n = c.N;
+  PWK_REQUIRE(slot >= 0 && slot <= 1, PWK_E_INVALID, "V_eff slot must be 0 or 1");
+  if (slot == 0) {
+    reserve_veff_slots(c, 1);
+    c.veffg64.ensure(n * sizeof(double2));
+    PWK_HIP(hipMemcpyAsync(c.veffg64.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
+  } else {
+    PWK_REQUIRE(c.vrn64.bytes >= 2 * n * sizeof(double), PWK_E_STATE, "second V_eff slot not reserved");
+  }
   c.work_b.ensure(n * sizeof(double2));
-  PWK_HIP(hipMemcpyAsync(c.veffg64.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
   PWK_HIP(hipMemcpyAsync(c.work_b.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
   launch_fft3d(c, c.work_b.p, 1, +1, PWK_FP64);
-  k_vr_reorder<<<blocks_for(n, 256), 256, 0, c.stream>>>(c.work_b.as<double2>(), c.vr64.as<double>(),
-                                                         c.vr32.as<float>(), c.vrn64.as<double>(),
-                                                         c.vrn32.as<float>(), c.dims[0], c.dims[1], c.dims[2]);
+  const size_t o = (size_t)slot * n;
+  k_vr_reorder<<<blocks_for(n, 256), 256, 0, c.stream>>>(c.work_b.as<double2>(), c.vr64.as<double>() + o,
+                                                         c.vr32.as<float>() + o, c.vrn64.as<double>() + o,
+                                                         c.vrn32.as<float>() + o, c.dims[0], c.dims[1], c.dims[2]);
   PWK_HIP(hipGetLastError());
-  c.have_v = true;
+  if (slot == 0) c.have_v = true;
 }
 
 void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int precision) {

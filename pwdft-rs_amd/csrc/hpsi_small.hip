@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
                                                          const typename V2s<T>::t* __restrict__ B,
                                                          const typename V2s<T>::t* __restrict__ DBH, T inv_n,
                                                          const double* __restrict__ coef,
-                                                         const typename V2s<T>::t* __restrict__ X) {
+                                                         const typename V2s<T>::t* __restrict__ X, int nkv) {
   using V2 = typename V2s<T>::t;
   extern __shared__ unsigned char smem_raw[];
   V2* G = reinterpret_cast<V2*>(smem_raw);
@@ -39,7 +39,10 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
   // on XCD i % 8; each XCD takes a contiguous range of bands, which keeps the bands of one
   // k-point (and its projector blocks) in one XCD's L2
   const int q = (int)gridDim.x >> 3, r = (int)gridDim.x & 7, xcd = (int)blockIdx.x & 7;
-  const int b = xcd * q + min(xcd, r) + ((int)blockIdx.x >> 3), k = b / nbk;
+  const int b = xcd * q + min(xcd, r) + ((int)blockIdx.x >> 3), kb = b / nbk;
+  // band block kb: k-point kb mod nkv, V(r) channel kb / nkv (coefficients per block)
+  const int k = nkv > 0 ? kb % nkv : kb;
+  if (nkv > 0) vr += (size_t)(kb / nkv) * N * N * N;
   const V2* p = psi + (size_t)b * npw;
   for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
     G[i].x = 0;
@@ -84,9 +87,9 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
   V2* h = hpsi + (size_t)b * npw;
   T alpha = 0, cshift = 0, beta = 0;
   if (coef) {
-    alpha = (T)coef[3 * k];
-    cshift = (T)coef[3 * k + 1];
-    beta = (T)coef[3 * k + 2];
+    alpha = (T)coef[3 * kb];
+    cshift = (T)coef[3 * kb + 1];
+    beta = (T)coef[3 * kb + 2];
   }
   const V2* xb = X ? X + (size_t)b * npw : nullptr;
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
@@ -213,7 +216,7 @@ bool small_len(int n) {
 
 template <typename T>
 void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi, const void* kin, int nch,
-                    const void* B, const void* DBH, const double* coef, const void* X) {
+                    const void* B, const void* DBH, const double* coef, const void* X, int nkv) {
   using V2 = typename V2s<T>::t;
   const int n = c.dims[0];
   const size_t sh = small_lds_bytes<T>(n, nch);
@@ -226,7 +229,7 @@ void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi
     ensure_lds_limit((const void*)k_hpsi_small<T, L>, sh);                                                    \
     k_hpsi_small<T, L><<<(unsigned)nbands, SMALL_NT, sh, c.stream>>>(                                          \
         (const V2*)psi, (V2*)hpsi, (const T*)kin, vr, c.g2fft.as<int>(), c.npw, nbk, nch, (const V2*)B,        \
-        (const V2*)DBH, inv_n, coef, (const V2*)X);                                                            \
+        (const V2*)DBH, inv_n, coef, (const V2*)X, nkv);                                                       \
     break;
     PWK_SMALL_SIZES(PWK_CASE)
 #undef PWK_CASE
@@ -290,15 +293,19 @@ bool hpsi_small_supported(const Context& c, int precision, int nch) {
 }
 
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
-                       int nch, const void* B, const void* DBH, const double* coef, const void* X) {
+                       int nch, const void* B, const void* DBH, const double* coef, const void* X, int nkv) {
   c.check_basis();
   PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
-  PWK_REQUIRE(nbands > 0 && nbk > 0, PWK_E_INVALID, "band counts must be positive");
+  PWK_REQUIRE(nbands > 0 && nbk > 0 && nbands % nbk == 0, PWK_E_INVALID, "band counts must be positive");
   PWK_REQUIRE(hpsi_small_supported(c, precision, nch), PWK_E_UNSUPPORTED, "grid too large for the fused kernel");
+  if (nkv > 0) {
+    const size_t slots = ((size_t)(nbands / nbk) + nkv - 1) / nkv;
+    PWK_REQUIRE(c.vrn64.bytes >= slots * c.N * sizeof(double), PWK_E_STATE, "V_eff slots not set");
+  }
   if (precision == PWK_FP64)
-    launch_small_t<double>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X);
+    launch_small_t<double>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv);
   else
-    launch_small_t<float>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X);
+    launch_small_t<float>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv);
 }
 
 }  // namespace pwk

@@ -512,6 +512,10 @@ struct Solve {
   // spin-up channel first
   int ns = 1;
   int sc = 0;  // channel the eigensolver works on
+  // LSDA with the filtered solver on small grids: both channels go through each eigensolver
+  // launch as 2·nk blocks (V(r) slot per channel); nke = blocks per eigensolver launch
+  bool fold = false;
+  int nke = 0;
 
   // device arrays: rho_in / rho_new / rho_xc / vxc / veff_g hold ns channels of N points;
   // t_in, m_in, t_out, m_out are ρ_total and m = ρ↑ − ρ↓ of the input and output density
@@ -575,6 +579,7 @@ struct Solve {
   std::vector<double> coef_host;
 
   size_t vblk() const { return (size_t)nk * npw * nbs; }  // eigenvector elements per channel
+  size_t eblk() const { return (size_t)nke * npw * nbs; }  // eigenvector elements per eigensolver call
   double2* Vc() { return V.as<double2>() + (size_t)sc * vblk(); }
   double* ev_sub_c() { return ev_sub.data() + (size_t)sc * nk * nbs; }
   double* ub_c() { return ub.data() + (size_t)sc * nk; }
@@ -585,6 +590,11 @@ struct Solve {
   const void* DBH_of(int k) { return DBHall64.as<double2>() + (size_t)k * npw * nch_all; }
   void hpsi_all(const void* psi, void* hpsi, int nbands, int prec) {
     const bool f64 = prec == PWK_FP64;
+    if (fold) {
+      launch_hpsi_small(c, nke * nbands, nbands, psi, hpsi, prec, f64 ? kin_all64.p : kin_all32.p, nch_all,
+                        f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, nullptr, nullptr, nk);
+      return;
+    }
     launch_hpsi_multi(c, nk, nbands, psi, hpsi, prec, f64 ? kin_all64.p : kin_all32.p, nch_all,
                       f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, projall.p);
   }
@@ -593,12 +603,12 @@ struct Solve {
   void hpsi_cheb(const void* Y, const void* X, const double* cf, void* out, int prec) {
     const bool f64 = prec == PWK_FP64;
     if (prec != PWK_SPLIT_BF16 && hpsi_small_supported(c, prec, nch_all)) {
-      launch_hpsi_small(c, nk * nbs, nbs, Y, out, prec, f64 ? kin_all64.p : kin_all32.p, nch_all,
-                        f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, cf, X);
+      launch_hpsi_small(c, nke * nbs, nbs, Y, out, prec, f64 ? kin_all64.p : kin_all32.p, nch_all,
+                        f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, cf, X, fold ? nk : 0);
       return;
     }
     hpsi_all(Y, HYf.p, nbs, prec);
-    launch_cheb_step(c, vblk(), (size_t)npw * nbs, HYf.p, Y, X, cf, out, prec);
+    launch_cheb_step(c, eblk(), (size_t)npw * nbs, HYf.p, Y, X, cf, out, prec);
   }
 
   void setup();
@@ -685,6 +695,7 @@ void Solve::setup() {
   }
   kpts = kpoints(lat, grid, shift, p.use_symmetry == 2 ? identity_ops() : ops, p.use_symmetry == 2 ? false : trev);
   nk = (int)kpts.size();
+  nke = nk;
   wk.resize((size_t)nk);
   for (int k = 0; k < nk; ++k) wk[(size_t)k] = kpts[(size_t)k].weight;
   c.kd.clear();
@@ -914,20 +925,25 @@ void Solve::setup() {
     PWK_REQUIRE(heev_supported(nbs, nbs), PWK_E_UNSUPPORTED, "too many bands for the subspace eigensolver");
     fprec = p.precision;
     degree = p.cheb_degree > 0 ? p.cheb_degree : 10;
-    const size_t blk = (size_t)nk * npw * nbs;
+    if (ns == 2 && fprec != PWK_SPLIT_BF16) {
+      reserve_veff_slots(c, 2);  // V(r) of both channels stays resident
+      fold = hpsi_small_supported(c, PWK_FP64, nch_all) && hpsi_small_supported(c, fprec, nch_all);
+    }
+    nke = fold ? ns * nk : nk;
+    const size_t blk = (size_t)nke * npw * nbs;
     // FP64-sized filter buffers: a filtered solve switches its H·ψ to FP64 once the residual
     // target falls below what FP32 resolves
     for (DevBuf* b : {&F0, &F1, &F2, &HYf}) b->ensure(blk * 16);
     for (DevBuf* b : {&Y64, &HY64, &YH, &R64}) b->ensure(blk * 16);
-    resid.ensure((size_t)nk * nbs * 8);
-    resid_host.assign((size_t)nk * nbs, 0.0);
-    for (DevBuf* b : {&sS, &sH, &sU, &sT, &sX, &sXH, &sW, &sC}) b->ensure((size_t)nk * nbs * nbs * 16);
-    s_ev.ensure((size_t)nk * nbs * 8);
-    s_lam.ensure((size_t)2 * nk * nbs * 8);  // Ritz values, then residual norms (fused path)
-    s_work.ensure(heev_work_doubles(nbs, nbs, nk) * 8);
+    resid.ensure((size_t)nke * nbs * 8);
+    resid_host.assign((size_t)nke * nbs, 0.0);
+    for (DevBuf* b : {&sS, &sH, &sU, &sT, &sX, &sXH, &sW, &sC}) b->ensure((size_t)nke * nbs * nbs * 16);
+    s_ev.ensure((size_t)nke * nbs * 8);
+    s_lam.ensure((size_t)2 * nke * nbs * 8);  // Ritz values, then residual norms (fused path)
+    s_work.ensure(heev_work_doubles(nbs, nbs, nke) * 8);
     projall.ensure((size_t)nk * std::max(1, nch_all) * nbs * 16);
     ev_sub.assign((size_t)ns * nk * nbs, 0.0);
-    coef.ensure((size_t)std::max(degree, 1) * nk * 3 * 8);
+    coef.ensure((size_t)std::max(degree, 1) * nke * 3 * 8);
     ub.assign((size_t)ns * nk, 0.0);
     // start block: the nb lowest-kinetic plane waves of each k (plus a seeded perturbation)
     lowest.ensure((size_t)nbs * nk * 4);
@@ -1093,8 +1109,8 @@ void Solve::rayleigh_ritz(const double2* Y, const double2* HY) {
   TimedScope ts(c, "scf_rayleigh_ritz");
   const int m = nbs;
   if (rr_small_supported(m)) {
-    const size_t nkm = (size_t)nk * m;
-    launch_rr_small(c, npw, m, nb, nk, Y, HY, Vc(), s_lam.as<double>(), s_lam.as<double>() + nkm, 1e-14);
+    const size_t nkm = (size_t)nke * m;
+    launch_rr_small(c, npw, m, nb, nke, Y, HY, Vc(), s_lam.as<double>(), s_lam.as<double>() + nkm, 1e-14);
     rr_host.resize(2 * nkm);
     PWK_HIP(hipMemcpyAsync(rr_host.data(), s_lam.p, 2 * nkm * 8, hipMemcpyDeviceToHost, c.stream));
     PWK_HIP(hipStreamSynchronize(c.stream));
@@ -1103,24 +1119,24 @@ void Solve::rayleigh_ritz(const double2* Y, const double2* HY) {
     return;
   }
   const long long blk = (long long)npw * m, sm = (long long)m * m;
-  launch_conj_transpose(c, npw, m, Y, npw, blk, YH.p, m, blk, nk);
-  launch_zgemm_mfma_batched(c, m, m, npw, YH.p, m, blk, Y, npw, blk, sS.p, m, sm, nk, false, PWK_FP64);
-  launch_zgemm_mfma_batched(c, m, m, npw, YH.p, m, blk, HY, npw, blk, sH.p, m, sm, nk, false, PWK_FP64);
-  launch_hermitize(c, m, sS.p, nk);
-  launch_hermitize(c, m, sH.p, nk);
-  launch_heev_batched(c, m, m, nk, sS.p, s_work.p, s_ev.as<double>(), sU.p, nullptr);
-  launch_lowdin_scale(c, m, s_ev.as<double>(), 1e-14, sU.p, nk);  // sU is now X
-  launch_zgemm_mfma_batched(c, m, m, m, sH.p, m, sm, sU.p, m, sm, sT.p, m, sm, nk, false, PWK_FP64);
-  launch_conj_transpose(c, m, m, sU.p, m, sm, sXH.p, m, sm, nk);
-  launch_zgemm_mfma_batched(c, m, m, m, sXH.p, m, sm, sT.p, m, sm, sX.p, m, sm, nk, false, PWK_FP64);
-  launch_hermitize(c, m, sX.p, nk);
-  launch_heev_batched(c, m, m, nk, sX.p, s_work.p, s_lam.as<double>(), sW.p, nullptr);
-  launch_zgemm_mfma_batched(c, m, m, m, sU.p, m, sm, sW.p, m, sm, sC.p, m, sm, nk, false, PWK_FP64);
-  launch_zgemm_mfma_batched(c, npw, m, m, Y, npw, blk, sC.p, m, sm, Vc(), npw, blk, nk, false, PWK_FP64);
+  launch_conj_transpose(c, npw, m, Y, npw, blk, YH.p, m, blk, nke);
+  launch_zgemm_mfma_batched(c, m, m, npw, YH.p, m, blk, Y, npw, blk, sS.p, m, sm, nke, false, PWK_FP64);
+  launch_zgemm_mfma_batched(c, m, m, npw, YH.p, m, blk, HY, npw, blk, sH.p, m, sm, nke, false, PWK_FP64);
+  launch_hermitize(c, m, sS.p, nke);
+  launch_hermitize(c, m, sH.p, nke);
+  launch_heev_batched(c, m, m, nke, sS.p, s_work.p, s_ev.as<double>(), sU.p, nullptr);
+  launch_lowdin_scale(c, m, s_ev.as<double>(), 1e-14, sU.p, nke);  // sU is now X
+  launch_zgemm_mfma_batched(c, m, m, m, sH.p, m, sm, sU.p, m, sm, sT.p, m, sm, nke, false, PWK_FP64);
+  launch_conj_transpose(c, m, m, sU.p, m, sm, sXH.p, m, sm, nke);
+  launch_zgemm_mfma_batched(c, m, m, m, sXH.p, m, sm, sT.p, m, sm, sX.p, m, sm, nke, false, PWK_FP64);
+  launch_hermitize(c, m, sX.p, nke);
+  launch_heev_batched(c, m, m, nke, sX.p, s_work.p, s_lam.as<double>(), sW.p, nullptr);
+  launch_zgemm_mfma_batched(c, m, m, m, sU.p, m, sm, sW.p, m, sm, sC.p, m, sm, nke, false, PWK_FP64);
+  launch_zgemm_mfma_batched(c, npw, m, m, Y, npw, blk, sC.p, m, sm, Vc(), npw, blk, nke, false, PWK_FP64);
   // residuals of the nb bands used for occupations: H·V = (H·Y)·C
-  launch_zgemm_mfma_batched(c, npw, m, m, HY, npw, blk, sC.p, m, sm, R64.p, npw, blk, nk, false, PWK_FP64);
-  launch_resid_norms(c, npw, m, nb, nk, Vc(), R64.p, s_lam.as<double>(), resid.as<double>());
-  PWK_HIP(hipMemcpyAsync(ev_sub_c(), s_lam.p, (size_t)nk * nbs * 8, hipMemcpyDeviceToHost, c.stream));
+  launch_zgemm_mfma_batched(c, npw, m, m, HY, npw, blk, sC.p, m, sm, R64.p, npw, blk, nke, false, PWK_FP64);
+  launch_resid_norms(c, npw, m, nb, nke, Vc(), R64.p, s_lam.as<double>(), resid.as<double>());
+  PWK_HIP(hipMemcpyAsync(ev_sub_c(), s_lam.p, (size_t)nke * nbs * 8, hipMemcpyDeviceToHost, c.stream));
   PWK_HIP(hipMemcpyAsync(resid_host.data(), resid.p, resid_host.size() * 8, hipMemcpyDeviceToHost, c.stream));
   PWK_HIP(hipStreamSynchronize(c.stream));
 }
@@ -1133,41 +1149,41 @@ void Solve::lanczos_bounds() {
   const int steps = std::min(10, npw);
   const size_t per = (size_t)npw * 2;  // reals per k
   DevBuf a, b0, b1, none, red, cf;
-  a.ensure((size_t)nk * npw * 16);
-  b0.ensure((size_t)nk * npw * 16);
-  b1.ensure((size_t)nk * npw * 16);
-  none.ensure((size_t)nk * 4);
-  red.ensure((size_t)nk * 8);
-  cf.ensure((size_t)nk * 3 * 8);
+  a.ensure((size_t)nke * npw * 16);
+  b0.ensure((size_t)nke * npw * 16);
+  b1.ensure((size_t)nke * npw * 16);
+  none.ensure((size_t)nke * 4);
+  red.ensure((size_t)nke * 8);
+  cf.ensure((size_t)nke * 3 * 8);
   double2* v0 = a.as<double2>();
   double2* v1 = b0.as<double2>();
   double2* w = b1.as<double2>();
-  const std::vector<int> minus((size_t)nk, -1);
+  const std::vector<int> minus((size_t)nke, -1);
   PWK_HIP(hipMemcpyAsync(none.p, minus.data(), minus.size() * 4, hipMemcpyHostToDevice, c.stream));
-  launch_start_block(c, npw, 1, nk, none.as<int>(), v1);
-  PWK_HIP(hipMemsetAsync(v0, 0, (size_t)nk * npw * 16, c.stream));
-  launch_block_dot(c, per, nk, (const double*)v1, (const double*)v1, red.as<double>());
-  launch_block_scale(c, per, nk, red.as<double>(), 1, (double*)v1);
-  std::vector<std::vector<double>> al((size_t)nk), be((size_t)nk);
-  std::vector<double> hv((size_t)nk), cfh((size_t)nk * 3), beta((size_t)nk, 0.0), inv((size_t)nk);
+  launch_start_block(c, npw, 1, nke, none.as<int>(), v1);
+  PWK_HIP(hipMemsetAsync(v0, 0, (size_t)nke * npw * 16, c.stream));
+  launch_block_dot(c, per, nke, (const double*)v1, (const double*)v1, red.as<double>());
+  launch_block_scale(c, per, nke, red.as<double>(), 1, (double*)v1);
+  std::vector<std::vector<double>> al((size_t)nke), be((size_t)nke);
+  std::vector<double> hv((size_t)nke), cfh((size_t)nke * 3), beta((size_t)nke, 0.0), inv((size_t)nke);
   for (int j = 0; j < steps; ++j) {
     hpsi_all(v1, w, 1, PWK_FP64);
-    launch_block_dot(c, per, nk, (const double*)v1, (const double*)w, red.as<double>());
-    PWK_HIP(hipMemcpyAsync(hv.data(), red.p, (size_t)nk * 8, hipMemcpyDeviceToHost, c.stream));
+    launch_block_dot(c, per, nke, (const double*)v1, (const double*)w, red.as<double>());
+    PWK_HIP(hipMemcpyAsync(hv.data(), red.p, (size_t)nke * 8, hipMemcpyDeviceToHost, c.stream));
     PWK_HIP(hipStreamSynchronize(c.stream));
-    for (int k = 0; k < nk; ++k) {
+    for (int k = 0; k < nke; ++k) {
       al[(size_t)k].push_back(hv[(size_t)k]);
       cfh[(size_t)3 * k] = 1.0;
       cfh[(size_t)3 * k + 1] = hv[(size_t)k];
       cfh[(size_t)3 * k + 2] = beta[(size_t)k];
     }
     PWK_HIP(hipMemcpyAsync(cf.p, cfh.data(), cfh.size() * 8, hipMemcpyHostToDevice, c.stream));
-    launch_cheb_step(c, (size_t)nk * npw, (size_t)npw, w, v1, v0, cf.as<double>(), w, PWK_FP64);
-    launch_block_dot(c, per, nk, (const double*)w, (const double*)w, red.as<double>());
-    PWK_HIP(hipMemcpyAsync(hv.data(), red.p, (size_t)nk * 8, hipMemcpyDeviceToHost, c.stream));
+    launch_cheb_step(c, (size_t)nke * npw, (size_t)npw, w, v1, v0, cf.as<double>(), w, PWK_FP64);
+    launch_block_dot(c, per, nke, (const double*)w, (const double*)w, red.as<double>());
+    PWK_HIP(hipMemcpyAsync(hv.data(), red.p, (size_t)nke * 8, hipMemcpyDeviceToHost, c.stream));
     PWK_HIP(hipStreamSynchronize(c.stream));
     bool stop = false;
-    for (int k = 0; k < nk; ++k) {
+    for (int k = 0; k < nke; ++k) {
       beta[(size_t)k] = std::sqrt(std::max(hv[(size_t)k], 0.0));
       be[(size_t)k].push_back(beta[(size_t)k]);
       inv[(size_t)k] = beta[(size_t)k] > 1e-300 ? 1.0 / beta[(size_t)k] : 0.0;
@@ -1175,12 +1191,12 @@ void Solve::lanczos_bounds() {
     }
     if (stop || j + 1 == steps) break;
     std::swap(v0, v1);  // v0 <- v_j
-    PWK_HIP(hipMemcpyAsync(red.p, inv.data(), (size_t)nk * 8, hipMemcpyHostToDevice, c.stream));
-    launch_block_scale(c, per, nk, red.as<double>(), 0, (double*)w);
+    PWK_HIP(hipMemcpyAsync(red.p, inv.data(), (size_t)nke * 8, hipMemcpyHostToDevice, c.stream));
+    launch_block_scale(c, per, nke, red.as<double>(), 0, (double*)w);
     std::swap(v1, w);  // v1 <- v_{j+1}
     PWK_HIP(hipStreamSynchronize(c.stream));  // inv is host memory reused next step
   }
-  for (int k = 0; k < nk; ++k) {
+  for (int k = 0; k < nke; ++k) {
     const auto& A = al[(size_t)k];
     const auto& Bv = be[(size_t)k];
     double bound = -INFINITY;
@@ -1200,9 +1216,9 @@ void Solve::lanczos_bounds() {
 double Solve::chebyshev_pass(int fprec) {
   TimedScope ts(c, "scf_chebyshev_filter");
   ++passes_total;
-  const size_t blk = (size_t)nk * npw * nbs;
-  coef_host.assign((size_t)degree * nk * 3, 0.0);
-  for (int k = 0; k < nk; ++k) {
+  const size_t blk = (size_t)nke * npw * nbs;
+  coef_host.assign((size_t)degree * nke * 3, 0.0);
+  for (int k = 0; k < nke; ++k) {
     const double* lam = ev_sub_c() + (size_t)k * nbs;
     const double a0 = lam[0];
     double a = lam[nbs - 1], b = ub_c()[k];
@@ -1216,7 +1232,7 @@ double Solve::chebyshev_pass(int fprec) {
     coef_host[(size_t)3 * k + 2] = 0.0;
     for (int i = 1; i < degree; ++i) {
       const double sn = 1.0 / (tau - sigma);
-      double* cf = &coef_host[((size_t)i * nk + k) * 3];
+      double* cf = &coef_host[((size_t)i * nke + k) * 3];
       cf[0] = 2.0 * sn / e;
       cf[1] = cc;
       cf[2] = sigma * sn;
@@ -1232,7 +1248,7 @@ double Solve::chebyshev_pass(int fprec) {
   void *X = buf[0], *Y = buf[1], *Z = buf[2];
   hpsi_cheb(X, nullptr, coef.as<double>(), Y, fprec);
   for (int i = 1; i < degree; ++i) {
-    hpsi_cheb(Y, X, coef.as<double>() + (size_t)i * nk * 3, Z, fprec);
+    hpsi_cheb(Y, X, coef.as<double>() + (size_t)i * nke * 3, Z, fprec);
     void* t = X;
     X = Y;
     Y = Z;
@@ -1245,7 +1261,7 @@ double Solve::chebyshev_pass(int fprec) {
   hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
   rayleigh_ritz(Y64.as<double2>(), HY64.as<double2>());
   double worst = 0.0;
-  for (int k = 0; k < nk; ++k)
+  for (int k = 0; k < nke; ++k)
     for (int b = 0; b < nb; ++b) worst = std::max(worst, resid_host[(size_t)k * nbs + b]);
   return worst;
 }
@@ -1253,7 +1269,7 @@ double Solve::chebyshev_pass(int fprec) {
 void Solve::eig_chebyshev(bool first) {
   if (first) {
     lanczos_bounds();
-    PWK_HIP(hipMemcpyAsync(Y64.p, Vc(), vblk() * 16, hipMemcpyDeviceToDevice, c.stream));
+    PWK_HIP(hipMemcpyAsync(Y64.p, Vc(), eblk() * 16, hipMemcpyDeviceToDevice, c.stream));
     hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
     rayleigh_ritz(Y64.as<double2>(), HY64.as<double2>());
   }
@@ -1269,7 +1285,7 @@ void Solve::eig_chebyshev(bool first) {
                                 prec == PWK_FP64 ? "fp64" : "reduced", worst, target);
     if (worst < target) break;
   }
-  for (int k = 0; k < nk; ++k)
+  for (int k = 0; k < nke; ++k)
     for (int b = 0; b < nb; ++b) ev_host[(size_t)k * nb + b] = ev_sub_c()[(size_t)k * nbs + b];
 }
 
@@ -1343,8 +1359,14 @@ void Solve::run(double* eig_out, double* rho_out) {
       exc_in = host_small[1] * dvol - (host_small[2] + host_small[3]) * dvol;
     }
 
-    // ---- eigenpairs per channel ------------------------------------------------------------
-    for (sc = 0; sc < ns; ++sc) {
+    // ---- eigenpairs per channel (both channels in one pass of launches when folded) ----------
+    if (fold) {
+      for (int s = 0; s < ns; ++s) set_veff_device(c, veff_g.as<double2>() + (size_t)s * N, s);
+      sc = 0;
+      ev_host = ev.data();
+      eig_chebyshev(iter == 0);
+    }
+    for (sc = 0; sc < (fold ? 0 : ns); ++sc) {
       set_veff_device(c, veff_g.as<double2>() + (size_t)sc * N);
       double* evc = ev.data() + (size_t)sc * nk * nb;
       if (chebyshev) {

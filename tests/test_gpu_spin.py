@@ -99,6 +99,29 @@ def test_spin_xc_consistency(scf_mod, gpu, oracle, eigensolver, precision):
     _compare(r, ro, 2, iters=1 if eigensolver == 0 else 3)
 
 
+def test_spin_channels_folded_equal_sequential(scf_mod, gpu):
+    # on small grids the filtered solver runs both spin channels through each launch (V(r) per
+    # channel); with the fused kernels off it solves the channels one after the other
+    deck = scf_mod.CrystalDeck(fcc(5.431), SI_FRAC, [0, 0], [SI_UPF])
+    p = _si_params(scf_mod, nspin=2, starting_magnetization={0: 0.3}, max_iter=60, energy_threshold=1e-6,
+                   eigensolver=1, precision="fp64")
+    print("spin: folded vs sequential channels", flush=True)
+    try:
+        gpu.set_fused_hpsi(True)
+        rf = scf_mod.run_scf(gpu, deck, p)
+        gpu.set_fused_hpsi(False)
+        rs = scf_mod.run_scf(gpu, deck, p)
+    finally:
+        gpu.set_fused_hpsi(True)
+    print("folded", rf["e_total"], rf["n_iter"], "sequential", rs["e_total"], rs["n_iter"], flush=True)
+    assert rf["converged"] == 1 and rs["converged"] == 1
+    # the folded pass loop runs until both channels meet the residual target, the sequential
+    # one per channel, so the trajectories differ within the SCF tolerance
+    for t in TERMS:
+        assert abs(rf[t] - rs[t]) < TOL_TERM * 2, t
+    assert np.abs(rf["eigenvalues"] - rs["eigenvalues"]).max() < 1e-4
+
+
 def test_kerker_mixing_nspin1_matches_oracle(scf_mod, gpu, oracle):
     # Kerker-preconditioned Anderson (src/scf/mixing/anderson.rs:34-97, kerker.rs) on the Si deck
     deck = scf_mod.CrystalDeck(fcc(5.431), SI_FRAC, [0, 0], [SI_UPF])
