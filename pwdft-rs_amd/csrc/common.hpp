@@ -192,6 +192,8 @@ void launch_convert(Context& c, const void* src, void* dst, size_t n, bool to_fp
 // fft3d.hip
 bool fft_size_supported(int n);
 void launch_fft3d(Context& c, void* data, int batch, int sign, int precision);
+// out = FFT_sign(in), one batch item, FP64 (one launch on small cubic grids)
+void launch_fft3d_copy(Context& c, const void* in, void* out, int sign);
 // out (double2 N) = FFT_sign(r·scale + 0i): one launch on small cubic grids
 void launch_fft3d_r2c(Context& c, const double* r, double scale, void* out, int sign);
 // hpsi.hip
@@ -199,9 +201,12 @@ void setup_basis(Context& c, int npw, const int32_t* miller_host);
 // V_eff(G) on the device -> V(r) for H·ψ; slot 1 holds a second channel (LSDA) for the fused
 // small-grid H·ψ and needs reserve_veff_slots(c, 2) first
 void reserve_veff_slots(Context& c, int nslots);
-void set_veff_device(Context& c, const void* veff_g_dev, int slot = 0);
+// (need_g false: skip the copy of V_eff(G) kept for the dense-H builder)
+void set_veff_device(Context& c, const void* veff_g_dev, int slot = 0, bool need_g = true);
 void launch_hpsi(Context& c, int ik, int nb, const void* psi, void* hpsi, int precision);
-void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision);
+// rho += Σ_b fw_b |ψ_b(r)|² (overwrite: rho = Σ ...)
+void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision,
+                    bool overwrite = false);
 void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, int precision, const void* kin,
                        int nch, const void* B, const void* DBH, void* proj);
 // hpsi_small.hip: the whole H·ψ of nbands bands (band b at k-point b / nbk) in one launch;
@@ -214,7 +219,8 @@ void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* h
                        const void* X = nullptr, int nkv = 0);
 // ρ += Σ_b fw_b |ψ_b(r)|² on small cubic grids (fw on the device)
 bool density_small_supported(const Context& c, int precision);
-void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision);
+void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision,
+                          bool overwrite = false);
 // dense.hip
 void launch_dense_h(Context& c, int ik, int with_nl, void* h);
 void launch_dense_h_batched(Context& c, int nk, const double* kin, int nch, const void* B, const void* DBH, void* h);

@@ -76,7 +76,8 @@ void fft3d_t(Context& c, void* data, int batch) {
 // launch for the three passes. With real_in the input is real_in·scale + 0i (FP64 only).
 template <typename T, int N, int SIGN>
 __global__ void __launch_bounds__(SMALL_NT) k_fft3d_small(typename V2s<T>::t* __restrict__ data,
-                                                          const double* __restrict__ real_in, double scale) {
+                                                          const double* __restrict__ real_in, double scale,
+                                                          const typename V2s<T>::t* __restrict__ src) {
   using V2 = typename V2s<T>::t;
   extern __shared__ unsigned char smem_raw[];
   V2* G = reinterpret_cast<V2*>(smem_raw);
@@ -88,7 +89,8 @@ __global__ void __launch_bounds__(SMALL_NT) k_fft3d_small(typename V2s<T>::t* __
       G[i].y = 0;
     }
   } else {
-    for (int i = threadIdx.x; i < NN; i += SMALL_NT) G[i] = d[i];
+    const V2* in = src ? src + (size_t)blockIdx.x * NN : d;  // out of place with src
+    for (int i = threadIdx.x; i < NN; i += SMALL_NT) G[i] = in[i];
   }
   __syncthreads();
   lds_lines<T, N, 0, SIGN>(G, nullptr);
@@ -121,7 +123,8 @@ bool fft_small_ok(const Context& c, int precision) {
 }
 
 template <typename T, int SIGN>
-void fft3d_small_t(Context& c, void* data, int batch, const double* real_in, double scale) {
+void fft3d_small_t(Context& c, void* data, int batch, const double* real_in, double scale,
+                   const void* src = nullptr) {
   using V2 = typename V2s<T>::t;
   const int n = c.dims[0];
   const size_t sh = (size_t)n * n * n * sizeof(V2);
@@ -129,7 +132,8 @@ void fft3d_small_t(Context& c, void* data, int batch, const double* real_in, dou
 #define PWK_CASE(L)                                                                                          \
   case L:                                                                                                    \
     ensure_lds_limit((const void*)k_fft3d_small<T, L, SIGN>, sh);                                           \
-    k_fft3d_small<T, L, SIGN><<<(unsigned)batch, SMALL_NT, sh, c.stream>>>((V2*)data, real_in, scale);       \
+    k_fft3d_small<T, L, SIGN><<<(unsigned)batch, SMALL_NT, sh, c.stream>>>((V2*)data, real_in, scale,        \
+                                                                          (const V2*)src);                 \
     break;
     PWK_FFT_SMALL_SIZES(PWK_CASE)
 #undef PWK_CASE
@@ -159,6 +163,19 @@ void launch_fft3d_r2c(Context& c, const double* r, double scale, void* out, int 
   }
   k_r2c_scale<<<blocks_for(c.N, 256), 256, 0, c.stream>>>(c.N, r, scale, (double2*)out);
   PWK_HIP(hipGetLastError());
+  launch_fft3d(c, out, 1, sign, PWK_FP64);
+}
+
+void launch_fft3d_copy(Context& c, const void* in, void* out, int sign) {
+  c.check_grid();
+  if (fft_small_ok(c, PWK_FP64)) {
+    if (sign < 0)
+      fft3d_small_t<double, -1>(c, out, 1, nullptr, 1.0, in);
+    else
+      fft3d_small_t<double, 1>(c, out, 1, nullptr, 1.0, in);
+    return;
+  }
+  PWK_HIP(hipMemcpyAsync(out, in, c.N * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
   launch_fft3d(c, out, 1, sign, PWK_FP64);
 }
 

@@ -570,20 +570,21 @@ void reserve_veff_slots(Context& c, int nslots) {
   c.vrn32.ensure(n * sizeof(float));
 }
 
-void set_veff_device(Context& c, const void* veff_g_dev, int slot) {
+void set_veff_device(Context& c, const void* veff_g_dev, int slot, bool need_g) {
   c.check_grid();
   size_t n = c.N;
   PWK_REQUIRE(slot >= 0 && slot <= 1, PWK_E_INVALID, "V_eff slot must be 0 or 1");
   if (slot == 0) {
     reserve_veff_slots(c, 1);
-    c.veffg64.ensure(n * sizeof(double2));
-    PWK_HIP(hipMemcpyAsync(c.veffg64.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
+    if (need_g) {
+      c.veffg64.ensure(n * sizeof(double2));
+      PWK_HIP(hipMemcpyAsync(c.veffg64.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
+    }
   } else {
     PWK_REQUIRE(c.vrn64.bytes >= 2 * n * sizeof(double), PWK_E_STATE, "second V_eff slot not reserved");
   }
   c.work_b.ensure(n * sizeof(double2));
-  PWK_HIP(hipMemcpyAsync(c.work_b.p, veff_g_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
-  launch_fft3d(c, c.work_b.p, 1, +1, PWK_FP64);
+  launch_fft3d_copy(c, veff_g_dev, c.work_b.p, +1);
   const size_t o = (size_t)slot * n;
   k_vr_reorder<<<blocks_for(n, 256), 256, 0, c.stream>>>(c.work_b.as<double2>(), c.vr64.as<double>() + o,
                                                          c.vr32.as<float>() + o, c.vrn64.as<double>() + o,
@@ -653,16 +654,18 @@ void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, 
   }
 }
 
-void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision) {
+void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision,
+                    bool overwrite) {
   c.check_basis();
   PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
   const int dprec = precision == PWK_FP64 ? PWK_FP64 : PWK_FP32;
   if (density_small_supported(c, dprec)) {
     c.work_b.ensure((size_t)nb * sizeof(double));
     PWK_HIP(hipMemcpyAsync(c.work_b.p, fw_host, (size_t)nb * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    launch_density_small(c, nb, psi, c.work_b.as<double>(), rho, dprec);
+    launch_density_small(c, nb, psi, c.work_b.as<double>(), rho, dprec, overwrite);
     return;
   }
+  if (overwrite) PWK_HIP(hipMemsetAsync(rho, 0, c.N * sizeof(double), c.stream));
   if (precision == PWK_FP64)
     density_t<double>(c, nb, psi, fw_host, rho);
   else

@@ -186,10 +186,10 @@ __global__ void __launch_bounds__(SMALL_NT) k_density_small(const typename V2s<T
 }
 
 __global__ void __launch_bounds__(256) k_density_reduce(size_t n, int ngroups, const double* __restrict__ partial,
-                                                        double* __restrict__ rho) {
+                                                        double* __restrict__ rho, int overwrite) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  double s = rho[i];
+  double s = overwrite ? 0.0 : rho[i];
   for (int g = 0; g < ngroups; ++g) s += partial[(size_t)g * n + i];
   rho[i] = s;
 }
@@ -240,7 +240,7 @@ void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi
 }
 
 template <typename T>
-void launch_density_small_t(Context& c, int nb, const void* psi, const double* fw_dev, void* rho) {
+void launch_density_small_t(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, bool overwrite) {
   using V2 = typename V2s<T>::t;
   const int n = c.dims[0];
   const size_t sh = small_lds_bytes<T>(n, 0);
@@ -261,20 +261,22 @@ void launch_density_small_t(Context& c, int nb, const void* psi, const double* f
       throw Error(PWK_E_UNSUPPORTED, "grid has no fused small-grid kernel");
   }
   PWK_HIP(hipGetLastError());
-  k_density_reduce<<<blocks_for(c.N, 256), 256, 0, c.stream>>>(c.N, ng, c.work_a.as<double>(), (double*)rho);
+  k_density_reduce<<<blocks_for(c.N, 256), 256, 0, c.stream>>>(c.N, ng, c.work_a.as<double>(), (double*)rho,
+                                                               overwrite ? 1 : 0);
   PWK_HIP(hipGetLastError());
 }
 
 }  // namespace
 
-void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision) {
+void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision,
+                          bool overwrite) {
   c.check_basis();
   PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
   PWK_REQUIRE(density_small_supported(c, precision), PWK_E_UNSUPPORTED, "grid too large for the fused kernel");
   if (precision == PWK_FP64)
-    launch_density_small_t<double>(c, nb, psi, fw_dev, rho);
+    launch_density_small_t<double>(c, nb, psi, fw_dev, rho, overwrite);
   else
-    launch_density_small_t<float>(c, nb, psi, fw_dev, rho);
+    launch_density_small_t<float>(c, nb, psi, fw_dev, rho, overwrite);
 }
 
 bool density_small_supported(const Context& c, int precision) {

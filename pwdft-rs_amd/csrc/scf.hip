@@ -253,9 +253,14 @@ __global__ void k_scale(size_t n, double* __restrict__ x, double s) {
   if (i < n) x[i] *= s;
 }
 
-__global__ void k_sub(size_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ o) {
+// Anderson history entry: the input and its residual out − in
+__global__ void k_hist_store(size_t n, const double* __restrict__ in, const double* __restrict__ out,
+                             double* __restrict__ hin, double* __restrict__ hres) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) o[i] = a[i] - b[i];
+  if (i >= n) return;
+  const double x = in[i];
+  hin[i] = x;
+  hres[i] = out[i] - x;
 }
 
 // ρ_sym(G) = (1/N_ops) Σ_S e^{-2πi n·τ_S} ρ(R_S^T n)  (src/symmetry/density/g_space.rs:147-252)
@@ -324,14 +329,6 @@ __global__ void __launch_bounds__(RED_NT) k_dots(size_t n, DotArgs args, double*
   }
   const double s = block_reduce(acc, sh);
   if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
-}
-
-__global__ void k_dots_finish(int nterms, int nblk, const double* __restrict__ partial, double* __restrict__ out) {
-  const int t = threadIdx.x;
-  if (t >= nterms) return;
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[(size_t)t * nblk + b];
-  out[t] = s;
 }
 
 // per-band expectation values: e[2b] = Σ_G |ψ|² t_G, e[2b+1] = Re Σ_G conj(ψ)·x
@@ -522,8 +519,9 @@ struct Solve {
   // (nspin = 2), xc_tot the clamped ρ_total + ρ_core
   DevBuf vloc_g, vloc_r, core_r, rho_in, rho_g_in, rho_new, rho_g_new, rho_xc, exc, vxc, vxc_g, veff_g, cwork;
   DevBuf t_in, m_in, t_out, m_out, xc_tot;
-  DevBuf Hb, W, V, V32, sym_rot, sym_tau, partial, dots_out, nl_p, nl_x, expect;
-  double* host_small = nullptr;  // pinned
+  DevBuf Hb, W, V, V32, sym_rot, sym_tau, partial, nl_p, nl_x, expect;
+  double* host_small = nullptr;    // pinned: reduction results
+  double* host_partial = nullptr;  // pinned: per-block partial sums
 
   // Anderson history of one mixed quantity (src/scf/mixing/anderson.rs:20-28)
   struct Mixer {
@@ -538,6 +536,7 @@ struct Solve {
   Solve(Context& cc, const pwk_crystal& a, const pwk_scf_params& b, pwk_scf_result& r) : c(cc), cr(a), p(b), res(r) {}
   ~Solve() {
     if (host_small) (void)hipHostFree(host_small);
+    if (host_partial) (void)hipHostFree(host_partial);
   }
 
   unsigned gb(size_t n) const { return blocks_for(n, 256); }
@@ -551,10 +550,16 @@ struct Solve {
     const int nblk = (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (n + RED_NT - 1) / RED_NT));
     k_dots<<<dim3((unsigned)nblk, (unsigned)terms.size()), RED_NT, 0, c.stream>>>(n, a, partial.as<double>());
     PWK_HIP(hipGetLastError());
-    k_dots_finish<<<1, 64, 0, c.stream>>>((int)terms.size(), nblk, partial.as<double>(), dots_out.as<double>());
-    PWK_HIP(hipGetLastError());
-    PWK_HIP(hipMemcpyAsync(host_small, dots_out.p, terms.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    // the per-block partials come back and are summed here in block order (the second stage of
+    // the fixed-order reduction)
+    const size_t np = terms.size() * (size_t)nblk;
+    PWK_HIP(hipMemcpyAsync(host_partial, partial.p, np * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     PWK_HIP(hipStreamSynchronize(c.stream));
+    for (size_t t = 0; t < terms.size(); ++t) {
+      double sum = 0.0;
+      for (int b = 0; b < nblk; ++b) sum += host_partial[t * nblk + b];
+      host_small[t] = sum;
+    }
   }
 
   // ρ(r) → ρ(G) = FFT_fwd(ρ)/N  (src/scf/energy.rs:250-259)
@@ -795,8 +800,9 @@ void Solve::setup() {
   if (ns == 2)
     for (DevBuf* b : {&t_in, &m_in, &t_out, &m_out, &xc_tot}) b->ensure(N * 8);
   partial.ensure((size_t)MAX_TERMS * RED_BLOCKS * 8);
-  dots_out.ensure(MAX_TERMS * 8);
   PWK_HIP(hipHostMalloc((void**)&host_small, 64 * sizeof(double), hipHostMallocDefault));
+  PWK_HIP(hipHostMalloc((void**)&host_partial, (size_t)MAX_TERMS * RED_BLOCKS * sizeof(double),
+                        hipHostMallocDefault));
   // V_loc(r) for the local energy, ρ_core(r), the start density (IFFT, real part)
   PWK_HIP(hipMemcpyAsync(cwork.p, vloc_g.p, N * 16, hipMemcpyDeviceToDevice, c.stream));
   launch_fft3d(c, cwork.p, 1, +1, PWK_FP64);
@@ -1054,8 +1060,7 @@ void Solve::mix(Mixer& mx, const double* in, const double* out, double* result) 
   mx.hslot.push_back(slot);
   double* hin = mx.hist.as<double>() + (size_t)slot * 2 * N;
   double* hres = hin + N;
-  PWK_HIP(hipMemcpyAsync(hin, in, N * 8, hipMemcpyDeviceToDevice, c.stream));
-  k_sub<<<gb(N), 256, 0, c.stream>>>(N, out, in, hres);
+  k_hist_store<<<gb(N), 256, 0, c.stream>>>(N, in, out, hin, hres);
   PWK_HIP(hipGetLastError());
   if (mx.kerker) {
     launch_fft3d_r2c(c, hres, 1.0, cwork.p, -1);
@@ -1361,13 +1366,13 @@ void Solve::run(double* eig_out, double* rho_out) {
 
     // ---- eigenpairs per channel (both channels in one pass of launches when folded) ----------
     if (fold) {
-      for (int s = 0; s < ns; ++s) set_veff_device(c, veff_g.as<double2>() + (size_t)s * N, s);
+      for (int s = 0; s < ns; ++s) set_veff_device(c, veff_g.as<double2>() + (size_t)s * N, s, false);
       sc = 0;
       ev_host = ev.data();
       eig_chebyshev(iter == 0);
     }
     for (sc = 0; sc < (fold ? 0 : ns); ++sc) {
-      set_veff_device(c, veff_g.as<double2>() + (size_t)sc * N);
+      set_veff_device(c, veff_g.as<double2>() + (size_t)sc * N, 0, !chebyshev);
       double* evc = ev.data() + (size_t)sc * nk * nb;
       if (chebyshev) {
         // Chebyshev-filtered subspace iteration on H·ψ, FP64 Rayleigh-Ritz
@@ -1410,7 +1415,6 @@ void Solve::run(double* eig_out, double* rho_out) {
     if (p.precision != PWK_FP64) launch_convert(c, V.p, V32.p, (size_t)ns * vblk() * 2, true);
     for (int s = 0; s < ns; ++s) {
       double* rn = rho_new_s(s);
-      PWK_HIP(hipMemsetAsync(rn, 0, N * 8, c.stream));
       // the basis is the same G-sphere at every k, so all nk·nb bands go through one launch
       std::vector<double> fw((size_t)nk * nbs, 0.0);  // buffer bands carry no weight
       for (int k = 0; k < nk; ++k)
@@ -1419,9 +1423,9 @@ void Solve::run(double* eig_out, double* rho_out) {
       {
         TimedScope ts(c, "scf_density");
         if (p.precision == PWK_FP64)
-          launch_density(c, nk * nbs, V.as<double2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP64);
+          launch_density(c, nk * nbs, V.as<double2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP64, true);
         else
-          launch_density(c, nk * nbs, V32.as<float2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP32);
+          launch_density(c, nk * nbs, V32.as<float2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP32, true);
       }
       dots({DotTerm{rn, nullptr, nullptr, nullptr, 0}}, N);
       const double integral = host_small[0] * dvol;
