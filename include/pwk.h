@@ -69,6 +69,13 @@ void* pwk_ctx_stream(pwk_ctx* ctx);
  * for every other grid. Both compute the same product; the switch exists for
  * comparisons. */
 int pwk_ctx_set_fused_hpsi(pwk_ctx* ctx, int on);
+/* No counterpart in the reference. On, pwk_scf_run captures each filter pass of
+ * the Chebyshev-filtered solver (fused small-grid path) as a HIP graph per spin
+ * channel and filter precision and replays it; off (the default) launches the
+ * kernels one by one. Both run the same kernels in the same order and give
+ * identical results. Capture must not overlap allocations by other threads on
+ * the device (the serving workers never capture). */
+int pwk_ctx_set_graphs(pwk_ctx* ctx, int on);
 
 /* ---------------------------------------------------------------------------
  * Kernel timing (no counterpart in the reference, which has no GPU timers):

@@ -133,6 +133,10 @@ int pwk_ctx_set_fused_hpsi(pwk_ctx* ctx, int on) {
   return guarded([&] { ctx_of(ctx).small_hpsi = on != 0; });
 }
 
+int pwk_ctx_set_graphs(pwk_ctx* ctx, int on) {
+  return guarded([&] { ctx_of(ctx).graphs = on != 0; });
+}
+
 int pwk_profile_enable(pwk_ctx* ctx, int on) {
   return guarded([&] {
     Context& c = ctx_of(ctx);

@@ -155,6 +155,7 @@ struct Context {
   DevBuf vr64, vr32;   // real N, V(r) in plane-major order [i3][i1][i2]
   DevBuf vrn64, vrn32; // real N, V(r) in grid order [i1][i2][i3]
   bool small_hpsi = true;  // fused one-launch H·ψ on small cubic grids (hpsi_small.hip)
+  bool graphs = false;     // replay captured filter passes in the SCF (scf.hip), opt-in
 
   std::vector<KData> kd;
 

@@ -549,7 +549,8 @@ void setup_basis(Context& c, int npw, const int32_t* m) {
   c.nx1 = (int)x1list.size();
   auto up = [&](DevBuf& d, const void* h, size_t bytes) {
     d.ensure(bytes);
-    PWK_HIP(hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice));
+    PWK_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c.stream));
+    PWK_HIP(hipStreamSynchronize(c.stream));
   };
   up(c.miller, m, (size_t)npw * 3 * sizeof(int32_t));
   up(c.g2fft, g2fft.data(), g2fft.size() * sizeof(int32_t));

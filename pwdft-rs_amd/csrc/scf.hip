@@ -17,6 +17,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <set>
 
 #include "common.hpp"
 #include "tables.hpp"
@@ -537,6 +539,8 @@ struct Solve {
   ~Solve() {
     if (host_small) (void)hipHostFree(host_small);
     if (host_partial) (void)hipHostFree(host_partial);
+    if (rr_pinned) (void)hipHostFree(rr_pinned);
+    for (auto& kv : pass_graphs) (void)hipGraphExecDestroy(kv.second);
   }
 
   unsigned gb(size_t n) const { return blocks_for(n, 256); }
@@ -576,7 +580,7 @@ struct Solve {
   DevBuf F0, F1, F2, HYf, Y64, HY64, YH, sS, sH, sU, sT, sX, sXH, sW, sC, s_ev, s_lam, s_work, lowest;
   DevBuf kin_all64, kin_all32;              // nk × npw kinetic diagonals
   DevBuf R64, resid;                        // H·V after Rayleigh-Ritz, residual norms
-  std::vector<double> resid_host, rr_host;
+  std::vector<double> resid_host;
   double last_delta = INFINITY;             // density change of the previous SCF iteration
   int passes_total = 0;
   DevBuf Ball64, DBHall64, Ball32, DBHall32, projall, coef;
@@ -619,6 +623,12 @@ struct Solve {
   void setup();
   double ewald();
   void rayleigh_ritz(const double2* Y, const double2* HY);
+  void rr_launch(const double2* Y, const double2* HY);
+  void rr_collect();
+  void rr_general(const double2* Y, const double2* HY);
+  std::map<int, hipGraphExec_t> pass_graphs;  // captured filter passes per (channel, precision)
+  std::set<int> pass_warm;
+  double* rr_pinned = nullptr;                // pinned: Ritz values and residual norms
   void lanczos_bounds();
   double chebyshev_pass(int prec);
   void eig_chebyshev(bool first);
@@ -946,6 +956,7 @@ void Solve::setup() {
     for (DevBuf* b : {&sS, &sH, &sU, &sT, &sX, &sXH, &sW, &sC}) b->ensure((size_t)nke * nbs * nbs * 16);
     s_ev.ensure((size_t)nke * nbs * 8);
     s_lam.ensure((size_t)2 * nke * nbs * 8);  // Ritz values, then residual norms (fused path)
+    PWK_HIP(hipHostMalloc((void**)&rr_pinned, (size_t)2 * nke * nbs * 8, hipHostMallocDefault));
     s_work.ensure(heev_work_doubles(nbs, nbs, nke) * 8);
     projall.ensure((size_t)nk * std::max(1, nch_all) * nbs * 16);
     ev_sub.assign((size_t)ns * nk * nbs, 0.0);
@@ -1111,18 +1122,33 @@ void Solve::mix(Mixer& mx, const double* in, const double* out, double* result) 
 // go to ev_sub, the residual norms |HV − λV| of the nb occupied-range bands to resid_host.
 // Blocks of at most 32 bands per k take one fused launch per call.
 void Solve::rayleigh_ritz(const double2* Y, const double2* HY) {
-  TimedScope ts(c, "scf_rayleigh_ritz");
-  const int m = nbs;
-  if (rr_small_supported(m)) {
-    const size_t nkm = (size_t)nke * m;
-    launch_rr_small(c, npw, m, nb, nke, Y, HY, Vc(), s_lam.as<double>(), s_lam.as<double>() + nkm, 1e-14);
-    rr_host.resize(2 * nkm);
-    PWK_HIP(hipMemcpyAsync(rr_host.data(), s_lam.p, 2 * nkm * 8, hipMemcpyDeviceToHost, c.stream));
-    PWK_HIP(hipStreamSynchronize(c.stream));
-    std::memcpy(ev_sub_c(), rr_host.data(), nkm * 8);
-    std::memcpy(resid_host.data(), rr_host.data() + nkm, nkm * 8);
+  rr_launch(Y, HY);
+  rr_collect();
+}
+
+// the fused path leaves Ritz values and residual norms on the device for rr_collect
+void Solve::rr_launch(const double2* Y, const double2* HY) {
+  if (rr_small_supported(nbs)) {
+    TimedScope ts(c, "scf_rayleigh_ritz");
+    const size_t nkm = (size_t)nke * nbs;
+    launch_rr_small(c, npw, nbs, nb, nke, Y, HY, Vc(), s_lam.as<double>(), s_lam.as<double>() + nkm, 1e-14);
     return;
   }
+  rr_general(Y, HY);
+}
+
+void Solve::rr_collect() {
+  if (!rr_small_supported(nbs)) return;  // the general path has already copied its results
+  const size_t nkm = (size_t)nke * nbs;
+  PWK_HIP(hipMemcpyAsync(rr_pinned, s_lam.p, 2 * nkm * 8, hipMemcpyDeviceToHost, c.stream));
+  PWK_HIP(hipStreamSynchronize(c.stream));
+  std::memcpy(ev_sub_c(), rr_pinned, nkm * 8);
+  std::memcpy(resid_host.data(), rr_pinned + nkm, nkm * 8);
+}
+
+void Solve::rr_general(const double2* Y, const double2* HY) {
+  TimedScope ts(c, "scf_rayleigh_ritz");
+  const int m = nbs;
   const long long blk = (long long)npw * m, sm = (long long)m * m;
   launch_conj_transpose(c, npw, m, Y, npw, blk, YH.p, m, blk, nke);
   launch_zgemm_mfma_batched(c, m, m, npw, YH.p, m, blk, Y, npw, blk, sS.p, m, sm, nke, false, PWK_FP64);
@@ -1245,26 +1271,64 @@ double Solve::chebyshev_pass(int fprec) {
     }
   }
   PWK_HIP(hipMemcpyAsync(coef.p, coef_host.data(), coef_host.size() * 8, hipMemcpyHostToDevice, c.stream));
-  void* buf[3] = {F0.p, F1.p, F2.p};
-  if (fprec == PWK_FP64)
-    PWK_HIP(hipMemcpyAsync(buf[0], Vc(), blk * 16, hipMemcpyDeviceToDevice, c.stream));
-  else
-    launch_convert(c, Vc(), buf[0], blk * 2, true);
-  void *X = buf[0], *Y = buf[1], *Z = buf[2];
-  hpsi_cheb(X, nullptr, coef.as<double>(), Y, fprec);
-  for (int i = 1; i < degree; ++i) {
-    hpsi_cheb(Y, X, coef.as<double>() + (size_t)i * nke * 3, Z, fprec);
-    void* t = X;
-    X = Y;
-    Y = Z;
-    Z = t;
+  // the pass's launches: filter, FP64 H·ψ of the filtered block, Rayleigh-Ritz. The addresses
+  // depend only on the channel and the filter precision, so on the fused small-grid path the
+  // sequence is captured once per (channel, precision) as a HIP graph and replayed (one launch
+  // call instead of fourteen); the coefficients are uploaded before each replay
+  auto body = [&]() {
+    void* buf[3] = {F0.p, F1.p, F2.p};
+    if (fprec == PWK_FP64)
+      PWK_HIP(hipMemcpyAsync(buf[0], Vc(), blk * 16, hipMemcpyDeviceToDevice, c.stream));
+    else
+      launch_convert(c, Vc(), buf[0], blk * 2, true);
+    void *X = buf[0], *Y = buf[1], *Z = buf[2];
+    hpsi_cheb(X, nullptr, coef.as<double>(), Y, fprec);
+    for (int i = 1; i < degree; ++i) {
+      hpsi_cheb(Y, X, coef.as<double>() + (size_t)i * nke * 3, Z, fprec);
+      void* t = X;
+      X = Y;
+      Y = Z;
+      Z = t;
+    }
+    if (fprec == PWK_FP64)
+      PWK_HIP(hipMemcpyAsync(Y64.p, Y, blk * 16, hipMemcpyDeviceToDevice, c.stream));
+    else
+      launch_convert(c, Y, Y64.p, blk * 2, false);
+    hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
+    rr_launch(Y64.as<double2>(), HY64.as<double2>());
+  };
+  const bool graph = c.graphs && !c.prof.on && rr_small_supported(nbs) && hpsi_small_supported(c, PWK_FP64, nch_all) &&
+                     hpsi_small_supported(c, fprec, nch_all) && fprec != PWK_SPLIT_BF16;
+  if (graph) {
+    const int key = sc * 4 + fprec;
+    auto it = pass_graphs.find(key);
+    if (it != pass_graphs.end()) {
+      PWK_HIP(hipGraphLaunch(it->second, c.stream));
+    } else if (!pass_warm.count(key)) {
+      pass_warm.insert(key);  // first pass runs directly (one-time kernel attribute set-up)
+      body();
+    } else {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ex = nullptr;
+      PWK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      try {
+        body();
+      } catch (...) {
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      PWK_HIP(hipStreamEndCapture(c.stream, &g));
+      const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      PWK_HIP(ie);
+      pass_graphs[key] = ex;
+      PWK_HIP(hipGraphLaunch(ex, c.stream));
+    }
+  } else {
+    body();
   }
-  if (fprec == PWK_FP64)
-    PWK_HIP(hipMemcpyAsync(Y64.p, Y, blk * 16, hipMemcpyDeviceToDevice, c.stream));
-  else
-    launch_convert(c, Y, Y64.p, blk * 2, false);
-  hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
-  rayleigh_ritz(Y64.as<double2>(), HY64.as<double2>());
+  rr_collect();
   double worst = 0.0;
   for (int k = 0; k < nke; ++k)
     for (int b = 0; b < nb; ++b) worst = std::max(worst, resid_host[(size_t)k * nbs + b]);

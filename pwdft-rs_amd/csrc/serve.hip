@@ -71,6 +71,9 @@ struct Server {
           (void)hipStreamDestroy(c.stream);
         }
       } guard{c};
+      // stream capture in one worker would be disturbed by allocation and release in another,
+      // so served solves launch the filter passes directly
+      c.graphs = false;
       hipDeviceProp_t prop;
       PWK_HIP(hipGetDeviceProperties(&prop, device));
       c.cu_count = prop.multiProcessorCount;

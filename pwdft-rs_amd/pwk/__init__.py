@@ -52,6 +52,7 @@ _SIGS = {
     "pwk_ctx_sync": (C.c_int, [_vp]),
     "pwk_ctx_stream": (_vp, [_vp]),
     "pwk_ctx_set_fused_hpsi": (C.c_int, [_vp, C.c_int]),
+    "pwk_ctx_set_graphs": (C.c_int, [_vp, C.c_int]),
     "pwk_dev_zgemm": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_longlong, _vp, C.c_int,
                                 C.c_longlong, _vp, C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "pwk_profile_enable": (C.c_int, [_vp, C.c_int]),
@@ -171,6 +172,10 @@ class GpuAccelerator:
         """C (+)= A·B on device pointers (column-major), pwk_dev_zgemm."""
         _check(lib().pwk_dev_zgemm(self.h, m, n, k, a_ptr, lda, strides[0], b_ptr, ldb, strides[1], c_ptr, ldc,
                                    strides[2], batch, int(accumulate), _prec(precision)))
+
+    def set_graphs(self, on=True):
+        """Replay captured filter passes in SCF solves (opt-in) or launch kernel by kernel (default)."""
+        _check(lib().pwk_ctx_set_graphs(self.h, int(on)))
 
     def set_fused_hpsi(self, on=True):
         """Fused one-launch H·psi on small cubic grids (default on) or the z-stick path."""

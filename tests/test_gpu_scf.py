@@ -208,6 +208,31 @@ def test_orthorhombic_cell_matches_oracle(scf_mod, gpu, oracle, eigensolver, pre
         assert abs(r[t] - ro[t]) / 8 < TOL_TERM, t
 
 
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_filter_pass_graphs_equal_direct_launches(scf_mod, gpu, nspin):
+    # the captured filter passes (replayed every pass, with the solve's own host work and
+    # synchronisations between replays) run the same kernels in the same order as direct
+    # launches: the whole solve is bitwise identical
+    lat = fcc(10.26 * BOHR)
+    p = scf_mod.ScfParams(ecut_ev=12 * RY, kgrid=(4, 4, 4), n_bands=8, max_iter=100, conv_threshold=1e-8,
+                          energy_threshold=1e-8 * RY, mixing_ndim=8, smearing_sigma=0.01 * RY, eigensolver=1,
+                          precision="fp32", nspin=nspin, starting_magnetization={0: 0.3})
+    print("graphs vs direct, nspin", nspin, flush=True)
+    try:
+        gpu.set_graphs(True)
+        _, rg = _run(scf_mod, gpu, lat, p)
+        gpu.set_graphs(False)
+        _, rd = _run(scf_mod, gpu, lat, p)
+    finally:
+        gpu.set_graphs(False)
+    assert rg["converged"] == 1 and rd["converged"] == 1
+    assert rg["n_iter"] == rd["n_iter"]
+    for t in TERMS:
+        assert rg[t] == rd[t], t
+    assert np.array_equal(rg["eigenvalues"], rd["eigenvalues"])
+    assert np.array_equal(rg["rho"], rd["rho"])
+
+
 def test_larger_basis_chebyshev_matches_dense_and_oracle(scf_mod, gpu, oracle):
     # Si2 FCC at 25 Ry (537 PW, 24³), 4×4×4 → 8 IBZ k, 8 bands, conv 1e-6: the batched H·ψ
     # eigensolver (all k in one launch sequence, FP64 and FP32 filters) against the dense
