@@ -34,32 +34,111 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::pwk::Error((code), (msg));    \
   } while (0)
 
+// Free list of device (or pinned host) blocks owned by one worker thread of the request
+// loop. Buffers released on that thread go back to the list and are handed out again to
+// its later allocations. All of them are used on the thread's one stream, so reuse is
+// stream-ordered, and no hipFree (which waits for the whole device) runs between solves.
+struct BlockCache {
+  // limit: bytes kept on the free list; blocks released beyond it are freed at once
+  BlockCache(bool host_pinned, size_t limit) : pinned(host_pinned), limit_(limit) {}
+  BlockCache(const BlockCache&) = delete;
+  BlockCache& operator=(const BlockCache&) = delete;
+  ~BlockCache() { trim(0); }
+  // a block of at least `bytes` (rounded up to 256); `bytes` returns the block's size
+  void* take(size_t& bytes);
+  void give(void* p, size_t bytes);
+  void trim(size_t limit);  // frees the largest blocks until at most `limit` bytes stay cached
+  size_t cached() const { return cached_; }
+  size_t held() const { return held_; }  // bytes allocated through this cache, cached or in use
+
+ private:
+  bool pinned;
+  size_t limit_;
+  std::mutex mu;
+  std::multimap<size_t, void*> free_;
+  size_t cached_ = 0, held_ = 0;
+};
+// Caches that DevBuf / PinnedBuf allocations of the calling thread go through (none: plain
+// hipMalloc / hipHostMalloc). Set by the request loop's workers.
+extern thread_local BlockCache* tl_dev_cache;
+extern thread_local BlockCache* tl_host_cache;
+
 // Grow-only device buffer.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  BlockCache* owner = nullptr;  // cache the block returns to, if any
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), owner(o.owner) {
     o.p = nullptr;
     o.bytes = 0;
+    o.owner = nullptr;
   }
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
+  ~DevBuf() { release(); }
   void ensure(size_t b) {
     if (b <= bytes) return;
-    if (p) PWK_HIP(hipFree(p));
-    p = nullptr;
-    bytes = 0;
-    PWK_HIP(hipMalloc(&p, b));
-    bytes = b;
+    release();
+    if (tl_dev_cache) {
+      size_t got = b;
+      p = tl_dev_cache->take(got);
+      owner = tl_dev_cache;
+      bytes = got;
+    } else {
+      PWK_HIP(hipMalloc(&p, b));
+      bytes = b;
+    }
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (owner)
+        owner->give(p, bytes);
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
+    owner = nullptr;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Grow-only pinned host buffer (hipHostMalloc), through the thread's cache when one is set.
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  BlockCache* owner = nullptr;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    release();
+    if (tl_host_cache) {
+      size_t got = b;
+      p = tl_host_cache->take(got);
+      owner = tl_host_cache;
+      bytes = got;
+    } else {
+      PWK_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+      bytes = b;
+    }
+  }
+  void release() {
+    if (p) {
+      if (owner)
+        owner->give(p, bytes);
+      else
+        (void)hipHostFree(p);
+    }
+    p = nullptr;
+    bytes = 0;
+    owner = nullptr;
   }
   template <class T>
   T* as() const {

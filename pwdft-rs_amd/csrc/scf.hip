@@ -522,6 +522,7 @@ struct Solve {
   DevBuf vloc_g, vloc_r, core_r, rho_in, rho_g_in, rho_new, rho_g_new, rho_xc, exc, vxc, vxc_g, veff_g, cwork;
   DevBuf t_in, m_in, t_out, m_out, xc_tot;
   DevBuf Hb, W, V, V32, sym_rot, sym_tau, partial, nl_p, nl_x, expect;
+  PinnedBuf pin_small, pin_partial, pin_rr;
   double* host_small = nullptr;    // pinned: reduction results
   double* host_partial = nullptr;  // pinned: per-block partial sums
 
@@ -537,9 +538,6 @@ struct Solve {
 
   Solve(Context& cc, const pwk_crystal& a, const pwk_scf_params& b, pwk_scf_result& r) : c(cc), cr(a), p(b), res(r) {}
   ~Solve() {
-    if (host_small) (void)hipHostFree(host_small);
-    if (host_partial) (void)hipHostFree(host_partial);
-    if (rr_pinned) (void)hipHostFree(rr_pinned);
     for (auto& kv : pass_graphs) (void)hipGraphExecDestroy(kv.second);
   }
 
@@ -810,9 +808,10 @@ void Solve::setup() {
   if (ns == 2)
     for (DevBuf* b : {&t_in, &m_in, &t_out, &m_out, &xc_tot}) b->ensure(N * 8);
   partial.ensure((size_t)MAX_TERMS * RED_BLOCKS * 8);
-  PWK_HIP(hipHostMalloc((void**)&host_small, 64 * sizeof(double), hipHostMallocDefault));
-  PWK_HIP(hipHostMalloc((void**)&host_partial, (size_t)MAX_TERMS * RED_BLOCKS * sizeof(double),
-                        hipHostMallocDefault));
+  pin_small.ensure(64 * sizeof(double));
+  pin_partial.ensure((size_t)MAX_TERMS * RED_BLOCKS * sizeof(double));
+  host_small = pin_small.as<double>();
+  host_partial = pin_partial.as<double>();
   // V_loc(r) for the local energy, ρ_core(r), the start density (IFFT, real part)
   PWK_HIP(hipMemcpyAsync(cwork.p, vloc_g.p, N * 16, hipMemcpyDeviceToDevice, c.stream));
   launch_fft3d(c, cwork.p, 1, +1, PWK_FP64);
@@ -956,7 +955,8 @@ void Solve::setup() {
     for (DevBuf* b : {&sS, &sH, &sU, &sT, &sX, &sXH, &sW, &sC}) b->ensure((size_t)nke * nbs * nbs * 16);
     s_ev.ensure((size_t)nke * nbs * 8);
     s_lam.ensure((size_t)2 * nke * nbs * 8);  // Ritz values, then residual norms (fused path)
-    PWK_HIP(hipHostMalloc((void**)&rr_pinned, (size_t)2 * nke * nbs * 8, hipHostMallocDefault));
+    pin_rr.ensure((size_t)2 * nke * nbs * 8);
+    rr_pinned = pin_rr.as<double>();
     s_work.ensure(heev_work_doubles(nbs, nbs, nke) * 8);
     projall.ensure((size_t)nk * std::max(1, nch_all) * nbs * 16);
     ev_sub.assign((size_t)ns * nk * nbs, 0.0);

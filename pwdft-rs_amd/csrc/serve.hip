@@ -1,13 +1,15 @@
 // Request loop for SCF solves on one device (continuous batching).
 //
-// A server owns n_slots worker threads, each with its own context and HIP
-// stream. Requests queue in arrival order; a free slot admits the oldest
-// request whose estimated device footprint fits in the free HBM left after
-// the solves already running, so solves of different cell sizes run side by
-// side and overlap on the device while later arrivals wait for memory or a
-// slot. Each solve keeps ψ, ρ and the mixer history resident in its own
-// context for its whole lifetime. A request larger than the whole budget
-// fails with PWK_E_OOM instead of waiting forever.
+// A server owns n_slots worker threads, each with its own HIP stream and a
+// cache of released device blocks, so back-to-back solves on a worker reuse
+// memory instead of returning it (hipFree waits for the whole device).
+// Requests queue in arrival order; a free slot admits the oldest request
+// whose estimated device footprint fits in the free HBM left after the
+// solves already running, so solves of different cell sizes run side by side
+// and overlap on the device while later arrivals wait for memory or a slot.
+// Each solve keeps ψ, ρ and the mixer history resident in its own context
+// for its whole lifetime. A request larger than the whole budget fails with
+// PWK_E_OOM instead of waiting forever.
 //
 // Solves are independent and each is deterministic (fixed-order reductions),
 // so a result does not depend on what else is running.
@@ -22,6 +24,64 @@
 #include "common.hpp"
 
 namespace pwk {
+
+thread_local BlockCache* tl_dev_cache = nullptr;
+thread_local BlockCache* tl_host_cache = nullptr;
+
+void* BlockCache::take(size_t& bytes) {
+  size_t b = (bytes + 255) & ~(size_t)255;
+  if (b == 0) b = 256;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    // smallest cached block that fits, if it wastes at most half of the request
+    auto it = free_.lower_bound(b);
+    if (it != free_.end() && it->first <= b + b / 2) {
+      void* p = it->second;
+      bytes = it->first;
+      cached_ -= it->first;
+      free_.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = pinned ? hipHostMalloc(&p, b, hipHostMallocDefault) : hipMalloc(&p, b);
+  if (e == hipErrorOutOfMemory) {  // give the cached blocks back and try once more
+    (void)hipGetLastError();
+    trim(0);
+    e = pinned ? hipHostMalloc(&p, b, hipHostMallocDefault) : hipMalloc(&p, b);
+  }
+  if (e != hipSuccess)
+    throw Error(e == hipErrorOutOfMemory ? PWK_E_OOM : PWK_E_DEVICE,
+                std::string(pinned ? "hipHostMalloc" : "hipMalloc") + ": " + hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(mu);
+  held_ += b;
+  bytes = b;
+  return p;
+}
+
+void BlockCache::give(void* p, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    free_.emplace(bytes, p);
+    cached_ += bytes;
+    if (cached_ <= limit_) return;
+  }
+  trim(limit_);
+}
+
+void BlockCache::trim(size_t limit) {
+  std::lock_guard<std::mutex> lk(mu);
+  while (cached_ > limit && !free_.empty()) {
+    auto it = std::prev(free_.end());
+    if (pinned)
+      (void)hipHostFree(it->second);
+    else
+      (void)hipFree(it->second);
+    cached_ -= it->first;
+    held_ -= it->first;
+    free_.erase(it);
+  }
+}
 
 struct ServeJob {
   int64_t id = 0;
@@ -38,7 +98,9 @@ struct ServeJob {
 
 struct Server {
   int device = 0;
-  size_t budget = 0;  // bytes the server may hold for running solves
+  size_t budget = 0;     // bytes the server may hold for running solves
+  size_t cache_cap = 0;  // bytes each worker may keep cached between solves
+  static constexpr size_t HOST_CACHE_CAP = (size_t)64 << 20;  // pinned host bytes per worker
   size_t in_use = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -50,33 +112,30 @@ struct Server {
 
   bool admissible(const ServeJob& j) const { return j.bytes > budget || in_use + j.bytes <= budget; }
 
-  // Runs one solve in a fresh context (own stream, buffers released afterwards,
-  // so the memory accounting above matches what the device holds).
-  void solve(ServeJob& job) {
+  // Runs one solve in a fresh context on the worker's stream. The context's buffers come
+  // from the worker's block cache and go back to it when the solve ends.
+  void solve(ServeJob& job, hipStream_t stream, int cu_count) {
     if (job.bytes > budget) {
       job.status = PWK_E_OOM;
       job.error = "solve does not fit in the server's device-memory budget";
       return;
     }
     try {
-      PWK_HIP(hipSetDevice(device));
       Context c;
       c.device = device;
-      PWK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+      c.stream = stream;
+      c.cu_count = cu_count;
       struct StreamGuard {
         Context& c;
         ~StreamGuard() {
           (void)hipStreamSynchronize(c.stream);
           c.kd.clear();
-          (void)hipStreamDestroy(c.stream);
+          c.stream = nullptr;  // owned by the worker
         }
       } guard{c};
       // stream capture in one worker would be disturbed by allocation and release in another,
       // so served solves launch the filter passes directly
       c.graphs = false;
-      hipDeviceProp_t prop;
-      PWK_HIP(hipGetDeviceProperties(&prop, device));
-      c.cu_count = prop.multiProcessorCount;
       int npw, dims[3], nk, nops;
       scf_tables(job.cr, job.p, &npw, dims, &nk, &nops);
       const size_t ns = job.p.nspin == 2 ? 2 : 1;  // eigenvalues of both spin channels
@@ -93,6 +152,21 @@ struct Server {
   }
 
   void worker_main() {
+    // per-worker stream and block caches, kept across solves
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    try {
+      PWK_HIP(hipSetDevice(device));
+      PWK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+      hipDeviceProp_t prop;
+      PWK_HIP(hipGetDeviceProperties(&prop, device));
+      cu_count = prop.multiProcessorCount;
+    } catch (const Error&) {
+      stream = nullptr;
+    }
+    BlockCache dev_cache(false, cache_cap), host_cache(true, HOST_CACHE_CAP);
+    tl_dev_cache = &dev_cache;
+    tl_host_cache = &host_cache;
     for (;;) {
       std::shared_ptr<ServeJob> job;
       {
@@ -113,7 +187,14 @@ struct Server {
         if (job->bytes <= budget) in_use += job->bytes;
       }
       job->t_start = std::chrono::steady_clock::now();
-      solve(*job);
+      if (stream) {
+        solve(*job, stream, cu_count);
+        dev_cache.trim(cache_cap);
+        host_cache.trim(HOST_CACHE_CAP);
+      } else {
+        job->status = PWK_E_DEVICE;
+        job->error = "worker stream could not be created";
+      }
       job->t_end = std::chrono::steady_clock::now();
       {
         std::lock_guard<std::mutex> lk(mu);
@@ -123,6 +204,12 @@ struct Server {
       cv_done.notify_all();
       cv_work.notify_all();
     }
+    if (stream) (void)hipStreamSynchronize(stream);
+    tl_dev_cache = nullptr;
+    tl_host_cache = nullptr;
+    dev_cache.trim(0);
+    host_cache.trim(0);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -183,7 +270,10 @@ int pwk_server_create(int device, int n_slots, double hbm_fraction, pwk_server**
     PWK_HIP(hipMemGetInfo(&free_b, &total_b));
     auto srv = std::make_unique<pwk_server>();
     srv->s.device = device;
-    srv->s.budget = (size_t)((double)free_b * hbm_fraction);
+    // one eighth of the budget is left to the workers' caches of released blocks
+    const size_t total = (size_t)((double)free_b * hbm_fraction);
+    srv->s.cache_cap = total / 8 / (size_t)n_slots;
+    srv->s.budget = total - srv->s.cache_cap * (size_t)n_slots;
     for (int i = 0; i < n_slots; ++i) srv->s.workers.emplace_back([p = &srv->s] { p->worker_main(); });
     *out = srv.release();
   });
