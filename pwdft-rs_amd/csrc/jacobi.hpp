@@ -61,8 +61,16 @@ __device__ inline void jacobi_lds(int n, JacobiLds& J) {
       // round-robin pairing: position 0 stays, positions 1..N-1 have rotated `round` times
       if (t < N / 2) {
         const int u = N - 1 - t;
-        int p = t == 0 ? 0 : ((t - 1 - round) % (N - 1) + (N - 1)) % (N - 1) + 1;
-        int q = ((u - 1 - round) % (N - 1) + (N - 1)) % (N - 1) + 1;
+        // (t - 1 - round) and (u - 1 - round) lie in (-(N - 1), N - 1): one wrap is enough
+        int p = 0;
+        if (t != 0) {
+          p = t - 1 - round;
+          p += p < 0 ? N - 1 : 0;
+          p += 1;
+        }
+        int q = u - 1 - round;
+        q += q < 0 ? N - 1 : 0;
+        q += 1;
         if (p > q) {
           const int tt = p;
           p = q;
@@ -71,15 +79,20 @@ __device__ inline void jacobi_lds(int n, JacobiLds& J) {
         double c = 1.0, sn = 0.0;
         double2 ph = make_double2(1.0, 0.0);
         if (q < n) {
+          // with d = (A_qq - A_pp)/2, g = |A_pq|: tan θ = sign(d)·g/(|d| + sqrt(d² + g²)), so
+          // c = den/sqrt(den² + g²), s = sign(d)·g/sqrt(den² + g²) with den = |d| + sqrt(d² + g²),
+          // and the phase A_pq/g: one sqrt and two reciprocal square roots on the chain
           const double a = J.A[p][p].x, bq = J.A[q][q].x;
           const double2 apq = J.A[p][q];
-          const double g = sqrt(apq.x * apq.x + apq.y * apq.y);
-          if (g > 1e-300) {
-            ph = make_double2(apq.x / g, apq.y / g);
-            const double tau = (bq - a) / (2.0 * g);
-            const double tq = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + tq * tq);
-            sn = tq * c;
+          const double g2 = apq.x * apq.x + apq.y * apq.y;
+          if (g2 > 1e-300) {  // |A_pq| above 1e-150: no denormal reaches rsqrt
+            const double ig = rsqrt(g2);
+            ph = make_double2(apq.x * ig, apq.y * ig);
+            const double d = 0.5 * (bq - a);
+            const double den = fabs(d) + sqrt(d * d + g2);
+            const double inv = rsqrt(den * den + g2);
+            c = den * inv;
+            sn = (d >= 0.0 ? 1.0 : -1.0) * (g2 * ig) * inv;
           }
         }
         J.rp[t] = p;

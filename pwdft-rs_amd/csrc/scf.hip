@@ -63,9 +63,17 @@ struct RadialArgs {
 
 __device__ inline double sinc_d(double x) { return x < 1e-10 ? 1.0 - x * x / 6.0 : sin(x) / x; }
 
+// One wave per grid point: the lanes split the radial mesh (and the atoms) and the partial
+// sums meet in a butterfly, so the whole device works on the mesh integrals.
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 __global__ void __launch_bounds__(256) k_radial(RadialArgs a) {
-  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= a.n) return;
+  const size_t idx = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (idx >= a.n) return;  // wave-uniform
   const int nx = a.dims[0], ny = a.dims[1], nz = a.dims[2];
   const int i1 = (int)(idx / ((size_t)ny * nz)), i2 = (int)((idx / nz) % ny), i3 = (int)(idx % nz);
   const double n1 = i1 > nx / 2 ? i1 - nx : i1, n2 = i2 > ny / 2 ? i2 - ny : i2, n3 = i3 > nz / 2 ? i3 - nz : i3;
@@ -81,12 +89,15 @@ __global__ void __launch_bounds__(256) k_radial(RadialArgs a) {
     const double* wc = a.wc + a.off[s];
     const double* wr = a.wr + a.off[s];
     double iv = 0.0, ic = 0.0, ir = 0.0;
-    for (int i = 0; i < a.mesh[s]; ++i) {
+    for (int i = lane; i < a.mesh[s]; i += 64) {
       const double j0 = sinc_d(gn * r[i]);
       iv += wv[i] * j0;
       ic += wc[i] * j0;
       ir += wr[i] * j0;
     }
+    iv = wave_sum(iv);
+    ic = wave_sum(ic);
+    ir = wave_sum(ir);
     double vg;
     if (gn < 1e-12) {
       vg = a.v0[s];
@@ -97,7 +108,7 @@ __global__ void __launch_bounds__(256) k_radial(RadialArgs a) {
     const double cg = a.has_core[s] ? fourpi * (ic / 3.0) / a.omega : 0.0;
     const double sg = a.has_rho[s] ? (ir / 3.0) / a.omega
                                    : a.z[s] * exp(-0.5 * gn * gn * a.sad_sigma * a.sad_sigma) / a.omega;
-    for (int at = 0; at < a.natoms; ++at) {
+    for (int at = lane; at < a.natoms; at += 64) {
       if (a.sp[at] != s) continue;
       const double ph = -(gx * a.tau[3 * at] + gy * a.tau[3 * at + 1] + gz * a.tau[3 * at + 2]);
       double sn, cs;
@@ -110,9 +121,37 @@ __global__ void __launch_bounds__(256) k_radial(RadialArgs a) {
       sa.y += sn * sg;
     }
   }
-  a.vloc[idx] = vl;
-  a.core[idx] = co;
-  a.sad[idx] = sa;
+  vl = make_double2(wave_sum(vl.x), wave_sum(vl.y));
+  co = make_double2(wave_sum(co.x), wave_sum(co.y));
+  sa = make_double2(wave_sum(sa.x), wave_sum(sa.y));
+  if (lane == 0) {
+    a.vloc[idx] = vl;
+    a.core[idx] = co;
+    a.sad[idx] = sa;
+  }
+}
+
+// Lanczos recurrence coefficients per k for the spectrum bound of the filtered eigensolver, on
+// the device so the steps need no host round trip. stage 0: α_j = red[k] and the step coefficients
+// (1, α_j, β_{j-1}) of w ← H v_j − α_j v_j − β_{j−1} v_{j−1}; stage 1: β_j = sqrt(red[k]) and
+// 1/β_j (0 below 1e-12, where the recurrence has broken down).
+__global__ void k_lanczos_coef(int nke, int j, int stage, const double* __restrict__ red, double* __restrict__ beta,
+                               double* __restrict__ cf, double* __restrict__ al, double* __restrict__ be,
+                               double* __restrict__ inv) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nke) return;
+  if (stage == 0) {
+    const double a = red[k];
+    al[(size_t)j * nke + k] = a;
+    cf[3 * k] = 1.0;
+    cf[3 * k + 1] = a;
+    cf[3 * k + 2] = beta[k];
+  } else {
+    const double b = sqrt(fmax(red[k], 0.0));
+    be[(size_t)j * nke + k] = b;
+    beta[k] = b;
+    inv[k] = b < 1e-12 ? 0.0 : 1.0 / b;
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -797,7 +836,7 @@ void Solve::setup() {
   ra.sad = sad_g.as<double2>();
   {
     TimedScope ts(c, "scf_radial");
-    k_radial<<<gb(N), 256, 0, c.stream>>>(ra);
+    k_radial<<<blocks_for(N, 4), 256, 0, c.stream>>>(ra);
     PWK_HIP(hipGetLastError());
   }
   // working arrays
@@ -1179,13 +1218,18 @@ void Solve::lanczos_bounds() {
   TimedScope ts(c, "scf_lanczos");
   const int steps = std::min(10, npw);
   const size_t per = (size_t)npw * 2;  // reals per k
-  DevBuf a, b0, b1, none, red, cf;
+  DevBuf a, b0, b1, none, red, cf, beta, inv, ab;
   a.ensure((size_t)nke * npw * 16);
   b0.ensure((size_t)nke * npw * 16);
   b1.ensure((size_t)nke * npw * 16);
   none.ensure((size_t)nke * 4);
   red.ensure((size_t)nke * 8);
   cf.ensure((size_t)nke * 3 * 8);
+  beta.ensure((size_t)nke * 8);
+  inv.ensure((size_t)nke * 8);
+  ab.ensure((size_t)2 * steps * nke * 8);  // α_j (rows 0..steps), then β_j
+  double* al_d = ab.as<double>();
+  double* be_d = al_d + (size_t)steps * nke;
   double2* v0 = a.as<double2>();
   double2* v1 = b0.as<double2>();
   double2* w = b1.as<double2>();
@@ -1193,51 +1237,43 @@ void Solve::lanczos_bounds() {
   PWK_HIP(hipMemcpyAsync(none.p, minus.data(), minus.size() * 4, hipMemcpyHostToDevice, c.stream));
   launch_start_block(c, npw, 1, nke, none.as<int>(), v1);
   PWK_HIP(hipMemsetAsync(v0, 0, (size_t)nke * npw * 16, c.stream));
+  PWK_HIP(hipMemsetAsync(beta.p, 0, (size_t)nke * 8, c.stream));
   launch_block_dot(c, per, nke, (const double*)v1, (const double*)v1, red.as<double>());
   launch_block_scale(c, per, nke, red.as<double>(), 1, (double*)v1);
-  std::vector<std::vector<double>> al((size_t)nke), be((size_t)nke);
-  std::vector<double> hv((size_t)nke), cfh((size_t)nke * 3), beta((size_t)nke, 0.0), inv((size_t)nke);
+  // the recurrence coefficients stay on the device; every step runs, and the host keeps the steps
+  // up to the first β_j below 1e-12 on any k (the remaining ones act on zero vectors)
+  const unsigned kb = blocks_for((size_t)nke, 64);
   for (int j = 0; j < steps; ++j) {
     hpsi_all(v1, w, 1, PWK_FP64);
     launch_block_dot(c, per, nke, (const double*)v1, (const double*)w, red.as<double>());
-    PWK_HIP(hipMemcpyAsync(hv.data(), red.p, (size_t)nke * 8, hipMemcpyDeviceToHost, c.stream));
-    PWK_HIP(hipStreamSynchronize(c.stream));
-    for (int k = 0; k < nke; ++k) {
-      al[(size_t)k].push_back(hv[(size_t)k]);
-      cfh[(size_t)3 * k] = 1.0;
-      cfh[(size_t)3 * k + 1] = hv[(size_t)k];
-      cfh[(size_t)3 * k + 2] = beta[(size_t)k];
-    }
-    PWK_HIP(hipMemcpyAsync(cf.p, cfh.data(), cfh.size() * 8, hipMemcpyHostToDevice, c.stream));
+    k_lanczos_coef<<<kb, 64, 0, c.stream>>>(nke, j, 0, red.as<double>(), beta.as<double>(), cf.as<double>(), al_d,
+                                            be_d, inv.as<double>());
     launch_cheb_step(c, (size_t)nke * npw, (size_t)npw, w, v1, v0, cf.as<double>(), w, PWK_FP64);
     launch_block_dot(c, per, nke, (const double*)w, (const double*)w, red.as<double>());
-    PWK_HIP(hipMemcpyAsync(hv.data(), red.p, (size_t)nke * 8, hipMemcpyDeviceToHost, c.stream));
-    PWK_HIP(hipStreamSynchronize(c.stream));
-    bool stop = false;
-    for (int k = 0; k < nke; ++k) {
-      beta[(size_t)k] = std::sqrt(std::max(hv[(size_t)k], 0.0));
-      be[(size_t)k].push_back(beta[(size_t)k]);
-      inv[(size_t)k] = beta[(size_t)k] > 1e-300 ? 1.0 / beta[(size_t)k] : 0.0;
-      stop |= beta[(size_t)k] < 1e-12;
-    }
-    if (stop || j + 1 == steps) break;
+    k_lanczos_coef<<<kb, 64, 0, c.stream>>>(nke, j, 1, red.as<double>(), beta.as<double>(), cf.as<double>(), al_d,
+                                            be_d, inv.as<double>());
+    PWK_HIP(hipGetLastError());
+    if (j + 1 == steps) break;
     std::swap(v0, v1);  // v0 <- v_j
-    PWK_HIP(hipMemcpyAsync(red.p, inv.data(), (size_t)nke * 8, hipMemcpyHostToDevice, c.stream));
-    launch_block_scale(c, per, nke, red.as<double>(), 0, (double*)w);
+    launch_block_scale(c, per, nke, inv.as<double>(), 0, (double*)w);
     std::swap(v1, w);  // v1 <- v_{j+1}
-    PWK_HIP(hipStreamSynchronize(c.stream));  // inv is host memory reused next step
   }
-  for (int k = 0; k < nke; ++k) {
-    const auto& A = al[(size_t)k];
-    const auto& Bv = be[(size_t)k];
-    double bound = -INFINITY;
-    for (size_t i = 0; i < A.size(); ++i) {
-      const double r = std::fabs(i > 0 ? Bv[i - 1] : 0.0) + std::fabs(i + 1 < A.size() ? Bv[i] : 0.0);
-      bound = std::max(bound, A[i] + r);
-    }
-    ub_c()[k] = bound + std::fabs(Bv.back());
-  }
+  std::vector<double> abh((size_t)2 * steps * nke);
+  PWK_HIP(hipMemcpyAsync(abh.data(), ab.p, abh.size() * 8, hipMemcpyDeviceToHost, c.stream));
   PWK_HIP(hipStreamSynchronize(c.stream));
+  int used = steps;
+  for (int j = 0; j < steps && used == steps; ++j)
+    for (int k = 0; k < nke; ++k)
+      if (abh[(size_t)(steps + j) * nke + k] < 1e-12) used = j + 1;
+  for (int k = 0; k < nke; ++k) {
+    double bound = -INFINITY;
+    for (int i = 0; i < used; ++i) {
+      const double bl = i > 0 ? abh[(size_t)(steps + i - 1) * nke + k] : 0.0;
+      const double br = i + 1 < used ? abh[(size_t)(steps + i) * nke + k] : 0.0;
+      bound = std::max(bound, abh[(size_t)i * nke + k] + std::fabs(bl) + std::fabs(br));
+    }
+    ub_c()[k] = bound + std::fabs(abh[(size_t)(steps + used - 1) * nke + k]);
+  }
 }
 
 // One filtered pass over all k: Y = p_k(H_k)·V_k with the scaled Chebyshev

@@ -379,6 +379,21 @@ def test_dense_eigh_tiny_sizes(gpu):
         _check_pairs(hm, w[0], v[0], np.linalg.eigvalsh(hm), 1e-12, 1e-12)
 
 
+def test_dense_eigh_jacobi_sizes(gpu):
+    # the LDS Jacobi solver (n <= 32) in both of its forms: one barrier a round (a thread for every
+    # rotation block, n <= 8 at one wave) and rotations shared through LDS (larger n); generic,
+    # exactly degenerate and diagonal spectra, odd n (padded pairing)
+    rng = np.random.default_rng(21)
+    for n in (5, 8, 12, 17, 32):
+        spectra = [np.sort(rng.standard_normal(n)) * 3,
+                   np.array([-1.0] * 2 + [0.5] * 2 + list(np.linspace(1.0, 4.0, n - 4)))]
+        mats = [_hermitian_with_spectrum(rng, lam) for lam in spectra]
+        mats.append(np.diag(np.arange(n, dtype=float)[::-1]).astype(np.complex128))
+        w, v = gpu.dense_eigh(np.stack(mats), n)
+        for i, hm in enumerate(mats):
+            _check_pairs(hm, w[i], v[i], np.linalg.eigvalsh(hm), 1e-12, 1e-10)
+
+
 def test_dense_eigh_kohn_sham(gpu, oracle, si_sys):
     # dense H of Si at two k-points: eigenvalues equal the oracle's FP64 EVD
     rng = np.random.default_rng(3)
