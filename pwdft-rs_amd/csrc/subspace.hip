@@ -157,7 +157,8 @@ __global__ void __launch_bounds__(256) k_start_block(int npw, int nb, const int*
 }
 
 // Rayleigh-Ritz of one k-point's block in one workgroup, for m = nbs <= JMAX: overlap S = YᴴY and
-// projected Hs = Yᴴ(HY), S = U s Uᴴ (Jacobi), X = U s^{-1/2} (Löwdin, s floored at floor_rel·s_max),
+// projected Hs = Yᴴ(HY), X with Xᴴ S X = I (X = L^{-H} from the Cholesky factor of S while its
+// columns are independent; else X = U s^{-1/2}, S = U s Uᴴ by Jacobi, s floored at floor_rel·s_max),
 // H' = Xᴴ Hs X = W λ Wᴴ (Jacobi), C = X·W in ascending λ, then V = Y·C and the residual norms
 // |(HY)·C_j − λ_j V_j| of the first nres columns. One launch replaces the transposes, GEMMs,
 // symmetrisations, two batched eigensolves and the residual pass of the general path.
@@ -231,6 +232,8 @@ __global__ void __launch_bounds__(RR_NT) k_rr_small(int npw, int m, int nres, co
       const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
       J.A[i][j] = h;
       J.A[j][i] = make_double2(h.x, -h.y);
+      L.Ys[i][j] = h;  // S, kept for the Löwdin form
+      L.Ys[j][i] = make_double2(h.x, -h.y);
       const double2 c = L.Hs[i][j], d = L.Hs[j][i];
       const double2 g = make_double2(0.5 * (c.x + d.x), 0.5 * (c.y - d.y));
       L.Hs[i][j] = g;
@@ -239,15 +242,67 @@ __global__ void __launch_bounds__(RR_NT) k_rr_small(int npw, int m, int nres, co
   }
   for (int e = t; e < m * m; e += RR_NT) J.V[e % m][e / m] = make_double2(e % m == e / m ? 1.0 : 0.0, 0.0);
   __syncthreads();
-  jacobi_lds(m, J);
-  // X = U diag(1/sqrt(max(s_j, floor_rel·s_max)))
-  double smax = J.A[J.order[m - 1]][J.order[m - 1]].x;
-  for (int e = t; e < m * m; e += RR_NT) {
-    const int r = e % m, j = e / m;
-    const double f = 1.0 / sqrt(fmax(J.A[j][j].x, floor_rel * smax));
-    L.X[r][j] = make_double2(J.V[r][j].x * f, J.V[r][j].y * f);
+  // Cholesky S = L·Lᴴ, one barrier per column, while every pivot keeps more than 1e-8 of its
+  // column's weight S_jj (the columns are far from dependent; the test ignores their scales).
+  // X = L^{-H} satisfies Xᴴ·S·X = I and spans the same space as the Löwdin X, so the Ritz pairs
+  // are the same. L goes to C (lower triangle); the trailing update reads column j, which step j
+  // does not write.
+  bool chol = true;
+  for (int j = 0; j < m; ++j) {
+    const double d = J.A[j][j].x;  // the same on every thread: the branch is block-uniform
+    if (!(d > 1e-8 * L.Ys[j][j].x)) {
+      chol = false;
+      break;
+    }
+    const double sd = sqrt(d), isd = 1.0 / sd, id = isd * isd;
+    for (int e = t; e < m * m; e += RR_NT) {
+      const int i = e % m, l = e / m;
+      if (l == j && i >= j) {
+        const double2 a = J.A[i][j];
+        L.C[i][j] = i == j ? make_double2(sd, 0.0) : make_double2(a.x * isd, a.y * isd);
+      } else if (l > j && i >= l) {
+        const double2 a = J.A[i][j], b = J.A[l][j];  // A_il -= A_ij·conj(A_lj)/d
+        const double2 cur = J.A[i][l];
+        J.A[i][l] = make_double2(cur.x - (a.x * b.x + a.y * b.y) * id, cur.y - (a.y * b.x - a.x * b.y) * id);
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  __syncthreads();  // every thread has read the failing pivot before the Löwdin form rewrites A
+  if (chol) {
+    // L^{-1} by forward substitution on all columns of I at once (B = I in J.V): step j sets
+    // L^{-1}_{jc} = B_jc / L_jj and B_ic -= L_ij·L^{-1}_{jc} below; X_cj = conj(L^{-1}_{jc})
+    for (int j = 0; j < m; ++j) {
+      const double ild = 1.0 / L.C[j][j].x;
+      for (int e = t; e < m * m; e += RR_NT) {
+        const int i = e % m, cc = e / m;
+        const double2 bj = J.V[j][cc];
+        const double2 x = make_double2(bj.x * ild, bj.y * ild);
+        if (i == j) {
+          L.X[cc][j] = make_double2(x.x, -x.y);
+        } else if (i > j) {
+          const double2 lij = L.C[i][j], b = J.V[i][cc];
+          J.V[i][cc] = make_double2(b.x - (lij.x * x.x - lij.y * x.y), b.y - (lij.x * x.y + lij.y * x.x));
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+    // Löwdin: S = U s Uᴴ (Jacobi), X = U diag(1/sqrt(max(s_j, floor_rel·s_max)))
+    for (int e = t; e < m * m; e += RR_NT) {
+      J.A[e % m][e / m] = L.Ys[e % m][e / m];
+      J.V[e % m][e / m] = make_double2(e % m == e / m ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    jacobi_lds(m, J);
+    const double smax = J.A[J.order[m - 1]][J.order[m - 1]].x;
+    for (int e = t; e < m * m; e += RR_NT) {
+      const int r = e % m, j = e / m;
+      const double f = 1.0 / sqrt(fmax(J.A[j][j].x, floor_rel * smax));
+      L.X[r][j] = make_double2(J.V[r][j].x * f, J.V[r][j].y * f);
+    }
+    __syncthreads();
+  }
   // T = Hs·X into C, then H' = Xᴴ·T into J.A (Hermitian part), V = I
   for (int e = t; e < m * m; e += RR_NT) {
     const int i = e % m, j = e / m;
