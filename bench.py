@@ -715,7 +715,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4, help="timed steps; one step = --slots solves served at once")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--slots", type=int, default=12)
+    ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--fp64-solves", type=int, default=5)
     ap.add_argument("--scf-precision", default="fp32", choices=["fp32", "split_bf16", "fp64"])
     ap.add_argument("--scf-eigensolver", default="chebyshev", choices=["dense", "chebyshev"],

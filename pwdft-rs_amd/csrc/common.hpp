@@ -207,6 +207,18 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   int cu_count = 0;
+  // When set (by the request loop's workers, which own it), sync() sleeps on this event
+  // (hipEventBlockingSync) instead of spinning, leaving the host cores to the other workers.
+  hipEvent_t sleep_event = nullptr;
+
+  void sync() {
+    if (sleep_event) {
+      PWK_HIP(hipEventRecord(sleep_event, stream));
+      PWK_HIP(hipEventSynchronize(sleep_event));
+    } else {
+      PWK_HIP(hipStreamSynchronize(stream));
+    }
+  }
 
   // FFT grid (pwk_prepare)
   int dims[3] = {0, 0, 0};

@@ -550,7 +550,7 @@ void setup_basis(Context& c, int npw, const int32_t* m) {
   auto up = [&](DevBuf& d, const void* h, size_t bytes) {
     d.ensure(bytes);
     PWK_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c.stream));
-    PWK_HIP(hipStreamSynchronize(c.stream));
+    c.sync();
   };
   up(c.miller, m, (size_t)npw * 3 * sizeof(int32_t));
   up(c.g2fft, g2fft.data(), g2fft.size() * sizeof(int32_t));

@@ -595,7 +595,7 @@ struct Solve {
     // the fixed-order reduction)
     const size_t np = terms.size() * (size_t)nblk;
     PWK_HIP(hipMemcpyAsync(host_partial, partial.p, np * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    PWK_HIP(hipStreamSynchronize(c.stream));
+    c.sync();
     for (size_t t = 0; t < terms.size(); ++t) {
       double sum = 0.0;
       for (int b = 0; b < nblk; ++b) sum += host_partial[t * nblk + b];
@@ -926,7 +926,7 @@ void Solve::setup() {
     PWK_HIP(hipMemcpyAsync(kin_all32.as<float>() + (size_t)k * npw, kd.kin32.p, (size_t)npw * 4,
                            hipMemcpyDeviceToDevice, c.stream));
     kd.have_kin = true;
-    PWK_HIP(hipStreamSynchronize(c.stream));  // tk is reused
+    c.sync();  // tk is reused
     const double kv[3] = {kc.x, kc.y, kc.z};
     build_nonlocal_device(c, k, kv, gflat.data(), cr.natoms, tau.data(), spi.data(), cr.nspecies, nproj.data(),
                           pl.data(), dij.data(), mesh.data(), rr.data(), rab.data(), chi.data(), omega);
@@ -947,7 +947,7 @@ void Solve::setup() {
         PWK_HIP(hipMemcpyAsync(DBHall64.as<double2>() + o, kd.DBH64.p, e * 16, hipMemcpyDeviceToDevice, c.stream));
         PWK_HIP(hipMemcpyAsync(Ball32.as<float2>() + o, kd.B32.p, e * 8, hipMemcpyDeviceToDevice, c.stream));
         PWK_HIP(hipMemcpyAsync(DBHall32.as<float2>() + o, kd.DBH32.p, e * 8, hipMemcpyDeviceToDevice, c.stream));
-        PWK_HIP(hipStreamSynchronize(c.stream));
+        c.sync();
       }
       kd.B64.release();
       kd.DBH64.release();
@@ -1014,9 +1014,9 @@ void Solve::setup() {
     launch_start_block(c, npw, nbs, nk, lowest.as<int>(), V.p);
     if (ns == 2)  // the same start block for the spin-down channel
       PWK_HIP(hipMemcpyAsync(V.as<double2>() + vblk(), V.p, vblk() * 16, hipMemcpyDeviceToDevice, c.stream));
-    PWK_HIP(hipStreamSynchronize(c.stream));
+    c.sync();
   }
-  PWK_HIP(hipStreamSynchronize(c.stream));
+  c.sync();
 }
 
 // src/ewald.rs:43-143
@@ -1057,7 +1057,7 @@ double Solve::ewald() {
   PWK_HIP(hipGetLastError());
   std::vector<double> hp(2 * nblk);
   PWK_HIP(hipMemcpyAsync(hp.data(), part.p, 2 * nblk * 8, hipMemcpyDeviceToHost, c.stream));
-  PWK_HIP(hipStreamSynchronize(c.stream));
+  c.sync();
   double er = 0.0, eq = 0.0;
   for (int i = 0; i < nblk; ++i) er += hp[(size_t)i];
   for (int i = 0; i < nblk; ++i) eq += hp[(size_t)nblk + i];
@@ -1180,7 +1180,7 @@ void Solve::rr_collect() {
   if (!rr_small_supported(nbs)) return;  // the general path has already copied its results
   const size_t nkm = (size_t)nke * nbs;
   PWK_HIP(hipMemcpyAsync(rr_pinned, s_lam.p, 2 * nkm * 8, hipMemcpyDeviceToHost, c.stream));
-  PWK_HIP(hipStreamSynchronize(c.stream));
+  c.sync();
   std::memcpy(ev_sub_c(), rr_pinned, nkm * 8);
   std::memcpy(resid_host.data(), rr_pinned + nkm, nkm * 8);
 }
@@ -1208,7 +1208,7 @@ void Solve::rr_general(const double2* Y, const double2* HY) {
   launch_resid_norms(c, npw, m, nb, nke, Vc(), R64.p, s_lam.as<double>(), resid.as<double>());
   PWK_HIP(hipMemcpyAsync(ev_sub_c(), s_lam.p, (size_t)nke * nbs * 8, hipMemcpyDeviceToHost, c.stream));
   PWK_HIP(hipMemcpyAsync(resid_host.data(), resid.p, resid_host.size() * 8, hipMemcpyDeviceToHost, c.stream));
-  PWK_HIP(hipStreamSynchronize(c.stream));
+  c.sync();
 }
 
 // Upper bound of the spectrum of every H_k: ten Lanczos steps from a seeded
@@ -1260,7 +1260,7 @@ void Solve::lanczos_bounds() {
   }
   std::vector<double> abh((size_t)2 * steps * nke);
   PWK_HIP(hipMemcpyAsync(abh.data(), ab.p, abh.size() * 8, hipMemcpyDeviceToHost, c.stream));
-  PWK_HIP(hipStreamSynchronize(c.stream));
+  c.sync();
   int used = steps;
   for (int j = 0; j < steps && used == steps; ++j)
     for (int k = 0; k < nke; ++k)
@@ -1488,7 +1488,7 @@ void Solve::run(double* eig_out, double* rho_out) {
                             nullptr);
         PWK_HIP(hipMemcpyAsync(evc, W.as<double>() + (size_t)sc * nk * nb, (size_t)nk * nb * 8,
                                hipMemcpyDeviceToHost, c.stream));
-        PWK_HIP(hipStreamSynchronize(c.stream));
+        c.sync();
       }
     }
     sc = 0;
@@ -1652,7 +1652,7 @@ void Solve::run(double* eig_out, double* rho_out) {
                                                            expect.as<double>());
           PWK_HIP(hipGetLastError());
           PWK_HIP(hipMemcpyAsync(e2.data(), expect.p, e2.size() * 8, hipMemcpyDeviceToHost, c.stream));
-          PWK_HIP(hipStreamSynchronize(c.stream));
+          c.sync();
           double ek = 0.0, enl = 0.0;
           for (int b = 0; b < nb; ++b) {
             const double f = occ[((size_t)s * nk + k) * nb + b];
@@ -1666,7 +1666,7 @@ void Solve::run(double* eig_out, double* rho_out) {
       if (eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);
       if (rho_out) {
         PWK_HIP(hipMemcpyAsync(rho_out, rho_tot_new, N * 8, hipMemcpyDeviceToHost, c.stream));
-        PWK_HIP(hipStreamSynchronize(c.stream));
+        c.sync();
       }
       break;
     }
@@ -1682,7 +1682,7 @@ void Solve::run(double* eig_out, double* rho_out) {
     }
   }
   if (!res.converged && eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);
-  PWK_HIP(hipStreamSynchronize(c.stream));
+  c.sync();
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

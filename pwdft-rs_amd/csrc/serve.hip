@@ -114,7 +114,7 @@ struct Server {
 
   // Runs one solve in a fresh context on the worker's stream. The context's buffers come
   // from the worker's block cache and go back to it when the solve ends.
-  void solve(ServeJob& job, hipStream_t stream, int cu_count) {
+  void solve(ServeJob& job, hipStream_t stream, hipEvent_t sleep_event, int cu_count) {
     if (job.bytes > budget) {
       job.status = PWK_E_OOM;
       job.error = "solve does not fit in the server's device-memory budget";
@@ -125,6 +125,7 @@ struct Server {
       c.device = device;
       c.stream = stream;
       c.cu_count = cu_count;
+      c.sleep_event = sleep_event;
       struct StreamGuard {
         Context& c;
         ~StreamGuard() {
@@ -154,10 +155,12 @@ struct Server {
   void worker_main() {
     // per-worker stream and block caches, kept across solves
     hipStream_t stream = nullptr;
+    hipEvent_t sleep_event = nullptr;
     int cu_count = 0;
     try {
       PWK_HIP(hipSetDevice(device));
       PWK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+      PWK_HIP(hipEventCreateWithFlags(&sleep_event, hipEventBlockingSync | hipEventDisableTiming));
       hipDeviceProp_t prop;
       PWK_HIP(hipGetDeviceProperties(&prop, device));
       cu_count = prop.multiProcessorCount;
@@ -188,7 +191,7 @@ struct Server {
       }
       job->t_start = std::chrono::steady_clock::now();
       if (stream) {
-        solve(*job, stream, cu_count);
+        solve(*job, stream, sleep_event, cu_count);
         dev_cache.trim(cache_cap);
         host_cache.trim(HOST_CACHE_CAP);
       } else {
@@ -209,6 +212,7 @@ struct Server {
     tl_host_cache = nullptr;
     dev_cache.trim(0);
     host_cache.trim(0);
+    if (sleep_event) (void)hipEventDestroy(sleep_event);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
