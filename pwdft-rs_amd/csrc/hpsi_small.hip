@@ -44,46 +44,131 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
   const int k = nkv > 0 ? kb % nkv : kb;
   if (nkv > 0) vr += (size_t)(kb / nkv) * N * N * N;
   const V2* p = psi + (size_t)b * npw;
+  const T* t = kin + (size_t)k * npw;
+  const V2* bk = B + (size_t)k * npw * nch;
+  const V2* xb = X ? X + (size_t)b * npw : nullptr;
+  // operands of the epilogue for this thread's first plane wave, loaded now so that their latency
+  // hides behind the transforms (projector rows for up to PF channels)
+  constexpr int PF = 16;
+  const int g0 = threadIdx.x;
+  const bool pf = g0 < npw && nch <= PF;
+  V2 pf_b[PF], pf_v, pf_x;
+  T pf_t = 0;
+  pf_v.x = pf_v.y = pf_x.x = pf_x.y = 0;
+  if (pf) {
+    pf_v = p[g0];
+    pf_t = t[g0];
+    if (xb) pf_x = xb[g0];
+#pragma unroll
+    for (int ch = 0; ch < PF; ++ch)
+      if (ch < nch) pf_b[ch] = bk[(size_t)ch * npw + g0];
+  }
+  // the projections of up to 16 channels (one group of four per wave) when npw <= 256, and
+  // V(r) of the thread's x-line, are loaded now too and used after the first transforms
+  constexpr int NW = SMALL_NT / 64, CPW = 4, PGI = SMALL_NT / 64;
+  constexpr bool PRE_V = N * N <= SMALL_NT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const V2* dk = DBH + (size_t)k * nch * npw;
+  const bool pre_proj = nch > 0 && nch <= NW * CPW && npw <= 64 * PGI;
+  V2 pd[CPW][PGI], pv[PGI];
+  if (pre_proj) {
+#pragma unroll
+    for (int it = 0; it < PGI; ++it) {
+      const int g = lane + 64 * it;
+      if (g < npw) {
+        pv[it] = p[g];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j)
+          if (wave + j * NW < nch) pd[j][it] = dk[(size_t)g * nch + wave + j * NW];
+      }
+    }
+  }
+  T vline[PRE_V ? N : 1];
+  if constexpr (PRE_V) {
+    if (threadIdx.x < N * N) {
+#pragma unroll
+      for (int f = 0; f < N; ++f) vline[f] = vr[threadIdx.x + f * N * N];
+    }
+  }
   for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
     G[i].x = 0;
     G[i].y = 0;
   }
-  __syncthreads();
+  lds_barrier();
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = p[g];
-  // projections P_c = Σ_G (D·B^H)[c, G] ψ(G): one wave per channel, lanes over G
-  if (nch > 0) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const V2* dk = DBH + (size_t)k * nch * npw;
-    for (int ch = wave; ch < nch; ch += SMALL_NT / 64) {
-      T ar = 0, ai = 0;
+  // projections P_c = Σ_G (D·B^H)[c, G] ψ(G): one wave per channel, lanes over G; a wave runs
+  // up to four of its channels side by side (same order of additions per channel)
+  if (nch > 0 && !pre_proj) {
+    for (int c0 = wave; c0 < nch; c0 += NW * CPW) {
+      T ar[CPW], ai[CPW];
+#pragma unroll
+      for (int j = 0; j < CPW; ++j) ar[j] = ai[j] = 0;
       for (int g = lane; g < npw; g += 64) {
-        const V2 d = dk[(size_t)g * nch + ch], v = p[g];
-        ar += d.x * v.x - d.y * v.y;
-        ai += d.x * v.y + d.y * v.x;
+        const V2 v = p[g];
+        V2 d[CPW];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j)
+          if (c0 + j * NW < nch) d[j] = dk[(size_t)g * nch + c0 + j * NW];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j)
+          if (c0 + j * NW < nch) {
+            ar[j] += d[j].x * v.x - d[j].y * v.y;
+            ai[j] += d[j].x * v.y + d[j].y * v.x;
+          }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        ar += __shfl_xor(ar, off, 64);
-        ai += __shfl_xor(ai, off, 64);
-      }
-      if (lane == 0) {
-        proj[ch].x = ar;
-        proj[ch].y = ai;
+#pragma unroll
+      for (int j = 0; j < CPW; ++j) {
+        for (int off = 32; off > 0; off >>= 1) {
+          ar[j] += __shfl_xor(ar[j], off, 64);
+          ai[j] += __shfl_xor(ai[j], off, 64);
+        }
+        if (lane == 0 && c0 + j * NW < nch) {
+          proj[c0 + j * NW].x = ar[j];
+          proj[c0 + j * NW].y = ai[j];
+        }
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
   lds_lines<T, N, 2, 1>(G, nullptr);
-  __syncthreads();
+  if (pre_proj) {  // the prefetched projections, same order of additions as the loop above
+    T ar[CPW], ai[CPW];
+#pragma unroll
+    for (int j = 0; j < CPW; ++j) ar[j] = ai[j] = 0;
+#pragma unroll
+    for (int it = 0; it < PGI; ++it)
+      if (lane + 64 * it < npw) {
+#pragma unroll
+        for (int j = 0; j < CPW; ++j)
+          if (wave + j * NW < nch) {
+            ar[j] += pd[j][it].x * pv[it].x - pd[j][it].y * pv[it].y;
+            ai[j] += pd[j][it].x * pv[it].y + pd[j][it].y * pv[it].x;
+          }
+      }
+#pragma unroll
+    for (int j = 0; j < CPW; ++j) {
+      for (int off = 32; off > 0; off >>= 1) {
+        ar[j] += __shfl_xor(ar[j], off, 64);
+        ai[j] += __shfl_xor(ai[j], off, 64);
+      }
+      if (lane == 0 && wave + j * NW < nch) {
+        proj[wave + j * NW].x = ar[j];
+        proj[wave + j * NW].y = ai[j];
+      }
+    }
+  }
+  lds_barrier();
   lds_lines<T, N, 1, 1>(G, nullptr);
-  __syncthreads();
-  lds_lines<T, N, 0, 1>(G, vr);
-  __syncthreads();
+  lds_barrier();
+  if constexpr (PRE_V)
+    lds_xlines_v<T, N, 1>(G, vline);
+  else
+    lds_lines<T, N, 0, 1>(G, vr);
+  lds_barrier();
   lds_lines<T, N, 1, -1>(G, nullptr);
-  __syncthreads();
+  lds_barrier();
   lds_lines<T, N, 2, -1>(G, nullptr);
-  __syncthreads();
-  const T* t = kin + (size_t)k * npw;
-  const V2* bk = B + (size_t)k * npw * nch;
+  lds_barrier();
   V2* h = hpsi + (size_t)b * npw;
   T alpha = 0, cshift = 0, beta = 0;
   if (coef) {
@@ -91,15 +176,25 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
     cshift = (T)coef[3 * kb + 1];
     beta = (T)coef[3 * kb + 2];
   }
-  const V2* xb = X ? X + (size_t)b * npw : nullptr;
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
-    const V2 v = p[g], w = G[g2fft[g]];
-    const T tk = t[g];
+    const bool first = pf && g == g0;
+    const V2 v = first ? pf_v : p[g], w = G[g2fft[g]];
+    const T tk = first ? pf_t : t[g];
     T ox = tk * v.x + w.x * inv_n, oy = tk * v.y + w.y * inv_n;
-    for (int ch = 0; ch < nch; ++ch) {
-      const V2 bb = bk[(size_t)ch * npw + g], pc = proj[ch];
-      ox += bb.x * pc.x - bb.y * pc.y;
-      oy += bb.x * pc.y + bb.y * pc.x;
+    if (first) {
+#pragma unroll
+      for (int ch = 0; ch < PF; ++ch)
+        if (ch < nch) {
+          const V2 bb = pf_b[ch], pc = proj[ch];
+          ox += bb.x * pc.x - bb.y * pc.y;
+          oy += bb.x * pc.y + bb.y * pc.x;
+        }
+    } else {
+      for (int ch = 0; ch < nch; ++ch) {
+        const V2 bb = bk[(size_t)ch * npw + g], pc = proj[ch];
+        ox += bb.x * pc.x - bb.y * pc.y;
+        oy += bb.x * pc.y + bb.y * pc.x;
+      }
     }
     V2 o;
     if (coef) {
@@ -107,7 +202,7 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
       o.x = alpha * (ox - cshift * v.x);
       o.y = alpha * (oy - cshift * v.y);
       if (xb) {
-        const V2 xv = xb[g];
+        const V2 xv = first ? pf_x : xb[g];
         o.x -= beta * xv.x;
         o.y -= beta * xv.y;
       }

@@ -19,6 +19,15 @@ struct V2s<float> {
 
 constexpr int SMALL_NT = 256;
 
+// Workgroup barrier that orders LDS only: it waits for this wave's LDS operations, not for its
+// outstanding global loads, so operands prefetched into registers stay in flight across it.
+// Sufficient wherever the workgroup exchanges data through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // One set of N² lines through the LDS grid G[x][y][z] (z fastest). AXIS 2: z-lines
 // (stride 1), 1: y-lines (stride N), 0: x-lines (stride N²). With vr, the x pass runs
 // inverse -> × V(r) -> forward on each line while it is in registers.
@@ -63,6 +72,35 @@ __device__ __forceinline__ void lds_lines(typename V2s<T>::t* G, const T* __rest
       G[base + f * es].x = re[FftOrder<N>::pos(f)];
       G[base + f * es].y = im[FftOrder<N>::pos(f)];
     }
+  }
+}
+
+// The x pass of lds_lines with V(r) of the thread's line already in registers (vline[f] =
+// V at x = f of line threadIdx.x); for grids with at most one x-line per thread.
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void lds_xlines_v(typename V2s<T>::t* G, const T (&vline)[N]) {
+  static_assert(N * N <= SMALL_NT, "one x-line per thread");
+  constexpr int es = N * N;
+  const int l = threadIdx.x;
+  if (l >= N * N) return;
+  T re[N], im[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    re[j] = G[l + j * es].x;
+    im[j] = G[l + j * es].y;
+  }
+  fft_dif<T, N, SIGN>(re, im);
+  T r2[N], m2[N];
+#pragma unroll
+  for (int f = 0; f < N; ++f) {
+    r2[f] = re[FftOrder<N>::pos(f)] * vline[f];
+    m2[f] = im[FftOrder<N>::pos(f)] * vline[f];
+  }
+  fft_dif<T, N, -SIGN>(r2, m2);
+#pragma unroll
+  for (int f = 0; f < N; ++f) {
+    G[l + f * es].x = r2[FftOrder<N>::pos(f)];
+    G[l + f * es].y = m2[FftOrder<N>::pos(f)];
   }
 }
 
