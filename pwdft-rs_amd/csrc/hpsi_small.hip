@@ -285,7 +285,16 @@ __global__ void __launch_bounds__(256) k_density_reduce(size_t n, int ngroups, c
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   double s = overwrite ? 0.0 : rho[i];
-  for (int g = 0; g < ngroups; ++g) s += partial[(size_t)g * n + i];
+  // sixteen partials in flight at a time, added in group order
+  constexpr int U = 16;
+  for (int g0 = 0; g0 < ngroups; g0 += U) {
+    double v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) v[j] = g0 + j < ngroups ? partial[(size_t)(g0 + j) * n + i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      if (g0 + j < ngroups) s += v[j];
+  }
   rho[i] = s;
 }
 

@@ -305,33 +305,55 @@ __global__ void k_hist_store(size_t n, const double* __restrict__ in, const doub
 }
 
 // ρ_sym(G) = (1/N_ops) Σ_S e^{-2πi n·τ_S} ρ(R_S^T n)  (src/symmetry/density/g_space.rs:147-252)
+// Sixteen lanes per grid point evaluate the operations (lane l takes l, l + 16, ...); the
+// terms are then added in operation order, as one thread adding them in sequence would
+// (nops <= 64).
+constexpr int SYM_MAX_OPS = 64;
+
 __global__ void __launch_bounds__(256) k_sym_gather(size_t n, int nx, int ny, int nz, int nops,
                                                     const int* __restrict__ rot, const double* __restrict__ tau,
                                                     const double2* __restrict__ in, double2* __restrict__ out,
                                                     double inv) {
-  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= n) return;
+  const size_t idx = (size_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int sub = threadIdx.x & 15, base = threadIdx.x & 48;  // lane of the group's first thread, in its wave
+  if (idx >= n) return;  // the whole group of sixteen lanes
   const int i1 = (int)(idx / ((size_t)ny * nz)), i2 = (int)((idx / nz) % ny), i3 = (int)(idx % nz);
   const int nd[3] = {i1 > nx / 2 ? i1 - nx : i1, i2 > ny / 2 ? i2 - ny : i2, i3 > nz / 2 ? i3 - nz : i3};
   const int dd[3] = {nx, ny, nz};
-  double ar = 0.0, ai = 0.0;
-  for (int o = 0; o < nops; ++o) {
-    const int* R = rot + 9 * o;
-    int src[3];
-    for (int c = 0; c < 3; ++c) {
-      const int ns = R[c] * nd[0] + R[3 + c] * nd[1] + R[6 + c] * nd[2];
-      src[c] = ((ns % dd[c]) + dd[c]) % dd[c];
+  constexpr int S = SYM_MAX_OPS / 16;
+  double tr[S], ti[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    tr[q] = ti[q] = 0.0;
+    const int o = sub + 16 * q;
+    if (o < nops) {
+      const int* R = rot + 9 * o;
+      int src[3];
+      for (int c = 0; c < 3; ++c) {
+        const int ns = R[c] * nd[0] + R[3 + c] * nd[1] + R[6 + c] * nd[2];
+        src[c] = ((ns % dd[c]) + dd[c]) % dd[c];
+      }
+      const double2 v = in[((size_t)src[0] * ny + src[1]) * nz + src[2]];
+      const double arg = 6.283185307179586 * ((double)nd[0] * tau[3 * o] + (double)nd[1] * tau[3 * o + 1] +
+                                              (double)nd[2] * tau[3 * o + 2]);
+      double sn, cs;
+      sincos(arg, &sn, &cs);
+      // (cos, -sin) · v
+      tr[q] = cs * v.x + sn * v.y;
+      ti[q] = cs * v.y - sn * v.x;
     }
-    const double2 v = in[((size_t)src[0] * ny + src[1]) * nz + src[2]];
-    const double arg = 6.283185307179586 * ((double)nd[0] * tau[3 * o] + (double)nd[1] * tau[3 * o + 1] +
-                                            (double)nd[2] * tau[3 * o + 2]);
-    double sn, cs;
-    sincos(arg, &sn, &cs);
-    // (cos, -sin) · v
-    ar += cs * v.x + sn * v.y;
-    ai += cs * v.y - sn * v.x;
   }
-  out[idx] = make_double2(ar * inv, ai * inv);
+  double ar = 0.0, ai = 0.0;
+#pragma unroll
+  for (int q = 0; q < S; ++q)
+    for (int l = 0; l < 16; ++l) {
+      const double xr = __shfl(tr[q], base + l, 64), xi = __shfl(ti[q], base + l, 64);
+      if (16 * q + l < nops) {
+        ar += xr;
+        ai += xi;
+      }
+    }
+  if (sub == 0) out[idx] = make_double2(ar * inv, ai * inv);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -956,6 +978,7 @@ void Solve::setup() {
     }
   }
   // symmetry tables for the density
+  PWK_REQUIRE(ops.size() <= (size_t)SYM_MAX_OPS, PWK_E_UNSUPPORTED, "more symmetry operations than the gather kernel takes");
   std::vector<int> rot;
   std::vector<double> tr;
   for (const SymOp& o : ops) {
@@ -1538,7 +1561,7 @@ void Solve::run(double* eig_out, double* rho_out) {
       if (ops.size() > 1) {
         TimedScope ts(c, "scf_symmetrize");
         launch_fft3d_r2c(c, rn, 1.0, cwork.p, -1);
-        k_sym_gather<<<gb(N), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
+        k_sym_gather<<<blocks_for(N, 16), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
                                                    sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
                                                    rho_g_new.as<double2>(), 1.0 / (double)ops.size());
         launch_fft3d(c, rho_g_new.p, 1, +1, PWK_FP64);
