@@ -77,9 +77,18 @@ __global__ void __launch_bounds__(256) k_zgemm_mfma(int M, int N, int K, int kch
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
-  const int m0 = blockIdx.x * BMC, n0 = blockIdx.y * BN;
-  // blockIdx.z = batch · nsplit + split
-  const int zb = (int)blockIdx.z / nsplit, zs = (int)blockIdx.z % nsplit;
+  // One-dimensional grid over (row tile, column tile, batch · nsplit + split). Workgroup i runs on
+  // XCD i mod 8, and each XCD takes a contiguous range of tiles in an order where neighbours
+  // share an operand in that XCD's L2: with split K, the row and column tiles of one K chunk
+  // (they read the same B rows); without, the column tiles of one row tile (the same A rows).
+  const int mt = (M + BMC - 1) / BMC, nt = (N + BN - 1) / BN, plane = mt * nt;
+  const int total = (int)gridDim.x, xcd = (int)blockIdx.x & 7;
+  const int lg = xcd * (total >> 3) + min(xcd, total & 7) + ((int)blockIdx.x >> 3);
+  const int bz = lg / plane, rem = lg % plane;
+  const int bx = nsplit > 1 ? rem % mt : rem / nt, by = nsplit > 1 ? rem / mt : rem % nt;
+  const int m0 = bx * BMC, n0 = by * BN;
+  // bz = batch · nsplit + split
+  const int zb = bz / nsplit, zs = bz % nsplit;
   A += (size_t)zb * sA;
   B += (size_t)zb * sB;
   C += (size_t)zb * sC;
@@ -356,7 +365,8 @@ void zgemm_mfma_t(Context& c, int M, int N, int K, const void* A, int lda, const
   kchunk = ((kchunk + BKC - 1) / BKC) * BKC;
   nsplit = (K + kchunk - 1) / kchunk;
   PWK_REQUIRE((long)nsplit * bt.count <= 65535, PWK_E_UNSUPPORTED, "GEMM batch too large");
-  dim3 grid((unsigned)mt, (unsigned)nt, (unsigned)(nsplit * bt.count));
+  PWK_REQUIRE((long)mt * nt * nsplit * bt.count <= 0x7fffffffL, PWK_E_UNSUPPORTED, "GEMM grid too large");
+  dim3 grid((unsigned)((long)mt * nt * nsplit * bt.count));
   void* part = nullptr;
   if (nsplit > 1) {
     c.gemm_ws.ensure((size_t)bt.count * nsplit * M * N * sizeof(V2));
