@@ -550,7 +550,6 @@ void setup_basis(Context& c, int npw, const int32_t* m) {
   auto up = [&](DevBuf& d, const void* h, size_t bytes) {
     d.ensure(bytes);
     PWK_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c.stream));
-    c.sync();
   };
   up(c.miller, m, (size_t)npw * 3 * sizeof(int32_t));
   up(c.g2fft, g2fft.data(), g2fft.size() * sizeof(int32_t));
@@ -560,6 +559,7 @@ void setup_basis(Context& c, int npw, const int32_t* m) {
   up(c.st_xy, xy.data(), xy.size() * sizeof(int32_t));
   up(c.x1list, x1list.data(), x1list.size() * sizeof(int32_t));
   up(c.st_x1, sx1.data(), sx1.size() * sizeof(int32_t));
+  c.sync();  // the host tables go out of scope here
 }
 
 void reserve_veff_slots(Context& c, int nslots) {

@@ -5,6 +5,7 @@
 //   DBH[c,G] = Σ_{j: l_j = l_i} D_ij conj(B[G, (a, j, m)])
 // One lane per plane wave; the radial mesh is read through L1/L2.
 #include <algorithm>
+#include <cstring>
 
 #include "common.hpp"
 
@@ -169,11 +170,31 @@ __global__ void __launch_bounds__(256) k_qnorm(int npw, const double* __restrict
   qn[ig] = sqrt(x * x + y * y + z * z);
 }
 
-template <class T>
-void upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
-  d.ensure(std::max<size_t>(1, h.size()) * sizeof(T));
-  if (!h.empty()) PWK_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
-}
+// Host tables of one build packed into a single upload (each array at a 256-byte aligned offset,
+// at least one element), so a build costs one copy instead of one per table.
+struct Pack {
+  std::vector<unsigned char> blob;
+  template <class T>
+  size_t add(const T* h, size_t n) {
+    const size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + std::max<size_t>(1, n) * sizeof(T), 0);
+    if (n) std::memcpy(blob.data() + off, h, n * sizeof(T));
+    return off;
+  }
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    return add(v.data(), v.size());
+  }
+};
+
+// A typed view of one array inside the uploaded pack.
+struct PackView {
+  void* p;
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
 
 }  // namespace
 
@@ -243,25 +264,20 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
   kd.nch = nch;
   if (nch == 0) return;
   hipStream_t st = c.stream;
-  DevBuf d_k, d_g, d_tau, d_qn, d_pl, d_pm, d_pro, d_pco, d_r, d_rab, d_chi, d_F, d_ca, d_cp, d_cl, d_cm, d_ti, d_tj,
-      d_td;
-  upload(d_k, std::vector<double>(kvec, kvec + 3), st);
-  upload(d_g, std::vector<double>(gvec, gvec + (size_t)npw * 3), st);
-  upload(d_tau, std::vector<double>(tau, tau + (size_t)natoms * 3), st);
-  upload(d_pl, pl, st);
-  upload(d_pm, pmesh, st);
-  upload(d_pro, profs, st);
-  upload(d_pco, pcofs, st);
-  upload(d_r, std::vector<double>(r, r + m0[nspecies]), st);
-  upload(d_rab, std::vector<double>(rab, rab + m0[nspecies]), st);
-  upload(d_chi, std::vector<double>(chi, chi + c0[nspecies]), st);
-  upload(d_ca, ch_atom, st);
-  upload(d_cp, ch_proj, st);
-  upload(d_cl, ch_l, st);
-  upload(d_cm, ch_m, st);
-  upload(d_ti, tri_ci, st);
-  upload(d_tj, tri_cj, st);
-  upload(d_td, tri_d, st);
+  Pack pk;
+  const size_t o_k = pk.add(kvec, 3), o_g = pk.add(gvec, (size_t)npw * 3), o_tau = pk.add(tau, (size_t)natoms * 3);
+  const size_t o_pl = pk.add(pl), o_pm = pk.add(pmesh), o_pro = pk.add(profs), o_pco = pk.add(pcofs);
+  const size_t o_r = pk.add(r, (size_t)m0[nspecies]), o_rab = pk.add(rab, (size_t)m0[nspecies]);
+  const size_t o_chi = pk.add(chi, (size_t)c0[nspecies]);
+  const size_t o_ca = pk.add(ch_atom), o_cp = pk.add(ch_proj), o_cl = pk.add(ch_l), o_cm = pk.add(ch_m);
+  const size_t o_ti = pk.add(tri_ci), o_tj = pk.add(tri_cj), o_td = pk.add(tri_d);
+  DevBuf d_pack, d_qn, d_F;
+  d_pack.ensure(pk.blob.size());
+  PWK_HIP(hipMemcpyAsync(d_pack.p, pk.blob.data(), pk.blob.size(), hipMemcpyHostToDevice, st));
+  auto at = [&](size_t off) { return PackView{static_cast<unsigned char*>(d_pack.p) + off}; };
+  const PackView d_k = at(o_k), d_g = at(o_g), d_tau = at(o_tau), d_pl = at(o_pl), d_pm = at(o_pm), d_pro = at(o_pro),
+             d_pco = at(o_pco), d_r = at(o_r), d_rab = at(o_rab), d_chi = at(o_chi), d_ca = at(o_ca),
+             d_cp = at(o_cp), d_cl = at(o_cl), d_cm = at(o_cm), d_ti = at(o_ti), d_tj = at(o_tj), d_td = at(o_td);
   d_qn.ensure((size_t)npw * sizeof(double));
   d_F.ensure((size_t)std::max(1, nprojtot) * npw * sizeof(double));
   k_qnorm<<<blocks_for((size_t)npw, 256), 256, 0, st>>>(npw, d_k.as<double>(), d_g.as<double>(), d_qn.as<double>());

@@ -926,7 +926,7 @@ void Solve::setup() {
     rab.insert(rab.end(), sp.rab, sp.rab + sp.mesh);
     for (int i = 0; i < sp.nproj * sp.mesh; ++i) chi.push_back(sp.beta[i]);
   }
-  std::vector<double> gflat((size_t)npw * 3), tk((size_t)npw);
+  std::vector<double> gflat((size_t)npw * 3);
   for (int i = 0; i < npw; ++i) {
     gflat[(size_t)3 * i] = gvec[(size_t)i].x;
     gflat[(size_t)3 * i + 1] = gvec[(size_t)i].y;
@@ -935,20 +935,25 @@ void Solve::setup() {
   chebyshev = p.eigensolver == 1 || (p.eigensolver < 0 && !(npw <= 1500 && heev_supported(npw, nb)));
   kin_all64.ensure((size_t)nk * npw * 8);
   kin_all32.ensure((size_t)nk * npw * 4);
+  // kinetic diagonals of all k-points: one upload and one conversion, then per-k copies
+  std::vector<double> tk_all((size_t)nk * npw);
   for (int k = 0; k < nk; ++k) {
     const Vec3 kc = kpts[(size_t)k].cart;
-    for (int i = 0; i < npw; ++i) tk[(size_t)i] = kHbar2Over2m * vnorm2(vadd(kc, gvec[(size_t)i]));
+    for (int i = 0; i < npw; ++i)
+      tk_all[(size_t)k * npw + i] = kHbar2Over2m * vnorm2(vadd(kc, gvec[(size_t)i]));
+  }
+  PWK_HIP(hipMemcpyAsync(kin_all64.p, tk_all.data(), tk_all.size() * 8, hipMemcpyHostToDevice, c.stream));
+  launch_convert(c, kin_all64.p, kin_all32.p, tk_all.size(), true);
+  for (int k = 0; k < nk; ++k) {
+    const Vec3 kc = kpts[(size_t)k].cart;
     KData& kd = c.k(k);
     kd.kin64.ensure((size_t)npw * 8);
     kd.kin32.ensure((size_t)npw * 4);
-    PWK_HIP(hipMemcpyAsync(kd.kin64.p, tk.data(), (size_t)npw * 8, hipMemcpyHostToDevice, c.stream));
-    launch_convert(c, kd.kin64.p, kd.kin32.p, (size_t)npw, true);
-    PWK_HIP(hipMemcpyAsync(kin_all64.as<double>() + (size_t)k * npw, kd.kin64.p, (size_t)npw * 8,
+    PWK_HIP(hipMemcpyAsync(kd.kin64.p, kin_all64.as<double>() + (size_t)k * npw, (size_t)npw * 8,
                            hipMemcpyDeviceToDevice, c.stream));
-    PWK_HIP(hipMemcpyAsync(kin_all32.as<float>() + (size_t)k * npw, kd.kin32.p, (size_t)npw * 4,
+    PWK_HIP(hipMemcpyAsync(kd.kin32.p, kin_all32.as<float>() + (size_t)k * npw, (size_t)npw * 4,
                            hipMemcpyDeviceToDevice, c.stream));
     kd.have_kin = true;
-    c.sync();  // tk is reused
     const double kv[3] = {kc.x, kc.y, kc.z};
     build_nonlocal_device(c, k, kv, gflat.data(), cr.natoms, tau.data(), spi.data(), cr.nspecies, nproj.data(),
                           pl.data(), dij.data(), mesh.data(), rr.data(), rab.data(), chi.data(), omega);
