@@ -331,10 +331,16 @@ size_t heev_work_doubles(int n, int nb, int nk);
 bool heev_supported(int n, int nb);
 void launch_heev_batched(Context& c, int n, int nb, int nk, void* H, void* work, double* w, void* V, int* status);
 // nonlocal.hip
+// Where a build writes its projectors: the k-point's own buffers (KData B64, DBH64, B32, DBH32)
+// by default, or caller-owned arrays of npw × nch entries (the SCF's batched blocks).
+struct NonlocalDest {
+  int nch = 0;  // channel count the arrays are sized for (checked)
+  void *b64 = nullptr, *dbh64 = nullptr, *b32 = nullptr, *dbh32 = nullptr;
+};
 void build_nonlocal_device(Context& c, int ik, const double* kvec, const double* gvec, int natoms, const double* tau,
                            const int32_t* atom_species, int nspecies, const int32_t* nproj, const int32_t* proj_l,
                            const double* dij, const int32_t* mesh, const double* r, const double* rab,
-                           const double* chi, double omega);
+                           const double* chi, double omega, const NonlocalDest* dest = nullptr);
 
 // subspace.hip: Chebyshev filter and Rayleigh-Ritz helpers, batched over blocks
 void launch_cheb_step(Context& c, size_t n, size_t per, const void* HY, const void* Y, const void* X,

@@ -201,7 +201,7 @@ struct PackView {
 void build_nonlocal_device(Context& c, int ik, const double* kvec, const double* gvec, int natoms, const double* tau,
                            const int32_t* atom_species, int nspecies, const int32_t* nproj, const int32_t* proj_l,
                            const double* dij, const int32_t* mesh, const double* r, const double* rab,
-                           const double* chi, double omega) {
+                           const double* chi, double omega, const NonlocalDest* dest) {
   c.check_basis();
   KData& kd = c.k(ik);
   const int npw = c.npw;
@@ -289,21 +289,35 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
                                       d_chi.as<double>(), d_F.as<double>());
     PWK_HIP(hipGetLastError());
   }
-  kd.B64.ensure((size_t)npw * nch * sizeof(double2));
-  kd.DBH64.ensure((size_t)npw * nch * sizeof(double2));
-  kd.B32.ensure((size_t)npw * nch * sizeof(float2));
-  kd.DBH32.ensure((size_t)npw * nch * sizeof(float2));
+  void *b64, *dbh64, *b32, *dbh32;
+  if (dest) {
+    PWK_REQUIRE(dest->nch == nch, PWK_E_STATE, "projector destination sized for another channel count");
+    b64 = dest->b64;
+    dbh64 = dest->dbh64;
+    b32 = dest->b32;
+    dbh32 = dest->dbh32;
+  } else {
+    kd.B64.ensure((size_t)npw * nch * sizeof(double2));
+    kd.DBH64.ensure((size_t)npw * nch * sizeof(double2));
+    kd.B32.ensure((size_t)npw * nch * sizeof(float2));
+    kd.DBH32.ensure((size_t)npw * nch * sizeof(float2));
+    b64 = kd.B64.p;
+    dbh64 = kd.DBH64.p;
+    b32 = kd.B32.p;
+    dbh32 = kd.DBH32.p;
+  }
   k_build_b<<<blocks_for((size_t)npw, 256), 256, 0, st>>>(npw, nch, lmax, d_k.as<double>(), d_g.as<double>(),
                                                           d_tau.as<double>(), d_ca.as<int>(), d_cp.as<int>(),
                                                           d_cl.as<int>(), d_cm.as<int>(), d_F.as<double>(),
-                                                          1.0 / std::sqrt(omega), kd.B64.as<double2>());
+                                                          1.0 / std::sqrt(omega), static_cast<double2*>(b64));
   PWK_HIP(hipGetLastError());
   k_build_dbh<<<blocks_for((size_t)npw, 256), 256, 0, st>>>(npw, nch, (int)tri_d.size(), d_ti.as<int>(),
-                                                            d_tj.as<int>(), d_td.as<double>(), kd.B64.as<double2>(),
-                                                            kd.DBH64.as<double2>());
+                                                            d_tj.as<int>(), d_td.as<double>(),
+                                                            static_cast<const double2*>(b64),
+                                                            static_cast<double2*>(dbh64));
   PWK_HIP(hipGetLastError());
-  launch_convert(c, kd.B64.p, kd.B32.p, (size_t)npw * nch * 2, true);
-  launch_convert(c, kd.DBH64.p, kd.DBH32.p, (size_t)npw * nch * 2, true);
+  launch_convert(c, b64, b32, (size_t)npw * nch * 2, true);
+  launch_convert(c, dbh64, dbh32, (size_t)npw * nch * 2, true);
   // temporaries are freed at scope exit; make sure the kernels have consumed them
   PWK_HIP(hipStreamSynchronize(st));
 }

@@ -935,6 +935,24 @@ void Solve::setup() {
   chebyshev = p.eigensolver == 1 || (p.eigensolver < 0 && !(npw <= 1500 && heev_supported(npw, nb)));
   kin_all64.ensure((size_t)nk * npw * 8);
   kin_all32.ensure((size_t)nk * npw * 4);
+  // projector channels: (2l + 1) per projector per atom, as the build lays them out
+  nch_all = 0;
+  {
+    std::vector<int> pofs((size_t)cr.nspecies + 1, 0);
+    for (int s = 0; s < cr.nspecies; ++s) pofs[(size_t)s + 1] = pofs[(size_t)s] + nproj[(size_t)s];
+    for (int a = 0; a < cr.natoms; ++a) {
+      const int s = spi[(size_t)a];
+      PWK_REQUIRE(s >= 0 && s < cr.nspecies, PWK_E_INVALID, "atom species out of range");
+      for (int i = 0; i < nproj[(size_t)s]; ++i) nch_all += 2 * pl[(size_t)(pofs[(size_t)s] + i)] + 1;
+    }
+  }
+  {
+    const size_t e = (size_t)nk * npw * std::max(1, nch_all);
+    Ball64.ensure(e * 16);
+    DBHall64.ensure(e * 16);
+    Ball32.ensure(e * 8);
+    DBHall32.ensure(e * 8);
+  }
   // kinetic diagonals of all k-points: one upload and one conversion, then per-k copies
   std::vector<double> tk_all((size_t)nk * npw);
   for (int k = 0; k < nk; ++k) {
@@ -955,32 +973,17 @@ void Solve::setup() {
                            hipMemcpyDeviceToDevice, c.stream));
     kd.have_kin = true;
     const double kv[3] = {kc.x, kc.y, kc.z};
+    // projectors of all k-points in contiguous blocks for the batched GEMMs, built in place
+    NonlocalDest dst;
+    dst.nch = nch_all;
+    const size_t o = (size_t)k * npw * nch_all;
+    dst.b64 = Ball64.as<double2>() + o;
+    dst.dbh64 = DBHall64.as<double2>() + o;
+    dst.b32 = Ball32.as<float2>() + o;
+    dst.dbh32 = DBHall32.as<float2>() + o;
     build_nonlocal_device(c, k, kv, gflat.data(), cr.natoms, tau.data(), spi.data(), cr.nspecies, nproj.data(),
-                          pl.data(), dij.data(), mesh.data(), rr.data(), rab.data(), chi.data(), omega);
-    {
-      // projectors of all k-points in contiguous blocks for the batched GEMMs
-      if (k == 0) {
-        nch_all = kd.nch;
-        const size_t e = (size_t)nk * npw * std::max(1, nch_all);
-        Ball64.ensure(e * 16);
-        DBHall64.ensure(e * 16);
-        Ball32.ensure(e * 8);
-        DBHall32.ensure(e * 8);
-      }
-      PWK_REQUIRE(kd.nch == nch_all, PWK_E_STATE, "projector count differs between k-points");
-      const size_t e = (size_t)npw * nch_all, o = (size_t)k * e;
-      if (nch_all > 0) {
-        PWK_HIP(hipMemcpyAsync(Ball64.as<double2>() + o, kd.B64.p, e * 16, hipMemcpyDeviceToDevice, c.stream));
-        PWK_HIP(hipMemcpyAsync(DBHall64.as<double2>() + o, kd.DBH64.p, e * 16, hipMemcpyDeviceToDevice, c.stream));
-        PWK_HIP(hipMemcpyAsync(Ball32.as<float2>() + o, kd.B32.p, e * 8, hipMemcpyDeviceToDevice, c.stream));
-        PWK_HIP(hipMemcpyAsync(DBHall32.as<float2>() + o, kd.DBH32.p, e * 8, hipMemcpyDeviceToDevice, c.stream));
-        c.sync();
-      }
-      kd.B64.release();
-      kd.DBH64.release();
-      kd.B32.release();
-      kd.DBH32.release();
-    }
+                          pl.data(), dij.data(), mesh.data(), rr.data(), rab.data(), chi.data(), omega, &dst);
+    kd.nch = nch_all;
   }
   // symmetry tables for the density
   PWK_REQUIRE(ops.size() <= (size_t)SYM_MAX_OPS, PWK_E_UNSUPPORTED, "more symmetry operations than the gather kernel takes");
