@@ -953,7 +953,8 @@ void Solve::setup() {
     Ball32.ensure(e * 8);
     DBHall32.ensure(e * 8);
   }
-  // kinetic diagonals of all k-points: one upload and one conversion, then per-k copies
+  // kinetic diagonals of all k-points (the solve reads them from these blocks): one upload, one
+  // conversion
   std::vector<double> tk_all((size_t)nk * npw);
   for (int k = 0; k < nk; ++k) {
     const Vec3 kc = kpts[(size_t)k].cart;
@@ -965,13 +966,6 @@ void Solve::setup() {
   for (int k = 0; k < nk; ++k) {
     const Vec3 kc = kpts[(size_t)k].cart;
     KData& kd = c.k(k);
-    kd.kin64.ensure((size_t)npw * 8);
-    kd.kin32.ensure((size_t)npw * 4);
-    PWK_HIP(hipMemcpyAsync(kd.kin64.p, kin_all64.as<double>() + (size_t)k * npw, (size_t)npw * 8,
-                           hipMemcpyDeviceToDevice, c.stream));
-    PWK_HIP(hipMemcpyAsync(kd.kin32.p, kin_all32.as<float>() + (size_t)k * npw, (size_t)npw * 4,
-                           hipMemcpyDeviceToDevice, c.stream));
-    kd.have_kin = true;
     const double kv[3] = {kc.x, kc.y, kc.z};
     // projectors of all k-points in contiguous blocks for the batched GEMMs, built in place
     NonlocalDest dst;
@@ -1665,35 +1659,43 @@ void Solve::run(double* eig_out, double* rho_out) {
         res.magnetization = std::fabs(n_s[0] - n_s[1]);
       }
       // kinetic and nonlocal expectation values, summed over channels (src/scf/energy.rs:382-510)
-      nl_x.ensure((size_t)npw * nb * 16);
-      expect.ensure((size_t)2 * nb * 8);
-      std::vector<double> e2((size_t)2 * nb);
-      for (int s = 0; s < ns; ++s)
+      // all k-points of a channel at once: batched projector GEMMs, one expectation launch per k,
+      // one copy back
+      nl_x.ensure((size_t)nk * npw * nb * 16);
+      expect.ensure((size_t)2 * nk * nb * 8);
+      if (nch_all > 0) nl_p.ensure((size_t)nk * nch_all * nb * 16);
+      std::vector<double> e2((size_t)2 * nk * nb);
+      for (int s = 0; s < ns; ++s) {
+        const double2* psi0 = V.as<double2>() + (size_t)s * vblk();
+        if (nch_all > 0) {
+          launch_zgemm_mfma_batched(c, nch_all, nb, npw, DBHall64.p, nch_all, (long long)nch_all * npw, psi0, npw,
+                                    (long long)npw * nbs, nl_p.p, nch_all, (long long)nch_all * nb, nk, false,
+                                    PWK_FP64);
+          launch_zgemm_mfma_batched(c, npw, nb, nch_all, Ball64.p, npw, (long long)npw * nch_all, nl_p.p, nch_all,
+                                    (long long)nch_all * nb, nl_x.p, npw, (long long)npw * nb, nk, false, PWK_FP64);
+        } else {
+          PWK_HIP(hipMemsetAsync(nl_x.p, 0, (size_t)nk * npw * nb * 16, c.stream));
+        }
+        for (int k = 0; k < nk; ++k)
+          k_band_expect<<<(unsigned)nb, 256, 0, c.stream>>>(npw, psi0 + (size_t)k * npw * nbs,
+                                                           kin_all64.as<double>() + (size_t)k * npw,
+                                                           nl_x.as<double2>() + (size_t)k * npw * nb,
+                                                           expect.as<double>() + (size_t)2 * k * nb);
+        PWK_HIP(hipGetLastError());
+        PWK_HIP(hipMemcpyAsync(e2.data(), expect.p, e2.size() * 8, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
         for (int k = 0; k < nk; ++k) {
-          KData& kd = c.k(k);
-          const double2* psi = V.as<double2>() + (size_t)s * vblk() + (size_t)k * npw * nbs;
-          if (kd.nch > 0) {
-            nl_p.ensure((size_t)kd.nch * nb * 16);
-            launch_zgemm_mfma(c, kd.nch, nb, npw, DBH_of(k), kd.nch, psi, npw, nl_p.p, kd.nch, false, PWK_FP64);
-            launch_zgemm_mfma(c, npw, nb, kd.nch, B_of(k), npw, nl_p.p, kd.nch, nl_x.p, npw, false, PWK_FP64);
-          } else {
-            PWK_HIP(hipMemsetAsync(nl_x.p, 0, (size_t)npw * nb * 16, c.stream));
-          }
-          k_band_expect<<<(unsigned)nb, 256, 0, c.stream>>>(npw, psi, kd.kin64.as<double>(), nl_x.as<double2>(),
-                                                           expect.as<double>());
-          PWK_HIP(hipGetLastError());
-          PWK_HIP(hipMemcpyAsync(e2.data(), expect.p, e2.size() * 8, hipMemcpyDeviceToHost, c.stream));
-          c.sync();
           double ek = 0.0, enl = 0.0;
           for (int b = 0; b < nb; ++b) {
             const double f = occ[((size_t)s * nk + k) * nb + b];
             if (f == 0.0) continue;
-            ek += f * wk[(size_t)k] * e2[(size_t)2 * b];
-            enl += f * wk[(size_t)k] * e2[(size_t)2 * b + 1];
+            ek += f * wk[(size_t)k] * e2[(size_t)2 * k * nb + 2 * b];
+            enl += f * wk[(size_t)k] * e2[(size_t)2 * k * nb + 2 * b + 1];
           }
           res.e_kinetic += ek;
           res.e_nonlocal += enl;
         }
+      }
       if (eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);
       if (rho_out) {
         PWK_HIP(hipMemcpyAsync(rho_out, rho_tot_new, N * 8, hipMemcpyDeviceToHost, c.stream));
