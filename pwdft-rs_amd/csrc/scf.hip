@@ -582,7 +582,7 @@ struct Solve {
   // (nspin = 2), xc_tot the clamped ρ_total + ρ_core
   DevBuf vloc_g, vloc_r, core_r, rho_in, rho_g_in, rho_new, rho_g_new, rho_xc, exc, vxc, vxc_g, veff_g, cwork;
   DevBuf t_in, m_in, t_out, m_out, xc_tot;
-  DevBuf Hb, W, V, V32, sym_rot, sym_tau, partial, nl_p, nl_x, expect;
+  DevBuf Hb, W, V, V32, sym_rot, sym_tau, nl_p, nl_x, expect;
   PinnedBuf pin_small, pin_partial, pin_rr;
   double* host_small = nullptr;    // pinned: reduction results
   double* host_partial = nullptr;  // pinned: per-block partial sums
@@ -611,12 +611,10 @@ struct Solve {
     a.nterms = (int)terms.size();
     for (size_t i = 0; i < terms.size(); ++i) a.t[i] = terms[i];
     const int nblk = (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (n + RED_NT - 1) / RED_NT));
-    k_dots<<<dim3((unsigned)nblk, (unsigned)terms.size()), RED_NT, 0, c.stream>>>(n, a, partial.as<double>());
+    // the per-block partials are written straight into pinned host memory (no copy launch) and
+    // summed here in block order (the second stage of the fixed-order reduction)
+    k_dots<<<dim3((unsigned)nblk, (unsigned)terms.size()), RED_NT, 0, c.stream>>>(n, a, host_partial);
     PWK_HIP(hipGetLastError());
-    // the per-block partials come back and are summed here in block order (the second stage of
-    // the fixed-order reduction)
-    const size_t np = terms.size() * (size_t)nblk;
-    PWK_HIP(hipMemcpyAsync(host_partial, partial.p, np * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     c.sync();
     for (size_t t = 0; t < terms.size(); ++t) {
       double sum = 0.0;
@@ -868,7 +866,6 @@ void Solve::setup() {
   veff_g.ensure((size_t)ns * N * 16);
   if (ns == 2)
     for (DevBuf* b : {&t_in, &m_in, &t_out, &m_out, &xc_tot}) b->ensure(N * 8);
-  partial.ensure((size_t)MAX_TERMS * RED_BLOCKS * 8);
   pin_small.ensure(64 * sizeof(double));
   pin_partial.ensure((size_t)MAX_TERMS * RED_BLOCKS * sizeof(double));
   host_small = pin_small.as<double>();
@@ -1195,7 +1192,8 @@ void Solve::rr_launch(const double2* Y, const double2* HY) {
   if (rr_small_supported(nbs)) {
     TimedScope ts(c, "scf_rayleigh_ritz");
     const size_t nkm = (size_t)nke * nbs;
-    launch_rr_small(c, npw, nbs, nb, nke, Y, HY, Vc(), s_lam.as<double>(), s_lam.as<double>() + nkm, 1e-14);
+    // Ritz values and residual norms go straight into pinned host memory
+    launch_rr_small(c, npw, nbs, nb, nke, Y, HY, Vc(), rr_pinned, rr_pinned + nkm, 1e-14);
     return;
   }
   rr_general(Y, HY);
@@ -1204,7 +1202,6 @@ void Solve::rr_launch(const double2* Y, const double2* HY) {
 void Solve::rr_collect() {
   if (!rr_small_supported(nbs)) return;  // the general path has already copied its results
   const size_t nkm = (size_t)nke * nbs;
-  PWK_HIP(hipMemcpyAsync(rr_pinned, s_lam.p, 2 * nkm * 8, hipMemcpyDeviceToHost, c.stream));
   c.sync();
   std::memcpy(ev_sub_c(), rr_pinned, nkm * 8);
   std::memcpy(resid_host.data(), rr_pinned + nkm, nkm * 8);
