@@ -142,6 +142,33 @@ def test_served_results_equal_alone(scf_mod, gpu):
         assert m["latency_s"] > 0
 
 
+def test_served_worker_reuses_memory_across_solves(scf_mod, gpu):
+    # one worker runs solves of different sizes back to back, so each solve after the first takes
+    # device and pinned blocks that earlier solves released (with their old contents); every
+    # result equals the same solve run alone
+    decks, params = [], []
+    for a_scale, ecut_ry, kg, prec in ((1.00, 16, 4, "fp32"), (0.98, 12, 4, "fp64"), (1.01, 14, 1, "fp32"),
+                                       (1.00, 16, 4, "fp32")):
+        lat = fcc(10.26 * BOHR * a_scale)
+        decks.append(scf_mod.CrystalDeck(lat, SI_FRAC, [0, 0], [SI_UPF]))
+        params.append(scf_mod.ScfParams(ecut_ev=ecut_ry * RY, kgrid=(kg, kg, kg), n_bands=8, max_iter=60,
+                                        conv_threshold=1e-7, energy_threshold=1e-6, smearing_sigma=0.01 * RY,
+                                        eigensolver=1, precision=prec))
+    alone = [scf_mod.run_scf(gpu, d, p) for d, p in zip(decks, params)]
+    srv = scf_mod.Server(0, n_slots=1)
+    try:
+        served = [srv.wait(srv.submit(d, p)) for d, p in zip(decks, params)]
+    finally:
+        srv.close()
+    for a, m in zip(alone, served):
+        print("alone", a["n_iter"], a["e_total"], "served", m["n_iter"], m["e_total"], flush=True)
+        assert m["converged"] == 1 and a["converged"] == 1
+        assert m["n_iter"] == a["n_iter"]
+        for t in TERMS:
+            assert m[t] == a[t], t
+        assert np.array_equal(m["eigenvalues"], a["eigenvalues"])
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_small_deck_chebyshev_filtered(scf_mod, gpu, small_deck_results, precision):
     # the H·ψ-based eigensolver (FP64 Rayleigh-Ritz; filter H·ψ in `precision`) reaches the
