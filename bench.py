@@ -746,8 +746,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported (SCF solves would shard by "
-              "k-point across GPUs with one all-reduce of the density per iteration; this build does not launch "
-              "it).", file=sys.stderr)
+              "k-point across GPUs with one all-reduce of the density per iteration; that multi-GPU path is not "
+              "implemented).", file=sys.stderr)
         sys.exit(2)
 
     if args.hw_queues:
