@@ -622,12 +622,7 @@ int pwk_scf_tables(const pwk_crystal* crystal, const pwk_scf_params* params, int
                    int* nops) {
   return guarded([&] {
     PWK_REQUIRE(crystal && params && npw && dims && nk && nops, PWK_E_INVALID, "null argument");
-    PWK_REQUIRE(crystal->natoms >= 1 && crystal->species && crystal->species_data && crystal->frac, PWK_E_INVALID,
-                "bad crystal description");
-    for (int a = 0; a < crystal->natoms; ++a)
-      PWK_REQUIRE(crystal->species[a] >= 0 && crystal->species[a] < crystal->nspecies, PWK_E_INVALID,
-                  "atom species out of range");
-    pwk::scf_tables(*crystal, *params, npw, dims, nk, nops);
+    pwk::scf_tables(*crystal, *params, npw, dims, nk, nops);  // validates the request first
   });
 }
 

@@ -362,6 +362,9 @@ void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y
 // scf.hip
 void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
              double* rho);
+// checks every field the solve reads (pointers, species range, radial meshes, scalar parameters);
+// scf_tables and scf_run call it first
+void validate_scf_request(const pwk_crystal& cr, const pwk_scf_params& p);
 void scf_tables(const pwk_crystal& cr, const pwk_scf_params& p, int* npw, int dims[3], int* nk, int* nops);
 
 // Raises a kernel's dynamic-LDS limit once (the attribute call costs host time that

@@ -32,7 +32,9 @@ namespace {
 constexpr int MAXSP = 8;
 constexpr int RED_NT = 256;
 constexpr int RED_BLOCKS = 128;
-constexpr int MAX_TERMS = 48;
+constexpr int MAX_TERMS = 48;   // dot products per reduction launch
+constexpr int MAX_HIST = 32;    // Anderson history entries (mixing_ndim)
+constexpr int MAX_DOTS = MAX_HIST - 1 + (MAX_HIST - 1) * MAX_HIST / 2;  // terms of one Anderson step
 
 // ---------------------------------------------------------------------------------------
 // radial transforms on the grid: V_loc(G) (src/scf/potentials.rs:18-51 with
@@ -421,9 +423,9 @@ __global__ void __launch_bounds__(256) k_band_expect(int npw, const double2* __r
 struct MixArgs {
   int m;
   double beta;
-  double coef[16];
-  const double* in[16];
-  const double* res[16];
+  double coef[MAX_HIST];
+  const double* in[MAX_HIST];
+  const double* res[MAX_HIST];
 };
 
 __global__ void k_anderson_combine(size_t n, MixArgs a, double* __restrict__ out) {
@@ -604,17 +606,21 @@ struct Solve {
 
   unsigned gb(size_t n) const { return blocks_for(n, 256); }
 
-  // Σ terms on the device, results copied to host_small[0..nterms)
+  // Σ terms on the device, results copied to host_small[0..nterms); launches of at most
+  // MAX_TERMS terms each
   void dots(const std::vector<DotTerm>& terms, size_t n) {
-    PWK_REQUIRE(terms.size() <= (size_t)MAX_TERMS, PWK_E_INVALID, "too many reduction terms");
-    DotArgs a{};
-    a.nterms = (int)terms.size();
-    for (size_t i = 0; i < terms.size(); ++i) a.t[i] = terms[i];
+    PWK_REQUIRE(terms.size() <= (size_t)MAX_DOTS, PWK_E_INVALID, "too many reduction terms");
     const int nblk = (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (n + RED_NT - 1) / RED_NT));
     // the per-block partials are written straight into pinned host memory (no copy launch) and
     // summed here in block order (the second stage of the fixed-order reduction)
-    k_dots<<<dim3((unsigned)nblk, (unsigned)terms.size()), RED_NT, 0, c.stream>>>(n, a, host_partial);
-    PWK_HIP(hipGetLastError());
+    for (size_t t0 = 0; t0 < terms.size(); t0 += MAX_TERMS) {
+      const size_t nt = std::min(terms.size() - t0, (size_t)MAX_TERMS);
+      DotArgs a{};
+      a.nterms = (int)nt;
+      for (size_t i = 0; i < nt; ++i) a.t[i] = terms[t0 + i];
+      k_dots<<<dim3((unsigned)nblk, (unsigned)nt), RED_NT, 0, c.stream>>>(n, a, host_partial + t0 * nblk);
+      PWK_HIP(hipGetLastError());
+    }
     c.sync();
     for (size_t t = 0; t < terms.size(); ++t) {
       double sum = 0.0;
@@ -866,8 +872,8 @@ void Solve::setup() {
   veff_g.ensure((size_t)ns * N * 16);
   if (ns == 2)
     for (DevBuf* b : {&t_in, &m_in, &t_out, &m_out, &xc_tot}) b->ensure(N * 8);
-  pin_small.ensure(64 * sizeof(double));
-  pin_partial.ensure((size_t)MAX_TERMS * RED_BLOCKS * sizeof(double));
+  pin_small.ensure((size_t)MAX_DOTS * sizeof(double));
+  pin_partial.ensure((size_t)MAX_DOTS * RED_BLOCKS * sizeof(double));
   host_small = pin_small.as<double>();
   host_partial = pin_partial.as<double>();
   // V_loc(r) for the local energy, ρ_core(r), the start density (IFFT, real part)
@@ -1423,7 +1429,7 @@ void Solve::run(double* eig_out, double* rho_out) {
   const double spin = ns == 2 ? 1.0 : 2.0;  // spin_factor = 2/nspin (src/scf/context.rs:188)
   const double dvol = omega / (double)N;
   const double fourpi_e2_half_omega = 4.0 * kPiC * kE2 * 0.5 * omega;
-  const int ndim = std::max(1, std::min(p.mixing_ndim, 16));
+  const int ndim = std::max(1, p.mixing_ndim);  // at most MAX_HIST (validate_scf_request)
   for (Mixer* m : {&mix_tot, &mix_mag}) {
     m->ndim = ndim;
     m->hslot.clear();
@@ -1720,11 +1726,53 @@ void Solve::run(double* eig_out, double* rho_out) {
 
 void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
              double* rho) {
+  validate_scf_request(cr, p);
   Solve s(c, cr, p, out);
   s.run(eig, rho);
 }
 
+// Checks a crystal and its parameters before any table is built from them (ScfParams::validate,
+// src/scf/mod.rs:166-262, and the crystal checks of run_scf, :354-372): every pointer the solve
+// dereferences, the species range, the radial meshes and the scalar parameters.
+void validate_scf_request(const pwk_crystal& cr, const pwk_scf_params& p) {
+  PWK_REQUIRE(cr.natoms >= 1 && cr.frac && cr.species && cr.species_data, PWK_E_INVALID,
+              "bad crystal description (atoms, positions, species and species data are required)");
+  PWK_REQUIRE(cr.nspecies >= 1 && cr.nspecies <= MAXSP, PWK_E_INVALID, "nspecies must be in 1..8");
+  for (int a = 0; a < cr.natoms; ++a) {
+    PWK_REQUIRE(cr.species[a] >= 0 && cr.species[a] < cr.nspecies, PWK_E_INVALID, "atom species out of range");
+    for (int d = 0; d < 3; ++d) PWK_REQUIRE(std::isfinite(cr.frac[3 * a + d]), PWK_E_INVALID, "non-finite position");
+  }
+  for (int i = 0; i < 9; ++i) PWK_REQUIRE(std::isfinite(cr.lattice[i]), PWK_E_INVALID, "non-finite lattice");
+  for (int s = 0; s < cr.nspecies; ++s) {
+    const pwk_species& sp = cr.species_data[s];
+    PWK_REQUIRE(sp.mesh >= 1 && sp.r && sp.rab && sp.v_local, PWK_E_INVALID, "species without radial mesh");
+    PWK_REQUIRE(sp.nproj >= 0, PWK_E_INVALID, "negative projector count");
+    PWK_REQUIRE(sp.nproj == 0 || (sp.proj_l && sp.beta && sp.dij), PWK_E_INVALID, "projectors without data");
+    for (int i = 0; i < sp.nproj; ++i)
+      PWK_REQUIRE(sp.proj_l[i] >= 0 && sp.proj_l[i] <= 3, PWK_E_UNSUPPORTED, "projector l must be in 0..3");
+    PWK_REQUIRE(std::isfinite(sp.z_valence) && sp.z_valence >= 0.0, PWK_E_INVALID, "bad valence charge");
+  }
+  PWK_REQUIRE(std::isfinite(p.ecut_ev) && p.ecut_ev > 0.0, PWK_E_INVALID, "ecut must be positive");
+  PWK_REQUIRE(p.n_bands >= 1, PWK_E_INVALID, "n_bands must be positive");
+  PWK_REQUIRE(p.max_iter >= 1, PWK_E_INVALID, "max_iter must be positive");
+  PWK_REQUIRE(p.conv_threshold > 0.0, PWK_E_INVALID, "conv_threshold must be positive");
+  PWK_REQUIRE(p.mixing_beta > 0.0 && p.mixing_beta <= 1.0, PWK_E_INVALID, "mixing_beta must be in (0, 1]");
+  PWK_REQUIRE(p.smearing_sigma >= 0.0, PWK_E_INVALID, "smearing_sigma must be non-negative");
+  PWK_REQUIRE(p.precision == PWK_FP64 || p.precision == PWK_FP32 || p.precision == PWK_SPLIT_BF16, PWK_E_INVALID,
+              "unknown precision");
+  PWK_REQUIRE(p.nspin >= 0 && p.nspin <= 2, PWK_E_INVALID, "nspin must be 1 or 2");
+  PWK_REQUIRE(p.mixing_mode == 0 || p.mixing_mode == 1, PWK_E_INVALID, "mixing_mode must be 0 (plain) or 1 (Kerker)");
+  PWK_REQUIRE(p.mixing_ndim <= MAX_HIST, PWK_E_INVALID, "mixing_ndim above the 32 history entries the mixer holds");
+  PWK_REQUIRE(p.eigensolver >= -1 && p.eigensolver <= 1, PWK_E_INVALID, "eigensolver must be -1, 0 or 1");
+  PWK_REQUIRE(p.ecutrho_ratio >= 0 && p.cheb_degree >= 0 && p.cheb_degree <= 64, PWK_E_INVALID,
+              "bad ecutrho_ratio or cheb_degree");
+  for (int d = 0; d < 3; ++d)
+    PWK_REQUIRE(p.kgrid[d] >= 0 && p.kgrid[d] <= 64 && p.fft_dims[d] >= 0 && p.fft_dims[d] <= 1024, PWK_E_INVALID,
+                "bad k-grid or FFT dims");
+}
+
 void scf_tables(const pwk_crystal& cr, const pwk_scf_params& p, int* npw, int dims[3], int* nk, int* nops) {
+  validate_scf_request(cr, p);
   Lat lat;
   for (int i = 0; i < 3; ++i) lat.v[i] = {cr.lattice[3 * i], cr.lattice[3 * i + 1], cr.lattice[3 * i + 2]};
   std::vector<int32_t> miller;

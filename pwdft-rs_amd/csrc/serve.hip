@@ -217,10 +217,13 @@ struct Server {
   }
 };
 
-// Device bytes one solve holds at its peak (dominant terms of scf.hip).
+// Device bytes one solve holds at its peak: the buffers Solve::setup and the iteration allocate
+// (scf.hip), for the eigensolver setup() will choose (dense H batch and its scratch, or the
+// filter blocks and subspace matrices of the Chebyshev path), plus the context's grid and
+// stick workspaces.
 size_t scf_footprint(const pwk_crystal& cr, const pwk_scf_params& p) {
   int npw, dims[3], nk, nops;
-  scf_tables(cr, p, &npw, dims, &nk, &nops);
+  scf_tables(cr, p, &npw, dims, &nk, &nops);  // validates the request
   const size_t n = (size_t)npw, nb = (size_t)std::min(p.n_bands, npw), N = (size_t)dims[0] * dims[1] * dims[2];
   size_t nch = 0;
   for (int a = 0; a < cr.natoms; ++a) {
@@ -228,12 +231,28 @@ size_t scf_footprint(const pwk_crystal& cr, const pwk_scf_params& p) {
     for (int i = 0; i < s.nproj; ++i) nch += (size_t)(2 * s.proj_l[i] + 1);
   }
   const size_t ns = p.nspin == 2 ? 2 : 1;
-  size_t b = (size_t)nk * n * n * 16;                   // dense H batch
-  b += heev_work_doubles(npw, (int)nb, nk) * 8;         // eigensolver scratch
-  b += ns * nk * n * nb * (16 + 8);                     // eigenvectors (FP64 + FP32 copy) per channel
-  b += (size_t)nk * n * nch * 16 * 3;                   // B, D·B^H (FP64) and FP32 copies
-  b += N * (16 * 10 + 8 * 12) * ns + ns * 2 * std::max(1, p.mixing_ndim) * N * 8;  // grids, mixer histories
-  b += (size_t)nb * dims[2] * n * 16 + (64u << 20);     // stick buffers, slack
+  // the same rule as Solve::setup
+  const bool cheb = p.eigensolver == 1 || (p.eigensolver < 0 && !(npw <= 1500 && heev_supported(npw, (int)nb)));
+  const size_t nbs = cheb ? std::min(n, nb + std::max<size_t>(8, nb / 4)) : nb;
+  // both spin channels go through one eigensolver launch on small cubic grids (at most 2·nk blocks)
+  const size_t nke = ns * (size_t)nk;
+  size_t b = 0;
+  b += N * (12 + 3 * 16 + 3 * 8) + ns * N * (4 * 8 + 16) + 4 * N * 16;  // |G|², radial tables, grids
+  if (ns == 2) b += 5 * N * 8;
+  b += 2 * N * 24;                                                        // V(r) slots, V_eff(G)
+  b += ns * 2 * (size_t)std::max(1, p.mixing_ndim) * N * 8;               // mixer histories
+  b += (size_t)nk * n * (12 + nch * 48);                                  // kinetic, B, D·B^H (FP64 + FP32)
+  b += ns * (size_t)nk * n * nbs * (p.precision == PWK_FP64 ? 16 : 24);   // eigenvectors (+ FP32 copy)
+  if (cheb) {
+    b += 8 * nke * n * nbs * 16;                                          // filter and Rayleigh-Ritz blocks
+    b += 8 * nke * nbs * nbs * 16 + heev_work_doubles((int)nbs, (int)nbs, (int)nke) * 8;
+    b += (size_t)nk * std::max<size_t>(1, nch) * nbs * 16 + 3 * nke * n * 16;  // projections, Lanczos
+  } else {
+    b += (size_t)nk * n * n * 16 + heev_work_doubles(npw, (int)nb, nk) * 8;    // dense H batch, scratch
+  }
+  b += (size_t)nk * n * nb * 16 + (size_t)nk * std::max<size_t>(1, nch) * nb * 16;  // final expectations
+  // density and H·psi workspaces: stick buffers of the band batch or per-group density partials
+  b += std::max(nke * nbs * (size_t)dims[2] * n * 16, (size_t)128 * N * 8) + ((size_t)64 << 20);
   return b;
 }
 

@@ -1,0 +1,229 @@
+"""The benchmark decks beyond the small SCF deck, on the device through the C ABI.
+
+  * H·ψ at scale: 64-atom cubic Si, a = 20.52 bohr, 20 Ry (13133 plane waves on 60³),
+    Γ, 144 bands, 2 s + 2 p KB projectors per atom (512 channels); seeded ψ and V_eff.
+    FP64 within 1e-12, FP32 and split-bf16 within 2e-5 of the oracle's FFT-route H·ψ
+    (relative to max |Hψ|).
+  * Batched uneven k: 8-atom cubic Si, 16 Ry (1189 plane waves, 25³), 6×6×6 MP without
+    symmetry (216 k), 24 bands: the FP32-filter solve against the FP64 one per energy term,
+    and the 2×2×2 subset of the same cell against stored oracle energies.
+  * Precision study: 64-atom diamond C, a = 13.48 bohr, atoms displaced by N(0, 0.05 bohr),
+    30 Ry (6859 plane waves on 45³), 2×2×2 MP, 160 bands: FP32-filter SCF against FP64 per
+    term (1e-5 Ry/atom), and the device H·ψ of the displaced cell against the oracle's.
+  * Serving: C and Si requests of 2, 8 and 16 atoms, LDA and LSDA, submitted together; every
+    served result equals the same solve alone.
+
+Reference anchors: src/potential/nonlocal.rs:432-450 (the projector product),
+src/scf/driver.rs:461-472 (the k-loop batched here), src/scf/driver_spin.rs (LSDA).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import C_UPF, GOLDEN, SI_UPF
+
+pytestmark = pytest.mark.gpu
+
+RY = 13.605693122994
+BOHR = 0.529177210903
+TOL_TERM = 1e-5 * RY  # eV per atom
+TERMS = ["e_total", "e_band", "e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_ewald", "e_smearing"]
+
+
+@pytest.fixture(scope="module")
+def scf_mod(pwk_mod):
+    from pwk import scf
+
+    return scf
+
+
+def _diamond(a_bohr, n):
+    from pwk import decks
+
+    return decks.diamond_supercell(a_bohr * BOHR, (n, n, n))
+
+
+def _subset_projectors(oracle_sys, k, dij, keep_proj):
+    """B and D·B^H of the oracle restricted to the projectors `keep_proj` of every atom
+    (block-diagonal in (atom, l, m), src/potential/nonlocal.rs:346-379)."""
+    B, _, meta = oracle_sys.nonlocal_(k)
+    keep = [c for c in range(B.shape[1]) if meta[c][1] in keep_proj]
+    Bk, mk = B[:, keep], meta[keep]
+    DBH = np.zeros((len(keep), B.shape[0]), dtype=np.complex128)
+    for i, ci in enumerate(mk):
+        for j, cj in enumerate(mk):
+            if ci[0] == cj[0] and ci[3] == cj[3] and ci[2] == cj[2] and abs(dij[ci[1], cj[1]]) >= 1e-20:
+                DBH[i] += dij[ci[1], cj[1]] * Bk[:, j].conj()
+    return Bk, DBH
+
+
+def _hpsi_case(gpu, oracle, lat, frac, upf, ecut_ev, nb, kfrac, keep_proj, seed):
+    """Device H·ψ (FP64, FP32, split-bf16) and the oracle's FFT-route H·ψ of one k-point."""
+    from pwk import decks
+
+    u = decks.read_upf(upf)
+    sel = list(keep_proj)
+    species = [dict(l=[u["l"][i] for i in sel], dij=u["dij"][np.ix_(sel, sel)], r=u["r"], rab=u["rab"],
+                    chi=u["chi"][sel])]
+    miller, gvec = decks.basis(lat, ecut_ev)
+    dims = decks.fft_dims(miller)
+    rec = decks.reciprocal(lat)
+    k = np.asarray(kfrac) @ rec
+    gpu.prepare_buffers(dims, decks.grid_g2(lat, dims))
+    gpu.set_basis(miller)
+    gpu.set_nk(1)
+    gpu.set_kinetic(0, decks.kinetic(gvec, k))
+    veff_g = decks.seeded_veff_g(dims, seed + 1)
+    gpu.set_veff(veff_g)
+    nch = gpu.build_nonlocal(0, k, gvec, frac @ lat, [0] * len(frac), species, decks.volume(lat))
+    psi = decks.seeded_psi(((gvec * decks.BOHR_TO_ANG) ** 2).sum(1), nb, seed)
+    out = {p: gpu.hpsi(0, psi, p) for p in ("fp64", "fp32", "split_bf16")}
+    z = u["atomic_number"]
+    s = oracle.System(lat, [0] * len(frac), [z] * len(frac), frac, [upf], ecut_ev)
+    assert s.npw == len(miller) and tuple(s.dims) == tuple(dims)
+    Bk, DBH = _subset_projectors(s, k, species[0]["dij"], keep_proj)
+    assert Bk.shape[1] == nch
+    ref = s.hpsi_with_nl(k, veff_g, Bk, DBH, psi)
+    return out, ref, len(miller), tuple(dims), nch
+
+
+def test_hpsi_si64_60cubed_144_bands_matches_oracle(gpu, oracle):
+    lat, frac = _diamond(20.52 / 2, 2)
+    print("hpsi Si64 60^3 144 bands 512 channels", flush=True)
+    out, ref, npw, dims, nch = _hpsi_case(gpu, oracle, lat, frac, SI_UPF, 20 * RY, 144, [0, 0, 0], (0, 1, 2, 3),
+                                         20261202)
+    assert npw == 13133 and dims == (60, 60, 60) and nch == 512
+    scale = np.abs(ref).max()
+    errs = {p: np.abs(o - ref).max() / scale for p, o in out.items()}
+    print("rel err", errs, flush=True)
+    assert errs["fp64"] < 1e-12
+    assert errs["fp32"] < 2e-5
+    assert errs["split_bf16"] < 2e-5
+
+
+def test_hpsi_c64_displaced_matches_oracle(gpu, oracle):
+    # the precision-study cell: C64 a = 13.48 bohr, atoms displaced by N(0, 0.05 bohr) (seed
+    # 20261202), 30 Ry (6859 PW on 45³), all C.upf projectors (512 channels), 160 bands at the
+    # MP point (½, ½, 0)
+    rng = np.random.default_rng(20261202)
+    lat, frac = _diamond(13.48 / 2, 2)
+    frac = frac + rng.normal(0.0, 0.05 * BOHR, frac.shape) @ np.linalg.inv(lat)
+    from pwk import decks
+
+    nproj = len(decks.read_upf(C_UPF)["l"])
+    print("hpsi C64 45^3 160 bands", flush=True)
+    out, ref, npw, dims, nch = _hpsi_case(gpu, oracle, lat, frac, C_UPF, 30 * RY, 160, [0.5, 0.5, 0.0],
+                                         tuple(range(nproj)), 7)
+    assert npw == 6859 and dims == (45, 45, 45) and nch == 512
+    scale = np.abs(ref).max()
+    errs = {p: np.abs(o - ref).max() / scale for p, o in out.items()}
+    print("rel err", errs, flush=True)
+    assert errs["fp64"] < 1e-12
+    assert errs["fp32"] < 2e-5
+    assert errs["split_bf16"] < 2e-5
+
+
+def _si8_deck(scf_mod):
+    lat, frac = _diamond(10.26, 1)
+    return scf_mod.CrystalDeck(lat, frac, [0] * 8, [SI_UPF])
+
+
+def test_si8_216k_fp32_filter_matches_fp64(scf_mod, gpu):
+    deck = _si8_deck(scf_mod)
+    kw = dict(ecut_ev=16 * RY, kgrid=(6, 6, 6), use_symmetry=False, n_bands=24, max_iter=60, conv_threshold=1e-7,
+              energy_threshold=1e-7, mixing_ndim=8, smearing_sigma=0.01 * RY, eigensolver=1)
+    print("si8 216k fp64", flush=True)
+    r64 = scf_mod.run_scf(gpu, deck, scf_mod.ScfParams(precision="fp64", **kw))
+    print("si8 216k fp32", flush=True)
+    r32 = scf_mod.run_scf(gpu, deck, scf_mod.ScfParams(precision="fp32", **kw))
+    print("fp64", r64["n_iter"], r64["e_total"], r64["seconds"], "fp32", r32["n_iter"], r32["e_total"],
+          r32["seconds"], flush=True)
+    for r in (r64, r32):
+        assert r["converged"] == 1 and r["nk"] == 216 and r["npw"] == 1189 and tuple(r["dims"]) == (25, 25, 25)
+        assert abs(r["integrated_charge"] - 32.0) < 1e-8
+    for t in TERMS:
+        assert abs(r32[t] - r64[t]) / 8 < TOL_TERM, t
+
+
+@pytest.mark.parametrize("eigensolver,precision", [(0, "fp64"), (1, "fp32")])
+def test_si8_k222_matches_stored_oracle(scf_mod, gpu, eigensolver, precision):
+    # the 2×2×2 subset of the batched-k cell against the oracle's solve of the same deck
+    # (stored by tests/golden/oracle/make_oracle_decks.py: its dense FP64 eigensolve at 1189 PW
+    # takes about two minutes on eight host cores)
+    with open(os.path.join(GOLDEN, "oracle", "oracle_decks.json")) as f:
+        ro = json.load(f)["si8_cubic_16ry_k222"]
+    deck = _si8_deck(scf_mod)
+    p = scf_mod.ScfParams(ecut_ev=16 * RY, kgrid=(2, 2, 2), use_symmetry=False, n_bands=24, max_iter=60,
+                          conv_threshold=1e-8, energy_threshold=1e-8 * RY, mixing_ndim=8, smearing_sigma=0.01 * RY,
+                          eigensolver=eigensolver, precision=precision)
+    print("si8 k222", eigensolver, precision, flush=True)
+    r = scf_mod.run_scf(gpu, deck, p)
+    print({t: (r[t], ro[t]) for t in TERMS}, r["n_iter"], ro["n_iter"], flush=True)
+    assert r["converged"] == 1 and r["nk"] == ro["nk"] == 8 and r["npw"] == ro["npw"]
+    for t in TERMS:
+        assert abs(r[t] - ro[t]) / 8 < TOL_TERM, t
+    assert abs(r["fermi"] - ro["fermi"]) < 1e-4
+    assert np.abs(r["eigenvalues"][0][:16] - np.array(ro["eigenvalues_k0"])[:16]).max() < 1e-4
+
+
+def test_c64_precision_study_per_term(scf_mod, gpu):
+    rng = np.random.default_rng(20261202)
+    lat, frac = _diamond(13.48 / 2, 2)
+    frac = frac + rng.normal(0.0, 0.05 * BOHR, frac.shape) @ np.linalg.inv(lat)
+    deck = scf_mod.CrystalDeck(lat, frac, [0] * 64, [C_UPF])
+    kw = dict(ecut_ev=30 * RY, kgrid=(2, 2, 2), use_symmetry=True, n_bands=160, max_iter=60, conv_threshold=1e-6,
+              energy_threshold=1e-6, mixing_ndim=8, smearing_sigma=0.01 * RY, eigensolver=1)
+    print("c64 fp64", flush=True)
+    r64 = scf_mod.run_scf(gpu, deck, scf_mod.ScfParams(precision="fp64", **kw))
+    print("c64 fp32", flush=True)
+    r32 = scf_mod.run_scf(gpu, deck, scf_mod.ScfParams(precision="fp32", **kw))
+    print("fp64", r64["n_iter"], r64["e_total"], r64["seconds"], "fp32", r32["n_iter"], r32["e_total"],
+          r32["seconds"], flush=True)
+    for r in (r64, r32):
+        assert r["converged"] == 1 and r["npw"] == 6859 and tuple(r["dims"]) == (45, 45, 45) and r["nk"] == 8
+        assert abs(r["integrated_charge"] - 256.0) < 1e-8
+    for t in TERMS:
+        assert abs(r32[t] - r64[t]) / 64 < TOL_TERM, t
+
+
+def test_served_mixed_cells_equal_alone(scf_mod, gpu):
+    # C and Si, 2/8/16 atoms, LDA and LSDA (starting moments 0.4-0.8), strained and jittered,
+    # submitted at once to a 4-slot server; each served result equals the same solve alone
+    from pwk import decks
+
+    rng = np.random.default_rng(11)
+    reqs = []
+    for el, natoms, nspin, mag, ecut_ry, kg in (("C", 2, 1, 0.0, 20, (3, 3, 3)), ("Si", 8, 2, 0.6, 12, (2, 2, 2)),
+                                                ("C", 8, 2, 0.4, 16, (1, 1, 1)), ("Si", 16, 2, 0.8, 12, (1, 1, 1)),
+                                                ("Si", 2, 2, 0.5, 14, (3, 3, 3)), ("C", 16, 1, 0.0, 14, (1, 1, 1))):
+        a = {"Si": 10.26, "C": 6.74}[el] * (1.0 + rng.uniform(-0.03, 0.03)) * BOHR
+        if natoms == 2:
+            lat, frac = decks.fcc_primitive(a)
+        else:
+            lat, frac = decks.diamond_supercell(a, (1, 1, 2) if natoms == 16 else (1, 1, 1))
+        frac = frac + rng.normal(0.0, 0.03 * BOHR, frac.shape) @ np.linalg.inv(lat)
+        deck = scf_mod.CrystalDeck(lat, frac, [0] * natoms, [SI_UPF if el == "Si" else C_UPF])
+        p = scf_mod.ScfParams(ecut_ev=ecut_ry * RY, kgrid=kg, use_symmetry=True, n_bands=int(0.6 * 4 * natoms) + 2,
+                              max_iter=100, conv_threshold=1e-6, energy_threshold=1e-6, smearing_sigma=0.01 * RY,
+                              precision="fp32", eigensolver=1, nspin=nspin, starting_magnetization={0: mag})
+        reqs.append((deck, p, (el, natoms, nspin)))
+    print("served mixed cells: alone", flush=True)
+    alone = [scf_mod.run_scf(gpu, d, p) for d, p, _ in reqs]
+    print("served mixed cells: served", flush=True)
+    srv = scf_mod.Server(0, n_slots=4)
+    try:
+        tickets = [srv.submit(d, p) for d, p, _ in reqs]
+        served = [srv.wait(t) for t in tickets]
+    finally:
+        srv.close()
+    for (_, _, info), a, m in zip(reqs, alone, served):
+        print(info, "alone", a["converged"], a["n_iter"], a["e_total"], a["magnetization"], "served", m["n_iter"],
+              m["e_total"], flush=True)
+        assert a["converged"] == 1 and m["converged"] == 1
+        assert m["n_iter"] == a["n_iter"]
+        for t in TERMS:
+            assert m[t] == a[t], t
+        assert np.array_equal(m["eigenvalues"], a["eigenvalues"])
+        assert m["magnetization"] == a["magnetization"]

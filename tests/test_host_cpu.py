@@ -129,3 +129,58 @@ def test_seeded_inputs_are_reproducible():
     vr = np.fft.ifftn(v.reshape(6, 6, 6)) * 216
     assert np.abs(vr.imag).max() < 1e-12
     assert np.abs(vr.real).max() <= decks.RY_TO_EV
+
+
+def _si_deck_and_params():
+    from pwk import decks, scf
+
+    lat, frac = decks.fcc_primitive(5.431)
+    deck = scf.CrystalDeck(lat, frac, [0, 0], [SI_UPF])
+    return deck, scf.ScfParams(ecut_ev=100.0, kgrid=(2, 2, 2))
+
+
+def _tables_rc(pwk_mod, deck, prm):
+    from pwk import scf
+
+    npw, nk, nops = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    dims = (ctypes.c_int * 3)()
+    L = scf._lib()
+    return L.pwk_scf_tables(ctypes.byref(deck.c), ctypes.byref(prm), ctypes.byref(npw), dims, ctypes.byref(nk),
+                            ctypes.byref(nops))
+
+
+def test_malformed_requests_are_rejected_before_any_table(pwk_mod):
+    # the checks every SCF entry point (pwk_scf_tables, pwk_scf_run, pwk_server_submit) runs first:
+    # a bad species index, missing arrays or out-of-range parameters give PWK_E_INVALID instead of
+    # host reads out of bounds
+    from pwk import scf
+
+    deck, p = _si_deck_and_params()
+    assert _tables_rc(pwk_mod, deck, scf._params(p)) == 0
+    bad_species = np.array([0, 3], dtype=np.int32)
+    deck.c.species = bad_species.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    assert _tables_rc(pwk_mod, deck, scf._params(p)) == 1
+    assert b"species out of range" in pwk_mod.lib().pwk_last_error()
+    deck, p = _si_deck_and_params()
+    deck.c.frac = ctypes.POINTER(ctypes.c_double)()
+    assert _tables_rc(pwk_mod, deck, scf._params(p)) == 1
+    deck, p = _si_deck_and_params()
+    deck.c.nspecies = 9
+    assert _tables_rc(pwk_mod, deck, scf._params(p)) == 1
+    deck, p = _si_deck_and_params()
+    deck._sp[0].r = ctypes.POINTER(ctypes.c_double)()
+    assert _tables_rc(pwk_mod, deck, scf._params(p)) == 1
+    assert b"radial mesh" in pwk_mod.lib().pwk_last_error()
+    for field, value in (("ecut_ev", -1.0), ("n_bands", 0), ("mixing_beta", 0.0), ("mixing_beta", 1.5),
+                         ("nspin", 3), ("mixing_ndim", 33), ("conv_threshold", 0.0), ("eigensolver", 2),
+                         ("smearing_sigma", -0.1), ("max_iter", 0)):
+        deck, p = _si_deck_and_params()
+        prm = scf._params(p)
+        setattr(prm, field, value)
+        assert _tables_rc(pwk_mod, deck, prm) == 1, field
+    prm = scf._params(p)
+    prm.precision = 7
+    assert _tables_rc(pwk_mod, deck, prm) == 1
+    prm = scf._params(p)
+    prm.mixing_ndim = 32  # the largest history the mixer holds
+    assert _tables_rc(pwk_mod, deck, prm) == 0
