@@ -63,10 +63,12 @@ int pwk_ctx_destroy(pwk_ctx* ctx);
 int pwk_ctx_sync(pwk_ctx* ctx);
 /* The context's hipStream_t, for callers that enqueue their own work. */
 void* pwk_ctx_stream(pwk_ctx* ctx);
-/* No counterpart in the reference. On (the default), H·psi on cubic grids of at
+/* No counterpart in the reference. 1 (the default): H·psi on cubic grids of at
  * most 24 points per side runs as one fused launch per call (whole grid in LDS,
- * projectors included); off selects the z-stick kernels and projector GEMMs used
- * for every other grid. Both compute the same product; the switch exists for
+ * projectors included), and each Chebyshev filter pass of pwk_scf_run runs as one
+ * launch for all bands and all steps of the recurrence; 2: the fused H·psi with
+ * one launch per filter step; 0: the z-stick kernels and projector GEMMs used for
+ * every other grid. All compute the same products; the switch exists for
  * comparisons. */
 int pwk_ctx_set_fused_hpsi(pwk_ctx* ctx, int on);
 /* No counterpart in the reference. On, pwk_scf_run captures each filter pass of

@@ -130,7 +130,12 @@ int pwk_ctx_sync(pwk_ctx* ctx) {
 void* pwk_ctx_stream(pwk_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
 
 int pwk_ctx_set_fused_hpsi(pwk_ctx* ctx, int on) {
-  return guarded([&] { ctx_of(ctx).small_hpsi = on != 0; });
+  return guarded([&] {
+    PWK_REQUIRE(on >= 0 && on <= 2, PWK_E_INVALID, "mode must be 0, 1 or 2");
+    Context& c = ctx_of(ctx);
+    c.small_hpsi = on != 0;
+    c.fused_filter = on == 1;
+  });
 }
 
 int pwk_ctx_set_graphs(pwk_ctx* ctx, int on) {

@@ -246,6 +246,7 @@ struct Context {
   DevBuf vr64, vr32;   // real N, V(r) in plane-major order [i3][i1][i2]
   DevBuf vrn64, vrn32; // real N, V(r) in grid order [i1][i2][i3]
   bool small_hpsi = true;  // fused one-launch H·ψ on small cubic grids (hpsi_small.hip)
+  bool fused_filter = true;  // with small_hpsi: the whole Chebyshev filter pass in one launch
   bool graphs = false;     // replay captured filter passes in the SCF (scf.hip), opt-in
 
   std::vector<KData> kd;
@@ -309,6 +310,13 @@ bool hpsi_small_supported(const Context& c, int precision, int nch);
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
                        int nch, const void* B, const void* DBH, const double* coef = nullptr,
                        const void* X = nullptr, int nkv = 0);
+// the whole degree-`degree` Chebyshev filter of nbands bands in one launch on small cubic grids:
+// Yout (FP64) = p(H)·V (FP64) with the recurrence in `precision`; coef (device) holds (α, c, β)
+// per step and band block, coef[(i·nblocks + kb)·3 + j]
+bool cheb_filter_small_supported(const Context& c, int precision, int nch);
+void launch_cheb_filter_small(Context& c, int nbands, int nbk, const void* V, void* Yout, int precision,
+                              const void* kin, int nch, const void* B, const void* DBH, const double* coef, int degree,
+                              int nkv = 0);
 // ρ += Σ_b fw_b |ψ_b(r)|² on small cubic grids (fw on the device)
 bool density_small_supported(const Context& c, int precision);
 void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision,
@@ -316,10 +324,8 @@ void launch_density_small(Context& c, int nb, const void* psi, const double* fw_
 // dense.hip
 void launch_dense_h(Context& c, int ik, int with_nl, void* h);
 void launch_dense_h_batched(Context& c, int nk, const double* kin, int nch, const void* B, const void* DBH, void* h);
-// gemm.hip: complex C = alpha·op(A)·B + beta·C, column-major, op = N only
-void launch_zgemm(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
-                  bool accumulate, int precision);
-// gemm_mfma.hip: the same product on the matrix cores (FP32, FP64 or 3-term split-bf16)
+// gemm_mfma.hip: complex C (+)= A·B, column-major, on the matrix cores (FP32, FP64 or 3-term
+// split-bf16)
 void launch_zgemm_mfma(Context& c, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C,
                        int ldc, bool accumulate, int precision);
 // `batch` independent products with element strides sA, sB, sC between them

@@ -214,6 +214,136 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
   }
 }
 
+// The whole Chebyshev filter pass of one band in one workgroup: the `degree` steps of the scaled
+// three-term recurrence
+//   Y_1 = α_0 (H − c_0) Y_0,   Y_{i+1} = α_i (H − c_i) Y_i − β_i Y_{i−1}
+// with Y_0 the band's FP64 vector of V rounded to T and the result written back as FP64 to Yout.
+// The recurrence is independent per band, so the two live vectors stay in LDS between the steps
+// (cur and prv below; the new vector overwrites prv element by element, since each thread owns
+// its plane waves) and V(r) of the thread's x-line stays in registers for the whole pass. Each
+// step is the H·ψ of k_hpsi_small with its Chebyshev epilogue, in the same order of operations,
+// so the pass gives what `degree` launches of k_hpsi_small give. coef holds (α, c, β) per step and
+// band block: coef[(i·nblk + kb)·3 + j], nblk = gridDim.x / nbk.
+template <typename T, int N>
+__global__ void __launch_bounds__(SMALL_NT) k_cheb_filter_small(const double2* __restrict__ V,
+                                                                double2* __restrict__ Yout, const T* __restrict__ kin,
+                                                                const T* __restrict__ vr,
+                                                                const int* __restrict__ g2fft, int npw, int nbk,
+                                                                int nch, const typename V2s<T>::t* __restrict__ B,
+                                                                const typename V2s<T>::t* __restrict__ DBH, T inv_n,
+                                                                const double* __restrict__ coef, int degree, int nkv) {
+  using V2 = typename V2s<T>::t;
+  extern __shared__ unsigned char smem_raw[];
+  V2* G = reinterpret_cast<V2*>(smem_raw);
+  V2* proj = G + N * N * N;
+  V2* cur = proj + nch;
+  V2* prv = cur + npw;
+  const int nblk = (int)gridDim.x / nbk;
+  // XCD-aware band order, as in k_hpsi_small
+  const int q = (int)gridDim.x >> 3, r = (int)gridDim.x & 7, xcd = (int)blockIdx.x & 7;
+  const int b = xcd * q + min(xcd, r) + ((int)blockIdx.x >> 3), kb = b / nbk;
+  const int k = nkv > 0 ? kb % nkv : kb;
+  if (nkv > 0) vr += (size_t)(kb / nkv) * N * N * N;
+  const T* t = kin + (size_t)k * npw;
+  const V2* bk = B + (size_t)k * npw * nch;
+  const V2* dk = DBH + (size_t)k * nch * npw;
+  constexpr int NW = SMALL_NT / 64, CPW = 4;
+  constexpr bool PRE_V = N * N <= SMALL_NT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T vline[PRE_V ? N : 1];
+  if constexpr (PRE_V) {
+    if (threadIdx.x < N * N) {
+#pragma unroll
+      for (int f = 0; f < N; ++f) vline[f] = vr[threadIdx.x + f * N * N];
+    }
+  }
+  const double2* v0 = V + (size_t)b * npw;
+  for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
+    const double2 x = v0[g];
+    cur[g].x = (T)x.x;
+    cur[g].y = (T)x.y;
+  }
+  for (int step = 0; step < degree; ++step) {
+    for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
+      G[i].x = 0;
+      G[i].y = 0;
+    }
+    lds_barrier();
+    for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = cur[g];
+    // projections P_c = Σ_G (D·B^H)[c, G] Y(G): one wave per channel, up to four side by side
+    for (int c0 = wave; c0 < nch; c0 += NW * CPW) {
+      T ar[CPW], ai[CPW];
+#pragma unroll
+      for (int j = 0; j < CPW; ++j) ar[j] = ai[j] = 0;
+      for (int g = lane; g < npw; g += 64) {
+        const V2 v = cur[g];
+        V2 d[CPW];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j)
+          if (c0 + j * NW < nch) d[j] = dk[(size_t)g * nch + c0 + j * NW];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j)
+          if (c0 + j * NW < nch) {
+            ar[j] += d[j].x * v.x - d[j].y * v.y;
+            ai[j] += d[j].x * v.y + d[j].y * v.x;
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < CPW; ++j) {
+        for (int off = 32; off > 0; off >>= 1) {
+          ar[j] += __shfl_xor(ar[j], off, 64);
+          ai[j] += __shfl_xor(ai[j], off, 64);
+        }
+        if (lane == 0 && c0 + j * NW < nch) {
+          proj[c0 + j * NW].x = ar[j];
+          proj[c0 + j * NW].y = ai[j];
+        }
+      }
+    }
+    lds_barrier();
+    lds_lines<T, N, 2, 1>(G, nullptr);
+    lds_barrier();
+    lds_lines<T, N, 1, 1>(G, nullptr);
+    lds_barrier();
+    if constexpr (PRE_V)
+      lds_xlines_v<T, N, 1>(G, vline);
+    else
+      lds_lines<T, N, 0, 1>(G, vr);
+    lds_barrier();
+    lds_lines<T, N, 1, -1>(G, nullptr);
+    lds_barrier();
+    lds_lines<T, N, 2, -1>(G, nullptr);
+    lds_barrier();
+    const double* cf = coef + ((size_t)step * nblk + kb) * 3;
+    const T alpha = (T)cf[0], cshift = (T)cf[1], beta = (T)cf[2];
+    for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
+      const V2 v = cur[g], w = G[g2fft[g]];
+      const T tk = t[g];
+      T ox = tk * v.x + w.x * inv_n, oy = tk * v.y + w.y * inv_n;
+      for (int ch = 0; ch < nch; ++ch) {
+        const V2 bb = bk[(size_t)ch * npw + g], pc = proj[ch];
+        ox += bb.x * pc.x - bb.y * pc.y;
+        oy += bb.x * pc.y + bb.y * pc.x;
+      }
+      V2 o;
+      o.x = alpha * (ox - cshift * v.x);
+      o.y = alpha * (oy - cshift * v.y);
+      if (step > 0) {
+        const V2 xv = prv[g];
+        o.x -= beta * xv.x;
+        o.y -= beta * xv.y;
+      }
+      prv[g] = o;
+    }
+    lds_barrier();
+    V2* s = cur;
+    cur = prv;
+    prv = s;
+  }
+  double2* y = Yout + (size_t)b * npw;
+  for (int g = threadIdx.x; g < npw; g += SMALL_NT) y[g] = make_double2((double)cur[g].x, (double)cur[g].y);
+}
+
 // Band density on small cubic grids: workgroup g takes bands [g·gsize, (g+1)·gsize), runs the
 // inverse transform of each in LDS and keeps Σ_b fw_b |ψ_b(r)|² of its x-lines in FP64
 // registers (bands with fw < 1e-15 skipped, src/scf/density.rs:14,72); the partial sums of
@@ -344,6 +474,36 @@ void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi
 }
 
 template <typename T>
+size_t filter_lds_bytes(int n, int nch, int npw) {
+  return ((size_t)n * n * n + (size_t)nch + 2 * (size_t)npw) * sizeof(typename V2s<T>::t);
+}
+
+template <typename T>
+void launch_filter_small_t(Context& c, int nbands, int nbk, const void* V, void* Yout, const void* kin, int nch,
+                           const void* B, const void* DBH, const double* coef, int degree, int nkv) {
+  using V2 = typename V2s<T>::t;
+  const int n = c.dims[0];
+  const size_t sh = filter_lds_bytes<T>(n, nch, c.npw);
+  const T* vr = std::is_same<T, double>::value ? (const T*)c.vrn64.p : (const T*)c.vrn32.p;
+  const T inv_n = (T)(1.0 / (double)c.N);
+  TimedScope ts(c, "cheb_filter_small");
+  switch (n) {
+#define PWK_CASE(L)                                                                                            \
+  case L:                                                                                                      \
+    ensure_lds_limit((const void*)k_cheb_filter_small<T, L>, sh);                                             \
+    k_cheb_filter_small<T, L><<<(unsigned)nbands, SMALL_NT, sh, c.stream>>>(                                   \
+        (const double2*)V, (double2*)Yout, (const T*)kin, vr, c.g2fft.as<int>(), c.npw, nbk, nch, (const V2*)B, \
+        (const V2*)DBH, inv_n, coef, degree, nkv);                                                             \
+    break;
+    PWK_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "grid has no fused small-grid kernel");
+  }
+  PWK_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_density_small_t(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, bool overwrite) {
   using V2 = typename V2s<T>::t;
   const int n = c.dims[0];
@@ -396,6 +556,32 @@ bool hpsi_small_supported(const Context& c, int precision, int nch) {
   const size_t sh = precision == PWK_FP64 ? small_lds_bytes<double>(c.dims[0], nch)
                                           : small_lds_bytes<float>(c.dims[0], nch);
   return sh <= kLdsMax;
+}
+
+bool cheb_filter_small_supported(const Context& c, int precision, int nch) {
+  if (precision == PWK_SPLIT_BF16 || !hpsi_small_supported(c, precision, nch)) return false;
+  const size_t sh = precision == PWK_FP64 ? filter_lds_bytes<double>(c.dims[0], nch, c.npw)
+                                          : filter_lds_bytes<float>(c.dims[0], nch, c.npw);
+  return sh <= kLdsMax;
+}
+
+void launch_cheb_filter_small(Context& c, int nbands, int nbk, const void* V, void* Yout, int precision,
+                              const void* kin, int nch, const void* B, const void* DBH, const double* coef, int degree,
+                              int nkv) {
+  c.check_basis();
+  PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
+  PWK_REQUIRE(nbands > 0 && nbk > 0 && nbands % nbk == 0, PWK_E_INVALID, "band counts must be positive");
+  PWK_REQUIRE(degree >= 1, PWK_E_INVALID, "filter degree must be positive");
+  PWK_REQUIRE(cheb_filter_small_supported(c, precision, nch), PWK_E_UNSUPPORTED,
+              "grid too large for the fused filter kernel");
+  if (nkv > 0) {
+    const size_t slots = ((size_t)(nbands / nbk) + nkv - 1) / nkv;
+    PWK_REQUIRE(c.vrn64.bytes >= slots * c.N * sizeof(double), PWK_E_STATE, "V_eff slots not set");
+  }
+  if (precision == PWK_FP64)
+    launch_filter_small_t<double>(c, nbands, nbk, V, Yout, kin, nch, B, DBH, coef, degree, nkv);
+  else
+    launch_filter_small_t<float>(c, nbands, nbk, V, Yout, kin, nch, B, DBH, coef, degree, nkv);
 }
 
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,

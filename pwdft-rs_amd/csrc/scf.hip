@@ -1339,7 +1339,18 @@ double Solve::chebyshev_pass(int fprec) {
   // depend only on the channel and the filter precision, so on the fused small-grid path the
   // sequence is captured once per (channel, precision) as a HIP graph and replayed (one launch
   // call instead of fourteen); the coefficients are uploaded before each replay
+  const bool fused_filter = c.fused_filter && cheb_filter_small_supported(c, fprec, nch_all);
   auto body = [&]() {
+    if (fused_filter) {
+      // the whole recurrence of every band in one launch, result straight into Y64
+      const bool f64 = fprec == PWK_FP64;
+      launch_cheb_filter_small(c, nke * nbs, nbs, Vc(), Y64.p, fprec, f64 ? kin_all64.p : kin_all32.p, nch_all,
+                               f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, coef.as<double>(), degree,
+                               fold ? nk : 0);
+      hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
+      rr_launch(Y64.as<double2>(), HY64.as<double2>());
+      return;
+    }
     void* buf[3] = {F0.p, F1.p, F2.p};
     if (fprec == PWK_FP64)
       PWK_HIP(hipMemcpyAsync(buf[0], Vc(), blk * 16, hipMemcpyDeviceToDevice, c.stream));

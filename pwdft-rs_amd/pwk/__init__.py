@@ -178,7 +178,9 @@ class GpuAccelerator:
         _check(lib().pwk_ctx_set_graphs(self.h, int(on)))
 
     def set_fused_hpsi(self, on=True):
-        """Fused one-launch H·psi on small cubic grids (default on) or the z-stick path."""
+        """Fused one-launch H·psi on small cubic grids: True / 1 (default) also runs each
+        Chebyshev filter pass as one launch, 2 keeps one launch per filter step, False / 0
+        selects the z-stick path."""
         _check(lib().pwk_ctx_set_fused_hpsi(self.h, int(on)))
 
     # ---- kernel timing -----------------------------------------------------------

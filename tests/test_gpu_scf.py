@@ -210,6 +210,30 @@ def test_filtered_solve_fused_and_stick_hpsi_agree(scf_mod, gpu, small_deck_resu
         assert abs(rf[t] - r64[t]) < TOL_TERM * 2, t
 
 
+@pytest.mark.parametrize("precision,nspin", [("fp64", 1), ("fp32", 1), ("fp32", 2)])
+def test_one_launch_filter_pass_equals_step_launches(scf_mod, gpu, precision, nspin):
+    # each filter pass as one launch (all bands, all ten steps, the recurrence vectors in LDS)
+    # against one fused H·psi launch per step: the same arithmetic in the same order
+    lat = fcc(10.26 * BOHR)
+    p = scf_mod.ScfParams(ecut_ev=12 * RY, kgrid=(4, 4, 4), n_bands=8, max_iter=100, conv_threshold=1e-8,
+                          energy_threshold=1e-8 * RY, mixing_ndim=8, smearing_sigma=0.01 * RY, eigensolver=1,
+                          precision=precision, nspin=nspin, starting_magnetization={0: 0.5})
+    print("scf one-launch filter vs step launches", precision, nspin, flush=True)
+    try:
+        gpu.set_fused_hpsi(1)
+        _, r1 = _run(scf_mod, gpu, lat, p)
+        gpu.set_fused_hpsi(2)
+        _, r2 = _run(scf_mod, gpu, lat, p)
+    finally:
+        gpu.set_fused_hpsi(1)
+    print("one launch", r1["n_iter"], r1["e_total"], "step launches", r2["n_iter"], r2["e_total"], flush=True)
+    assert r1["converged"] == 1 and r2["converged"] == 1
+    assert r1["n_iter"] == r2["n_iter"]
+    for t in TERMS:
+        assert abs(r1[t] - r2[t]) < 1e-9, t
+    assert np.abs(r1["eigenvalues"] - r2["eigenvalues"]).max() < 1e-9
+
+
 @pytest.mark.parametrize("eigensolver,precision,bands", [(0, "fp64", 20), (1, "fp32", 20), (1, "fp64", 30)])
 def test_orthorhombic_cell_matches_oracle(scf_mod, gpu, oracle, eigensolver, precision, bands):
     # an 8-atom Si cell strained to a = (1.12, 0.94, 1.00)·5.431 Å at 8 Ry: a (24, 18, 18) grid,
