@@ -17,7 +17,11 @@ density (setup included in its time).
     solves overlap on the device continuously; filtered eigensolver with FP32
     filter H·ψ and density FFTs, FP64 Rayleigh-Ritz, energies, Ewald and
     mixing. Its total energy is checked against the FP64 form: |ΔE| per atom
-    must stay below 1e-5 Ry.
+    must stay below 1e-5 Ry, else bench.py exits with status 1 after its line;
+  * speed-up decomposition (--no-decomposition skips it): the filtered
+    eigensolver in FP64 served at the same slot count, and the FP64 and mixed
+    filtered forms one solve at a time, so the ratio to the plain FP64 form
+    splits into algorithm, concurrency and precision factors.
 
 H·ψ at scale (reported beside, --no-hpsi skips it): 64-atom cubic Si supercell
 a = 20.52 bohr, ecutwfc 20 Ry -> 13133 plane waves on 60³, Γ, 144 bands;
@@ -48,6 +52,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "pwdft-rs_amd"))
 
 METRIC = "SCF solves/s + p50/p99 latency, mixed-prec vs FP64 GPU at |ΔE|<1e-5 Ry/atom"
+ACCURACY_BAR = 1e-5  # Ry per atom, |E_total| of the served form against the FP64 form
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
 FP64_MFMA_PEAK_TFLOPS = 78.6
@@ -55,6 +60,31 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+# The CPU baseline runs on the cores this process may use, at most 16: the share of host cores
+# that goes with one GPU on the benchmark machines (the OpenMP loops of the oracle are the
+# reference's rayon loops over k-points and grid points, src/scf/driver.rs:461-472).
+CPU_SHARE = 16
+
+
+def host_cpu():
+    """Threads for the CPU baseline and what the host reports (nproc, usable cores, model)."""
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": max(1, min(CPU_SHARE, usable)), "nproc": nproc, "usable_cores": usable, "model": model}
 
 
 def build_workload(acc, np, torch, nb, seed=20261202):
@@ -201,7 +231,8 @@ def hpsi_bench(args, acc):
         try:
             import pyoracle
 
-            threads = pyoracle.set_threads(args.cpu_threads or min(16, os.cpu_count() or 1))
+            hc = host_cpu()
+            threads = pyoracle.set_threads(args.cpu_threads or hc["threads"])
             log(f"cpu baseline: oracle FFT-route FP64 H·psi, {nb} bands, {threads} threads")
             upf = os.path.join(ROOT, "tests", "golden", "pseudo", "Si.upf")
             s = pyoracle.System(w["lat"], [0] * len(w["frac"]), [14] * len(w["frac"]), w["frac"], [upf], w["ecut"])
@@ -226,7 +257,7 @@ def hpsi_bench(args, acc):
                 err_cpu = float(np.abs(out64.cpu().numpy() - hcpu).max() / np.abs(hcpu).max())
             cpu = {"value": 1.0 / tc, "unit": "Hpsi/s", "cores": threads, "kind": "port",
                    "sample": f"full workload: {nb} bands, one FP64 FFT-route H·psi (oracle), {tc:.2f} s wall",
-                   "gpu_fp64_vs_cpu_max_rel_err": err_cpu}
+                   "gpu_fp64_vs_cpu_max_rel_err": err_cpu, "host": hc}
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline failed: {e!r}")
             cpu = None
@@ -277,74 +308,82 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
-def scf_algorithmic(name, t, precision, bands=None, nkp=None):
+def scf_algorithmic(name, t, precision, degree=10):
     """Algorithmic bytes (HBM-bound kernels) or flops (MFMA kernels) per launch of the SCF
-    kernels on the small deck. The filtered eigensolver applies H·psi to B = nk·nbs bands
-    at once (nbs = nb plus buffer bands); s_c / s_r are the complex / real sizes."""
+    kernels on the small deck, at the launch shape of the solve: the filtered eigensolver works
+    on B = nk·nbs bands at once (nbs = nb plus buffer bands), each launch covers all k-points.
+    s_c / s_r are the complex / real sizes of the kernel's arithmetic."""
     n, nk, N, nch = t["npw"], t["nk"], t["N"], t["nch"]
     nbs = min(n, t["nb"] + max(8, t["nb"] // 4))
-    B = nk * nbs if bands is None else bands  # bands per launch
-    nk = nk if nkp is None else nkp  # k-points per launch
+    B = nk * nbs
     sc, sr = (16, 8) if precision == "fp64" else (8, 4)
-    if name == "hpsi_zinv":
-        return "hbm", B * (n + N) * sc
-    if name == "hpsi_plane":
-        return "hbm", B * 2 * N * sc + N * sr
-    if name == "hpsi_zfwd":
-        return "hbm", B * (N + 2 * n) * sc
-    if name in ("hpsi_nl_project", "hpsi_nl_expand"):
-        return "flop", 8.0 * n * nch * B
-    if name == "scf_density":
-        return "hbm", B * (2 * N * sc + n * sc)
+    if name == "cheb_filter_small":
+        # the whole filter pass of every band: V (FP64) in and the filtered block (FP64) out once
+        # per band; kinetic diagonal and projector blocks once per k-point; V(r) once
+        return "hbm", B * 2 * n * 16 + nk * (n * sr + 2 * n * nch * sc) + N * sr
     if name == "hpsi_small":
         # fused H·psi: psi in and H·psi out once per band; kinetic diagonal and projector blocks
-        # once per k-point and V(r) once per launch (L2-resident across the bands)
+        # once per k-point and V(r) once per launch
         return "hbm", B * 2 * n * sc + nk * (n * sr + 2 * n * nch * sc) + N * sr
+    if name == "scf_density":
+        return "hbm", B * (2 * N * sc + n * sc)
     return None, None
 
 
-def local_kernel_probe(deck, p, nbands, precision, nch=0, reps=60):
-    """Per-launch device time of the H·psi kernels at the served solve's launch shape
-    (nbands = nk·nbs bands on the deck's grid, nch seeded projector channels), with the stream
-    kept busy so the HIP events bracket the kernels rather than host launch gaps."""
-    import numpy as np
-    import torch
-    import pwk
-    from pwk import decks, scf
+def fft_flops_filter(t, degree=10):
+    """Floating-point work of one filter launch (all bands, all steps): two 3D FFTs of
+    5·N·log2 N flops each, the V(r) product, the projections and their expansion (8·npw·nch
+    each) and the recurrence's elementwise update, per band and step."""
+    n, nk, N, nch = t["npw"], t["nk"], t["N"], t["nch"]
+    nbs = min(n, t["nb"] + max(8, t["nb"] // 4))
+    per = 2 * 5 * N * math.log2(N) + 2 * N + 16 * n * nch + 12 * n
+    return degree * nk * nbs * per
 
-    lat = deck.lattice
-    miller = scf.basis(lat, p.ecut_ev)
-    dims = scf.fft_dims(miller)
-    acc = pwk.GpuAccelerator(0)
-    acc.prepare_buffers(dims, decks.grid_g2(lat, dims))
-    acc.set_basis(miller)
-    acc.set_nk(1)
-    gvec = miller @ decks.reciprocal(lat)
-    acc.set_kinetic(0, decks.kinetic(gvec, [0.0, 0.0, 0.0]))
-    acc.set_veff(decks.seeded_veff_g(dims, 7))
-    if nch:
-        rng = np.random.default_rng(5)
-        Bm = (rng.standard_normal((len(miller), nch)) + 1j * rng.standard_normal((len(miller), nch))) * 0.1
-        acc.set_nonlocal(0, Bm, Bm.conj().T.copy())
-    dt = torch.complex64 if precision != "fp64" else torch.complex128
-    psi = torch.randn(nbands, len(miller), dtype=dt, device="cuda:0")
-    out = torch.empty_like(psi)
-    torch.cuda.synchronize()
-    for _ in range(5):
-        acc.hpsi_dev(0, nbands, psi.data_ptr(), out.data_ptr(), precision)
-    acc.sync()
-    acc.profile_reset()
-    acc.profile(True)
-    # a bounded spin kernel at the head of the context's stream lets every launch and event of the
-    # probe queue up behind it, so the events time the kernels back to back
-    with torch.cuda.stream(torch.cuda.ExternalStream(acc.stream)):
-        torch.cuda._sleep(20_000_000)
-    for _ in range(reps):
-        acc.hpsi_dev(0, nbands, psi.data_ptr(), out.data_ptr(), precision)
-    prof = acc.profile_read()
-    acc.profile(False)
-    acc.close()
-    return {k: 1e3 * ms / max(n, 1) for k, (ms, n) in prof.items()}
+
+def served_run(scf, deck, p, slots, warmup, steps):
+    """A closed loop of `slots` clients on a fresh server with `slots` workers: each client
+    submits its next request when the previous one returns (a step is one request per client).
+    Returns solves/s over the timed steps, latencies and the results."""
+    import threading
+
+    srv = scf.Server(0, n_slots=slots)
+    lock = threading.Lock()
+    lat, results, errors = [], [], []
+
+    def client(n, record):
+        try:
+            for _ in range(n):
+                r = srv.wait(srv.submit(deck, p))
+                if record:
+                    with lock:
+                        lat.append(r["latency_s"])
+                        results.append(r)
+        except Exception as e:  # surfaced after the join
+            with lock:
+                errors.append(repr(e))
+
+    def run_clients(n, record):
+        th = [threading.Thread(target=client, args=(n, record)) for _ in range(slots)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+
+    try:
+        if warmup:
+            run_clients(warmup, False)
+        t0 = time.perf_counter()
+        run_clients(steps, True)
+        wall = time.perf_counter() - t0
+    finally:
+        srv.close()
+    if errors:
+        raise SystemExit(f"served solve failed: {errors[0]}")
+    if any(r["converged"] != 1 for r in results):
+        raise SystemExit("a served solve did not converge")
+    return {"solves_per_s": len(results) / wall, "wall_s": wall, "solves": len(results),
+            "p50_latency_ms": 1e3 * percentile(lat, 50), "p99_latency_ms": 1e3 * percentile(lat, 99),
+            "n_iter": results[0]["n_iter"], "results": results}
 
 
 def scf_bench(args):
@@ -370,61 +409,56 @@ def scf_bench(args):
         if r64["converged"] != 1:
             raise SystemExit("FP64 solve did not converge")
     wall64 = time.perf_counter() - t0
+    fp64_rate = args.fp64_solves / wall64
     # kernel profile of one FP64 solve (library HIP-event timing)
     acc.profile_reset()
     acc.profile(True)
-    rprof = scf.run_scf(acc, deck64, p64)
+    scf.run_scf(acc, deck64, p64)
     prof = acc.profile_read()
     acc.profile(False)
     acc.profile_reset()
     kern = {k: {"avg_us": 1e3 * ms / max(n, 1), "launches": n, "total_ms": ms} for k, (ms, n) in prof.items()}
 
-    # served mixed form: a closed loop of --slots clients, each submitting its next request as soon
-    # as its previous one returns, so the server stays full without a per-step tail; a step is one
-    # request per client
-    import threading
-
-    srv = scf.Server(0, n_slots=args.slots)
+    # served mixed form (the headline): --slots closed-loop clients
     log(f"scf: served {args.scf_precision} form, {args.slots} slots, {args.warmup} warmup + {args.steps} steps")
-    lock = threading.Lock()
-    lat_mx, results, errors = [], [], []
-
-    def client(n, record):
-        try:
-            for _ in range(n):
-                r = srv.wait(srv.submit(deckmx, pmx))
-                if record:
-                    with lock:
-                        lat_mx.append(r["latency_s"])
-                        results.append(r)
-        except Exception as e:  # surfaced after the join
-            with lock:
-                errors.append(repr(e))
-
-    def run_clients(n, record):
-        th = [threading.Thread(target=client, args=(n, record)) for _ in range(args.slots)]
-        for t in th:
-            t.start()
-        for t in th:
-            t.join()
-
-    if args.warmup:
-        run_clients(args.warmup, False)
-    t0 = time.perf_counter()
-    run_clients(args.steps, True)
-    wall_mx = time.perf_counter() - t0
-    srv.close()
-    if errors:
-        raise SystemExit(f"served solve failed: {errors[0]}")
-    n_mx = len(results)
-    if any(r["converged"] != 1 for r in results):
-        raise SystemExit("a served solve did not converge")
+    mx = served_run(scf, deckmx, pmx, args.slots, args.warmup, args.steps)
+    results = mx.pop("results")
     de = max(abs(r["e_total"] - r64["e_total"]) for r in results) / RY / natoms
     terms = ["e_band", "e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_ewald", "e_smearing"]
     de_terms = {t: max(abs(r[t] - r64[t]) for r in results) / RY / natoms for t in terms}
 
-    # kernel profile of one solve of the served form, on this context (library HIP-event timing
-    # on the context's stream); the roofline is for its dominant single kernel
+    # where the speed-up comes from: the same filtered eigensolver in FP64 served at the same
+    # slot count (precision), and the FP64 and mixed filtered forms one solve at a time
+    # (algorithm; concurrency is the rest)
+    decomposition = None
+    if not args.no_decomposition and args.scf_eigensolver == "chebyshev":
+        _, pf64 = small_deck("fp64", 1)
+        log("scf: decomposition runs (filtered FP64 served; filtered FP64 and mixed one at a time)")
+        f64_srv = served_run(scf, deckmx, pf64, args.slots, args.warmup, args.steps)
+        f64_srv.pop("results")
+        one = max(4, args.slots // 2)
+        f64_one = served_run(scf, deckmx, pf64, 1, 2, one)
+        f64_one.pop("results")
+        mx_one = served_run(scf, deckmx, pmx, 1, 2, one)
+        mx_one.pop("results")
+        decomposition = {
+            "fp64_dense_one_at_a_time": round(fp64_rate, 2),
+            "fp64_filtered_one_at_a_time": round(f64_one["solves_per_s"], 2),
+            "mixed_filtered_one_at_a_time": round(mx_one["solves_per_s"], 2),
+            "fp64_filtered_served": round(f64_srv["solves_per_s"], 2),
+            "mixed_filtered_served": round(mx["solves_per_s"], 2),
+            "algorithm_x": round(f64_one["solves_per_s"] / fp64_rate, 3),
+            "concurrency_x_fp64": round(f64_srv["solves_per_s"] / f64_one["solves_per_s"], 3),
+            "precision_x_served": round(mx["solves_per_s"] / f64_srv["solves_per_s"], 3),
+            "precision_x_one_at_a_time": round(mx_one["solves_per_s"] / f64_one["solves_per_s"], 3),
+            "fp64_filtered_served_latency_ms": [round(f64_srv["p50_latency_ms"], 2),
+                                                round(f64_srv["p99_latency_ms"], 2)],
+            "n_iter": {"fp64_dense": r64["n_iter"], "fp64_filtered": f64_srv["n_iter"], "mixed_filtered": mx["n_iter"]},
+        }
+
+    # kernel profile of one solve of the served form on this context (library HIP-event timing
+    # on the context's stream, the stream the kernels run on); the roofline is for its dominant
+    # single kernel, at the launch shape of the solve
     acc.profile_reset()
     acc.profile(True)
     scf.run_scf(acc, deckmx, pmx)
@@ -435,48 +469,31 @@ def scf_bench(args):
     nch = sum(2 * l + 1 for l in deck64.upfs[0]["l"]) * natoms
     info = {"npw": tab["npw"], "nb": min(8, tab["npw"]), "nk": tab["nk"], "N": int(np.prod(tab["dims"])),
             "nch": nch}
-    single = {k: v for k, v in kern_mx.items() if k.startswith("hpsi_") or k in ("dense_eigh", "scf_density")}
+    nbs = min(info["npw"], info["nb"] + max(8, info["nb"] // 4))
+    shape = {"bands_per_launch": info["nk"] * nbs, "nk": info["nk"], "npw": info["npw"], "grid": list(tab["dims"]),
+             "channels": nch, "precision": args.scf_precision}
+    single = {k: v for k, v in kern_mx.items() if k in ("cheb_filter_small", "hpsi_small", "scf_density")}
     name, kv = max(single.items(), key=lambda x: x[1]["total_ms"])
     bound, amount = scf_algorithmic(name, info, args.scf_precision)
     dur = kv["avg_us"] * 1e-6
-    probe_us = None
-    if name in ("hpsi_zinv", "hpsi_plane", "hpsi_zfwd", "hpsi_small"):
-        nbs = min(info["npw"], info["nb"] + max(8, info["nb"] // 4))
-        probe = local_kernel_probe(deckmx, pmx, info["nk"] * nbs, args.scf_precision,
-                                   nch=info["nch"] if name == "hpsi_small" else 0)
-        probe_us = probe.get(name)
-        if probe_us:
-            dur = probe_us * 1e-6
-            if name == "hpsi_small":  # the probe's bands all sit at one k-point
-                bound, amount = scf_algorithmic(name, info, args.scf_precision, bands=info["nk"] * nbs, nkp=1)
-    if bound == "flop":
-        peak = FP64_MFMA_PEAK_TFLOPS if args.scf_precision == "fp64" else FP32_MFMA_PEAK_TFLOPS
-        roofline = {"kernel": name, "bound": "mfma", "achieved": round(amount / dur / 1e12, 5), "peak": peak,
-                    "unit": "TFLOP/s", "frac": round(amount / dur / 1e12 / peak, 6), "traffic": None,
-                    "algorithmic_per_launch": amount}
-    elif bound == "hbm":
-        roofline = {"kernel": name, "bound": "hbm", "achieved": round(amount / dur / 1e9, 3), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(amount / dur / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
-                    "algorithmic_per_launch": amount}
-    else:
-        roofline = {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
-                    "traffic": None}
-    roofline["avg_us"] = round(dur * 1e6, 2)
-    roofline["avg_us_in_solve_events"] = round(kv["avg_us"], 2)
-    roofline["launches_per_solve"] = kv["launches"]
-    if probe_us:
-        roofline["note"] = ("dominant single kernel of a served-form solve; avg_us from back-to-back launches at "
-                            "the solve's launch shape (the in-solve event figure includes host launch gaps)")
-    else:
-        roofline["note"] = ("dominant single kernel of a served-form solve (8 k-points x 16 bands per batched "
-                            "launch); avg_us from HIP events inside the solve, where launches this small are "
-                            "latency-bound")
+    roofline = {"kernel": name, "bound": "hbm", "achieved": round(amount / dur / 1e9, 3), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(amount / dur / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
+                "algorithmic_per_launch": amount, "avg_us": round(kv["avg_us"], 2),
+                "launches_per_solve": kv["launches"], "shape": shape}
+    if name == "cheb_filter_small":
+        fl = fft_flops_filter(info)
+        roofline["flops_per_launch"] = fl
+        roofline["fp32_tflops"] = round(fl / dur / 1e12, 3)
+        roofline["note"] = ("dominant kernel of a served-form solve: one launch per filter pass (all k-points x "
+                            "bands, ten recurrence steps, grids and vectors in LDS); avg_us from HIP events on the "
+                            "solve's stream; LDS- and latency-bound, so its HBM fraction is small by design")
     pmc_file = os.path.join(ROOT, "profiles", "scf_si2_12ry_pmc.json")
     if os.path.exists(pmc_file):
         try:
             t = json.load(open(pmc_file)).get("kernels", {}).get(name)
-            if t and t.get("precision") == args.scf_precision:
+            if t and t.get("shape") == shape:
                 roofline["traffic"] = t.get("hbm_bytes_per_launch")
+                roofline["traffic_source"] = "profiles/scf_si2_12ry_pmc.json (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
         except Exception:
             pass
 
@@ -485,7 +502,8 @@ def scf_bench(args):
         try:
             import pyoracle
 
-            threads = pyoracle.set_threads(args.cpu_threads or min(16, os.cpu_count() or 1))
+            hc = host_cpu()
+            threads = pyoracle.set_threads(args.cpu_threads or hc["threads"])
             s = pyoracle.System(deck64.lattice, [0, 0], [14, 14], SI_FRAC,
                                 [os.path.join(ROOT, "tests", "golden", "pseudo", "Si.upf")], 12 * RY)
             k, w = pyoracle.kpoints(deck64.lattice, [14, 14], SI_FRAC, [4, 4, 4], (0, 0, 0), mode=2)
@@ -497,23 +515,24 @@ def scf_bench(args):
             cpu = {"value": 1.0 / tc, "unit": "SCF solves/s", "cores": threads, "kind": "port",
                    "sample": f"one full solve of the same deck by the FP64 oracle ({int(ro['n_iter'])} iterations, "
                              f"{tc:.2f} s wall)",
-                   "gpu_fp64_vs_cpu_abs_dE_eV": abs(r64["e_total"] - ro["e_total"])}
+                   "host": hc, "gpu_fp64_vs_cpu_abs_dE_eV": abs(r64["e_total"] - ro["e_total"])}
         except Exception as e:
             log(f"cpu baseline failed: {e!r}")
     acc.close()
     return {
-        "value": n_mx / wall_mx,
-        "ms_per_step": 1e3 * wall_mx / max(args.steps, 1),
-        "p50_latency_ms": 1e3 * percentile(lat_mx, 50),
-        "p99_latency_ms": 1e3 * percentile(lat_mx, 99),
-        "solves_timed": n_mx,
-        "fp64_gpu": {"solves_per_s": args.fp64_solves / wall64, "p50_latency_ms": 1e3 * percentile(lat64, 50),
+        "value": mx["solves_per_s"],
+        "ms_per_step": 1e3 * mx["wall_s"] / max(args.steps, 1),
+        "p50_latency_ms": mx["p50_latency_ms"],
+        "p99_latency_ms": mx["p99_latency_ms"],
+        "solves_timed": mx["solves"],
+        "fp64_gpu": {"solves_per_s": fp64_rate, "p50_latency_ms": 1e3 * percentile(lat64, 50),
                      "p99_latency_ms": 1e3 * percentile(lat64, 99), "n_iter": r64["n_iter"],
                      "e_total_eV": r64["e_total"]},
-        "speedup_vs_fp64_gpu": (n_mx / wall_mx) / (args.fp64_solves / wall64),
+        "speedup_vs_fp64_gpu": mx["solves_per_s"] / fp64_rate,
+        "speedup_decomposition": decomposition,
         "abs_dE_Ry_per_atom_vs_fp64": de,
         "abs_dE_terms_Ry_per_atom": de_terms,
-        "n_iter": results[0]["n_iter"],
+        "n_iter": mx["n_iter"],
         "kernels_fp64_solve_us": {k: {"avg_us": round(v["avg_us"], 2), "launches": v["launches"]}
                                   for k, v in kern.items()},
         "kernels_served_solve_us": {k: {"avg_us": round(v["avg_us"], 2), "launches": v["launches"]}
@@ -731,7 +750,10 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "split_bf16"], help="H·psi precision")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 H·psi comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (0: all, max 16)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads for the CPU baseline (0: the usable host cores, at most 16)")
+    ap.add_argument("--no-decomposition", action="store_true",
+                    help="skip the runs that split the speed-up into algorithm, concurrency and precision")
     ap.add_argument("--stream", type=int, default=48, help="requests in the serving stream (0: skip it)")
     ap.add_argument("--stream-rate", type=float, default=40.0, help="Poisson arrival rate of the stream, 1/s")
     ap.add_argument("--stream-seed", type=int, default=20261202)
@@ -811,7 +833,10 @@ def main():
         "p99_latency_ms": sc["p99_latency_ms"],
         "fp64_gpu": sc["fp64_gpu"],
         "speedup_vs_fp64_gpu": sc["speedup_vs_fp64_gpu"],
+        "speedup_decomposition": sc["speedup_decomposition"],
         "abs_dE_Ry_per_atom_vs_fp64": sc["abs_dE_Ry_per_atom_vs_fp64"],
+        "accuracy_bar_Ry_per_atom": ACCURACY_BAR,
+        "accuracy_ok": sc["abs_dE_Ry_per_atom_vs_fp64"] < ACCURACY_BAR,
         "abs_dE_terms_Ry_per_atom": sc["abs_dE_terms_Ry_per_atom"],
         "kernels_fp64_solve_us": sc["kernels_fp64_solve_us"],
         "kernels_served_solve_us": sc["kernels_served_solve_us"],
@@ -824,6 +849,10 @@ def main():
     if args.precision_study:
         out["precision_study"] = precision_study()
     print(json.dumps(out), flush=True)
+    if not out["accuracy_ok"]:
+        log(f"served form misses the accuracy bar: |dE| = {sc['abs_dE_Ry_per_atom_vs_fp64']:.3e} Ry/atom "
+            f"against the FP64 form (bar {ACCURACY_BAR:g})")
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -217,6 +217,55 @@ int pwk_density_accumulate(pwk_ctx* ctx, int nb, const double* psi, const double
 int pwk_dev_density_accumulate(pwk_ctx* ctx, int nb, const void* psi, const double* fw_host, void* rho,
                                int precision);
 
+/* ---- k × band batches ----------------------------------------------------------------
+ * The reference applies its Hamiltonian and accumulates the density k-point by
+ * k-point in a rayon loop (src/scf/driver.rs:461-472; compute_density takes one
+ * eigenvector matrix per k-point, src/scf/density.rs:41-46). A batch names, per
+ * entry, a k-point slot (pwk_set_nk) and a band count; the entries' bands are
+ * packed back to back (entry 0's bands first), each band npw coefficients as in
+ * pwk_hpsi. Band counts may differ between entries; npw is the context's basis,
+ * which the reference shares between all k-points (src/basis.rs:54). */
+typedef struct pwk_kbatch {
+  int nentries;
+  const int32_t* ik;      /* k-point slot of each entry */
+  const int32_t* nbands;  /* bands of each entry (>= 1) */
+} pwk_kbatch;
+
+/* H·ψ of every band of a batch (semantic of pwk_hpsi per entry). On cubic grids of
+ * at most 24 points per side all bands go through one fused launch that reads each
+ * band's k-point from a table (the k-points must share one projector channel
+ * count); elsewhere one H·ψ per entry. */
+int pwk_hpsi_batch(pwk_ctx* ctx, const pwk_kbatch* batch, const double* psi, double* hpsi, int precision);
+int pwk_dev_hpsi_batch(pwk_ctx* ctx, const pwk_kbatch* batch, const void* psi, void* hpsi, int precision);
+/* compute_density's accumulation over a batch (src/scf/density.rs:59-97): rho +=
+ * Σ_b fw_b |IFFT_unnorm(scatter ψ_b)|² with fw = occupation × k-point weight per band
+ * (bands with fw < 1e-15 skipped), all bands in one density pass. The rescaling to
+ * the electron count (density.rs:99-108) stays with the caller, as in
+ * pwk_density_accumulate. */
+int pwk_density_accumulate_batch(pwk_ctx* ctx, const pwk_kbatch* batch, const double* psi, const double* fw,
+                                 double* rho, int precision);
+int pwk_dev_density_accumulate_batch(pwk_ctx* ctx, const pwk_kbatch* batch, const void* psi, const double* fw_host,
+                                     void* rho, int precision);
+
+/* diagonalize_subspace (src/eigensolver/dense.rs:183-260), the warm-start
+ * Rayleigh-Ritz of diagonalize_dispatch (src/scf/driver.rs:125-184), on nk dense
+ * Hamiltonians at once, FP64: per k, HV = H·V_prev, H_sub = V_prev^H·HV (lower
+ * triangle read), H_sub = U·diag(λ)·U^H, V = V_prev·U; when the largest residual
+ * ||H v_i − λ_i v_i||₂ of a k-point exceeds tol (<= 0: WFRX_RESIDUAL_TOL = 1e-6,
+ * dense.rs:37) that k-point gets the full diagonalize_lowest instead
+ * (dense.rs:250-254), and with v_prev NULL every k-point does (dense.rs:205-207).
+ *   h: nk column-major n×n complex128 matrices (not modified); v_prev, v: nk
+ *   column-major n×nb blocks (must not overlap); w: nk×nb eigenvalues ascending;
+ *   used_full (nk ints: 1 where the full solve ran) and max_residual (nk doubles:
+ *   the gate's value, 0 without v_prev) may be NULL.
+ * The device form synchronises the context's stream (the gate is read on the host,
+ * as the reference decides it per call); used_full and max_residual are host
+ * arrays in both forms. */
+int pwk_subspace_rr(pwk_ctx* ctx, int n, int nb, int nk, const double* h, const double* v_prev, double tol, double* w,
+                    double* v, int* used_full, double* max_residual);
+int pwk_dev_subspace_rr(pwk_ctx* ctx, int n, int nb, int nk, const void* h, const void* v_prev, double tol, void* w,
+                        void* v, int* used_full, double* max_residual);
+
 /* ---- Integer tables (host; bit-exact with the reference) -------------------------- */
 
 /* G-sphere basis |G|² ≤ ecut/(ħ²/2m), Miller indices in n1 → n2 → n3 loop
@@ -265,6 +314,23 @@ typedef struct pwk_species {
 } pwk_species;
 
 #define PWK_MAX_SPECIES 8
+
+/* UPF v2 reader: UpfPseudoPotential::load_from_path (src/pseudopotential/upf/mod.rs:98-109)
+ * and parse_upf_body (src/pseudopotential/upf/parse.rs:20-140), host only. Converts to
+ * the units above (Bohr -> Å, Ry -> eV, χ to Å^-1/2, 4πr²ρ_atom to e/Å, ρ_core to e/Å³).
+ * A file without a <UPF or <PP_HEADER marker, a missing or malformed attribute or
+ * block, or a block whose number count differs from the header's is PWK_E_INVALID
+ * (message in pwk_last_error). PP_RHOATOM absent reads as zeros; rho_core is set only
+ * with the core-correction flag (zeros when its block is absent). */
+typedef struct pwk_upf pwk_upf;
+int pwk_upf_load(const char* path, pwk_upf** out);
+int pwk_upf_parse(const char* text, size_t len, pwk_upf** out);
+/* Fills a pwk_species whose arrays point into the parsed file's arrays, valid until
+ * pwk_upf_free. */
+int pwk_upf_species(const pwk_upf* upf, pwk_species* out);
+/* Element symbol (NUL-terminated, at most 3 characters) and l_max of the header. */
+int pwk_upf_info(const pwk_upf* upf, char symbol[4], int* l_max);
+int pwk_upf_free(pwk_upf* upf);
 
 /* Crystal (src/crystal.rs:83-121): lattice rows a, b, c in Å, fractional
  * positions, a species index per atom into species_data (at most

@@ -335,6 +335,7 @@ int pwk_set_basis(pwk_ctx* ctx, int npw, const int32_t* miller) {
     Context& c = ctx_of(ctx);
     pwk::setup_basis(c, npw, miller);
     c.kd.clear();
+    c.pack_valid = false;
   });
 }
 
@@ -343,6 +344,7 @@ int pwk_set_nk(pwk_ctx* ctx, int nk) {
     Context& c = ctx_of(ctx);
     PWK_REQUIRE(nk >= 0, PWK_E_INVALID, "nk must be non-negative");
     c.kd.resize((size_t)nk);
+    c.pack_valid = false;
   });
 }
 
@@ -351,6 +353,7 @@ int pwk_set_kinetic(pwk_ctx* ctx, int ik, const double* t) {
     Context& c = ctx_of(ctx);
     c.check_basis();
     pwk::KData& kd = c.k(ik);
+    c.pack_valid = false;
     kd.kin64.ensure((size_t)c.npw * 8);
     kd.kin32.ensure((size_t)c.npw * 4);
     PWK_HIP(hipMemcpyAsync(kd.kin64.p, t, (size_t)c.npw * 8, hipMemcpyHostToDevice, c.stream));
@@ -385,6 +388,7 @@ int pwk_set_nonlocal(pwk_ctx* ctx, int ik, int nch, const double* B, const doubl
     c.check_basis();
     pwk::KData& kd = c.k(ik);
     PWK_REQUIRE(nch >= 0, PWK_E_INVALID, "nch must be non-negative");
+    c.pack_valid = false;
     kd.nch = nch;
     if (nch == 0) return;
     size_t n = (size_t)c.npw * nch;
@@ -530,6 +534,109 @@ int pwk_dev_density_accumulate(pwk_ctx* ctx, int nb, const void* psi, const doub
     Context& c = ctx_of(ctx);
     check_precision(precision);
     pwk::launch_density(c, nb, psi, fw_host, rho, precision);
+  });
+}
+
+// ---- k × band batches and the warm-start Rayleigh-Ritz ----------------------------------
+namespace {
+size_t kbatch_bands(const pwk_kbatch* b) {
+  PWK_REQUIRE(b && b->nentries >= 1 && b->ik && b->nbands, PWK_E_INVALID, "empty or null k-point batch");
+  size_t total = 0;
+  for (int e = 0; e < b->nentries; ++e) {
+    PWK_REQUIRE(b->nbands[e] >= 1, PWK_E_INVALID, "every batch entry needs at least one band");
+    total += (size_t)b->nbands[e];
+  }
+  return total;
+}
+}  // namespace
+
+int pwk_hpsi_batch(pwk_ctx* ctx, const pwk_kbatch* batch, const double* psi, double* hpsi, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_basis();
+    const size_t total = kbatch_bands(batch);
+    PWK_REQUIRE(psi && hpsi, PWK_E_INVALID, "bad psi/hpsi");
+    const size_t nd = (size_t)2 * c.npw * total;
+    DevBuf in, out;
+    to_device(c, in, psi, nd, precision);
+    out.ensure(nd * rsize(precision));
+    pwk::launch_hpsi_batch(c, batch->nentries, batch->ik, batch->nbands, in.p, out.p, precision);
+    to_host(c, hpsi, out, nd, precision);
+  });
+}
+
+int pwk_dev_hpsi_batch(pwk_ctx* ctx, const pwk_kbatch* batch, const void* psi, void* hpsi, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    kbatch_bands(batch);
+    PWK_REQUIRE(psi && hpsi, PWK_E_INVALID, "bad psi/hpsi");
+    pwk::launch_hpsi_batch(c, batch->nentries, batch->ik, batch->nbands, psi, hpsi, precision);
+  });
+}
+
+int pwk_density_accumulate_batch(pwk_ctx* ctx, const pwk_kbatch* batch, const double* psi, const double* fw,
+                                 double* rho, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_basis();
+    const size_t total = kbatch_bands(batch);
+    PWK_REQUIRE(psi && fw && rho, PWK_E_INVALID, "bad density arguments");
+    const size_t nd = (size_t)2 * c.npw * total;
+    DevBuf in, r;
+    to_device(c, in, psi, nd, precision);
+    r.ensure(c.N * 8);
+    PWK_HIP(hipMemcpyAsync(r.p, rho, c.N * 8, hipMemcpyHostToDevice, c.stream));
+    pwk::launch_density_batch(c, batch->nentries, batch->ik, batch->nbands, in.p, fw, r.p, precision);
+    to_host(c, rho, r, c.N, PWK_FP64);
+  });
+}
+
+int pwk_dev_density_accumulate_batch(pwk_ctx* ctx, const pwk_kbatch* batch, const void* psi, const double* fw_host,
+                                     void* rho, int precision) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    kbatch_bands(batch);
+    PWK_REQUIRE(psi && fw_host && rho, PWK_E_INVALID, "bad density arguments");
+    pwk::launch_density_batch(c, batch->nentries, batch->ik, batch->nbands, psi, fw_host, rho, precision);
+  });
+}
+
+int pwk_dev_subspace_rr(pwk_ctx* ctx, int n, int nb, int nk, const void* h, const void* v_prev, double tol, void* w,
+                        void* v, int* used_full, double* max_residual) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_REQUIRE(h && w && v, PWK_E_INVALID, "null subspace buffer");
+    PWK_REQUIRE(n >= 1 && nb >= 1 && nb <= n && nk >= 1, PWK_E_INVALID, "bad subspace shape");
+    PWK_REQUIRE(v != v_prev, PWK_E_INVALID, "v and v_prev must not overlap");
+    pwk::subspace_rr(c, n, nb, nk, h, v_prev, tol > 0.0 ? tol : 1e-6, (double*)w, v, used_full, max_residual);
+  });
+}
+
+int pwk_subspace_rr(pwk_ctx* ctx, int n, int nb, int nk, const double* h, const double* v_prev, double tol, double* w,
+                    double* v, int* used_full, double* max_residual) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_REQUIRE(h && w && v, PWK_E_INVALID, "null subspace buffer");
+    PWK_REQUIRE(n >= 1 && nb >= 1 && nb <= n && nk >= 1, PWK_E_INVALID, "bad subspace shape");
+    const size_t nh = (size_t)2 * n * n * nk, nv = (size_t)2 * n * nb * nk, nw = (size_t)nb * nk;
+    DevBuf dh, dp, dw, dv;
+    dh.ensure(nh * 8);
+    dw.ensure(nw * 8);
+    dv.ensure(nv * 8);
+    PWK_HIP(hipMemcpyAsync(dh.p, h, nh * 8, hipMemcpyHostToDevice, c.stream));
+    if (v_prev) {
+      dp.ensure(nv * 8);
+      PWK_HIP(hipMemcpyAsync(dp.p, v_prev, nv * 8, hipMemcpyHostToDevice, c.stream));
+    }
+    pwk::subspace_rr(c, n, nb, nk, dh.p, v_prev ? dp.p : nullptr, tol > 0.0 ? tol : 1e-6, dw.as<double>(), dv.p,
+                     used_full, max_residual);
+    PWK_HIP(hipMemcpyAsync(w, dw.p, nw * 8, hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipMemcpyAsync(v, dv.p, nv * 8, hipMemcpyDeviceToHost, c.stream));
+    PWK_HIP(hipStreamSynchronize(c.stream));
   });
 }
 

@@ -250,6 +250,13 @@ struct Context {
   bool graphs = false;     // replay captured filter passes in the SCF (scf.hip), opt-in
 
   std::vector<KData> kd;
+  // the kinetic diagonals and projector blocks of all k-points packed k-major (ragged k × band
+  // H·ψ batches on small grids); rebuilt after any change to the per-k data
+  bool pack_valid = false;
+  int pack_nch = 0;
+  DevBuf kin_pack64, kin_pack32, B_pack64, B_pack32, DBH_pack64, DBH_pack32;
+  DevBuf band_tab;                  // int32 k-point per band of the last batch
+  std::vector<int32_t> band_tab_host;
 
   // workspaces
   DevBuf work_a, work_b;  // stick buffers / FFT scratch
@@ -302,14 +309,20 @@ void launch_density(Context& c, int nb, const void* psi, const double* fw_host, 
                     bool overwrite = false);
 void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, int precision, const void* kin,
                        int nch, const void* B, const void* DBH, void* proj);
+// k × band batches with a band count per entry (bands of the entries packed back to back)
+void launch_hpsi_batch(Context& c, int nent, const int32_t* ik, const int32_t* nbands, const void* psi, void* hpsi,
+                       int precision);
+void launch_density_batch(Context& c, int nent, const int32_t* ik, const int32_t* nbands, const void* psi,
+                          const double* fw_host, void* rho, int precision);
 // hpsi_small.hip: the whole H·ψ of nbands bands (band b at k-point b / nbk) in one launch;
 // with coef (α, c, β per k-point, device) the output is α(Hψ − cψ) − βX instead
 bool hpsi_small_supported(const Context& c, int precision, int nch);
 // nkv > 0: the band blocks of consecutive groups of nkv k-points use V(r) slot 0, 1, ...
 // (spin channels of LSDA; kinetic and projector blocks are indexed by block mod nkv)
+// band_k (device, nbands entries): k-point block of every band, for ragged k × band batches
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
                        int nch, const void* B, const void* DBH, const double* coef = nullptr,
-                       const void* X = nullptr, int nkv = 0);
+                       const void* X = nullptr, int nkv = 0, const int* band_k = nullptr);
 // the whole degree-`degree` Chebyshev filter of nbands bands in one launch on small cubic grids:
 // Yout (FP64) = p(H)·V (FP64) with the recurrence in `precision`; coef (device) holds (α, c, β)
 // per step and band block, coef[(i·nblocks + kb)·3 + j]
@@ -365,6 +378,11 @@ void launch_resid_norms(Context& c, int npw, int nbs, int nres, int nk, const vo
 bool rr_small_supported(int m);
 void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y, const void* HY, void* V,
                      double* lam, double* res, double floor_rel);
+// warm-start Rayleigh-Ritz of nk dense n×n Hamiltonians on n×nb start blocks (Vprev may be
+// null: full solve), residual gate tol per block; w (nk×nb) and V (nk blocks of n×nb) on the
+// device, used_full / max_resid (nk each, host, may be null)
+void subspace_rr(Context& c, int n, int nb, int nk, const void* H, const void* Vprev, double tol, double* w, void* V,
+                 int* used_full, double* max_resid);
 // scf.hip
 void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
              double* rho);

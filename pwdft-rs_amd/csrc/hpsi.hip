@@ -655,6 +655,110 @@ void launch_hpsi_multi(Context& c, int nk, int nb, const void* psi, void* hpsi, 
   }
 }
 
+namespace {
+
+// Packs the kinetic diagonals and projector blocks of every k-point k-major into the context's
+// pack buffers (device copies), once per change of the per-k data. False when the k-points do
+// not share one channel count or miss data (the caller then goes k-point by k-point).
+bool pack_kdata(Context& c) {
+  if (c.pack_valid) return true;
+  const int nk = (int)c.kd.size();
+  if (nk == 0) return false;
+  const int nch = c.kd[0].nch;
+  for (const KData& kd : c.kd)
+    if (!kd.have_kin || kd.nch != nch || (nch > 0 && !(kd.B64.p && kd.DBH64.p && kd.B32.p && kd.DBH32.p)))
+      return false;
+  const size_t npw = (size_t)c.npw, blk = npw * (size_t)nch;
+  c.kin_pack64.ensure((size_t)nk * npw * 8);
+  c.kin_pack32.ensure((size_t)nk * npw * 4);
+  if (nch > 0) {
+    c.B_pack64.ensure((size_t)nk * blk * 16);
+    c.DBH_pack64.ensure((size_t)nk * blk * 16);
+    c.B_pack32.ensure((size_t)nk * blk * 8);
+    c.DBH_pack32.ensure((size_t)nk * blk * 8);
+  }
+  for (int k = 0; k < nk; ++k) {
+    const KData& kd = c.kd[(size_t)k];
+    PWK_HIP(hipMemcpyAsync(c.kin_pack64.as<double>() + (size_t)k * npw, kd.kin64.p, npw * 8, hipMemcpyDeviceToDevice,
+                           c.stream));
+    PWK_HIP(hipMemcpyAsync(c.kin_pack32.as<float>() + (size_t)k * npw, kd.kin32.p, npw * 4, hipMemcpyDeviceToDevice,
+                           c.stream));
+    if (nch == 0) continue;
+    PWK_HIP(hipMemcpyAsync(c.B_pack64.as<double2>() + (size_t)k * blk, kd.B64.p, blk * 16, hipMemcpyDeviceToDevice,
+                           c.stream));
+    PWK_HIP(hipMemcpyAsync(c.DBH_pack64.as<double2>() + (size_t)k * blk, kd.DBH64.p, blk * 16,
+                           hipMemcpyDeviceToDevice, c.stream));
+    PWK_HIP(hipMemcpyAsync(c.B_pack32.as<float2>() + (size_t)k * blk, kd.B32.p, blk * 8, hipMemcpyDeviceToDevice,
+                           c.stream));
+    PWK_HIP(hipMemcpyAsync(c.DBH_pack32.as<float2>() + (size_t)k * blk, kd.DBH32.p, blk * 8, hipMemcpyDeviceToDevice,
+                           c.stream));
+  }
+  c.pack_nch = nch;
+  c.pack_valid = true;
+  return true;
+}
+
+size_t batch_bands(Context& c, int nent, const int32_t* ik, const int32_t* nbands) {
+  PWK_REQUIRE(nent >= 1 && ik && nbands, PWK_E_INVALID, "empty or null k-point batch");
+  size_t total = 0;
+  for (int e = 0; e < nent; ++e) {
+    PWK_REQUIRE(nbands[e] >= 1, PWK_E_INVALID, "every batch entry needs at least one band");
+    const KData& kd = c.k(ik[e]);
+    PWK_REQUIRE(kd.have_kin, PWK_E_STATE, "kinetic diagonal of a batch k-point not set (pwk_set_kinetic)");
+    total += (size_t)nbands[e];
+  }
+  PWK_REQUIRE(total <= (size_t)0x7fffffff, PWK_E_INVALID, "batch too large");
+  return total;
+}
+
+}  // namespace
+
+// H·ψ of a k × band batch with a band count per entry: entry e holds nbands[e] bands of k-point
+// ik[e], the entries' bands packed back to back in psi and hpsi. On small cubic grids the whole
+// batch is one fused launch that reads each band's k-point from a band table; elsewhere, or when
+// the k-points' channel counts differ, one H·ψ per entry.
+void launch_hpsi_batch(Context& c, int nent, const int32_t* ik, const int32_t* nbands, const void* psi, void* hpsi,
+                       int precision) {
+  c.check_basis();
+  const size_t total = batch_bands(c, nent, ik, nbands);
+  int nch = c.k(ik[0]).nch;
+  if (precision != PWK_SPLIT_BF16 && c.have_v && hpsi_small_supported(c, precision, nch) && pack_kdata(c)) {
+    nch = c.pack_nch;
+    std::vector<int32_t> tab(total);
+    size_t b = 0;
+    for (int e = 0; e < nent; ++e)
+      for (int j = 0; j < nbands[e]; ++j) tab[b++] = ik[e];
+    if (tab != c.band_tab_host) {
+      c.band_tab_host = tab;
+      c.band_tab.ensure(total * sizeof(int32_t));
+      PWK_HIP(hipMemcpyAsync(c.band_tab.p, c.band_tab_host.data(), total * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c.stream));
+      PWK_HIP(hipStreamSynchronize(c.stream));  // the host table may change with the next batch
+    }
+    const bool f64 = precision == PWK_FP64;
+    launch_hpsi_small(c, (int)total, 1, psi, hpsi, precision, f64 ? c.kin_pack64.p : c.kin_pack32.p, nch,
+                      f64 ? c.B_pack64.p : c.B_pack32.p, f64 ? c.DBH_pack64.p : c.DBH_pack32.p, nullptr, nullptr, 0,
+                      c.band_tab.as<int>());
+    return;
+  }
+  const size_t es = precision == PWK_FP64 ? sizeof(double2) : sizeof(float2);
+  size_t off = 0;
+  for (int e = 0; e < nent; ++e) {
+    launch_hpsi(c, ik[e], nbands[e], (const char*)psi + off * es, (char*)hpsi + off * es, precision);
+    off += (size_t)nbands[e] * c.npw;
+  }
+}
+
+// ρ += Σ_b fw_b |ψ_b(r)|² over every band of a k × band batch (fw per band, occupation × k
+// weight, entries back to back): the density does not depend on k, so all bands go through one
+// density launch.
+void launch_density_batch(Context& c, int nent, const int32_t* ik, const int32_t* nbands, const void* psi,
+                          const double* fw_host, void* rho, int precision) {
+  c.check_basis();
+  const size_t total = batch_bands(c, nent, ik, nbands);
+  launch_density(c, (int)total, psi, fw_host, rho, precision);
+}
+
 void launch_density(Context& c, int nb, const void* psi, const double* fw_host, void* rho, int precision,
                     bool overwrite) {
   c.check_basis();

@@ -30,7 +30,8 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
                                                          const typename V2s<T>::t* __restrict__ B,
                                                          const typename V2s<T>::t* __restrict__ DBH, T inv_n,
                                                          const double* __restrict__ coef,
-                                                         const typename V2s<T>::t* __restrict__ X, int nkv) {
+                                                         const typename V2s<T>::t* __restrict__ X, int nkv,
+                                                         const int* __restrict__ band_k) {
   using V2 = typename V2s<T>::t;
   extern __shared__ unsigned char smem_raw[];
   V2* G = reinterpret_cast<V2*>(smem_raw);
@@ -40,8 +41,9 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
   // k-point (and its projector blocks) in one XCD's L2
   const int q = (int)gridDim.x >> 3, r = (int)gridDim.x & 7, xcd = (int)blockIdx.x & 7;
   const int b = xcd * q + min(xcd, r) + ((int)blockIdx.x >> 3), kb = b / nbk;
-  // band block kb: k-point kb mod nkv, V(r) channel kb / nkv (coefficients per block)
-  const int k = nkv > 0 ? kb % nkv : kb;
+  // band block kb: k-point kb mod nkv, V(r) channel kb / nkv (coefficients per block); with a
+  // band table (ragged k × band batches) the k-point of every band is read from it
+  const int k = band_k ? band_k[b] : nkv > 0 ? kb % nkv : kb;
   if (nkv > 0) vr += (size_t)(kb / nkv) * N * N * N;
   const V2* p = psi + (size_t)b * npw;
   const T* t = kin + (size_t)k * npw;
@@ -450,7 +452,7 @@ bool small_len(int n) {
 
 template <typename T>
 void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi, const void* kin, int nch,
-                    const void* B, const void* DBH, const double* coef, const void* X, int nkv) {
+                    const void* B, const void* DBH, const double* coef, const void* X, int nkv, const int* band_k) {
   using V2 = typename V2s<T>::t;
   const int n = c.dims[0];
   const size_t sh = small_lds_bytes<T>(n, nch);
@@ -463,7 +465,7 @@ void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi
     ensure_lds_limit((const void*)k_hpsi_small<T, L>, sh);                                                    \
     k_hpsi_small<T, L><<<(unsigned)nbands, SMALL_NT, sh, c.stream>>>(                                          \
         (const V2*)psi, (V2*)hpsi, (const T*)kin, vr, c.g2fft.as<int>(), c.npw, nbk, nch, (const V2*)B,        \
-        (const V2*)DBH, inv_n, coef, (const V2*)X, nkv);                                                       \
+        (const V2*)DBH, inv_n, coef, (const V2*)X, nkv, band_k);                                               \
     break;
     PWK_SMALL_SIZES(PWK_CASE)
 #undef PWK_CASE
@@ -585,7 +587,8 @@ void launch_cheb_filter_small(Context& c, int nbands, int nbk, const void* V, vo
 }
 
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
-                       int nch, const void* B, const void* DBH, const double* coef, const void* X, int nkv) {
+                       int nch, const void* B, const void* DBH, const double* coef, const void* X, int nkv,
+                       const int* band_k) {
   c.check_basis();
   PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
   PWK_REQUIRE(nbands > 0 && nbk > 0 && nbands % nbk == 0, PWK_E_INVALID, "band counts must be positive");
@@ -594,10 +597,11 @@ void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* h
     const size_t slots = ((size_t)(nbands / nbk) + nkv - 1) / nkv;
     PWK_REQUIRE(c.vrn64.bytes >= slots * c.N * sizeof(double), PWK_E_STATE, "V_eff slots not set");
   }
+  PWK_REQUIRE(!band_k || (!coef && nkv == 0), PWK_E_INVALID, "a band table excludes filter coefficients");
   if (precision == PWK_FP64)
-    launch_small_t<double>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv);
+    launch_small_t<double>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv, band_k);
   else
-    launch_small_t<float>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv);
+    launch_small_t<float>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv, band_k);
 }
 
 }  // namespace pwk

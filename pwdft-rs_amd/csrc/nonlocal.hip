@@ -261,6 +261,7 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
       }
   }
   int nch = (int)ch_atom.size();
+  c.pack_valid = false;
   kd.nch = nch;
   if (nch == 0) return;
   hipStream_t st = c.stream;

@@ -778,6 +778,7 @@ void Solve::setup() {
   for (int k = 0; k < nk; ++k) wk[(size_t)k] = kpts[(size_t)k].weight;
   c.kd.clear();
   c.kd.resize((size_t)nk);
+  c.pack_valid = false;
 
   // radial tables → V_loc(G), ρ_core(r), start density
   RadialArgs ra{};

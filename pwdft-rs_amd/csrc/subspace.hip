@@ -3,6 +3,9 @@
 // the Gram products Y^H·Y and Y^H·(HY), Hermitian symmetrisation, the Löwdin
 // column scaling, per-block dot products and the seeded start block. All are
 // HBM/L2-bound elementwise passes over nk blocks of npw × nb.
+#include <algorithm>
+#include <cstring>
+
 #include "common.hpp"
 #include "jacobi.hpp"
 
@@ -78,6 +81,22 @@ __global__ void __launch_bounds__(256) k_hermitize(int m, double2* __restrict__ 
   const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
   A[(size_t)j * m + i] = h;
   A[(size_t)i * m + j] = make_double2(h.x, -h.y);
+}
+
+// The Hermitian matrix of each block from its lower triangle (upper = conjugate of lower, real
+// diagonal), which is all faer's self_adjoint_eigen(Side::Lower) reads of the projected
+// Hamiltonian (src/eigensolver/dense.rs:66-75, 231-234).
+__global__ void __launch_bounds__(256) k_lower_to_full(int m, double2* __restrict__ A) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)m * m) return;
+  A += (size_t)blockIdx.y * m * m;
+  const int i = (int)(idx % m), j = (int)(idx / m);  // row i, column j
+  if (i < j) {
+    const double2 l = A[(size_t)i * m + j];  // element (j, i) of the lower triangle
+    A[(size_t)j * m + i] = make_double2(l.x, -l.y);
+  } else if (i == j) {
+    A[(size_t)j * m + i].y = 0.0;
+  }
 }
 
 // U_b[:, j] *= 1/sqrt(max(s_bj, floor·s_b,max))  (Löwdin S^{-1/2} factor; s ascending)
@@ -406,6 +425,79 @@ __global__ void __launch_bounds__(RR_NT) k_rr_small(int npw, int m, int nres, co
 }  // namespace
 
 bool rr_small_supported(int m) { return m >= 1 && m <= JMAX; }
+
+// Warm-start Rayleigh-Ritz on dense Hamiltonians, batched over k: diagonalize_subspace
+// (src/eigensolver/dense.rs:183-260) for every block. HV = H·V_prev, H_sub = V_prev^H·HV (its
+// lower triangle), H_sub = U·diag(λ)·U^H, V = V_prev·U, residuals ||H v_i − λ_i v_i||₂
+// (eigenvalue_residuals, :112-143); a block whose largest residual exceeds tol gets the full
+// diagonalize_lowest instead (:250-254), from an untouched copy of its H. Without V_prev every
+// block takes the full solve (:205-207). Synchronises the stream: the gate is read on the host.
+void subspace_rr(Context& c, int n, int nb, int nk, const void* H, const void* Vprev, double tol, double* w, void* V,
+                 int* used_full, double* max_resid) {
+  PWK_REQUIRE(n >= 1 && nb >= 1 && nb <= n && nk >= 1, PWK_E_INVALID, "bad subspace shape");
+  PWK_REQUIRE(heev_supported(n, nb) && heev_supported(nb, nb), PWK_E_UNSUPPORTED,
+              "matrix too large for the batched eigensolver");
+  const long long nn = (long long)n * n, nbk = (long long)n * nb, bb = (long long)nb * nb;
+  std::vector<int> full((size_t)nk, 1);
+  std::vector<double> worst((size_t)nk, 0.0);
+  DevBuf hcopy;
+  if (Vprev) {
+    TimedScope ts(c, "subspace_rr");
+    DevBuf HV, VH, Hs, U, res;
+    HV.ensure((size_t)nk * nbk * 16);
+    VH.ensure((size_t)nk * nbk * 16);
+    Hs.ensure((size_t)nk * bb * 16);
+    U.ensure((size_t)nk * bb * 16);
+    res.ensure((size_t)nk * nb * 8);
+    launch_zgemm_mfma_batched(c, n, nb, n, H, n, nn, Vprev, n, nbk, HV.p, n, nbk, nk, false, PWK_FP64);
+    launch_conj_transpose(c, n, nb, Vprev, n, nbk, VH.p, nb, nbk, nk);
+    launch_zgemm_mfma_batched(c, nb, nb, n, VH.p, nb, nbk, HV.p, n, nbk, Hs.p, nb, bb, nk, false, PWK_FP64);
+    k_lower_to_full<<<dim3(blocks_for((size_t)bb, 256), (unsigned)nk), 256, 0, c.stream>>>(nb, Hs.as<double2>());
+    PWK_HIP(hipGetLastError());
+    c.eig_work.ensure(heev_work_doubles(nb, nb, nk) * 8);
+    launch_heev_batched(c, nb, nb, nk, Hs.p, c.eig_work.p, w, U.p, nullptr);
+    launch_zgemm_mfma_batched(c, n, nb, nb, Vprev, n, nbk, U.p, nb, bb, V, n, nbk, nk, false, PWK_FP64);
+    launch_zgemm_mfma_batched(c, n, nb, n, H, n, nn, V, n, nbk, HV.p, n, nbk, nk, false, PWK_FP64);
+    launch_resid_norms(c, n, nb, nb, nk, V, HV.p, w, res.as<double>());
+    std::vector<double> r((size_t)nk * nb);
+    PWK_HIP(hipMemcpyAsync(r.data(), res.p, r.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    for (int k = 0; k < nk; ++k) {
+      double m = 0.0;
+      for (int b = 0; b < nb; ++b) m = std::max(m, r[(size_t)k * nb + b]);
+      worst[(size_t)k] = m;
+      full[(size_t)k] = m > tol ? 1 : 0;
+    }
+  }
+  std::vector<int> redo;
+  for (int k = 0; k < nk; ++k)
+    if (full[(size_t)k]) redo.push_back(k);
+  if (!redo.empty()) {
+    // the full solve of the gated blocks, batched: their H copied together (the solver
+    // overwrites its input), eigenpairs scattered back
+    TimedScope ts(c, "subspace_rr_full");
+    const int nf = (int)redo.size();
+    DevBuf hf, wf, vf;
+    hf.ensure((size_t)nf * nn * 16);
+    wf.ensure((size_t)nf * nb * 8);
+    vf.ensure((size_t)nf * nbk * 16);
+    for (int i = 0; i < nf; ++i)
+      PWK_HIP(hipMemcpyAsync(hf.as<double2>() + (size_t)i * nn, (const double2*)H + (size_t)redo[(size_t)i] * nn,
+                             (size_t)nn * 16, hipMemcpyDeviceToDevice, c.stream));
+    c.eig_work.ensure(heev_work_doubles(n, nb, nf) * 8);
+    launch_heev_batched(c, n, nb, nf, hf.p, c.eig_work.p, wf.as<double>(), vf.p, nullptr);
+    for (int i = 0; i < nf; ++i) {
+      const int k = redo[(size_t)i];
+      PWK_HIP(hipMemcpyAsync(w + (size_t)k * nb, wf.as<double>() + (size_t)i * nb, (size_t)nb * 8,
+                             hipMemcpyDeviceToDevice, c.stream));
+      PWK_HIP(hipMemcpyAsync((double2*)V + (size_t)k * nbk, vf.as<double2>() + (size_t)i * nbk, (size_t)nbk * 16,
+                             hipMemcpyDeviceToDevice, c.stream));
+    }
+    c.sync();
+  }
+  if (used_full) std::memcpy(used_full, full.data(), full.size() * sizeof(int));
+  if (max_resid) std::memcpy(max_resid, worst.data(), worst.size() * 8);
+}
 
 void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y, const void* HY, void* V,
                      double* lam, double* res, double floor_rel) {

@@ -86,7 +86,35 @@ _SIGS = {
     "pwk_dev_dense_eigh": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "pwk_density_accumulate": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_int]),
     "pwk_dev_density_accumulate": (C.c_int, [_vp, C.c_int, _vp, _dp, _vp, C.c_int]),
+    "pwk_upf_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "pwk_upf_parse": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(_vp)]),
+    "pwk_upf_species": (C.c_int, [_vp, _vp]),
+    "pwk_upf_info": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_int)]),
+    "pwk_upf_free": (C.c_int, [_vp]),
+    "pwk_hpsi_batch": (C.c_int, [_vp, _vp, _dp, _dp, C.c_int]),
+    "pwk_dev_hpsi_batch": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
+    "pwk_density_accumulate_batch": (C.c_int, [_vp, _vp, _dp, _dp, _dp, C.c_int]),
+    "pwk_dev_density_accumulate_batch": (C.c_int, [_vp, _vp, _vp, _dp, _vp, C.c_int]),
+    "pwk_subspace_rr": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double, _dp, _dp, _ip, _dp]),
+    "pwk_dev_subspace_rr": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_double, _vp, _vp, _ip, _dp]),
 }
+
+
+class KBatch(C.Structure):
+    """pwk_kbatch: per entry a k-point slot and a band count; the entries' bands are packed
+    back to back."""
+    _fields_ = [("nentries", C.c_int), ("ik", _ip), ("nbands", _ip)]
+
+
+def kbatch(iks, nbands):
+    """A pwk_kbatch over (k-point slot, band count) pairs; keeps its arrays alive."""
+    ik = np.ascontiguousarray(iks, dtype=np.int32)
+    nb = np.ascontiguousarray(nbands, dtype=np.int32)
+    if ik.shape != nb.shape or ik.ndim != 1:
+        raise ValueError("one band count per k-point entry")
+    b = KBatch(len(ik), ik.ctypes.data_as(_ip), nb.ctypes.data_as(_ip))
+    b._keep = (ik, nb)
+    return b
 
 
 def lib():
@@ -322,6 +350,55 @@ class GpuAccelerator:
 
     def hpsi_dev(self, ik, nb, psi_ptr, hpsi_ptr, precision="fp32"):
         _check(lib().pwk_dev_hpsi(self.h, ik, nb, psi_ptr, hpsi_ptr, _prec(precision)))
+
+    def hpsi_batch(self, iks, nbands, psi, precision="fp64"):
+        """H·psi of a k × band batch (pwk_hpsi_batch): entry e holds nbands[e] bands of k-point
+        slot iks[e]; psi (sum(nbands), npw) with the entries' bands back to back."""
+        p = _c128(psi)
+        b = kbatch(iks, nbands)
+        if p.shape[0] != int(np.sum(nbands)):
+            raise ValueError("psi rows must equal the batch's band count")
+        out = np.empty_like(p)
+        _check(lib().pwk_hpsi_batch(self.h, C.byref(b), _d(p.view(np.float64)), _d(out.view(np.float64)),
+                                    _prec(precision)))
+        return out
+
+    def hpsi_batch_dev(self, iks, nbands, psi_ptr, hpsi_ptr, precision="fp32"):
+        b = kbatch(iks, nbands)
+        _check(lib().pwk_dev_hpsi_batch(self.h, C.byref(b), psi_ptr, hpsi_ptr, _prec(precision)))
+
+    def density_batch(self, iks, nbands, psi, fw, rho=None, precision="fp64"):
+        """rho += Σ_b fw_b |ψ_b(r)|² over a k × band batch (pwk_density_accumulate_batch)."""
+        p = _c128(psi)
+        b = kbatch(iks, nbands)
+        if p.shape[0] != int(np.sum(nbands)) or len(fw) != p.shape[0]:
+            raise ValueError("psi rows and fw must match the batch's band count")
+        acc = np.zeros(self.n_grid) if rho is None else _f64(rho).copy()
+        _check(lib().pwk_density_accumulate_batch(self.h, C.byref(b), _d(p.view(np.float64)), _d(_f64(fw)), _d(acc),
+                                                  _prec(precision)))
+        return acc
+
+    def subspace_rr(self, h, v_prev, nb, tol=0.0):
+        """diagonalize_subspace (src/eigensolver/dense.rs:183-260) batched over k: h (nk, n, n),
+        v_prev (nk, n, nb) or None. Returns (w (nk, nb), v (nk, n, nb), used_full (nk,),
+        max_residual (nk,))."""
+        h = np.asarray(h, dtype=np.complex128)
+        if h.ndim == 2:
+            h = h[None]
+        nk, n, _ = h.shape
+        hc = np.ascontiguousarray(np.transpose(h, (0, 2, 1)))
+        vp = None
+        if v_prev is not None:
+            v_prev = np.asarray(v_prev, dtype=np.complex128).reshape(nk, n, -1)[:, :, :nb]
+            vp = np.ascontiguousarray(np.transpose(v_prev, (0, 2, 1)))
+        w = np.empty((nk, nb))
+        v = np.empty((nk, nb, n), dtype=np.complex128)
+        used = np.zeros(nk, dtype=np.int32)
+        res = np.zeros(nk)
+        _check(lib().pwk_subspace_rr(self.h, n, nb, nk, _d(hc.view(np.float64)),
+                                     _d(vp.view(np.float64)) if vp is not None else None, float(tol), _d(w),
+                                     _d(v.view(np.float64)), used.ctypes.data_as(_ip), _d(res)))
+        return w, np.transpose(v, (0, 2, 1)), used, res
 
     def dense_h(self, ik, with_nl=True):
         h = np.empty((self.npw, self.npw), dtype=np.complex128)  # filled column-major

@@ -9,7 +9,6 @@ G-vector arithmetic repeat the reference's floating-point operation order
 from __future__ import annotations
 
 import math
-import re
 import os
 
 import numpy as np
@@ -130,55 +129,53 @@ def kinetic(gvec, k):
     return HBAR2_OVER_2M * ((q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1]) + q[:, 2] * q[:, 2])
 
 
-ATOMIC_NUMBER = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Ne": 10, "Na": 11,
-                 "Mg": 12, "Al": 13, "Si": 14, "P": 15, "S": 16, "Cl": 17, "Ar": 18, "K": 19, "Ca": 20, "Ti": 22,
-                 "Fe": 26, "Co": 27, "Ni": 28, "Cu": 29, "Zn": 30, "Ga": 31, "Ge": 32, "As": 33, "Mn": 25}
-
-
 def read_upf(path):
-    """Minimal UPF v2 reader for the radial tables (src/pseudopotential/upf/parse.rs:20-140)."""
-    text = open(path).read()
+    """UPF v2 pseudopotential through libpwk's reader (pwk_upf_load, the reference's
+    src/pseudopotential/upf/parse.rs:20-140), as numpy arrays in eV / Å units: r, rab,
+    v_local, chi (nproj × mesh, χ = r·β), dij (nproj × nproj), l, rho_atom (None when absent
+    or all zero), core (None without the core-correction flag)."""
+    import ctypes as C
 
-    def attr(s, name):
-        i = s.index(name + '="') + len(name) + 2
-        return s[i:s.index('"', i)].strip()
+    from . import PwkError, lib
 
-    def block(tag, n):
-        i = text.index("<" + tag)
-        i = text.index(">", i) + 1
-        j = text.index("</" + tag + ">", i)
-        v = np.array(text[i:j].split(), dtype=np.float64)
-        assert v.size == n, tag
-        return v
+    L = lib()
+    h = C.c_void_p()
+    st = L.pwk_upf_load(os.fsencode(path), C.byref(h))
+    if st != 0:
+        raise PwkError(st, L.pwk_last_error().decode())
+    try:
+        from .scf import Species
 
-    mesh = int(attr(text, "mesh_size"))
-    nproj = int(attr(text, "number_of_proj"))
-    out = {
-        "element": attr(text, "element"),
-        "z_valence": float(attr(text, "z_valence")),
-        "r": block("PP_R", mesh) * BOHR_TO_ANG,
-        "rab": block("PP_RAB", mesh) * BOHR_TO_ANG,
-        "dij": block("PP_DIJ", nproj * nproj).reshape(nproj, nproj) * RY_TO_EV,
-    }
-    ls, chi = [], []
-    for i in range(1, nproj + 1):
-        tag = f"PP_BETA.{i}"
-        s = text.index("<" + tag)
-        head = text[s:text.index(">", s)]
-        ls.append(int(attr(head, "angular_momentum")))
-        chi.append(block(tag, mesh) / math.sqrt(BOHR_TO_ANG))
-    out["l"] = ls
-    out["chi"] = np.array(chi)
-    out["v_local"] = block("PP_LOCAL", mesh) * RY_TO_EV
-    out["atomic_number"] = ATOMIC_NUMBER[out["element"]]
-    out["rho_atom"] = None
-    if "<PP_RHOATOM" in text:
-        rho = block("PP_RHOATOM", mesh) / BOHR_TO_ANG
-        if np.any(np.abs(rho) > 1e-20):
-            out["rho_atom"] = rho
-    out["core"] = None
-    if re.search(r'core_correction="[Tt]"', text) or "nlcc=.true." in text:
-        out["core"] = block("PP_NLCC", mesh) / (BOHR_TO_ANG * BOHR_TO_ANG * BOHR_TO_ANG) if "<PP_NLCC" in text else np.zeros(mesh)
+        sp = Species()
+        st = L.pwk_upf_species(h, C.byref(sp))
+        if st != 0:
+            raise PwkError(st, L.pwk_last_error().decode())
+        sym = C.create_string_buffer(4)
+        lmax = C.c_int()
+        L.pwk_upf_info(h, sym, C.byref(lmax))
+        m, npj = sp.mesh, sp.nproj
+
+        def arr(ptr, n):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if ptr and n else None
+
+        out = {
+            "element": sym.value.decode(),
+            "atomic_number": sp.atomic_number,
+            "z_valence": sp.z_valence,
+            "l_max": lmax.value,
+            "r": arr(sp.r, m),
+            "rab": arr(sp.rab, m),
+            "v_local": arr(sp.v_local, m),
+            "l": [int(x) for x in (arr(sp.proj_l, npj) if npj else [])],
+            "chi": arr(sp.beta, npj * m).reshape(npj, m) if npj else np.zeros((0, m)),
+            "dij": arr(sp.dij, npj * npj).reshape(npj, npj) if npj else np.zeros((0, 0)),
+            "rho_atom": arr(sp.rho_atom, m),
+            "core": arr(sp.rho_core, m),
+        }
+    finally:
+        L.pwk_upf_free(h)
+    if out["rho_atom"] is not None and not np.any(np.abs(out["rho_atom"]) > 1e-20):
+        out["rho_atom"] = None
     return out
 
 

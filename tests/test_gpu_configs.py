@@ -132,8 +132,11 @@ def _si8_deck(scf_mod):
 
 def test_si8_216k_fp32_filter_matches_fp64(scf_mod, gpu):
     deck = _si8_deck(scf_mod)
+    # |ΔE| < 1e-6 eV as in the reference's own deck (tests/vgc5_per_component_si.rs:259): at 1e-7 eV
+    # the FP64 solve sits at Δρ ~ 3e-11 with |ΔE| ~ 5e-7 eV, where the reference algorithm's DIIS
+    # step is ill-conditioned (src/scf/mixing/linalg.rs:24-52) and jumps away again
     kw = dict(ecut_ev=16 * RY, kgrid=(6, 6, 6), use_symmetry=False, n_bands=24, max_iter=60, conv_threshold=1e-7,
-              energy_threshold=1e-7, mixing_ndim=8, smearing_sigma=0.01 * RY, eigensolver=1)
+              energy_threshold=1e-6, mixing_ndim=8, smearing_sigma=0.01 * RY, eigensolver=1)
     print("si8 216k fp64", flush=True)
     r64 = scf_mod.run_scf(gpu, deck, scf_mod.ScfParams(precision="fp64", **kw))
     print("si8 216k fp32", flush=True)

@@ -5,7 +5,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import SI_UPF, C_UPF
+from conftest import C_UPF, GOLDEN, SI_UPF
 
 
 def test_library_exports_every_header_symbol(pwk_mod):
@@ -184,3 +184,74 @@ def test_malformed_requests_are_rejected_before_any_table(pwk_mod):
     prm = scf._params(p)
     prm.mixing_ndim = 32  # the largest history the mixer holds
     assert _tables_rc(pwk_mod, deck, prm) == 0
+
+
+# ---- the library's UPF reader (pwk_upf_load): the reference parser's own tests
+# (src/pseudopotential/upf/parse.rs:153-240) and its error cases (xml.rs) --------------------
+def test_upf_reader_reference_checks():
+    from pwk import decks
+
+    u = decks.read_upf(SI_UPF)
+    # test_parse_header
+    assert u["element"] == "Si" and u["atomic_number"] == 14
+    assert abs(u["z_valence"] - 4.0) < 1e-10 and u["l_max"] >= 1 and len(u["l"]) > 0
+    # test_radial_grid_monotonic, test_radial_grid_units
+    assert np.all(np.diff(u["r"]) > 0)
+    assert u["r"][0] < 0.01 and u["r"][-1] > 1.0
+    # test_v_local_coulomb_tail: V_local → -Z e²/r within 1% ten points from the end
+    e2 = 14.399645351950548
+    r_far, v_far = u["r"][-10], u["v_local"][-10]
+    v_c = -u["z_valence"] * e2 / r_far
+    assert abs((v_far - v_c) / v_c) < 0.01
+    # test_dij_matrix_size
+    assert u["dij"].shape == (len(u["l"]), len(u["l"]))
+    # test_si_core_charge_integrates_to_partial_core: ∫4πr²ρ_core dr ≈ 0.74 e
+    q = np.sum(4.0 * np.pi * u["r"] ** 2 * u["core"] * u["rab"])
+    assert abs(q - 0.74) < 0.05
+
+
+def test_upf_reader_dij_equals_reference_csv():
+    from pwk import decks
+
+    u = decks.read_upf(SI_UPF)
+    rows = np.loadtxt(os.path.join(GOLDEN, "csv", "dij_si_reference.csv"), delimiter=",", skiprows=1)
+    d = np.zeros_like(u["dij"])
+    for i, j, v in rows:
+        d[int(i), int(j)] = v * 13.605693122994
+    assert np.array_equal(u["dij"], d)
+
+
+def _parse(pwk_mod, text):
+    import ctypes as C
+
+    L = pwk_mod.lib()
+    h = C.c_void_p()
+    b = text.encode()
+    st = L.pwk_upf_parse(b, len(b), C.byref(h))
+    if st == 0:
+        L.pwk_upf_free(h)
+    return st, L.pwk_last_error().decode()
+
+
+def test_upf_reader_errors(pwk_mod):
+    good = open(SI_UPF).read()
+    assert _parse(pwk_mod, good)[0] == 0
+    # not UPF at all
+    st, msg = _parse(pwk_mod, "<qe_pp:pseudo>\n</qe_pp:pseudo>")
+    assert st == 1 and "unrecognized" in msg
+    # a block with one number fewer than mesh_size
+    i = good.index("</PP_LOCAL>")
+    j = good.rindex(" ", 0, good.rindex("E", 0, i))
+    st, msg = _parse(pwk_mod, good[:j] + good[i:])
+    assert st == 1 and "PP_LOCAL" in msg and "expected" in msg
+    # a Fortran D exponent is not a float (the reference parses with f64::from_str)
+    a = good.index("E+00", good.index("<PP_LOCAL"))
+    st, msg = _parse(pwk_mod, good[:a] + "D+00" + good[a + 4:])
+    assert st == 1 and "parse" in msg
+    # missing projector block
+    st, msg = _parse(pwk_mod, good.replace("<PP_BETA.2", "<PP_XETA.2").replace("</PP_BETA.2>", "</PP_XETA.2>"))
+    assert st == 1 and "PP_BETA.2" in msg
+    # unknown element symbol
+    k = good.index('element="') + len('element="')
+    st, msg = _parse(pwk_mod, good[:k] + "Xx" + good[k + 2:])
+    assert st == 1 and "unknown element" in msg
