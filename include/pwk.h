@@ -208,6 +208,15 @@ int pwk_dev_dense_h(pwk_ctx* ctx, int ik, int with_nl, void* h /* double2 npw×n
  * one-workgroup-per-matrix solver holds in LDS (n about 2500). */
 int pwk_dense_eigh(pwk_ctx* ctx, int n, int nb, int nk, double* h, double* w, double* v);
 int pwk_dev_dense_eigh(pwk_ctx* ctx, int n, int nb, int nk, void* h, void* w, void* v);
+/* No counterpart in the reference: pwk_dev_dense_eigh for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, kernels and results; the
+ * eigensolver's workspace is a static 16 MiB array per GPU instead of memory
+ * the context allocates, so the call allocates nothing. Batches needing more
+ * than 2^21 workspace doubles (nk*(4n + 6n*nb)) return PWK_E_INVALID. All calls
+ * on one GPU share that workspace: calls that overlap on one GPU (two streams,
+ * two threads) race; serialise them. */
+int pwk_dev_dense_eigh_noalloc(pwk_ctx* ctx, int n, int nb, int nk, void* h, void* w, void* v);
 
 /* Density accumulation (src/scf/density.rs:59-97): for each band b with
  * fw[b] >= 1e-15, rho[r] += fw[b]·|IFFT_unnorm(scatter ψ_b)(r)|². rho is real

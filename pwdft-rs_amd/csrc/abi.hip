@@ -105,6 +105,7 @@ int pwk_ctx_create(int device, pwk_ctx** out) {
     hipDeviceProp_t prop;
     PWK_HIP(hipGetDeviceProperties(&prop, device));
     p->c.cu_count = prop.multiProcessorCount;
+    pwk::init_noalloc_scratch(device);
     *out = p.release();
   });
 }
@@ -509,6 +510,16 @@ int pwk_dev_dense_eigh(pwk_ctx* ctx, int n, int nb, int nk, void* h, void* w, vo
     PWK_REQUIRE(h && w && v, PWK_E_INVALID, "null eigensolver buffer");
     c.eig_work.ensure(pwk::heev_work_doubles(n, nb, nk) * 8);
     pwk::launch_heev_batched(c, n, nb, nk, h, c.eig_work.p, (double*)w, v, nullptr);
+  });
+}
+
+int pwk_dev_dense_eigh_noalloc(pwk_ctx* ctx, int n, int nb, int nk, void* h, void* w, void* v) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    PWK_REQUIRE(h && w && v, PWK_E_INVALID, "null eigensolver buffer");
+    PWK_REQUIRE(n >= 1 && nb >= 1 && nb <= n && nk >= 1, PWK_E_INVALID, "bad eigensolver shape");
+    PWK_REQUIRE(pwk::heev_supported(n, nb), PWK_E_UNSUPPORTED, "matrix too large for the batched eigensolver");
+    pwk::launch_heev_noalloc(c, n, nb, nk, h, (double*)w, v);
   });
 }
 

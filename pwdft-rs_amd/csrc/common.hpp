@@ -349,6 +349,12 @@ void launch_zgemm_mfma_batched(Context& c, int M, int N, int K, const void* A, i
 size_t heev_work_doubles(int n, int nb, int nk);
 bool heev_supported(int n, int nb);
 void launch_heev_batched(Context& c, int n, int nb, int nk, void* H, void* work, double* w, void* V, int* status);
+// the same with the static per-device workspace instead of an allocated one (no allocation);
+// init_noalloc_scratch looks up that workspace's device address, once per device, with the
+// device current
+void init_noalloc_scratch(int device);
+size_t noalloc_heev_capacity_doubles();
+void launch_heev_noalloc(Context& c, int n, int nb, int nk, void* H, double* w, void* V);
 // nonlocal.hip
 // Where a build writes its projectors: the k-point's own buffers (KData B64, DBH64, B32, DBH32)
 // by default, or caller-owned arrays of npw × nch entries (the SCF's batched blocks).

@@ -456,6 +456,47 @@ size_t heev_lds_bytes(int n, int nb) {
 
 size_t heev_work_doubles(int n, int nb, int nk) { return (size_t)nk * eig_stride(n, nb); }
 
+// ---- the allocation-free form -----------------------------------------------------------
+// Workspace of pwk_dev_dense_eigh_noalloc: one static device array per GPU (16 MiB, held from
+// the first use of the library on that GPU until the process ends). Host code never passes the
+// array's name to a kernel: its device address is looked up once per device, with that device
+// current, when a context is created on it, and only that address is used afterwards. Every
+// call on one GPU shares the array; the eigensolver writes each workspace entry before reading
+// it, so nothing needs clearing between calls, and calls that overlap on one GPU race.
+constexpr size_t kNoAllocEigDoubles = (size_t)1 << 21;
+static __device__ __align__(256) double scratch[kNoAllocEigDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void init_noalloc_scratch(int device) {
+  PWK_REQUIRE(device >= 0 && device < kMaxDevices, PWK_E_NO_DEVICE, "device ordinal above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  PWK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  PWK_REQUIRE(p != nullptr, PWK_E_DEVICE, "static eigensolver workspace has no device address");
+  g_scratch_ptr[device] = static_cast<double*>(p);
+}
+
+size_t noalloc_heev_capacity_doubles() { return kNoAllocEigDoubles; }
+
+void launch_heev_noalloc(Context& c, int n, int nb, int nk, void* H, double* w, void* V) {
+  PWK_REQUIRE(n >= 1 && nb >= 1 && nb <= n && nk >= 1, PWK_E_INVALID, "bad eigensolver shape");
+  PWK_REQUIRE(heev_work_doubles(n, nb, nk) <= kNoAllocEigDoubles, PWK_E_INVALID,
+              "batch needs more than the 16 MiB static workspace (nk*(4n + 6n*nb) doubles above 2^21)");
+  double* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c.device >= 0 && c.device < kMaxDevices) ws = g_scratch_ptr[c.device];
+  }
+  PWK_REQUIRE(ws != nullptr, PWK_E_STATE, "static eigensolver workspace not set up on this device");
+  launch_heev_batched(c, n, nb, nk, H, ws, w, V, nullptr);
+}
+
 bool heev_supported(int n, int nb) { return n >= 1 && nb >= 1 && nb <= n && heev_lds_base(n, nb) <= kLdsCap; }
 
 void launch_heev_batched(Context& c, int n, int nb, int nk, void* H, void* work, double* w, void* V, int* status) {

@@ -84,6 +84,7 @@ _SIGS = {
     "pwk_dev_dense_h": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "pwk_dense_eigh": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
     "pwk_dev_dense_eigh": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "pwk_dev_dense_eigh_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "pwk_density_accumulate": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_int]),
     "pwk_dev_density_accumulate": (C.c_int, [_vp, C.c_int, _vp, _dp, _vp, C.c_int]),
     "pwk_upf_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
