@@ -71,13 +71,23 @@ void* pwk_ctx_stream(pwk_ctx* ctx);
  * every other grid. All compute the same products; the switch exists for
  * comparisons. */
 int pwk_ctx_set_fused_hpsi(pwk_ctx* ctx, int on);
-/* No counterpart in the reference. On, pwk_scf_run captures each filter pass of
- * the Chebyshev-filtered solver (fused small-grid path) as a HIP graph per spin
- * channel and filter precision and replays it; off (the default) launches the
- * kernels one by one. Both run the same kernels in the same order and give
- * identical results. Capture must not overlap allocations by other threads on
- * the device (the serving workers never capture). */
+/* No counterpart in the reference. On, pwk_scf_run captures every SCF iteration
+ * after the first of a device-resident solve (below) as one HIP graph per filter
+ * precision and replays it, and on the host-synchronised path each filter pass of
+ * the Chebyshev-filtered solver (fused small-grid path) per spin channel and filter
+ * precision; off (the default) launches the kernels one by one. Both run the same
+ * kernels in the same order and give identical results. Capture must not overlap
+ * allocations by other threads on the device (the serving workers never capture). */
 int pwk_ctx_set_graphs(pwk_ctx* ctx, int on);
+/* No counterpart in the reference. On (the default), a filtered solve on a small
+ * cubic grid (one-launch filter, fused Rayleigh-Ritz and density) keeps every
+ * scalar of an iteration on the device: the filter passes stop through a device
+ * residual gate, the Fermi level, occupations, band energy, -TS, the density
+ * normalisation and the Anderson/DIIS solve run as kernels, and the host
+ * synchronises once per iteration to read the energies and test convergence. Off:
+ * the host takes those steps between launches (the path every other solve uses).
+ * Both follow the reference's algorithm; results agree to rounding. */
+int pwk_ctx_set_resident(pwk_ctx* ctx, int on);
 
 /* ---------------------------------------------------------------------------
  * Kernel timing (no counterpart in the reference, which has no GPU timers):

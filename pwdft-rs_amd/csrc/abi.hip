@@ -143,6 +143,10 @@ int pwk_ctx_set_graphs(pwk_ctx* ctx, int on) {
   return guarded([&] { ctx_of(ctx).graphs = on != 0; });
 }
 
+int pwk_ctx_set_resident(pwk_ctx* ctx, int on) {
+  return guarded([&] { ctx_of(ctx).resident = on != 0; });
+}
+
 int pwk_profile_enable(pwk_ctx* ctx, int on) {
   return guarded([&] {
     Context& c = ctx_of(ctx);

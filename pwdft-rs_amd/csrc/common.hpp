@@ -247,6 +247,7 @@ struct Context {
   DevBuf vrn64, vrn32; // real N, V(r) in grid order [i1][i2][i3]
   bool small_hpsi = true;  // fused one-launch H·ψ on small cubic grids (hpsi_small.hip)
   bool fused_filter = true;  // with small_hpsi: the whole Chebyshev filter pass in one launch
+  bool resident = true;      // device-resident SCF iterations where the filtered solver allows them
   bool graphs = false;     // replay captured filter passes in the SCF (scf.hip), opt-in
 
   std::vector<KData> kd;
@@ -322,14 +323,15 @@ bool hpsi_small_supported(const Context& c, int precision, int nch);
 // band_k (device, nbands entries): k-point block of every band, for ragged k × band batches
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
                        int nch, const void* B, const void* DBH, const double* coef = nullptr,
-                       const void* X = nullptr, int nkv = 0, const int* band_k = nullptr);
+                       const void* X = nullptr, int nkv = 0, const int* band_k = nullptr,
+                       const int* skip = nullptr);
 // the whole degree-`degree` Chebyshev filter of nbands bands in one launch on small cubic grids:
 // Yout (FP64) = p(H)·V (FP64) with the recurrence in `precision`; coef (device) holds (α, c, β)
 // per step and band block, coef[(i·nblocks + kb)·3 + j]
 bool cheb_filter_small_supported(const Context& c, int precision, int nch);
 void launch_cheb_filter_small(Context& c, int nbands, int nbk, const void* V, void* Yout, int precision,
                               const void* kin, int nch, const void* B, const void* DBH, const double* coef, int degree,
-                              int nkv = 0);
+                              int nkv = 0, const int* skip = nullptr);
 // ρ += Σ_b fw_b |ψ_b(r)|² on small cubic grids (fw on the device)
 bool density_small_supported(const Context& c, int precision);
 void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision,
@@ -382,8 +384,9 @@ void launch_resid_norms(Context& c, int npw, int nbs, int nres, int nk, const vo
 // whole Rayleigh-Ritz step per k-point in one workgroup (m <= 32): V = Y·C, Ritz values lam
 // (nk × m) and residual norms res (nk × m, first nres columns)
 bool rr_small_supported(int m);
+// skip (device, may be null): the launch does nothing while *skip is nonzero
 void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y, const void* HY, void* V,
-                     double* lam, double* res, double floor_rel);
+                     double* lam, double* res, double floor_rel, const int* skip = nullptr);
 // warm-start Rayleigh-Ritz of nk dense n×n Hamiltonians on n×nb start blocks (Vprev may be
 // null: full solve), residual gate tol per block; w (nk×nb) and V (nk blocks of n×nb) on the
 // device, used_full / max_resid (nk each, host, may be null)

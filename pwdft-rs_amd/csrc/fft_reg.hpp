@@ -54,7 +54,12 @@ constexpr double cos_frac(long e, long m) {
 constexpr double sin_frac(long e, long m) { return cos_frac(4 * e - m, 4 * m); }  // sin(a) = cos(a - π/2)
 
 // ---- factorisation -----------------------------------------------------------
-constexpr int first_radix(int n) {
+// Radix order of the decimation in frequency: ORD 0 takes the largest power-of-four factor
+// first (4, then 2, 3, 5); ORD 1 the odd radices first (5, 3, then 4, 2). Both compute the same
+// transform; which one runs faster for a length and kernel is measured, not assumed (the FFT
+// autotuner in autotune.hip chooses per launch shape).
+constexpr int first_radix(int n, int ord = 0) {
+  if (ord == 1) return n % 5 == 0 ? 5 : n % 3 == 0 ? 3 : n % 4 == 0 ? 4 : n % 2 == 0 ? 2 : n;
   return n % 4 == 0 ? 4 : n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : n % 5 == 0 ? 5 : n;
 }
 constexpr bool fft_supported(int n) {
@@ -65,12 +70,12 @@ constexpr bool fft_supported(int n) {
   return n == 1;
 }
 
-template <int N>
+template <int N, int ORD = 0>
 struct FftOrder {
   // position of output frequency f in the DIF result
   static constexpr int pos_rec(int f, int m) {
     if (m == 1) return 0;
-    int p = first_radix(m);
+    int p = first_radix(m, ORD);
     int mm = m / p;
     return (f % p) * mm + pos_rec(f / p, mm);
   }
@@ -180,10 +185,10 @@ __device__ __forceinline__ void bfly5(T* re, T* im, int i0, int i1, int i2, int 
 }
 
 // One DIF stage over all blocks of size M, then recurse on M/p.
-template <typename T, int N, int M, int SIGN>
+template <typename T, int N, int M, int SIGN, int ORD>
 __device__ __forceinline__ void dif_stages(T* re, T* im) {
   if constexpr (M > 1) {
-    constexpr int p = first_radix(M);
+    constexpr int p = first_radix(M, ORD);
     constexpr int m = M / p;
 #pragma unroll
     for (int b = 0; b < N; b += M) {
@@ -213,14 +218,14 @@ __device__ __forceinline__ void dif_stages(T* re, T* im) {
         }
       }
     }
-    dif_stages<T, N, m, SIGN>(re, im);
+    dif_stages<T, N, m, SIGN, ORD>(re, im);
   }
 }
 
-// In-place FFT; the result for frequency f sits at index FftOrder<N>::pos(f).
-template <typename T, int N, int SIGN>
+// In-place FFT; the result for frequency f sits at index FftOrder<N, ORD>::pos(f).
+template <typename T, int N, int SIGN, int ORD = 0>
 __device__ __forceinline__ void fft_dif(T (&re)[N], T (&im)[N]) {
-  dif_stages<T, N, N, SIGN>(re, im);
+  dif_stages<T, N, N, SIGN, ORD>(re, im);
 }
 
 }  // namespace pwk

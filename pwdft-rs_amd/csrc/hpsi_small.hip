@@ -31,8 +31,10 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
                                                          const typename V2s<T>::t* __restrict__ DBH, T inv_n,
                                                          const double* __restrict__ coef,
                                                          const typename V2s<T>::t* __restrict__ X, int nkv,
-                                                         const int* __restrict__ band_k) {
+                                                         const int* __restrict__ band_k,
+                                                         const int* __restrict__ skip) {
   using V2 = typename V2s<T>::t;
+  if (skip && *skip) return;  // a filter pass after the residual gate passed (uniform)
   extern __shared__ unsigned char smem_raw[];
   V2* G = reinterpret_cast<V2*>(smem_raw);
   V2* proj = G + N * N * N;  // nch projections
@@ -233,8 +235,10 @@ __global__ void __launch_bounds__(SMALL_NT) k_cheb_filter_small(const double2* _
                                                                 const int* __restrict__ g2fft, int npw, int nbk,
                                                                 int nch, const typename V2s<T>::t* __restrict__ B,
                                                                 const typename V2s<T>::t* __restrict__ DBH, T inv_n,
-                                                                const double* __restrict__ coef, int degree, int nkv) {
+                                                                const double* __restrict__ coef, int degree, int nkv,
+                                                                const int* __restrict__ skip) {
   using V2 = typename V2s<T>::t;
+  if (skip && *skip) return;  // a filter pass after the residual gate passed (uniform)
   extern __shared__ unsigned char smem_raw[];
   V2* G = reinterpret_cast<V2*>(smem_raw);
   V2* proj = G + N * N * N;
@@ -452,7 +456,8 @@ bool small_len(int n) {
 
 template <typename T>
 void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi, const void* kin, int nch,
-                    const void* B, const void* DBH, const double* coef, const void* X, int nkv, const int* band_k) {
+                    const void* B, const void* DBH, const double* coef, const void* X, int nkv, const int* band_k,
+                    const int* skip) {
   using V2 = typename V2s<T>::t;
   const int n = c.dims[0];
   const size_t sh = small_lds_bytes<T>(n, nch);
@@ -465,7 +470,7 @@ void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi
     ensure_lds_limit((const void*)k_hpsi_small<T, L>, sh);                                                    \
     k_hpsi_small<T, L><<<(unsigned)nbands, SMALL_NT, sh, c.stream>>>(                                          \
         (const V2*)psi, (V2*)hpsi, (const T*)kin, vr, c.g2fft.as<int>(), c.npw, nbk, nch, (const V2*)B,        \
-        (const V2*)DBH, inv_n, coef, (const V2*)X, nkv, band_k);                                               \
+        (const V2*)DBH, inv_n, coef, (const V2*)X, nkv, band_k, skip);                                         \
     break;
     PWK_SMALL_SIZES(PWK_CASE)
 #undef PWK_CASE
@@ -482,7 +487,7 @@ size_t filter_lds_bytes(int n, int nch, int npw) {
 
 template <typename T>
 void launch_filter_small_t(Context& c, int nbands, int nbk, const void* V, void* Yout, const void* kin, int nch,
-                           const void* B, const void* DBH, const double* coef, int degree, int nkv) {
+                           const void* B, const void* DBH, const double* coef, int degree, int nkv, const int* skip) {
   using V2 = typename V2s<T>::t;
   const int n = c.dims[0];
   const size_t sh = filter_lds_bytes<T>(n, nch, c.npw);
@@ -495,7 +500,7 @@ void launch_filter_small_t(Context& c, int nbands, int nbk, const void* V, void*
     ensure_lds_limit((const void*)k_cheb_filter_small<T, L>, sh);                                             \
     k_cheb_filter_small<T, L><<<(unsigned)nbands, SMALL_NT, sh, c.stream>>>(                                   \
         (const double2*)V, (double2*)Yout, (const T*)kin, vr, c.g2fft.as<int>(), c.npw, nbk, nch, (const V2*)B, \
-        (const V2*)DBH, inv_n, coef, degree, nkv);                                                             \
+        (const V2*)DBH, inv_n, coef, degree, nkv, skip);                                                       \
     break;
     PWK_SMALL_SIZES(PWK_CASE)
 #undef PWK_CASE
@@ -569,7 +574,7 @@ bool cheb_filter_small_supported(const Context& c, int precision, int nch) {
 
 void launch_cheb_filter_small(Context& c, int nbands, int nbk, const void* V, void* Yout, int precision,
                               const void* kin, int nch, const void* B, const void* DBH, const double* coef, int degree,
-                              int nkv) {
+                              int nkv, const int* skip) {
   c.check_basis();
   PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
   PWK_REQUIRE(nbands > 0 && nbk > 0 && nbands % nbk == 0, PWK_E_INVALID, "band counts must be positive");
@@ -581,14 +586,14 @@ void launch_cheb_filter_small(Context& c, int nbands, int nbk, const void* V, vo
     PWK_REQUIRE(c.vrn64.bytes >= slots * c.N * sizeof(double), PWK_E_STATE, "V_eff slots not set");
   }
   if (precision == PWK_FP64)
-    launch_filter_small_t<double>(c, nbands, nbk, V, Yout, kin, nch, B, DBH, coef, degree, nkv);
+    launch_filter_small_t<double>(c, nbands, nbk, V, Yout, kin, nch, B, DBH, coef, degree, nkv, skip);
   else
-    launch_filter_small_t<float>(c, nbands, nbk, V, Yout, kin, nch, B, DBH, coef, degree, nkv);
+    launch_filter_small_t<float>(c, nbands, nbk, V, Yout, kin, nch, B, DBH, coef, degree, nkv, skip);
 }
 
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,
                        int nch, const void* B, const void* DBH, const double* coef, const void* X, int nkv,
-                       const int* band_k) {
+                       const int* band_k, const int* skip) {
   c.check_basis();
   PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
   PWK_REQUIRE(nbands > 0 && nbk > 0 && nbands % nbk == 0, PWK_E_INVALID, "band counts must be positive");
@@ -599,9 +604,9 @@ void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* h
   }
   PWK_REQUIRE(!band_k || (!coef && nkv == 0), PWK_E_INVALID, "a band table excludes filter coefficients");
   if (precision == PWK_FP64)
-    launch_small_t<double>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv, band_k);
+    launch_small_t<double>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv, band_k, skip);
   else
-    launch_small_t<float>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv, band_k);
+    launch_small_t<float>(c, nbands, nbk, psi, hpsi, kin, nch, B, DBH, coef, X, nkv, band_k, skip);
 }
 
 }  // namespace pwk

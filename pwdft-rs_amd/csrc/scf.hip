@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <set>
+#include <unordered_map>
 
 #include "common.hpp"
 #include "tables.hpp"
@@ -48,7 +49,7 @@ struct RadialArgs {
   int dims[3];
   double b[9];  // reciprocal vectors, rows
   double omega, sad_sigma;
-  int nspecies, natoms;
+  int nspecies, natoms, nshell;
   int off[MAXSP], mesh[MAXSP];
   double z[MAXSP], v0[MAXSP];
   int has_core[MAXSP], has_rho[MAXSP];
@@ -58,6 +59,9 @@ struct RadialArgs {
   const double* wr;  // Simpson weight · rab · 4πr²ρ_atom
   const double* tau;  // natoms × 3, Cartesian Å
   const int* sp;
+  const double* shell_g2;  // |G|² of each shell
+  const int* shell;        // shell of each grid point
+  double* tab;             // nshell × nspecies × (V_loc, ρ_core, ρ_atom) form factors
   double2* vloc;
   double2* core;
   double2* sad;
@@ -65,26 +69,22 @@ struct RadialArgs {
 
 __device__ inline double sinc_d(double x) { return x < 1e-10 ? 1.0 - x * x / 6.0 : sin(x) / x; }
 
-// One wave per grid point: the lanes split the radial mesh (and the atoms) and the partial
-// sums meet in a butterfly, so the whole device works on the mesh integrals.
+// One wave per grid point or shell: the lanes split the radial mesh (and the atoms) and the
+// partial sums meet in a butterfly, so the whole device works on the mesh integrals.
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
-__global__ void __launch_bounds__(256) k_radial(RadialArgs a) {
-  const size_t idx = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+// The radial form factors of every species at every distinct |G| (one wave per shell): the
+// transforms depend on |G| only, so they are evaluated once per shell of equal |G|² instead of
+// once per grid point (src/scf/potentials.rs:18-51 evaluates v_local_of_g per point).
+__global__ void __launch_bounds__(256) k_radial_shell(RadialArgs a) {
+  const int sh = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (idx >= a.n) return;  // wave-uniform
-  const int nx = a.dims[0], ny = a.dims[1], nz = a.dims[2];
-  const int i1 = (int)(idx / ((size_t)ny * nz)), i2 = (int)((idx / nz) % ny), i3 = (int)(idx % nz);
-  const double n1 = i1 > nx / 2 ? i1 - nx : i1, n2 = i2 > ny / 2 ? i2 - ny : i2, n3 = i3 > nz / 2 ? i3 - nz : i3;
-  const double gx = (n1 * a.b[0] + n2 * a.b[3]) + n3 * a.b[6];
-  const double gy = (n1 * a.b[1] + n2 * a.b[4]) + n3 * a.b[7];
-  const double gz = (n1 * a.b[2] + n2 * a.b[5]) + n3 * a.b[8];
-  const double gn = sqrt(gx * gx + gy * gy + gz * gz);
+  if (sh >= a.nshell) return;  // wave-uniform
+  const double g2 = a.shell_g2[sh], gn = sqrt(g2);
   const double fourpi = 4.0 * 3.141592653589793;
-  double2 vl = make_double2(0.0, 0.0), co = vl, sa = vl;
   for (int s = 0; s < a.nspecies; ++s) {
     const double* r = a.r + a.off[s];
     const double* wv = a.wv + a.off[s];
@@ -100,28 +100,51 @@ __global__ void __launch_bounds__(256) k_radial(RadialArgs a) {
     iv = wave_sum(iv);
     ic = wave_sum(ic);
     ir = wave_sum(ir);
-    double vg;
-    if (gn < 1e-12) {
-      vg = a.v0[s];
-    } else {
-      const double g2 = gn * gn;
-      vg = fourpi / a.omega * (iv / 3.0) - fourpi * a.z[s] * 14.399645351950548 * exp(-g2 / 4.0) / (a.omega * g2);
+    if (lane == 0) {
+      double vg;
+      if (gn < 1e-12) {
+        vg = a.v0[s];
+      } else {
+        vg = fourpi / a.omega * (iv / 3.0) - fourpi * a.z[s] * 14.399645351950548 * exp(-g2 / 4.0) / (a.omega * g2);
+      }
+      const double cg = a.has_core[s] ? fourpi * (ic / 3.0) / a.omega : 0.0;
+      const double sg = a.has_rho[s] ? (ir / 3.0) / a.omega
+                                     : a.z[s] * exp(-0.5 * g2 * a.sad_sigma * a.sad_sigma) / a.omega;
+      double* t = a.tab + ((size_t)sh * a.nspecies + s) * 3;
+      t[0] = vg;
+      t[1] = cg;
+      t[2] = sg;
     }
-    const double cg = a.has_core[s] ? fourpi * (ic / 3.0) / a.omega : 0.0;
-    const double sg = a.has_rho[s] ? (ir / 3.0) / a.omega
-                                   : a.z[s] * exp(-0.5 * gn * gn * a.sad_sigma * a.sad_sigma) / a.omega;
-    for (int at = lane; at < a.natoms; at += 64) {
-      if (a.sp[at] != s) continue;
-      const double ph = -(gx * a.tau[3 * at] + gy * a.tau[3 * at + 1] + gz * a.tau[3 * at + 2]);
-      double sn, cs;
-      sincos(ph, &sn, &cs);
-      vl.x += cs * vg;
-      vl.y += sn * vg;
-      co.x += cs * cg;
-      co.y += sn * cg;
-      sa.x += cs * sg;
-      sa.y += sn * sg;
-    }
+  }
+}
+
+// V_loc(G), ρ_core(G) and the atomic start density (src/scf/initial_density.rs:64-166) on the
+// grid: Σ_atoms e^{-iG·τ} × the form factor of the point's shell, one wave per grid point with
+// the lanes over the atoms.
+__global__ void __launch_bounds__(256) k_radial(RadialArgs a) {
+  const size_t idx = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (idx >= a.n) return;  // wave-uniform
+  const int nx = a.dims[0], ny = a.dims[1], nz = a.dims[2];
+  const int i1 = (int)(idx / ((size_t)ny * nz)), i2 = (int)((idx / nz) % ny), i3 = (int)(idx % nz);
+  const double n1 = i1 > nx / 2 ? i1 - nx : i1, n2 = i2 > ny / 2 ? i2 - ny : i2, n3 = i3 > nz / 2 ? i3 - nz : i3;
+  const double gx = (n1 * a.b[0] + n2 * a.b[3]) + n3 * a.b[6];
+  const double gy = (n1 * a.b[1] + n2 * a.b[4]) + n3 * a.b[7];
+  const double gz = (n1 * a.b[2] + n2 * a.b[5]) + n3 * a.b[8];
+  const double* tab = a.tab + (size_t)a.shell[idx] * a.nspecies * 3;
+  double2 vl = make_double2(0.0, 0.0), co = vl, sa = vl;
+  for (int at = lane; at < a.natoms; at += 64) {
+    const double* t = tab + (size_t)a.sp[at] * 3;
+    const double vg = t[0], cg = t[1], sg = t[2];
+    const double ph = -(gx * a.tau[3 * at] + gy * a.tau[3 * at + 1] + gz * a.tau[3 * at + 2]);
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    vl.x += cs * vg;
+    vl.y += sn * vg;
+    co.x += cs * cg;
+    co.y += sn * cg;
+    sa.x += cs * sg;
+    sa.y += sn * sg;
   }
   vl = make_double2(wave_sum(vl.x), wave_sum(vl.y));
   co = make_double2(wave_sum(co.x), wave_sum(co.y));
@@ -438,6 +461,411 @@ __global__ void k_anderson_combine(size_t n, MixArgs a, double* __restrict__ out
 }
 
 // ---------------------------------------------------------------------------------------
+// Device-resident SCF iteration: the scalars of an iteration (reduction results, Fermi
+// level, band energy, -TS, electrons per channel, mixing coefficients) live in one device
+// array S, and the decisions the host path takes between launches (another filter pass or
+// not, the filter coefficients, occupations, the DIIS solve, the history slot) are taken by
+// kernels, so an iteration is a fixed sequence of launches with one host synchronisation at
+// its end (and can be captured and replayed as a HIP graph).
+// ---------------------------------------------------------------------------------------
+constexpr int S_HAR = 0;     // Harris-side reductions (up to 4 terms)
+constexpr int S_INT = 4;     // density integral per channel (2)
+constexpr int S_OUT = 8;     // output-density reductions (up to 8 terms)
+constexpr int S_EF = 16;     // Fermi level, then the level of each channel (3)
+constexpr int S_EBAND = 19;  // Σ f w ε
+constexpr int S_TS = 20;     // TS of the smearing
+constexpr int S_NS = 21;     // electrons per channel (2)
+constexpr int S_DELTA = 23;  // RMS Δρ of the latest iteration (filter target)
+constexpr int S_PASSES = 24; // filter passes run in the iteration
+constexpr int S_PASSED = 25; // 1 once the residual gate has passed in the iteration
+constexpr int S_MIXC = 32;   // mixing coefficients, two mixers × MAX_HIST
+constexpr int S_GRAM = S_MIXC + 2 * MAX_HIST;  // DIIS sums, two mixers × MAX_DOTS
+constexpr int S_SIZE = S_GRAM + 2 * MAX_DOTS;
+
+// Second stage of the fixed-order reduction: out[t] = Σ_b partial[t][b] in block order (the
+// order the host path adds them in).
+__global__ void k_dots_final(int nterms, int nblk, const double* __restrict__ partial, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nterms) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(size_t)t * nblk + b];
+  out[t] = s;
+}
+
+// flag[0]: the residual gate has passed in this iteration (the remaining passes do nothing)
+__global__ void k_iter_begin(int* flag, double* S) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    flag[0] = 0;
+    S[S_PASSES] = 0.0;
+    S[S_PASSED] = 0.0;
+  }
+}
+
+__device__ double wg_sum(double v, double* sh) {  // deterministic sum over a 256-thread workgroup
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+__device__ double wg_max(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// After a Rayleigh-Ritz step (one workgroup): with gate set, the residual gate of the pass that
+// ran (the largest residual norm of the nb bands used for occupations against the target 0.1·Δρ
+// of the latest iteration, clamped to [1e-9, 1e-3]; once it passes, the remaining passes of the
+// iteration do nothing) and the pass count; then, unless the gate has passed, the coefficients
+// of the next pass from the current Ritz values (the host path's chebyshev_pass): per block k the
+// interval [a, b] = [highest kept Ritz value, spectrum bound], scaled at the lowest Ritz value,
+// and the three-term coefficients of every step.
+__global__ void __launch_bounds__(256) k_pass_gate(int nke, int nbs, int nb, int degree, int gate,
+                                                   const double* __restrict__ lam, const double* __restrict__ res,
+                                                   const double* __restrict__ ub, double* __restrict__ S,
+                                                   int* __restrict__ flag, double* __restrict__ coef) {
+  __shared__ double sh[4];
+  const int done = flag[0];
+  __syncthreads();  // every thread has read the flag before thread 0 may set it below
+  if (done) return;  // uniform
+  bool passed = false;
+  if (gate) {
+    double w = 0.0;
+    for (int i = threadIdx.x; i < nke * nb; i += 256) w = fmax(w, res[(size_t)(i / nb) * nbs + i % nb]);
+    w = wg_max(w, sh);
+    passed = w < fmax(1e-9, fmin(1e-3, 0.1 * S[S_DELTA]));  // the same on every thread
+    if (threadIdx.x == 0) {
+      S[S_PASSES] += 1.0;
+      if (passed) {
+        flag[0] = 1;
+        S[S_PASSED] = 1.0;
+      }
+    }
+  }
+  // the coefficients are computed also when the gate has passed: the next iteration's first
+  // pass uses them
+  for (int k = threadIdx.x; k < nke; k += 256) {
+    const double* l = lam + (size_t)k * nbs;
+    const double a0 = l[0];
+    double a = l[nbs - 1], b = ub[k];
+    if (!(b > a + 1e-6)) b = a + fmax(1.0, fabs(a));
+    if (!(a > a0)) a = a0 + 1e-3 * (b - a0);
+    const double e = 0.5 * (b - a), cc = 0.5 * (b + a);
+    double sigma = e / (a0 - cc);
+    const double tau = 2.0 / sigma;
+    coef[(size_t)3 * k] = sigma / e;
+    coef[(size_t)3 * k + 1] = cc;
+    coef[(size_t)3 * k + 2] = 0.0;
+    for (int i = 1; i < degree; ++i) {
+      const double sn = 1.0 / (tau - sigma);
+      double* cf = coef + ((size_t)i * nke + k) * 3;
+      cf[0] = 2.0 * sn / e;
+      cf[1] = cc;
+      cf[2] = sigma * sn;
+      sigma = sn;
+    }
+  }
+}
+
+// src/scf/smearing.rs:112-130
+__device__ inline double fd_occ_dev(double e, double ef, double sigma) {
+  if (sigma < 1e-15) return e < ef ? 1.0 : (fabs(e - ef) < 1e-12 ? 0.5 : 0.0);
+  const double x = (e - ef) / sigma;
+  if (x > 40.0) return 0.0;
+  if (x < -40.0) return 1.0;
+  return 1.0 / (1.0 + exp(x));
+}
+
+struct FermiArgs {
+  int nk, nb, nbs, ns, fixed;  // Ritz values of channel s, k-point k at lam[(s·nk + k)·nbs + b]
+  double n_el, sigma, spin, tot_mag;
+  const double* lam;
+  const double* wk;
+  double* S;
+  double* occ;  // ns·nk·nb occupations f·spin
+  double* fw;   // ns·nk·nbs density weights f·spin·w_k (0 on the buffer bands)
+  double* ev;   // ns·nk·nb eigenvalues
+};
+
+// Fermi level over the entries [e0, e1) of the (channel, k, band) list: the electron count
+// (src/scf/smearing.rs:49-90) bracketed from the lowest and highest eigenvalue padded by
+// 10·max(σ, 0.1), then narrowed by multisection until the bracket is below 1e-14 wide (the
+// bisection's stopping width), 16 probes per round instead of one: round r evaluates the count
+// at lo + (j + 1)(hi − lo)/17, j = 0..15, one group of 16 lanes per probe, and keeps the
+// sub-interval where the count crosses the target. Every reduction is a fixed shuffle tree, so
+// the result does not depend on scheduling.
+__device__ double fermi_bisect(const FermiArgs& a, int e0, int e1, double target, double* sh) {
+  const int nb = a.nb;
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = e0 + (int)threadIdx.x; i < e1; i += 256) {
+    const double e = a.lam[(size_t)(i / nb) * a.nbs + i % nb];
+    lo = fmin(lo, e);
+    hi = fmax(hi, e);
+  }
+  lo = -wg_max(-lo, sh);
+  hi = wg_max(hi, sh);
+  const double pad = 10.0 * fmax(a.sigma, 0.1);
+  lo -= pad;
+  hi += pad;
+  __shared__ double cnt_sh[16];
+  const int probe = threadIdx.x >> 4, sub = threadIdx.x & 15;
+  for (int round = 0; round < 64 && !(fabs(hi - lo) < 1e-14); ++round) {
+    const double step = (hi - lo) / 17.0;
+    const double mid = lo + (double)(probe + 1) * step;
+    double cnt = 0.0;
+    for (int i = e0 + sub; i < e1; i += 16) {
+      const int kk = i / nb;
+      cnt += a.wk[kk % a.nk] * (fd_occ_dev(a.lam[(size_t)kk * a.nbs + i % nb], mid, a.sigma) * a.spin);
+    }
+    for (int o = 8; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 16);
+    __syncthreads();
+    if (sub == 0) cnt_sh[probe] = cnt;
+    __syncthreads();
+    // the first probe whose count reaches the target bounds the level from above
+    int j = 16;
+    for (int q = 15; q >= 0; --q)
+      if (!(cnt_sh[q] < target)) j = q;
+    const double nlo = j == 0 ? lo : lo + (double)j * step;
+    const double nhi = j == 16 ? hi : lo + (double)(j + 1) * step;
+    lo = nlo;
+    hi = nhi;
+  }
+  __syncthreads();
+  return 0.5 * (lo + hi);
+}
+
+// Fermi level(s), occupations, the density weights, the band energy, TS and the electrons per
+// channel (src/scf/smearing.rs:49-90, 202-223; src/scf/driver_spin.rs:407-474 for the two
+// levels of a fixed moment), one workgroup.
+__global__ void __launch_bounds__(256) k_fermi(FermiArgs a) {
+  __shared__ double sh[4];
+  const int per = a.nk * a.nb, tot = a.ns * per;
+  double ef, efs[2];
+  if (a.ns == 2 && a.fixed) {
+    efs[0] = fermi_bisect(a, 0, per, 0.5 * (a.n_el + a.tot_mag), sh);
+    efs[1] = fermi_bisect(a, per, tot, (a.n_el - a.tot_mag) / 2.0, sh);
+    ef = 0.5 * (efs[0] + efs[1]);
+  } else {
+    ef = fermi_bisect(a, 0, tot, a.n_el, sh);
+    efs[0] = efs[1] = ef;
+  }
+  double eb = 0.0, ts = 0.0, n0 = 0.0, n1 = 0.0;
+  for (int i = threadIdx.x; i < tot; i += 256) {
+    const int kk = i / a.nb, b = i % a.nb, s = kk / a.nk;
+    const double e = a.lam[(size_t)kk * a.nbs + b], w = a.wk[kk % a.nk];
+    const double f = fd_occ_dev(e, efs[s], a.sigma) * a.spin;
+    a.occ[i] = f;
+    a.ev[i] = e;
+    eb += f * w * e;
+    (s == 0 ? n0 : n1) += f * w;
+    if (a.sigma >= 1e-15) {
+      const double x = (e - ef) / a.sigma;
+      if (fabs(x) <= 30.0) {
+        double g = 1.0 / (1.0 + exp(x));
+        g = fmin(fmax(g, 1e-30), 1.0 - 1e-30);
+        ts += w * (-(g * log(g) + (1.0 - g) * log(1.0 - g)));
+      }
+    }
+  }
+  // density weights: each thread evaluates its own entries (the same arithmetic as occ above),
+  // so no thread reads what another one wrote
+  for (int i = threadIdx.x; i < a.ns * a.nk * a.nbs; i += 256) {
+    const int kk = i / a.nbs, b = i % a.nbs, s = kk / a.nk;
+    a.fw[i] = b < a.nb ? (fd_occ_dev(a.lam[(size_t)kk * a.nbs + b], efs[s], a.sigma) * a.spin) * a.wk[kk % a.nk]
+                       : 0.0;
+  }
+  eb = wg_sum(eb, sh);
+  ts = wg_sum(ts, sh);
+  n0 = wg_sum(n0, sh);
+  n1 = wg_sum(n1, sh);
+  if (threadIdx.x == 0) {
+    a.S[S_EF] = ef;
+    a.S[S_EF + 1] = efs[0];
+    a.S[S_EF + 2] = efs[1];
+    a.S[S_EBAND] = eb;
+    a.S[S_TS] = a.sigma < 1e-15 ? 0.0 : ts * a.sigma * a.spin;
+    a.S[S_NS] = n0;
+    a.S[S_NS + 1] = n1;
+  }
+}
+
+// ρ *= target / (∫ρ) when |∫ρ| > 1e-15 (src/scf/density.rs:99-108); the integral is
+// S[i_int]·dvol, the target n_el or the channel's electron count S[i_target]
+__global__ void k_scale_to(size_t n, double* __restrict__ rho, const double* __restrict__ S, int i_int, double dvol,
+                           int i_target, double n_el) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double integral = S[i_int] * dvol;
+  if (!(fabs(integral) > 1e-15)) return;
+  const double target = i_target >= 0 ? S[i_target] : n_el;
+  rho[i] *= target / integral;
+}
+
+// the latest RMS Δρ (max over channels) for the next iteration's filter target
+__global__ void k_iter_end(int ns, double dvol, double omega, double* S) {
+  if (threadIdx.x || blockIdx.x) return;
+  double d = sqrt(S[S_OUT + 3] * dvol / omega);
+  if (ns == 2) d = fmax(sqrt(S[S_OUT + 4] * dvol / omega), sqrt(S[S_OUT + 5] * dvol / omega));
+  S[S_DELTA] = d;
+}
+
+// Anderson history on the device (src/scf/mixing/anderson.rs:112-140): slot order, oldest first
+struct MixState {
+  int m, next;
+  int order[MAX_HIST];
+};
+
+__global__ void k_hist_advance(MixState* st, int ndim) {
+  if (threadIdx.x || blockIdx.x) return;
+  int slot;
+  if (st->m < ndim) {
+    slot = st->next++;
+    st->order[st->m++] = slot;
+  } else {
+    slot = st->order[0];
+    for (int j = 0; j + 1 < ndim; ++j) st->order[j] = st->order[j + 1];
+    st->order[ndim - 1] = slot;
+  }
+}
+
+// the new entry: input and residual (residual from `res`, or out − in when res is null)
+__global__ void k_hist_put(size_t n, const double* __restrict__ in, const double* __restrict__ out,
+                           const double* __restrict__ res, double* __restrict__ hist, const MixState* __restrict__ st) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double* hin = hist + (size_t)st->order[st->m - 1] * 2 * n;
+  const double x = in[i];
+  hin[i] = x;
+  hin[n + i] = res ? res[i] : out[i] - x;
+}
+
+__global__ void k_resid(size_t n, const double* __restrict__ in, const double* __restrict__ out,
+                        double* __restrict__ r) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = out[i] - in[i];
+}
+
+// DIIS sums of one Anderson step for the largest history: term t < M (M = ndim − 1) is
+// Σ (r_t − r_l)·r_l, then the upper triangle Σ (r_i − r_l)(r_j − r_l), i ≤ j < M, row by row;
+// terms beyond the current history are zero. Same per-element products and block partials as
+// k_dots on the host path.
+__global__ void __launch_bounds__(RED_NT) k_gram(size_t n, int M, const double* __restrict__ hist,
+                                                 const MixState* __restrict__ st, double* __restrict__ partial) {
+  __shared__ double sh[RED_NT / 64];
+  const int t = blockIdx.y, mm = st->m - 1;
+  int i, j;
+  bool bterm = t < M;
+  if (bterm) {
+    i = t;
+    j = -1;
+  } else {
+    int u = t - M;
+    i = 0;
+    while (u >= M - i) {
+      u -= M - i;
+      ++i;
+    }
+    j = i + u;
+  }
+  double acc = 0.0;
+  if (i < mm && j < mm) {
+    const double* rl = hist + (size_t)st->order[mm] * 2 * n + n;
+    const double* ri = hist + (size_t)st->order[i] * 2 * n + n;
+    const double* rj = bterm ? nullptr : hist + (size_t)st->order[j] * 2 * n + n;
+    for (size_t e = (size_t)blockIdx.x * RED_NT + threadIdx.x; e < n; e += (size_t)gridDim.x * RED_NT) {
+      const double x = ri[e] - rl[e];
+      const double y = bterm ? rl[e] - 0.0 : rj[e] - rl[e];
+      acc += x * y;
+    }
+  }
+  const double s = block_reduce(acc, sh);
+  if (threadIdx.x == 0) partial[(size_t)t * gridDim.x + blockIdx.x] = s;
+}
+
+// src/scf/mixing/linalg.rs:24-52 (the host path's small_solve), one thread
+__device__ void small_solve_dev(double* A, double* b, int n, double* x) {
+#pragma clang fp contract(off)
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (fabs(A[r * n + c]) > fabs(A[p * n + c])) p = r;
+    if (p != c) {
+      for (int j = 0; j < n; ++j) {
+        const double t = A[c * n + j];
+        A[c * n + j] = A[p * n + j];
+        A[p * n + j] = t;
+      }
+      const double t = b[c];
+      b[c] = b[p];
+      b[p] = t;
+    }
+    if (fabs(A[c * n + c]) < 1e-15) {
+      for (int i = 0; i < n; ++i) x[i] = 1.0 / (double)(n + 1);
+      return;
+    }
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[r * n + c] / A[c * n + c];
+      for (int j = c; j < n; ++j) A[r * n + j] -= f * A[c * n + j];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= A[i * n + j] * x[j];
+    x[i] = s / A[i * n + i];
+  }
+}
+
+// The Anderson coefficients of the current history from the DIIS sums (anderson.rs:170-203)
+__global__ void k_diis(const MixState* __restrict__ st, int M, const double* __restrict__ gram,
+                       double* __restrict__ coef) {
+#pragma clang fp contract(off)
+  __shared__ double A[(MAX_HIST - 1) * (MAX_HIST - 1)], b[MAX_HIST], x[MAX_HIST];
+  if (threadIdx.x || blockIdx.x) return;
+  const int m = st->m;
+  if (m < 2) {
+    coef[0] = 1.0;
+    return;
+  }
+  const int mm = m - 1;
+  for (int i = 0; i < mm; ++i) b[i] = -gram[i];
+  for (int i = 0; i < mm; ++i)
+    for (int j = i; j < mm; ++j) {
+      const double v = gram[M + i * M - i * (i - 1) / 2 + (j - i)];
+      A[i * mm + j] = v;
+      A[j * mm + i] = v;
+    }
+  small_solve_dev(A, b, mm, x);
+  double asum = 0.0;
+  for (int j = 0; j < mm; ++j) asum += x[j];
+  for (int j = 0; j < mm; ++j) coef[j] = x[j];
+  coef[m - 1] = 1.0 - asum;
+}
+
+// out = c_last·(in_l + β r_l) + Σ_j α_j (in_j + β r_j) over the device history
+__global__ void k_anderson_dev(size_t n, double beta, const double* __restrict__ hist, const MixState* __restrict__ st,
+                               const double* __restrict__ coef, double* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int last = st->m - 1;
+  const double* hl = hist + (size_t)st->order[last] * 2 * n;
+  double v = coef[last] * (hl[i] + beta * hl[n + i]);
+  for (int j = 0; j < last; ++j) {
+    const double* hj = hist + (size_t)st->order[j] * 2 * n;
+    v += coef[j] * (hj[i] + beta * hj[n + i]);
+  }
+  out[i] = v;
+}
+
+__global__ void k_publish(int n, const double* __restrict__ S, double* __restrict__ host) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) host[i] = S[i];
+}
+
+// ---------------------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------------------
 std::vector<SymOp> identity_ops() {
@@ -602,6 +1030,7 @@ struct Solve {
   Solve(Context& cc, const pwk_crystal& a, const pwk_scf_params& b, pwk_scf_result& r) : c(cc), cr(a), p(b), res(r) {}
   ~Solve() {
     for (auto& kv : pass_graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : iter_graphs) (void)hipGraphExecDestroy(kv.second);
   }
 
   unsigned gb(size_t n) const { return blocks_for(n, 256); }
@@ -699,6 +1128,36 @@ struct Solve {
   void xc_spin(const double* up, const double* dn);
   void mix(Mixer& m, const double* in, const double* out, double* result);
   void run(double* eig_out, double* rho_out);
+
+  // stages shared by the host-synchronised and the device-resident iteration
+  struct IterScalars {
+    double eh_in, exc_in, e_h_out, exc_int, evxc, delta, e_loc, charge;
+  };
+  const double* rho_tot_new = nullptr;  // output total density of the latest iteration
+  std::vector<DotTerm> veff_stage();
+  std::vector<DotTerm> output_stage();
+  void symmetrize(double* rn);
+  IterScalars scalars(const double* har, const double* out) const;
+  void finalize(const std::vector<double>& occ, const IterScalars& v, double e_total, double e_harris, double ef,
+                double ts, double e_band, const double* n_s);
+  void run_host(double e_ewald, std::vector<double>& ev);
+
+  // device-resident iteration (filtered eigensolver on small cubic grids)
+  DevBuf dS, dpart, dflag, dmix, lam_dev, ub_dev, wk_dev, fw_dev, occ_dev, ev_dev, rtmp;
+  PinnedBuf pin_S;
+  double* hS = nullptr;                        // pinned copy of the iteration scalars
+  std::map<int, hipGraphExec_t> iter_graphs;   // captured iterations per filter precision
+  std::set<int> iter_warm;
+  bool resident_supported() const;
+  void dots_dev(const std::vector<DotTerm>& terms, size_t n, int off);
+  void mix_dev(int which, const double* in, const double* out, double* result);
+  void pass_dev(int fp);
+  void gate_dev(bool gate);
+  int pass_budget = 6;  // filter passes launched per iteration after the first
+  void resident_body(int fp, int max_passes);
+  void resident_tail();
+  void launch_iteration(int fp, int passes);
+  void run_resident(double e_ewald, std::vector<double>& ev);
 };
 
 void Solve::setup() {
@@ -836,7 +1295,40 @@ void Solve::setup() {
     tau[(size_t)3 * a + 2] = t.z;
     spi[(size_t)a] = cr.species[a];
   }
-  DevBuf dr, dwv, dwc, dwr, dtau, dsp, core_g, sad_g;
+  // shells of equal |G|² (the arithmetic of the grid kernels, in grid order of first appearance)
+  std::vector<double> shell_g2;
+  std::vector<int> shell_of((size_t)N);
+  {
+    std::unordered_map<uint64_t, int> seen;
+    seen.reserve(256);
+    for (size_t idx = 0; idx < N; ++idx) {
+      const int i1 = (int)(idx / ((size_t)dims[1] * dims[2])), i2 = (int)((idx / dims[2]) % dims[1]),
+                i3 = (int)(idx % dims[2]);
+      const double n1 = i1 > dims[0] / 2 ? i1 - dims[0] : i1, n2 = i2 > dims[1] / 2 ? i2 - dims[1] : i2,
+                   n3 = i3 > dims[2] / 2 ? i3 - dims[2] : i3;
+      double g[3];
+      for (int d = 0; d < 3; ++d) g[d] = (n1 * ra.b[d] + n2 * ra.b[3 + d]) + n3 * ra.b[6 + d];
+      const double g2 = (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
+      uint64_t key;
+      std::memcpy(&key, &g2, 8);
+      auto it = seen.find(key);
+      if (it == seen.end()) {
+        it = seen.emplace(key, (int)shell_g2.size()).first;
+        shell_g2.push_back(g2);
+      }
+      shell_of[idx] = it->second;
+    }
+  }
+  ra.nshell = (int)shell_g2.size();
+  DevBuf dr, dwv, dwc, dwr, dtau, dsp, core_g, sad_g, dshg2, dshell, dtab;
+  dshg2.ensure(shell_g2.size() * 8);
+  dshell.ensure(shell_of.size() * 4);
+  dtab.ensure(shell_g2.size() * (size_t)cr.nspecies * 3 * 8);
+  PWK_HIP(hipMemcpyAsync(dshg2.p, shell_g2.data(), shell_g2.size() * 8, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(dshell.p, shell_of.data(), shell_of.size() * 4, hipMemcpyHostToDevice, c.stream));
+  ra.shell_g2 = dshg2.as<double>();
+  ra.shell = dshell.as<int>();
+  ra.tab = dtab.as<double>();
   dr.ensure(hr.size() * 8);
   dwv.ensure(hr.size() * 8);
   dwc.ensure(hr.size() * 8);
@@ -863,6 +1355,7 @@ void Solve::setup() {
   ra.sad = sad_g.as<double2>();
   {
     TimedScope ts(c, "scf_radial");
+    k_radial_shell<<<blocks_for((size_t)ra.nshell, 4), 256, 0, c.stream>>>(ra);
     k_radial<<<blocks_for(N, 4), 256, 0, c.stream>>>(ra);
     PWK_HIP(hipGetLastError());
   }
@@ -1008,6 +1501,9 @@ void Solve::setup() {
     PWK_REQUIRE(heev_supported(nbs, nbs), PWK_E_UNSUPPORTED, "too many bands for the subspace eigensolver");
     fprec = p.precision;
     degree = p.cheb_degree > 0 ? p.cheb_degree : 10;
+    // V(r) is resident from here on: the fused-kernel checks below (and the choice of the
+    // device-resident iteration) see the buffers it goes to
+    reserve_veff_slots(c, 1);
     if (ns == 2 && fprec != PWK_SPLIT_BF16) {
       reserve_veff_slots(c, 2);  // V(r) of both channels stays resident
       fold = hpsi_small_supported(c, PWK_FP64, nch_all) && hpsi_small_supported(c, fprec, nch_all);
@@ -1434,13 +1930,168 @@ void Solve::eig_chebyshev(bool first) {
     for (int b = 0; b < nb; ++b) ev_host[(size_t)k * nb + b] = ev_sub_c()[(size_t)k * nbs + b];
 }
 
+// V_eff of every channel from the input density (src/scf/driver.rs:359-431; nspin = 2: common
+// Hartree of ρ↑ + ρ↓, LSDA per channel, src/scf/driver_spin.rs:263-339). Returns the Harris-side
+// reduction terms: the Hartree sum of ρ_in(G), ∫ρ_xc,in·ε_xc,in and ∫ρ_σ,in·v_σ,in
+// (driver_spin.rs:634-651).
+std::vector<DotTerm> Solve::veff_stage() {
+  const double* core = have_core ? core_r.as<double>() : nullptr;
+  if (ns == 1) {
+    r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
+    veff_from(rho_in.as<double>(), rho_g_in.as<double2>(), rho_xc.as<double>(), exc.as<double>(), vxc.as<double>());
+    return {DotTerm{rho_g_in.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+            DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+            DotTerm{rho_in.as<double>(), nullptr, vxc.as<double>(), nullptr, 0}};
+  }
+  k_spin_combine<<<gb(N), 256, 0, c.stream>>>(N, rho_in_s(0), rho_in_s(1), t_in.as<double>(), m_in.as<double>());
+  PWK_HIP(hipGetLastError());
+  r_to_g(t_in.as<double>(), rho_g_in.as<double2>());
+  xc_spin(rho_in_s(0), rho_in_s(1));
+  k_add_core<<<gb(N), 256, 0, c.stream>>>(N, t_in.as<double>(), core, 1.0, xc_tot.as<double>());
+  PWK_HIP(hipGetLastError());
+  for (int s = 0; s < 2; ++s) {
+    r_to_g(vxc.as<double>() + (size_t)s * N, vxc_g.as<double2>());
+    launch_hartree_veff(c, rho_g_in.p, 4.0 * kPiC * kE2, vloc_g.p, vxc_g.p, veff_g.as<double2>() + (size_t)s * N,
+                        PWK_FP64, true, true, true);
+  }
+  return {DotTerm{rho_g_in.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+          DotTerm{xc_tot.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+          DotTerm{rho_in_s(0), nullptr, vxc.as<double>(), nullptr, 0},
+          DotTerm{rho_in_s(1), nullptr, vxc.as<double>() + N, nullptr, 0}};
+}
+
+// G-space symmetrisation of one channel's density (src/symmetry/density/g_space.rs)
+void Solve::symmetrize(double* rn) {
+  if (ops.size() <= 1) return;
+  TimedScope ts(c, "scf_symmetrize");
+  launch_fft3d_r2c(c, rn, 1.0, cwork.p, -1);
+  k_sym_gather<<<blocks_for(N, 16), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
+                                                       sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
+                                                       rho_g_new.as<double2>(), 1.0 / (double)ops.size());
+  launch_fft3d(c, rho_g_new.p, 1, +1, PWK_FP64);
+  k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, rho_g_new.as<double2>(), 1.0 / (double)N, rn, 0);
+  PWK_HIP(hipGetLastError());
+}
+
+// The output-density side (driver.rs:483-560; driver_spin.rs:525-604): ρ_total of the output
+// density, its ρ(G) and xc, and the reduction terms: Hartree sum, ∫ρ_xc·ε_xc, ∫ρ_σ·v_σ, the
+// Δρ norms per channel, ∫ρ·V_loc and ∫ρ.
+std::vector<DotTerm> Solve::output_stage() {
+  const double* core = have_core ? core_r.as<double>() : nullptr;
+  if (ns == 1) {
+    rho_tot_new = rho_new.as<double>();
+    r_to_g(rho_tot_new, rho_g_new.as<double2>());
+    k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_tot_new, core, 1.0, rho_xc.as<double>());
+    launch_lda(c, rho_xc.p, exc.p, vxc.p, PWK_FP64);
+    return {DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+            DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+            DotTerm{rho_new.as<double>(), nullptr, vxc.as<double>(), nullptr, 0},
+            DotTerm{rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>(), rho_new.as<double>(), 0},
+            DotTerm{rho_new.as<double>(), nullptr, vloc_r.as<double>(), nullptr, 0},
+            DotTerm{rho_new.as<double>(), nullptr, nullptr, nullptr, 0}};
+  }
+  rho_tot_new = t_out.as<double>();
+  k_spin_combine<<<gb(N), 256, 0, c.stream>>>(N, rho_new_s(0), rho_new_s(1), t_out.as<double>(), m_out.as<double>());
+  PWK_HIP(hipGetLastError());
+  r_to_g(rho_tot_new, rho_g_new.as<double2>());
+  xc_spin(rho_new_s(0), rho_new_s(1));
+  k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_tot_new, core, 1.0, xc_tot.as<double>());
+  PWK_HIP(hipGetLastError());
+  return {DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
+          DotTerm{xc_tot.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
+          DotTerm{rho_new_s(0), nullptr, vxc.as<double>(), nullptr, 0},
+          DotTerm{rho_new_s(1), nullptr, vxc.as<double>() + N, nullptr, 0},
+          DotTerm{rho_in_s(0), rho_new_s(0), rho_in_s(0), rho_new_s(0), 0},
+          DotTerm{rho_in_s(1), rho_new_s(1), rho_in_s(1), rho_new_s(1), 0},
+          DotTerm{rho_tot_new, nullptr, vloc_r.as<double>(), nullptr, 0},
+          DotTerm{rho_tot_new, nullptr, nullptr, nullptr, 0}};
+}
+
+// Energies of one iteration from the Harris-side (har) and output-side (out) reduction results.
+Solve::IterScalars Solve::scalars(const double* har, const double* out) const {
+  const double dvol = omega / (double)N, fe = 4.0 * kPiC * kE2 * 0.5 * omega;
+  IterScalars v;
+  v.eh_in = har[0] * fe;
+  v.e_h_out = out[0] * fe;
+  v.exc_int = out[1] * dvol;
+  if (ns == 1) {
+    v.exc_in = har[1] * dvol - har[2] * dvol;
+    v.evxc = out[2] * dvol;
+    v.delta = std::sqrt(out[3] * dvol / omega);
+    v.e_loc = out[4] * dvol;
+    v.charge = out[5] * dvol;
+  } else {
+    v.exc_in = har[1] * dvol - (har[2] + har[3]) * dvol;
+    v.evxc = (out[2] + out[3]) * dvol;
+    v.delta = std::max(std::sqrt(out[4] * dvol / omega), std::sqrt(out[5] * dvol / omega));
+    v.e_loc = out[6] * dvol;
+    v.charge = out[7] * dvol;
+  }
+  return v;
+}
+
+// The converged result: the energy split, and the kinetic and nonlocal expectation values
+// summed over channels (src/scf/energy.rs:382-510), all k-points of a channel at once: batched
+// projector GEMMs, one expectation launch per k, one copy back.
+void Solve::finalize(const std::vector<double>& occ, const IterScalars& v, double e_total, double e_harris, double ef,
+                     double ts, double e_band, const double* n_s) {
+  res.converged = 1;
+  res.e_total = e_total;
+  res.e_harris = e_harris;
+  res.fermi = ef;
+  res.ts = ts;
+  res.e_band = e_band;
+  res.e_hartree = v.e_h_out;
+  res.e_xc = v.exc_int;
+  res.e_vxc = v.evxc;
+  res.e_smearing = -ts;
+  res.e_local = v.e_loc;
+  res.integrated_charge = v.charge;
+  if (ns == 2) {
+    res.n_up = n_s[0];
+    res.n_down = n_s[1];
+    res.magnetization = std::fabs(n_s[0] - n_s[1]);
+  }
+  nl_x.ensure((size_t)nk * npw * nb * 16);
+  expect.ensure((size_t)2 * nk * nb * 8);
+  if (nch_all > 0) nl_p.ensure((size_t)nk * nch_all * nb * 16);
+  std::vector<double> e2((size_t)2 * nk * nb);
+  for (int s = 0; s < ns; ++s) {
+    const double2* psi0 = V.as<double2>() + (size_t)s * vblk();
+    if (nch_all > 0) {
+      launch_zgemm_mfma_batched(c, nch_all, nb, npw, DBHall64.p, nch_all, (long long)nch_all * npw, psi0, npw,
+                                (long long)npw * nbs, nl_p.p, nch_all, (long long)nch_all * nb, nk, false, PWK_FP64);
+      launch_zgemm_mfma_batched(c, npw, nb, nch_all, Ball64.p, npw, (long long)npw * nch_all, nl_p.p, nch_all,
+                                (long long)nch_all * nb, nl_x.p, npw, (long long)npw * nb, nk, false, PWK_FP64);
+    } else {
+      PWK_HIP(hipMemsetAsync(nl_x.p, 0, (size_t)nk * npw * nb * 16, c.stream));
+    }
+    for (int k = 0; k < nk; ++k)
+      k_band_expect<<<(unsigned)nb, 256, 0, c.stream>>>(npw, psi0 + (size_t)k * npw * nbs,
+                                                       kin_all64.as<double>() + (size_t)k * npw,
+                                                       nl_x.as<double2>() + (size_t)k * npw * nb,
+                                                       expect.as<double>() + (size_t)2 * k * nb);
+    PWK_HIP(hipGetLastError());
+    PWK_HIP(hipMemcpyAsync(e2.data(), expect.p, e2.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    for (int k = 0; k < nk; ++k) {
+      double ek = 0.0, enl = 0.0;
+      for (int b = 0; b < nb; ++b) {
+        const double f = occ[((size_t)s * nk + k) * nb + b];
+        if (f == 0.0) continue;
+        ek += f * wk[(size_t)k] * e2[(size_t)2 * k * nb + 2 * b];
+        enl += f * wk[(size_t)k] * e2[(size_t)2 * k * nb + 2 * b + 1];
+      }
+      res.e_kinetic += ek;
+      res.e_nonlocal += enl;
+    }
+  }
+}
+
 void Solve::run(double* eig_out, double* rho_out) {
   const auto t0 = std::chrono::steady_clock::now();
   setup();
   const double e_ewald = ewald();
-  const double spin = ns == 2 ? 1.0 : 2.0;  // spin_factor = 2/nspin (src/scf/context.rs:188)
-  const double dvol = omega / (double)N;
-  const double fourpi_e2_half_omega = 4.0 * kPiC * kE2 * 0.5 * omega;
   const int ndim = std::max(1, p.mixing_ndim);  // at most MAX_HIST (validate_scf_request)
   for (Mixer* m : {&mix_tot, &mix_mag}) {
     m->ndim = ndim;
@@ -1450,14 +2101,6 @@ void Solve::run(double* eig_out, double* rho_out) {
   mix_tot.hist.ensure((size_t)2 * ndim * N * 8);
   mix_tot.kerker = p.mixing_mode == 1;
   if (ns == 2) mix_mag.hist.ensure((size_t)2 * ndim * N * 8);  // always plain (driver_spin.rs:76-98)
-  const double* core = have_core ? core_r.as<double>() : nullptr;
-  const int nkb = ns * nk;  // (channel, k) entries, spin-up first
-  std::vector<double> wk_all((size_t)nkb);
-  for (int e = 0; e < nkb; ++e) wk_all[(size_t)e] = wk[(size_t)(e % nk)];
-
-  std::vector<double> ev((size_t)nkb * nb);
-  bool have_prev = false;
-  double e_prev = 0.0;
   res = pwk_scf_result{};
   res.nk = nk;
   res.n_bands = nb;
@@ -1466,43 +2109,36 @@ void Solve::run(double* eig_out, double* rho_out) {
   for (int d = 0; d < 3; ++d) res.dims[d] = c.dims[d];
   res.e_ewald = e_ewald;
   res.n_ops = (int)ops.size();
+  std::vector<double> ev((size_t)ns * nk * nb);
+  if (resident_supported())
+    run_resident(e_ewald, ev);
+  else
+    run_host(e_ewald, ev);
+  if (eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);
+  if (res.converged && rho_out) {
+    PWK_HIP(hipMemcpyAsync(rho_out, rho_tot_new, N * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  }
+  c.sync();
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
 
+// The iteration with the host between the launches (every eigensolver path and grid): the
+// Fermi level, occupations and the DIIS solve on the host, reductions read back as they
+// complete.
+void Solve::run_host(double e_ewald, std::vector<double>& ev) {
+  const double spin = ns == 2 ? 1.0 : 2.0;  // spin_factor = 2/nspin (src/scf/context.rs:188)
+  const double dvol = omega / (double)N;
+  const int nkb = ns * nk;  // (channel, k) entries, spin-up first
+  std::vector<double> wk_all((size_t)nkb);
+  for (int e = 0; e < nkb; ++e) wk_all[(size_t)e] = wk[(size_t)(e % nk)];
+  bool have_prev = false;
+  double e_prev = 0.0;
   for (int iter = 0; iter < p.max_iter; ++iter) {
     // ---- V_eff per channel from the input density ---------------------------------------
-    double eh_in, exc_in;
-    if (ns == 1) {
-      r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
-      veff_from(rho_in.as<double>(), rho_g_in.as<double2>(), rho_xc.as<double>(), exc.as<double>(),
-                vxc.as<double>());
-      // input-side xc for the Harris energy: ρ_xc,in · ε_xc,in and ρ_in · v_xc,in
-      dots({DotTerm{rho_g_in.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
-            DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
-            DotTerm{rho_in.as<double>(), nullptr, vxc.as<double>(), nullptr, 0}},
-           N);
-      eh_in = host_small[0] * fourpi_e2_half_omega;
-      exc_in = host_small[1] * dvol - host_small[2] * dvol;
-    } else {
-      // common Hartree from ρ↑ + ρ↓, LSDA per channel (src/scf/driver_spin.rs:263-339)
-      k_spin_combine<<<gb(N), 256, 0, c.stream>>>(N, rho_in_s(0), rho_in_s(1), t_in.as<double>(), m_in.as<double>());
-      PWK_HIP(hipGetLastError());
-      r_to_g(t_in.as<double>(), rho_g_in.as<double2>());
-      xc_spin(rho_in_s(0), rho_in_s(1));
-      k_add_core<<<gb(N), 256, 0, c.stream>>>(N, t_in.as<double>(), core, 1.0, xc_tot.as<double>());
-      PWK_HIP(hipGetLastError());
-      for (int s = 0; s < 2; ++s) {
-        r_to_g(vxc.as<double>() + (size_t)s * N, vxc_g.as<double2>());
-        launch_hartree_veff(c, rho_g_in.p, 4.0 * kPiC * kE2, vloc_g.p, vxc_g.p, veff_g.as<double2>() + (size_t)s * N,
-                            PWK_FP64, true, true, true);
-      }
-      // Harris side (driver_spin.rs:634-651): ∫(ρ_tot,in + ρ_core)·ε_xc,in − ∫Σ_σ ρ_σ,in·v_σ,in
-      dots({DotTerm{rho_g_in.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
-            DotTerm{xc_tot.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
-            DotTerm{rho_in_s(0), nullptr, vxc.as<double>(), nullptr, 0},
-            DotTerm{rho_in_s(1), nullptr, vxc.as<double>() + N, nullptr, 0}},
-           N);
-      eh_in = host_small[0] * fourpi_e2_half_omega;
-      exc_in = host_small[1] * dvol - (host_small[2] + host_small[3]) * dvol;
-    }
+    dots(veff_stage(), N);
+    double har[4];
+    std::memcpy(har, host_small, sizeof(har));
 
     // ---- eigenpairs per channel (both channels in one pass of launches when folded) ----------
     if (fold) {
@@ -1574,67 +2210,12 @@ void Solve::run(double* eig_out, double* rho_out) {
         k_scale<<<gb(N), 256, 0, c.stream>>>(N, rn, target / integral);
         PWK_HIP(hipGetLastError());
       }
-      // ---- symmetrisation (src/symmetry/density/g_space.rs) --------------------------------
-      if (ops.size() > 1) {
-        TimedScope ts(c, "scf_symmetrize");
-        launch_fft3d_r2c(c, rn, 1.0, cwork.p, -1);
-        k_sym_gather<<<blocks_for(N, 16), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
-                                                   sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
-                                                   rho_g_new.as<double2>(), 1.0 / (double)ops.size());
-        launch_fft3d(c, rho_g_new.p, 1, +1, PWK_FP64);
-        k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, rho_g_new.as<double2>(), 1.0 / (double)N, rn, 0);
-        PWK_HIP(hipGetLastError());
-      }
+      symmetrize(rn);
     }
 
     // ---- output-density energies -------------------------------------------------------------
-    // output-side xc reuses the work arrays (input-side values were reduced above)
-    double e_h_out, exc_int, evxc, delta, e_loc, charge;
-    const double* rho_tot_new;
-    if (ns == 1) {
-      rho_tot_new = rho_new.as<double>();
-      r_to_g(rho_tot_new, rho_g_new.as<double2>());
-      k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_tot_new, core, 1.0, rho_xc.as<double>());
-      launch_lda(c, rho_xc.p, exc.p, vxc.p, PWK_FP64);
-      dots({DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
-            DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
-            DotTerm{rho_new.as<double>(), nullptr, vxc.as<double>(), nullptr, 0},
-            DotTerm{rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>(), rho_new.as<double>(), 0},
-            DotTerm{rho_new.as<double>(), nullptr, vloc_r.as<double>(), nullptr, 0},
-            DotTerm{rho_new.as<double>(), nullptr, nullptr, nullptr, 0}},
-           N);
-      e_h_out = host_small[0] * fourpi_e2_half_omega;
-      exc_int = host_small[1] * dvol;
-      evxc = host_small[2] * dvol;
-      delta = std::sqrt(host_small[3] * dvol / omega);
-      e_loc = host_small[4] * dvol;
-      charge = host_small[5] * dvol;
-    } else {
-      // driver_spin.rs:525-604: Hartree of ρ_total, LSDA on the output channels, Δρ per channel
-      rho_tot_new = t_out.as<double>();
-      k_spin_combine<<<gb(N), 256, 0, c.stream>>>(N, rho_new_s(0), rho_new_s(1), t_out.as<double>(),
-                                                  m_out.as<double>());
-      PWK_HIP(hipGetLastError());
-      r_to_g(rho_tot_new, rho_g_new.as<double2>());
-      xc_spin(rho_new_s(0), rho_new_s(1));
-      k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_tot_new, core, 1.0, xc_tot.as<double>());
-      PWK_HIP(hipGetLastError());
-      dots({DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
-            DotTerm{xc_tot.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
-            DotTerm{rho_new_s(0), nullptr, vxc.as<double>(), nullptr, 0},
-            DotTerm{rho_new_s(1), nullptr, vxc.as<double>() + N, nullptr, 0},
-            DotTerm{rho_in_s(0), rho_new_s(0), rho_in_s(0), rho_new_s(0), 0},
-            DotTerm{rho_in_s(1), rho_new_s(1), rho_in_s(1), rho_new_s(1), 0},
-            DotTerm{rho_tot_new, nullptr, vloc_r.as<double>(), nullptr, 0},
-            DotTerm{rho_tot_new, nullptr, nullptr, nullptr, 0}},
-           N);
-      e_h_out = host_small[0] * fourpi_e2_half_omega;
-      exc_int = host_small[1] * dvol;
-      evxc = (host_small[2] + host_small[3]) * dvol;
-      delta = std::max(std::sqrt(host_small[4] * dvol / omega), std::sqrt(host_small[5] * dvol / omega));
-      e_loc = host_small[6] * dvol;
-      charge = host_small[7] * dvol;
-    }
+    dots(output_stage(), N);
+    const IterScalars v = scalars(har, host_small);
     double e_band = 0.0;
     for (int e = 0; e < nkb; ++e) {
       double sk = 0.0;
@@ -1642,80 +2223,21 @@ void Solve::run(double* eig_out, double* rho_out) {
       e_band += sk;
     }
     const double ts = smearing_ts(ev, nkb, nb, wk_all, ef, p.smearing_sigma, spin);
-    const double e_xc_out = exc_int - evxc;
-    const double e_total = e_band - e_h_out + e_xc_out + e_ewald + (-ts);
-    const double e_harris = e_band - eh_in + exc_in + e_ewald + (-ts);
+    const double e_xc_out = v.exc_int - v.evxc;
+    const double e_total = e_band - v.e_h_out + e_xc_out + e_ewald + (-ts);
+    const double e_harris = e_band - v.eh_in + v.exc_in + e_ewald + (-ts);
     const double de = have_prev ? std::fabs(e_total - e_prev) : INFINITY;
     const bool e_conv = have_prev && de < p.energy_threshold;
     have_prev = true;
     e_prev = e_total;
     res.n_iter = iter + 1;
-    res.final_delta = delta;
-    last_delta = delta;
+    res.final_delta = v.delta;
+    last_delta = v.delta;
     if (p.verbose)
-      std::fprintf(stderr, "[pwk scf] iter %3d  E=%.10f  dE=%.3e  delta=%.3e  EF=%.6f%s\n", iter, e_total, de, delta,
+      std::fprintf(stderr, "[pwk scf] iter %3d  E=%.10f  dE=%.3e  delta=%.3e  EF=%.6f%s\n", iter, e_total, de, v.delta,
                    ef, ns == 2 ? (std::string("  M=") + std::to_string(n_s[0] - n_s[1])).c_str() : "");
-    if (delta < p.conv_threshold && e_conv) {
-      res.converged = 1;
-      res.e_total = e_total;
-      res.e_harris = e_harris;
-      res.fermi = ef;
-      res.ts = ts;
-      res.e_band = e_band;
-      res.e_hartree = e_h_out;
-      res.e_xc = exc_int;
-      res.e_vxc = evxc;
-      res.e_smearing = -ts;
-      res.e_local = e_loc;
-      res.integrated_charge = charge;
-      if (ns == 2) {
-        res.n_up = n_s[0];
-        res.n_down = n_s[1];
-        res.magnetization = std::fabs(n_s[0] - n_s[1]);
-      }
-      // kinetic and nonlocal expectation values, summed over channels (src/scf/energy.rs:382-510)
-      // all k-points of a channel at once: batched projector GEMMs, one expectation launch per k,
-      // one copy back
-      nl_x.ensure((size_t)nk * npw * nb * 16);
-      expect.ensure((size_t)2 * nk * nb * 8);
-      if (nch_all > 0) nl_p.ensure((size_t)nk * nch_all * nb * 16);
-      std::vector<double> e2((size_t)2 * nk * nb);
-      for (int s = 0; s < ns; ++s) {
-        const double2* psi0 = V.as<double2>() + (size_t)s * vblk();
-        if (nch_all > 0) {
-          launch_zgemm_mfma_batched(c, nch_all, nb, npw, DBHall64.p, nch_all, (long long)nch_all * npw, psi0, npw,
-                                    (long long)npw * nbs, nl_p.p, nch_all, (long long)nch_all * nb, nk, false,
-                                    PWK_FP64);
-          launch_zgemm_mfma_batched(c, npw, nb, nch_all, Ball64.p, npw, (long long)npw * nch_all, nl_p.p, nch_all,
-                                    (long long)nch_all * nb, nl_x.p, npw, (long long)npw * nb, nk, false, PWK_FP64);
-        } else {
-          PWK_HIP(hipMemsetAsync(nl_x.p, 0, (size_t)nk * npw * nb * 16, c.stream));
-        }
-        for (int k = 0; k < nk; ++k)
-          k_band_expect<<<(unsigned)nb, 256, 0, c.stream>>>(npw, psi0 + (size_t)k * npw * nbs,
-                                                           kin_all64.as<double>() + (size_t)k * npw,
-                                                           nl_x.as<double2>() + (size_t)k * npw * nb,
-                                                           expect.as<double>() + (size_t)2 * k * nb);
-        PWK_HIP(hipGetLastError());
-        PWK_HIP(hipMemcpyAsync(e2.data(), expect.p, e2.size() * 8, hipMemcpyDeviceToHost, c.stream));
-        c.sync();
-        for (int k = 0; k < nk; ++k) {
-          double ek = 0.0, enl = 0.0;
-          for (int b = 0; b < nb; ++b) {
-            const double f = occ[((size_t)s * nk + k) * nb + b];
-            if (f == 0.0) continue;
-            ek += f * wk[(size_t)k] * e2[(size_t)2 * k * nb + 2 * b];
-            enl += f * wk[(size_t)k] * e2[(size_t)2 * k * nb + 2 * b + 1];
-          }
-          res.e_kinetic += ek;
-          res.e_nonlocal += enl;
-        }
-      }
-      if (eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);
-      if (rho_out) {
-        PWK_HIP(hipMemcpyAsync(rho_out, rho_tot_new, N * 8, hipMemcpyDeviceToHost, c.stream));
-        c.sync();
-      }
+    if (v.delta < p.conv_threshold && e_conv) {
+      finalize(occ, v, e_total, e_harris, ef, ts, e_band, n_s);
       break;
     }
     // ---- mixing: ρ (nspin = 1), or ρ_total and m with separate histories (CCMX,
@@ -1729,9 +2251,286 @@ void Solve::run(double* eig_out, double* rho_out) {
       PWK_HIP(hipGetLastError());
     }
   }
-  if (!res.converged && eig_out) std::memcpy(eig_out, ev.data(), ev.size() * 8);
-  c.sync();
-  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ---------------------------------------------------------------------------------------
+// the device-resident iteration
+// ---------------------------------------------------------------------------------------
+bool Solve::resident_supported() const {
+  if (!chebyshev || !c.resident || p.verbose || fprec == PWK_SPLIT_BF16 || !(ns == 1 || fold)) return false;
+  const int dprec = p.precision == PWK_FP64 ? PWK_FP64 : PWK_FP32;
+  return c.fused_filter && cheb_filter_small_supported(c, fprec, nch_all) &&
+         cheb_filter_small_supported(c, PWK_FP64, nch_all) && hpsi_small_supported(c, PWK_FP64, nch_all) &&
+         rr_small_supported(nbs) && density_small_supported(c, dprec);
+}
+
+// Σ terms on the device into S[off ...): block partials, then their sums in block order
+void Solve::dots_dev(const std::vector<DotTerm>& terms, size_t n, int off) {
+  const int nblk = (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (n + RED_NT - 1) / RED_NT));
+  for (size_t t0 = 0; t0 < terms.size(); t0 += MAX_TERMS) {
+    const size_t nt = std::min(terms.size() - t0, (size_t)MAX_TERMS);
+    DotArgs a{};
+    a.nterms = (int)nt;
+    for (size_t i = 0; i < nt; ++i) a.t[i] = terms[t0 + i];
+    k_dots<<<dim3((unsigned)nblk, (unsigned)nt), RED_NT, 0, c.stream>>>(n, a, dpart.as<double>() + t0 * nblk);
+  }
+  k_dots_final<<<blocks_for(terms.size(), 64), 64, 0, c.stream>>>((int)terms.size(), nblk, dpart.as<double>(),
+                                                                   dS.as<double>() + off);
+  PWK_HIP(hipGetLastError());
+}
+
+// One Anderson step on the device (the host path's mix): history slot and order, the entry,
+// the DIIS sums and solve, the combination.
+void Solve::mix_dev(int which, const double* in, const double* out, double* result) {
+  TimedScope ts(c, "scf_mix");
+  Mixer& mx = which ? mix_mag : mix_tot;
+  MixState* st = dmix.as<MixState>() + which;
+  double* S = dS.as<double>();
+  k_hist_advance<<<1, 64, 0, c.stream>>>(st, mx.ndim);
+  const double* r = nullptr;
+  if (mx.kerker) {
+    k_resid<<<gb(N), 256, 0, c.stream>>>(N, in, out, rtmp.as<double>());
+    launch_fft3d_r2c(c, rtmp.as<double>(), 1.0, cwork.p, -1);
+    k_kerker<<<gb(N), 256, 0, c.stream>>>(N, c.g2_64.as<double>(), kerker_q2, cwork.as<double2>());
+    launch_fft3d(c, cwork.p, 1, +1, PWK_FP64);
+    k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, cwork.as<double2>(), 1.0 / (double)N, rtmp.as<double>(), 0);
+    r = rtmp.as<double>();
+  }
+  k_hist_put<<<gb(N), 256, 0, c.stream>>>(N, in, out, r, mx.hist.as<double>(), st);
+  const int M = mx.ndim - 1;
+  double* gram = S + S_GRAM + which * MAX_DOTS;
+  if (M > 0) {
+    const int nterms = M + M * (M + 1) / 2;
+    const int nblk = (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (N + RED_NT - 1) / RED_NT));
+    k_gram<<<dim3((unsigned)nblk, (unsigned)nterms), RED_NT, 0, c.stream>>>(N, M, mx.hist.as<double>(), st,
+                                                                            dpart.as<double>());
+    k_dots_final<<<blocks_for((size_t)nterms, 64), 64, 0, c.stream>>>(nterms, nblk, dpart.as<double>(), gram);
+  }
+  k_diis<<<1, 64, 0, c.stream>>>(st, M, gram, S + S_MIXC + which * MAX_HIST);
+  k_anderson_dev<<<gb(N), 256, 0, c.stream>>>(N, p.mixing_beta, mx.hist.as<double>(), st,
+                                              S + S_MIXC + which * MAX_HIST, result);
+  PWK_HIP(hipGetLastError());
+}
+
+// One filter pass of every block with device-side control: coefficients from the current Ritz
+// values, the one-launch filter, the FP64 H·ψ of the filtered block, Rayleigh-Ritz into the
+// device Ritz values, then the residual gate (once it has passed, the following passes of the
+// iteration launch but do nothing).
+void Solve::pass_dev(int fp) {
+  TimedScope ts(c, "scf_chebyshev_filter");
+  const bool f64 = fp == PWK_FP64;
+  const size_t nkm = (size_t)nke * nbs;
+  int* flag = dflag.as<int>();
+  const int nkv = fold ? nk : 0;
+  launch_cheb_filter_small(c, nke * nbs, nbs, Vc(), Y64.p, fp, f64 ? kin_all64.p : kin_all32.p, nch_all,
+                           f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, coef.as<double>(), degree, nkv,
+                           flag);
+  launch_hpsi_small(c, nke * nbs, nbs, Y64.p, HY64.p, PWK_FP64, kin_all64.p, nch_all, Ball64.p, DBHall64.p, nullptr,
+                    nullptr, nkv, nullptr, flag);
+  launch_rr_small(c, npw, nbs, nb, nke, Y64.p, HY64.p, Vc(), lam_dev.as<double>(), lam_dev.as<double>() + nkm, 1e-14,
+                  flag);
+  gate_dev(true);
+}
+
+// the residual gate of the pass that ran and the coefficients of the next one
+void Solve::gate_dev(bool gate) {
+  TimedScope ts(c, "scf_pass_gate");
+  const size_t nkm = (size_t)nke * nbs;
+  k_pass_gate<<<1, 256, 0, c.stream>>>(nke, nbs, nb, degree, gate ? 1 : 0, lam_dev.as<double>(),
+                                       lam_dev.as<double>() + nkm, ub_dev.as<double>(), dS.as<double>(),
+                                       dflag.as<int>(), coef.as<double>());
+  PWK_HIP(hipGetLastError());
+}
+
+// Everything of one iteration after the first: V_eff, the filter passes, Fermi level and
+// occupations, density, symmetrisation, energies, mixing; no host synchronisation.
+void Solve::resident_body(int fp, int max_passes) {
+  double* S = dS.as<double>();
+  k_iter_begin<<<1, 64, 0, c.stream>>>(dflag.as<int>(), S);
+  dots_dev(veff_stage(), N, S_HAR);
+  for (int s = 0; s < (fold ? ns : 1); ++s) set_veff_device(c, veff_g.as<double2>() + (size_t)s * N, s, false);
+  for (int pass = 0; pass < max_passes; ++pass) pass_dev(fp);
+  resident_tail();
+}
+
+// Fermi level to the end of the iteration (shared by the first iteration and the others)
+void Solve::resident_tail() {
+  double* S = dS.as<double>();
+  const double dvol = omega / (double)N;
+  FermiArgs fa{};
+  fa.nk = nk;
+  fa.nb = nb;
+  fa.nbs = nbs;
+  fa.ns = ns;
+  fa.fixed = ns == 2 && p.fixed_magnetization;
+  fa.n_el = n_el;
+  fa.sigma = p.smearing_sigma;
+  fa.spin = ns == 2 ? 1.0 : 2.0;
+  fa.tot_mag = p.tot_magnetization;
+  fa.lam = lam_dev.as<double>();
+  fa.wk = wk_dev.as<double>();
+  fa.S = S;
+  fa.occ = occ_dev.as<double>();
+  fa.fw = fw_dev.as<double>();
+  fa.ev = ev_dev.as<double>();
+  {
+    TimedScope ts(c, "scf_fermi");
+    k_fermi<<<1, 256, 0, c.stream>>>(fa);
+    PWK_HIP(hipGetLastError());
+  }
+  const int dprec = p.precision == PWK_FP64 ? PWK_FP64 : PWK_FP32;
+  if (dprec != PWK_FP64) launch_convert(c, V.p, V32.p, (size_t)ns * vblk() * 2, true);
+  for (int s = 0; s < ns; ++s) {
+    double* rn = rho_new_s(s);
+    {
+      TimedScope ts(c, "scf_density");
+      const void* psi = dprec == PWK_FP64 ? (const void*)(V.as<double2>() + (size_t)s * vblk())
+                                          : (const void*)(V32.as<float2>() + (size_t)s * vblk());
+      launch_density_small(c, nk * nbs, psi, fw_dev.as<double>() + (size_t)s * nk * nbs, rn, dprec, true);
+    }
+    dots_dev({DotTerm{rn, nullptr, nullptr, nullptr, 0}}, N, S_INT + s);
+    k_scale_to<<<gb(N), 256, 0, c.stream>>>(N, rn, S, S_INT + s, dvol, ns == 2 ? S_NS + s : -1, n_el);
+    PWK_HIP(hipGetLastError());
+    symmetrize(rn);
+  }
+  dots_dev(output_stage(), N, S_OUT);
+  k_iter_end<<<1, 64, 0, c.stream>>>(ns, dvol, omega, S);
+  if (ns == 1) {
+    mix_dev(0, rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>());
+  } else {
+    mix_dev(0, t_in.as<double>(), t_out.as<double>(), t_in.as<double>());
+    mix_dev(1, m_in.as<double>(), m_out.as<double>(), m_in.as<double>());
+    k_spin_split<<<gb(N), 256, 0, c.stream>>>(N, t_in.as<double>(), m_in.as<double>(), rho_in_s(0), rho_in_s(1));
+  }
+  k_publish<<<blocks_for((size_t)S_SIZE, 256), 256, 0, c.stream>>>(S_SIZE, S, hS);
+  PWK_HIP(hipGetLastError());
+}
+
+void Solve::run_resident(double e_ewald, std::vector<double>& ev) {
+  const size_t nkm = (size_t)nke * nbs;
+  dS.ensure((size_t)S_SIZE * 8);
+  pin_S.ensure((size_t)S_SIZE * 8);
+  hS = pin_S.as<double>();
+  dpart.ensure((size_t)MAX_DOTS * RED_BLOCKS * 8);
+  dflag.ensure(4 * sizeof(int));
+  dmix.ensure(2 * sizeof(MixState));
+  lam_dev.ensure(2 * nkm * 8);
+  ub_dev.ensure((size_t)nke * 8);
+  wk_dev.ensure((size_t)nk * 8);
+  fw_dev.ensure((size_t)ns * nk * nbs * 8);
+  occ_dev.ensure((size_t)ns * nk * nb * 8);
+  ev_dev.ensure((size_t)ns * nk * nb * 8);
+  if (mix_tot.kerker) rtmp.ensure(N * 8);
+  // zeroed scalars and history state, the filter target of the first iteration (no previous
+  // Δρ: the loosest target), the k-point weights
+  std::vector<double> s0((size_t)S_SIZE, 0.0);
+  s0[S_DELTA] = 1e300;
+  const std::vector<MixState> m0(2, MixState{});
+  PWK_HIP(hipMemcpyAsync(dS.p, s0.data(), s0.size() * 8, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(dmix.p, m0.data(), m0.size() * sizeof(MixState), hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(wk_dev.p, wk.data(), wk.size() * 8, hipMemcpyHostToDevice, c.stream));
+
+  // the first iteration: spectrum bounds (Lanczos, one host synchronisation), the Rayleigh-Ritz
+  // of the start block, up to twelve filter passes
+  double* S = dS.as<double>();
+  k_iter_begin<<<1, 64, 0, c.stream>>>(dflag.as<int>(), S);
+  dots_dev(veff_stage(), N, S_HAR);
+  for (int s = 0; s < (fold ? ns : 1); ++s) set_veff_device(c, veff_g.as<double2>() + (size_t)s * N, s, false);
+  sc = 0;
+  lanczos_bounds();
+  PWK_HIP(hipMemcpyAsync(ub_dev.p, ub.data(), (size_t)nke * 8, hipMemcpyHostToDevice, c.stream));
+  PWK_HIP(hipMemcpyAsync(Y64.p, Vc(), eblk() * 16, hipMemcpyDeviceToDevice, c.stream));
+  launch_hpsi_small(c, nke * nbs, nbs, Y64.p, HY64.p, PWK_FP64, kin_all64.p, nch_all, Ball64.p, DBHall64.p, nullptr,
+                    nullptr, fold ? nk : 0);
+  launch_rr_small(c, npw, nbs, nb, nke, Y64.p, HY64.p, Vc(), lam_dev.as<double>(), lam_dev.as<double>() + nkm, 1e-14);
+  gate_dev(false);  // coefficients of the first pass
+  // the first iteration launches directly, so its passes stop at the gate as on the host path
+  for (int pass = 0; pass < 12; ++pass) {
+    pass_dev(fprec);
+    k_publish<<<1, 64, 0, c.stream>>>(S_PASSED + 1, S, hS);
+    PWK_HIP(hipGetLastError());
+    c.sync();
+    if (hS[S_PASSED] != 0.0) break;
+  }
+  resident_tail();
+
+  bool have_prev = false;
+  double e_prev = 0.0;
+  for (int iter = 0; iter < p.max_iter; ++iter) {
+    if (iter > 0) {
+      // a reduced-precision filter hands over to FP64 once the residual target (0.1 Δρ) falls
+      // below 3e-6, as on the host path
+      const double target = std::max(1e-9, std::min(1e-3, 0.1 * last_delta));
+      const int fp = (fprec != PWK_FP64 && target < 3e-6) ? PWK_FP64 : fprec;
+      launch_iteration(fp, pass_budget);
+    }
+    c.sync();
+    // passes launched in the next iteration: one more than this one needed (at least two), all six
+    // of the host path when this iteration's gate did not pass; a pass after the gate has passed
+    // is an empty launch, so the budget follows what the iterations need
+    const int used = (int)hS[S_PASSES];
+    pass_budget = hS[S_PASSED] != 0.0 ? std::min(6, std::max(2, used + 1)) : 6;
+    passes_total += used;
+    const IterScalars v = scalars(hS + S_HAR, hS + S_OUT);
+    const double e_band = hS[S_EBAND], ts = hS[S_TS], ef = hS[S_EF];
+    const double e_xc_out = v.exc_int - v.evxc;
+    const double e_total = e_band - v.e_h_out + e_xc_out + e_ewald + (-ts);
+    const double e_harris = e_band - v.eh_in + v.exc_in + e_ewald + (-ts);
+    const double de = have_prev ? std::fabs(e_total - e_prev) : INFINITY;
+    const bool e_conv = have_prev && de < p.energy_threshold;
+    have_prev = true;
+    e_prev = e_total;
+    res.n_iter = iter + 1;
+    res.final_delta = v.delta;
+    last_delta = v.delta;
+    const bool done = v.delta < p.conv_threshold && e_conv;
+    if (done || iter + 1 == p.max_iter) {
+      std::vector<double> occ(ev.size());
+      PWK_HIP(hipMemcpyAsync(ev.data(), ev_dev.p, ev.size() * 8, hipMemcpyDeviceToHost, c.stream));
+      PWK_HIP(hipMemcpyAsync(occ.data(), occ_dev.p, occ.size() * 8, hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+      if (done) finalize(occ, v, e_total, e_harris, ef, ts, e_band, hS + S_NS);
+      break;
+    }
+  }
+}
+
+// An iteration after the first: launched directly, or replayed from a HIP graph captured
+// once per filter precision and pass budget (after a direct run has set every kernel attribute
+// and sized every buffer, so nothing is allocated while capturing).
+void Solve::launch_iteration(int fp, int passes) {
+  const bool graph = c.graphs && !c.prof.on;
+  if (!graph) {
+    resident_body(fp, passes);
+    return;
+  }
+  const int key = fp * 64 + passes;  // one graph per filter precision and pass budget
+  auto it = iter_graphs.find(key);
+  if (it != iter_graphs.end()) {
+    PWK_HIP(hipGraphLaunch(it->second, c.stream));
+    return;
+  }
+  if (!iter_warm.count(key)) {
+    iter_warm.insert(key);
+    resident_body(fp, passes);
+    return;
+  }
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  PWK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    resident_body(fp, passes);
+  } catch (...) {
+    (void)hipStreamEndCapture(c.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  PWK_HIP(hipStreamEndCapture(c.stream, &g));
+  const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  PWK_HIP(ie);
+  iter_graphs[key] = ex;
+  PWK_HIP(hipGraphLaunch(ex, c.stream));
 }
 
 }  // namespace

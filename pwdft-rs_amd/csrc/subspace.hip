@@ -196,7 +196,8 @@ struct RrLds {
 __global__ void __launch_bounds__(RR_NT) k_rr_small(int npw, int m, int nres, const double2* __restrict__ Yb,
                                                     const double2* __restrict__ HYb, double2* __restrict__ Vb,
                                                     double* __restrict__ lam_out, double* __restrict__ res_out,
-                                                    double floor_rel) {
+                                                    double floor_rel, const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a filter pass after the residual gate passed (uniform)
   extern __shared__ unsigned char smem_raw[];
   RrLds& L = *reinterpret_cast<RrLds*>(smem_raw);
   JacobiLds& J = L.J;
@@ -500,12 +501,12 @@ void subspace_rr(Context& c, int n, int nb, int nk, const void* H, const void* V
 }
 
 void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y, const void* HY, void* V,
-                     double* lam, double* res, double floor_rel) {
+                     double* lam, double* res, double floor_rel, const int* skip) {
   PWK_REQUIRE(rr_small_supported(m) && nres <= m && npw >= m && nk >= 1, PWK_E_INVALID, "bad Rayleigh-Ritz shape");
   const size_t sh = sizeof(RrLds);
   ensure_lds_limit((const void*)k_rr_small, sh);
   k_rr_small<<<(unsigned)nk, RR_NT, sh, c.stream>>>(npw, m, nres, (const double2*)Y, (const double2*)HY,
-                                                   (double2*)V, lam, res, floor_rel);
+                                                   (double2*)V, lam, res, floor_rel, skip);
   PWK_HIP(hipGetLastError());
 }
 

@@ -53,6 +53,7 @@ _SIGS = {
     "pwk_ctx_stream": (_vp, [_vp]),
     "pwk_ctx_set_fused_hpsi": (C.c_int, [_vp, C.c_int]),
     "pwk_ctx_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "pwk_ctx_set_resident": (C.c_int, [_vp, C.c_int]),
     "pwk_dev_zgemm": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_longlong, _vp, C.c_int,
                                 C.c_longlong, _vp, C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "pwk_profile_enable": (C.c_int, [_vp, C.c_int]),
@@ -205,6 +206,10 @@ class GpuAccelerator:
     def set_graphs(self, on=True):
         """Replay captured filter passes in SCF solves (opt-in) or launch kernel by kernel (default)."""
         _check(lib().pwk_ctx_set_graphs(self.h, int(on)))
+
+    def set_resident(self, on=True):
+        """Device-resident SCF iterations (default on) or the host-synchronised loop."""
+        _check(lib().pwk_ctx_set_resident(self.h, int(on)))
 
     def set_fused_hpsi(self, on=True):
         """Fused one-launch H·psi on small cubic grids: True / 1 (default) also runs each
