@@ -340,13 +340,16 @@ def fft_flops_filter(t, degree=10):
     return degree * nk * nbs * per
 
 
-def served_run(scf, deck, p, slots, warmup, steps):
+def served_run(scf, deck, p, slots, warmup, steps, batch_us=-1):
     """A closed loop of `slots` clients on a fresh server with `slots` workers: each client
     submits its next request when the previous one returns (a step is one request per client).
-    Returns solves/s over the timed steps, latencies and the results."""
+    batch_us >= 0 sets the server's cross-request pass-batching window (0: off).
+    Returns solves/s over the timed steps, latencies, the batching counts and the results."""
     import threading
 
     srv = scf.Server(0, n_slots=slots)
+    if batch_us >= 0:
+        srv.set_pass_batching(batch_us)
     lock = threading.Lock()
     lat, results, errors = [], [], []
 
@@ -372,9 +375,11 @@ def served_run(scf, deck, p, slots, warmup, steps):
     try:
         if warmup:
             run_clients(warmup, False)
+        l0, p0 = srv.pass_stats()
         t0 = time.perf_counter()
         run_clients(steps, True)
         wall = time.perf_counter() - t0
+        l1, p1 = srv.pass_stats()
     finally:
         srv.close()
     if errors:
@@ -383,7 +388,9 @@ def served_run(scf, deck, p, slots, warmup, steps):
         raise SystemExit("a served solve did not converge")
     return {"solves_per_s": len(results) / wall, "wall_s": wall, "solves": len(results),
             "p50_latency_ms": 1e3 * percentile(lat, 50), "p99_latency_ms": 1e3 * percentile(lat, 99),
-            "n_iter": results[0]["n_iter"], "results": results}
+            "n_iter": results[0]["n_iter"], "results": results,
+            "pass_batches": {"launches": l1 - l0, "solve_passes": p1 - p0,
+                             "mean_solves_per_launch": round((p1 - p0) / max(1, l1 - l0), 2)}}
 
 
 def scf_bench(args):
@@ -421,7 +428,7 @@ def scf_bench(args):
 
     # served mixed form (the headline): --slots closed-loop clients
     log(f"scf: served {args.scf_precision} form, {args.slots} slots, {args.warmup} warmup + {args.steps} steps")
-    mx = served_run(scf, deckmx, pmx, args.slots, args.warmup, args.steps)
+    mx = served_run(scf, deckmx, pmx, args.slots, args.warmup, args.steps, args.pass_batch_us)
     results = mx.pop("results")
     de = max(abs(r["e_total"] - r64["e_total"]) for r in results) / RY / natoms
     terms = ["e_band", "e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_ewald", "e_smearing"]
@@ -434,8 +441,11 @@ def scf_bench(args):
     if not args.no_decomposition and args.scf_eigensolver == "chebyshev":
         _, pf64 = small_deck("fp64", 1)
         log("scf: decomposition runs (filtered FP64 served; filtered FP64 and mixed one at a time)")
-        f64_srv = served_run(scf, deckmx, pf64, args.slots, args.warmup, args.steps)
+        f64_srv = served_run(scf, deckmx, pf64, args.slots, args.warmup, args.steps, args.pass_batch_us)
         f64_srv.pop("results")
+        # the same served mixed form with every filter pass launched alone (no cross-request batching)
+        mx_alone = served_run(scf, deckmx, pmx, args.slots, args.warmup, args.steps, 0)
+        mx_alone.pop("results")
         one = max(4, args.slots // 2)
         f64_one = served_run(scf, deckmx, pf64, 1, 2, one)
         f64_one.pop("results")
@@ -451,6 +461,8 @@ def scf_bench(args):
             "concurrency_x_fp64": round(f64_srv["solves_per_s"] / f64_one["solves_per_s"], 3),
             "precision_x_served": round(mx["solves_per_s"] / f64_srv["solves_per_s"], 3),
             "precision_x_one_at_a_time": round(mx_one["solves_per_s"] / f64_one["solves_per_s"], 3),
+            "mixed_filtered_served_unbatched": round(mx_alone["solves_per_s"], 2),
+            "pass_batching_x": round(mx["solves_per_s"] / mx_alone["solves_per_s"], 3),
             "fp64_filtered_served_latency_ms": [round(f64_srv["p50_latency_ms"], 2),
                                                 round(f64_srv["p99_latency_ms"], 2)],
             "n_iter": {"fp64_dense": r64["n_iter"], "fp64_filtered": f64_srv["n_iter"], "mixed_filtered": mx["n_iter"]},
@@ -525,6 +537,7 @@ def scf_bench(args):
         "p50_latency_ms": mx["p50_latency_ms"],
         "p99_latency_ms": mx["p99_latency_ms"],
         "solves_timed": mx["solves"],
+        "pass_batches": mx["pass_batches"],
         "fp64_gpu": {"solves_per_s": fp64_rate, "p50_latency_ms": 1e3 * percentile(lat64, 50),
                      "p99_latency_ms": 1e3 * percentile(lat64, 99), "n_iter": r64["n_iter"],
                      "e_total_eV": r64["e_total"]},
@@ -735,6 +748,9 @@ def main():
     ap.add_argument("--steps", type=int, default=4, help="timed steps; one step = --slots solves served at once")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--pass-batch-us", type=int, default=-1,
+                    help="cross-request filter-pass batching window of the served runs, microseconds "
+                         "(0: off; default: the server's own)")
     ap.add_argument("--fp64-solves", type=int, default=5)
     ap.add_argument("--scf-precision", default="fp32", choices=["fp32", "split_bf16", "fp64"])
     ap.add_argument("--scf-eigensolver", default="chebyshev", choices=["dense", "chebyshev"],

@@ -448,6 +448,20 @@ int pwk_server_submit(pwk_server* srv, const pwk_crystal* crystal, const pwk_scf
 int pwk_server_wait(pwk_server* srv, int64_t ticket, pwk_scf_result* result, double* eigenvalues, double* rho,
                     double* latency_s);
 int pwk_server_destroy(pwk_server* srv);
+/* Cross-request pass batching. Served Chebyshev-filtered solves on small
+ * cubic grids that share grid, basis size, projector channels, band counts,
+ * filter degree and filter precision run their filter passes (filter, FP64
+ * H·psi, Rayleigh-Ritz) together: the first solve to reach a pass waits up to
+ * window_us microseconds for others (or until every running solve has
+ * arrived, at most 16), then launches one kernel per step over all of them.
+ * Served solves run the host-synchronised iteration (one synchronisation per
+ * filter pass), in which each member waits for its batch on the host. The reference's k-point loop
+ * (src/scf/driver.rs:461-472) is the dimension the batch extends across
+ * requests. Results do not depend on the batching. window_us 0 runs every
+ * pass alone; the default is 200. pwk_server_pass_stats returns the number
+ * of batched launches and of solve passes they carried since creation. */
+int pwk_server_set_pass_batching(pwk_server* srv, int window_us);
+int pwk_server_pass_stats(pwk_server* srv, int64_t* launches, int64_t* passes);
 
 #ifdef __cplusplus
 }

@@ -202,6 +202,57 @@ struct KData {
   DevBuf B32, DBH32;    // float2 copies
 };
 
+// ---- filter passes of several concurrent solves in one set of launches ----------------------
+// A filter pass (filter, FP64 H·ψ, Rayleigh-Ritz; scf.hip) of every solve that shares the shape
+// below can run in one launch per kernel, a grid row per solve. The operands travel by value in
+// the kernel arguments, so a batch holds at most MAX_PASS_BATCH solves.
+constexpr int MAX_PASS_BATCH = 16;
+
+struct PassShape {
+  int fp = 0;      // filter precision (PWK_FP64 / PWK_FP32)
+  int n = 0;       // cubic grid edge
+  int npw = 0, nch = 0, nbs = 0, nb = 0, degree = 0;
+  double inv_n = 0.0;
+  bool operator==(const PassShape& o) const {
+    return fp == o.fp && n == o.n && npw == o.npw && nch == o.nch && nbs == o.nbs && nb == o.nb &&
+           degree == o.degree && inv_n == o.inv_n;
+  }
+};
+// one solve's operands of the small-grid filter or H·ψ launch (in -> out over nbands bands)
+struct PassSmallReq {
+  const void* in;
+  void* out;
+  const void* kin;
+  const void* vr;
+  const int* g2fft;
+  const void* B;
+  const void* DBH;
+  const double* coef;
+  const int* skip;
+  int nbands, nkv;
+};
+struct PassSmallMulti {
+  PassSmallReq r[MAX_PASS_BATCH];
+};
+struct PassRrReq {
+  const double2* Y;
+  const double2* HY;
+  double2* V;
+  double* lam;
+  double* res;
+  const int* skip;
+  int nk;
+};
+struct PassRrMulti {
+  PassRrReq r[MAX_PASS_BATCH];
+};
+struct PassReq {
+  PassSmallReq filt, hpsi;
+  PassRrReq rr;
+};
+// Groups the passes that concurrent solves reach at about the same time (serve.hip).
+struct PassBatcher;
+
 // Device state of one calculation.
 struct Context {
   int device = 0;
@@ -249,6 +300,10 @@ struct Context {
   bool fused_filter = true;  // with small_hpsi: the whole Chebyshev filter pass in one launch
   bool resident = true;      // device-resident SCF iterations where the filtered solver allows them
   bool graphs = false;     // replay captured filter passes in the SCF (scf.hip), opt-in
+  // set by the request loop's workers: filter passes go through this batcher, with the worker's
+  // two events marking where the pass may start and when it has completed
+  PassBatcher* batcher = nullptr;
+  hipEvent_t batch_ready = nullptr, batch_done = nullptr;
 
   std::vector<KData> kd;
   // the kinetic diagonals and projector blocks of all k-points packed k-major (ragged k × band
@@ -392,6 +447,18 @@ void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y
 // device, used_full / max_resid (nk each, host, may be null)
 void subspace_rr(Context& c, int n, int nb, int nk, const void* H, const void* Vprev, double tol, double* w, void* V,
                  int* used_full, double* max_resid);
+// several solves' filter passes in one launch per kernel (grid row r = request r; max_* the
+// largest band and k-block counts of the requests)
+void launch_filter_multi(hipStream_t stream, const PassShape& sh, const PassSmallMulti& m, int n, int max_bands);
+void launch_hpsi64_multi(hipStream_t stream, const PassShape& sh, const PassSmallMulti& m, int n, int max_bands);
+void launch_rr_multi(hipStream_t stream, const PassShape& sh, const PassRrMulti& m, int n, int max_nk,
+                     double floor_rel);
+// scf.hip: the whole pass (the three launches above) of n solves on one stream
+void launch_pass_batch(hipStream_t stream, const PassShape& sh, const PassReq* const* reqs, int n);
+// serve.hip: runs this solve's pass through the batcher (c.batcher set); on return later work
+// on c.stream is ordered after the pass, and after c.sync() its results (including those the
+// pass writes to host memory) are readable
+void pass_batch_join(Context& c, const PassShape& sh, const PassReq& r);
 // scf.hip
 void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
              double* rho);

@@ -23,7 +23,7 @@ namespace {
 // With coef (α, c, β per k-point) the output is the Chebyshev recurrence step
 // α_k (Hψ − c_k ψ) − β_k X (X may be null) instead of Hψ.
 template <typename T, int N>
-__global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::t* __restrict__ psi,
+__device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const typename V2s<T>::t* __restrict__ psi,
                                                          typename V2s<T>::t* __restrict__ hpsi,
                                                          const T* __restrict__ kin, const T* __restrict__ vr,
                                                          const int* __restrict__ g2fft, int npw, int nbk, int nch,
@@ -41,8 +41,8 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
   // XCD-aware band order: workgroups are dealt to the 8 XCDs round-robin, so workgroup i runs
   // on XCD i % 8; each XCD takes a contiguous range of bands, which keeps the bands of one
   // k-point (and its projector blocks) in one XCD's L2
-  const int q = (int)gridDim.x >> 3, r = (int)gridDim.x & 7, xcd = (int)blockIdx.x & 7;
-  const int b = xcd * q + min(xcd, r) + ((int)blockIdx.x >> 3), kb = b / nbk;
+  const int q = nblocks >> 3, r = nblocks & 7, xcd = bid & 7;
+  const int b = xcd * q + min(xcd, r) + (bid >> 3), kb = b / nbk;
   // band block kb: k-point kb mod nkv, V(r) channel kb / nkv (coefficients per block); with a
   // band table (ragged k × band batches) the k-point of every band is read from it
   const int k = band_k ? band_k[b] : nkv > 0 ? kb % nkv : kb;
@@ -227,9 +227,9 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::
 // its plane waves) and V(r) of the thread's x-line stays in registers for the whole pass. Each
 // step is the H·ψ of k_hpsi_small with its Chebyshev epilogue, in the same order of operations,
 // so the pass gives what `degree` launches of k_hpsi_small give. coef holds (α, c, β) per step and
-// band block: coef[(i·nblk + kb)·3 + j], nblk = gridDim.x / nbk.
+// band block: coef[(i·nblk + kb)·3 + j], nblk = nblocks / nbk (nblocks: the bands of the pass).
 template <typename T, int N>
-__global__ void __launch_bounds__(SMALL_NT) k_cheb_filter_small(const double2* __restrict__ V,
+__device__ __forceinline__ void cheb_filter_body(int bid, int nblocks, const double2* __restrict__ V,
                                                                 double2* __restrict__ Yout, const T* __restrict__ kin,
                                                                 const T* __restrict__ vr,
                                                                 const int* __restrict__ g2fft, int npw, int nbk,
@@ -244,10 +244,10 @@ __global__ void __launch_bounds__(SMALL_NT) k_cheb_filter_small(const double2* _
   V2* proj = G + N * N * N;
   V2* cur = proj + nch;
   V2* prv = cur + npw;
-  const int nblk = (int)gridDim.x / nbk;
+  const int nblk = nblocks / nbk;
   // XCD-aware band order, as in k_hpsi_small
-  const int q = (int)gridDim.x >> 3, r = (int)gridDim.x & 7, xcd = (int)blockIdx.x & 7;
-  const int b = xcd * q + min(xcd, r) + ((int)blockIdx.x >> 3), kb = b / nbk;
+  const int q = nblocks >> 3, r = nblocks & 7, xcd = bid & 7;
+  const int b = xcd * q + min(xcd, r) + (bid >> 3), kb = b / nbk;
   const int k = nkv > 0 ? kb % nkv : kb;
   if (nkv > 0) vr += (size_t)(kb / nkv) * N * N * N;
   const T* t = kin + (size_t)k * npw;
@@ -348,6 +348,59 @@ __global__ void __launch_bounds__(SMALL_NT) k_cheb_filter_small(const double2* _
   }
   double2* y = Yout + (size_t)b * npw;
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) y[g] = make_double2((double)cur[g].x, (double)cur[g].y);
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(SMALL_NT) k_hpsi_small(const typename V2s<T>::t* __restrict__ psi,
+                                                         typename V2s<T>::t* __restrict__ hpsi,
+                                                         const T* __restrict__ kin, const T* __restrict__ vr,
+                                                         const int* __restrict__ g2fft, int npw, int nbk, int nch,
+                                                         const typename V2s<T>::t* __restrict__ B,
+                                                         const typename V2s<T>::t* __restrict__ DBH, T inv_n,
+                                                         const double* __restrict__ coef,
+                                                         const typename V2s<T>::t* __restrict__ X, int nkv,
+                                                         const int* __restrict__ band_k,
+                                                         const int* __restrict__ skip) {
+  hpsi_small_body<T, N>((int)blockIdx.x, (int)gridDim.x, psi, hpsi, kin, vr, g2fft, npw, nbk, nch, B, DBH, inv_n,
+                        coef, X, nkv, band_k, skip);
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(SMALL_NT) k_cheb_filter_small(const double2* __restrict__ V,
+                                                                double2* __restrict__ Yout, const T* __restrict__ kin,
+                                                                const T* __restrict__ vr,
+                                                                const int* __restrict__ g2fft, int npw, int nbk,
+                                                                int nch, const typename V2s<T>::t* __restrict__ B,
+                                                                const typename V2s<T>::t* __restrict__ DBH, T inv_n,
+                                                                const double* __restrict__ coef, int degree, int nkv,
+                                                                const int* __restrict__ skip) {
+  cheb_filter_body<T, N>((int)blockIdx.x, (int)gridDim.x, V, Yout, kin, vr, g2fft, npw, nbk, nch, B, DBH, inv_n, coef,
+                         degree, nkv, skip);
+}
+
+// The same products for several solves in one launch (the passes of concurrent requests that
+// share grid, basis size, projector count, band counts and filter degree): blockIdx.y selects the
+// request, blockIdx.x the band within it; each band runs exactly the code of the one-solve
+// kernels above, so a request's result does not depend on the batch it ran in.
+template <typename T, int N>
+__global__ void __launch_bounds__(SMALL_NT) k_cheb_filter_multi(PassSmallMulti m, int npw, int nbk, int nch,
+                                                                T inv_n, int degree) {
+  using V2 = typename V2s<T>::t;
+  const PassSmallReq& q = m.r[blockIdx.y];
+  if ((int)blockIdx.x >= q.nbands) return;  // uniform: the request has fewer bands than the grid row
+  cheb_filter_body<T, N>((int)blockIdx.x, q.nbands, (const double2*)q.in, (double2*)q.out, (const T*)q.kin,
+                         (const T*)q.vr, q.g2fft, npw, nbk, nch, (const V2*)q.B, (const V2*)q.DBH, inv_n, q.coef,
+                         degree, q.nkv, q.skip);
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(SMALL_NT) k_hpsi_multi(PassSmallMulti m, int npw, int nbk, int nch, T inv_n) {
+  using V2 = typename V2s<T>::t;
+  const PassSmallReq& q = m.r[blockIdx.y];
+  if ((int)blockIdx.x >= q.nbands) return;  // uniform
+  hpsi_small_body<T, N>((int)blockIdx.x, q.nbands, (const V2*)q.in, (V2*)q.out, (const T*)q.kin, (const T*)q.vr,
+                        q.g2fft, npw, nbk, nch, (const V2*)q.B, (const V2*)q.DBH, inv_n, nullptr, nullptr, q.nkv,
+                        nullptr, q.skip);
 }
 
 // Band density on small cubic grids: workgroup g takes bands [g·gsize, (g+1)·gsize), runs the
@@ -589,6 +642,61 @@ void launch_cheb_filter_small(Context& c, int nbands, int nbk, const void* V, vo
     launch_filter_small_t<double>(c, nbands, nbk, V, Yout, kin, nch, B, DBH, coef, degree, nkv, skip);
   else
     launch_filter_small_t<float>(c, nbands, nbk, V, Yout, kin, nch, B, DBH, coef, degree, nkv, skip);
+}
+
+namespace {
+
+template <typename T>
+void launch_filter_multi_t(hipStream_t stream, const PassShape& sh, const PassSmallMulti& m, int n, int max_bands) {
+  const size_t lds = filter_lds_bytes<T>(sh.n, sh.nch, sh.npw);
+  const T inv_n = (T)sh.inv_n;
+  const dim3 grid((unsigned)max_bands, (unsigned)n);
+  switch (sh.n) {
+#define PWK_CASE(L)                                                                                           \
+  case L:                                                                                                     \
+    ensure_lds_limit((const void*)k_cheb_filter_multi<T, L>, lds);                                           \
+    k_cheb_filter_multi<T, L><<<grid, SMALL_NT, lds, stream>>>(m, sh.npw, sh.nbs, sh.nch, inv_n, sh.degree); \
+    break;
+    PWK_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "grid has no fused small-grid kernel");
+  }
+  PWK_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_filter_multi(hipStream_t stream, const PassShape& sh, const PassSmallMulti& m, int n, int max_bands) {
+  PWK_REQUIRE(n >= 1 && n <= MAX_PASS_BATCH && max_bands >= 1 && sh.degree >= 1 && small_len(sh.n), PWK_E_INVALID,
+              "bad batched filter shape");
+  const size_t lds = sh.fp == PWK_FP64 ? filter_lds_bytes<double>(sh.n, sh.nch, sh.npw)
+                                       : filter_lds_bytes<float>(sh.n, sh.nch, sh.npw);
+  PWK_REQUIRE(lds <= kLdsMax, PWK_E_UNSUPPORTED, "grid too large for the fused filter kernel");
+  if (sh.fp == PWK_FP64)
+    launch_filter_multi_t<double>(stream, sh, m, n, max_bands);
+  else
+    launch_filter_multi_t<float>(stream, sh, m, n, max_bands);
+}
+
+void launch_hpsi64_multi(hipStream_t stream, const PassShape& sh, const PassSmallMulti& m, int n, int max_bands) {
+  PWK_REQUIRE(n >= 1 && n <= MAX_PASS_BATCH && max_bands >= 1 && small_len(sh.n), PWK_E_INVALID,
+              "bad batched H·psi shape");
+  const size_t lds = small_lds_bytes<double>(sh.n, sh.nch);
+  PWK_REQUIRE(lds <= kLdsMax, PWK_E_UNSUPPORTED, "grid too large for the fused kernel");
+  const dim3 grid((unsigned)max_bands, (unsigned)n);
+  switch (sh.n) {
+#define PWK_CASE(L)                                                                                       \
+  case L:                                                                                                 \
+    ensure_lds_limit((const void*)k_hpsi_multi<double, L>, lds);                                         \
+    k_hpsi_multi<double, L><<<grid, SMALL_NT, lds, stream>>>(m, sh.npw, sh.nbs, sh.nch, sh.inv_n);       \
+    break;
+    PWK_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "grid has no fused small-grid kernel");
+  }
+  PWK_HIP(hipGetLastError());
 }
 
 void launch_hpsi_small(Context& c, int nbands, int nbk, const void* psi, void* hpsi, int precision, const void* kin,

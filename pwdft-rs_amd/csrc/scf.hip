@@ -523,11 +523,10 @@ __device__ double wg_max(double v, double* sh) {
 // of the next pass from the current Ritz values (the host path's chebyshev_pass): per block k the
 // interval [a, b] = [highest kept Ritz value, spectrum bound], scaled at the lowest Ritz value,
 // and the three-term coefficients of every step.
-__global__ void __launch_bounds__(256) k_pass_gate(int nke, int nbs, int nb, int degree, int gate,
-                                                   const double* __restrict__ lam, const double* __restrict__ res,
-                                                   const double* __restrict__ ub, double* __restrict__ S,
-                                                   int* __restrict__ flag, double* __restrict__ coef) {
-  __shared__ double sh[4];
+__device__ __forceinline__ void pass_gate_body(int nke, int nbs, int nb, int degree, int gate,
+                                               const double* __restrict__ lam, const double* __restrict__ res,
+                                               const double* __restrict__ ub, double* __restrict__ S,
+                                               int* __restrict__ flag, double* __restrict__ coef, double* sh) {
   const int done = flag[0];
   __syncthreads();  // every thread has read the flag before thread 0 may set it below
   if (done) return;  // uniform
@@ -568,6 +567,14 @@ __global__ void __launch_bounds__(256) k_pass_gate(int nke, int nbs, int nb, int
       sigma = sn;
     }
   }
+}
+
+__global__ void __launch_bounds__(256) k_pass_gate(int nke, int nbs, int nb, int degree, int gate,
+                                                   const double* __restrict__ lam, const double* __restrict__ res,
+                                                   const double* __restrict__ ub, double* __restrict__ S,
+                                                   int* __restrict__ flag, double* __restrict__ coef) {
+  __shared__ double sh[4];
+  pass_gate_body(nke, nbs, nb, degree, gate, lam, res, ub, S, flag, coef, sh);
 }
 
 // src/scf/smearing.rs:112-130
@@ -1871,7 +1878,29 @@ double Solve::chebyshev_pass(int fprec) {
   };
   const bool graph = c.graphs && !c.prof.on && rr_small_supported(nbs) && hpsi_small_supported(c, PWK_FP64, nch_all) &&
                      hpsi_small_supported(c, fprec, nch_all) && fprec != PWK_SPLIT_BF16;
-  if (graph) {
+  if (c.batcher && fused_filter && !graph && rr_small_supported(nbs) && hpsi_small_supported(c, PWK_FP64, nch_all)) {
+    // served solve: the pass may share its three launches with the passes of other solves
+    // (the coefficients above are uploaded on this stream before the pass's start is marked)
+    const bool f64 = fprec == PWK_FP64;
+    const int nkv = fold ? nk : 0, nbands = (int)(nke * nbs);
+    const size_t nkm = (size_t)nke * nbs;
+    PassShape sh;
+    sh.fp = f64 ? PWK_FP64 : PWK_FP32;
+    sh.n = c.dims[0];
+    sh.npw = npw;
+    sh.nch = nch_all;
+    sh.nbs = nbs;
+    sh.nb = nb;
+    sh.degree = degree;
+    sh.inv_n = 1.0 / (double)c.N;
+    PassReq r{};
+    r.filt = {Vc(), Y64.p, f64 ? kin_all64.p : kin_all32.p, f64 ? c.vrn64.p : c.vrn32.p, c.g2fft.as<int>(),
+              f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, coef.as<double>(), nullptr, nbands, nkv};
+    r.hpsi = {Y64.p, HY64.p, kin_all64.p, c.vrn64.p, c.g2fft.as<int>(), Ball64.p, DBHall64.p, nullptr, nullptr,
+              nbands, nkv};
+    r.rr = {Y64.as<double2>(), HY64.as<double2>(), Vc(), rr_pinned, rr_pinned + nkm, nullptr, (int)nke};
+    pass_batch_join(c, sh, r);
+  } else if (graph) {
     const int key = sc * 4 + fprec;
     auto it = pass_graphs.find(key);
     if (it != pass_graphs.end()) {
@@ -2534,6 +2563,26 @@ void Solve::launch_iteration(int fp, int passes) {
 }
 
 }  // namespace
+
+// The filter passes of n solves: filter, FP64 H·ψ and Rayleigh-Ritz over every solve's k-point
+// blocks, three launches in all (the launches Solve::chebyshev_pass makes for one solve on the
+// fused small-grid path).
+void launch_pass_batch(hipStream_t stream, const PassShape& sh, const PassReq* const* reqs, int n) {
+  PWK_REQUIRE(n >= 1 && n <= MAX_PASS_BATCH, PWK_E_INVALID, "bad pass batch size");
+  PassSmallMulti f{}, h{};
+  PassRrMulti r{};
+  int max_bands = 1, max_nk = 1;
+  for (int i = 0; i < n; ++i) {
+    f.r[i] = reqs[i]->filt;
+    h.r[i] = reqs[i]->hpsi;
+    r.r[i] = reqs[i]->rr;
+    max_bands = std::max(max_bands, reqs[i]->filt.nbands);
+    max_nk = std::max(max_nk, reqs[i]->rr.nk);
+  }
+  launch_filter_multi(stream, sh, f, n, max_bands);
+  launch_hpsi64_multi(stream, sh, h, n, max_bands);
+  launch_rr_multi(stream, sh, r, n, max_nk, 1e-14);
+}
 
 void scf_run(Context& c, const pwk_crystal& cr, const pwk_scf_params& p, pwk_scf_result& out, double* eig,
              double* rho) {

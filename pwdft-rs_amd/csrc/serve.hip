@@ -13,6 +13,14 @@
 //
 // Solves are independent and each is deterministic (fixed-order reductions),
 // so a result does not depend on what else is running.
+//
+// Concurrent solves of the same shape (grid, basis size, projector channels, band counts, filter
+// degree and precision) share their filter passes: a solve that reaches a pass joins an open
+// group of that shape, or opens one and gathers for a short window; the group's passes then run
+// as one launch per kernel on the opening solve's stream, ordered after every member's earlier
+// work by events, and every member continues once the batch has completed. Each band and k-point
+// block of a batched launch runs the code of the one-solve launch, so the batch changes no result.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -83,6 +91,109 @@ void BlockCache::trim(size_t limit) {
   }
 }
 
+struct PassBatcher {
+  struct Member {
+    const PassReq* r;
+    hipEvent_t ready, done;
+  };
+  struct Group {
+    PassShape sh;
+    std::vector<Member> m;
+    bool launched = false;
+    int status = PWK_OK;
+    std::string error;
+  };
+  std::mutex mu;
+  std::condition_variable cv;
+  int window_us = 200;  // how long a group's first solve gathers others; 0: every pass alone
+  int running = 0;      // solves in progress (a group of that many launches at once)
+  int64_t launches = 0, passes = 0;  // batched launches and the solve passes they carried
+  std::vector<std::shared_ptr<Group>> open;
+};
+
+// Keeps the batcher's count of solves in progress.
+struct PassBatchScope {
+  PassBatcher* b;
+  explicit PassBatchScope(PassBatcher* pb) : b(pb) {
+    if (!b) return;
+    std::lock_guard<std::mutex> lk(b->mu);
+    ++b->running;
+  }
+  ~PassBatchScope() {
+    if (!b) return;
+    {
+      std::lock_guard<std::mutex> lk(b->mu);
+      --b->running;
+    }
+    b->cv.notify_all();  // a gathering group may now be complete
+  }
+};
+
+void pass_batch_join(Context& c, const PassShape& sh, const PassReq& r) {
+  PassBatcher& b = *c.batcher;
+  PWK_HIP(hipEventRecord(c.batch_ready, c.stream));  // where this solve's pass may start
+  std::unique_lock<std::mutex> lk(b.mu);
+  std::shared_ptr<PassBatcher::Group> g;
+  const bool gather = b.window_us > 0;
+  if (gather)
+    for (auto& o : b.open)
+      if (o->sh == sh && (int)o->m.size() < MAX_PASS_BATCH) {
+        g = o;
+        break;
+      }
+  const bool leader = !g;
+  if (leader) {
+    g = std::make_shared<PassBatcher::Group>();
+    g->sh = sh;
+    if (gather) b.open.push_back(g);
+  }
+  g->m.push_back({&r, c.batch_ready, c.batch_done});
+  if (!leader) {
+    // the group's first solve launches the batch; this stream continues after it
+    b.cv.notify_all();
+    b.cv.wait(lk, [&] { return g->launched; });
+    if (g->status != PWK_OK) throw Error(g->status, g->error);
+    lk.unlock();
+    // wait on the host: a stream-side wait would hold the hardware queue this stream shares with
+    // other solves' streams until the whole batch has run
+    PWK_HIP(hipEventSynchronize(c.batch_done));
+    return;
+  }
+  if (gather) {
+    const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(b.window_us);
+    b.cv.wait_until(lk, until, [&] { return (int)g->m.size() >= std::min(MAX_PASS_BATCH, b.running); });
+    b.open.erase(std::find(b.open.begin(), b.open.end(), g));
+  }
+  const std::vector<PassBatcher::Member> mem = g->m;  // closed: no solve joins after this
+  lk.unlock();
+  int st = PWK_OK;
+  std::string err;
+  try {
+    std::vector<const PassReq*> rq;
+    for (size_t i = 0; i < mem.size(); ++i) {
+      rq.push_back(mem[i].r);
+      if (i > 0) PWK_HIP(hipStreamWaitEvent(c.stream, mem[i].ready, 0));
+    }
+    launch_pass_batch(c.stream, sh, rq.data(), (int)rq.size());
+    for (size_t i = 1; i < mem.size(); ++i) PWK_HIP(hipEventRecord(mem[i].done, c.stream));
+  } catch (const Error& e) {
+    st = e.code;
+    err = e.what();
+  } catch (const std::exception& e) {
+    st = PWK_E_DEVICE;
+    err = e.what();
+  }
+  lk.lock();
+  g->status = st;
+  g->error = err;
+  g->launched = true;
+  b.launches += 1;
+  b.passes += (int64_t)mem.size();
+  lk.unlock();
+  b.cv.notify_all();
+  if (st != PWK_OK) throw Error(st, err);
+}
+
 struct ServeJob {
   int64_t id = 0;
   pwk_crystal cr{};
@@ -109,12 +220,14 @@ struct Server {
   int64_t next_id = 1;
   bool stopping = false;
   std::vector<std::thread> workers;
+  PassBatcher batcher;
 
   bool admissible(const ServeJob& j) const { return j.bytes > budget || in_use + j.bytes <= budget; }
 
   // Runs one solve in a fresh context on the worker's stream. The context's buffers come
   // from the worker's block cache and go back to it when the solve ends.
-  void solve(ServeJob& job, hipStream_t stream, hipEvent_t sleep_event, int cu_count) {
+  void solve(ServeJob& job, hipStream_t stream, hipEvent_t sleep_event, hipEvent_t ready, hipEvent_t done,
+             int cu_count) {
     if (job.bytes > budget) {
       job.status = PWK_E_OOM;
       job.error = "solve does not fit in the server's device-memory budget";
@@ -126,6 +239,9 @@ struct Server {
       c.stream = stream;
       c.cu_count = cu_count;
       c.sleep_event = sleep_event;
+      c.batcher = &batcher;
+      c.batch_ready = ready;
+      c.batch_done = done;
       struct StreamGuard {
         Context& c;
         ~StreamGuard() {
@@ -137,11 +253,16 @@ struct Server {
       // stream capture in one worker would be disturbed by allocation and release in another,
       // so served solves launch the filter passes directly
       c.graphs = false;
+      // served solves run the host-synchronised iteration: the hardware queues the workers'
+      // streams share are the limit here, not the host, and that loop launches fewer kernels
+      // per solve than the device-resident one; its filter passes go through the batcher
+      c.resident = false;
       int npw, dims[3], nk, nops;
       scf_tables(job.cr, job.p, &npw, dims, &nk, &nops);
       const size_t ns = job.p.nspin == 2 ? 2 : 1;  // eigenvalues of both spin channels
       job.eig.assign(ns * (size_t)nk * (size_t)std::min(job.p.n_bands, npw), 0.0);
       job.rho.assign((size_t)dims[0] * dims[1] * dims[2], 0.0);
+      PassBatchScope running(&batcher);
       scf_run(c, job.cr, job.p, job.res, job.eig.data(), job.rho.data());
     } catch (const Error& e) {
       job.status = e.code;
@@ -155,12 +276,14 @@ struct Server {
   void worker_main() {
     // per-worker stream and block caches, kept across solves
     hipStream_t stream = nullptr;
-    hipEvent_t sleep_event = nullptr;
+    hipEvent_t sleep_event = nullptr, ready = nullptr, done = nullptr;
     int cu_count = 0;
     try {
       PWK_HIP(hipSetDevice(device));
       PWK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
       PWK_HIP(hipEventCreateWithFlags(&sleep_event, hipEventBlockingSync | hipEventDisableTiming));
+      PWK_HIP(hipEventCreateWithFlags(&ready, hipEventBlockingSync | hipEventDisableTiming));
+      PWK_HIP(hipEventCreateWithFlags(&done, hipEventBlockingSync | hipEventDisableTiming));
       hipDeviceProp_t prop;
       PWK_HIP(hipGetDeviceProperties(&prop, device));
       cu_count = prop.multiProcessorCount;
@@ -190,8 +313,8 @@ struct Server {
         if (job->bytes <= budget) in_use += job->bytes;
       }
       job->t_start = std::chrono::steady_clock::now();
-      if (stream) {
-        solve(*job, stream, sleep_event, cu_count);
+      if (stream && ready && done) {
+        solve(*job, stream, sleep_event, ready, done, cu_count);
         dev_cache.trim(cache_cap);
         host_cache.trim(HOST_CACHE_CAP);
       } else {
@@ -213,6 +336,8 @@ struct Server {
     dev_cache.trim(0);
     host_cache.trim(0);
     if (sleep_event) (void)hipEventDestroy(sleep_event);
+    if (ready) (void)hipEventDestroy(ready);
+    if (done) (void)hipEventDestroy(done);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -341,6 +466,24 @@ int pwk_server_wait(pwk_server* srv, int64_t ticket, pwk_scf_result* result, dou
   if (latency_s) *latency_s = std::chrono::duration<double>(job->t_end - job->t_submit).count();
   if (job->status != PWK_OK) pwk::set_last_error(job->error);
   return job->status;
+}
+
+int pwk_server_set_pass_batching(pwk_server* srv, int window_us) {
+  return serve_guarded([&] {
+    PWK_REQUIRE(srv && window_us >= 0 && window_us <= 100000, PWK_E_INVALID,
+                "bad batching window (0..100000 microseconds)");
+    std::lock_guard<std::mutex> lk(srv->s.batcher.mu);
+    srv->s.batcher.window_us = window_us;
+  });
+}
+
+int pwk_server_pass_stats(pwk_server* srv, int64_t* launches, int64_t* passes) {
+  return serve_guarded([&] {
+    PWK_REQUIRE(srv && launches && passes, PWK_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(srv->s.batcher.mu);
+    *launches = srv->s.batcher.launches;
+    *passes = srv->s.batcher.passes;
+  });
 }
 
 int pwk_server_destroy(pwk_server* srv) {

@@ -193,15 +193,15 @@ struct RrLds {
   double red[16][JMAX];
 };
 
-__global__ void __launch_bounds__(RR_NT) k_rr_small(int npw, int m, int nres, const double2* __restrict__ Yb,
-                                                    const double2* __restrict__ HYb, double2* __restrict__ Vb,
-                                                    double* __restrict__ lam_out, double* __restrict__ res_out,
-                                                    double floor_rel, const int* __restrict__ skip) {
+__device__ __forceinline__ void rr_small_body(int k, int npw, int m, int nres, const double2* __restrict__ Yb,
+                                              const double2* __restrict__ HYb, double2* __restrict__ Vb,
+                                              double* __restrict__ lam_out, double* __restrict__ res_out,
+                                              double floor_rel, const int* __restrict__ skip) {
   if (skip && *skip) return;  // a filter pass after the residual gate passed (uniform)
   extern __shared__ unsigned char smem_raw[];
   RrLds& L = *reinterpret_cast<RrLds*>(smem_raw);
   JacobiLds& J = L.J;
-  const int k = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   const double2* Y = Yb + (size_t)k * npw * m;
   const double2* HY = HYb + (size_t)k * npw * m;
   double2* V = Vb + (size_t)k * npw * m;
@@ -423,6 +423,21 @@ __global__ void __launch_bounds__(RR_NT) k_rr_small(int npw, int m, int nres, co
   }
 }
 
+__global__ void __launch_bounds__(RR_NT) k_rr_small(int npw, int m, int nres, const double2* __restrict__ Yb,
+                                                    const double2* __restrict__ HYb, double2* __restrict__ Vb,
+                                                    double* __restrict__ lam_out, double* __restrict__ res_out,
+                                                    double floor_rel, const int* __restrict__ skip) {
+  rr_small_body((int)blockIdx.x, npw, m, nres, Yb, HYb, Vb, lam_out, res_out, floor_rel, skip);
+}
+
+// the Rayleigh-Ritz steps of several solves' passes in one launch: blockIdx.y selects the request,
+// blockIdx.x its k-point block (the code of k_rr_small per block)
+__global__ void __launch_bounds__(RR_NT) k_rr_multi(PassRrMulti mr, int npw, int m, int nres, double floor_rel) {
+  const PassRrReq& q = mr.r[blockIdx.y];
+  if ((int)blockIdx.x >= q.nk) return;  // uniform
+  rr_small_body((int)blockIdx.x, npw, m, nres, q.Y, q.HY, q.V, q.lam, q.res, floor_rel, q.skip);
+}
+
 }  // namespace
 
 bool rr_small_supported(int m) { return m >= 1 && m <= JMAX; }
@@ -507,6 +522,16 @@ void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y
   ensure_lds_limit((const void*)k_rr_small, sh);
   k_rr_small<<<(unsigned)nk, RR_NT, sh, c.stream>>>(npw, m, nres, (const double2*)Y, (const double2*)HY,
                                                    (double2*)V, lam, res, floor_rel, skip);
+  PWK_HIP(hipGetLastError());
+}
+
+void launch_rr_multi(hipStream_t stream, const PassShape& sh, const PassRrMulti& m, int n, int max_nk,
+                     double floor_rel) {
+  PWK_REQUIRE(n >= 1 && n <= MAX_PASS_BATCH && rr_small_supported(sh.nbs) && sh.nb <= sh.nbs && sh.npw >= sh.nbs,
+              PWK_E_INVALID, "bad batched Rayleigh-Ritz shape");
+  const size_t lds = sizeof(RrLds);
+  ensure_lds_limit((const void*)k_rr_multi, lds);
+  k_rr_multi<<<dim3((unsigned)max_nk, (unsigned)n), RR_NT, lds, stream>>>(m, sh.npw, sh.nbs, sh.nb, floor_rel);
   PWK_HIP(hipGetLastError());
 }
 

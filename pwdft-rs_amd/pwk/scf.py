@@ -255,6 +255,8 @@ _SIGS.update({
     "pwk_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Crystal), C.POINTER(Params), C.POINTER(C.c_int64)]),
     "pwk_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Result), _dp, _dp, C.POINTER(C.c_double)]),
     "pwk_server_destroy": (C.c_int, [C.c_void_p]),
+    "pwk_server_set_pass_batching": (C.c_int, [C.c_void_p, C.c_int]),
+    "pwk_server_pass_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
 })
 
 
@@ -290,6 +292,16 @@ class Server:
         out["rho"] = rho
         out["latency_s"] = lat.value
         return out
+
+    def set_pass_batching(self, window_us: int):
+        """Gathering window of cross-request filter-pass batches in microseconds (0: off)."""
+        _check(_lib().pwk_server_set_pass_batching(self.h, int(window_us)))
+
+    def pass_stats(self):
+        """(batched pass launches, solve passes they carried) since creation."""
+        a, b = C.c_int64(), C.c_int64()
+        _check(_lib().pwk_server_pass_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def close(self):
         if self.h:

@@ -154,7 +154,12 @@ def test_served_worker_reuses_memory_across_solves(scf_mod, gpu):
         params.append(scf_mod.ScfParams(ecut_ev=ecut_ry * RY, kgrid=(kg, kg, kg), n_bands=8, max_iter=60,
                                         conv_threshold=1e-7, energy_threshold=1e-6, smearing_sigma=0.01 * RY,
                                         eigensolver=1, precision=prec))
-    alone = [scf_mod.run_scf(gpu, d, p) for d, p in zip(decks, params)]
+    # served solves run the host-synchronised iteration
+    try:
+        gpu.set_resident(False)
+        alone = [scf_mod.run_scf(gpu, d, p) for d, p in zip(decks, params)]
+    finally:
+        gpu.set_resident(True)
     srv = scf_mod.Server(0, n_slots=1)
     try:
         served = [srv.wait(srv.submit(d, p)) for d, p in zip(decks, params)]
@@ -421,3 +426,45 @@ def test_aloc_pin(scf_mod, gpu, oracle, eigensolver, precision):
             assert abs(r[t] - ro[t]) / 2 < TOL_TERM, t
     else:  # another trajectory stopped at conv 1e-5: the stationary total only
         assert abs(r["e_total"] - ro["e_total"]) < 1e-3
+
+
+def test_served_pass_batches_equal_alone(scf_mod, gpu):
+    # concurrent filtered solves of one shape share their filter passes (filter, FP64 H·psi and
+    # Rayleigh-Ritz launched once over all of them); each result equals the same solve run alone,
+    # bit for bit, and the batcher reports launches that carried several solves.
+    # Mixed precisions, a jittered cell (more IBZ k-points: a ragged batch) and a folded LSDA
+    # solve share the grid, basis and band counts.
+    lat = fcc(10.26 * BOHR)
+    deck = scf_mod.CrystalDeck(lat, SI_FRAC, [0, 0], [SI_UPF])
+    deck_j = scf_mod.CrystalDeck(lat, [[0.0, 0.0, 0.0], [0.26, 0.245, 0.25]], [0, 0], [SI_UPF])
+    kw = dict(ecut_ev=12 * RY, kgrid=(4, 4, 4), n_bands=8, max_iter=60, conv_threshold=1e-7, energy_threshold=1e-6,
+              smearing_sigma=0.01 * RY, eigensolver=1)
+    reqs = [(deck, scf_mod.ScfParams(precision="fp32", **kw)),
+            (deck_j, scf_mod.ScfParams(precision="fp32", **kw)),
+            (deck, scf_mod.ScfParams(precision="fp32", **kw)),
+            (deck, scf_mod.ScfParams(precision="fp64", **kw)),
+            (deck_j, scf_mod.ScfParams(precision="fp64", **kw)),
+            (deck, scf_mod.ScfParams(precision="fp32", nspin=2, starting_magnetization={0: 0.2}, **kw))]
+    try:
+        gpu.set_resident(False)  # served solves run the host-synchronised iteration
+        alone = [scf_mod.run_scf(gpu, d, p) for d, p in reqs]
+    finally:
+        gpu.set_resident(True)
+    srv = scf_mod.Server(0, n_slots=len(reqs))
+    try:
+        srv.set_pass_batching(5000)  # a wide window: the solves gather into shared launches
+        tickets = [srv.submit(d, p) for d, p in reqs]
+        served = [srv.wait(t) for t in tickets]
+        launches, passes = srv.pass_stats()
+    finally:
+        srv.close()
+    print("pass batches: launches", launches, "solve passes", passes, flush=True)
+    assert passes > launches > 0
+    for a, m in zip(alone, served):
+        print("alone", a["nk"], a["n_iter"], a["e_total"], "served", m["n_iter"], m["e_total"], flush=True)
+        assert m["converged"] == 1 and a["converged"] == 1
+        assert m["n_iter"] == a["n_iter"]
+        for t in TERMS:
+            assert m[t] == a[t], t
+        assert np.array_equal(m["eigenvalues"], a["eigenvalues"])
+        assert np.array_equal(m["rho"], a["rho"])

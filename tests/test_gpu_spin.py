@@ -190,7 +190,11 @@ def test_served_spin_equals_alone(scf_mod, gpu):
     deck = scf_mod.CrystalDeck(fcc(5.431), SI_FRAC, [0, 0], [SI_UPF])
     p_spin = _si_params(scf_mod, nspin=2, starting_magnetization={0: 0.2})
     p_plain = _si_params(scf_mod)
-    alone = scf_mod.run_scf(gpu, deck, p_spin)
+    try:
+        gpu.set_resident(False)  # served solves run the host-synchronised iteration
+        alone = scf_mod.run_scf(gpu, deck, p_spin)
+    finally:
+        gpu.set_resident(True)
     srv = scf_mod.Server(0, n_slots=3)
     try:
         tickets = [srv.submit(deck, p) for p in (p_spin, p_plain, p_spin)]
