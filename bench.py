@@ -709,6 +709,7 @@ def serving_stream(args):
         per.append(d)
     # alone-vs-served and FP64 checks on a sample of the small requests
     acc = pwk.GpuAccelerator(0)
+    acc.set_resident(False)  # served solves run the host-synchronised iteration
     checks = []
     sample = [(k, reqs[k]) for k, (info, r, e) in enumerate(results)
               if r is not None and r["converged"] == 1 and info["atoms"] <= 8][:args.stream_checks]

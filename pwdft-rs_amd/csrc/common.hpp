@@ -202,6 +202,19 @@ struct KData {
   DevBuf B32, DBH32;    // float2 copies
 };
 
+// The seeded perturbation of entry g of start vector j of block k (a hash of the three indices,
+// components in [-0.005, 0.005)): the Chebyshev start blocks and the Lanczos start vector.
+__device__ inline double2 start_vector_entry(int g, int j, int k) {
+  uint32_t h = (uint32_t)g * 0x9E3779B1u ^ ((uint32_t)j * 0x85EBCA77u + 0x165667B1u) ^ ((uint32_t)k * 0xC2B2AE3Du);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  const double r1 = (double)(h & 0xFFFF) / 65536.0 - 0.5, r2 = (double)(h >> 16) / 65536.0 - 0.5;
+  return make_double2(1e-2 * r1, 1e-2 * r2);
+}
+
 // ---- filter passes of several concurrent solves in one set of launches ----------------------
 // A filter pass (filter, FP64 H·ψ, Rayleigh-Ritz; scf.hip) of every solve that shares the shape
 // below can run in one launch per kernel, a grid row per solve. The operands travel by value in
@@ -343,6 +356,9 @@ void launch_hartree_veff(Context& c, const void* rho_g, double fourpi_e2, const 
                          void* out, int precision, bool with_local, bool with_xc, bool with_hartree);
 void launch_veff_sum(Context& c, const void* a, const void* b, const void* d, void* out, int precision);
 void launch_lda(Context& c, const void* rho, void* exc, void* vxc, int precision);
+// ρ_xc = max(ρ + cs·ρ_core, 0) (ρ when core is null) into rho_xc, then LDA of it (FP64, N points)
+void launch_lda_core(Context& c, const double* rho, const double* core, double cs, double* rho_xc, double* exc,
+                     double* vxc);
 void launch_lsda(Context& c, const void* ru, const void* rd, void* exc, void* vu, void* vd, int precision);
 void launch_convert(Context& c, const void* src, void* dst, size_t n, bool to_fp32);  // double<->float arrays
 // fft3d.hip
@@ -387,10 +403,17 @@ bool cheb_filter_small_supported(const Context& c, int precision, int nch);
 void launch_cheb_filter_small(Context& c, int nbands, int nbk, const void* V, void* Yout, int precision,
                               const void* kin, int nch, const void* B, const void* DBH, const double* coef, int degree,
                               int nkv = 0, const int* skip = nullptr);
-// ρ += Σ_b fw_b |ψ_b(r)|² on small cubic grids (fw on the device)
+// the whole Lanczos recurrence for the spectral bound of nke band blocks in one launch on small
+// cubic grids (FP64; block kb at k-point kb mod nkv and V(r) slot kb / nkv when nkv > 0):
+// α_j → ab[j·nke + kb], β_j → ab[(steps + j)·nke + kb] (device)
+bool lanczos_small_supported(const Context& c, int nch);
+void launch_lanczos_small(Context& c, int nke, int nkv, int steps, const double* kin, int nch, const void* B,
+                          const void* DBH, double* ab);
+// ρ += Σ_b fw_b |ψ_b(r)|² on small cubic grids (fw device-readable: device or pinned host
+// memory); psi_fp64: ψ is FP64 and, for an FP32 density, rounded to FP32 as it is loaded
 bool density_small_supported(const Context& c, int precision);
 void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision,
-                          bool overwrite = false);
+                          bool overwrite = false, bool psi_fp64 = false);
 // dense.hip
 void launch_dense_h(Context& c, int ik, int with_nl, void* h);
 void launch_dense_h_batched(Context& c, int nk, const double* kin, int nch, const void* B, const void* DBH, void* h);
@@ -423,6 +446,12 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
                            const int32_t* atom_species, int nspecies, const int32_t* nproj, const int32_t* proj_l,
                            const double* dij, const int32_t* mesh, const double* r, const double* rab,
                            const double* chi, double omega, const NonlocalDest* dest = nullptr);
+// the same for the nkb k-points ik0 .. ik0 + nkb - 1 (kvecs: nkb × 3) in one launch per kernel;
+// dest (required for nkb > 1) holds their blocks back to back, npw × nch entries each
+void build_nonlocal_batch(Context& c, int ik0, int nkb, const double* kvecs, const double* gvec, int natoms,
+                          const double* tau, const int32_t* atom_species, int nspecies, const int32_t* nproj,
+                          const int32_t* proj_l, const double* dij, const int32_t* mesh, const double* r,
+                          const double* rab, const double* chi, double omega, const NonlocalDest* dest);
 
 // subspace.hip: Chebyshev filter and Rayleigh-Ritz helpers, batched over blocks
 void launch_cheb_step(Context& c, size_t n, size_t per, const void* HY, const void* Y, const void* X,

@@ -143,6 +143,22 @@ __global__ void __launch_bounds__(256) k_lda(size_t n, const T* __restrict__ rho
   vxc[i] = v;
 }
 
+// ρ_xc = max(ρ + cs·ρ_core, 0) (ρ without a core, src/scf/energy.rs:314-324) written out, and
+// LDA of it, in one pass (FP64)
+__global__ void __launch_bounds__(256) k_lda_core(size_t n, const double* __restrict__ rho,
+                                                  const double* __restrict__ core, double cs,
+                                                  double* __restrict__ rho_xc, double* __restrict__ exc,
+                                                  double* __restrict__ vxc) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double r = core ? fmax(rho[i] + core[i] * cs, 0.0) : rho[i];
+  rho_xc[i] = r;
+  double e, v;
+  lda_point(r, e, v);
+  exc[i] = e;
+  vxc[i] = v;
+}
+
 // ---- LSDA (src/potential/xc.rs:374-392, 465-596) ------------------------------
 template <typename T>
 __device__ __forceinline__ void pz_rs(T rs, bool pol, T& ec, T& vc) {
@@ -246,6 +262,13 @@ void launch_veff_sum(Context& c, const void* a, const void* b, const void* d, vo
   else
     k_sum3<float><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const float2*)a, (const float2*)b,
                                                             (const float2*)d, (float2*)out);
+  PWK_HIP(hipGetLastError());
+}
+
+void launch_lda_core(Context& c, const double* rho, const double* core, double cs, double* rho_xc, double* exc,
+                     double* vxc) {
+  c.check_grid();
+  k_lda_core<<<blocks_for(c.N, 256), 256, 0, c.stream>>>(c.N, rho, core, cs, rho_xc, exc, vxc);
   PWK_HIP(hipGetLastError());
 }
 

@@ -218,6 +218,181 @@ __device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const type
   }
 }
 
+// The local part of H·v for a vector held in LDS (cur, npw entries): scatter onto the grid G,
+// projections proj[c] = Σ_G (D·B^H)[c, G] v(G) (one wave per channel, up to four side by side),
+// inverse FFT z, y, x with V(r) applied between the x transforms (vline: V(r) of the thread's
+// x-line, when N² <= SMALL_NT; else read from vr), forward FFT y, z. On return G holds
+// FFT[V·IFFT v] (unnormalised) and proj the projections; the caller's epilogue adds the kinetic
+// and nonlocal terms in the order k_hpsi_small uses.
+template <typename T, int N>
+__device__ __forceinline__ void small_h_core(typename V2s<T>::t* G, typename V2s<T>::t* proj,
+                                             const typename V2s<T>::t* cur, const T (&vline)[N * N <= SMALL_NT ? N : 1],
+                                             const T* __restrict__ vr, const int* __restrict__ g2fft, int npw, int nch,
+                                             const typename V2s<T>::t* __restrict__ dk) {
+  using V2 = typename V2s<T>::t;
+  constexpr int NW = SMALL_NT / 64, CPW = 4;
+  constexpr bool PRE_V = N * N <= SMALL_NT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
+    G[i].x = 0;
+    G[i].y = 0;
+  }
+  lds_barrier();
+  for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = cur[g];
+  for (int c0 = wave; c0 < nch; c0 += NW * CPW) {
+    T ar[CPW], ai[CPW];
+#pragma unroll
+    for (int j = 0; j < CPW; ++j) ar[j] = ai[j] = 0;
+    for (int g = lane; g < npw; g += 64) {
+      const V2 v = cur[g];
+      V2 d[CPW];
+#pragma unroll
+      for (int j = 0; j < CPW; ++j)
+        if (c0 + j * NW < nch) d[j] = dk[(size_t)g * nch + c0 + j * NW];
+#pragma unroll
+      for (int j = 0; j < CPW; ++j)
+        if (c0 + j * NW < nch) {
+          ar[j] += d[j].x * v.x - d[j].y * v.y;
+          ai[j] += d[j].x * v.y + d[j].y * v.x;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CPW; ++j) {
+      for (int off = 32; off > 0; off >>= 1) {
+        ar[j] += __shfl_xor(ar[j], off, 64);
+        ai[j] += __shfl_xor(ai[j], off, 64);
+      }
+      if (lane == 0 && c0 + j * NW < nch) {
+        proj[c0 + j * NW].x = ar[j];
+        proj[c0 + j * NW].y = ai[j];
+      }
+    }
+  }
+  lds_barrier();
+  lds_lines<T, N, 2, 1>(G, nullptr);
+  lds_barrier();
+  lds_lines<T, N, 1, 1>(G, nullptr);
+  lds_barrier();
+  if constexpr (PRE_V)
+    lds_xlines_v<T, N, 1>(G, vline);
+  else
+    lds_lines<T, N, 0, 1>(G, vr);
+  lds_barrier();
+  lds_lines<T, N, 1, -1>(G, nullptr);
+  lds_barrier();
+  lds_lines<T, N, 2, -1>(G, nullptr);
+  lds_barrier();
+}
+
+// Re⟨x, y⟩ over n complex entries held in LDS, summed as the 2n reals in the order of
+// k_block_dot (subspace.hip): thread t takes reals t, t + SMALL_NT, ..., then the wave sums
+// by shuffle-down and the four wave sums pairwise. Every thread returns the total.
+__device__ __forceinline__ double lds_real_dot(const double2* x, const double2* y, int n, double* red) {
+  const double* xr = reinterpret_cast<const double*>(x);
+  const double* yr = reinterpret_cast<const double*>(y);
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < 2 * n; i += SMALL_NT) acc += xr[i] * yr[i];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  const double s = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return s;
+}
+
+// Lanczos recurrence for the upper spectral bound, every step in one launch: one workgroup per
+// band block kb (k-point kb mod nkv with V(r) slot kb / nkv when nkv > 0, else k-point kb), the
+// three vectors in LDS. Each step is the sequence the step launches of the general path run
+// (H·v as in k_hpsi_small; α = Re⟨v, Hv⟩; w = (Hv − α v) − β v_prev; β' = |w|; v ← w/β', zero
+// when β' < 1e-12), with the same reductions, so the coefficients are those of that path;
+// α_j goes to ab[j·nke + kb], β_j to ab[(steps + j)·nke + kb].
+template <int N>
+__global__ void __launch_bounds__(SMALL_NT) k_lanczos_small(const int* __restrict__ g2fft, int npw, int nch,
+                                                            const double* __restrict__ kin,
+                                                            const double* __restrict__ vr,
+                                                            const double2* __restrict__ B,
+                                                            const double2* __restrict__ DBH, double inv_n, int nkv,
+                                                            int steps, double* __restrict__ ab) {
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ double red[4];
+  double2* G = reinterpret_cast<double2*>(smem_raw);
+  double2* proj = G + N * N * N;
+  double2* v0 = proj + nch;
+  double2* v1 = v0 + npw;
+  double2* w = v1 + npw;
+  const int kb = blockIdx.x, nke = gridDim.x;
+  const int k = nkv > 0 ? kb % nkv : kb;
+  if (nkv > 0) vr += (size_t)(kb / nkv) * N * N * N;
+  const double* t = kin + (size_t)k * npw;
+  const double2* bk = B + (size_t)k * npw * nch;
+  const double2* dk = DBH + (size_t)k * nch * npw;
+  constexpr bool PRE_V = N * N <= SMALL_NT;
+  double vline[PRE_V ? N : 1];
+  if constexpr (PRE_V) {
+    if (threadIdx.x < N * N) {
+#pragma unroll
+      for (int f = 0; f < N; ++f) vline[f] = vr[threadIdx.x + f * N * N];
+    }
+  }
+  for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
+    v1[g] = start_vector_entry(g, 0, kb);
+    v0[g] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  {
+    const double s = lds_real_dot(v1, v1, npw, red);
+    for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
+      v1[g].x *= 1.0 / sqrt(s);
+      v1[g].y *= 1.0 / sqrt(s);
+    }
+  }
+  double beta = 0.0;
+  for (int j = 0; j < steps; ++j) {
+    lds_barrier();
+    small_h_core<double, N>(G, proj, v1, vline, vr, g2fft, npw, nch, dk);
+    for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
+      const double2 v = v1[g], hw = G[g2fft[g]];
+      const double tk = t[g];
+      double ox = tk * v.x + hw.x * inv_n, oy = tk * v.y + hw.y * inv_n;
+      for (int ch = 0; ch < nch; ++ch) {
+        const double2 bb = bk[(size_t)ch * npw + g], pc = proj[ch];
+        ox += bb.x * pc.x - bb.y * pc.y;
+        oy += bb.x * pc.y + bb.y * pc.x;
+      }
+      w[g] = make_double2(ox, oy);
+    }
+    __syncthreads();
+    const double a = lds_real_dot(v1, w, npw, red);
+    const double alpha = 1.0, cshift = a;
+    for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
+      const double2 h = w[g], y = v1[g], x = v0[g];
+      double2 o;
+      o.x = alpha * (h.x - cshift * y.x);
+      o.y = alpha * (h.y - cshift * y.y);
+      o.x -= beta * x.x;
+      o.y -= beta * x.y;
+      w[g] = o;
+    }
+    __syncthreads();
+    const double b = sqrt(fmax(lds_real_dot(w, w, npw, red), 0.0));
+    if (threadIdx.x == 0) {
+      ab[(size_t)j * nke + kb] = a;
+      ab[(size_t)(steps + j) * nke + kb] = b;
+    }
+    beta = b;
+    if (j + 1 == steps) break;
+    const double inv = b < 1e-12 ? 0.0 : 1.0 / b;
+    double2* s0 = v0;
+    v0 = v1;
+    for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
+      w[g].x *= inv;
+      w[g].y *= inv;
+    }
+    v1 = w;
+    w = s0;
+  }
+}
+
 // The whole Chebyshev filter pass of one band in one workgroup: the `degree` steps of the scaled
 // three-term recurrence
 //   Y_1 = α_0 (H − c_0) Y_0,   Y_{i+1} = α_i (H − c_i) Y_i − β_i Y_{i−1}
@@ -244,6 +419,7 @@ __device__ __forceinline__ void cheb_filter_body(int bid, int nblocks, const dou
   V2* proj = G + N * N * N;
   V2* cur = proj + nch;
   V2* prv = cur + npw;
+  double* cfl = reinterpret_cast<double*>(prv + npw);  // this block's (α, c, β) of every step
   const int nblk = nblocks / nbk;
   // XCD-aware band order, as in k_hpsi_small
   const int q = nblocks >> 3, r = nblocks & 7, xcd = bid & 7;
@@ -253,9 +429,7 @@ __device__ __forceinline__ void cheb_filter_body(int bid, int nblocks, const dou
   const T* t = kin + (size_t)k * npw;
   const V2* bk = B + (size_t)k * npw * nch;
   const V2* dk = DBH + (size_t)k * nch * npw;
-  constexpr int NW = SMALL_NT / 64, CPW = 4;
   constexpr bool PRE_V = N * N <= SMALL_NT;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   T vline[PRE_V ? N : 1];
   if constexpr (PRE_V) {
     if (threadIdx.x < N * N) {
@@ -269,59 +443,18 @@ __device__ __forceinline__ void cheb_filter_body(int bid, int nblocks, const dou
     cur[g].x = (T)x.x;
     cur[g].y = (T)x.y;
   }
+  // the coefficients are loaded now (they may sit in host memory) and stored to LDS after the
+  // first transforms, so their latency hides behind them (3·degree <= SMALL_NT)
+  const double cpre = (int)threadIdx.x < 3 * degree
+                          ? coef[((size_t)(threadIdx.x / 3) * nblk + kb) * 3 + threadIdx.x % 3]
+                          : 0.0;
   for (int step = 0; step < degree; ++step) {
-    for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
-      G[i].x = 0;
-      G[i].y = 0;
+    small_h_core<T, N>(G, proj, cur, vline, vr, g2fft, npw, nch, dk);
+    if (step == 0) {
+      if ((int)threadIdx.x < 3 * degree) cfl[threadIdx.x] = cpre;
+      lds_barrier();
     }
-    lds_barrier();
-    for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = cur[g];
-    // projections P_c = Σ_G (D·B^H)[c, G] Y(G): one wave per channel, up to four side by side
-    for (int c0 = wave; c0 < nch; c0 += NW * CPW) {
-      T ar[CPW], ai[CPW];
-#pragma unroll
-      for (int j = 0; j < CPW; ++j) ar[j] = ai[j] = 0;
-      for (int g = lane; g < npw; g += 64) {
-        const V2 v = cur[g];
-        V2 d[CPW];
-#pragma unroll
-        for (int j = 0; j < CPW; ++j)
-          if (c0 + j * NW < nch) d[j] = dk[(size_t)g * nch + c0 + j * NW];
-#pragma unroll
-        for (int j = 0; j < CPW; ++j)
-          if (c0 + j * NW < nch) {
-            ar[j] += d[j].x * v.x - d[j].y * v.y;
-            ai[j] += d[j].x * v.y + d[j].y * v.x;
-          }
-      }
-#pragma unroll
-      for (int j = 0; j < CPW; ++j) {
-        for (int off = 32; off > 0; off >>= 1) {
-          ar[j] += __shfl_xor(ar[j], off, 64);
-          ai[j] += __shfl_xor(ai[j], off, 64);
-        }
-        if (lane == 0 && c0 + j * NW < nch) {
-          proj[c0 + j * NW].x = ar[j];
-          proj[c0 + j * NW].y = ai[j];
-        }
-      }
-    }
-    lds_barrier();
-    lds_lines<T, N, 2, 1>(G, nullptr);
-    lds_barrier();
-    lds_lines<T, N, 1, 1>(G, nullptr);
-    lds_barrier();
-    if constexpr (PRE_V)
-      lds_xlines_v<T, N, 1>(G, vline);
-    else
-      lds_lines<T, N, 0, 1>(G, vr);
-    lds_barrier();
-    lds_lines<T, N, 1, -1>(G, nullptr);
-    lds_barrier();
-    lds_lines<T, N, 2, -1>(G, nullptr);
-    lds_barrier();
-    const double* cf = coef + ((size_t)step * nblk + kb) * 3;
-    const T alpha = (T)cf[0], cshift = (T)cf[1], beta = (T)cf[2];
+    const T alpha = (T)cfl[3 * step], cshift = (T)cfl[3 * step + 1], beta = (T)cfl[3 * step + 2];
     for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
       const V2 v = cur[g], w = G[g2fft[g]];
       const T tk = t[g];
@@ -408,8 +541,8 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_multi(PassSmallMulti m, int n
 // registers (bands with fw < 1e-15 skipped, src/scf/density.rs:14,72); the partial sums of
 // the groups are added in group order by k_density_reduce, so the result does not depend on
 // scheduling.
-template <typename T, int N>
-__global__ void __launch_bounds__(SMALL_NT) k_density_small(const typename V2s<T>::t* __restrict__ psi,
+template <typename T, int N, typename TIn>
+__global__ void __launch_bounds__(SMALL_NT) k_density_small(const TIn* __restrict__ psi,
                                                             const int* __restrict__ g2fft, int npw, int nb, int gsize,
                                                             const double* __restrict__ fw,
                                                             double* __restrict__ partial) {
@@ -432,8 +565,14 @@ __global__ void __launch_bounds__(SMALL_NT) k_density_small(const typename V2s<T
       G[i].y = 0;
     }
     __syncthreads();
-    const V2* p = psi + (size_t)b * npw;
-    for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = p[g];
+    const TIn* p = psi + (size_t)b * npw;
+    for (int g = threadIdx.x; g < npw; g += SMALL_NT) {  // FP64 input is rounded to T here
+      const TIn v = p[g];
+      V2 w;
+      w.x = (T)v.x;
+      w.y = (T)v.y;
+      G[g2fft[g]] = w;
+    }
     __syncthreads();
     lds_lines<T, N, 2, 1>(G, nullptr);
     __syncthreads();
@@ -533,9 +672,12 @@ void launch_small_t(Context& c, int nbands, int nbk, const void* psi, void* hpsi
   PWK_HIP(hipGetLastError());
 }
 
+constexpr int kMaxFilterDegree = 64;  // 3 coefficients per step, one thread each
+
 template <typename T>
 size_t filter_lds_bytes(int n, int nch, int npw) {
-  return ((size_t)n * n * n + (size_t)nch + 2 * (size_t)npw) * sizeof(typename V2s<T>::t);
+  return ((size_t)n * n * n + (size_t)nch + 2 * (size_t)npw) * sizeof(typename V2s<T>::t) +
+         3 * kMaxFilterDegree * sizeof(double);
 }
 
 template <typename T>
@@ -563,9 +705,8 @@ void launch_filter_small_t(Context& c, int nbands, int nbk, const void* V, void*
   PWK_HIP(hipGetLastError());
 }
 
-template <typename T>
+template <typename T, typename TIn>
 void launch_density_small_t(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, bool overwrite) {
-  using V2 = typename V2s<T>::t;
   const int n = c.dims[0];
   const size_t sh = small_lds_bytes<T>(n, 0);
   const int gsize = std::max(1, (nb + 127) / 128);  // at most 128 groups
@@ -575,8 +716,9 @@ void launch_density_small_t(Context& c, int nb, const void* psi, const double* f
   switch (n) {
 #define PWK_CASE(L)                                                                                            \
   case L:                                                                                                      \
-    ensure_lds_limit((const void*)k_density_small<T, L>, sh);                                                 \
-    k_density_small<T, L><<<(unsigned)ng, SMALL_NT, sh, c.stream>>>((const V2*)psi, c.g2fft.as<int>(), c.npw,  \
+    ensure_lds_limit((const void*)k_density_small<T, L, TIn>, sh);                                            \
+    k_density_small<T, L, TIn><<<(unsigned)ng, SMALL_NT, sh, c.stream>>>((const TIn*)psi, c.g2fft.as<int>(),  \
+                                                                         c.npw,                             \
                                                                     nb, gsize, fw_dev, c.work_a.as<double>());  \
     break;
     PWK_SMALL_SIZES(PWK_CASE)
@@ -593,14 +735,16 @@ void launch_density_small_t(Context& c, int nb, const void* psi, const double* f
 }  // namespace
 
 void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision,
-                          bool overwrite) {
+                          bool overwrite, bool psi_fp64) {
   c.check_basis();
   PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
   PWK_REQUIRE(density_small_supported(c, precision), PWK_E_UNSUPPORTED, "grid too large for the fused kernel");
   if (precision == PWK_FP64)
-    launch_density_small_t<double>(c, nb, psi, fw_dev, rho, overwrite);
+    launch_density_small_t<double, double2>(c, nb, psi, fw_dev, rho, overwrite);
+  else if (psi_fp64)
+    launch_density_small_t<float, double2>(c, nb, psi, fw_dev, rho, overwrite);
   else
-    launch_density_small_t<float>(c, nb, psi, fw_dev, rho, overwrite);
+    launch_density_small_t<float, float2>(c, nb, psi, fw_dev, rho, overwrite);
 }
 
 bool density_small_supported(const Context& c, int precision) {
@@ -631,7 +775,7 @@ void launch_cheb_filter_small(Context& c, int nbands, int nbk, const void* V, vo
   c.check_basis();
   PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
   PWK_REQUIRE(nbands > 0 && nbk > 0 && nbands % nbk == 0, PWK_E_INVALID, "band counts must be positive");
-  PWK_REQUIRE(degree >= 1, PWK_E_INVALID, "filter degree must be positive");
+  PWK_REQUIRE(degree >= 1 && degree <= kMaxFilterDegree, PWK_E_INVALID, "filter degree must be in 1..64");
   PWK_REQUIRE(cheb_filter_small_supported(c, precision, nch), PWK_E_UNSUPPORTED,
               "grid too large for the fused filter kernel");
   if (nkv > 0) {
@@ -668,8 +812,9 @@ void launch_filter_multi_t(hipStream_t stream, const PassShape& sh, const PassSm
 }  // namespace
 
 void launch_filter_multi(hipStream_t stream, const PassShape& sh, const PassSmallMulti& m, int n, int max_bands) {
-  PWK_REQUIRE(n >= 1 && n <= MAX_PASS_BATCH && max_bands >= 1 && sh.degree >= 1 && small_len(sh.n), PWK_E_INVALID,
-              "bad batched filter shape");
+  PWK_REQUIRE(n >= 1 && n <= MAX_PASS_BATCH && max_bands >= 1 && sh.degree >= 1 && sh.degree <= kMaxFilterDegree &&
+                  small_len(sh.n),
+              PWK_E_INVALID, "bad batched filter shape");
   const size_t lds = sh.fp == PWK_FP64 ? filter_lds_bytes<double>(sh.n, sh.nch, sh.npw)
                                        : filter_lds_bytes<float>(sh.n, sh.nch, sh.npw);
   PWK_REQUIRE(lds <= kLdsMax, PWK_E_UNSUPPORTED, "grid too large for the fused filter kernel");
@@ -690,6 +835,43 @@ void launch_hpsi64_multi(hipStream_t stream, const PassShape& sh, const PassSmal
   case L:                                                                                                 \
     ensure_lds_limit((const void*)k_hpsi_multi<double, L>, lds);                                         \
     k_hpsi_multi<double, L><<<grid, SMALL_NT, lds, stream>>>(m, sh.npw, sh.nbs, sh.nch, sh.inv_n);       \
+    break;
+    PWK_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "grid has no fused small-grid kernel");
+  }
+  PWK_HIP(hipGetLastError());
+}
+
+size_t lanczos_small_lds_bytes(int n, int nch, int npw) {
+  return ((size_t)n * n * n + (size_t)nch + 3 * (size_t)npw) * sizeof(double2);
+}
+
+bool lanczos_small_supported(const Context& c, int nch) {
+  return hpsi_small_supported(c, PWK_FP64, nch) && lanczos_small_lds_bytes(c.dims[0], nch, c.npw) <= kLdsMax;
+}
+
+void launch_lanczos_small(Context& c, int nke, int nkv, int steps, const double* kin, int nch, const void* B,
+                          const void* DBH, double* ab) {
+  c.check_basis();
+  PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
+  PWK_REQUIRE(nke > 0 && steps >= 1, PWK_E_INVALID, "bad Lanczos shape");
+  PWK_REQUIRE(lanczos_small_supported(c, nch), PWK_E_UNSUPPORTED, "grid too large for the fused Lanczos kernel");
+  if (nkv > 0) {
+    const size_t slots = ((size_t)nke + nkv - 1) / nkv;
+    PWK_REQUIRE(c.vrn64.bytes >= slots * c.N * sizeof(double), PWK_E_STATE, "V_eff slots not set");
+  }
+  const size_t sh = lanczos_small_lds_bytes(c.dims[0], nch, c.npw);
+  TimedScope ts(c, "lanczos_small");
+  switch (c.dims[0]) {
+#define PWK_CASE(L)                                                                                              \
+  case L:                                                                                                        \
+    ensure_lds_limit((const void*)k_lanczos_small<L>, sh);                                                      \
+    k_lanczos_small<L><<<(unsigned)nke, SMALL_NT, sh, c.stream>>>(c.g2fft.as<int>(), c.npw, nch, kin,           \
+                                                                c.vrn64.as<double>(), (const double2*)B,        \
+                                                                (const double2*)DBH, 1.0 / (double)c.N, nkv,   \
+                                                                steps, ab);                                     \
     break;
     PWK_SMALL_SIZES(PWK_CASE)
 #undef PWK_CASE

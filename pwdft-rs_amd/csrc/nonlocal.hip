@@ -1,5 +1,5 @@
-// Kleinman-Bylander projector matrices of one k-point built on the device
-// (src/potential/nonlocal.rs:190-382):
+// Kleinman-Bylander projector matrices of a batch of k-points built on the device, each kernel
+// over all of them at once (blockIdx.y = k-point), src/potential/nonlocal.rs:190-382:
 //   F_i(q)   = 4π ∫ χ_i(r) j_l(q r) r dr   (Simpson, src/numerics.rs:29-58)
 //   B[G, c]  = Ω^{-1/2} e^{-iG·τ_a} F_i(|k+G|) Y_lm(k+G)   (channel c = (atom a, projector i, m))
 //   DBH[c,G] = Σ_{j: l_j = l_i} D_ij conj(B[G, (a, j, m)])
@@ -55,6 +55,8 @@ __global__ void __launch_bounds__(256) k_form_factors(int npw, int nprojtot, con
   const int lane = threadIdx.x & 63;
   const long pair = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (pair >= (long)npw * nprojtot) return;  // wave-uniform
+  qn += (size_t)blockIdx.y * npw;
+  F += (size_t)blockIdx.y * nprojtot * npw;
   const int ig = (int)(pair % npw), p = (int)(pair / npw);
   const double q = qn[ig];
   const int l = p_l[p], n = p_mesh[p];
@@ -116,10 +118,13 @@ __global__ void __launch_bounds__(256) k_build_b(int npw, int nch, int lmax, con
                                                  const double* __restrict__ gvec, const double* __restrict__ tau,
                                                  const int* __restrict__ ch_atom, const int* __restrict__ ch_proj,
                                                  const int* __restrict__ ch_l, const int* __restrict__ ch_m,
-                                                 const double* __restrict__ F, double inv_sqrt_omega,
+                                                 const double* __restrict__ F, int nprojtot, double inv_sqrt_omega,
                                                  double2* __restrict__ B) {
   int ig = blockIdx.x * blockDim.x + threadIdx.x;
   if (ig >= npw) return;
+  kvec += 3 * blockIdx.y;
+  F += (size_t)blockIdx.y * nprojtot * npw;
+  B += (size_t)blockIdx.y * npw * nch;
   double gx = gvec[3 * ig], gy = gvec[3 * ig + 1], gz = gvec[3 * ig + 2];
   double y[16];
   ylm_dev(kvec[0] + gx, kvec[1] + gy, kvec[2] + gz, lmax, y);
@@ -149,6 +154,8 @@ __global__ void __launch_bounds__(256) k_build_dbh(int npw, int nch, int ntrip, 
                                                    const double2* __restrict__ B, double2* __restrict__ DBH) {
   int ig = blockIdx.x * blockDim.x + threadIdx.x;
   if (ig >= npw) return;
+  B += (size_t)blockIdx.y * npw * nch;
+  DBH += (size_t)blockIdx.y * npw * nch;
   double2* row = DBH + (size_t)ig * nch;
   for (int c = 0; c < nch; ++c) row[c] = make_double2(0.0, 0.0);
   for (int t = 0; t < ntrip; ++t) {
@@ -166,6 +173,8 @@ __global__ void __launch_bounds__(256) k_qnorm(int npw, const double* __restrict
                                                const double* __restrict__ gvec, double* __restrict__ qn) {
   int ig = blockIdx.x * blockDim.x + threadIdx.x;
   if (ig >= npw) return;
+  kvec += 3 * blockIdx.y;
+  qn += (size_t)blockIdx.y * npw;
   double x = kvec[0] + gvec[3 * ig], y = kvec[1] + gvec[3 * ig + 1], z = kvec[2] + gvec[3 * ig + 2];
   qn[ig] = sqrt(x * x + y * y + z * z);
 }
@@ -202,8 +211,17 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
                            const int32_t* atom_species, int nspecies, const int32_t* nproj, const int32_t* proj_l,
                            const double* dij, const int32_t* mesh, const double* r, const double* rab,
                            const double* chi, double omega, const NonlocalDest* dest) {
+  build_nonlocal_batch(c, ik, 1, kvec, gvec, natoms, tau, atom_species, nspecies, nproj, proj_l, dij, mesh, r, rab,
+                       chi, omega, dest);
+}
+
+void build_nonlocal_batch(Context& c, int ik0, int nkb, const double* kvecs, const double* gvec, int natoms,
+                          const double* tau, const int32_t* atom_species, int nspecies, const int32_t* nproj,
+                          const int32_t* proj_l, const double* dij, const int32_t* mesh, const double* r,
+                          const double* rab, const double* chi, double omega, const NonlocalDest* dest) {
   c.check_basis();
-  KData& kd = c.k(ik);
+  PWK_REQUIRE(nkb >= 1 && (nkb == 1 || dest), PWK_E_INVALID, "a batch of k-points needs a destination");
+  for (int i = 0; i < nkb; ++i) c.k(ik0 + i);  // range check
   const int npw = c.npw;
   PWK_REQUIRE(natoms >= 0 && nspecies > 0, PWK_E_INVALID, "bad atom/species counts");
   // per-species offsets into the concatenated arrays
@@ -262,11 +280,11 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
   }
   int nch = (int)ch_atom.size();
   c.pack_valid = false;
-  kd.nch = nch;
+  for (int i = 0; i < nkb; ++i) c.k(ik0 + i).nch = nch;
   if (nch == 0) return;
   hipStream_t st = c.stream;
   Pack pk;
-  const size_t o_k = pk.add(kvec, 3), o_g = pk.add(gvec, (size_t)npw * 3), o_tau = pk.add(tau, (size_t)natoms * 3);
+  const size_t o_k = pk.add(kvecs, (size_t)3 * nkb), o_g = pk.add(gvec, (size_t)npw * 3), o_tau = pk.add(tau, (size_t)natoms * 3);
   const size_t o_pl = pk.add(pl), o_pm = pk.add(pmesh), o_pro = pk.add(profs), o_pco = pk.add(pcofs);
   const size_t o_r = pk.add(r, (size_t)m0[nspecies]), o_rab = pk.add(rab, (size_t)m0[nspecies]);
   const size_t o_chi = pk.add(chi, (size_t)c0[nspecies]);
@@ -279,12 +297,13 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
   const PackView d_k = at(o_k), d_g = at(o_g), d_tau = at(o_tau), d_pl = at(o_pl), d_pm = at(o_pm), d_pro = at(o_pro),
              d_pco = at(o_pco), d_r = at(o_r), d_rab = at(o_rab), d_chi = at(o_chi), d_ca = at(o_ca),
              d_cp = at(o_cp), d_cl = at(o_cl), d_cm = at(o_cm), d_ti = at(o_ti), d_tj = at(o_tj), d_td = at(o_td);
-  d_qn.ensure((size_t)npw * sizeof(double));
-  d_F.ensure((size_t)std::max(1, nprojtot) * npw * sizeof(double));
-  k_qnorm<<<blocks_for((size_t)npw, 256), 256, 0, st>>>(npw, d_k.as<double>(), d_g.as<double>(), d_qn.as<double>());
+  d_qn.ensure((size_t)nkb * npw * sizeof(double));
+  d_F.ensure((size_t)nkb * std::max(1, nprojtot) * npw * sizeof(double));
+  const dim3 gpw(blocks_for((size_t)npw, 256), (unsigned)nkb);
+  k_qnorm<<<gpw, 256, 0, st>>>(npw, d_k.as<double>(), d_g.as<double>(), d_qn.as<double>());
   PWK_HIP(hipGetLastError());
   if (nprojtot > 0) {
-    const unsigned g = blocks_for((size_t)npw * nprojtot, 4);
+    const dim3 g(blocks_for((size_t)npw * nprojtot, 4), (unsigned)nkb);
     k_form_factors<<<g, 256, 0, st>>>(npw, nprojtot, d_qn.as<double>(), d_pl.as<int>(), d_pm.as<int>(),
                                       d_pro.as<long>(), d_pco.as<long>(), d_r.as<double>(), d_rab.as<double>(),
                                       d_chi.as<double>(), d_F.as<double>());
@@ -298,6 +317,7 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
     b32 = dest->b32;
     dbh32 = dest->dbh32;
   } else {
+    KData& kd = c.k(ik0);
     kd.B64.ensure((size_t)npw * nch * sizeof(double2));
     kd.DBH64.ensure((size_t)npw * nch * sizeof(double2));
     kd.B32.ensure((size_t)npw * nch * sizeof(float2));
@@ -307,18 +327,16 @@ void build_nonlocal_device(Context& c, int ik, const double* kvec, const double*
     b32 = kd.B32.p;
     dbh32 = kd.DBH32.p;
   }
-  k_build_b<<<blocks_for((size_t)npw, 256), 256, 0, st>>>(npw, nch, lmax, d_k.as<double>(), d_g.as<double>(),
-                                                          d_tau.as<double>(), d_ca.as<int>(), d_cp.as<int>(),
-                                                          d_cl.as<int>(), d_cm.as<int>(), d_F.as<double>(),
-                                                          1.0 / std::sqrt(omega), static_cast<double2*>(b64));
+  // the destination holds the k-points' blocks back to back (npw × nch each)
+  k_build_b<<<gpw, 256, 0, st>>>(npw, nch, lmax, d_k.as<double>(), d_g.as<double>(), d_tau.as<double>(),
+                                 d_ca.as<int>(), d_cp.as<int>(), d_cl.as<int>(), d_cm.as<int>(), d_F.as<double>(),
+                                 nprojtot, 1.0 / std::sqrt(omega), static_cast<double2*>(b64));
   PWK_HIP(hipGetLastError());
-  k_build_dbh<<<blocks_for((size_t)npw, 256), 256, 0, st>>>(npw, nch, (int)tri_d.size(), d_ti.as<int>(),
-                                                            d_tj.as<int>(), d_td.as<double>(),
-                                                            static_cast<const double2*>(b64),
-                                                            static_cast<double2*>(dbh64));
+  k_build_dbh<<<gpw, 256, 0, st>>>(npw, nch, (int)tri_d.size(), d_ti.as<int>(), d_tj.as<int>(), d_td.as<double>(),
+                                   static_cast<const double2*>(b64), static_cast<double2*>(dbh64));
   PWK_HIP(hipGetLastError());
-  launch_convert(c, b64, b32, (size_t)npw * nch * 2, true);
-  launch_convert(c, dbh64, dbh32, (size_t)npw * nch * 2, true);
+  launch_convert(c, b64, b32, (size_t)nkb * npw * nch * 2, true);
+  launch_convert(c, dbh64, dbh32, (size_t)nkb * npw * nch * 2, true);
   // temporaries are freed at scope exit; make sure the kernels have consumed them
   PWK_HIP(hipStreamSynchronize(st));
 }

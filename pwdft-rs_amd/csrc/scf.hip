@@ -1128,7 +1128,12 @@ struct Solve {
   std::map<int, hipGraphExec_t> pass_graphs;  // captured filter passes per (channel, precision)
   std::set<int> pass_warm;
   double* rr_pinned = nullptr;                // pinned: Ritz values and residual norms
+  PinnedBuf coef_pin;                         // pinned: filter coefficients of the one-launch pass
+  PinnedBuf fw_pin;                           // pinned: band weights of the small-grid density
   void lanczos_bounds();
+  void lanczos_steps(int steps, size_t per, double2* v0, double2* v1, double2* w, DevBuf& none, DevBuf& red,
+                     DevBuf& cf, DevBuf& beta, DevBuf& inv, double* al_d, double* be_d);
+  void lanczos_finish(int steps, DevBuf& ab);
   double chebyshev_pass(int prec);
   void eig_chebyshev(bool first);
   void veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out, double* vxc_out);
@@ -1143,7 +1148,7 @@ struct Solve {
   const double* rho_tot_new = nullptr;  // output total density of the latest iteration
   std::vector<DotTerm> veff_stage();
   std::vector<DotTerm> output_stage();
-  void symmetrize(double* rn);
+  void symmetrize(double* rn, double scale = 1.0);
   IterScalars scalars(const double* har, const double* out) const;
   void finalize(const std::vector<double>& occ, const IterScalars& v, double e_total, double e_harris, double ef,
                 double ts, double e_band, const double* n_s);
@@ -1467,21 +1472,26 @@ void Solve::setup() {
   }
   PWK_HIP(hipMemcpyAsync(kin_all64.p, tk_all.data(), tk_all.size() * 8, hipMemcpyHostToDevice, c.stream));
   launch_convert(c, kin_all64.p, kin_all32.p, tk_all.size(), true);
-  for (int k = 0; k < nk; ++k) {
-    const Vec3 kc = kpts[(size_t)k].cart;
-    KData& kd = c.k(k);
-    const double kv[3] = {kc.x, kc.y, kc.z};
-    // projectors of all k-points in contiguous blocks for the batched GEMMs, built in place
+  {
+    // projectors of all k-points in contiguous blocks for the batched products, built in place,
+    // every k-point in the same launches
+    std::vector<double> kvs((size_t)3 * nk);
+    for (int k = 0; k < nk; ++k) {
+      const Vec3 kc = kpts[(size_t)k].cart;
+      kvs[(size_t)3 * k] = kc.x;
+      kvs[(size_t)3 * k + 1] = kc.y;
+      kvs[(size_t)3 * k + 2] = kc.z;
+    }
     NonlocalDest dst;
     dst.nch = nch_all;
-    const size_t o = (size_t)k * npw * nch_all;
-    dst.b64 = Ball64.as<double2>() + o;
-    dst.dbh64 = DBHall64.as<double2>() + o;
-    dst.b32 = Ball32.as<float2>() + o;
-    dst.dbh32 = DBHall32.as<float2>() + o;
-    build_nonlocal_device(c, k, kv, gflat.data(), cr.natoms, tau.data(), spi.data(), cr.nspecies, nproj.data(),
-                          pl.data(), dij.data(), mesh.data(), rr.data(), rab.data(), chi.data(), omega, &dst);
-    kd.nch = nch_all;
+    dst.b64 = Ball64.p;
+    dst.dbh64 = DBHall64.p;
+    dst.b32 = Ball32.p;
+    dst.dbh32 = DBHall32.p;
+    build_nonlocal_batch(c, 0, nk, kvs.data(), gflat.data(), cr.natoms, tau.data(), spi.data(), cr.nspecies,
+                         nproj.data(), pl.data(), dij.data(), mesh.data(), rr.data(), rab.data(), chi.data(), omega,
+                         &dst);
+    for (int k = 0; k < nk; ++k) c.k(k).nch = nch_all;
   }
   // symmetry tables for the density
   PWK_REQUIRE(ops.size() <= (size_t)SYM_MAX_OPS, PWK_E_UNSUPPORTED, "more symmetry operations than the gather kernel takes");
@@ -1608,9 +1618,7 @@ double Solve::ewald() {
 // V_eff(G) = V_loc + V_H[ρ] + FFT(v_xc[ρ + ρ_core])/N  (src/scf/driver.rs:359-431)
 void Solve::veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out,
                       double* vxc_out) {
-  k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_r, have_core ? core_r.as<double>() : nullptr, 1.0, rho_xc_out);
-  PWK_HIP(hipGetLastError());
-  launch_lda(c, rho_xc_out, exc_out, vxc_out, PWK_FP64);
+  launch_lda_core(c, rho_r, have_core ? core_r.as<double>() : nullptr, 1.0, rho_xc_out, exc_out, vxc_out);
   r_to_g(vxc_out, vxc_g.as<double2>());
   launch_hartree_veff(c, rho_g, 4.0 * kPiC * kE2, vloc_g.p, vxc_g.p, veff_g.p, PWK_FP64, true, true, true);
 }
@@ -1750,21 +1758,33 @@ void Solve::lanczos_bounds() {
   TimedScope ts(c, "scf_lanczos");
   const int steps = std::min(10, npw);
   const size_t per = (size_t)npw * 2;  // reals per k
-  DevBuf a, b0, b1, none, red, cf, beta, inv, ab;
-  a.ensure((size_t)nke * npw * 16);
-  b0.ensure((size_t)nke * npw * 16);
-  b1.ensure((size_t)nke * npw * 16);
-  none.ensure((size_t)nke * 4);
-  red.ensure((size_t)nke * 8);
-  cf.ensure((size_t)nke * 3 * 8);
-  beta.ensure((size_t)nke * 8);
-  inv.ensure((size_t)nke * 8);
+  DevBuf ab;
   ab.ensure((size_t)2 * steps * nke * 8);  // α_j (rows 0..steps), then β_j
   double* al_d = ab.as<double>();
   double* be_d = al_d + (size_t)steps * nke;
-  double2* v0 = a.as<double2>();
-  double2* v1 = b0.as<double2>();
-  double2* w = b1.as<double2>();
+  if (c.fused_filter && lanczos_small_supported(c, nch_all)) {
+    // small cubic grids: every step of every block in one launch, vectors in LDS
+    launch_lanczos_small(c, nke, fold ? nk : 0, steps, kin_all64.as<double>(), nch_all, Ball64.p, DBHall64.p, al_d);
+  } else {
+    DevBuf a, b0, b1, none, red, cf, beta, inv;
+    a.ensure((size_t)nke * npw * 16);
+    b0.ensure((size_t)nke * npw * 16);
+    b1.ensure((size_t)nke * npw * 16);
+    none.ensure((size_t)nke * 4);
+    red.ensure((size_t)nke * 8);
+    cf.ensure((size_t)nke * 3 * 8);
+    beta.ensure((size_t)nke * 8);
+    inv.ensure((size_t)nke * 8);
+    lanczos_steps(steps, per, a.as<double2>(), b0.as<double2>(), b1.as<double2>(), none, red, cf, beta, inv, al_d,
+                  be_d);
+  }
+  lanczos_finish(steps, ab);
+}
+
+// The Lanczos recurrence as step launches (any grid): start vector, then per step H·v, α, the
+// three-term update, β and the normalisation, batched over the band blocks.
+void Solve::lanczos_steps(int steps, size_t per, double2* v0, double2* v1, double2* w, DevBuf& none, DevBuf& red,
+                          DevBuf& cf, DevBuf& beta, DevBuf& inv, double* al_d, double* be_d) {
   const std::vector<int> minus((size_t)nke, -1);
   PWK_HIP(hipMemcpyAsync(none.p, minus.data(), minus.size() * 4, hipMemcpyHostToDevice, c.stream));
   launch_start_block(c, npw, 1, nke, none.as<int>(), v1);
@@ -1790,6 +1810,11 @@ void Solve::lanczos_bounds() {
     launch_block_scale(c, per, nke, inv.as<double>(), 0, (double*)w);
     std::swap(v1, w);  // v1 <- v_{j+1}
   }
+}
+
+// Upper bound per block from the recurrence coefficients (Gershgorin on the tridiagonal matrix
+// of the steps kept, plus the last β)
+void Solve::lanczos_finish(int steps, DevBuf& ab) {
   std::vector<double> abh((size_t)2 * steps * nke);
   PWK_HIP(hipMemcpyAsync(abh.data(), ab.p, abh.size() * 8, hipMemcpyDeviceToHost, c.stream));
   c.sync();
@@ -1838,18 +1863,29 @@ double Solve::chebyshev_pass(int fprec) {
       sigma = sn;
     }
   }
-  PWK_HIP(hipMemcpyAsync(coef.p, coef_host.data(), coef_host.size() * 8, hipMemcpyHostToDevice, c.stream));
   // the pass's launches: filter, FP64 H·ψ of the filtered block, Rayleigh-Ritz. The addresses
   // depend only on the channel and the filter precision, so on the fused small-grid path the
   // sequence is captured once per (channel, precision) as a HIP graph and replayed (one launch
   // call instead of fourteen); the coefficients are uploaded before each replay
   const bool fused_filter = c.fused_filter && cheb_filter_small_supported(c, fprec, nch_all);
+  const bool graph = c.graphs && !c.prof.on && rr_small_supported(nbs) && hpsi_small_supported(c, PWK_FP64, nch_all) &&
+                     hpsi_small_supported(c, fprec, nch_all) && fprec != PWK_SPLIT_BF16;
+  // the one-launch filter reads its coefficients once, so it takes them from pinned host memory
+  // (no upload); the previous pass has completed (its results were read), so the buffer is free
+  const double* cf_dev = coef.as<double>();
+  if (fused_filter && !graph) {
+    coef_pin.ensure(coef_host.size() * 8);
+    std::memcpy(coef_pin.p, coef_host.data(), coef_host.size() * 8);
+    cf_dev = coef_pin.as<double>();
+  } else {
+    PWK_HIP(hipMemcpyAsync(coef.p, coef_host.data(), coef_host.size() * 8, hipMemcpyHostToDevice, c.stream));
+  }
   auto body = [&]() {
     if (fused_filter) {
       // the whole recurrence of every band in one launch, result straight into Y64
       const bool f64 = fprec == PWK_FP64;
       launch_cheb_filter_small(c, nke * nbs, nbs, Vc(), Y64.p, fprec, f64 ? kin_all64.p : kin_all32.p, nch_all,
-                               f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, coef.as<double>(), degree,
+                               f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, cf_dev, degree,
                                fold ? nk : 0);
       hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
       rr_launch(Y64.as<double2>(), HY64.as<double2>());
@@ -1876,8 +1912,6 @@ double Solve::chebyshev_pass(int fprec) {
     hpsi_all(Y64.p, HY64.p, nbs, PWK_FP64);
     rr_launch(Y64.as<double2>(), HY64.as<double2>());
   };
-  const bool graph = c.graphs && !c.prof.on && rr_small_supported(nbs) && hpsi_small_supported(c, PWK_FP64, nch_all) &&
-                     hpsi_small_supported(c, fprec, nch_all) && fprec != PWK_SPLIT_BF16;
   if (c.batcher && fused_filter && !graph && rr_small_supported(nbs) && hpsi_small_supported(c, PWK_FP64, nch_all)) {
     // served solve: the pass may share its three launches with the passes of other solves
     // (the coefficients above are uploaded on this stream before the pass's start is marked)
@@ -1895,7 +1929,7 @@ double Solve::chebyshev_pass(int fprec) {
     sh.inv_n = 1.0 / (double)c.N;
     PassReq r{};
     r.filt = {Vc(), Y64.p, f64 ? kin_all64.p : kin_all32.p, f64 ? c.vrn64.p : c.vrn32.p, c.g2fft.as<int>(),
-              f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, coef.as<double>(), nullptr, nbands, nkv};
+              f64 ? Ball64.p : Ball32.p, f64 ? DBHall64.p : DBHall32.p, cf_dev, nullptr, nbands, nkv};
     r.hpsi = {Y64.p, HY64.p, kin_all64.p, c.vrn64.p, c.g2fft.as<int>(), Ball64.p, DBHall64.p, nullptr, nullptr,
               nbands, nkv};
     r.rr = {Y64.as<double2>(), HY64.as<double2>(), Vc(), rr_pinned, rr_pinned + nkm, nullptr, (int)nke};
@@ -1990,10 +2024,12 @@ std::vector<DotTerm> Solve::veff_stage() {
 }
 
 // G-space symmetrisation of one channel's density (src/symmetry/density/g_space.rs)
-void Solve::symmetrize(double* rn) {
+// ρ ← symmetrised (scale·ρ); scale applies only with more than one operation (the caller scales
+// otherwise)
+void Solve::symmetrize(double* rn, double scale) {
   if (ops.size() <= 1) return;
   TimedScope ts(c, "scf_symmetrize");
-  launch_fft3d_r2c(c, rn, 1.0, cwork.p, -1);
+  launch_fft3d_r2c(c, rn, scale, cwork.p, -1);
   k_sym_gather<<<blocks_for(N, 16), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
                                                        sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
                                                        rho_g_new.as<double2>(), 1.0 / (double)ops.size());
@@ -2010,8 +2046,7 @@ std::vector<DotTerm> Solve::output_stage() {
   if (ns == 1) {
     rho_tot_new = rho_new.as<double>();
     r_to_g(rho_tot_new, rho_g_new.as<double2>());
-    k_add_core<<<gb(N), 256, 0, c.stream>>>(N, rho_tot_new, core, 1.0, rho_xc.as<double>());
-    launch_lda(c, rho_xc.p, exc.p, vxc.p, PWK_FP64);
+    launch_lda_core(c, rho_tot_new, core, 1.0, rho_xc.as<double>(), exc.as<double>(), vxc.as<double>());
     return {DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
             DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
             DotTerm{rho_new.as<double>(), nullptr, vxc.as<double>(), nullptr, 0},
@@ -2217,7 +2252,10 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
         for (int b = 0; b < nb; ++b) n_s[s] += occ[((size_t)s * nk + k) * nb + b] * wk[(size_t)k];
 
     // ---- density per channel (src/scf/density.rs:41-111) -----------------------------------
-    if (p.precision != PWK_FP64) launch_convert(c, V.p, V32.p, (size_t)ns * vblk() * 2, true);
+    const int dprec = p.precision == PWK_FP64 ? PWK_FP64 : PWK_FP32;
+    const bool dsmall = density_small_supported(c, dprec);
+    if (p.precision != PWK_FP64 && !dsmall) launch_convert(c, V.p, V32.p, (size_t)ns * vblk() * 2, true);
+    if (dsmall) fw_pin.ensure((size_t)ns * nk * nbs * 8);
     for (int s = 0; s < ns; ++s) {
       double* rn = rho_new_s(s);
       // the basis is the same G-sphere at every k, so all nk·nb bands go through one launch
@@ -2227,19 +2265,29 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
           fw[(size_t)k * nbs + b] = occ[((size_t)s * nk + k) * nb + b] * wk[(size_t)k];
       {
         TimedScope ts(c, "scf_density");
-        if (p.precision == PWK_FP64)
+        if (dsmall) {
+          // small cubic grids: the weights from pinned host memory (the previous iteration's
+          // density has completed), the FP64 bands read directly
+          double* fwp = fw_pin.as<double>() + (size_t)s * nk * nbs;
+          std::memcpy(fwp, fw.data(), fw.size() * 8);
+          launch_density_small(c, nk * nbs, V.as<double2>() + (size_t)s * vblk(), fwp, rn, dprec, true, true);
+        } else if (p.precision == PWK_FP64) {
           launch_density(c, nk * nbs, V.as<double2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP64, true);
-        else
+        } else {
           launch_density(c, nk * nbs, V32.as<float2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP32, true);
+        }
       }
       dots({DotTerm{rn, nullptr, nullptr, nullptr, 0}}, N);
       const double integral = host_small[0] * dvol;
-      if (std::fabs(integral) > 1e-15) {
-        const double target = ns == 2 ? n_s[s] : n_el;
-        k_scale<<<gb(N), 256, 0, c.stream>>>(N, rn, target / integral);
+      // normalised to the electron count: by the symmetrisation's first transform when there is
+      // one (the same product), else in place
+      double scale = 1.0;
+      if (std::fabs(integral) > 1e-15) scale = (ns == 2 ? n_s[s] : n_el) / integral;
+      if (ops.size() <= 1 && scale != 1.0) {
+        k_scale<<<gb(N), 256, 0, c.stream>>>(N, rn, scale);
         PWK_HIP(hipGetLastError());
       }
-      symmetrize(rn);
+      symmetrize(rn, scale);
     }
 
     // ---- output-density energies -------------------------------------------------------------
@@ -2408,14 +2456,13 @@ void Solve::resident_tail() {
     PWK_HIP(hipGetLastError());
   }
   const int dprec = p.precision == PWK_FP64 ? PWK_FP64 : PWK_FP32;
-  if (dprec != PWK_FP64) launch_convert(c, V.p, V32.p, (size_t)ns * vblk() * 2, true);
   for (int s = 0; s < ns; ++s) {
     double* rn = rho_new_s(s);
     {
+      // the FP64 bands are read directly (rounded to FP32 on load for an FP32 density)
       TimedScope ts(c, "scf_density");
-      const void* psi = dprec == PWK_FP64 ? (const void*)(V.as<double2>() + (size_t)s * vblk())
-                                          : (const void*)(V32.as<float2>() + (size_t)s * vblk());
-      launch_density_small(c, nk * nbs, psi, fw_dev.as<double>() + (size_t)s * nk * nbs, rn, dprec, true);
+      launch_density_small(c, nk * nbs, V.as<double2>() + (size_t)s * vblk(), fw_dev.as<double>() + (size_t)s * nk * nbs,
+                           rn, dprec, true, true);
     }
     dots_dev({DotTerm{rn, nullptr, nullptr, nullptr, 0}}, N, S_INT + s);
     k_scale_to<<<gb(N), 256, 0, c.stream>>>(N, rn, S, S_INT + s, dvol, ns == 2 ? S_NS + s : -1, n_el);

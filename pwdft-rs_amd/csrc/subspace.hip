@@ -163,14 +163,7 @@ __global__ void __launch_bounds__(256) k_start_block(int npw, int nb, const int*
   if (idx >= (size_t)npw * nb) return;
   const int k = blockIdx.y;
   const int g = (int)(idx % npw), j = (int)(idx / npw);
-  uint32_t h = (uint32_t)g * 0x9E3779B1u ^ ((uint32_t)j * 0x85EBCA77u + 0x165667B1u) ^ ((uint32_t)k * 0xC2B2AE3Du);
-  h ^= h >> 15;
-  h *= 0x2C1B3C6Du;
-  h ^= h >> 12;
-  h *= 0x297A2D39u;
-  h ^= h >> 15;
-  const double r1 = (double)(h & 0xFFFF) / 65536.0 - 0.5, r2 = (double)(h >> 16) / 65536.0 - 0.5;
-  double2 v = make_double2(1e-2 * r1, 1e-2 * r2);
+  double2 v = start_vector_entry(g, j, k);
   if (g == lowest[(size_t)k * nb + j]) v.x += 1.0;
   V[(size_t)k * npw * nb + idx] = v;
 }

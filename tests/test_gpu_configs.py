@@ -213,7 +213,11 @@ def test_served_mixed_cells_equal_alone(scf_mod, gpu):
                               precision="fp32", eigensolver=1, nspin=nspin, starting_magnetization={0: mag})
         reqs.append((deck, p, (el, natoms, nspin)))
     print("served mixed cells: alone", flush=True)
-    alone = [scf_mod.run_scf(gpu, d, p) for d, p, _ in reqs]
+    try:
+        gpu.set_resident(False)  # served solves run the host-synchronised iteration
+        alone = [scf_mod.run_scf(gpu, d, p) for d, p, _ in reqs]
+    finally:
+        gpu.set_resident(True)
     print("served mixed cells: served", flush=True)
     srv = scf_mod.Server(0, n_slots=4)
     try:
