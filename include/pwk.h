@@ -89,6 +89,24 @@ int pwk_ctx_set_graphs(pwk_ctx* ctx, int on);
  * Both follow the reference's algorithm; results agree to rounding. */
 int pwk_ctx_set_resident(pwk_ctx* ctx, int on);
 
+/* Full-grid FFT plans (grids that are not small and cubic). No counterpart in
+ * the reference (rustfft plans its transforms on the CPU, src/fft.rs:19-114).
+ * A plan is (ord: radix order of the line transforms, 0 = powers of four
+ * first, 1 = odd radices first; tpb: lanes per workgroup of the line passes,
+ * 64/128/256; zlds: 1 stages the contiguous pass through LDS). With autotune
+ * on (the default), the first transform of a (grid, precision, batch) in the
+ * process times every candidate plan on the context's stream and keeps the
+ * fastest for the whole process; tpb and zlds change no arithmetic, ord
+ * changes the rounding only. pwk_ctx_set_fft_plan forces one plan on this
+ * context (ord < 0 clears it). pwk_fft_plan_query returns the plan the
+ * context would use for its grid (tuning it now if needed) with the tuned and
+ * the default plan's microseconds per transform and the candidate count
+ * (0 when no tuning ran: forced plan, autotune off or stream capture). */
+int pwk_ctx_set_fft_autotune(pwk_ctx* ctx, int on);
+int pwk_ctx_set_fft_plan(pwk_ctx* ctx, int ord, int tpb, int zlds);
+int pwk_fft_plan_query(pwk_ctx* ctx, int batch, int precision, int plan[3], double* us_tuned, double* us_default,
+                       int* candidates);
+
 /* ---------------------------------------------------------------------------
  * Kernel timing (no counterpart in the reference, which has no GPU timers):
  * when enabled, every kernel the context launches is bracketed with HIP

@@ -147,6 +147,46 @@ int pwk_ctx_set_resident(pwk_ctx* ctx, int on) {
   return guarded([&] { ctx_of(ctx).resident = on != 0; });
 }
 
+int pwk_ctx_set_fft_autotune(pwk_ctx* ctx, int on) {
+  return guarded([&] { ctx_of(ctx).fft_autotune = on != 0; });
+}
+
+int pwk_ctx_set_fft_plan(pwk_ctx* ctx, int ord, int tpb, int zlds) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    if (ord < 0) {
+      c.fft_plan_forced = false;
+      c.fft_plan = pwk::FftPlan{};
+      return;
+    }
+    PWK_REQUIRE((ord == 0 || ord == 1) && (tpb == 64 || tpb == 128 || tpb == 256) && (zlds == 0 || zlds == 1),
+                PWK_E_INVALID, "bad FFT plan (ord 0/1, tpb 64/128/256, zlds 0/1)");
+    c.fft_plan.ord = ord;
+    c.fft_plan.tpb = tpb;
+    c.fft_plan.zlds = zlds;
+    c.fft_plan_forced = true;
+  });
+}
+
+int pwk_fft_plan_query(pwk_ctx* ctx, int batch, int precision, int plan[3], double* us_tuned, double* us_default,
+                       int* candidates) {
+  return guarded([&] {
+    Context& c = ctx_of(ctx);
+    check_precision(precision);
+    c.check_grid();
+    PWK_REQUIRE(batch > 0 && plan, PWK_E_INVALID, "batch must be positive and plan non-null");
+    double tb = 0.0, td = 0.0;
+    int nc = 0;
+    const pwk::FftPlan pl = pwk::fft_plan_for(c, batch, precision, &tb, &td, &nc);
+    plan[0] = pl.ord;
+    plan[1] = pl.tpb;
+    plan[2] = pl.zlds;
+    if (us_tuned) *us_tuned = tb;
+    if (us_default) *us_default = td;
+    if (candidates) *candidates = nc;
+  });
+}
+
 int pwk_profile_enable(pwk_ctx* ctx, int on) {
   return guarded([&] {
     Context& c = ctx_of(ctx);

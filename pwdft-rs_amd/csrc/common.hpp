@@ -266,6 +266,12 @@ struct PassReq {
 // Groups the passes that concurrent solves reach at about the same time (serve.hip).
 struct PassBatcher;
 
+// How the full-grid 3D FFT runs (autotune.hip): radix order of the lines (0/1), lanes per
+// workgroup of the line passes (64/128/256), and whether the contiguous pass is staged in LDS.
+struct FftPlan {
+  int ord = 0, tpb = 64, zlds = 0;
+};
+
 // Device state of one calculation.
 struct Context {
   int device = 0;
@@ -317,6 +323,10 @@ struct Context {
   // two events marking where the pass may start and when it has completed
   PassBatcher* batcher = nullptr;
   hipEvent_t batch_ready = nullptr, batch_done = nullptr;
+  // full-grid FFT plans: tuned per (grid, precision, batch) on first use, or one forced plan
+  bool fft_autotune = true;
+  bool fft_plan_forced = false;
+  FftPlan fft_plan;
 
   std::vector<KData> kd;
   // the kinetic diagonals and projector blocks of all k-points packed k-major (ragged k × band
@@ -361,6 +371,12 @@ void launch_lda_core(Context& c, const double* rho, const double* core, double c
                      double* vxc);
 void launch_lsda(Context& c, const void* ru, const void* rd, void* exc, void* vu, void* vd, int precision);
 void launch_convert(Context& c, const void* src, void* dst, size_t n, bool to_fp32);  // double<->float arrays
+// autotune.hip: the full-grid batched 3D FFT with the context's plan (tuned on first use of a
+// grid, precision and batch in the process when the context allows it); the plan it uses, with
+// the tuned and the default plan's microseconds and the candidate count when it was tuned
+void fft3d_planned(Context& c, void* data, int batch, int sign, int precision);
+FftPlan fft_plan_for(Context& c, int batch, int precision, double* us_best = nullptr, double* us_default = nullptr,
+                     int* ncand = nullptr);
 // fft3d.hip
 bool fft_size_supported(int n);
 void launch_fft3d(Context& c, void* data, int batch, int sign, int precision);

@@ -688,7 +688,7 @@ void launch_filter_small_t(Context& c, int nbands, int nbk, const void* V, void*
   const size_t sh = filter_lds_bytes<T>(n, nch, c.npw);
   const T* vr = std::is_same<T, double>::value ? (const T*)c.vrn64.p : (const T*)c.vrn32.p;
   const T inv_n = (T)(1.0 / (double)c.N);
-  TimedScope ts(c, "cheb_filter_small");
+  TimedScope ts(c, std::is_same<T, double>::value ? "cheb_filter_small_fp64" : "cheb_filter_small_fp32");
   switch (n) {
 #define PWK_CASE(L)                                                                                            \
   case L:                                                                                                      \

@@ -54,6 +54,9 @@ _SIGS = {
     "pwk_ctx_set_fused_hpsi": (C.c_int, [_vp, C.c_int]),
     "pwk_ctx_set_graphs": (C.c_int, [_vp, C.c_int]),
     "pwk_ctx_set_resident": (C.c_int, [_vp, C.c_int]),
+    "pwk_ctx_set_fft_autotune": (C.c_int, [_vp, C.c_int]),
+    "pwk_ctx_set_fft_plan": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "pwk_fft_plan_query": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _dp, C.POINTER(C.c_int)]),
     "pwk_dev_zgemm": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_longlong, _vp, C.c_int,
                                 C.c_longlong, _vp, C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "pwk_profile_enable": (C.c_int, [_vp, C.c_int]),
@@ -210,6 +213,25 @@ class GpuAccelerator:
     def set_resident(self, on=True):
         """Device-resident SCF iterations (default on) or the host-synchronised loop."""
         _check(lib().pwk_ctx_set_resident(self.h, int(on)))
+
+    def set_fft_autotune(self, on=True):
+        """Tune the full-grid FFT plan on first use of a grid (default on)."""
+        _check(lib().pwk_ctx_set_fft_autotune(self.h, int(on)))
+
+    def set_fft_plan(self, ord=-1, tpb=64, zlds=0):
+        """Force the full-grid FFT plan (radix order, lanes per workgroup, LDS-staged
+        contiguous pass); ord < 0 returns to the tuned plan."""
+        _check(lib().pwk_ctx_set_fft_plan(self.h, int(ord), int(tpb), int(zlds)))
+
+    def fft_plan(self, batch=1, precision="fp64"):
+        """The plan this context uses for its grid: dict(ord, tpb, zlds, us_tuned, us_default,
+        candidates); tunes it now if it has not been tuned in this process."""
+        plan = (C.c_int * 3)()
+        tb, td, nc = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().pwk_fft_plan_query(self.h, int(batch), _prec(precision), plan, C.byref(tb), C.byref(td),
+                                        C.byref(nc)))
+        return {"ord": plan[0], "tpb": plan[1], "zlds": plan[2], "us_tuned": tb.value, "us_default": td.value,
+                "candidates": nc.value}
 
     def set_fused_hpsi(self, on=True):
         """Fused one-launch H·psi on small cubic grids: True / 1 (default) also runs each

@@ -136,6 +136,40 @@ def test_fft3d_matches_dft(gpu, prec, tol):
         assert rel(b, np.fft.ifftn(x) * x.size) < tol, dims
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 3e-6)])
+def test_fft_plans_agree_and_autotune_picks_one(gpu, prec, tol):
+    # every full-grid FFT plan (radix order x lanes per workgroup x LDS-staged contiguous pass)
+    # computes the DFT; plans of one radix order give bit-identical results (they only move
+    # data); the autotuned plan is one of them, no slower than the default plan in the same
+    # timing run, and the transform it runs equals that plan forced
+    rng = np.random.default_rng(21)
+    try:
+        for dims in [(24, 18, 18), (25, 25, 25), (45, 45, 45), (60, 60, 60)]:
+            gpu.prepare_buffers(dims, np.zeros(int(np.prod(dims))))
+            x = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
+            ref = np.fft.fftn(x)
+            outs = {}
+            for o in (0, 1):
+                for tpb in (64, 128, 256):
+                    for z in (0, 1):
+                        if z and tpb * (dims[2] + 1) * (16 if prec == "fp64" else 8) > 160 * 1024:
+                            continue
+                        gpu.set_fft_plan(o, tpb, z)
+                        f = gpu.fft3d(x.reshape(-1), -1, prec).reshape(dims)
+                        assert rel(f, ref) < tol, (dims, o, tpb, z)
+                        outs[(o, tpb, z)] = f
+            for (o, tpb, z), f in outs.items():
+                assert np.array_equal(f, outs[(o, 64, 0)]), (dims, o, tpb, z)
+            gpu.set_fft_plan(-1)
+            pl = gpu.fft_plan(1, prec)
+            print("fft plan", dims, prec, pl, flush=True)
+            assert pl["candidates"] >= 3 and pl["us_tuned"] <= pl["us_default"]
+            f = gpu.fft3d(x.reshape(-1), -1, prec).reshape(dims)
+            assert np.array_equal(f, outs[(pl["ord"], pl["tpb"], pl["zlds"])]), dims
+    finally:
+        gpu.set_fft_plan(-1)
+
+
 def test_unsupported_grid_is_an_error(gpu, pwk_mod):
     gpu.prepare_buffers((7, 7, 7), np.zeros(343))
     with pytest.raises(pwk_mod.PwkError) as e:
