@@ -55,6 +55,7 @@ METRIC = "SCF solves/s + p50/p99 latency, mixed-prec vs FP64 GPU at |ΔE|<1e-5 R
 ACCURACY_BAR = 1e-5  # Ry per atom, |E_total| of the served form against the FP64 form
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
@@ -142,20 +143,28 @@ def kernel_profile(acc, fn, reps):
     return {k: {"avg_us": 1e3 * ms / max(n, 1), "launches": n} for k, (ms, n) in prof.items()}
 
 
-def algorithmic(name, w, nb, sc, sr):
-    """Algorithmic bytes / flops per launch of each H·ψ kernel (one launch covers all nb bands).
+def n_sticks(miller):
+    """Distinct (n1, n2) columns of the G-sphere: the z-sticks the pruned passes move."""
+    return len({(int(m[0]), int(m[1])) for m in miller})
 
-    Local part, 3-pass fused schedule: z-pass write N·s_c, xy-pass read+write
-    2·N·s_c (+ V once, N·s_r), z-pass read N·s_c, ψ in / Hψ out npw·s_c each.
-    Nonlocal: 8·npw·nch·nb flops per complex GEMM.
+
+def algorithmic(name, w, nb, sc, sr):
+    """Algorithmic bytes / flops per launch of each H·ψ kernel (one launch covers all nb bands),
+    for the sphere-pruned schedule the kernels run: S z-sticks of nz points carry the band
+    between the passes (the stick buffer), not the whole grid.
+
+    z-inverse: ψ in (npw·s_c) and the stick buffer out (S·nz·s_c); plane pass: the stick buffer
+    in and out (2·S·nz·s_c) plus V(r) once per launch (N·s_r); z-forward: the stick buffer and ψ
+    in, Hψ out ((S·nz + 2·npw)·s_c). Nonlocal: 8·npw·nch·nb flops per complex GEMM.
     """
     N, npw, nch = w["n_grid"], w["npw"], w["nch"]
+    snz = n_sticks(w["miller"]) * w["dims"][2]
     if name == "hpsi_zinv":
-        return "hbm", nb * (npw * sc + N * sc)
+        return "hbm", nb * (npw + snz) * sc
     if name == "hpsi_plane":
-        return "hbm", nb * 2 * N * sc + N * sr
+        return "hbm", nb * 2 * snz * sc + N * sr
     if name == "hpsi_zfwd":
-        return "hbm", nb * (N * sc + 2 * npw * sc)
+        return "hbm", nb * (snz + 2 * npw) * sc
     if name in ("hpsi_nl_project", "hpsi_nl_expand"):
         return "mfma", 8.0 * npw * nch * nb
     return None, None
@@ -210,6 +219,9 @@ def hpsi_bench(args, acc):
     dur_s = info["avg_us"] * 1e-6
     if bound == "hbm":
         achieved, peak, unit = amount / dur_s / 1e9, HBM_PEAK_GBS, "GB/s"
+    elif prec == "split_bf16":
+        # three BF16 products per complex product on the BF16 matrix cores
+        achieved, peak, unit = 3 * amount / dur_s / 1e12, BF16_MFMA_PEAK_TFLOPS, "TFLOP/s"
     else:
         achieved, peak, unit = amount / dur_s / 1e12, FP32_MFMA_PEAK_TFLOPS, "TFLOP/s"
     traffic = None
@@ -852,6 +864,7 @@ def main():
         "config": sc["config"],
         "p50_latency_ms": sc["p50_latency_ms"],
         "p99_latency_ms": sc["p99_latency_ms"],
+        "pass_batches": sc["pass_batches"],
         "fp64_gpu": sc["fp64_gpu"],
         "speedup_vs_fp64_gpu": sc["speedup_vs_fp64_gpu"],
         "speedup_decomposition": sc["speedup_decomposition"],

@@ -476,7 +476,7 @@ int pwk_server_destroy(pwk_server* srv);
  * filter pass), in which each member waits for its batch on the host. The reference's k-point loop
  * (src/scf/driver.rs:461-472) is the dimension the batch extends across
  * requests. Results do not depend on the batching. window_us 0 runs every
- * pass alone; the default is 200. pwk_server_pass_stats returns the number
+ * pass alone; the default is 100. pwk_server_pass_stats returns the number
  * of batched launches and of solve passes they carried since creation. */
 int pwk_server_set_pass_batching(pwk_server* srv, int window_us);
 int pwk_server_pass_stats(pwk_server* srv, int64_t* launches, int64_t* passes);

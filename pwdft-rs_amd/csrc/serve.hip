@@ -105,7 +105,7 @@ struct PassBatcher {
   };
   std::mutex mu;
   std::condition_variable cv;
-  int window_us = 200;  // how long a group's first solve gathers others; 0: every pass alone
+  int window_us = 100;  // how long a group's first solve gathers others; 0: every pass alone
   int running = 0;      // solves in progress (a group of that many launches at once)
   int64_t launches = 0, passes = 0;  // batched launches and the solve passes they carried
   std::vector<std::shared_ptr<Group>> open;
