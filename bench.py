@@ -487,7 +487,7 @@ def scf_bench(args):
     acc.set_resident(False)
     acc.profile_reset()
     acc.profile(True)
-    scf.run_scf(acc, deckmx, pmx)
+    r_prof = scf.run_scf(acc, deckmx, pmx)
     prof_mx = acc.profile_read()
     acc.profile(False)
     acc.profile_reset()
@@ -498,7 +498,7 @@ def scf_bench(args):
             "nch": nch}
     nbs = min(info["npw"], info["nb"] + max(8, info["nb"] // 4))
     shape = {"bands_per_launch": info["nk"] * nbs, "nk": info["nk"], "npw": info["npw"], "grid": list(tab["dims"]),
-             "channels": nch, "precision": args.scf_precision}
+             "channels": nch, "precision": args.scf_precision, "degree": r_prof["cheb_degree"]}
     fname = "cheb_filter_small_" + ("fp64" if args.scf_precision == "fp64" else "fp32")
     single = {k: v for k, v in kern_mx.items() if k in (fname, "hpsi_small", "scf_density")}
     name, kv = max(single.items(), key=lambda x: x[1]["total_ms"])
@@ -509,11 +509,12 @@ def scf_bench(args):
                 "algorithmic_per_launch": amount, "avg_us": round(kv["avg_us"], 2),
                 "launches_per_solve": kv["launches"], "shape": shape}
     if name.startswith("cheb_filter_small"):
-        fl = fft_flops_filter(info)
+        fl = fft_flops_filter(info, r_prof["cheb_degree"])
         roofline["flops_per_launch"] = fl
         roofline["fp32_tflops"] = round(fl / dur / 1e12, 3)
+        roofline["degree"] = r_prof["cheb_degree"]
         roofline["note"] = ("dominant kernel of a served-form solve: one launch per filter pass (all k-points x "
-                            "bands, ten recurrence steps, grids and vectors in LDS); avg_us from HIP events on the "
+                            "bands, every recurrence step, grids and vectors in LDS); avg_us from HIP events on the "
                             "solve's stream; LDS- and latency-bound, so its HBM fraction is small by design")
     pmc_file = os.path.join(ROOT, "profiles", "scf_si2_12ry_pmc.json")
     if os.path.exists(pmc_file):

@@ -407,7 +407,7 @@ typedef struct pwk_scf_params {
   int eigensolver;        /* 0: dense (diagonalize_lowest per k); 1: Chebyshev-filtered
                              subspace iteration on H·ψ with FP64 Rayleigh-Ritz; -1: dense
                              up to 1500 plane waves, filtered above */
-  int cheb_degree;        /* filter polynomial degree (0: 10) */
+  int cheb_degree;        /* filter polynomial degree (0: 14 on cubic grids up to 24³, else 10) */
   int verbose;
   /* spin and mixing (src/scf/mod.rs:90-107) */
   int nspin;              /* 1 (0 reads as 1) or 2: LSDA, src/scf/driver_spin.rs */
@@ -432,6 +432,8 @@ typedef struct pwk_scf_result {
   int nspin;
   double magnetization;   /* |N_up - N_down| (nspin = 2) */
   double n_up, n_down;    /* electrons per spin channel (nspin = 2) */
+  int cheb_degree;        /* filter degree used (filtered eigensolver; else 0) */
+  int filter_passes;      /* filter passes run over the solve (filtered eigensolver; else 0) */
 } pwk_scf_result;
 
 /* run_scf (src/scf/mod.rs:326-400): nspin = 1 follows run_scf_unpolarized

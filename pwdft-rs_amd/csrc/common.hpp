@@ -366,6 +366,15 @@ void launch_hartree_veff(Context& c, const void* rho_g, double fourpi_e2, const 
                          void* out, int precision, bool with_local, bool with_xc, bool with_hartree);
 void launch_veff_sum(Context& c, const void* a, const void* b, const void* d, void* out, int precision);
 void launch_lda(Context& c, const void* rho, void* exc, void* vxc, int precision);
+// Small cubic grids (N ≤ 20, FP64 grid in LDS): one single-workgroup launch for ρ(G) = FFT₋(ρ/N),
+// ρ_xc and its LDA ε_xc, v_xc, and (veff_g non-null) V_eff(G) = V_loc + Hartree + FFT₋(v_xc/N)
+// with V(r) into V_eff slot `slot` (what pwk_set_veff's transform and reorder write); and
+// out = Re FFT₊(in)·scale. Same arithmetic as the separate launches.
+bool grid_small_supported(const Context& c);
+void launch_rho_xc_veff_small(Context& c, const double* rho_r, const double* core, double cs, double2* rho_g,
+                              double* rho_xc, double* exc, double* vxc, const double2* vloc_g, double fourpi_e2,
+                              double2* veff_g, int slot);
+void launch_fft_c2r_small(Context& c, const double2* in, double scale, double* out);
 // ρ_xc = max(ρ + cs·ρ_core, 0) (ρ when core is null) into rho_xc, then LDA of it (FP64, N points)
 void launch_lda_core(Context& c, const double* rho, const double* core, double cs, double* rho_xc, double* exc,
                      double* vxc);

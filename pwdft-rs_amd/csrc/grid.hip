@@ -1,6 +1,9 @@
 // Pointwise grid kernels: Hartree, V_eff assembly, PZ-LDA and LSDA XC.
-// Each is a single HBM pass (bandwidth-bound); FP64 and FP32 variants.
+// Each is a single HBM pass (bandwidth-bound); FP64 and FP32 variants. On small cubic grids the
+// SCF's chains of transforms and pointwise steps also run as single-workgroup kernels with the grid
+// in LDS (the transforms of k_fft3d_small, the pointwise code below).
 #include "common.hpp"
+#include "lds_fft.hpp"
 
 namespace pwk {
 
@@ -235,6 +238,92 @@ __global__ void __launch_bounds__(256) k_f2d(size_t n, const float* __restrict__
   if (i < n) d[i] = (double)s[i];
 }
 
+
+// The three LDS passes of k_fft3d_small (x, y, z lines) on the workgroup's grid.
+template <int N, int SIGN>
+__device__ __forceinline__ void small_fft3d(double2* G) {
+  __syncthreads();
+  lds_lines<double, N, 0, SIGN>(G, nullptr);
+  __syncthreads();
+  lds_lines<double, N, 1, SIGN>(G, nullptr);
+  __syncthreads();
+  lds_lines<double, N, 2, SIGN>(G, nullptr);
+  __syncthreads();
+}
+
+// ρ(G) = FFT₋(ρ(r)/N) into rho_g, and ρ_xc = max(ρ + cs·ρ_core, 0) with its LDA ε_xc, v_xc; with
+// VEFF also V_eff(G) = V_loc + 4πe² ρ(G)/|G|² + FFT₋(v_xc/N) into veff_g and V(r) = Re FFT₊(V_eff)
+// into the plane-major and grid-order arrays (FP64 and FP32). One workgroup, the grid in LDS; each
+// step is the arithmetic of the separate launches (r_to_g, k_lda_core, k_hartree_veff, the V(r)
+// transform and k_vr_reorder), in the same order, so the results are theirs.
+template <int N, bool VEFF>
+__global__ void __launch_bounds__(SMALL_NT) k_rho_xc_veff_small(
+    const double* __restrict__ rho_r, const double* __restrict__ core, double cs, double inv_n,
+    double2* __restrict__ rho_g, double* __restrict__ rho_xc, double* __restrict__ exc, double* __restrict__ vxc,
+    const double2* __restrict__ vloc_g, const double* __restrict__ g2, double fourpi_e2,
+    double2* __restrict__ veff_g, double* __restrict__ v64, float* __restrict__ v32, double* __restrict__ n64,
+    float* __restrict__ n32) {
+  extern __shared__ unsigned char smem_raw[];
+  double2* G = reinterpret_cast<double2*>(smem_raw);
+  constexpr int NN = N * N * N;
+  for (int i = threadIdx.x; i < NN; i += SMALL_NT) G[i] = make_double2(rho_r[i] * inv_n, 0.0);
+  small_fft3d<N, -1>(G);
+  for (int i = threadIdx.x; i < NN; i += SMALL_NT) {
+    rho_g[i] = G[i];
+    const double r = core ? fmax(rho_r[i] + core[i] * cs, 0.0) : rho_r[i];
+    rho_xc[i] = r;
+    double e, v;
+    lda_point(r, e, v);
+    exc[i] = e;
+    vxc[i] = v;
+    if constexpr (VEFF) G[i] = make_double2(v * inv_n, 0.0);
+  }
+  if constexpr (VEFF) {
+    small_fft3d<N, -1>(G);
+    for (int i = threadIdx.x; i < NN; i += SMALL_NT) {
+      double2 acc = make_double2(0.0, 0.0);
+      const double gg = g2[i];
+      if (gg > kG2Zero) {
+        const double2 r = rho_g[i];  // written by this thread above
+        acc.x = r.x * fourpi_e2 / gg;
+        acc.y = r.y * fourpi_e2 / gg;
+      }
+      const double2 a = vloc_g[i];
+      acc.x = a.x + acc.x;
+      acc.y = a.y + acc.y;
+      const double2 x = G[i];
+      acc.x += x.x;
+      acc.y += x.y;
+      veff_g[i] = acc;
+      G[i] = acc;
+    }
+    small_fft3d<N, 1>(G);
+    for (int g = threadIdx.x; g < NN; g += SMALL_NT) {
+      const int i1 = g / (N * N), i2 = (g / N) % N, i3 = g % N;
+      const size_t idx = ((size_t)i3 * N + i1) * N + i2;  // plane-major [i3][i1][i2]
+      const double v = G[g].x;
+      v64[idx] = v;
+      v32[idx] = (float)v;
+      n64[g] = v;
+      n32[g] = (float)v;
+    }
+  }
+}
+
+// out = Re FFT₊(in)·scale (FP64), the transform and k_c2r_scaled in one workgroup
+template <int N>
+__global__ void __launch_bounds__(SMALL_NT) k_fft_c2r_small(const double2* __restrict__ in, double scale,
+                                                            double* __restrict__ out) {
+  extern __shared__ unsigned char smem_raw[];
+  double2* G = reinterpret_cast<double2*>(smem_raw);
+  constexpr int NN = N * N * N;
+  for (int i = threadIdx.x; i < NN; i += SMALL_NT) G[i] = in[i];
+  small_fft3d<N, 1>(G);
+  for (int i = threadIdx.x; i < NN; i += SMALL_NT) out[i] = G[i].x * scale;
+}
+
+#define PWK_GRID_SMALL_SIZES(X) X(8) X(9) X(10) X(12) X(15) X(16) X(18) X(20)
+
 }  // namespace
 
 void launch_hartree_veff(Context& c, const void* rho_g, double fourpi_e2, const void* v_local, const void* v_xc,
@@ -262,6 +351,73 @@ void launch_veff_sum(Context& c, const void* a, const void* b, const void* d, vo
   else
     k_sum3<float><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, (const float2*)a, (const float2*)b,
                                                             (const float2*)d, (float2*)out);
+  PWK_HIP(hipGetLastError());
+}
+
+bool grid_small_supported(const Context& c) {
+  const int n = c.dims[0];
+  if (!c.small_hpsi || n != c.dims[1] || n != c.dims[2]) return false;
+  switch (n) {
+#define PWK_CASE(L) case L:
+    PWK_GRID_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    return (size_t)n * n * n * sizeof(double2) <= 160 * 1024;
+    default:
+      return false;
+  }
+}
+
+void launch_rho_xc_veff_small(Context& c, const double* rho_r, const double* core, double cs, double2* rho_g,
+                              double* rho_xc, double* exc, double* vxc, const double2* vloc_g, double fourpi_e2,
+                              double2* veff_g, int slot) {
+  PWK_REQUIRE(grid_small_supported(c), PWK_E_UNSUPPORTED, "grid has no fused single-workgroup kernels");
+  const bool veff = veff_g != nullptr;
+  const size_t o = (size_t)slot * c.N;
+  if (veff) {
+    if (slot == 0) reserve_veff_slots(c, 1);
+    PWK_REQUIRE(slot >= 0 && c.vrn64.bytes >= (o + c.N) * sizeof(double) && c.vr32.bytes >= (o + c.N) * sizeof(float),
+                PWK_E_STATE, "V_eff slot not reserved");
+  }
+  const size_t sh = c.N * sizeof(double2);
+  const double inv_n = 1.0 / (double)c.N;
+  switch (c.dims[0]) {
+#define PWK_CASE(L)                                                                                            \
+  case L:                                                                                                      \
+    if (veff) {                                                                                                \
+      ensure_lds_limit((const void*)k_rho_xc_veff_small<L, true>, sh);                                        \
+      k_rho_xc_veff_small<L, true><<<1, SMALL_NT, sh, c.stream>>>(                                            \
+          rho_r, core, cs, inv_n, rho_g, rho_xc, exc, vxc, vloc_g, c.g2_64.as<double>(), fourpi_e2, veff_g,     \
+          c.vr64.as<double>() + o, c.vr32.as<float>() + o, c.vrn64.as<double>() + o, c.vrn32.as<float>() + o); \
+    } else {                                                                                                   \
+      ensure_lds_limit((const void*)k_rho_xc_veff_small<L, false>, sh);                                       \
+      k_rho_xc_veff_small<L, false><<<1, SMALL_NT, sh, c.stream>>>(rho_r, core, cs, inv_n, rho_g, rho_xc, exc, \
+                                                                   vxc, nullptr, nullptr, 0.0, nullptr,         \
+                                                                   nullptr, nullptr, nullptr, nullptr);         \
+    }                                                                                                          \
+    break;
+    PWK_GRID_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "grid has no fused single-workgroup kernels");
+  }
+  PWK_HIP(hipGetLastError());
+  if (veff && slot == 0) c.have_v = true;
+}
+
+void launch_fft_c2r_small(Context& c, const double2* in, double scale, double* out) {
+  PWK_REQUIRE(grid_small_supported(c), PWK_E_UNSUPPORTED, "grid has no fused single-workgroup kernels");
+  const size_t sh = c.N * sizeof(double2);
+  switch (c.dims[0]) {
+#define PWK_CASE(L)                                                                    \
+  case L:                                                                              \
+    ensure_lds_limit((const void*)k_fft_c2r_small<L>, sh);                             \
+    k_fft_c2r_small<L><<<1, SMALL_NT, sh, c.stream>>>(in, scale, out);                 \
+    break;
+    PWK_GRID_SMALL_SIZES(PWK_CASE)
+#undef PWK_CASE
+    default:
+      throw Error(PWK_E_UNSUPPORTED, "grid has no fused single-workgroup kernels");
+  }
   PWK_HIP(hipGetLastError());
 }
 

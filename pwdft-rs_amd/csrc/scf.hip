@@ -1149,6 +1149,12 @@ struct Solve {
   std::vector<DotTerm> veff_stage();
   std::vector<DotTerm> output_stage();
   void symmetrize(double* rn, double scale = 1.0);
+  bool veff_r_ready = false;  // veff_stage wrote V(r) of channel 0 into slot 0 already
+  // V(r) of channel s into V_eff slot `slot` for H·ψ (from V_eff(G) of the channel)
+  void upload_veff(int s, int slot, bool need_g) {
+    if (veff_r_ready && s == 0 && slot == 0 && !need_g) return;
+    set_veff_device(c, veff_g.as<double2>() + (size_t)s * N, slot, need_g);
+  }
   IterScalars scalars(const double* har, const double* out) const;
   void finalize(const std::vector<double>& occ, const IterScalars& v, double e_total, double e_harris, double ef,
                 double ts, double e_band, const double* n_s);
@@ -1517,7 +1523,12 @@ void Solve::setup() {
   } else {
     PWK_REQUIRE(heev_supported(nbs, nbs), PWK_E_UNSUPPORTED, "too many bands for the subspace eigensolver");
     fprec = p.precision;
-    degree = p.cheb_degree > 0 ? p.cheb_degree : 10;
+    // filter degree: on cubic grids up to 24³ a recurrence step costs little next to the fixed
+    // cost of a pass (the Rayleigh-Ritz step and the launches), so passes of 14 steps; larger
+    // grids, where every step is a full H·ψ, 10. Chosen by the grid alone, so every execution
+    // mode of one deck runs the same recurrence.
+    const bool small_cubic = c.dims[0] == c.dims[1] && c.dims[1] == c.dims[2] && c.dims[0] <= 24;
+    degree = p.cheb_degree > 0 ? p.cheb_degree : small_cubic ? 14 : 10;
     // V(r) is resident from here on: the fused-kernel checks below (and the choice of the
     // device-resident iteration) see the buffers it goes to
     reserve_veff_slots(c, 1);
@@ -2000,8 +2011,18 @@ void Solve::eig_chebyshev(bool first) {
 std::vector<DotTerm> Solve::veff_stage() {
   const double* core = have_core ? core_r.as<double>() : nullptr;
   if (ns == 1) {
-    r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
-    veff_from(rho_in.as<double>(), rho_g_in.as<double2>(), rho_xc.as<double>(), exc.as<double>(), vxc.as<double>());
+    veff_r_ready = false;
+    if (grid_small_supported(c)) {
+      // small cubic grid: ρ(G), xc, V_eff(G) and V(r) (slot 0) in one launch
+      launch_rho_xc_veff_small(c, rho_in.as<double>(), core, 1.0, rho_g_in.as<double2>(), rho_xc.as<double>(),
+                               exc.as<double>(), vxc.as<double>(), vloc_g.as<double2>(), 4.0 * kPiC * kE2,
+                               veff_g.as<double2>(), 0);
+      veff_r_ready = true;
+    } else {
+      r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
+      veff_from(rho_in.as<double>(), rho_g_in.as<double2>(), rho_xc.as<double>(), exc.as<double>(),
+                vxc.as<double>());
+    }
     return {DotTerm{rho_g_in.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
             DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
             DotTerm{rho_in.as<double>(), nullptr, vxc.as<double>(), nullptr, 0}};
@@ -2033,6 +2054,10 @@ void Solve::symmetrize(double* rn, double scale) {
   k_sym_gather<<<blocks_for(N, 16), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
                                                        sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
                                                        rho_g_new.as<double2>(), 1.0 / (double)ops.size());
+  if (grid_small_supported(c)) {
+    launch_fft_c2r_small(c, rho_g_new.as<double2>(), 1.0 / (double)N, rn);
+    return;
+  }
   launch_fft3d(c, rho_g_new.p, 1, +1, PWK_FP64);
   k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, rho_g_new.as<double2>(), 1.0 / (double)N, rn, 0);
   PWK_HIP(hipGetLastError());
@@ -2045,8 +2070,13 @@ std::vector<DotTerm> Solve::output_stage() {
   const double* core = have_core ? core_r.as<double>() : nullptr;
   if (ns == 1) {
     rho_tot_new = rho_new.as<double>();
-    r_to_g(rho_tot_new, rho_g_new.as<double2>());
-    launch_lda_core(c, rho_tot_new, core, 1.0, rho_xc.as<double>(), exc.as<double>(), vxc.as<double>());
+    if (grid_small_supported(c)) {
+      launch_rho_xc_veff_small(c, rho_tot_new, core, 1.0, rho_g_new.as<double2>(), rho_xc.as<double>(),
+                               exc.as<double>(), vxc.as<double>(), nullptr, 0.0, nullptr, 0);
+    } else {
+      r_to_g(rho_tot_new, rho_g_new.as<double2>());
+      launch_lda_core(c, rho_tot_new, core, 1.0, rho_xc.as<double>(), exc.as<double>(), vxc.as<double>());
+    }
     return {DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
             DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
             DotTerm{rho_new.as<double>(), nullptr, vxc.as<double>(), nullptr, 0},
@@ -2111,6 +2141,8 @@ void Solve::finalize(const std::vector<double>& occ, const IterScalars& v, doubl
   res.e_smearing = -ts;
   res.e_local = v.e_loc;
   res.integrated_charge = v.charge;
+  res.cheb_degree = chebyshev ? degree : 0;
+  res.filter_passes = chebyshev ? passes_total : 0;
   if (ns == 2) {
     res.n_up = n_s[0];
     res.n_down = n_s[1];
@@ -2206,13 +2238,13 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
 
     // ---- eigenpairs per channel (both channels in one pass of launches when folded) ----------
     if (fold) {
-      for (int s = 0; s < ns; ++s) set_veff_device(c, veff_g.as<double2>() + (size_t)s * N, s, false);
+      for (int s = 0; s < ns; ++s) upload_veff(s, s, false);
       sc = 0;
       ev_host = ev.data();
       eig_chebyshev(iter == 0);
     }
     for (sc = 0; sc < (fold ? 0 : ns); ++sc) {
-      set_veff_device(c, veff_g.as<double2>() + (size_t)sc * N, 0, !chebyshev);
+      upload_veff(sc, 0, !chebyshev);
       double* evc = ev.data() + (size_t)sc * nk * nb;
       if (chebyshev) {
         // Chebyshev-filtered subspace iteration on H·ψ, FP64 Rayleigh-Ritz
@@ -2425,7 +2457,7 @@ void Solve::resident_body(int fp, int max_passes) {
   double* S = dS.as<double>();
   k_iter_begin<<<1, 64, 0, c.stream>>>(dflag.as<int>(), S);
   dots_dev(veff_stage(), N, S_HAR);
-  for (int s = 0; s < (fold ? ns : 1); ++s) set_veff_device(c, veff_g.as<double2>() + (size_t)s * N, s, false);
+  for (int s = 0; s < (fold ? ns : 1); ++s) upload_veff(s, s, false);
   for (int pass = 0; pass < max_passes; ++pass) pass_dev(fp);
   resident_tail();
 }
@@ -2511,7 +2543,7 @@ void Solve::run_resident(double e_ewald, std::vector<double>& ev) {
   double* S = dS.as<double>();
   k_iter_begin<<<1, 64, 0, c.stream>>>(dflag.as<int>(), S);
   dots_dev(veff_stage(), N, S_HAR);
-  for (int s = 0; s < (fold ? ns : 1); ++s) set_veff_device(c, veff_g.as<double2>() + (size_t)s * N, s, false);
+  for (int s = 0; s < (fold ? ns : 1); ++s) upload_veff(s, s, false);
   sc = 0;
   lanczos_bounds();
   PWK_HIP(hipMemcpyAsync(ub_dev.p, ub.data(), (size_t)nke * 8, hipMemcpyHostToDevice, c.stream));

@@ -50,7 +50,8 @@ class Result(C.Structure):
         (n, C.c_double) for n in ("e_total", "e_harris", "fermi", "ts", "final_delta", "integrated_charge", "e_band",
                                   "e_kinetic", "e_local", "e_nonlocal", "e_hartree", "e_xc", "e_vxc", "e_ewald",
                                   "e_smearing", "seconds")] + [
-        ("nspin", C.c_int), ("magnetization", C.c_double), ("n_up", C.c_double), ("n_down", C.c_double)]
+        ("nspin", C.c_int), ("magnetization", C.c_double), ("n_up", C.c_double), ("n_down", C.c_double),
+        ("cheb_degree", C.c_int), ("filter_passes", C.c_int)]
 
 
 _SIGS = {
