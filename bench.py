@@ -781,7 +781,9 @@ def main():
                     help="also run the 64-atom C/Si FP32-vs-FP64 SCF study (about two minutes)")
     ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--bands", type=int, default=144)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "split_bf16"], help="H·psi precision")
+    ap.add_argument("--precision", default="split_bf16", choices=["fp32", "split_bf16"],
+                    help="H·psi precision of the H·psi-at-scale line (local part FP32; projector GEMMs FP32 or "
+                         "3-term split-BF16 on the BF16 matrix cores)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 H·psi comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
