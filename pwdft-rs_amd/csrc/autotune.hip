@@ -16,6 +16,7 @@
 // context or overridden by an explicit plan.
 #include <chrono>
 #include <map>
+#include <type_traits>
 #include <mutex>
 #include <tuple>
 
@@ -116,6 +117,11 @@ constexpr bool orders_differ(int n) {
 template <typename T, int SIGN, int ORD>
 void pass_lines(hipStream_t st, void* data, int batch, long bs, int n, int tpb, int nlines, int cnt_in, long st_out,
                 long st_in, long es) {
+  if (!fft_len_compiled(n)) {  // lines beyond the register lengths: staged in LDS (fft_long.hip)
+    launch_long_lines(st, data, batch, bs, n, nlines, cnt_in, st_out, st_in, es, SIGN,
+                      std::is_same<T, double>::value ? PWK_FP64 : PWK_FP32);
+    return;
+  }
   const dim3 grid(blocks_for((size_t)nlines, tpb), (unsigned)batch);
   auto* d = static_cast<typename Cplx<T>::t*>(data);
   switch (n) {
@@ -162,7 +168,7 @@ void run_plan_t(Context& c, void* data, int batch, const FftPlan& pl) {
   // x (stride ny·nz), y (stride nz), z (contiguous)
   pass_lines<T, SIGN, ORD>(c.stream, data, batch, bs, nx, pl.tpb, ny * nz, ny * nz, 0, 1, (long)ny * nz);
   pass_lines<T, SIGN, ORD>(c.stream, data, batch, bs, ny, pl.tpb, nx * nz, nz, (long)ny * nz, 1, nz);
-  if (pl.zlds)
+  if (pl.zlds && fft_len_compiled(nz))
     pass_lines_lds<T, SIGN, ORD>(c.stream, data, batch, bs, nz, pl.tpb, nx * ny);
   else
     pass_lines<T, SIGN, ORD>(c.stream, data, batch, bs, nz, pl.tpb, nx * ny, nx * ny, 0, nz, 1);
@@ -208,7 +214,8 @@ std::vector<FftPlan> candidates(const int* dims, int precision) {
   for (int ord = 0; ord < nord; ++ord)
     for (int tpb : {64, 128, 256})
       for (int z = 0; z < 2; ++z) {
-        if (z && (size_t)tpb * (dims[2] + 1) * es > 64 * 1024) continue;  // at least two tiles per CU
+        if (z && (!fft_len_compiled(dims[2]) || (size_t)tpb * (dims[2] + 1) * es > 64 * 1024))
+          continue;  // a register-length contiguous pass, at least two tiles per CU
         FftPlan p;
         p.ord = ord;
         p.tpb = tpb;

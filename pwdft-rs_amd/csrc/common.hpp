@@ -386,6 +386,10 @@ void launch_convert(Context& c, const void* src, void* dst, size_t n, bool to_fp
 void fft3d_planned(Context& c, void* data, int batch, int sign, int precision);
 FftPlan fft_plan_for(Context& c, int batch, int precision, double* us_best = nullptr, double* us_default = nullptr,
                      int* ncand = nullptr);
+// fft_long.hip: lines of a long length n (fft_sizes.hpp) through LDS, the line-pass contract of the
+// planned transform (line l at (l / cnt_in)·st_out + (l % cnt_in)·st_in, element stride es)
+void launch_long_lines(hipStream_t st, void* data, int batch, long bs, int n, int nlines, int cnt_in, long st_out,
+                       long st_in, long es, int sign, int precision);
 // fft3d.hip
 bool fft_size_supported(int n);
 void launch_fft3d(Context& c, void* data, int batch, int sign, int precision);

@@ -87,7 +87,7 @@ __global__ void k_r2c_scale(size_t n, const double* __restrict__ r, double s, do
 
 }  // namespace
 
-bool fft_size_supported(int n) { return fft_len_compiled(n); }
+bool fft_size_supported(int n) { return fft_len_supported(n); }
 
 void launch_fft3d_r2c(Context& c, const double* r, double scale, void* out, int sign) {
   c.check_grid();
@@ -119,7 +119,7 @@ void launch_fft3d_copy(Context& c, const void* in, void* out, int sign) {
 void launch_fft3d(Context& c, void* data, int batch, int sign, int precision) {
   c.check_grid();
   for (int d = 0; d < 3; ++d)
-    PWK_REQUIRE(fft_len_compiled(c.dims[d]), PWK_E_UNSUPPORTED,
+    PWK_REQUIRE(fft_len_supported(c.dims[d]), PWK_E_UNSUPPORTED,
                 "FFT length " + std::to_string(c.dims[d]) + " has no compiled kernel");
   const int fprec = precision == PWK_FP64 ? PWK_FP64 : PWK_FP32;
   if (fft_small_ok(c, fprec)) {

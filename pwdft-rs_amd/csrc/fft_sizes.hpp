@@ -6,8 +6,16 @@
   X(8) X(9) X(10) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(45) X(48) X(50) \
   X(54) X(60) X(64) X(72) X(75) X(80) X(81)
 
+// Longer {2,3,5}-smooth lines (up to 256) as L1 × L2 with both factors transformed in registers
+// and the line staged in LDS (fft_long.hip): X(L, L1, L2).
+#define PWK_FOR_EACH_LONG_FFT_SIZE(X)                                                                     \
+  X(90, 9, 10) X(96, 8, 12) X(100, 10, 10) X(108, 9, 12) X(120, 10, 12) X(125, 5, 25) X(128, 8, 16)      \
+  X(135, 9, 15) X(144, 12, 12) X(150, 10, 15) X(160, 10, 16) X(162, 9, 18) X(180, 12, 15) X(192, 12, 16) \
+  X(200, 10, 20) X(216, 12, 18) X(225, 15, 15) X(240, 15, 16) X(243, 9, 27) X(250, 10, 25) X(256, 16, 16)
+
 namespace pwk {
 constexpr int kMaxFftLen = 81;
+constexpr int kMaxLongFftLen = 256;
 inline bool fft_len_compiled(int n) {
   switch (n) {
 #define PWK_CASE(n) case n:
@@ -18,4 +26,16 @@ inline bool fft_len_compiled(int n) {
       return false;
   }
 }
+inline bool fft_len_long(int n) {
+  switch (n) {
+#define PWK_CASE(n, a, b) case n:
+    PWK_FOR_EACH_LONG_FFT_SIZE(PWK_CASE)
+#undef PWK_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+// lengths any full-grid transform accepts: register lines up to 81, LDS-staged lines up to 256
+inline bool fft_len_supported(int n) { return fft_len_compiled(n) || fft_len_long(n); }
 }  // namespace pwk

@@ -267,7 +267,13 @@ __global__ void __launch_bounds__(256) k_vr_reorder(const double2* __restrict__ 
   n32[g] = (float)v;
 }
 
-// ---- full-grid path (grids with nx != ny) ----------------------------------------
+// ---- full-grid path (grids with nx != ny, or an axis beyond the register lengths) --------
+// the stick path's plane and z kernels hold whole lines in registers: square planes, every axis
+// one of the register lengths
+bool stick_path_ok(const Context& c) {
+  return c.dims[0] == c.dims[1] && fft_len_compiled(c.dims[0]) && fft_len_compiled(c.dims[2]);
+}
+
 // ψ scattered on the whole grid of each band of a chunk, batched 3D FFTs (one compiled
 // line length per axis), V(r) multiply and gather. Slower than the stick path but valid
 // for every grid shape the line kernels compile.
@@ -402,7 +408,7 @@ void hpsi_local(Context& c, const T* kin, int nb, int nbk, const void* psi, void
   using V2 = typename V2t<T>::t;
   int nx = c.dims[0], ny = c.dims[1], nz = c.dims[2];
   PWK_REQUIRE(c.have_v, PWK_E_STATE, "V_eff not set (pwk_set_veff)");
-  if (nx != ny) {
+  if (!stick_path_ok(c)) {
     hpsi_fullgrid<T>(c, kin, nb, nbk, psi, hpsi);
     return;
   }
@@ -466,8 +472,8 @@ void hpsi_local(Context& c, const T* kin, int nb, int nbk, const void* psi, void
 template <typename T>
 void density_t(Context& c, int nb, const void* psi, const double* fw_host, void* rho) {
   using V2 = typename V2t<T>::t;
-  int nx = c.dims[0], ny = c.dims[1], nz = c.dims[2];
-  if (nx != ny) {
+  int nx = c.dims[0], nz = c.dims[2];
+  if (!stick_path_ok(c)) {
     density_fullgrid<T>(c, nb, psi, fw_host, rho);
     return;
   }

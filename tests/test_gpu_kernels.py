@@ -127,7 +127,8 @@ def test_zgemm_mfma_shapes(gpu, prec, tol):
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 3e-6)])
 def test_fft3d_matches_dft(gpu, prec, tol):
     rng = np.random.default_rng(4)
-    for dims in [(15, 15, 15), (16, 18, 20), (24, 25, 27), (60, 60, 60), (72, 75, 80), (81, 81, 64)]:
+    for dims in [(15, 15, 15), (16, 18, 20), (24, 25, 27), (60, 60, 60), (72, 75, 80), (81, 81, 64), (90, 96, 100),
+                 (125, 128, 135), (256, 18, 20), (20, 243, 16)]:
         gpu.prepare_buffers(dims, np.zeros(int(np.prod(dims))))
         x = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
         f = gpu.fft3d(x.reshape(-1), -1, prec).reshape(dims)
@@ -193,10 +194,12 @@ def ham(acc, si_sys, oracle):
     return dict(veff=veff, k=k, B=B, DBH=DBH)
 
 
-@pytest.mark.parametrize("dims", [(72, 72, 75), (80, 80, 81), (81, 81, 72)])
+@pytest.mark.parametrize("dims", [(72, 72, 75), (80, 80, 81), (81, 81, 72), (90, 96, 100), (128, 128, 120),
+                                  (256, 250, 18)])
 def test_hpsi_on_large_grids(gpu, oracle, dims):
-    # the largest compiled line lengths through the z-stick and plane kernels: Si FCC at 150 eV
-    # on a forced grid (no aliasing: the product is the same as on the rule's grid)
+    # the largest register line lengths through the z-stick and plane kernels, and beyond them
+    # (an axis of 90-256 points) the full-grid path with LDS-staged lines: Si FCC at 150 eV on a
+    # forced grid (no aliasing: the product is the same as on the rule's grid)
     s = oracle.System(si_lat(), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 150.0, dims=list(dims))
     print("hpsi grid", dims, "npw", s.npw, flush=True)
     gpu.prepare_buffers(s.dims, s.g2())

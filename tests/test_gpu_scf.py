@@ -468,3 +468,21 @@ def test_served_pass_batches_equal_alone(scf_mod, gpu):
             assert m[t] == a[t], t
         assert np.array_equal(m["eigenvalues"], a["eigenvalues"])
         assert np.array_equal(m["rho"], a["rho"])
+
+
+@pytest.mark.parametrize("eigensolver", [0, 1])
+def test_scf_on_grid_beyond_register_lengths(scf_mod, gpu, oracle, eigensolver):
+    # an FFT grid with axes of 90 and 96 points (above the 81-point register lines): the density,
+    # H·psi and every grid transform take the full-grid path with LDS-staged lines; Si FCC 100 eV,
+    # Γ, 4 bands, against the oracle's solve on the same grid, every term within 1e-5 Ry/atom
+    dims = (90, 90, 96)
+    p = scf_mod.ScfParams(ecut_ev=100.0, fft_dims=dims, use_symmetry=False, n_bands=4, max_iter=40,
+                          conv_threshold=1e-6, energy_threshold=1e-5, mixing_ndim=4, smearing_sigma=0.05,
+                          eigensolver=eigensolver, precision="fp64")
+    _, r = _run(scf_mod, gpu, fcc(5.431), p)
+    s = oracle.System(fcc(5.431), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 100.0, dims=list(dims))
+    ro = s.scf([[0, 0, 0]], [1.0], n_bands=4, max_iter=40, conv=1e-6, ethr=1e-5, beta=0.3, ndim=4, sigma=0.05)
+    print("grid", dims, "gpu", r["e_total"], r["n_iter"], "oracle", ro["e_total"], ro["n_iter"], flush=True)
+    assert r["converged"] == 1 and tuple(r["dims"]) == dims
+    for t in TERMS:
+        assert abs(r[t] - ro[t]) / 2 < TOL_TERM, t
