@@ -1,4 +1,4 @@
-// Line transforms longer than the register-resident lengths (up to 256 points, {2,3,5}-smooth),
+// Line transforms longer than the register-resident lengths (up to 1024 points, {2,3,5}-smooth),
 // src/fft.rs:19-114 convention: X[k] = Σ_n x[n] e^{SIGN·2πi nk/L}.
 //
 // L = L1·L2 (the factor pair of fft_sizes.hpp), input index n = L2·n1 + n2, output index

@@ -128,7 +128,7 @@ def test_zgemm_mfma_shapes(gpu, prec, tol):
 def test_fft3d_matches_dft(gpu, prec, tol):
     rng = np.random.default_rng(4)
     for dims in [(15, 15, 15), (16, 18, 20), (24, 25, 27), (60, 60, 60), (72, 75, 80), (81, 81, 64), (90, 96, 100),
-                 (125, 128, 135), (256, 18, 20), (20, 243, 16)]:
+                 (125, 128, 135), (256, 18, 20), (20, 243, 16), (1024, 16, 18), (18, 20, 1000), (16, 729, 15)]:
         gpu.prepare_buffers(dims, np.zeros(int(np.prod(dims))))
         x = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
         f = gpu.fft3d(x.reshape(-1), -1, prec).reshape(dims)
@@ -195,10 +195,10 @@ def ham(acc, si_sys, oracle):
 
 
 @pytest.mark.parametrize("dims", [(72, 72, 75), (80, 80, 81), (81, 81, 72), (90, 96, 100), (128, 128, 120),
-                                  (256, 250, 18)])
+                                  (256, 250, 18), (512, 24, 20), (20, 18, 1024)])
 def test_hpsi_on_large_grids(gpu, oracle, dims):
     # the largest register line lengths through the z-stick and plane kernels, and beyond them
-    # (an axis of 90-256 points) the full-grid path with LDS-staged lines: Si FCC at 150 eV on a
+    # (an axis of 90-1024 points) the full-grid path with LDS-staged lines: Si FCC at 150 eV on a
     # forced grid (no aliasing: the product is the same as on the rule's grid)
     s = oracle.System(si_lat(), [0, 0], [14, 14], SI_FRAC, [SI_UPF], 150.0, dims=list(dims))
     print("hpsi grid", dims, "npw", s.npw, flush=True)
