@@ -164,8 +164,8 @@ void pass_batch_join(Context& c, const PassShape& sh, const PassReq& r) {
     b.cv.wait_until(lk, until, [&] { return (int)g->m.size() >= std::min(MAX_PASS_BATCH, b.running); });
     b.open.erase(std::find(b.open.begin(), b.open.end(), g));
   }
-  const std::vector<PassBatcher::Member> mem = g->m;  // closed: no solve joins after this
-  lk.unlock();
+  lk.unlock();  // closed: no solve joins after this, so g->m is fixed
+  const std::vector<PassBatcher::Member>& mem = g->m;
   int st = PWK_OK;
   std::string err;
   try {
