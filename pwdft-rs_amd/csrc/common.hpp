@@ -307,6 +307,8 @@ struct Context {
   DevBuf st_n3lo;  // int32: n3 of the first basis vector
   DevBuf st_xy;    // int32: i1*ny + i2 (wrapped grid column)
   int nx1 = 0;     // distinct wrapped i1 values carrying sticks
+  bool x1_runs = false;       // those values are the runs [0, x1_ra) and [nx - x1_rb, nx)
+  int x1_ra = 0, x1_rb = 0;
   DevBuf x1list;   // int32 nx1
   DevBuf st_x1;    // int32: index into x1list of the stick's i1
 

@@ -119,11 +119,13 @@ struct Plane {
   static constexpr size_t bytes = (size_t)NX * LD * sizeof(typename V2t<T>::t);
 };
 
+// the y-lines of the plane's rows that carry sticks: row x1list[r] of P, or (x1list null) row r of
+// a plane that holds only those rows
 template <typename T, int NY, int SIGN>
 __device__ __forceinline__ void plane_rows(typename V2t<T>::t* P, int nx1, const int* __restrict__ x1list) {
   constexpr int LD = NY + 1;
   for (int r = threadIdx.x; r < nx1; r += blockDim.x) {
-    typename V2t<T>::t* row = P + x1list[r] * LD;
+    typename V2t<T>::t* row = P + (x1list ? x1list[r] : r) * LD;
     T re[NY], im[NY];
 #pragma unroll
     for (int j = 0; j < NY; ++j) {
@@ -139,36 +141,47 @@ __device__ __forceinline__ void plane_rows(typename V2t<T>::t* P, int nx1, const
   }
 }
 
-// H·ψ plane pass for one (band, z-plane).
-template <typename T, int NX, int NY>
+// H·ψ plane pass: one (z-plane, band) per workgroup. COMPACT: the rows of the plane that carry
+// sticks are the two runs i1 < ra and i1 >= NX - rb (a sphere's columns), and only those
+// nx1 = ra + rb rows are held in LDS, plus a zero row and a discard row. The other rows are zero
+// through the y passes and their x-pass output is never gathered, so the x-lines read them from the
+// zero row and write them to the discard row. Same arithmetic as the whole plane; about half the LDS
+// for a sphere that spans half the axis, so more planes in flight per CU.
+template <typename T, int NX, int NY, bool COMPACT>
 __global__ void __launch_bounds__(64) k_plane_hpsi(typename V2t<T>::t* __restrict__ tbuf, const T* __restrict__ vr,
-                                                   int S, int nz, const int* __restrict__ st_xy, int nx1,
-                                                   const int* __restrict__ x1list) {
+                                                   int S, int nz, const int* __restrict__ st_xy,
+                                                   const int* __restrict__ st_x1, int nx1,
+                                                   const int* __restrict__ x1list, int ra, int rb) {
   extern __shared__ unsigned char smem_raw[];
   using V2 = typename V2t<T>::t;
   constexpr int LD = NY + 1;
   V2* P = reinterpret_cast<V2*>(smem_raw);
   const int i3 = blockIdx.x, b = blockIdx.y;
   V2* tp = tbuf + ((size_t)b * nz + i3) * S;
-  for (int i = threadIdx.x; i < NX * LD; i += blockDim.x) {
+  const int rows = COMPACT ? nx1 + 2 : NX;
+  for (int i = threadIdx.x; i < rows * LD; i += blockDim.x) {
     P[i].x = 0;
     P[i].y = 0;
   }
   __syncthreads();
   for (int s = threadIdx.x; s < S; s += blockDim.x) {
-    int xy = st_xy[s];
-    P[(xy / NY) * LD + (xy % NY)] = tp[s];
+    const int xy = st_xy[s];
+    P[(COMPACT ? st_x1[s] : xy / NY) * LD + (xy % NY)] = tp[s];
   }
   __syncthreads();
-  plane_rows<T, NY, 1>(P, nx1, x1list);
+  plane_rows<T, NY, 1>(P, nx1, COMPACT ? nullptr : x1list);
   __syncthreads();
   const T* v = vr + (size_t)i3 * NX * NY;
+  // held row of plane row j (COMPACT); a row without sticks reads row nx1 (kept zero) and writes
+  // row nx1 + 1 (discarded), so the loads and stores need no branch
+  auto row = [&](int j, int absent) { return j < ra ? j : (j >= NX - rb ? j - (NX - rb) + ra : absent); };
   for (int i2 = threadIdx.x; i2 < NY; i2 += blockDim.x) {
     T re[NX], im[NX];
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
-      re[j] = P[j * LD + i2].x;
-      im[j] = P[j * LD + i2].y;
+      const int r = COMPACT ? row(j, nx1) : j;
+      re[j] = P[r * LD + i2].x;
+      im[j] = P[r * LD + i2].y;
     }
     fft_dif<T, NX, 1>(re, im);
     T r2[NX], m2[NX];
@@ -181,16 +194,17 @@ __global__ void __launch_bounds__(64) k_plane_hpsi(typename V2t<T>::t* __restric
     fft_dif<T, NX, -1>(r2, m2);
 #pragma unroll
     for (int f = 0; f < NX; ++f) {
-      P[f * LD + i2].x = r2[FftOrder<NX>::pos(f)];
-      P[f * LD + i2].y = m2[FftOrder<NX>::pos(f)];
+      const int r = COMPACT ? row(f, nx1 + 1) : f;
+      P[r * LD + i2].x = r2[FftOrder<NX>::pos(f)];
+      P[r * LD + i2].y = m2[FftOrder<NX>::pos(f)];
     }
   }
   __syncthreads();
-  plane_rows<T, NY, -1>(P, nx1, x1list);
+  plane_rows<T, NY, -1>(P, nx1, COMPACT ? nullptr : x1list);
   __syncthreads();
   for (int s = threadIdx.x; s < S; s += blockDim.x) {
-    int xy = st_xy[s];
-    tp[s] = P[(xy / NY) * LD + (xy % NY)];
+    const int xy = st_xy[s];
+    tp[s] = P[(COMPACT ? st_x1[s] : xy / NY) * LD + (xy % NY)];
   }
 }
 
@@ -439,10 +453,19 @@ void hpsi_local(Context& c, const T* kin, int nb, int nbk, const void* psi, void
   switch (nx) {
 #define PWK_CASE(L)                                                                                      \
   case L: {                                                                                              \
-    size_t sh = Plane<T, L, L>::bytes;                                                                   \
-    ensure_lds_limit((const void*)k_plane_hpsi<T, L, L>, sh);                                           \
-    k_plane_hpsi<T, L, L><<<gp, 64, sh, c.stream>>>(T_, vr, c.nsticks, nz, c.st_xy.as<int>(), c.nx1,     \
-                                                    c.x1list.as<int>());                                \
+    if (c.x1_runs) {                                                                                     \
+      size_t sh = (size_t)(c.nx1 + 2) * (L + 1) * sizeof(V2);                                            \
+      ensure_lds_limit((const void*)k_plane_hpsi<T, L, L, true>, sh);                                   \
+      k_plane_hpsi<T, L, L, true><<<gp, 64, sh, c.stream>>>(T_, vr, c.nsticks, nz, c.st_xy.as<int>(),    \
+                                                            c.st_x1.as<int>(), c.nx1, c.x1list.as<int>(), \
+                                                            c.x1_ra, c.x1_rb);                           \
+    } else {                                                                                             \
+      size_t sh = Plane<T, L, L>::bytes;                                                                 \
+      ensure_lds_limit((const void*)k_plane_hpsi<T, L, L, false>, sh);                                  \
+      k_plane_hpsi<T, L, L, false><<<gp, 64, sh, c.stream>>>(T_, vr, c.nsticks, nz, c.st_xy.as<int>(),   \
+                                                             c.st_x1.as<int>(), c.nx1,                  \
+                                                             c.x1list.as<int>(), 0, 0);                 \
+    }                                                                                                    \
     break;                                                                                               \
   }
     PWK_FOR_EACH_FFT_SIZE(PWK_CASE)
@@ -553,6 +576,18 @@ void setup_basis(Context& c, int npw, const int32_t* m) {
   c.npw = npw;
   c.nsticks = (int)pw0.size();
   c.nx1 = (int)x1list.size();
+  // the stick rows as two runs [0, ra) and [nx - rb, nx) (a sphere's columns): the plane pass
+  // then holds only those rows
+  {
+    int ra = 0;
+    while (ra < c.nx1 && x1list[(size_t)ra] == ra) ++ra;
+    const int rb = c.nx1 - ra;
+    bool runs = ra + rb == c.nx1 && c.nx1 < nx;
+    for (int r = 0; r < rb && runs; ++r) runs = x1list[(size_t)(ra + r)] == nx - rb + r;
+    c.x1_runs = runs;
+    c.x1_ra = runs ? ra : 0;
+    c.x1_rb = runs ? rb : 0;
+  }
   auto up = [&](DevBuf& d, const void* h, size_t bytes) {
     d.ensure(bytes);
     PWK_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c.stream));
