@@ -10,6 +10,12 @@
 // latency-bound on the stick path: five launches and two HBM round trips of
 // the stick buffer per H·ψ. Here ψ is read once, Hψ written once, and the
 // projectors and V(r) stay in L2 across the bands of a launch.
+//
+// a*b + c is fused only where one source expression holds it (never across statements), so a
+// line of the transforms or of the epilogues compiles to the same arithmetic in every kernel of
+// this file that inlines it: the one-launch filter pass, the per-step H·ψ and the one-launch
+// Lanczos then agree bit for bit with the step launches they replace.
+#pragma clang fp contract(on)
 #include "common.hpp"
 #include "fft_reg.hpp"
 #include "lds_fft.hpp"
@@ -17,6 +23,100 @@
 namespace pwk {
 
 namespace {
+
+constexpr int kSmallWaves = SMALL_NT / 64;
+
+// LDS entries (complex) of the nonlocal scratch: the projections and the waves' partial sums
+inline size_t nl_lds_entries(int nch) { return (size_t)(1 + kSmallWaves) * nch; }
+
+// Partial projections P_c = Σ_G (D·B^H)[c, G] v(G) of a vector v held in LDS. D·B^H is column-major
+// nch × npw (entry (c, G) at G·nch + c), so each of its columns is contiguous over c: lanes take
+// channels, so that a wave reads whole columns, and the waves take contiguous quarters of the plane
+// waves, with PU columns in flight per lane. The PU products of a block are summed pairwise and
+// the block sums added in plane-wave order (rounding error grows with log2 PU + the block count,
+// not with the quarter's length). Wave w's sums go to part[w·nch + c]; nl_combine adds them after a
+// barrier.
+template <typename T>
+__device__ __forceinline__ void nl_partials(typename V2s<T>::t* part, const typename V2s<T>::t* v,
+                                            const typename V2s<T>::t* __restrict__ dk, int npw, int nch) {
+  using V2 = typename V2s<T>::t;
+  constexpr int PU = 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int per = (npw + kSmallWaves - 1) / kSmallWaves, g0 = wave * per, g1 = min(npw, g0 + per);
+  for (int c = lane; c < nch; c += 64) {
+    T ar = 0, ai = 0;
+    for (int gb = g0; gb < g1; gb += PU) {
+      V2 d[PU];
+#pragma unroll
+      for (int u = 0; u < PU; ++u)
+        if (gb + u < g1) d[u] = dk[(size_t)(gb + u) * nch + c];
+      T pr[PU], pi[PU];
+#pragma unroll
+      for (int u = 0; u < PU; ++u) {
+        pr[u] = pi[u] = 0;
+        if (gb + u < g1) {
+          const V2 x = v[gb + u];
+          pr[u] = d[u].x * x.x - d[u].y * x.y;
+          pi[u] = d[u].x * x.y + d[u].y * x.x;
+        }
+      }
+#pragma unroll
+      for (int h = PU / 2; h > 0; h >>= 1)
+#pragma unroll
+        for (int u = 0; u < h; ++u) {
+          pr[u] += pr[u + h];
+          pi[u] += pi[u + h];
+        }
+      ar += pr[0];
+      ai += pi[0];
+    }
+    part[wave * nch + c].x = ar;
+    part[wave * nch + c].y = ai;
+  }
+}
+
+// proj[c] = the waves' partial sums added in wave order
+template <typename T>
+__device__ __forceinline__ void nl_combine(typename V2s<T>::t* proj, const typename V2s<T>::t* part, int nch) {
+  using V2 = typename V2s<T>::t;
+  for (int c = threadIdx.x; c < nch; c += SMALL_NT) {
+    V2 a = part[c];
+#pragma unroll
+    for (int w = 1; w < kSmallWaves; ++w) {
+      a.x += part[w * nch + c].x;
+      a.y += part[w * nch + c].y;
+    }
+    proj[c] = a;
+  }
+}
+
+// (ox, oy) += b·p (complex), as four FMAs in a fixed order: every kernel that applies the
+// projectors uses this form, so that they agree bit for bit
+template <typename T>
+__device__ __forceinline__ void nl_term(T& ox, T& oy, const typename V2s<T>::t& b, const typename V2s<T>::t& p) {
+  ox = fma(b.x, p.x, ox);
+  ox = fma(-b.y, p.y, ox);
+  oy = fma(b.x, p.y, oy);
+  oy = fma(b.y, p.x, oy);
+}
+
+// (ox, oy) += Σ_c B[G, c] P_c in channel order. B is column-major npw × nch (entry (G, c) at
+// c·npw + G), so the lanes' loads of one channel coalesce; EC channels' loads are in flight at once.
+template <typename T>
+__device__ __forceinline__ void nl_expand(T& ox, T& oy, const typename V2s<T>::t* __restrict__ bk,
+                                          const typename V2s<T>::t* proj, int g, int npw, int nch) {
+  using V2 = typename V2s<T>::t;
+  constexpr int EC = 12;
+  for (int c0 = 0; c0 < nch; c0 += EC) {
+    V2 bb[EC];
+#pragma unroll
+    for (int j = 0; j < EC; ++j)
+      if (c0 + j < nch) bb[j] = bk[(size_t)(c0 + j) * npw + g];
+#pragma unroll
+    for (int j = 0; j < EC; ++j)
+      if (c0 + j < nch) nl_term<T>(ox, oy, bb[j], proj[c0 + j]);
+  }
+}
 
 // Band b (k-point b / nbk) of psi -> hpsi. vr is V(r) in natural grid order; g2fft the flat
 // grid index of each plane wave; B (npw × nch) and DBH (nch × npw) column-major blocks per k.
@@ -37,7 +137,8 @@ __device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const type
   if (skip && *skip) return;  // a filter pass after the residual gate passed (uniform)
   extern __shared__ unsigned char smem_raw[];
   V2* G = reinterpret_cast<V2*>(smem_raw);
-  V2* proj = G + N * N * N;  // nch projections
+  V2* proj = G + N * N * N;   // nch projections
+  V2* part = proj + nch;      // the waves' partial projections
   // XCD-aware band order: workgroups are dealt to the 8 XCDs round-robin, so workgroup i runs
   // on XCD i % 8; each XCD takes a contiguous range of bands, which keeps the bands of one
   // k-point (and its projector blocks) in one XCD's L2
@@ -50,6 +151,7 @@ __device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const type
   const V2* p = psi + (size_t)b * npw;
   const T* t = kin + (size_t)k * npw;
   const V2* bk = B + (size_t)k * npw * nch;
+  const V2* dk = DBH + (size_t)k * nch * npw;
   const V2* xb = X ? X + (size_t)b * npw : nullptr;
   // operands of the epilogue for this thread's first plane wave, loaded now so that their latency
   // hides behind the transforms (projector rows for up to PF channels)
@@ -67,26 +169,8 @@ __device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const type
     for (int ch = 0; ch < PF; ++ch)
       if (ch < nch) pf_b[ch] = bk[(size_t)ch * npw + g0];
   }
-  // the projections of up to 16 channels (one group of four per wave) when npw <= 256, and
-  // V(r) of the thread's x-line, are loaded now too and used after the first transforms
-  constexpr int NW = SMALL_NT / 64, CPW = 4, PGI = SMALL_NT / 64;
+  // V(r) of the thread's x-line is loaded now too and used after the first transforms
   constexpr bool PRE_V = N * N <= SMALL_NT;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const V2* dk = DBH + (size_t)k * nch * npw;
-  const bool pre_proj = nch > 0 && nch <= NW * CPW && npw <= 64 * PGI;
-  V2 pd[CPW][PGI], pv[PGI];
-  if (pre_proj) {
-#pragma unroll
-    for (int it = 0; it < PGI; ++it) {
-      const int g = lane + 64 * it;
-      if (g < npw) {
-        pv[it] = p[g];
-#pragma unroll
-        for (int j = 0; j < CPW; ++j)
-          if (wave + j * NW < nch) pd[j][it] = dk[(size_t)g * nch + wave + j * NW];
-      }
-    }
-  }
   T vline[PRE_V ? N : 1];
   if constexpr (PRE_V) {
     if (threadIdx.x < N * N) {
@@ -100,67 +184,10 @@ __device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const type
   }
   lds_barrier();
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = p[g];
-  // projections P_c = Σ_G (D·B^H)[c, G] ψ(G): one wave per channel, lanes over G; a wave runs
-  // up to four of its channels side by side (same order of additions per channel)
-  if (nch > 0 && !pre_proj) {
-    for (int c0 = wave; c0 < nch; c0 += NW * CPW) {
-      T ar[CPW], ai[CPW];
-#pragma unroll
-      for (int j = 0; j < CPW; ++j) ar[j] = ai[j] = 0;
-      for (int g = lane; g < npw; g += 64) {
-        const V2 v = p[g];
-        V2 d[CPW];
-#pragma unroll
-        for (int j = 0; j < CPW; ++j)
-          if (c0 + j * NW < nch) d[j] = dk[(size_t)g * nch + c0 + j * NW];
-#pragma unroll
-        for (int j = 0; j < CPW; ++j)
-          if (c0 + j * NW < nch) {
-            ar[j] += d[j].x * v.x - d[j].y * v.y;
-            ai[j] += d[j].x * v.y + d[j].y * v.x;
-          }
-      }
-#pragma unroll
-      for (int j = 0; j < CPW; ++j) {
-        for (int off = 32; off > 0; off >>= 1) {
-          ar[j] += __shfl_xor(ar[j], off, 64);
-          ai[j] += __shfl_xor(ai[j], off, 64);
-        }
-        if (lane == 0 && c0 + j * NW < nch) {
-          proj[c0 + j * NW].x = ar[j];
-          proj[c0 + j * NW].y = ai[j];
-        }
-      }
-    }
-  }
+  nl_partials<T>(part, p, dk, npw, nch);
   lds_barrier();
+  nl_combine<T>(proj, part, nch);
   lds_lines<T, N, 2, 1>(G, nullptr);
-  if (pre_proj) {  // the prefetched projections, same order of additions as the loop above
-    T ar[CPW], ai[CPW];
-#pragma unroll
-    for (int j = 0; j < CPW; ++j) ar[j] = ai[j] = 0;
-#pragma unroll
-    for (int it = 0; it < PGI; ++it)
-      if (lane + 64 * it < npw) {
-#pragma unroll
-        for (int j = 0; j < CPW; ++j)
-          if (wave + j * NW < nch) {
-            ar[j] += pd[j][it].x * pv[it].x - pd[j][it].y * pv[it].y;
-            ai[j] += pd[j][it].x * pv[it].y + pd[j][it].y * pv[it].x;
-          }
-      }
-#pragma unroll
-    for (int j = 0; j < CPW; ++j) {
-      for (int off = 32; off > 0; off >>= 1) {
-        ar[j] += __shfl_xor(ar[j], off, 64);
-        ai[j] += __shfl_xor(ai[j], off, 64);
-      }
-      if (lane == 0 && wave + j * NW < nch) {
-        proj[wave + j * NW].x = ar[j];
-        proj[wave + j * NW].y = ai[j];
-      }
-    }
-  }
   lds_barrier();
   lds_lines<T, N, 1, 1>(G, nullptr);
   lds_barrier();
@@ -184,31 +211,23 @@ __device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const type
     const bool first = pf && g == g0;
     const V2 v = first ? pf_v : p[g], w = G[g2fft[g]];
     const T tk = first ? pf_t : t[g];
-    T ox = tk * v.x + w.x * inv_n, oy = tk * v.y + w.y * inv_n;
+    T ox = fma(tk, v.x, w.x * inv_n), oy = fma(tk, v.y, w.y * inv_n);
     if (first) {
 #pragma unroll
       for (int ch = 0; ch < PF; ++ch)
-        if (ch < nch) {
-          const V2 bb = pf_b[ch], pc = proj[ch];
-          ox += bb.x * pc.x - bb.y * pc.y;
-          oy += bb.x * pc.y + bb.y * pc.x;
-        }
+        if (ch < nch) nl_term<T>(ox, oy, pf_b[ch], proj[ch]);
     } else {
-      for (int ch = 0; ch < nch; ++ch) {
-        const V2 bb = bk[(size_t)ch * npw + g], pc = proj[ch];
-        ox += bb.x * pc.x - bb.y * pc.y;
-        oy += bb.x * pc.y + bb.y * pc.x;
-      }
+      nl_expand<T>(ox, oy, bk, proj, g, npw, nch);
     }
     V2 o;
     if (coef) {
-      // the elementwise step of the filter (subspace.hip k_cheb_step), same operation order
-      o.x = alpha * (ox - cshift * v.x);
-      o.y = alpha * (oy - cshift * v.y);
+      // the elementwise step of the filter (subspace.hip k_cheb_step), with its FMAs explicit
+      o.x = alpha * fma(-cshift, v.x, ox);
+      o.y = alpha * fma(-cshift, v.y, oy);
       if (xb) {
         const V2 xv = first ? pf_x : xb[g];
-        o.x -= beta * xv.x;
-        o.y -= beta * xv.y;
+        o.x = fma(-beta, xv.x, o.x);
+        o.y = fma(-beta, xv.y, o.y);
       }
     } else {
       o.x = ox;
@@ -219,56 +238,29 @@ __device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const type
 }
 
 // The local part of H·v for a vector held in LDS (cur, npw entries): scatter onto the grid G,
-// projections proj[c] = Σ_G (D·B^H)[c, G] v(G) (one wave per channel, up to four side by side),
+// partial projections of v (nl_partials, summed into proj by nl_combine after the next barrier),
 // inverse FFT z, y, x with V(r) applied between the x transforms (vline: V(r) of the thread's
 // x-line, when N² <= SMALL_NT; else read from vr), forward FFT y, z. On return G holds
 // FFT[V·IFFT v] (unnormalised) and proj the projections; the caller's epilogue adds the kinetic
-// and nonlocal terms in the order k_hpsi_small uses.
+// and nonlocal terms in the order k_hpsi_small uses. gi0 is the grid index of plane wave
+// threadIdx.x (when threadIdx.x < npw), which the caller keeps in a register across steps.
 template <typename T, int N>
 __device__ __forceinline__ void small_h_core(typename V2s<T>::t* G, typename V2s<T>::t* proj,
-                                             const typename V2s<T>::t* cur, const T (&vline)[N * N <= SMALL_NT ? N : 1],
-                                             const T* __restrict__ vr, const int* __restrict__ g2fft, int npw, int nch,
+                                             typename V2s<T>::t* part, const typename V2s<T>::t* cur,
+                                             const T (&vline)[N * N <= SMALL_NT ? N : 1], const T* __restrict__ vr,
+                                             const int* __restrict__ g2fft, int gi0, int npw, int nch,
                                              const typename V2s<T>::t* __restrict__ dk) {
-  using V2 = typename V2s<T>::t;
-  constexpr int NW = SMALL_NT / 64, CPW = 4;
   constexpr bool PRE_V = N * N <= SMALL_NT;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
     G[i].x = 0;
     G[i].y = 0;
   }
   lds_barrier();
-  for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = cur[g];
-  for (int c0 = wave; c0 < nch; c0 += NW * CPW) {
-    T ar[CPW], ai[CPW];
-#pragma unroll
-    for (int j = 0; j < CPW; ++j) ar[j] = ai[j] = 0;
-    for (int g = lane; g < npw; g += 64) {
-      const V2 v = cur[g];
-      V2 d[CPW];
-#pragma unroll
-      for (int j = 0; j < CPW; ++j)
-        if (c0 + j * NW < nch) d[j] = dk[(size_t)g * nch + c0 + j * NW];
-#pragma unroll
-      for (int j = 0; j < CPW; ++j)
-        if (c0 + j * NW < nch) {
-          ar[j] += d[j].x * v.x - d[j].y * v.y;
-          ai[j] += d[j].x * v.y + d[j].y * v.x;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < CPW; ++j) {
-      for (int off = 32; off > 0; off >>= 1) {
-        ar[j] += __shfl_xor(ar[j], off, 64);
-        ai[j] += __shfl_xor(ai[j], off, 64);
-      }
-      if (lane == 0 && c0 + j * NW < nch) {
-        proj[c0 + j * NW].x = ar[j];
-        proj[c0 + j * NW].y = ai[j];
-      }
-    }
-  }
+  if ((int)threadIdx.x < npw) G[gi0] = cur[threadIdx.x];
+  for (int g = threadIdx.x + SMALL_NT; g < npw; g += SMALL_NT) G[g2fft[g]] = cur[g];
+  nl_partials<T>(part, cur, dk, npw, nch);
   lds_barrier();
+  nl_combine<T>(proj, part, nch);
   lds_lines<T, N, 2, 1>(G, nullptr);
   lds_barrier();
   lds_lines<T, N, 1, 1>(G, nullptr);
@@ -317,7 +309,8 @@ __global__ void __launch_bounds__(SMALL_NT) k_lanczos_small(const int* __restric
   __shared__ double red[4];
   double2* G = reinterpret_cast<double2*>(smem_raw);
   double2* proj = G + N * N * N;
-  double2* v0 = proj + nch;
+  double2* part = proj + nch;
+  double2* v0 = part + kSmallWaves * nch;
   double2* v1 = v0 + npw;
   double2* w = v1 + npw;
   const int kb = blockIdx.x, nke = gridDim.x;
@@ -334,6 +327,9 @@ __global__ void __launch_bounds__(SMALL_NT) k_lanczos_small(const int* __restric
       for (int f = 0; f < N; ++f) vline[f] = vr[threadIdx.x + f * N * N];
     }
   }
+  // grid index and kinetic energy of plane wave threadIdx.x, kept for every step
+  const int gi0 = (int)threadIdx.x < npw ? g2fft[threadIdx.x] : 0;
+  const double tk0 = (int)threadIdx.x < npw ? t[threadIdx.x] : 0.0;
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
     v1[g] = start_vector_entry(g, 0, kb);
     v0[g] = make_double2(0.0, 0.0);
@@ -349,18 +345,15 @@ __global__ void __launch_bounds__(SMALL_NT) k_lanczos_small(const int* __restric
   double beta = 0.0;
   for (int j = 0; j < steps; ++j) {
     lds_barrier();
-    small_h_core<double, N>(G, proj, v1, vline, vr, g2fft, npw, nch, dk);
-    for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
-      const double2 v = v1[g], hw = G[g2fft[g]];
-      const double tk = t[g];
-      double ox = tk * v.x + hw.x * inv_n, oy = tk * v.y + hw.y * inv_n;
-      for (int ch = 0; ch < nch; ++ch) {
-        const double2 bb = bk[(size_t)ch * npw + g], pc = proj[ch];
-        ox += bb.x * pc.x - bb.y * pc.y;
-        oy += bb.x * pc.y + bb.y * pc.x;
-      }
+    small_h_core<double, N>(G, proj, part, v1, vline, vr, g2fft, gi0, npw, nch, dk);
+    auto epi = [&](int g, int gi, double tk) {
+      const double2 v = v1[g], hw = G[gi];
+      double ox = fma(tk, v.x, hw.x * inv_n), oy = fma(tk, v.y, hw.y * inv_n);
+      nl_expand<double>(ox, oy, bk, proj, g, npw, nch);
       w[g] = make_double2(ox, oy);
-    }
+    };
+    if ((int)threadIdx.x < npw) epi(threadIdx.x, gi0, tk0);
+    for (int g = threadIdx.x + SMALL_NT; g < npw; g += SMALL_NT) epi(g, g2fft[g], t[g]);
     __syncthreads();
     const double a = lds_real_dot(v1, w, npw, red);
     const double alpha = 1.0, cshift = a;
@@ -417,7 +410,8 @@ __device__ __forceinline__ void cheb_filter_body(int bid, int nblocks, const dou
   extern __shared__ unsigned char smem_raw[];
   V2* G = reinterpret_cast<V2*>(smem_raw);
   V2* proj = G + N * N * N;
-  V2* cur = proj + nch;
+  V2* part = proj + nch;
+  V2* cur = part + kSmallWaves * nch;
   V2* prv = cur + npw;
   double* cfl = reinterpret_cast<double*>(prv + npw);  // this block's (α, c, β) of every step
   const int nblk = nblocks / nbk;
@@ -437,6 +431,9 @@ __device__ __forceinline__ void cheb_filter_body(int bid, int nblocks, const dou
       for (int f = 0; f < N; ++f) vline[f] = vr[threadIdx.x + f * N * N];
     }
   }
+  // grid index and kinetic energy of plane wave threadIdx.x, kept for every step
+  const int gi0 = (int)threadIdx.x < npw ? g2fft[threadIdx.x] : 0;
+  const T tk0 = (int)threadIdx.x < npw ? t[threadIdx.x] : (T)0;
   const double2* v0 = V + (size_t)b * npw;
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
     const double2 x = v0[g];
@@ -449,31 +446,28 @@ __device__ __forceinline__ void cheb_filter_body(int bid, int nblocks, const dou
                           ? coef[((size_t)(threadIdx.x / 3) * nblk + kb) * 3 + threadIdx.x % 3]
                           : 0.0;
   for (int step = 0; step < degree; ++step) {
-    small_h_core<T, N>(G, proj, cur, vline, vr, g2fft, npw, nch, dk);
+    small_h_core<T, N>(G, proj, part, cur, vline, vr, g2fft, gi0, npw, nch, dk);
     if (step == 0) {
       if ((int)threadIdx.x < 3 * degree) cfl[threadIdx.x] = cpre;
       lds_barrier();
     }
     const T alpha = (T)cfl[3 * step], cshift = (T)cfl[3 * step + 1], beta = (T)cfl[3 * step + 2];
-    for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
-      const V2 v = cur[g], w = G[g2fft[g]];
-      const T tk = t[g];
-      T ox = tk * v.x + w.x * inv_n, oy = tk * v.y + w.y * inv_n;
-      for (int ch = 0; ch < nch; ++ch) {
-        const V2 bb = bk[(size_t)ch * npw + g], pc = proj[ch];
-        ox += bb.x * pc.x - bb.y * pc.y;
-        oy += bb.x * pc.y + bb.y * pc.x;
-      }
+    auto epi = [&](int g, int gi, T tk) {
+      const V2 v = cur[g], w = G[gi];
+      T ox = fma(tk, v.x, w.x * inv_n), oy = fma(tk, v.y, w.y * inv_n);
+      nl_expand<T>(ox, oy, bk, proj, g, npw, nch);
       V2 o;
-      o.x = alpha * (ox - cshift * v.x);
-      o.y = alpha * (oy - cshift * v.y);
+      o.x = alpha * fma(-cshift, v.x, ox);
+      o.y = alpha * fma(-cshift, v.y, oy);
       if (step > 0) {
         const V2 xv = prv[g];
-        o.x -= beta * xv.x;
-        o.y -= beta * xv.y;
+        o.x = fma(-beta, xv.x, o.x);
+        o.y = fma(-beta, xv.y, o.y);
       }
       prv[g] = o;
-    }
+    };
+    if ((int)threadIdx.x < npw) epi(threadIdx.x, gi0, tk0);
+    for (int g = threadIdx.x + SMALL_NT; g < npw; g += SMALL_NT) epi(g, g2fft[g], t[g]);
     lds_barrier();
     V2* s = cur;
     cur = prv;
@@ -628,7 +622,7 @@ __global__ void __launch_bounds__(256) k_density_reduce(size_t n, int ngroups, c
 
 template <typename T>
 size_t small_lds_bytes(int n, int nch) {
-  return ((size_t)n * n * n + (size_t)nch) * sizeof(typename V2s<T>::t);
+  return ((size_t)n * n * n + nl_lds_entries(nch)) * sizeof(typename V2s<T>::t);
 }
 
 constexpr size_t kLdsMax = 160 * 1024;
@@ -676,7 +670,7 @@ constexpr int kMaxFilterDegree = 64;  // 3 coefficients per step, one thread eac
 
 template <typename T>
 size_t filter_lds_bytes(int n, int nch, int npw) {
-  return ((size_t)n * n * n + (size_t)nch + 2 * (size_t)npw) * sizeof(typename V2s<T>::t) +
+  return ((size_t)n * n * n + nl_lds_entries(nch) + 2 * (size_t)npw) * sizeof(typename V2s<T>::t) +
          3 * kMaxFilterDegree * sizeof(double);
 }
 
@@ -845,7 +839,7 @@ void launch_hpsi64_multi(hipStream_t stream, const PassShape& sh, const PassSmal
 }
 
 size_t lanczos_small_lds_bytes(int n, int nch, int npw) {
-  return ((size_t)n * n * n + (size_t)nch + 3 * (size_t)npw) * sizeof(double2);
+  return ((size_t)n * n * n + nl_lds_entries(nch) + 3 * (size_t)npw) * sizeof(double2);
 }
 
 bool lanczos_small_supported(const Context& c, int nch) {

@@ -165,13 +165,18 @@ __device__ inline void jacobi_lds(int n, JacobiLds& J) {
       __syncthreads();
     }
   }
-  // ascending order by rank (ties by index)
+  // ascending order by rank (ties by index). The order is total even for a NaN diagonal (NaNs
+  // after every number, among themselves by index), so the ranks are always a permutation and
+  // every entry of J.order is written.
   if (t < n) {
     const double di = J.A[t][t].x;
+    const bool ni = di != di;
     int rank = 0;
     for (int j = 0; j < n; ++j) {
       const double dj = J.A[j][j].x;
-      rank += (dj < di || (dj == di && j < t)) ? 1 : 0;
+      const bool nj = dj != dj;
+      const bool before = (ni || nj) ? (nj == ni ? j < t : ni) : (dj < di || (dj == di && j < t));
+      rank += before ? 1 : 0;
     }
     J.order[rank] = t;
   }
