@@ -500,8 +500,15 @@ def scf_bench(args):
     shape = {"bands_per_launch": info["nk"] * nbs, "nk": info["nk"], "npw": info["npw"], "grid": list(tab["dims"]),
              "channels": nch, "precision": args.scf_precision, "degree": r_prof["cheb_degree"]}
     fname = "cheb_filter_small_" + ("fp64" if args.scf_precision == "fp64" else "fp32")
-    single = {k: v for k, v in kern_mx.items() if k in (fname, "hpsi_small", "scf_density")}
-    name, kv = max(single.items(), key=lambda x: x[1]["total_ms"])
+    # the filter kernel of the headline precision is the roofline object whenever the solve runs
+    # it: with its FP64 passes after the precision hand-over (the same kernel in FP64) it is the
+    # largest share of the solve; otherwise the largest of the fused H·psi and the density
+    filt_ms = sum(v["total_ms"] for k, v in kern_mx.items() if k.startswith("cheb_filter_small"))
+    if fname in kern_mx:
+        name, kv = fname, kern_mx[fname]
+    else:
+        single = {k: v for k, v in kern_mx.items() if k in ("hpsi_small", "scf_density")}
+        name, kv = max(single.items(), key=lambda x: x[1]["total_ms"])
     bound, amount = scf_algorithmic(name, info, args.scf_precision)
     dur = kv["avg_us"] * 1e-6
     roofline = {"kernel": name, "bound": "hbm", "achieved": round(amount / dur / 1e9, 3), "peak": HBM_PEAK_GBS,
@@ -513,8 +520,10 @@ def scf_bench(args):
         roofline["flops_per_launch"] = fl
         roofline["fp32_tflops"] = round(fl / dur / 1e12, 3)
         roofline["degree"] = r_prof["cheb_degree"]
+        roofline["filter_ms_per_solve"] = round(filt_ms, 3)  # both precisions of the filter kernel
         roofline["note"] = ("dominant kernel of a served-form solve: one launch per filter pass (all k-points x "
-                            "bands, every recurrence step, grids and vectors in LDS); avg_us from HIP events on the "
+                            "bands, every recurrence step, grids and vectors in LDS), in the headline precision "
+                            "until the residual target reaches the FP64 hand-over; avg_us from HIP events on the "
                             "solve's stream; LDS- and latency-bound, so its HBM fraction is small by design")
     pmc_file = os.path.join(ROOT, "profiles", "scf_si2_12ry_pmc.json")
     if os.path.exists(pmc_file):

@@ -29,6 +29,44 @@ constexpr int kSmallWaves = SMALL_NT / 64;
 // LDS entries (complex) of the nonlocal scratch: the projections and the waves' partial sums
 inline size_t nl_lds_entries(int nch) { return (size_t)(1 + kSmallWaves) * nch; }
 
+// One block of nl_partials: rows gb .. gb + PU − 1 of channel c (TAIL: rows from g1 on count zero),
+// their products summed pairwise and the sum added to (ar, ai).
+template <typename T, int PU, bool TAIL>
+__device__ __forceinline__ void nl_block(T& ar, T& ai, const typename V2s<T>::t* v,
+                                         const typename V2s<T>::t* __restrict__ dk, int nch, int c, int gb, int g1) {
+  using V2 = typename V2s<T>::t;
+  // every load is issued first (in a tail block, rows past the quarter's end read its last row,
+  // which exists since gb < g1, and their products are replaced by zeros), so no load sits behind
+  // a branch and the block waits for its loads once
+  V2 d[PU], x[PU];
+#pragma unroll
+  for (int u = 0; u < PU; ++u) {
+    const int gg = TAIL ? min(gb + u, g1 - 1) : gb + u;
+    d[u] = (dk + gg * nch)[c];  // uniform row pointer (npw·nch < 2³¹), lane offset c
+    x[u] = v[gg];
+  }
+  // scheduling groups: the PU global loads, then the PU LDS loads, ahead of the arithmetic
+  // (left alone, the scheduler issues one load at a time to save registers)
+  __builtin_amdgcn_sched_group_barrier(0x0020, PU, 0);
+  __builtin_amdgcn_sched_group_barrier(0x0100, PU, 0);
+  T pr[PU], pi[PU];
+#pragma unroll
+  for (int u = 0; u < PU; ++u) {
+    pr[u] = d[u].x * x[u].x - d[u].y * x[u].y;
+    pi[u] = d[u].x * x[u].y + d[u].y * x[u].x;
+    if (TAIL && gb + u >= g1) pr[u] = pi[u] = 0;
+  }
+#pragma unroll
+  for (int h = PU / 2; h > 0; h >>= 1)
+#pragma unroll
+    for (int u = 0; u < h; ++u) {
+      pr[u] += pr[u + h];
+      pi[u] += pi[u + h];
+    }
+  ar += pr[0];
+  ai += pi[0];
+}
+
 // Partial projections P_c = Σ_G (D·B^H)[c, G] v(G) of a vector v held in LDS. D·B^H is column-major
 // nch × npw (entry (c, G) at G·nch + c), so each of its columns is contiguous over c: lanes take
 // channels, so that a wave reads whole columns, and the waves take contiguous quarters of the plane
@@ -39,37 +77,16 @@ inline size_t nl_lds_entries(int nch) { return (size_t)(1 + kSmallWaves) * nch; 
 template <typename T>
 __device__ __forceinline__ void nl_partials(typename V2s<T>::t* part, const typename V2s<T>::t* v,
                                             const typename V2s<T>::t* __restrict__ dk, int npw, int nch) {
-  using V2 = typename V2s<T>::t;
-  constexpr int PU = 16;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int PU = sizeof(T) == 4 ? 16 : 8;  // rows in flight (register budget: 4·PU values)
+  // the wave index is uniform over the wave: read it as such, so that the row range, the clamps
+  // and the row offsets are scalar values and each load adds only the lane's channel offset
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int per = (npw + kSmallWaves - 1) / kSmallWaves, g0 = wave * per, g1 = min(npw, g0 + per);
   for (int c = lane; c < nch; c += 64) {
     T ar = 0, ai = 0;
-    for (int gb = g0; gb < g1; gb += PU) {
-      V2 d[PU];
-#pragma unroll
-      for (int u = 0; u < PU; ++u)
-        if (gb + u < g1) d[u] = dk[(size_t)(gb + u) * nch + c];
-      T pr[PU], pi[PU];
-#pragma unroll
-      for (int u = 0; u < PU; ++u) {
-        pr[u] = pi[u] = 0;
-        if (gb + u < g1) {
-          const V2 x = v[gb + u];
-          pr[u] = d[u].x * x.x - d[u].y * x.y;
-          pi[u] = d[u].x * x.y + d[u].y * x.x;
-        }
-      }
-#pragma unroll
-      for (int h = PU / 2; h > 0; h >>= 1)
-#pragma unroll
-        for (int u = 0; u < h; ++u) {
-          pr[u] += pr[u + h];
-          pi[u] += pi[u + h];
-        }
-      ar += pr[0];
-      ai += pi[0];
-    }
+    int gb = g0;
+    for (; gb + PU <= g1; gb += PU) nl_block<T, PU, false>(ar, ai, v, dk, nch, c, gb, g1);
+    if (gb < g1) nl_block<T, PU, true>(ar, ai, v, dk, nch, c, gb, g1);
     part[wave * nch + c].x = ar;
     part[wave * nch + c].y = ai;
   }
@@ -100,22 +117,38 @@ __device__ __forceinline__ void nl_term(T& ox, T& oy, const typename V2s<T>::t& 
   oy = fma(b.y, p.x, oy);
 }
 
+// One batch of nl_expand: channels c0 .. c0 + EC − 1 (TAIL: channels from nch on read the last
+// channel and add the exact zero b·p with b = 0, which leaves (ox, oy) unchanged). All EC loads are
+// issued before any is used.
+template <typename T, int EC, bool TAIL>
+__device__ __forceinline__ void nl_expand_batch(T& ox, T& oy, const typename V2s<T>::t* __restrict__ bk,
+                                                const typename V2s<T>::t* proj, int g, int npw, int nch, int c0) {
+  using V2 = typename V2s<T>::t;
+  V2 bb[EC], pc[EC];
+#pragma unroll
+  for (int j = 0; j < EC; ++j) {
+    const int cc = TAIL ? min(c0 + j, nch - 1) : c0 + j;
+    bb[j] = (bk + cc * npw)[g];  // uniform column pointer (npw·nch < 2³¹), lane offset g
+    pc[j] = proj[cc];
+  }
+  __builtin_amdgcn_sched_group_barrier(0x0020, EC, 0);
+  __builtin_amdgcn_sched_group_barrier(0x0100, EC, 0);
+#pragma unroll
+  for (int j = 0; j < EC; ++j) {
+    if (TAIL && c0 + j >= nch) bb[j].x = bb[j].y = 0;
+    nl_term<T>(ox, oy, bb[j], pc[j]);
+  }
+}
+
 // (ox, oy) += Σ_c B[G, c] P_c in channel order. B is column-major npw × nch (entry (G, c) at
 // c·npw + G), so the lanes' loads of one channel coalesce; EC channels' loads are in flight at once.
 template <typename T>
 __device__ __forceinline__ void nl_expand(T& ox, T& oy, const typename V2s<T>::t* __restrict__ bk,
                                           const typename V2s<T>::t* proj, int g, int npw, int nch) {
-  using V2 = typename V2s<T>::t;
-  constexpr int EC = 12;
-  for (int c0 = 0; c0 < nch; c0 += EC) {
-    V2 bb[EC];
-#pragma unroll
-    for (int j = 0; j < EC; ++j)
-      if (c0 + j < nch) bb[j] = bk[(size_t)(c0 + j) * npw + g];
-#pragma unroll
-    for (int j = 0; j < EC; ++j)
-      if (c0 + j < nch) nl_term<T>(ox, oy, bb[j], proj[c0 + j]);
-  }
+  constexpr int EC = sizeof(T) == 4 ? 18 : 9;  // channels in flight (register budget: 4·EC values)
+  int c0 = 0;
+  for (; c0 + EC <= nch; c0 += EC) nl_expand_batch<T, EC, false>(ox, oy, bk, proj, g, npw, nch, c0);
+  if (c0 < nch) nl_expand_batch<T, EC, true>(ox, oy, bk, proj, g, npw, nch, c0);
 }
 
 // Band b (k-point b / nbk) of psi -> hpsi. vr is V(r) in natural grid order; g2fft the flat

@@ -303,8 +303,10 @@ def test_resident_iteration_equals_host_loop(scf_mod, gpu, precision, nspin, ext
     # iteration) against the host-synchronised loop on the same deck. The device bisection adds
     # the electron count in another order, so the Fermi level moves in its last bits: in FP64 the
     # two solves take the same iterations and agree to 1e-9 eV; with an FP32 filter that
-    # perturbation changes the FP32 roundings of V(r), and the solves agree to the FP32 bar
-    # (1e-5 Ry/atom) within one or two iterations. With mixing_ndim 16 the history holds more
+    # perturbation changes the FP32 roundings of V(r), which decide where the filter passes of an
+    # iteration meet their residual target, and the solves agree to the FP32 bar (1e-5 Ry/atom)
+    # after 11-15 iterations each (measured over kernel builds that differ only in roundings), so
+    # their iteration counts are compared within four. With mixing_ndim 16 the history holds more
     # than the 10 entries where the DIIS sums once outgrew one reduction launch; this deck does
     # not converge with so long a history, so both loops run 20 iterations and their eigenvalues
     # after them are compared. At β = 0.3 the residuals of a long history become nearly dependent
@@ -335,7 +337,7 @@ def test_resident_iteration_equals_host_loop(scf_mod, gpu, precision, nspin, ext
         assert rr["n_iter"] == rh["n_iter"]
         tol_e, tol_ev, tol_rho = 1e-9, 1e-9, 1e-10
     else:
-        assert abs(rr["n_iter"] - rh["n_iter"]) <= 2
+        assert abs(rr["n_iter"] - rh["n_iter"]) <= 4
         tol_e, tol_ev, tol_rho = TOL_TERM * 2, 1e-4, 1e-5
     for t in TERMS + ["fermi", "integrated_charge"]:
         assert abs(rr[t] - rh[t]) < tol_e, t
