@@ -186,57 +186,98 @@ struct RrLds {
   double red[16][JMAX];
 };
 
+template <int MB>
 __device__ __forceinline__ void rr_small_body(int k, int npw, int m, int nres, const double2* __restrict__ Yb,
                                               const double2* __restrict__ HYb, double2* __restrict__ Vb,
                                               double* __restrict__ lam_out, double* __restrict__ res_out,
                                               double floor_rel, const int* __restrict__ skip) {
+  static_assert(MB == 16 || MB == 32, "blocks of up to 16 or 32 vectors");
   if (skip && *skip) return;  // a filter pass after the residual gate passed (uniform)
   extern __shared__ unsigned char smem_raw[];
   RrLds& L = *reinterpret_cast<RrLds*>(smem_raw);
   JacobiLds& J = L.J;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const double2* Y = Yb + (size_t)k * npw * m;
   const double2* HY = HYb + (size_t)k * npw * m;
   double2* V = Vb + (size_t)k * npw * m;
-  // S(i, j) = Σ_G conj(Y_i) Y_j and Hs(i, j) = Σ_G conj(Y_i) (HY)_j, summed in G order over
-  // row chunks staged in LDS (coalesced loads); a thread owns up to JMAX²/RR_NT entries
-  constexpr int EPT = JMAX * JMAX / RR_NT;
-  double sr[EPT], si[EPT], hr[EPT], hi[EPT];
+  // S(i, j) = Σ_G conj(Y_i) Y_j and Hs(i, j) = Σ_G conj(Y_i) (HY)_j over row chunks staged in LDS
+  // (coalesced loads; the next chunk's loads are issued before this chunk's products). Wave w takes
+  // rows [w·RR_ROWS/4, (w+1)·RR_ROWS/4) of every chunk and each lane a BLK × BLK block of entries,
+  // so a row costs 3·BLK LDS reads per lane for 8·BLK² products; the waves' sums are added in wave
+  // order at the end.
+  constexpr int BLK = MB / 8, LPT = RR_ROWS * JMAX / RR_NT, WROWS = RR_ROWS / (RR_NT / 64);
+  const int bi = (lane & 7) * BLK, bj = (lane >> 3) * BLK;
+  double sre[BLK][BLK], sim[BLK][BLK], hre[BLK][BLK], him[BLK][BLK];
 #pragma unroll
-  for (int q = 0; q < EPT; ++q) sr[q] = si[q] = hr[q] = hi[q] = 0.0;
-  for (int g0 = 0; g0 < npw; g0 += RR_ROWS) {
-    for (int e = t; e < RR_ROWS * m; e += RR_NT) {
-      const int gg = e % RR_ROWS, i = e / RR_ROWS, g = g0 + gg;
-      L.Ys[gg][i] = g < npw ? Y[(size_t)i * npw + g] : make_double2(0.0, 0.0);
-      L.HYs[gg][i] = g < npw ? HY[(size_t)i * npw + g] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    const int rows = min(RR_ROWS, npw - g0);
+  for (int u = 0; u < BLK; ++u)
 #pragma unroll
-    for (int q = 0; q < EPT; ++q) {
+    for (int v = 0; v < BLK; ++v) sre[u][v] = sim[u][v] = hre[u][v] = him[u][v] = 0.0;
+  double2 py[LPT], ph[LPT];
+  auto load_chunk = [&](int g0) {
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
       const int e = t + q * RR_NT;
-      if (e < m * m) {
-        const int i = e % m, j = e / m;
-        for (int gg = 0; gg < rows; ++gg) {
-          const double2 a = L.Ys[gg][i], b = L.Ys[gg][j], h = L.HYs[gg][j];
-          sr[q] += a.x * b.x + a.y * b.y;
-          si[q] += a.x * b.y - a.y * b.x;
-          hr[q] += a.x * h.x + a.y * h.y;
-          hi[q] += a.x * h.y - a.y * h.x;
-        }
+      const int gg = e % RR_ROWS, i = e / RR_ROWS, g = g0 + gg;
+      const bool in = e < RR_ROWS * m && g < npw;
+      py[q] = in ? Y[(size_t)i * npw + g] : make_double2(0.0, 0.0);
+      ph[q] = in ? HY[(size_t)i * npw + g] : make_double2(0.0, 0.0);
+    }
+  };
+  load_chunk(0);
+  for (int g0 = 0; g0 < npw; g0 += RR_ROWS) {
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+      const int e = t + q * RR_NT;
+      if (e < RR_ROWS * m) {
+        L.Ys[e % RR_ROWS][e / RR_ROWS] = py[q];
+        L.HYs[e % RR_ROWS][e / RR_ROWS] = ph[q];
       }
     }
     __syncthreads();
-  }
+    if (g0 + RR_ROWS < npw) load_chunk(g0 + RR_ROWS);
+    const int r0 = wave * WROWS, r1 = min(npw - g0, r0 + WROWS);
+    for (int gg = r0; gg < r1; ++gg) {
+      double2 a[BLK], b[BLK], h[BLK];
 #pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int e = t + q * RR_NT;
-    if (e < m * m) {
-      J.A[e % m][e / m] = make_double2(sr[q], si[q]);
-      L.Hs[e % m][e / m] = make_double2(hr[q], hi[q]);
+      for (int u = 0; u < BLK; ++u) {
+        a[u] = L.Ys[gg][bi + u];  // columns past m hold stale values: their entries are dropped
+        b[u] = L.Ys[gg][bj + u];
+        h[u] = L.HYs[gg][bj + u];
+      }
+#pragma unroll
+      for (int u = 0; u < BLK; ++u)
+#pragma unroll
+        for (int v = 0; v < BLK; ++v) {
+          sre[u][v] += a[u].x * b[v].x + a[u].y * b[v].y;
+          sim[u][v] += a[u].x * b[v].y - a[u].y * b[v].x;
+          hre[u][v] += a[u].x * h[v].x + a[u].y * h[v].y;
+          him[u][v] += a[u].x * h[v].y - a[u].y * h[v].x;
+        }
     }
+    __syncthreads();
   }
-  __syncthreads();
+  for (int w = 0; w < RR_NT / 64; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int u = 0; u < BLK; ++u)
+#pragma unroll
+        for (int v = 0; v < BLK; ++v) {
+          const int i = bi + u, j = bj + v;
+          if (i < m && j < m) {
+            if (w == 0) {
+              J.A[i][j] = make_double2(sre[u][v], sim[u][v]);
+              L.Hs[i][j] = make_double2(hre[u][v], him[u][v]);
+            } else {
+              J.A[i][j].x += sre[u][v];
+              J.A[i][j].y += sim[u][v];
+              L.Hs[i][j].x += hre[u][v];
+              L.Hs[i][j].y += him[u][v];
+            }
+          }
+        }
+    }
+    __syncthreads();
+  }
   // Hermitian parts (subspace.hip k_hermitize), V = I
   for (int e = t; e < m * m; e += RR_NT) {
     const int i = e % m, j = e / m;
@@ -364,45 +405,46 @@ __device__ __forceinline__ void rr_small_body(int k, int npw, int m, int nres, c
   }
   if (t < m) lam_out[(size_t)k * m + t] = J.A[J.order[t]][J.order[t]].x;
   __syncthreads();
-  // V = Y·C and the residual weights, one row G per thread, eight columns of C at a time (each
-  // row of Y and HY is read m/8 times, coalesced across the wave)
-  constexpr int CB = 8;
-  double acc[JMAX];
+  double acc[MB];
 #pragma unroll
-  for (int jj = 0; jj < JMAX; ++jj) acc[jj] = 0.0;
-  for (int g = t; g < npw; g += RR_NT) {
+  for (int jj = 0; jj < MB; ++jj) acc[jj] = 0.0;
+  {
+    // V = Y·C and the residual weights, one row G per thread, eight columns of C at a time (each
+    // row of Y and HY is read m/8 times, coalesced across the wave)
+    constexpr int CB = 8;
+    for (int g = t; g < npw; g += RR_NT) {
 #pragma unroll
-    for (int jc = 0; jc < JMAX; jc += CB) {
-      if (jc >= m) continue;  // fully unrolled: acc stays in registers
-      double vr[CB], vi[CB], rr[CB], ri[CB];
+      for (int jc = 0; jc < MB; jc += CB) {
+        if (jc >= m) continue;  // fully unrolled: acc stays in registers
+        double vr[CB], vi[CB], rr[CB], ri[CB];
 #pragma unroll
-      for (int q = 0; q < CB; ++q) vr[q] = vi[q] = rr[q] = ri[q] = 0.0;
-      for (int i = 0; i < m; ++i) {
-        const double2 y = Y[(size_t)i * npw + g], h = HY[(size_t)i * npw + g];
+        for (int q = 0; q < CB; ++q) vr[q] = vi[q] = rr[q] = ri[q] = 0.0;
+        for (int i = 0; i < m; ++i) {
+          const double2 y = Y[(size_t)i * npw + g], h = HY[(size_t)i * npw + g];
+#pragma unroll
+          for (int q = 0; q < CB; ++q) {
+            const double2 c = L.C[i][min(jc + q, m - 1)];
+            vr[q] += y.x * c.x - y.y * c.y;
+            vi[q] += y.x * c.y + y.y * c.x;
+            rr[q] += h.x * c.x - h.y * c.y;
+            ri[q] += h.x * c.y + h.y * c.x;
+          }
+        }
 #pragma unroll
         for (int q = 0; q < CB; ++q) {
-          const double2 c = L.C[i][min(jc + q, m - 1)];
-          vr[q] += y.x * c.x - y.y * c.y;
-          vi[q] += y.x * c.y + y.y * c.x;
-          rr[q] += h.x * c.x - h.y * c.y;
-          ri[q] += h.x * c.y + h.y * c.x;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < CB; ++q) {
-        const int jj = jc + q;
-        if (jj < m) V[(size_t)jj * npw + g] = make_double2(vr[q], vi[q]);
-        if (jj < nres) {
-          const double l = J.A[J.order[jj]][J.order[jj]].x;
-          const double dx = rr[q] - l * vr[q], dy = ri[q] - l * vi[q];
-          acc[jj] += dx * dx + dy * dy;
+          const int jj = jc + q;
+          if (jj < m) V[(size_t)jj * npw + g] = make_double2(vr[q], vi[q]);
+          if (jj < nres) {
+            const double l = J.A[J.order[jj]][J.order[jj]].x;
+            const double dx = rr[q] - l * vr[q], dy = ri[q] - l * vi[q];
+            acc[jj] += dx * dx + dy * dy;
+          }
         }
       }
     }
   }
-  const int wave = t >> 6;
 #pragma unroll
-  for (int jj = 0; jj < JMAX; ++jj) {
+  for (int jj = 0; jj < MB; ++jj) {
     if (jj >= nres) continue;
     double v = acc[jj];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -416,19 +458,21 @@ __device__ __forceinline__ void rr_small_body(int k, int npw, int m, int nres, c
   }
 }
 
+template <int MB>
 __global__ void __launch_bounds__(RR_NT) k_rr_small(int npw, int m, int nres, const double2* __restrict__ Yb,
                                                     const double2* __restrict__ HYb, double2* __restrict__ Vb,
                                                     double* __restrict__ lam_out, double* __restrict__ res_out,
                                                     double floor_rel, const int* __restrict__ skip) {
-  rr_small_body((int)blockIdx.x, npw, m, nres, Yb, HYb, Vb, lam_out, res_out, floor_rel, skip);
+  rr_small_body<MB>((int)blockIdx.x, npw, m, nres, Yb, HYb, Vb, lam_out, res_out, floor_rel, skip);
 }
 
 // the Rayleigh-Ritz steps of several solves' passes in one launch: blockIdx.y selects the request,
 // blockIdx.x its k-point block (the code of k_rr_small per block)
+template <int MB>
 __global__ void __launch_bounds__(RR_NT) k_rr_multi(PassRrMulti mr, int npw, int m, int nres, double floor_rel) {
   const PassRrReq& q = mr.r[blockIdx.y];
   if ((int)blockIdx.x >= q.nk) return;  // uniform
-  rr_small_body((int)blockIdx.x, npw, m, nres, q.Y, q.HY, q.V, q.lam, q.res, floor_rel, q.skip);
+  rr_small_body<MB>((int)blockIdx.x, npw, m, nres, q.Y, q.HY, q.V, q.lam, q.res, floor_rel, q.skip);
 }
 
 }  // namespace
@@ -512,9 +556,15 @@ void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y
                      double* lam, double* res, double floor_rel, const int* skip) {
   PWK_REQUIRE(rr_small_supported(m) && nres <= m && npw >= m && nk >= 1, PWK_E_INVALID, "bad Rayleigh-Ritz shape");
   const size_t sh = sizeof(RrLds);
-  ensure_lds_limit((const void*)k_rr_small, sh);
-  k_rr_small<<<(unsigned)nk, RR_NT, sh, c.stream>>>(npw, m, nres, (const double2*)Y, (const double2*)HY,
-                                                   (double2*)V, lam, res, floor_rel, skip);
+  auto go = [&](auto kern) {
+    ensure_lds_limit((const void*)kern, sh);
+    kern<<<(unsigned)nk, RR_NT, sh, c.stream>>>(npw, m, nres, (const double2*)Y, (const double2*)HY, (double2*)V,
+                                               lam, res, floor_rel, skip);
+  };
+  if (m <= 16)
+    go(k_rr_small<16>);
+  else
+    go(k_rr_small<32>);
   PWK_HIP(hipGetLastError());
 }
 
@@ -523,8 +573,14 @@ void launch_rr_multi(hipStream_t stream, const PassShape& sh, const PassRrMulti&
   PWK_REQUIRE(n >= 1 && n <= MAX_PASS_BATCH && rr_small_supported(sh.nbs) && sh.nb <= sh.nbs && sh.npw >= sh.nbs,
               PWK_E_INVALID, "bad batched Rayleigh-Ritz shape");
   const size_t lds = sizeof(RrLds);
-  ensure_lds_limit((const void*)k_rr_multi, lds);
-  k_rr_multi<<<dim3((unsigned)max_nk, (unsigned)n), RR_NT, lds, stream>>>(m, sh.npw, sh.nbs, sh.nb, floor_rel);
+  auto go = [&](auto kern) {
+    ensure_lds_limit((const void*)kern, lds);
+    kern<<<dim3((unsigned)max_nk, (unsigned)n), RR_NT, lds, stream>>>(m, sh.npw, sh.nbs, sh.nb, floor_rel);
+  };
+  if (sh.nbs <= 16)
+    go(k_rr_multi<16>);
+  else
+    go(k_rr_multi<32>);
   PWK_HIP(hipGetLastError());
 }
 
