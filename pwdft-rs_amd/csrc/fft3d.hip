@@ -21,13 +21,14 @@ __global__ void __launch_bounds__(SMALL_NT) k_fft3d_small(typename V2s<T>::t* __
   constexpr int NN = N * N * N;
   V2* d = data + (size_t)blockIdx.x * NN;
   if (real_in) {
-    for (int i = threadIdx.x; i < NN; i += SMALL_NT) {
-      G[i].x = (T)(real_in[i] * scale);
-      G[i].y = 0;
-    }
+    grid_points<NN, double>([&](int i) { return real_in[i]; },
+                            [&](int i, double x) {
+                              G[i].x = (T)(x * scale);
+                              G[i].y = 0;
+                            });
   } else {
     const V2* in = src ? src + (size_t)blockIdx.x * NN : d;  // out of place with src
-    for (int i = threadIdx.x; i < NN; i += SMALL_NT) G[i] = in[i];
+    grid_points<NN, V2>([&](int i) { return in[i]; }, [&](int i, V2 x) { G[i] = x; });
   }
   __syncthreads();
   lds_lines<T, N, 0, SIGN>(G, nullptr);
@@ -36,7 +37,7 @@ __global__ void __launch_bounds__(SMALL_NT) k_fft3d_small(typename V2s<T>::t* __
   __syncthreads();
   lds_lines<T, N, 2, SIGN>(G, nullptr);
   __syncthreads();
-  for (int i = threadIdx.x; i < NN; i += SMALL_NT) d[i] = G[i];
+  grid_points<NN, V2>([&](int i) { return G[i]; }, [&](int i, V2 x) { d[i] = x; });
 }
 
 #define PWK_FFT_SMALL_SIZES(X) X(8) X(9) X(10) X(12) X(15) X(16) X(18) X(20) X(24)

@@ -266,47 +266,70 @@ __global__ void __launch_bounds__(SMALL_NT) k_rho_xc_veff_small(
   extern __shared__ unsigned char smem_raw[];
   double2* G = reinterpret_cast<double2*>(smem_raw);
   constexpr int NN = N * N * N;
-  for (int i = threadIdx.x; i < NN; i += SMALL_NT) G[i] = make_double2(rho_r[i] * inv_n, 0.0);
+  // every pass issues the loads of a thread's points before it uses any of them (grid_points)
+  grid_points<NN, double>([&](int i) { return rho_r[i]; },
+                          [&](int i, double r) { G[i] = make_double2(r * inv_n, 0.0); });
   small_fft3d<N, -1>(G);
-  for (int i = threadIdx.x; i < NN; i += SMALL_NT) {
-    rho_g[i] = G[i];
-    const double r = core ? fmax(rho_r[i] + core[i] * cs, 0.0) : rho_r[i];
-    rho_xc[i] = r;
-    double e, v;
-    lda_point(r, e, v);
-    exc[i] = e;
-    vxc[i] = v;
-    if constexpr (VEFF) G[i] = make_double2(v * inv_n, 0.0);
-  }
+  struct RhoIn {
+    double2 g;
+    double r, c;
+  };
+  grid_points<NN, RhoIn>(
+      [&](int i) {
+        RhoIn x;
+        x.g = G[i];
+        x.r = rho_r[i];
+        x.c = core ? core[i] : 0.0;
+        return x;
+      },
+      [&](int i, const RhoIn& x) {
+        rho_g[i] = x.g;
+        const double r = core ? fmax(x.r + x.c * cs, 0.0) : x.r;
+        rho_xc[i] = r;
+        double e, v;
+        lda_point(r, e, v);
+        exc[i] = e;
+        vxc[i] = v;
+        if constexpr (VEFF) G[i] = make_double2(v * inv_n, 0.0);
+      });
   if constexpr (VEFF) {
     small_fft3d<N, -1>(G);
-    for (int i = threadIdx.x; i < NN; i += SMALL_NT) {
-      double2 acc = make_double2(0.0, 0.0);
-      const double gg = g2[i];
-      if (gg > kG2Zero) {
-        const double2 r = rho_g[i];  // written by this thread above
-        acc.x = r.x * fourpi_e2 / gg;
-        acc.y = r.y * fourpi_e2 / gg;
-      }
-      const double2 a = vloc_g[i];
-      acc.x = a.x + acc.x;
-      acc.y = a.y + acc.y;
-      const double2 x = G[i];
-      acc.x += x.x;
-      acc.y += x.y;
-      veff_g[i] = acc;
-      G[i] = acc;
-    }
+    struct VIn {
+      double2 rg, vl, x;
+      double gg;
+    };
+    grid_points<NN, VIn>(
+        [&](int i) {
+          VIn y;
+          y.gg = g2[i];
+          y.rg = rho_g[i];  // written by this thread above
+          y.vl = vloc_g[i];
+          y.x = G[i];
+          return y;
+        },
+        [&](int i, const VIn& y) {
+          double2 acc = make_double2(0.0, 0.0);
+          if (y.gg > kG2Zero) {
+            acc.x = y.rg.x * fourpi_e2 / y.gg;
+            acc.y = y.rg.y * fourpi_e2 / y.gg;
+          }
+          acc.x = y.vl.x + acc.x;
+          acc.y = y.vl.y + acc.y;
+          acc.x += y.x.x;
+          acc.y += y.x.y;
+          veff_g[i] = acc;
+          G[i] = acc;
+        });
     small_fft3d<N, 1>(G);
-    for (int g = threadIdx.x; g < NN; g += SMALL_NT) {
-      const int i1 = g / (N * N), i2 = (g / N) % N, i3 = g % N;
-      const size_t idx = ((size_t)i3 * N + i1) * N + i2;  // plane-major [i3][i1][i2]
-      const double v = G[g].x;
-      v64[idx] = v;
-      v32[idx] = (float)v;
-      n64[g] = v;
-      n32[g] = (float)v;
-    }
+    grid_points<NN, double>([&](int g) { return G[g].x; },
+                            [&](int g, double v) {
+                              const int i1 = g / (N * N), i2 = (g / N) % N, i3 = g % N;
+                              const size_t idx = ((size_t)i3 * N + i1) * N + i2;  // plane-major [i3][i1][i2]
+                              v64[idx] = v;
+                              v32[idx] = (float)v;
+                              n64[g] = v;
+                              n32[g] = (float)v;
+                            });
   }
 }
 
@@ -317,9 +340,9 @@ __global__ void __launch_bounds__(SMALL_NT) k_fft_c2r_small(const double2* __res
   extern __shared__ unsigned char smem_raw[];
   double2* G = reinterpret_cast<double2*>(smem_raw);
   constexpr int NN = N * N * N;
-  for (int i = threadIdx.x; i < NN; i += SMALL_NT) G[i] = in[i];
+  grid_points<NN, double2>([&](int i) { return in[i]; }, [&](int i, double2 x) { G[i] = x; });
   small_fft3d<N, 1>(G);
-  for (int i = threadIdx.x; i < NN; i += SMALL_NT) out[i] = G[i].x * scale;
+  grid_points<NN, double>([&](int i) { return G[i].x; }, [&](int i, double x) { out[i] = x * scale; });
 }
 
 #define PWK_GRID_SMALL_SIZES(X) X(8) X(9) X(10) X(12) X(15) X(16) X(18) X(20)

@@ -28,6 +28,29 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// A pass of one workgroup over the NN points of a grid, thread t taking points t + q·SMALL_NT:
+// load(i) for up to sixteen of the thread's points is issued before use(i, value) runs on any of
+// them, so the loads (global memory written by earlier launches, often on another XCD) are in
+// flight together instead of one round trip per point.
+template <int NN, typename V, typename Load, typename Use>
+__device__ __forceinline__ void grid_points(Load load, Use use) {
+  constexpr int PER = (NN + SMALL_NT - 1) / SMALL_NT, CH = PER < 16 ? PER : 16;
+#pragma unroll
+  for (int q0 = 0; q0 < PER; q0 += CH) {
+    V v[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int i = threadIdx.x + (q0 + q) * SMALL_NT;
+      if (q0 + q < PER && i < NN) v[q] = load(i);
+    }
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int i = threadIdx.x + (q0 + q) * SMALL_NT;
+      if (q0 + q < PER && i < NN) use(i, v[q]);
+    }
+  }
+}
+
 // One set of N² lines through the LDS grid G[x][y][z] (z fastest). AXIS 2: z-lines
 // (stride 1), 1: y-lines (stride N), 0: x-lines (stride N²). With vr, the x pass runs
 // inverse -> × V(r) -> forward on each line while it is in registers.
