@@ -772,8 +772,9 @@ def serving_stream(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4, help="timed steps; one step = --slots solves served at once")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16,
+                    help="timed steps; one step = --slots solves served at once (16 x 16 = 256 timed solves)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--pass-batch-us", type=int, default=-1,
                     help="cross-request filter-pass batching window of the served runs, microseconds "

@@ -151,6 +151,75 @@ __device__ __forceinline__ void nl_expand(T& ox, T& oy, const typename V2s<T>::t
   if (c0 < nch) nl_expand_batch<T, EC, true>(ox, oy, bk, proj, g, npw, nch, c0);
 }
 
+// Pruned transform passes (grids with one line per thread, N² <= SMALL_NT): a z-line (x, y) carries
+// data only where the basis has plane waves in column (x, y), a y-line (x, z) only where plane x
+// has such a column. LineAct holds the thread's two flags (its line l = threadIdx.x in either
+// pass); small_line_activity derives them once per launch from the basis's grid indices, using the
+// LDS grid as scratch (before the grid's first use; every write of a flag stores 1).
+struct LineAct {
+  bool z = true, y = true;
+};
+
+template <int N>
+__device__ __forceinline__ LineAct small_line_activity(void* scratch, const int* __restrict__ g2fft, int npw) {
+  LineAct a;
+  if constexpr (N * N <= SMALL_NT) {
+    int* fl = reinterpret_cast<int*>(scratch);
+    for (int i = threadIdx.x; i < N * N + N; i += SMALL_NT) fl[i] = 0;
+    lds_barrier();
+    for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
+      const int col = g2fft[g] / N;  // x·N + y
+      fl[col] = 1;
+      fl[N * N + col / N] = 1;
+    }
+    lds_barrier();
+    const int t = threadIdx.x;
+    a.z = t < N * N && fl[t] != 0;
+    a.y = t < N * N && fl[N * N + t / N] != 0;
+    lds_barrier();
+  }
+  return a;
+}
+
+// The grid's state before the scatter of a vector: with pruned passes only the columns that carry
+// plane waves are cleared (the first z-pass writes zeros on every other z-line), else all of it.
+template <typename T, int N>
+__device__ __forceinline__ void small_clear(typename V2s<T>::t* G, const LineAct& act) {
+  if constexpr (N * N <= SMALL_NT) {
+    if ((int)threadIdx.x < N * N && act.z) {
+#pragma unroll
+      for (int z = 0; z < N; ++z) {
+        G[threadIdx.x * N + z].x = 0;
+        G[threadIdx.x * N + z].y = 0;
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
+      G[i].x = 0;
+      G[i].y = 0;
+    }
+  }
+}
+
+// The inverse z and y passes after the scatter (pruned where the grid allows: inactive lines are
+// written as zeros) and the forward y and z passes before the gather (pruned: inactive lines are
+// not needed).
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void small_zpass(typename V2s<T>::t* G, const LineAct& act) {
+  if constexpr (N * N <= SMALL_NT)
+    lds_lines_pruned<T, N, 2, SIGN, SIGN == 1>(G, act.z);
+  else
+    lds_lines<T, N, 2, SIGN>(G, nullptr);
+}
+
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void small_ypass(typename V2s<T>::t* G, const LineAct& act) {
+  if constexpr (N * N <= SMALL_NT)
+    lds_lines_pruned<T, N, 1, SIGN, SIGN == 1>(G, act.y);
+  else
+    lds_lines<T, N, 1, SIGN>(G, nullptr);
+}
+
 // Band b (k-point b / nbk) of psi -> hpsi. vr is V(r) in natural grid order; g2fft the flat
 // grid index of each plane wave; B (npw × nch) and DBH (nch × npw) column-major blocks per k.
 // With coef (α, c, β per k-point) the output is the Chebyshev recurrence step
@@ -211,27 +280,25 @@ __device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const type
       for (int f = 0; f < N; ++f) vline[f] = vr[threadIdx.x + f * N * N];
     }
   }
-  for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
-    G[i].x = 0;
-    G[i].y = 0;
-  }
+  const LineAct act = small_line_activity<N>(G, g2fft, npw);
+  small_clear<T, N>(G, act);
   lds_barrier();
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) G[g2fft[g]] = p[g];
   nl_partials<T>(part, p, dk, npw, nch);
   lds_barrier();
   nl_combine<T>(proj, part, nch);
-  lds_lines<T, N, 2, 1>(G, nullptr);
+  small_zpass<T, N, 1>(G, act);
   lds_barrier();
-  lds_lines<T, N, 1, 1>(G, nullptr);
+  small_ypass<T, N, 1>(G, act);
   lds_barrier();
   if constexpr (PRE_V)
     lds_xlines_v<T, N, 1>(G, vline);
   else
     lds_lines<T, N, 0, 1>(G, vr);
   lds_barrier();
-  lds_lines<T, N, 1, -1>(G, nullptr);
+  small_ypass<T, N, -1>(G, act);
   lds_barrier();
-  lds_lines<T, N, 2, -1>(G, nullptr);
+  small_zpass<T, N, -1>(G, act);
   lds_barrier();
   V2* h = hpsi + (size_t)b * npw;
   T alpha = 0, cshift = 0, beta = 0;
@@ -270,7 +337,8 @@ __device__ __forceinline__ void hpsi_small_body(int bid, int nblocks, const type
   }
 }
 
-// The local part of H·v for a vector held in LDS (cur, npw entries): scatter onto the grid G,
+// The local part of H·v for a vector held in LDS (cur, npw entries): scatter onto the grid G
+// (cleared as small_clear does; the passes pruned by act where the grid allows),
 // partial projections of v (nl_partials, summed into proj by nl_combine after the next barrier),
 // inverse FFT z, y, x with V(r) applied between the x transforms (vline: V(r) of the thread's
 // x-line, when N² <= SMALL_NT; else read from vr), forward FFT y, z. On return G holds
@@ -282,30 +350,27 @@ __device__ __forceinline__ void small_h_core(typename V2s<T>::t* G, typename V2s
                                              typename V2s<T>::t* part, const typename V2s<T>::t* cur,
                                              const T (&vline)[N * N <= SMALL_NT ? N : 1], const T* __restrict__ vr,
                                              const int* __restrict__ g2fft, int gi0, int npw, int nch,
-                                             const typename V2s<T>::t* __restrict__ dk) {
+                                             const typename V2s<T>::t* __restrict__ dk, const LineAct& act) {
   constexpr bool PRE_V = N * N <= SMALL_NT;
-  for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
-    G[i].x = 0;
-    G[i].y = 0;
-  }
+  small_clear<T, N>(G, act);
   lds_barrier();
   if ((int)threadIdx.x < npw) G[gi0] = cur[threadIdx.x];
   for (int g = threadIdx.x + SMALL_NT; g < npw; g += SMALL_NT) G[g2fft[g]] = cur[g];
   nl_partials<T>(part, cur, dk, npw, nch);
   lds_barrier();
   nl_combine<T>(proj, part, nch);
-  lds_lines<T, N, 2, 1>(G, nullptr);
+  small_zpass<T, N, 1>(G, act);
   lds_barrier();
-  lds_lines<T, N, 1, 1>(G, nullptr);
+  small_ypass<T, N, 1>(G, act);
   lds_barrier();
   if constexpr (PRE_V)
     lds_xlines_v<T, N, 1>(G, vline);
   else
     lds_lines<T, N, 0, 1>(G, vr);
   lds_barrier();
-  lds_lines<T, N, 1, -1>(G, nullptr);
+  small_ypass<T, N, -1>(G, act);
   lds_barrier();
-  lds_lines<T, N, 2, -1>(G, nullptr);
+  small_zpass<T, N, -1>(G, act);
   lds_barrier();
 }
 
@@ -363,6 +428,7 @@ __global__ void __launch_bounds__(SMALL_NT) k_lanczos_small(const int* __restric
   // grid index and kinetic energy of plane wave threadIdx.x, kept for every step
   const int gi0 = (int)threadIdx.x < npw ? g2fft[threadIdx.x] : 0;
   const double tk0 = (int)threadIdx.x < npw ? t[threadIdx.x] : 0.0;
+  const LineAct act = small_line_activity<N>(G, g2fft, npw);
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
     v1[g] = start_vector_entry(g, 0, kb);
     v0[g] = make_double2(0.0, 0.0);
@@ -378,7 +444,7 @@ __global__ void __launch_bounds__(SMALL_NT) k_lanczos_small(const int* __restric
   double beta = 0.0;
   for (int j = 0; j < steps; ++j) {
     lds_barrier();
-    small_h_core<double, N>(G, proj, part, v1, vline, vr, g2fft, gi0, npw, nch, dk);
+    small_h_core<double, N>(G, proj, part, v1, vline, vr, g2fft, gi0, npw, nch, dk, act);
     auto epi = [&](int g, int gi, double tk) {
       const double2 v = v1[g], hw = G[gi];
       double ox = fma(tk, v.x, hw.x * inv_n), oy = fma(tk, v.y, hw.y * inv_n);
@@ -467,6 +533,7 @@ __device__ __forceinline__ void cheb_filter_body(int bid, int nblocks, const dou
   // grid index and kinetic energy of plane wave threadIdx.x, kept for every step
   const int gi0 = (int)threadIdx.x < npw ? g2fft[threadIdx.x] : 0;
   const T tk0 = (int)threadIdx.x < npw ? t[threadIdx.x] : (T)0;
+  const LineAct act = small_line_activity<N>(G, g2fft, npw);
   const double2* v0 = V + (size_t)b * npw;
   for (int g = threadIdx.x; g < npw; g += SMALL_NT) {
     const double2 x = v0[g];
@@ -479,7 +546,7 @@ __device__ __forceinline__ void cheb_filter_body(int bid, int nblocks, const dou
                           ? coef[((size_t)(threadIdx.x / 3) * nblk + kb) * 3 + threadIdx.x % 3]
                           : 0.0;
   for (int step = 0; step < degree; ++step) {
-    small_h_core<T, N>(G, proj, part, cur, vline, vr, g2fft, gi0, npw, nch, dk);
+    small_h_core<T, N>(G, proj, part, cur, vline, vr, g2fft, gi0, npw, nch, dk, act);
     if (step == 0) {
       if ((int)threadIdx.x < 3 * degree) cfl[threadIdx.x] = cpre;
       lds_barrier();
@@ -583,14 +650,12 @@ __global__ void __launch_bounds__(SMALL_NT) k_density_small(const TIn* __restric
 #pragma unroll
     for (int x = 0; x < N; ++x) acc[li][x] = 0.0;
   const int b0 = blockIdx.x * gsize, b1 = min(nb, b0 + gsize);
+  const LineAct act = small_line_activity<N>(G, g2fft, npw);
   for (int b = b0; b < b1; ++b) {
     const double f = fw[b];
     if (f < 1e-15) continue;  // uniform over the workgroup
     __syncthreads();
-    for (int i = threadIdx.x; i < N * N * N; i += SMALL_NT) {
-      G[i].x = 0;
-      G[i].y = 0;
-    }
+    small_clear<T, N>(G, act);
     __syncthreads();
     const TIn* p = psi + (size_t)b * npw;
     for (int g = threadIdx.x; g < npw; g += SMALL_NT) {  // FP64 input is rounded to T here
@@ -601,9 +666,9 @@ __global__ void __launch_bounds__(SMALL_NT) k_density_small(const TIn* __restric
       G[g2fft[g]] = w;
     }
     __syncthreads();
-    lds_lines<T, N, 2, 1>(G, nullptr);
+    small_zpass<T, N, 1>(G, act);
     __syncthreads();
-    lds_lines<T, N, 1, 1>(G, nullptr);
+    small_ypass<T, N, 1>(G, act);
     __syncthreads();
 #pragma unroll
     for (int li = 0; li < LPT; ++li) {

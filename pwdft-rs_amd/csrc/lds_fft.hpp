@@ -98,6 +98,42 @@ __device__ __forceinline__ void lds_lines(typename V2s<T>::t* G, const T* __rest
   }
 }
 
+// One pass of lds_lines on a grid with one line per thread (N² <= SMALL_NT), pruned to the lines
+// that matter: an inactive line is left alone (ZERO = false: its result is never read) or filled
+// with zeros (ZERO = true: its input is zero, so its transform is zero). Active lines run the
+// transform of lds_lines exactly.
+template <typename T, int N, int AXIS, int SIGN, bool ZERO>
+__device__ __forceinline__ void lds_lines_pruned(typename V2s<T>::t* G, bool active) {
+  static_assert(N * N <= SMALL_NT, "one line per thread");
+  static_assert(AXIS == 1 || AXIS == 2, "y and z passes");
+  constexpr int es = AXIS == 2 ? 1 : N;
+  const int l = threadIdx.x;
+  if (l >= N * N) return;
+  const int base = AXIS == 2 ? l * N : (l / N) * N * N + (l % N);
+  if (!active) {
+    if constexpr (ZERO) {
+#pragma unroll
+      for (int f = 0; f < N; ++f) {
+        G[base + f * es].x = 0;
+        G[base + f * es].y = 0;
+      }
+    }
+    return;
+  }
+  T re[N], im[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    re[j] = G[base + j * es].x;
+    im[j] = G[base + j * es].y;
+  }
+  fft_dif<T, N, SIGN>(re, im);
+#pragma unroll
+  for (int f = 0; f < N; ++f) {
+    G[base + f * es].x = re[FftOrder<N>::pos(f)];
+    G[base + f * es].y = im[FftOrder<N>::pos(f)];
+  }
+}
+
 // The x pass of lds_lines with V(r) of the thread's line already in registers (vline[f] =
 // V at x = f of line threadIdx.x); for grids with at most one x-line per thread.
 template <typename T, int N, int SIGN>
