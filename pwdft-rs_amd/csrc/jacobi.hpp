@@ -10,14 +10,17 @@ namespace pwk {
 
 constexpr int JMAX = 32;
 
-struct JacobiLds {
-  double2 A[JMAX][JMAX + 1];  // matrix (row, column); one padding column against bank conflicts
-  double2 V[JMAX][JMAX + 1];  // accumulated rotations
-  double rc[JMAX / 2], rs[JMAX / 2];
-  double2 rph[JMAX / 2];
-  int rp[JMAX / 2], rq[JMAX / 2], order[JMAX];
+// LDS of the solver for matrices up to JM × JM
+template <int JM>
+struct JacobiLdsT {
+  double2 A[JM][JM + 1];  // matrix (row, column); one padding column against bank conflicts
+  double2 V[JM][JM + 1];  // accumulated rotations
+  double rc[JM / 2], rs[JM / 2];
+  double2 rph[JM / 2];
+  int rp[JM / 2], rq[JM / 2], order[JM];
   double red[32];  // per-wave partial sums (off, total) for up to 16 waves
 };
+using JacobiLds = JacobiLdsT<JMAX>;
 
 // Block-wide sum of (a, b) over all threads, in wave order (same result on every thread).
 __device__ inline void jacobi_sum2(double& a, double& b, double* red) {
@@ -43,7 +46,8 @@ __device__ inline void jacobi_sum2(double& a, double& b, double* red) {
 // Diagonalises J.A (n <= JMAX) in place: on return the diagonal of J.A holds the eigenvalues,
 // J.V = V_in · (product of the rotations), J.order[r] the index of the r-th smallest eigenvalue
 // (ties by index). Sweeps stop when the off-diagonal weight falls below 1e-30 of the total.
-__device__ inline void jacobi_lds(int n, JacobiLds& J) {
+template <int JM>
+__device__ inline void jacobi_lds(int n, JacobiLdsT<JM>& J) {
   const int t = threadIdx.x, nt = blockDim.x;
   const int N = n + (n & 1);
   for (int sweep = 0; sweep < 30; ++sweep) {

@@ -177,13 +177,16 @@ __global__ void __launch_bounds__(256) k_start_block(int npw, int nb, const int*
 constexpr int RR_NT = 256;
 constexpr int RR_ROWS = 32;  // plane-wave rows staged in LDS per chunk of the Gram products
 
+// sized for blocks of up to MB vectors (42 KB at MB = 16, 123 KB at 32), so a workgroup of the
+// common 16-vector blocks fits on a CU beside the filter workgroups of concurrent solves
+template <int MB>
 struct RrLds {
-  JacobiLds J;
-  double2 Hs[JMAX][JMAX + 1];
-  double2 X[JMAX][JMAX + 1];
-  double2 C[JMAX][JMAX + 1];
-  double2 Ys[RR_ROWS][JMAX + 1], HYs[RR_ROWS][JMAX + 1];
-  double red[16][JMAX];
+  JacobiLdsT<MB> J;
+  double2 Hs[MB][MB + 1];
+  double2 X[MB][MB + 1];
+  double2 C[MB][MB + 1];
+  double2 Ys[RR_ROWS][MB + 1], HYs[RR_ROWS][MB + 1];
+  double red[16][MB];
 };
 
 template <int MB>
@@ -194,8 +197,8 @@ __device__ __forceinline__ void rr_small_body(int k, int npw, int m, int nres, c
   static_assert(MB == 16 || MB == 32, "blocks of up to 16 or 32 vectors");
   if (skip && *skip) return;  // a filter pass after the residual gate passed (uniform)
   extern __shared__ unsigned char smem_raw[];
-  RrLds& L = *reinterpret_cast<RrLds*>(smem_raw);
-  JacobiLds& J = L.J;
+  RrLds<MB>& L = *reinterpret_cast<RrLds<MB>*>(smem_raw);
+  JacobiLdsT<MB>& J = L.J;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const double2* Y = Yb + (size_t)k * npw * m;
   const double2* HY = HYb + (size_t)k * npw * m;
@@ -205,7 +208,7 @@ __device__ __forceinline__ void rr_small_body(int k, int npw, int m, int nres, c
   // rows [w·RR_ROWS/4, (w+1)·RR_ROWS/4) of every chunk and each lane a BLK × BLK block of entries,
   // so a row costs 3·BLK LDS reads per lane for 8·BLK² products; the waves' sums are added in wave
   // order at the end.
-  constexpr int BLK = MB / 8, LPT = RR_ROWS * JMAX / RR_NT, WROWS = RR_ROWS / (RR_NT / 64);
+  constexpr int BLK = MB / 8, LPT = RR_ROWS * MB / RR_NT, WROWS = RR_ROWS / (RR_NT / 64);
   const int bi = (lane & 7) * BLK, bj = (lane >> 3) * BLK;
   double sre[BLK][BLK], sim[BLK][BLK], hre[BLK][BLK], him[BLK][BLK];
 #pragma unroll
@@ -555,16 +558,15 @@ void subspace_rr(Context& c, int n, int nb, int nk, const void* H, const void* V
 void launch_rr_small(Context& c, int npw, int m, int nres, int nk, const void* Y, const void* HY, void* V,
                      double* lam, double* res, double floor_rel, const int* skip) {
   PWK_REQUIRE(rr_small_supported(m) && nres <= m && npw >= m && nk >= 1, PWK_E_INVALID, "bad Rayleigh-Ritz shape");
-  const size_t sh = sizeof(RrLds);
-  auto go = [&](auto kern) {
+  auto go = [&](auto kern, size_t sh) {
     ensure_lds_limit((const void*)kern, sh);
     kern<<<(unsigned)nk, RR_NT, sh, c.stream>>>(npw, m, nres, (const double2*)Y, (const double2*)HY, (double2*)V,
                                                lam, res, floor_rel, skip);
   };
   if (m <= 16)
-    go(k_rr_small<16>);
+    go(k_rr_small<16>, sizeof(RrLds<16>));
   else
-    go(k_rr_small<32>);
+    go(k_rr_small<32>, sizeof(RrLds<32>));
   PWK_HIP(hipGetLastError());
 }
 
@@ -572,15 +574,14 @@ void launch_rr_multi(hipStream_t stream, const PassShape& sh, const PassRrMulti&
                      double floor_rel) {
   PWK_REQUIRE(n >= 1 && n <= MAX_PASS_BATCH && rr_small_supported(sh.nbs) && sh.nb <= sh.nbs && sh.npw >= sh.nbs,
               PWK_E_INVALID, "bad batched Rayleigh-Ritz shape");
-  const size_t lds = sizeof(RrLds);
-  auto go = [&](auto kern) {
+  auto go = [&](auto kern, size_t lds) {
     ensure_lds_limit((const void*)kern, lds);
     kern<<<dim3((unsigned)max_nk, (unsigned)n), RR_NT, lds, stream>>>(m, sh.npw, sh.nbs, sh.nb, floor_rel);
   };
   if (sh.nbs <= 16)
-    go(k_rr_multi<16>);
+    go(k_rr_multi<16>, sizeof(RrLds<16>));
   else
-    go(k_rr_multi<32>);
+    go(k_rr_multi<32>, sizeof(RrLds<32>));
   PWK_HIP(hipGetLastError());
 }
 
