@@ -330,55 +330,74 @@ __global__ void k_hist_store(size_t n, const double* __restrict__ in, const doub
 }
 
 // ρ_sym(G) = (1/N_ops) Σ_S e^{-2πi n·τ_S} ρ(R_S^T n)  (src/symmetry/density/g_space.rs:147-252)
-// Sixteen lanes per grid point evaluate the operations (lane l takes l, l + 16, ...); the
-// terms are then added in operation order, as one thread adding them in sequence would
-// (nops <= 64).
+// The source point and the phase of every (point, operation) pair depend only on the grid and
+// the operations, so they are tabulated once per solve (k_sym_table, [point][operation]); the
+// gather of an iteration reads them (k_sym_gather). Sixteen lanes per grid point evaluate the
+// operations (lane l takes l, l + 16, ...) into LDS; one lane then adds the terms in operation
+// order, as one thread adding them in sequence would (nops <= 64).
 constexpr int SYM_MAX_OPS = 64;
 
-__global__ void __launch_bounds__(256) k_sym_gather(size_t n, int nx, int ny, int nz, int nops,
-                                                    const int* __restrict__ rot, const double* __restrict__ tau,
-                                                    const double2* __restrict__ in, double2* __restrict__ out,
-                                                    double inv) {
-  const size_t idx = (size_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-  const int sub = threadIdx.x & 15, base = threadIdx.x & 48;  // lane of the group's first thread, in its wave
-  if (idx >= n) return;  // the whole group of sixteen lanes
+__global__ void __launch_bounds__(256) k_sym_table(size_t n, int nx, int ny, int nz, int nops,
+                                                   const int* __restrict__ rot, const double* __restrict__ tau,
+                                                   int* __restrict__ src_out, double2* __restrict__ ph_out) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * (size_t)nops) return;
+  const size_t idx = e / (size_t)nops;
+  const int o = (int)(e % (size_t)nops);
   const int i1 = (int)(idx / ((size_t)ny * nz)), i2 = (int)((idx / nz) % ny), i3 = (int)(idx % nz);
   const int nd[3] = {i1 > nx / 2 ? i1 - nx : i1, i2 > ny / 2 ? i2 - ny : i2, i3 > nz / 2 ? i3 - nz : i3};
   const int dd[3] = {nx, ny, nz};
+  const int* R = rot + 9 * o;
+  int src[3];
+  for (int c = 0; c < 3; ++c) {
+    const int ns = R[c] * nd[0] + R[3 + c] * nd[1] + R[6 + c] * nd[2];
+    src[c] = ((ns % dd[c]) + dd[c]) % dd[c];
+  }
+  src_out[e] = (src[0] * ny + src[1]) * nz + src[2];
+  const double arg = 6.283185307179586 * ((double)nd[0] * tau[3 * o] + (double)nd[1] * tau[3 * o + 1] +
+                                          (double)nd[2] * tau[3 * o + 2]);
+  double sn, cs;
+  sincos(arg, &sn, &cs);
+  ph_out[e] = make_double2(cs, sn);
+}
+
+__global__ void __launch_bounds__(256) k_sym_gather(size_t n, int nops, const int* __restrict__ src,
+                                                    const double2* __restrict__ ph,
+                                                    const double2* __restrict__ in, double2* __restrict__ out,
+                                                    double inv) {
+  __shared__ double2 term[16][SYM_MAX_OPS];
+  const int pt = threadIdx.x >> 4, sub = threadIdx.x & 15;
+  const size_t idx = (size_t)blockIdx.x * 16 + pt;
   constexpr int S = SYM_MAX_OPS / 16;
-  double tr[S], ti[S];
+  if (idx < n) {
+    int sx[S];
+    double2 p[S], v[S];
 #pragma unroll
-  for (int q = 0; q < S; ++q) {
-    tr[q] = ti[q] = 0.0;
-    const int o = sub + 16 * q;
-    if (o < nops) {
-      const int* R = rot + 9 * o;
-      int src[3];
-      for (int c = 0; c < 3; ++c) {
-        const int ns = R[c] * nd[0] + R[3 + c] * nd[1] + R[6 + c] * nd[2];
-        src[c] = ((ns % dd[c]) + dd[c]) % dd[c];
-      }
-      const double2 v = in[((size_t)src[0] * ny + src[1]) * nz + src[2]];
-      const double arg = 6.283185307179586 * ((double)nd[0] * tau[3 * o] + (double)nd[1] * tau[3 * o + 1] +
-                                              (double)nd[2] * tau[3 * o + 2]);
-      double sn, cs;
-      sincos(arg, &sn, &cs);
+    for (int q = 0; q < S; ++q) {
+      const int o = sub + 16 * q;
+      sx[q] = o < nops ? src[idx * nops + o] : 0;
+      p[q] = o < nops ? ph[idx * nops + o] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < S; ++q) v[q] = in[sx[q]];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      const int o = sub + 16 * q;
       // (cos, -sin) · v
-      tr[q] = cs * v.x + sn * v.y;
-      ti[q] = cs * v.y - sn * v.x;
+      if (o < nops)
+        term[pt][o] = make_double2(p[q].x * v[q].x + p[q].y * v[q].y, p[q].x * v[q].y - p[q].y * v[q].x);
     }
   }
-  double ar = 0.0, ai = 0.0;
-#pragma unroll
-  for (int q = 0; q < S; ++q)
-    for (int l = 0; l < 16; ++l) {
-      const double xr = __shfl(tr[q], base + l, 64), xi = __shfl(ti[q], base + l, 64);
-      if (16 * q + l < nops) {
-        ar += xr;
-        ai += xi;
-      }
+  __syncthreads();
+  if (idx < n && sub == 0) {
+    double ar = 0.0, ai = 0.0;
+    for (int o = 0; o < nops; ++o) {
+      const double2 t = term[pt][o];
+      ar += t.x;
+      ai += t.y;
     }
-  if (sub == 0) out[idx] = make_double2(ar * inv, ai * inv);
+    out[idx] = make_double2(ar * inv, ai * inv);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1019,7 +1038,7 @@ struct Solve {
   // (nspin = 2), xc_tot the clamped ρ_total + ρ_core
   DevBuf vloc_g, vloc_r, core_r, rho_in, rho_g_in, rho_new, rho_g_new, rho_xc, exc, vxc, vxc_g, veff_g, cwork;
   DevBuf t_in, m_in, t_out, m_out, xc_tot;
-  DevBuf Hb, W, V, V32, sym_rot, sym_tau, nl_p, nl_x, expect;
+  DevBuf Hb, W, V, V32, sym_rot, sym_tau, sym_src, sym_ph, nl_p, nl_x, expect;
   PinnedBuf pin_small, pin_partial, pin_rr;
   double* host_small = nullptr;    // pinned: reduction results
   double* host_partial = nullptr;  // pinned: per-block partial sums
@@ -1507,10 +1526,19 @@ void Solve::setup() {
     rot.insert(rot.end(), o.rot, o.rot + 9);
     tr.insert(tr.end(), o.tau, o.tau + 3);
   }
-  sym_rot.ensure(rot.size() * 4);
-  sym_tau.ensure(tr.size() * 8);
-  PWK_HIP(hipMemcpyAsync(sym_rot.p, rot.data(), rot.size() * 4, hipMemcpyHostToDevice, c.stream));
-  PWK_HIP(hipMemcpyAsync(sym_tau.p, tr.data(), tr.size() * 8, hipMemcpyHostToDevice, c.stream));
+  if (ops.size() > 1) {
+    sym_rot.ensure(rot.size() * 4);
+    sym_tau.ensure(tr.size() * 8);
+    PWK_HIP(hipMemcpyAsync(sym_rot.p, rot.data(), rot.size() * 4, hipMemcpyHostToDevice, c.stream));
+    PWK_HIP(hipMemcpyAsync(sym_tau.p, tr.data(), tr.size() * 8, hipMemcpyHostToDevice, c.stream));
+    const size_t ne = N * ops.size();
+    sym_src.ensure(ne * 4);
+    sym_ph.ensure(ne * 16);
+    k_sym_table<<<blocks_for(ne, 256), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
+                                                          sym_rot.as<int>(), sym_tau.as<double>(), sym_src.as<int>(),
+                                                          sym_ph.as<double2>());
+    PWK_HIP(hipGetLastError());
+  }
   // eigen-system buffers; the filtered solver carries buffer bands above the nb used for
   // occupations so that the filter's damped interval starts well above them
   nbs = chebyshev ? std::min(npw, nb + std::max(8, nb / 4)) : nb;
@@ -2051,9 +2079,9 @@ void Solve::symmetrize(double* rn, double scale) {
   if (ops.size() <= 1) return;
   TimedScope ts(c, "scf_symmetrize");
   launch_fft3d_r2c(c, rn, scale, cwork.p, -1);
-  k_sym_gather<<<blocks_for(N, 16), 256, 0, c.stream>>>(N, c.dims[0], c.dims[1], c.dims[2], (int)ops.size(),
-                                                       sym_rot.as<int>(), sym_tau.as<double>(), cwork.as<double2>(),
-                                                       rho_g_new.as<double2>(), 1.0 / (double)ops.size());
+  k_sym_gather<<<blocks_for(N, 16), 256, 0, c.stream>>>(N, (int)ops.size(), sym_src.as<int>(), sym_ph.as<double2>(),
+                                                       cwork.as<double2>(), rho_g_new.as<double2>(),
+                                                       1.0 / (double)ops.size());
   if (grid_small_supported(c)) {
     launch_fft_c2r_small(c, rho_g_new.as<double2>(), 1.0 / (double)N, rn);
     return;
