@@ -634,7 +634,16 @@ __global__ void __launch_bounds__(SMALL_NT) k_hpsi_multi(PassSmallMulti m, int n
 // inverse transform of each in LDS and keeps Σ_b fw_b |ψ_b(r)|² of its x-lines in FP64
 // registers (bands with fw < 1e-15 skipped, src/scf/density.rs:14,72); the partial sums of
 // the groups are added in group order by k_density_reduce, so the result does not depend on
-// scheduling.
+// scheduling. A group whose bands are all skipped stores nothing: its partial is zero and the
+// reduction passes over it (adding +0 to a sum that starts at +0 and adds no negative term
+// changes nothing).
+__device__ __forceinline__ bool density_group_empty(const double* __restrict__ fw, int g, int gsize, int nb) {
+  const int b0 = g * gsize, b1 = min(nb, b0 + gsize);
+  for (int b = b0; b < b1; ++b)
+    if (!(fw[b] < 1e-15)) return false;
+  return true;
+}
+
 template <typename T, int N, typename TIn>
 __global__ void __launch_bounds__(SMALL_NT) k_density_small(const TIn* __restrict__ psi,
                                                             const int* __restrict__ g2fft, int npw, int nb, int gsize,
@@ -650,6 +659,7 @@ __global__ void __launch_bounds__(SMALL_NT) k_density_small(const TIn* __restric
 #pragma unroll
     for (int x = 0; x < N; ++x) acc[li][x] = 0.0;
   const int b0 = blockIdx.x * gsize, b1 = min(nb, b0 + gsize);
+  if (density_group_empty(fw, (int)blockIdx.x, gsize, nb)) return;  // uniform
   const LineAct act = small_line_activity<N>(G, g2fft, npw);
   for (int b = b0; b < b1; ++b) {
     const double f = fw[b];
@@ -700,20 +710,37 @@ __global__ void __launch_bounds__(SMALL_NT) k_density_small(const TIn* __restric
   }
 }
 
-__global__ void __launch_bounds__(256) k_density_reduce(size_t n, int ngroups, const double* __restrict__ partial,
-                                                        double* __restrict__ rho, int overwrite) {
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+constexpr int DRED_NT = 128;  // threads of a density reduction block; it takes up to 128 groups
+__global__ void __launch_bounds__(DRED_NT) k_density_reduce(size_t n, int ngroups, int gsize, int nb,
+                                                            const double* __restrict__ fw,
+                                                            const double* __restrict__ partial,
+                                                            double* __restrict__ rho, int overwrite) {
+  // the groups with a weighted band, in group order: thread t judges group t and the ballots of the
+  // two waves give each live group its slot; the slots past the last live group name group 0, so
+  // the loads below need no guard
+  __shared__ int live[DRED_NT];
+  __shared__ int wcount[DRED_NT / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const bool on = t < ngroups && !density_group_empty(fw, t, gsize, nb);
+  const unsigned long long bal = __ballot(on);
+  if (lane == 0) wcount[w] = __popcll(bal);
+  __syncthreads();
+  const int m = wcount[0] + wcount[1];
+  if (on) live[(w ? wcount[0] : 0) + __popcll(bal & ((1ull << lane) - 1ull))] = t;
+  if (t >= m) live[t] = 0;
+  __syncthreads();
+  const size_t i = (size_t)blockIdx.x * DRED_NT + threadIdx.x;
   if (i >= n) return;
   double s = overwrite ? 0.0 : rho[i];
-  // sixteen partials in flight at a time, added in group order
-  constexpr int U = 16;
-  for (int g0 = 0; g0 < ngroups; g0 += U) {
+  // every live partial of the point is loaded before they are added, in group order
+  constexpr int U = 32;
+  for (int g0 = 0; g0 < m; g0 += U) {
     double v[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) v[j] = g0 + j < ngroups ? partial[(size_t)(g0 + j) * n + i] : 0.0;
+    for (int j = 0; j < U; ++j) v[j] = partial[(size_t)live[min(g0 + j, DRED_NT - 1)] * n + i];
 #pragma unroll
     for (int j = 0; j < U; ++j)
-      if (g0 + j < ngroups) s += v[j];
+      if (g0 + j < m) s += v[j];
   }
   rho[i] = s;
 }
@@ -801,7 +828,7 @@ template <typename T, typename TIn>
 void launch_density_small_t(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, bool overwrite) {
   const int n = c.dims[0];
   const size_t sh = small_lds_bytes<T>(n, 0);
-  const int gsize = std::max(1, (nb + 127) / 128);  // at most 128 groups
+  const int gsize = std::max(1, (nb + 127) / 128);  // at most 128 groups, as the reduction takes
   const int ng = (nb + gsize - 1) / gsize;
   c.work_a.ensure((size_t)ng * c.N * sizeof(double));
   TimedScope ts(c, "density_small");
@@ -819,8 +846,9 @@ void launch_density_small_t(Context& c, int nb, const void* psi, const double* f
       throw Error(PWK_E_UNSUPPORTED, "grid has no fused small-grid kernel");
   }
   PWK_HIP(hipGetLastError());
-  k_density_reduce<<<blocks_for(c.N, 256), 256, 0, c.stream>>>(c.N, ng, c.work_a.as<double>(), (double*)rho,
-                                                               overwrite ? 1 : 0);
+  k_density_reduce<<<blocks_for(c.N, DRED_NT), DRED_NT, 0, c.stream>>>(c.N, ng, gsize, nb, fw_dev,
+                                                                       c.work_a.as<double>(), (double*)rho,
+                                                                       overwrite ? 1 : 0);
   PWK_HIP(hipGetLastError());
 }
 
