@@ -377,9 +377,10 @@ void launch_rho_xc_veff_small(Context& c, const double* rho_r, const double* cor
                               double* rho_xc, double* exc, double* vxc, const double2* vloc_g, double fourpi_e2,
                               double2* veff_g, int slot);
 void launch_fft_c2r_small(Context& c, const double2* in, double scale, double* out);
-// ρ_xc = max(ρ + cs·ρ_core, 0) (ρ when core is null) into rho_xc, then LDA of it (FP64, N points)
+// ρ_xc = max(ρ + cs·ρ_core, 0) (ρ when core is null) into rho_xc, then LDA of it (FP64, N points);
+// with g, also g ← g·gs (N complex entries) in the same pass
 void launch_lda_core(Context& c, const double* rho, const double* core, double cs, double* rho_xc, double* exc,
-                     double* vxc);
+                     double* vxc, double2* g = nullptr, double gs = 1.0);
 void launch_lsda(Context& c, const void* ru, const void* rd, void* exc, void* vu, void* vd, int precision);
 void launch_convert(Context& c, const void* src, void* dst, size_t n, bool to_fp32);  // double<->float arrays
 // autotune.hip: the full-grid batched 3D FFT with the context's plan (tuned on first use of a

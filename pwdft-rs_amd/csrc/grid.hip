@@ -148,12 +148,17 @@ __global__ void __launch_bounds__(256) k_lda(size_t n, const T* __restrict__ rho
 
 // ρ_xc = max(ρ + cs·ρ_core, 0) (ρ without a core, src/scf/energy.rs:314-324) written out, and
 // LDA of it, in one pass (FP64)
+// g (optional): g[i] ← g[i]·gs alongside, for a ρ(G) the caller already has in another scale
 __global__ void __launch_bounds__(256) k_lda_core(size_t n, const double* __restrict__ rho,
                                                   const double* __restrict__ core, double cs,
                                                   double* __restrict__ rho_xc, double* __restrict__ exc,
-                                                  double* __restrict__ vxc) {
+                                                  double* __restrict__ vxc, double2* __restrict__ g, double gs) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (g) {
+    const double2 x = g[i];
+    g[i] = make_double2(x.x * gs, x.y * gs);
+  }
   const double r = core ? fmax(rho[i] + core[i] * cs, 0.0) : rho[i];
   rho_xc[i] = r;
   double e, v;
@@ -445,9 +450,9 @@ void launch_fft_c2r_small(Context& c, const double2* in, double scale, double* o
 }
 
 void launch_lda_core(Context& c, const double* rho, const double* core, double cs, double* rho_xc, double* exc,
-                     double* vxc) {
+                     double* vxc, double2* g, double gs) {
   c.check_grid();
-  k_lda_core<<<blocks_for(c.N, 256), 256, 0, c.stream>>>(c.N, rho, core, cs, rho_xc, exc, vxc);
+  k_lda_core<<<blocks_for(c.N, 256), 256, 0, c.stream>>>(c.N, rho, core, cs, rho_xc, exc, vxc, g, gs);
   PWK_HIP(hipGetLastError());
 }
 

@@ -1169,6 +1169,9 @@ struct Solve {
   std::vector<DotTerm> output_stage();
   void symmetrize(double* rn, double scale = 1.0);
   bool veff_r_ready = false;  // veff_stage wrote V(r) of channel 0 into slot 0 already
+  // symmetrize left N·ρ_sym(G) of the single channel's output density in rho_g_new (the
+  // transform output_stage would otherwise redo from ρ(r))
+  bool g_from_sym = false;
   // V(r) of channel s into V_eff slot `slot` for H·ψ (from V_eff(G) of the channel)
   void upload_veff(int s, int slot, bool need_g) {
     if (veff_r_ready && s == 0 && slot == 0 && !need_g) return;
@@ -2082,12 +2085,15 @@ void Solve::symmetrize(double* rn, double scale) {
   k_sym_gather<<<blocks_for(N, 16), 256, 0, c.stream>>>(N, (int)ops.size(), sym_src.as<int>(), sym_ph.as<double2>(),
                                                        cwork.as<double2>(), rho_g_new.as<double2>(),
                                                        1.0 / (double)ops.size());
+  g_from_sym = ns == 1;
   if (grid_small_supported(c)) {
     launch_fft_c2r_small(c, rho_g_new.as<double2>(), 1.0 / (double)N, rn);
     return;
   }
-  launch_fft3d(c, rho_g_new.p, 1, +1, PWK_FP64);
-  k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, rho_g_new.as<double2>(), 1.0 / (double)N, rn, 0);
+  // the transform runs on a copy: rho_g_new keeps the symmetrised ρ(G)
+  PWK_HIP(hipMemcpyAsync(cwork.p, rho_g_new.p, N * 16, hipMemcpyDeviceToDevice, c.stream));
+  launch_fft3d(c, cwork.p, 1, +1, PWK_FP64);
+  k_c2r_scaled<<<gb(N), 256, 0, c.stream>>>(N, cwork.as<double2>(), 1.0 / (double)N, rn, 0);
   PWK_HIP(hipGetLastError());
 }
 
@@ -2098,13 +2104,19 @@ std::vector<DotTerm> Solve::output_stage() {
   const double* core = have_core ? core_r.as<double>() : nullptr;
   if (ns == 1) {
     rho_tot_new = rho_new.as<double>();
-    if (grid_small_supported(c)) {
+    if (g_from_sym) {
+      // ρ(G) of the output density is the symmetrised one already in rho_g_new (times N): the
+      // forward transform of ρ(r) would only return it again
+      launch_lda_core(c, rho_tot_new, core, 1.0, rho_xc.as<double>(), exc.as<double>(), vxc.as<double>(),
+                      rho_g_new.as<double2>(), 1.0 / (double)N);
+    } else if (grid_small_supported(c)) {
       launch_rho_xc_veff_small(c, rho_tot_new, core, 1.0, rho_g_new.as<double2>(), rho_xc.as<double>(),
                                exc.as<double>(), vxc.as<double>(), nullptr, 0.0, nullptr, 0);
     } else {
       r_to_g(rho_tot_new, rho_g_new.as<double2>());
       launch_lda_core(c, rho_tot_new, core, 1.0, rho_xc.as<double>(), exc.as<double>(), vxc.as<double>());
     }
+    g_from_sym = false;
     return {DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
             DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
             DotTerm{rho_new.as<double>(), nullptr, vxc.as<double>(), nullptr, 0},
