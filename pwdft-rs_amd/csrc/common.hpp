@@ -373,9 +373,10 @@ void launch_lda(Context& c, const void* rho, void* exc, void* vxc, int precision
 // with V(r) into V_eff slot `slot` (what pwk_set_veff's transform and reorder write); and
 // out = Re FFT₊(in)·scale. Same arithmetic as the separate launches.
 bool grid_small_supported(const Context& c);
+// lda_ready: rho_xc, exc and vxc already hold the LDA of rho_r (+ core); it is not evaluated again
 void launch_rho_xc_veff_small(Context& c, const double* rho_r, const double* core, double cs, double2* rho_g,
                               double* rho_xc, double* exc, double* vxc, const double2* vloc_g, double fourpi_e2,
-                              double2* veff_g, int slot);
+                              double2* veff_g, int slot, bool lda_ready = false);
 void launch_fft_c2r_small(Context& c, const double2* in, double scale, double* out);
 // ρ_xc = max(ρ + cs·ρ_core, 0) (ρ when core is null) into rho_xc, then LDA of it (FP64, N points);
 // with g, also g ← g·gs (N complex entries) in the same pass

@@ -4,6 +4,7 @@
 // in LDS (the transforms of k_fft3d_small, the pointwise code below).
 #include "common.hpp"
 #include "lds_fft.hpp"
+#include "xc.hpp"
 
 namespace pwk {
 
@@ -75,65 +76,8 @@ __global__ void __launch_bounds__(256) k_sum3(size_t n, const typename Cx<T>::t*
   out[i] = r;
 }
 
-// ---- PZ-LDA (src/potential/xc.rs:129-141, 223-288) ----------------------------
-__device__ __forceinline__ void lda_point(double rho, double& exc, double& vxc) {
-  if (rho < kRhoFloor) {
-    exc = 0.0;
-    vxc = 0.0;
-    return;
-  }
-  double rb = rho * kBohr3;
-  double ex_ha = -0.75 * cbrt(3.0 * rb / kPiD);
-  double vx_ha = (4.0 / 3.0) * ex_ha;
-  double rs = cbrt(3.0 / (4.0 * kPiD * rb));
-  double ec, vc;
-  if (rs >= 1.0) {
-    const double gamma = -0.1423, b1 = 1.0529, b2 = 0.3334;
-    double srs = sqrt(rs);
-    double den = 1.0 + b1 * srs + b2 * rs;
-    ec = gamma / den;
-    double dec = -gamma * (b1 / (2.0 * srs) + b2) / (den * den);
-    vc = fma(rs / 3.0, -dec, ec);
-  } else {
-    const double a = 0.0311, b = -0.048, c = 0.0020, d = -0.0116;
-    double lr = log(rs);
-    ec = fma(d, rs, fma(c * rs, lr, fma(a, lr, b)));
-    double dec = fma(c, lr + 1.0, a / rs) + d;
-    vc = fma(rs / 3.0, -dec, ec);
-  }
-  exc = ex_ha * kHaToEv + ec * kHaToEv;
-  vxc = vx_ha * kHaToEv + vc * kHaToEv;
-}
-
-__device__ __forceinline__ void lda_point(float rho, float& exc, float& vxc) {
-  if (rho < 1e-30f) {
-    exc = 0.0f;
-    vxc = 0.0f;
-    return;
-  }
-  const float pi = 3.14159265358979f;
-  float rb = rho * (float)kBohr3;
-  float ex_ha = -0.75f * cbrtf(3.0f * rb / pi);
-  float vx_ha = (4.0f / 3.0f) * ex_ha;
-  float rs = cbrtf(3.0f / (4.0f * pi * rb));
-  float ec, vc;
-  if (rs >= 1.0f) {
-    const float gamma = -0.1423f, b1 = 1.0529f, b2 = 0.3334f;
-    float srs = sqrtf(rs);
-    float den = 1.0f + b1 * srs + b2 * rs;
-    ec = gamma / den;
-    float dec = -gamma * (b1 / (2.0f * srs) + b2) / (den * den);
-    vc = ec - rs / 3.0f * dec;
-  } else {
-    const float a = 0.0311f, b = -0.048f, c = 0.0020f, d = -0.0116f;
-    float lr = logf(rs);
-    ec = a * lr + b + c * rs * lr + d * rs;
-    float dec = a / rs + c * (lr + 1.0f) + d;
-    vc = ec - rs / 3.0f * dec;
-  }
-  exc = (ex_ha + ec) * (float)kHaToEv;
-  vxc = (vx_ha + vc) * (float)kHaToEv;
-}
+// ---- PZ-LDA (src/potential/xc.rs:129-141, 223-288): xc.hpp ----------------------
+using xc::lda_point;
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_lda(size_t n, const T* __restrict__ rho, T* __restrict__ exc,
@@ -267,7 +211,7 @@ __global__ void __launch_bounds__(SMALL_NT) k_rho_xc_veff_small(
     double2* __restrict__ rho_g, double* __restrict__ rho_xc, double* __restrict__ exc, double* __restrict__ vxc,
     const double2* __restrict__ vloc_g, const double* __restrict__ g2, double fourpi_e2,
     double2* __restrict__ veff_g, double* __restrict__ v64, float* __restrict__ v32, double* __restrict__ n64,
-    float* __restrict__ n32) {
+    float* __restrict__ n32, int lda_ready) {
   extern __shared__ unsigned char smem_raw[];
   double2* G = reinterpret_cast<double2*>(smem_raw);
   constexpr int NN = N * N * N;
@@ -279,16 +223,22 @@ __global__ void __launch_bounds__(SMALL_NT) k_rho_xc_veff_small(
     double2 g;
     double r, c;
   };
+  // lda_ready: rho_xc, exc and vxc already hold the LDA of this density (the mixing step that
+  // produced it evaluated them); r then carries the given v_xc
   grid_points<NN, RhoIn>(
       [&](int i) {
         RhoIn x;
         x.g = G[i];
-        x.r = rho_r[i];
-        x.c = core ? core[i] : 0.0;
+        x.r = lda_ready ? vxc[i] : rho_r[i];
+        x.c = (core && !lda_ready) ? core[i] : 0.0;
         return x;
       },
       [&](int i, const RhoIn& x) {
         rho_g[i] = x.g;
+        if (lda_ready) {
+          if constexpr (VEFF) G[i] = make_double2(x.r * inv_n, 0.0);
+          return;
+        }
         const double r = core ? fmax(x.r + x.c * cs, 0.0) : x.r;
         rho_xc[i] = r;
         double e, v;
@@ -397,7 +347,7 @@ bool grid_small_supported(const Context& c) {
 
 void launch_rho_xc_veff_small(Context& c, const double* rho_r, const double* core, double cs, double2* rho_g,
                               double* rho_xc, double* exc, double* vxc, const double2* vloc_g, double fourpi_e2,
-                              double2* veff_g, int slot) {
+                              double2* veff_g, int slot, bool lda_ready) {
   PWK_REQUIRE(grid_small_supported(c), PWK_E_UNSUPPORTED, "grid has no fused single-workgroup kernels");
   const bool veff = veff_g != nullptr;
   const size_t o = (size_t)slot * c.N;
@@ -415,12 +365,14 @@ void launch_rho_xc_veff_small(Context& c, const double* rho_r, const double* cor
       ensure_lds_limit((const void*)k_rho_xc_veff_small<L, true>, sh);                                        \
       k_rho_xc_veff_small<L, true><<<1, SMALL_NT, sh, c.stream>>>(                                            \
           rho_r, core, cs, inv_n, rho_g, rho_xc, exc, vxc, vloc_g, c.g2_64.as<double>(), fourpi_e2, veff_g,     \
-          c.vr64.as<double>() + o, c.vr32.as<float>() + o, c.vrn64.as<double>() + o, c.vrn32.as<float>() + o); \
+          c.vr64.as<double>() + o, c.vr32.as<float>() + o, c.vrn64.as<double>() + o, c.vrn32.as<float>() + o,   \
+          lda_ready ? 1 : 0);                                                                                  \
     } else {                                                                                                   \
       ensure_lds_limit((const void*)k_rho_xc_veff_small<L, false>, sh);                                       \
       k_rho_xc_veff_small<L, false><<<1, SMALL_NT, sh, c.stream>>>(rho_r, core, cs, inv_n, rho_g, rho_xc, exc, \
                                                                    vxc, nullptr, nullptr, 0.0, nullptr,         \
-                                                                   nullptr, nullptr, nullptr, nullptr);         \
+                                                                   nullptr, nullptr, nullptr, nullptr,          \
+                                                                   lda_ready ? 1 : 0);                          \
     }                                                                                                          \
     break;
     PWK_GRID_SMALL_SIZES(PWK_CASE)

@@ -22,6 +22,7 @@
 #include <unordered_map>
 
 #include "common.hpp"
+#include "xc.hpp"
 #include "tables.hpp"
 
 namespace pwk {
@@ -470,13 +471,26 @@ struct MixArgs {
   const double* res[MAX_HIST];
 };
 
-__global__ void k_anderson_combine(size_t n, MixArgs a, double* __restrict__ out) {
+// With rho_xc non-null, also the LDA of the result (ρ_xc = max(ρ + cs·ρ_core, 0), ε_xc, v_xc):
+// the next iteration's V_eff needs exactly these, and evaluating them here spreads the
+// transcendental work over the whole grid launch instead of the one workgroup of the V_eff kernel.
+__global__ void k_anderson_combine(size_t n, MixArgs a, double* __restrict__ out, const double* __restrict__ core,
+                                   double cs, double* __restrict__ rho_xc, double* __restrict__ exc,
+                                   double* __restrict__ vxc) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int last = a.m - 1;
   double v = a.coef[last] * (a.in[last][i] + a.beta * a.res[last][i]);
   for (int j = 0; j < last; ++j) v += a.coef[j] * (a.in[j][i] + a.beta * a.res[j][i]);
   out[i] = v;
+  if (rho_xc) {
+    const double r = core ? fmax(v + core[i] * cs, 0.0) : v;
+    rho_xc[i] = r;
+    double e, x;
+    xc::lda_point(r, e, x);
+    exc[i] = e;
+    vxc[i] = x;
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1157,7 +1171,8 @@ struct Solve {
   void eig_chebyshev(bool first);
   void veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out, double* vxc_out);
   void xc_spin(const double* up, const double* dn);
-  void mix(Mixer& m, const double* in, const double* out, double* result);
+  // lda: also the LDA of the result into rho_xc / exc / vxc (one spin channel)
+  void mix(Mixer& m, const double* in, const double* out, double* result, bool lda = false);
   void run(double* eig_out, double* rho_out);
 
   // stages shared by the host-synchronised and the device-resident iteration
@@ -1172,6 +1187,8 @@ struct Solve {
   // symmetrize left N·ρ_sym(G) of the single channel's output density in rho_g_new (the
   // transform output_stage would otherwise redo from ρ(r))
   bool g_from_sym = false;
+  // the mixing step left the LDA of rho_in (single channel) in rho_xc / exc / vxc
+  bool lda_in_ready = false;
   // V(r) of channel s into V_eff slot `slot` for H·ψ (from V_eff(G) of the channel)
   void upload_veff(int s, int slot, bool need_g) {
     if (veff_r_ready && s == 0 && slot == 0 && !need_g) return;
@@ -1680,7 +1697,7 @@ void Solve::xc_spin(const double* up, const double* dn) {
 // preconditioned when enabled, mixing/kerker.rs:5-29) enter the history, the oldest entry
 // leaves past ndim, and result = c_last·(in_l + β r_l) + Σ_j α_j (in_j + β r_j). result may
 // alias in.
-void Solve::mix(Mixer& mx, const double* in, const double* out, double* result) {
+void Solve::mix(Mixer& mx, const double* in, const double* out, double* result, bool lda) {
   TimedScope ts(c, "scf_mix");
   int slot;
   if ((int)mx.hslot.size() < mx.ndim) {
@@ -1734,7 +1751,9 @@ void Solve::mix(Mixer& mx, const double* in, const double* out, double* result) 
     for (int j = 0; j < mm; ++j) ma.coef[j] = alpha[(size_t)j];
     ma.coef[m - 1] = 1.0 - asum;
   }
-  k_anderson_combine<<<gb(N), 256, 0, c.stream>>>(N, ma, result);
+  k_anderson_combine<<<gb(N), 256, 0, c.stream>>>(N, ma, result, have_core ? core_r.as<double>() : nullptr, 1.0,
+                                                  lda ? rho_xc.as<double>() : nullptr, exc.as<double>(),
+                                                  vxc.as<double>());
   PWK_HIP(hipGetLastError());
 }
 
@@ -2041,13 +2060,17 @@ void Solve::eig_chebyshev(bool first) {
 // (driver_spin.rs:634-651).
 std::vector<DotTerm> Solve::veff_stage() {
   const double* core = have_core ? core_r.as<double>() : nullptr;
+  struct Once {  // the mixing step's LDA serves this stage only
+    bool& f;
+    ~Once() { f = false; }
+  } once{lda_in_ready};
   if (ns == 1) {
     veff_r_ready = false;
     if (grid_small_supported(c)) {
       // small cubic grid: ρ(G), xc, V_eff(G) and V(r) (slot 0) in one launch
       launch_rho_xc_veff_small(c, rho_in.as<double>(), core, 1.0, rho_g_in.as<double2>(), rho_xc.as<double>(),
                                exc.as<double>(), vxc.as<double>(), vloc_g.as<double2>(), 4.0 * kPiC * kE2,
-                               veff_g.as<double2>(), 0);
+                               veff_g.as<double2>(), 0, lda_in_ready);
       veff_r_ready = true;
     } else {
       r_to_g(rho_in.as<double>(), rho_g_in.as<double2>());
@@ -2268,6 +2291,8 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
   const int nkb = ns * nk;  // (channel, k) entries, spin-up first
   std::vector<double> wk_all((size_t)nkb);
   for (int e = 0; e < nkb; ++e) wk_all[(size_t)e] = wk[(size_t)(e % nk)];
+  lda_in_ready = false;
+  g_from_sym = false;
   bool have_prev = false;
   double e_prev = 0.0;
   for (int iter = 0; iter < p.max_iter; ++iter) {
@@ -2392,7 +2417,8 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
     // ---- mixing: ρ (nspin = 1), or ρ_total and m with separate histories (CCMX,
     // src/scf/driver_spin.rs:800-832) ---------------------------------------------------------
     if (ns == 1) {
-      mix(mix_tot, rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>());
+      lda_in_ready = grid_small_supported(c);
+      mix(mix_tot, rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>(), lda_in_ready);
     } else {
       mix(mix_tot, t_in.as<double>(), t_out.as<double>(), t_in.as<double>());
       mix(mix_mag, m_in.as<double>(), m_out.as<double>(), m_in.as<double>());
@@ -2556,6 +2582,8 @@ void Solve::resident_tail() {
 
 void Solve::run_resident(double e_ewald, std::vector<double>& ev) {
   const size_t nkm = (size_t)nke * nbs;
+  lda_in_ready = false;
+  g_from_sym = false;
   dS.ensure((size_t)S_SIZE * 8);
   pin_S.ensure((size_t)S_SIZE * 8);
   hS = pin_S.as<double>();
