@@ -439,12 +439,34 @@ __global__ void __launch_bounds__(RED_NT) k_dots(size_t n, DotArgs args, double*
   if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
-// per-band expectation values: e[2b] = Σ_G |ψ|² t_G, e[2b+1] = Re Σ_G conj(ψ)·x
+// Setup uploads of one group in one launch: item blockIdx.y's 32-bit words go from the pinned host
+// staging block (read by the device directly) to its device array.
+constexpr int UNPACK_MAX = 16;
+struct UnpackArgs {
+  const unsigned* src[UNPACK_MAX];
+  unsigned* dst[UNPACK_MAX];
+  unsigned words[UNPACK_MAX];
+};
+__global__ void __launch_bounds__(256) k_unpack(UnpackArgs a) {
+  const int j = blockIdx.y;
+  const unsigned* __restrict__ s = a.src[j];
+  unsigned* __restrict__ d = a.dst[j];
+  const unsigned n = a.words[j];
+  for (unsigned w = blockIdx.x * 256u + threadIdx.x; w < n; w += gridDim.x * 256u) d[w] = s[w];
+}
+
+// per-band expectation values: e[2b] = Σ_G |ψ|² t_G, e[2b+1] = Re Σ_G conj(ψ)·x; blockIdx.y = k
+// steps every operand by its k stride (one launch for all k-points)
 __global__ void __launch_bounds__(256) k_band_expect(int npw, const double2* __restrict__ psi,
                                                     const double* __restrict__ t, const double2* __restrict__ x,
-                                                    double* __restrict__ e) {
+                                                    double* __restrict__ e, size_t psi_k, size_t t_k, size_t x_k,
+                                                    size_t e_k) {
   __shared__ double sh[4];
   const int b = blockIdx.x;
+  psi += blockIdx.y * psi_k;
+  t += blockIdx.y * t_k;
+  x += blockIdx.y * x_k;
+  e += blockIdx.y * e_k;
   const double2* p = psi + (size_t)b * npw;
   const double2* q = x + (size_t)b * npw;
   double kin = 0.0, nl = 0.0;
@@ -1054,6 +1076,49 @@ struct Solve {
   DevBuf t_in, m_in, t_out, m_out, xc_tot;
   DevBuf Hb, W, V, V32, sym_rot, sym_tau, sym_src, sym_ph, nl_p, nl_x, expect;
   PinnedBuf pin_small, pin_partial, pin_rr;
+  // setup uploads: a group of host arrays is copied into the pinned staging block and delivered by
+  // one k_unpack launch (instead of a copy launch per array); a group's region stays untouched until
+  // the solve ends, so the launch may read it at any time
+  PinnedBuf pin_up;
+  size_t up_used = 0;
+  struct UpItem {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  std::vector<UpItem> up_items;
+  void up_add(void* dst, const void* src, size_t bytes) {
+    PWK_REQUIRE(bytes % 4 == 0, PWK_E_INVALID, "setup upload of a partial word");
+    if (bytes) up_items.push_back(UpItem{dst, src, bytes});
+  }
+  void up_flush() {
+    size_t total = 0;
+    for (const UpItem& it : up_items) total += (it.bytes + 255) & ~(size_t)255;
+    if (up_used + total > pin_up.bytes) {
+      c.sync();  // earlier groups' launches have read their regions
+      up_used = 0;
+      pin_up.ensure(std::max(total, (size_t)1 << 20));
+    }
+    for (size_t i0 = 0; i0 < up_items.size(); i0 += UNPACK_MAX) {
+      UnpackArgs a{};
+      const size_t n = std::min(up_items.size() - i0, (size_t)UNPACK_MAX);
+      unsigned most = 0;
+      for (size_t i = 0; i < n; ++i) {
+        const UpItem& it = up_items[i0 + i];
+        unsigned char* stage = (unsigned char*)pin_up.p + up_used;
+        std::memcpy(stage, it.src, it.bytes);
+        up_used += (it.bytes + 255) & ~(size_t)255;
+        a.src[i] = (const unsigned*)stage;
+        a.dst[i] = (unsigned*)it.dst;
+        a.words[i] = (unsigned)(it.bytes / 4);
+        most = std::max(most, a.words[i]);
+      }
+      const unsigned gx = std::min(64u, std::max(1u, (most + 255u) / 256u));
+      k_unpack<<<dim3(gx, (unsigned)n), 256, 0, c.stream>>>(a);
+      PWK_HIP(hipGetLastError());
+    }
+    up_items.clear();
+  }
   double* host_small = nullptr;    // pinned: reduction results
   double* host_partial = nullptr;  // pinned: per-block partial sums
 
@@ -1218,6 +1283,8 @@ struct Solve {
 };
 
 void Solve::setup() {
+  up_used = 0;  // a previous run of this solve has synchronised: its staging regions are free
+  up_items.clear();
   PWK_REQUIRE(cr.natoms >= 1 && cr.frac && cr.species && cr.nspecies >= 1 && cr.nspecies <= MAXSP &&
                   cr.species_data,
               PWK_E_INVALID, "bad crystal description (1..8 species)");
@@ -1381,8 +1448,8 @@ void Solve::setup() {
   dshg2.ensure(shell_g2.size() * 8);
   dshell.ensure(shell_of.size() * 4);
   dtab.ensure(shell_g2.size() * (size_t)cr.nspecies * 3 * 8);
-  PWK_HIP(hipMemcpyAsync(dshg2.p, shell_g2.data(), shell_g2.size() * 8, hipMemcpyHostToDevice, c.stream));
-  PWK_HIP(hipMemcpyAsync(dshell.p, shell_of.data(), shell_of.size() * 4, hipMemcpyHostToDevice, c.stream));
+  up_add(dshg2.p, shell_g2.data(), shell_g2.size() * 8);
+  up_add(dshell.p, shell_of.data(), shell_of.size() * 4);
   ra.shell_g2 = dshg2.as<double>();
   ra.shell = dshell.as<int>();
   ra.tab = dtab.as<double>();
@@ -1392,12 +1459,13 @@ void Solve::setup() {
   dwr.ensure(hr.size() * 8);
   dtau.ensure(tau.size() * 8);
   dsp.ensure(spi.size() * 4);
-  PWK_HIP(hipMemcpyAsync(dr.p, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, c.stream));
-  PWK_HIP(hipMemcpyAsync(dwv.p, hwv.data(), hr.size() * 8, hipMemcpyHostToDevice, c.stream));
-  PWK_HIP(hipMemcpyAsync(dwc.p, hwc.data(), hr.size() * 8, hipMemcpyHostToDevice, c.stream));
-  PWK_HIP(hipMemcpyAsync(dwr.p, hwr.data(), hr.size() * 8, hipMemcpyHostToDevice, c.stream));
-  PWK_HIP(hipMemcpyAsync(dtau.p, tau.data(), tau.size() * 8, hipMemcpyHostToDevice, c.stream));
-  PWK_HIP(hipMemcpyAsync(dsp.p, spi.data(), spi.size() * 4, hipMemcpyHostToDevice, c.stream));
+  up_add(dr.p, hr.data(), hr.size() * 8);
+  up_add(dwv.p, hwv.data(), hr.size() * 8);
+  up_add(dwc.p, hwc.data(), hr.size() * 8);
+  up_add(dwr.p, hwr.data(), hr.size() * 8);
+  up_add(dtau.p, tau.data(), tau.size() * 8);
+  up_add(dsp.p, spi.data(), spi.size() * 4);
+  up_flush();
   vloc_g.ensure(N * 16);
   core_g.ensure(N * 16);
   sad_g.ensure(N * 16);
@@ -1549,8 +1617,9 @@ void Solve::setup() {
   if (ops.size() > 1) {
     sym_rot.ensure(rot.size() * 4);
     sym_tau.ensure(tr.size() * 8);
-    PWK_HIP(hipMemcpyAsync(sym_rot.p, rot.data(), rot.size() * 4, hipMemcpyHostToDevice, c.stream));
-    PWK_HIP(hipMemcpyAsync(sym_tau.p, tr.data(), tr.size() * 8, hipMemcpyHostToDevice, c.stream));
+    up_add(sym_rot.p, rot.data(), rot.size() * 4);
+    up_add(sym_tau.p, tr.data(), tr.size() * 8);
+    up_flush();
     const size_t ne = N * ops.size();
     sym_src.ensure(ne * 4);
     sym_ph.ensure(ne * 16);
@@ -1649,8 +1718,9 @@ double Solve::ewald() {
   dq.ensure(q.size() * 8);
   const int nblk = 64;
   part.ensure(2 * nblk * 8);
-  PWK_HIP(hipMemcpyAsync(dpos.p, pos.data(), pos.size() * 8, hipMemcpyHostToDevice, c.stream));
-  PWK_HIP(hipMemcpyAsync(dq.p, q.data(), q.size() * 8, hipMemcpyHostToDevice, c.stream));
+  up_add(dpos.p, pos.data(), pos.size() * 8);
+  up_add(dq.p, q.data(), q.size() * 8);
+  up_flush();
   a.pos = dpos.as<double>();
   a.q = dq.as<double>();
   k_ewald_recip<<<nblk, 256, 0, c.stream>>>(a, part.as<double>());
@@ -2225,11 +2295,9 @@ void Solve::finalize(const std::vector<double>& occ, const IterScalars& v, doubl
     } else {
       PWK_HIP(hipMemsetAsync(nl_x.p, 0, (size_t)nk * npw * nb * 16, c.stream));
     }
-    for (int k = 0; k < nk; ++k)
-      k_band_expect<<<(unsigned)nb, 256, 0, c.stream>>>(npw, psi0 + (size_t)k * npw * nbs,
-                                                       kin_all64.as<double>() + (size_t)k * npw,
-                                                       nl_x.as<double2>() + (size_t)k * npw * nb,
-                                                       expect.as<double>() + (size_t)2 * k * nb);
+    k_band_expect<<<dim3((unsigned)nb, (unsigned)nk), 256, 0, c.stream>>>(
+        npw, psi0, kin_all64.as<double>(), nl_x.as<double2>(), expect.as<double>(), (size_t)npw * nbs, (size_t)npw,
+        (size_t)npw * nb, (size_t)2 * nb);
     PWK_HIP(hipGetLastError());
     PWK_HIP(hipMemcpyAsync(e2.data(), expect.p, e2.size() * 8, hipMemcpyDeviceToHost, c.stream));
     c.sync();
