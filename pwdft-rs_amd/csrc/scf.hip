@@ -1073,6 +1073,9 @@ struct Solve {
   // t_in, m_in, t_out, m_out are ρ_total and m = ρ↑ − ρ↓ of the input and output density
   // (nspin = 2), xc_tot the clamped ρ_total + ρ_core
   DevBuf vloc_g, vloc_r, core_r, rho_in, rho_g_in, rho_new, rho_g_new, rho_xc, exc, vxc, vxc_g, veff_g, cwork;
+  // LDA of the output density (one spin channel): kept apart from rho_xc / exc / vxc, which hold
+  // the input density's until the iteration's energy sums have read them
+  DevBuf rho_xc_o, exc_o, vxc_o;
   DevBuf t_in, m_in, t_out, m_out, xc_tot;
   DevBuf Hb, W, V, V32, sym_rot, sym_tau, sym_src, sym_ph, nl_p, nl_x, expect;
   PinnedBuf pin_small, pin_partial, pin_rr;
@@ -1488,6 +1491,8 @@ void Solve::setup() {
   for (DevBuf* b : {&vloc_r, &core_r, &exc}) b->ensure(N * 8);
   for (DevBuf* b : {&rho_in, &rho_new, &rho_xc, &vxc}) b->ensure((size_t)ns * N * 8);
   for (DevBuf* b : {&rho_g_in, &rho_g_new, &vxc_g, &cwork}) b->ensure(N * 16);
+  if (ns == 1)
+    for (DevBuf* b : {&rho_xc_o, &exc_o, &vxc_o}) b->ensure(N * 8);
   veff_g.ensure((size_t)ns * N * 16);
   if (ns == 2)
     for (DevBuf* b : {&t_in, &m_in, &t_out, &m_out, &xc_tot}) b->ensure(N * 8);
@@ -2197,22 +2202,22 @@ std::vector<DotTerm> Solve::output_stage() {
   const double* core = have_core ? core_r.as<double>() : nullptr;
   if (ns == 1) {
     rho_tot_new = rho_new.as<double>();
+    double *rx = rho_xc_o.as<double>(), *ex = exc_o.as<double>(), *vx = vxc_o.as<double>();
     if (g_from_sym) {
       // ρ(G) of the output density is the symmetrised one already in rho_g_new (times N): the
       // forward transform of ρ(r) would only return it again
-      launch_lda_core(c, rho_tot_new, core, 1.0, rho_xc.as<double>(), exc.as<double>(), vxc.as<double>(),
-                      rho_g_new.as<double2>(), 1.0 / (double)N);
+      launch_lda_core(c, rho_tot_new, core, 1.0, rx, ex, vx, rho_g_new.as<double2>(), 1.0 / (double)N);
     } else if (grid_small_supported(c)) {
-      launch_rho_xc_veff_small(c, rho_tot_new, core, 1.0, rho_g_new.as<double2>(), rho_xc.as<double>(),
-                               exc.as<double>(), vxc.as<double>(), nullptr, 0.0, nullptr, 0);
+      launch_rho_xc_veff_small(c, rho_tot_new, core, 1.0, rho_g_new.as<double2>(), rx, ex, vx, nullptr, 0.0, nullptr,
+                               0);
     } else {
       r_to_g(rho_tot_new, rho_g_new.as<double2>());
-      launch_lda_core(c, rho_tot_new, core, 1.0, rho_xc.as<double>(), exc.as<double>(), vxc.as<double>());
+      launch_lda_core(c, rho_tot_new, core, 1.0, rx, ex, vx);
     }
     g_from_sym = false;
     return {DotTerm{rho_g_new.as<double>(), nullptr, c.g2_64.as<double>(), nullptr, 1},
-            DotTerm{rho_xc.as<double>(), nullptr, exc.as<double>(), nullptr, 0},
-            DotTerm{rho_new.as<double>(), nullptr, vxc.as<double>(), nullptr, 0},
+            DotTerm{rx, nullptr, ex, nullptr, 0},
+            DotTerm{rho_new.as<double>(), nullptr, vx, nullptr, 0},
             DotTerm{rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>(), rho_new.as<double>(), 0},
             DotTerm{rho_new.as<double>(), nullptr, vloc_r.as<double>(), nullptr, 0},
             DotTerm{rho_new.as<double>(), nullptr, nullptr, nullptr, 0}};
@@ -2365,9 +2370,16 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
   double e_prev = 0.0;
   for (int iter = 0; iter < p.max_iter; ++iter) {
     // ---- V_eff per channel from the input density ---------------------------------------
-    dots(veff_stage(), N);
-    double har[4];
-    std::memcpy(har, host_small, sizeof(har));
+    // one spin channel: the Harris-side sums are taken with the output-side ones at the end of the
+    // iteration (one reduction launch and one host synchronisation fewer); each term is its own
+    // reduction, so the values do not change
+    const std::vector<DotTerm> hterms = veff_stage();
+    const bool defer = ns == 1;
+    double har[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!defer) {
+      dots(hterms, N);
+      std::memcpy(har, host_small, sizeof(har));
+    }
 
     // ---- eigenpairs per channel (both channels in one pass of launches when folded) ----------
     if (fold) {
@@ -2456,8 +2468,18 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
     }
 
     // ---- output-density energies -------------------------------------------------------------
-    dots(output_stage(), N);
-    const IterScalars v = scalars(har, host_small);
+    const double* outs = host_small;
+    if (defer) {
+      std::vector<DotTerm> all = hterms;
+      const std::vector<DotTerm> oterms = output_stage();
+      all.insert(all.end(), oterms.begin(), oterms.end());
+      dots(all, N);
+      std::memcpy(har, host_small, hterms.size() * sizeof(double));
+      outs = host_small + hterms.size();
+    } else {
+      dots(output_stage(), N);
+    }
+    const IterScalars v = scalars(har, outs);
     double e_band = 0.0;
     for (int e = 0; e < nkb; ++e) {
       double sk = 0.0;
