@@ -445,8 +445,11 @@ void launch_lanczos_small(Context& c, int nke, int nkv, int steps, const double*
 // ρ += Σ_b fw_b |ψ_b(r)|² on small cubic grids (fw device-readable: device or pinned host
 // memory); psi_fp64: ψ is FP64 and, for an FP32 density, rounded to FP32 as it is loaded
 bool density_small_supported(const Context& c, int precision);
+// wave_sums (device-writable, density_wave_sums(c) entries): also the 64-lane shuffle-tree sum of the
+// resulting ρ over each wave of 64 consecutive points, in point order (0 past the grid)
 void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision,
-                          bool overwrite = false, bool psi_fp64 = false);
+                          bool overwrite = false, bool psi_fp64 = false, double* wave_sums = nullptr);
+size_t density_wave_sums(const Context& c);
 // dense.hip
 void launch_dense_h(Context& c, int ik, int with_nl, void* h);
 void launch_dense_h_batched(Context& c, int nk, const double* kin, int nch, const void* B, const void* DBH, void* h);

@@ -711,10 +711,14 @@ __global__ void __launch_bounds__(SMALL_NT) k_density_small(const TIn* __restric
 }
 
 constexpr int DRED_NT = 128;  // threads of a density reduction block; it takes up to 128 groups
+// wave_sums (optional): the 64-lane shuffle-tree sum of ρ over each wave's points (0 past n), one
+// value per wave in point order; a 256-point block reduction that adds its four waves' tree sums in
+// wave order is rebuilt from them exactly.
 __global__ void __launch_bounds__(DRED_NT) k_density_reduce(size_t n, int ngroups, int gsize, int nb,
                                                             const double* __restrict__ fw,
                                                             const double* __restrict__ partial,
-                                                            double* __restrict__ rho, int overwrite) {
+                                                            double* __restrict__ rho, int overwrite,
+                                                            double* __restrict__ wave_sums) {
   // the groups with a weighted band, in group order: thread t judges group t and the ballots of the
   // two waves give each live group its slot; the slots past the last live group name group 0, so
   // the loads below need no guard
@@ -730,19 +734,26 @@ __global__ void __launch_bounds__(DRED_NT) k_density_reduce(size_t n, int ngroup
   if (t >= m) live[t] = 0;
   __syncthreads();
   const size_t i = (size_t)blockIdx.x * DRED_NT + threadIdx.x;
-  if (i >= n) return;
-  double s = overwrite ? 0.0 : rho[i];
-  // every live partial of the point is loaded before they are added, in group order
-  constexpr int U = 32;
-  for (int g0 = 0; g0 < m; g0 += U) {
-    double v[U];
+  double s = 0.0;
+  if (i < n) {
+    s = overwrite ? 0.0 : rho[i];
+    // every live partial of the point is loaded before they are added, in group order
+    constexpr int U = 32;
+    for (int g0 = 0; g0 < m; g0 += U) {
+      double v[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) v[j] = partial[(size_t)live[min(g0 + j, DRED_NT - 1)] * n + i];
+      for (int j = 0; j < U; ++j) v[j] = partial[(size_t)live[min(g0 + j, DRED_NT - 1)] * n + i];
 #pragma unroll
-    for (int j = 0; j < U; ++j)
-      if (g0 + j < m) s += v[j];
+      for (int j = 0; j < U; ++j)
+        if (g0 + j < m) s += v[j];
+    }
+    rho[i] = s;
   }
-  rho[i] = s;
+  if (wave_sums) {  // uniform
+    double v = s;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) wave_sums[(size_t)blockIdx.x * (DRED_NT / 64) + w] = v;
+  }
 }
 
 template <typename T>
@@ -825,7 +836,8 @@ void launch_filter_small_t(Context& c, int nbands, int nbk, const void* V, void*
 }
 
 template <typename T, typename TIn>
-void launch_density_small_t(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, bool overwrite) {
+void launch_density_small_t(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, bool overwrite,
+                            double* wave_sums) {
   const int n = c.dims[0];
   const size_t sh = small_lds_bytes<T>(n, 0);
   const int gsize = std::max(1, (nb + 127) / 128);  // at most 128 groups, as the reduction takes
@@ -848,24 +860,26 @@ void launch_density_small_t(Context& c, int nb, const void* psi, const double* f
   PWK_HIP(hipGetLastError());
   k_density_reduce<<<blocks_for(c.N, DRED_NT), DRED_NT, 0, c.stream>>>(c.N, ng, gsize, nb, fw_dev,
                                                                        c.work_a.as<double>(), (double*)rho,
-                                                                       overwrite ? 1 : 0);
+                                                                       overwrite ? 1 : 0, wave_sums);
   PWK_HIP(hipGetLastError());
 }
 
 }  // namespace
 
 void launch_density_small(Context& c, int nb, const void* psi, const double* fw_dev, void* rho, int precision,
-                          bool overwrite, bool psi_fp64) {
+                          bool overwrite, bool psi_fp64, double* wave_sums) {
   c.check_basis();
   PWK_REQUIRE(nb > 0, PWK_E_INVALID, "nb must be positive");
   PWK_REQUIRE(density_small_supported(c, precision), PWK_E_UNSUPPORTED, "grid too large for the fused kernel");
   if (precision == PWK_FP64)
-    launch_density_small_t<double, double2>(c, nb, psi, fw_dev, rho, overwrite);
+    launch_density_small_t<double, double2>(c, nb, psi, fw_dev, rho, overwrite, wave_sums);
   else if (psi_fp64)
-    launch_density_small_t<float, double2>(c, nb, psi, fw_dev, rho, overwrite);
+    launch_density_small_t<float, double2>(c, nb, psi, fw_dev, rho, overwrite, wave_sums);
   else
-    launch_density_small_t<float, float2>(c, nb, psi, fw_dev, rho, overwrite);
+    launch_density_small_t<float, float2>(c, nb, psi, fw_dev, rho, overwrite, wave_sums);
 }
+
+size_t density_wave_sums(const Context& c) { return ((c.N + DRED_NT - 1) / DRED_NT) * (DRED_NT / 64); }
 
 bool density_small_supported(const Context& c, int precision) {
   if (!c.small_hpsi || c.dims[0] != c.dims[1] || c.dims[1] != c.dims[2] || !small_len(c.dims[0])) return false;

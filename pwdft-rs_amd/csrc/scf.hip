@@ -2435,6 +2435,7 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
     if (dsmall) fw_pin.ensure((size_t)ns * nk * nbs * 8);
     for (int s = 0; s < ns; ++s) {
       double* rn = rho_new_s(s);
+      bool waves = false;
       // the basis is the same G-sphere at every k, so all nk·nb bands go through one launch
       std::vector<double> fw((size_t)nk * nbs, 0.0);  // buffer bands carry no weight
       for (int k = 0; k < nk; ++k)
@@ -2447,15 +2448,37 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
           // density has completed), the FP64 bands read directly
           double* fwp = fw_pin.as<double>() + (size_t)s * nk * nbs;
           std::memcpy(fwp, fw.data(), fw.size() * 8);
-          launch_density_small(c, nk * nbs, V.as<double2>() + (size_t)s * vblk(), fwp, rn, dprec, true, true);
+          // the group reduction also leaves the wave sums the ∫ρ reduction below is made of
+          waves = N <= (size_t)RED_BLOCKS * RED_NT && density_wave_sums(c) <= (size_t)MAX_DOTS * RED_BLOCKS;
+          launch_density_small(c, nk * nbs, V.as<double2>() + (size_t)s * vblk(), fwp, rn, dprec, true, true,
+                               waves ? host_partial : nullptr);
         } else if (p.precision == PWK_FP64) {
           launch_density(c, nk * nbs, V.as<double2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP64, true);
         } else {
           launch_density(c, nk * nbs, V32.as<float2>() + (size_t)s * vblk(), fw.data(), rn, PWK_FP32, true);
         }
       }
-      dots({DotTerm{rn, nullptr, nullptr, nullptr, 0}}, N);
-      const double integral = host_small[0] * dvol;
+      double isum;
+      if (waves) {
+        // Σ ρ in the order of the dot-product reduction (blocks of RED_NT points, each the sum of its
+        // waves' tree sums in wave order, blocks in order; one point per thread at this size)
+        c.sync();
+        const size_t nw = density_wave_sums(c);
+        const int nblk = (int)std::max<size_t>(1, (N + RED_NT - 1) / RED_NT);
+        isum = 0.0;
+        for (int b = 0; b < nblk; ++b) {
+          double part = 0.0;
+          for (int w = 0; w < RED_NT / 64; ++w) {
+            const size_t wi = (size_t)b * (RED_NT / 64) + w;
+            part += wi < nw ? host_partial[wi] : 0.0;
+          }
+          isum += part;
+        }
+      } else {
+        dots({DotTerm{rn, nullptr, nullptr, nullptr, 0}}, N);
+        isum = host_small[0];
+      }
+      const double integral = isum * dvol;
       // normalised to the electron count: by the symmetrisation's first transform when there is
       // one (the same product), else in place
       double scale = 1.0;
