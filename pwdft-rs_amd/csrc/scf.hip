@@ -455,6 +455,64 @@ __global__ void __launch_bounds__(256) k_unpack(UnpackArgs a) {
   for (unsigned w = blockIdx.x * 256u + threadIdx.x; w < n; w += gridDim.x * 256u) d[w] = s[w];
 }
 
+// The output density's LDA pass (k_lda_core with the ρ(G) scale) and the reduction partials of the
+// iteration's energy terms in one launch: block b takes the RED_NT points of k_dots' block b (one
+// point per thread, n <= RED_BLOCKS·RED_NT) and, after its LDA writes, forms each term's block partial
+// with k_dots' expressions and block_reduce's additions, so the partials are those k_dots would
+// write. A term may read what this launch writes: only the point's own thread writes and reads it.
+constexpr int LDA_DOTS_MAX = 12;  // terms of the fused launch (the iteration's energy sums take 9)
+__global__ void __launch_bounds__(RED_NT) k_lda_dots(size_t n, const double* rho, const double* core, double cs,
+                                                     double* rho_xc, double* exc, double* vxc, double2* g, double gs,
+                                                     DotArgs args, double* __restrict__ partial) {
+  __shared__ double sh[RED_NT / 64][LDA_DOTS_MAX];
+  const size_t i = (size_t)blockIdx.x * RED_NT + threadIdx.x;
+  if (i < n) {
+    const double2 x = g[i];
+    g[i] = make_double2(x.x * gs, x.y * gs);
+    const double r = core ? fmax(rho[i] + core[i] * cs, 0.0) : rho[i];
+    rho_xc[i] = r;
+    double e, v;
+    xc::lda_point(r, e, v);
+    exc[i] = e;
+    vxc[i] = v;
+  }
+  // every term's value at this point first (their loads in flight together), then the 64-lane
+  // trees of all terms, then each term's wave sums in wave order: block_reduce's arithmetic per term
+  double acc[LDA_DOTS_MAX];
+#pragma unroll
+  for (int t = 0; t < LDA_DOTS_MAX; ++t) {
+    acc[t] = 0.0;
+    if (t < args.nterms && i < n) {
+      const DotTerm& T = args.t[t];
+      if (T.kind == 1) {
+        const double g2 = T.b[i];
+        if (g2 > 1e-12) {
+          const double re = T.a[2 * i], im = T.a[2 * i + 1];
+          acc[t] += (re * re + im * im) / g2;
+        }
+      } else {
+        const double x = T.a[i] - (T.as ? T.as[i] : 0.0);
+        const double y = T.b ? T.b[i] - (T.bs ? T.bs[i] : 0.0) : 1.0;
+        acc[t] += x * y;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int t = 0; t < LDA_DOTS_MAX; ++t) acc[t] += __shfl_down(acc[t], off, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int t = 0; t < LDA_DOTS_MAX; ++t) sh[threadIdx.x >> 6][t] = acc[t];
+  __syncthreads();
+  if ((int)threadIdx.x < args.nterms) {
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int w = 0; w < RED_NT / 64; ++w) s += sh[w][t];
+    partial[(size_t)t * gridDim.x + blockIdx.x] = s;
+  }
+}
+
 // per-band expectation values: e[2b] = Σ_G |ψ|² t_G, e[2b+1] = Re Σ_G conj(ψ)·x; blockIdx.y = k
 // steps every operand by its k stride (one launch for all k-points)
 __global__ void __launch_bounds__(256) k_band_expect(int npw, const double2* __restrict__ psi,
@@ -1148,15 +1206,29 @@ struct Solve {
   void dots(const std::vector<DotTerm>& terms, size_t n) {
     PWK_REQUIRE(terms.size() <= (size_t)MAX_DOTS, PWK_E_INVALID, "too many reduction terms");
     const int nblk = (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (n + RED_NT - 1) / RED_NT));
-    // the per-block partials are written straight into pinned host memory (no copy launch) and
-    // summed here in block order (the second stage of the fixed-order reduction)
-    for (size_t t0 = 0; t0 < terms.size(); t0 += MAX_TERMS) {
-      const size_t nt = std::min(terms.size() - t0, (size_t)MAX_TERMS);
+    if (lda_pending) {
+      // the output LDA pass and the partials of every term in one launch (one point per thread)
+      lda_pending = false;
+      PWK_REQUIRE(terms.size() <= (size_t)LDA_DOTS_MAX && n <= (size_t)RED_BLOCKS * RED_NT, PWK_E_INVALID,
+                  "fused LDA reduction out of range");
       DotArgs a{};
-      a.nterms = (int)nt;
-      for (size_t i = 0; i < nt; ++i) a.t[i] = terms[t0 + i];
-      k_dots<<<dim3((unsigned)nblk, (unsigned)nt), RED_NT, 0, c.stream>>>(n, a, host_partial + t0 * nblk);
+      a.nterms = (int)terms.size();
+      for (size_t i = 0; i < terms.size(); ++i) a.t[i] = terms[i];
+      const LdaPass& q = lda_pass;
+      k_lda_dots<<<(unsigned)nblk, RED_NT, 0, c.stream>>>(n, q.rho, q.core, 1.0, q.rho_xc, q.exc, q.vxc, q.g, q.gs,
+                                                          a, host_partial);
       PWK_HIP(hipGetLastError());
+    } else {
+      // the per-block partials are written straight into pinned host memory (no copy launch) and
+      // summed here in block order (the second stage of the fixed-order reduction)
+      for (size_t t0 = 0; t0 < terms.size(); t0 += MAX_TERMS) {
+        const size_t nt = std::min(terms.size() - t0, (size_t)MAX_TERMS);
+        DotArgs a{};
+        a.nterms = (int)nt;
+        for (size_t i = 0; i < nt; ++i) a.t[i] = terms[t0 + i];
+        k_dots<<<dim3((unsigned)nblk, (unsigned)nt), RED_NT, 0, c.stream>>>(n, a, host_partial + t0 * nblk);
+        PWK_HIP(hipGetLastError());
+      }
     }
     c.sync();
     for (size_t t = 0; t < terms.size(); ++t) {
@@ -1255,6 +1327,18 @@ struct Solve {
   // symmetrize left N·ρ_sym(G) of the single channel's output density in rho_g_new (the
   // transform output_stage would otherwise redo from ρ(r))
   bool g_from_sym = false;
+  // output_stage leaves the output LDA pass (after a symmetrisation) to the iteration's reduction
+  // launch (k_lda_dots) when this is set; lda_pending then records it
+  bool fuse_out_lda = false;
+  struct LdaPass {
+    const double* rho;
+    const double* core;
+    double *rho_xc, *exc, *vxc;
+    double2* g;
+    double gs;
+  };
+  bool lda_pending = false;
+  LdaPass lda_pass{};
   // the mixing step left the LDA of rho_in (single channel) in rho_xc / exc / vxc
   bool lda_in_ready = false;
   // V(r) of channel s into V_eff slot `slot` for H·ψ (from V_eff(G) of the channel)
@@ -2203,7 +2287,11 @@ std::vector<DotTerm> Solve::output_stage() {
   if (ns == 1) {
     rho_tot_new = rho_new.as<double>();
     double *rx = rho_xc_o.as<double>(), *ex = exc_o.as<double>(), *vx = vxc_o.as<double>();
-    if (g_from_sym) {
+    if (g_from_sym && fuse_out_lda) {
+      // as below, run by the iteration's reduction launch
+      lda_pending = true;
+      lda_pass = LdaPass{rho_tot_new, core, rx, ex, vx, rho_g_new.as<double2>(), 1.0 / (double)N};
+    } else if (g_from_sym) {
       // ρ(G) of the output density is the symmetrised one already in rho_g_new (times N): the
       // forward transform of ρ(r) would only return it again
       launch_lda_core(c, rho_tot_new, core, 1.0, rx, ex, vx, rho_g_new.as<double2>(), 1.0 / (double)N);
@@ -2494,7 +2582,9 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
     const double* outs = host_small;
     if (defer) {
       std::vector<DotTerm> all = hterms;
+      fuse_out_lda = N <= (size_t)RED_BLOCKS * RED_NT;
       const std::vector<DotTerm> oterms = output_stage();
+      fuse_out_lda = false;
       all.insert(all.end(), oterms.begin(), oterms.end());
       dots(all, N);
       std::memcpy(har, host_small, hterms.size() * sizeof(double));
