@@ -455,17 +455,24 @@ __global__ void __launch_bounds__(256) k_unpack(UnpackArgs a) {
   for (unsigned w = blockIdx.x * 256u + threadIdx.x; w < n; w += gridDim.x * 256u) d[w] = s[w];
 }
 
-// The output density's LDA pass (k_lda_core with the ρ(G) scale) and the reduction partials of the
-// iteration's energy terms in one launch: block b takes the RED_NT points of k_dots' block b (one
+// The output density's LDA pass (k_lda_core with the ρ(G) scale), optionally the Anderson history
+// entry of the iteration (k_hist_store's), and the reduction partials of the iteration's energy
+// terms in one launch: block b takes the RED_NT points of k_dots' block b (one
 // point per thread, n <= RED_BLOCKS·RED_NT) and, after its LDA writes, forms each term's block partial
 // with k_dots' expressions and block_reduce's additions, so the partials are those k_dots would
 // write. A term may read what this launch writes: only the point's own thread writes and reads it.
 constexpr int LDA_DOTS_MAX = 12;  // terms of the fused launch (the iteration's energy sums take 9)
 __global__ void __launch_bounds__(RED_NT) k_lda_dots(size_t n, const double* rho, const double* core, double cs,
                                                      double* rho_xc, double* exc, double* vxc, double2* g, double gs,
-                                                     DotArgs args, double* __restrict__ partial) {
+                                                     const double* hist_in, double* hin, double* hres, DotArgs args,
+                                                     double* __restrict__ partial) {
   __shared__ double sh[RED_NT / 64][LDA_DOTS_MAX];
   const size_t i = (size_t)blockIdx.x * RED_NT + threadIdx.x;
+  if (hin && i < n) {  // k_hist_store's entry (in = hist_in, out = rho)
+    const double x = hist_in[i];
+    hin[i] = x;
+    hres[i] = rho[i] - x;
+  }
   if (i < n) {
     const double2 x = g[i];
     g[i] = make_double2(x.x * gs, x.y * gs);
@@ -1189,6 +1196,8 @@ struct Solve {
     std::vector<int> hslot;  // logical history order → slot
     int next_slot = 0, ndim = 1;
     bool kerker = false;
+    // the slot the next entry goes to (a new one until ndim are in use, then the oldest)
+    int peek_slot() const { return (int)hslot.size() < ndim ? next_slot : hslot.front(); }
   };
   Mixer mix_tot, mix_mag;
   double kerker_q2 = 0.0;
@@ -1216,7 +1225,7 @@ struct Solve {
       for (size_t i = 0; i < terms.size(); ++i) a.t[i] = terms[i];
       const LdaPass& q = lda_pass;
       k_lda_dots<<<(unsigned)nblk, RED_NT, 0, c.stream>>>(n, q.rho, q.core, 1.0, q.rho_xc, q.exc, q.vxc, q.g, q.gs,
-                                                          a, host_partial);
+                                                          q.hist_in, q.hin, q.hres, a, host_partial);
       PWK_HIP(hipGetLastError());
     } else {
       // the per-block partials are written straight into pinned host memory (no copy launch) and
@@ -1312,7 +1321,8 @@ struct Solve {
   void veff_from(const double* rho_r, const double2* rho_g, double* rho_xc_out, double* exc_out, double* vxc_out);
   void xc_spin(const double* up, const double* dn);
   // lda: also the LDA of the result into rho_xc / exc / vxc (one spin channel)
-  void mix(Mixer& m, const double* in, const double* out, double* result, bool lda = false);
+  // prestored: the entry (in, out − in) is already in the slot peek_slot() named
+  void mix(Mixer& m, const double* in, const double* out, double* result, bool lda = false, bool prestored = false);
   void run(double* eig_out, double* rho_out);
 
   // stages shared by the host-synchronised and the device-resident iteration
@@ -1336,9 +1346,12 @@ struct Solve {
     double *rho_xc, *exc, *vxc;
     double2* g;
     double gs;
+    const double* hist_in;  // with hin: the Anderson history entry (hist_in, rho − hist_in) too
+    double *hin, *hres;
   };
   bool lda_pending = false;
   LdaPass lda_pass{};
+  bool hist_prestored = false;  // this iteration's history entry is already stored
   // the mixing step left the LDA of rho_in (single channel) in rho_xc / exc / vxc
   bool lda_in_ready = false;
   // V(r) of channel s into V_eff slot `slot` for H·ψ (from V_eff(G) of the channel)
@@ -1856,7 +1869,7 @@ void Solve::xc_spin(const double* up, const double* dn) {
 // preconditioned when enabled, mixing/kerker.rs:5-29) enter the history, the oldest entry
 // leaves past ndim, and result = c_last·(in_l + β r_l) + Σ_j α_j (in_j + β r_j). result may
 // alias in.
-void Solve::mix(Mixer& mx, const double* in, const double* out, double* result, bool lda) {
+void Solve::mix(Mixer& mx, const double* in, const double* out, double* result, bool lda, bool prestored) {
   TimedScope ts(c, "scf_mix");
   int slot;
   if ((int)mx.hslot.size() < mx.ndim) {
@@ -1868,8 +1881,10 @@ void Solve::mix(Mixer& mx, const double* in, const double* out, double* result, 
   mx.hslot.push_back(slot);
   double* hin = mx.hist.as<double>() + (size_t)slot * 2 * N;
   double* hres = hin + N;
-  k_hist_store<<<gb(N), 256, 0, c.stream>>>(N, in, out, hin, hres);
-  PWK_HIP(hipGetLastError());
+  if (!prestored) {
+    k_hist_store<<<gb(N), 256, 0, c.stream>>>(N, in, out, hin, hres);
+    PWK_HIP(hipGetLastError());
+  }
   if (mx.kerker) {
     launch_fft3d_r2c(c, hres, 1.0, cwork.p, -1);
     k_kerker<<<gb(N), 256, 0, c.stream>>>(N, c.g2_64.as<double>(), kerker_q2, cwork.as<double2>());
@@ -2290,7 +2305,8 @@ std::vector<DotTerm> Solve::output_stage() {
     if (g_from_sym && fuse_out_lda) {
       // as below, run by the iteration's reduction launch
       lda_pending = true;
-      lda_pass = LdaPass{rho_tot_new, core, rx, ex, vx, rho_g_new.as<double2>(), 1.0 / (double)N};
+      lda_pass = LdaPass{rho_tot_new, core, rx, ex, vx, rho_g_new.as<double2>(), 1.0 / (double)N, nullptr, nullptr,
+                         nullptr};
     } else if (g_from_sym) {
       // ρ(G) of the output density is the symmetrised one already in rho_g_new (times N): the
       // forward transform of ρ(r) would only return it again
@@ -2454,6 +2470,7 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
   for (int e = 0; e < nkb; ++e) wk_all[(size_t)e] = wk[(size_t)(e % nk)];
   lda_in_ready = false;
   g_from_sym = false;
+  hist_prestored = false;
   bool have_prev = false;
   double e_prev = 0.0;
   for (int iter = 0; iter < p.max_iter; ++iter) {
@@ -2585,6 +2602,15 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
       fuse_out_lda = N <= (size_t)RED_BLOCKS * RED_NT;
       const std::vector<DotTerm> oterms = output_stage();
       fuse_out_lda = false;
+      if (lda_pending) {
+        // the mixing step's history entry goes with it (written also when the iteration then
+        // converges and does not mix: nothing reads the history after that)
+        double* hin = mix_tot.hist.as<double>() + (size_t)mix_tot.peek_slot() * 2 * N;
+        lda_pass.hist_in = rho_in.as<double>();
+        lda_pass.hin = hin;
+        lda_pass.hres = hin + N;
+        hist_prestored = true;
+      }
       all.insert(all.end(), oterms.begin(), oterms.end());
       dots(all, N);
       std::memcpy(har, host_small, hterms.size() * sizeof(double));
@@ -2621,7 +2647,8 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
     // src/scf/driver_spin.rs:800-832) ---------------------------------------------------------
     if (ns == 1) {
       lda_in_ready = grid_small_supported(c);
-      mix(mix_tot, rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>(), lda_in_ready);
+      mix(mix_tot, rho_in.as<double>(), rho_new.as<double>(), rho_in.as<double>(), lda_in_ready, hist_prestored);
+      hist_prestored = false;
     } else {
       mix(mix_tot, t_in.as<double>(), t_out.as<double>(), t_in.as<double>());
       mix(mix_mag, m_in.as<double>(), m_out.as<double>(), m_in.as<double>());
