@@ -462,11 +462,22 @@ __global__ void __launch_bounds__(256) k_unpack(UnpackArgs a) {
 // with k_dots' expressions and block_reduce's additions, so the partials are those k_dots would
 // write. A term may read what this launch writes: only the point's own thread writes and reads it.
 constexpr int LDA_DOTS_MAX = 12;  // terms of the fused launch (the iteration's energy sums take 9)
+// The Anderson step's sums over the new residual r_l = ρ − hist_in and the mm older residuals r_k
+// (the terms the mixing step reduces, in its order: (r_k − r_l)·r_l for k < mm, then (r_i − r_l)·(r_j − r_l)
+// for i ≤ j < mm); mm ≤ LDA_GRAM_MAX, each pair held at a fixed register slot
+constexpr int LDA_GRAM_MAX = 7;
+constexpr int LDA_GRAM_PAIRS = LDA_GRAM_MAX * (LDA_GRAM_MAX + 1) / 2;
+struct GramArgs {
+  int mm;
+  const double* res[LDA_GRAM_MAX];
+};
+__device__ __forceinline__ int gram_pair_slot(int i, int j) { return i * LDA_GRAM_MAX - i * (i - 1) / 2 + (j - i); }
 __global__ void __launch_bounds__(RED_NT) k_lda_dots(size_t n, const double* rho, const double* core, double cs,
                                                      double* rho_xc, double* exc, double* vxc, double2* g, double gs,
                                                      const double* hist_in, double* hin, double* hres, DotArgs args,
-                                                     double* __restrict__ partial) {
+                                                     GramArgs gram, double* __restrict__ partial) {
   __shared__ double sh[RED_NT / 64][LDA_DOTS_MAX];
+  __shared__ double shg[RED_NT / 64][LDA_GRAM_MAX + LDA_GRAM_PAIRS];
   const size_t i = (size_t)blockIdx.x * RED_NT + threadIdx.x;
   if (hin && i < n) {  // k_hist_store's entry (in = hist_in, out = rho)
     const double x = hist_in[i];
@@ -504,18 +515,59 @@ __global__ void __launch_bounds__(RED_NT) k_lda_dots(size_t n, const double* rho
       }
     }
   }
+  // the Anderson sums (k_dots' arithmetic: x = r_k − r_l, y = r_l − 0 or r_j − r_l, acc = 0 + x·y)
+  constexpr int NG = LDA_GRAM_MAX + LDA_GRAM_PAIRS;
+  double gacc[NG];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
+  for (int q = 0; q < NG; ++q) gacc[q] = 0.0;
+  if (gram.mm > 0 && i < n) {
+    const double rl = rho[i] - hist_in[i];  // the entry's residual, as stored above
+    double xk[LDA_GRAM_MAX];
+#pragma unroll
+    for (int k = 0; k < LDA_GRAM_MAX; ++k) xk[k] = k < gram.mm ? gram.res[k][i] - rl : 0.0;
+#pragma unroll
+    for (int k = 0; k < LDA_GRAM_MAX; ++k)
+      if (k < gram.mm) gacc[k] += xk[k] * (rl - 0.0);
+#pragma unroll
+    for (int a = 0; a < LDA_GRAM_MAX; ++a)
+#pragma unroll
+      for (int b = a; b < LDA_GRAM_MAX; ++b)
+        if (b < gram.mm) gacc[LDA_GRAM_MAX + gram_pair_slot(a, b)] += xk[a] * xk[b];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
     for (int t = 0; t < LDA_DOTS_MAX; ++t) acc[t] += __shfl_down(acc[t], off, 64);
-  if ((threadIdx.x & 63) == 0)
+    if (gram.mm > 0)  // uniform
+#pragma unroll
+      for (int q = 0; q < NG; ++q) gacc[q] += __shfl_down(gacc[q], off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int t = 0; t < LDA_DOTS_MAX; ++t) sh[threadIdx.x >> 6][t] = acc[t];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) shg[threadIdx.x >> 6][q] = gacc[q];
+  }
   __syncthreads();
-  if ((int)threadIdx.x < args.nterms) {
-    const int t = threadIdx.x;
+  const int ngram = gram.mm > 0 ? gram.mm + gram.mm * (gram.mm + 1) / 2 : 0;
+  const int t = threadIdx.x;
+  if (t < args.nterms + ngram) {
+    int slot;  // the term's register slot
+    bool energy = t < args.nterms;
+    if (energy) {
+      slot = t;
+    } else if (t - args.nterms < gram.mm) {
+      slot = t - args.nterms;
+    } else {
+      int u = t - args.nterms - gram.mm, a = 0;  // pair (a, b) in the mixing step's order
+      while (u >= gram.mm - a) {
+        u -= gram.mm - a;
+        ++a;
+      }
+      slot = LDA_GRAM_MAX + gram_pair_slot(a, a + u);
+    }
     double s = 0.0;
-    for (int w = 0; w < RED_NT / 64; ++w) s += sh[w][t];
+    for (int w = 0; w < RED_NT / 64; ++w) s += energy ? sh[w][slot] : shg[w][slot];
     partial[(size_t)t * gridDim.x + blockIdx.x] = s;
   }
 }
@@ -1215,6 +1267,7 @@ struct Solve {
   void dots(const std::vector<DotTerm>& terms, size_t n) {
     PWK_REQUIRE(terms.size() <= (size_t)MAX_DOTS, PWK_E_INVALID, "too many reduction terms");
     const int nblk = (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (n + RED_NT - 1) / RED_NT));
+    int ngram = 0;  // the fused launch's mixing sums, after the terms
     if (lda_pending) {
       // the output LDA pass and the partials of every term in one launch (one point per thread)
       lda_pending = false;
@@ -1224,8 +1277,10 @@ struct Solve {
       a.nterms = (int)terms.size();
       for (size_t i = 0; i < terms.size(); ++i) a.t[i] = terms[i];
       const LdaPass& q = lda_pass;
+      const int mm = q.gram.mm;
+      ngram = mm > 0 ? mm + mm * (mm + 1) / 2 : 0;
       k_lda_dots<<<(unsigned)nblk, RED_NT, 0, c.stream>>>(n, q.rho, q.core, 1.0, q.rho_xc, q.exc, q.vxc, q.g, q.gs,
-                                                          q.hist_in, q.hin, q.hres, a, host_partial);
+                                                          q.hist_in, q.hin, q.hres, a, q.gram, host_partial);
       PWK_HIP(hipGetLastError());
     } else {
       // the per-block partials are written straight into pinned host memory (no copy launch) and
@@ -1240,7 +1295,7 @@ struct Solve {
       }
     }
     c.sync();
-    for (size_t t = 0; t < terms.size(); ++t) {
+    for (size_t t = 0; t < terms.size() + (size_t)ngram; ++t) {
       double sum = 0.0;
       for (int b = 0; b < nblk; ++b) sum += host_partial[t * nblk + b];
       host_small[t] = sum;
@@ -1348,10 +1403,12 @@ struct Solve {
     double gs;
     const double* hist_in;  // with hin: the Anderson history entry (hist_in, rho − hist_in) too
     double *hin, *hres;
+    GramArgs gram;  // gram.mm > 0: the mixing step's sums too, after the energy terms
   };
   bool lda_pending = false;
   LdaPass lda_pass{};
   bool hist_prestored = false;  // this iteration's history entry is already stored
+  std::vector<double> gram_pre;  // the mixing step's sums, when the fused launch took them
   // the mixing step left the LDA of rho_in (single channel) in rho_xc / exc / vxc
   bool lda_in_ready = false;
   // V(r) of channel s into V_eff slot `slot` for H·ψ (from V_eff(G) of the channel)
@@ -1909,7 +1966,12 @@ void Solve::mix(Mixer& mx, const double* in, const double* out, double* result, 
     for (int i = 0; i < mm; ++i) t.push_back(DotTerm{ma.res[i], rl, rl, nullptr, 0});
     for (int i = 0; i < mm; ++i)
       for (int j = i; j < mm; ++j) t.push_back(DotTerm{ma.res[i], rl, ma.res[j], rl, 0});
-    dots(t, N);
+    if (prestored && gram_pre.size() == t.size()) {
+      // the iteration's fused launch took these sums (the same terms in the same order)
+      std::memcpy(host_small, gram_pre.data(), t.size() * sizeof(double));
+    } else {
+      dots(t, N);
+    }
     std::vector<double> A((size_t)mm * mm), bvec((size_t)mm);
     for (int i = 0; i < mm; ++i) bvec[(size_t)i] = -host_small[i];
     int q = mm;
@@ -2306,7 +2368,7 @@ std::vector<DotTerm> Solve::output_stage() {
       // as below, run by the iteration's reduction launch
       lda_pending = true;
       lda_pass = LdaPass{rho_tot_new, core, rx, ex, vx, rho_g_new.as<double2>(), 1.0 / (double)N, nullptr, nullptr,
-                         nullptr};
+                         nullptr, GramArgs{}};
     } else if (g_from_sym) {
       // ρ(G) of the output density is the symmetrised one already in rho_g_new (times N): the
       // forward transform of ρ(r) would only return it again
@@ -2471,6 +2533,7 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
   lda_in_ready = false;
   g_from_sym = false;
   hist_prestored = false;
+  gram_pre.clear();
   bool have_prev = false;
   double e_prev = 0.0;
   for (int iter = 0; iter < p.max_iter; ++iter) {
@@ -2610,11 +2673,24 @@ void Solve::run_host(double e_ewald, std::vector<double>& ev) {
         lda_pass.hin = hin;
         lda_pass.hres = hin + N;
         hist_prestored = true;
+        // and, without Kerker preconditioning, the mixing step's sums over the history as it will
+        // stand with this entry (its logical order: the older entries, then this one)
+        std::vector<int> order = mix_tot.hslot;
+        if ((int)order.size() >= mix_tot.ndim) order.erase(order.begin());
+        order.push_back(mix_tot.peek_slot());
+        const int mm = (int)order.size() - 1;
+        if (!mix_tot.kerker && mm >= 1 && mm <= LDA_GRAM_MAX) {
+          lda_pass.gram.mm = mm;
+          for (int k = 0; k < mm; ++k)
+            lda_pass.gram.res[k] = mix_tot.hist.as<double>() + (size_t)order[(size_t)k] * 2 * N + N;
+        }
       }
       all.insert(all.end(), oterms.begin(), oterms.end());
+      const int mm = lda_pending ? lda_pass.gram.mm : 0;
       dots(all, N);
       std::memcpy(har, host_small, hterms.size() * sizeof(double));
       outs = host_small + hterms.size();
+      gram_pre.assign(host_small + all.size(), host_small + all.size() + (mm > 0 ? mm + mm * (mm + 1) / 2 : 0));
     } else {
       dots(output_stage(), N);
     }
