@@ -304,47 +304,50 @@ __device__ __forceinline__ void rr_small_body(int k, int npw, int m, int nres, c
   // X = L^{-H} satisfies Xᴴ·S·X = I and spans the same space as the Löwdin X, so the Ritz pairs
   // are the same. L goes to C (lower triangle); the trailing update reads column j, which step j
   // does not write.
-  bool chol = true;
-  for (int j = 0; j < m; ++j) {
-    const double d = J.A[j][j].x;  // the same on every thread: the branch is block-uniform
-    if (!(d > 1e-8 * L.Ys[j][j].x)) {
-      chol = false;
-      break;
-    }
-    const double sd = sqrt(d), isd = 1.0 / sd, id = isd * isd;
+  // L^{-1} by forward substitution on all columns of I at once (B = I in J.V): step j sets
+  // L^{-1}_{jc} = B_jc / L_jj and B_ic -= L_ij·L^{-1}_{jc} below; X_cj = conj(L^{-1}_{jc}). Step j needs
+  // only column j of L, complete once Cholesky step j has passed its barrier, and touches J.V and
+  // L.X, which the factorisation does not: it runs in the barrier interval of Cholesky step j + 1.
+  auto inverse_step = [&](int j) {
+    const double ild = 1.0 / L.C[j][j].x;
     for (int e = t; e < m * m; e += RR_NT) {
-      const int i = e % m, l = e / m;
-      if (l == j && i >= j) {
-        const double2 a = J.A[i][j];
-        L.C[i][j] = i == j ? make_double2(sd, 0.0) : make_double2(a.x * isd, a.y * isd);
-      } else if (l > j && i >= l) {
-        const double2 a = J.A[i][j], b = J.A[l][j];  // A_il -= A_ij·conj(A_lj)/d
-        const double2 cur = J.A[i][l];
-        J.A[i][l] = make_double2(cur.x - (a.x * b.x + a.y * b.y) * id, cur.y - (a.y * b.x - a.x * b.y) * id);
+      const int i = e % m, cc = e / m;
+      const double2 bj = J.V[j][cc];
+      const double2 x = make_double2(bj.x * ild, bj.y * ild);
+      if (i == j) {
+        L.X[cc][j] = make_double2(x.x, -x.y);
+      } else if (i > j) {
+        const double2 lij = L.C[i][j], b = J.V[i][cc];
+        J.V[i][cc] = make_double2(b.x - (lij.x * x.x - lij.y * x.y), b.y - (lij.x * x.y + lij.y * x.x));
       }
     }
+  };
+  bool chol = true;
+  for (int j = 0; j <= m; ++j) {
+    if (j < m) {
+      const double d = J.A[j][j].x;  // the same on every thread: the branch is block-uniform
+      if (!(d > 1e-8 * L.Ys[j][j].x)) {
+        chol = false;
+        break;
+      }
+      const double sd = sqrt(d), isd = 1.0 / sd, id = isd * isd;
+      for (int e = t; e < m * m; e += RR_NT) {
+        const int i = e % m, l = e / m;
+        if (l == j && i >= j) {
+          const double2 a = J.A[i][j];
+          L.C[i][j] = i == j ? make_double2(sd, 0.0) : make_double2(a.x * isd, a.y * isd);
+        } else if (l > j && i >= l) {
+          const double2 a = J.A[i][j], b = J.A[l][j];  // A_il -= A_ij·conj(A_lj)/d
+          const double2 cur = J.A[i][l];
+          J.A[i][l] = make_double2(cur.x - (a.x * b.x + a.y * b.y) * id, cur.y - (a.y * b.x - a.x * b.y) * id);
+        }
+      }
+    }
+    if (j > 0) inverse_step(j - 1);
     __syncthreads();
   }
   __syncthreads();  // every thread has read the failing pivot before the Löwdin form rewrites A
-  if (chol) {
-    // L^{-1} by forward substitution on all columns of I at once (B = I in J.V): step j sets
-    // L^{-1}_{jc} = B_jc / L_jj and B_ic -= L_ij·L^{-1}_{jc} below; X_cj = conj(L^{-1}_{jc})
-    for (int j = 0; j < m; ++j) {
-      const double ild = 1.0 / L.C[j][j].x;
-      for (int e = t; e < m * m; e += RR_NT) {
-        const int i = e % m, cc = e / m;
-        const double2 bj = J.V[j][cc];
-        const double2 x = make_double2(bj.x * ild, bj.y * ild);
-        if (i == j) {
-          L.X[cc][j] = make_double2(x.x, -x.y);
-        } else if (i > j) {
-          const double2 lij = L.C[i][j], b = J.V[i][cc];
-          J.V[i][cc] = make_double2(b.x - (lij.x * x.x - lij.y * x.y), b.y - (lij.x * x.y + lij.y * x.x));
-        }
-      }
-      __syncthreads();
-    }
-  } else {
+  if (!chol) {  // (with the Cholesky factor, X is complete: its last step ran in the loop's final interval)
     // Löwdin: S = U s Uᴴ (Jacobi), X = U diag(1/sqrt(max(s_j, floor_rel·s_max)))
     for (int e = t; e < m * m; e += RR_NT) {
       J.A[e % m][e / m] = L.Ys[e % m][e / m];
